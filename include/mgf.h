@@ -1,0 +1,261 @@
+/*
+ * mgf.h — C ABI of the MI355X (gfx950) Eulerian MAC-grid step.
+ *
+ * The step is the one CGridFluidSolver::update() runs in
+ * junwei-jiang/Hybrid-Fluid-Simulation (MPhysics/GridFluidSolver.cpp:336-397):
+ *   boundary refresh -> semi-Lagrangian advection of the face-centred velocity
+ *   and of the fluid-domain indicator -> body forces -> pressure projection
+ *   (markers, 7-point matrix, right-hand side, (P)CG, pressure gradient)
+ *   -> velocity extrapolation -> buffer flip.
+ *
+ * Conventions shared with the reference (kept identical on purpose):
+ *   - Grids are cell-centred with resolution (nx, ny, nz), origin and spacing
+ *     (MPhysics/Grid.h:5-24).  Storage is structure-of-arrays, x fastest:
+ *     idx = z*nx*ny + y*nx + x (MPhysics/EulerMathTool.cu:63-73).
+ *   - Face-centred vector fields store three arrays with extents
+ *     (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1)
+ *     (MPhysics/FaceCenteredVectorGrid.cpp:74-114).
+ *   - All field values are IEEE float32; UNKNOWN is FLT_MAX
+ *     (MPhysics/MMathExtra.h:48-58).
+ *
+ * Every function returns an mgf_status (0 = success) and never throws.
+ * Host buffers passed in are caller-owned and copied; device state is owned by
+ * the library.  A scene handle is not thread-safe; calls on one context are
+ * serialised on that context's HIP stream.
+ */
+#ifndef MGF_H
+#define MGF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGF_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------ */
+typedef int32_t mgf_status;
+#define MGF_OK                 0
+#define MGF_ERR_INVALID        1  /* bad handle, size or parameter            */
+#define MGF_ERR_DEVICE         2  /* HIP runtime error                         */
+#define MGF_ERR_NOMEM          3  /* device allocation failed                  */
+#define MGF_ERR_NOT_CONVERGED  4  /* CG hit its iteration cap (result kept)    */
+#define MGF_ERR_NONFINITE      5  /* NaN/Inf found in an input or the solve    */
+#define MGF_ERR_UNSUPPORTED    6  /* feature outside this build                */
+
+/* ---- enumerations ------------------------------------------------------ */
+/* Back-trace order; values match EAdvectionAccuracy RK1/RK2/RK3
+ * (MPhysics/SemiLagrangian.cpp:188-347). */
+#define MGF_RK1 1
+#define MGF_RK2 2
+#define MGF_RK3 3
+
+/* CG stopping rule.
+ * MGF_TOL_ABS_SQRT_NORM: stop when sqrt(||r||_2) < tolerance, the reference
+ *   rule (MPhysics/CuMatrixFreePCG.cpp:51-52 with threshold 1e-3,
+ *   MPhysics/CuMatrixFreePCG.h:28).
+ * MGF_TOL_RELATIVE: stop when ||r||_2 <= tolerance * ||b||_2. */
+#define MGF_TOL_ABS_SQRT_NORM 0
+#define MGF_TOL_RELATIVE      1
+
+/* Arithmetic of the pressure solve.
+ * MGF_PRECISION_PLAIN: the reference's plain FP32 CG (FP32 vectors and
+ *   scalars, one kernel per BLAS-1 operation, scalars read by the host).
+ * MGF_PRECISION_MIXED: FP32 vectors inside fused kernels, FP64 dot products
+ *   and alpha/beta kept on the device, FP64 pressure accumulated over outer
+ *   iterative-refinement sweeps whose residual b - A x is formed in FP64. */
+#define MGF_PRECISION_PLAIN 0
+#define MGF_PRECISION_MIXED 1
+
+/* Preconditioner.  IDENTITY is the reference's (MPhysics/PressureSolver.cpp:
+ * 12-15); JACOBI uses the inverse diagonal recomputed from the markers. */
+#define MGF_PRECOND_IDENTITY 0
+#define MGF_PRECOND_JACOBI   1
+
+/* Field identifiers for upload/download. */
+#define MGF_FIELD_VEL_X        0  /* (nx+1)*ny*nz  face-centred velocity      */
+#define MGF_FIELD_VEL_Y        1  /* nx*(ny+1)*nz                             */
+#define MGF_FIELD_VEL_Z        2  /* nx*ny*(nz+1)                             */
+#define MGF_FIELD_FLUID_DOMAIN 3  /* nx*ny*nz  fluid indicator (<0 = fluid)   */
+#define MGF_FIELD_SOLID_SDF    4  /* nx*ny*nz  total solid marker -k/FLT_MAX
+                                     (CBoundarys::getTotalBoundarysSDF); on
+                                     upload it replaces the solids' result     */
+#define MGF_FIELD_SOLID_VEL_X  5  /* face-centred solid velocity              */
+#define MGF_FIELD_SOLID_VEL_Y  6
+#define MGF_FIELD_SOLID_VEL_Z  7
+#define MGF_FIELD_MARKERS      8  /* nx*ny*nz  2 solid / 1 fluid / 0 air (download) */
+#define MGF_FIELD_PRESSURE     9  /* nx*ny*nz  CG solution x (download)       */
+#define MGF_FIELD_RHS         10  /* nx*ny*nz  right-hand side b (download)   */
+#define MGF_FIELD_MATRIX_A    11  /* 4*nx*ny*nz [diag,+x,+y,+z] (download only;
+                                     built from the markers on request)        */
+#define MGF_FIELD_DIVERGENCE  12  /* nx*ny*nz  divergence of the velocity
+                                     (download only; computed on request)      */
+#define MGF_FIELD_COUNT       13
+
+/* ---- descriptors -------------------------------------------------------- */
+typedef struct mgf_scene_desc {
+    int32_t nx, ny, nz;        /* cell resolution (ghost layer included)     */
+    float   origin[3];
+    float   spacing[3];
+    float   dt;                /* CGridFluidSolver m_DeltaT                   */
+    float   density;           /* CPressureSolver m_density (1.0)             */
+    float   gravity[3];        /* CExternalForcesSolver m_Gravity (0,-9.8,0)  */
+    float   external_force[3]; /* CExternalForcesSolver m_ExternalForces      */
+    int32_t rk_order;          /* MGF_RK1..3 (reference update() uses RK3)    */
+    int32_t tol_mode;          /* MGF_TOL_*                                   */
+    double  tolerance;         /* 1e-3 with ABS_SQRT_NORM reproduces the ref. */
+    int64_t max_iterations;    /* <= 0: number of cells (PressureSolver.cpp:82) */
+    int32_t precision;         /* MGF_PRECISION_*                             */
+    int32_t preconditioner;    /* MGF_PRECOND_*                               */
+    int32_t extrapolation_layers; /* 1000 in the reference (GridFluidSolver.cpp:176) */
+    int32_t max_refinements;   /* MIXED: outer sweeps cap (<= 0: 8)          */
+    double  inner_tolerance;   /* MIXED: relative tolerance of one inner CG
+                                  (<= 0: 1e-3)                                 */
+} mgf_scene_desc;
+
+/* Per-call statistics (summed over the steps of one mgf_step call). */
+typedef struct mgf_step_stats {
+    int64_t cg_iterations;       /* inner CG iterations (A*p applications)    */
+    int32_t refinement_sweeps;   /* MIXED outer sweeps                        */
+    int32_t converged;           /* 1 when every solve met its tolerance     */
+    double  residual_rel;        /* last solve: ||b - A x||_2 / ||b||_2, FP64 */
+    double  residual_recurrence; /* last solve: final ||r||_2 of the recurrence */
+    double  b_norm;              /* last solve: ||b||_2                        */
+    int32_t extrapolation_layers;/* last step: layers that were non-empty     */
+    int32_t steps;               /* steps completed                           */
+    double  ms_advect;           /* wall time per stage, summed (host clock,  */
+    double  ms_pressure;         /*   only when timing is enabled)            */
+    double  ms_extrapolate;
+    double  ms_total;
+    int64_t fluid_cells;         /* last step: cells with marker 1            */
+} mgf_step_stats;
+
+typedef struct mgf_context_s* mgf_context;
+typedef struct mgf_scene_s*   mgf_scene;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces CCudaContextManager::initCudaContext / freeCudaContext
+ * (MPhysics/CudaContextManager.cpp:3-40). */
+MGF_API mgf_status mgf_init(int32_t device, mgf_context* out);
+MGF_API mgf_status mgf_shutdown(mgf_context ctx);
+MGF_API const char* mgf_status_string(mgf_status s);
+MGF_API const char* mgf_last_error(void);
+/* Fills desc with the reference defaults for a grid: RK3, ABS_SQRT_NORM 1e-3,
+ * max_iterations = cells, 1000 extrapolation layers, gravity (0,-9.8,0),
+ * density 1, PLAIN precision, identity preconditioner. */
+MGF_API mgf_status mgf_scene_desc_defaults(mgf_scene_desc* desc, int32_t nx,
+        int32_t ny, int32_t nz, const float origin[3], const float spacing[3],
+        float dt);
+
+/* ---- scene (CGridFluidSolver) -------------------------------------------- */
+/* CGridFluidSolver(dt, res, origin, spacing) + resizeGridFluidSolver
+ * (MPhysics/GridFluidSolver.cpp:32-41,161-196).  Fields start at zero. */
+MGF_API mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc,
+        mgf_scene* out);
+MGF_API mgf_status mgf_scene_destroy(mgf_scene scene);
+MGF_API mgf_status mgf_scene_get_desc(mgf_scene scene, mgf_scene_desc* out);
+/* Changes the solver parameters that do not affect storage (dt, forces,
+ * tolerance, precision, preconditioner, order, layers). */
+MGF_API mgf_status mgf_scene_set_params(mgf_scene scene, const mgf_scene_desc* desc);
+/* Device bytes held by the scene. */
+MGF_API mgf_status mgf_scene_device_bytes(mgf_scene scene, int64_t* out);
+
+/* CGridFluidSolver::addSolidBoundary / CBoundarys::addBoundary
+ * (MPhysics/GridFluidSolver.cpp:203-206, MPhysics/Boundary.cpp:92-113):
+ * a cell-centred SDF of nx*ny*nz floats (negative inside) and its velocity. */
+MGF_API mgf_status mgf_scene_add_solid(mgf_scene scene, const float* sdf,
+        int64_t count, const float velocity[3]);
+/* CGridFluidSolver::generateFluidDomainFromBBox
+ * (MPhysics/GridFluidSolver.cpp:551-555, MPhysics/EulerSolverTool.cu:581-608):
+ * fluid indicator -500 at cell centres strictly inside (min,max), +500 else. */
+MGF_API mgf_status mgf_scene_fluid_box(mgf_scene scene, const float box_min[3],
+        const float box_max[3]);
+
+/* setVelocityField / setFluidDomainField / setSolidVelField and the getters
+ * (MPhysics/GridFluidSolver.cpp:214-229, 251-324). */
+MGF_API mgf_status mgf_scene_upload(mgf_scene scene, int32_t field,
+        const float* host, int64_t count);
+MGF_API mgf_status mgf_scene_download(mgf_scene scene, int32_t field,
+        float* host, int64_t count);
+MGF_API mgf_status mgf_scene_field_count(mgf_scene scene, int32_t field,
+        int64_t* out);
+
+/* Stages of one step, each backing one reference method. */
+/* CBoundarys::updateBoundarys (MPhysics/Boundary.cpp:115-150). */
+MGF_API mgf_status mgf_scene_update_boundaries(mgf_scene scene);
+/* _computeAdvection (MPhysics/GridFluidSolver.cpp:399-416): velocity and fluid
+ * domain advected through the current velocity with desc.rk_order. */
+MGF_API mgf_status mgf_scene_advect(mgf_scene scene);
+/* _computeExternalForces -> CExternalForcesSolver::applyExternalForces
+ * (MPhysics/ExternalForcesSolver.cpp:50-59). */
+MGF_API mgf_status mgf_scene_apply_forces(mgf_scene scene);
+/* _computePressure -> CPressureSolver::solvePressure
+ * (MPhysics/PressureSolver.cpp:85-107): markers, system, CG, gradient. */
+MGF_API mgf_status mgf_scene_pressure_solve(mgf_scene scene, mgf_step_stats* stats);
+/* _extrapolatingVel -> extrapolatingDataInvoker(FCV)
+ * (MPhysics/GridFluidSolver.cpp:527-537, MPhysics/EulerSolverTool.cu:904-969). */
+MGF_API mgf_status mgf_scene_extrapolate_velocity(mgf_scene scene,
+        int32_t* layers_used);
+/* CGridFluidSolver::update() run nsteps times
+ * (MPhysics/GridFluidSolver.cpp:336-397). */
+MGF_API mgf_status mgf_step(mgf_scene scene, int32_t nsteps, mgf_step_stats* stats);
+/* Blocks until the scene's queued work is done. */
+MGF_API mgf_status mgf_scene_sync(mgf_scene scene);
+/* Host wall-clock timing of stages in mgf_step (adds synchronisation). */
+MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
+
+/* ---- operator entry points on caller device memory ---------------------- */
+/* These run one reference operator on device buffers the caller owns (for
+ * example buffers from mgf_device_alloc).  All pointers are device pointers.
+ * The grid is given as resolution/origin/spacing. */
+typedef struct mgf_grid {
+    int32_t nx, ny, nz;
+    float   origin[3];
+    float   spacing[3];
+} mgf_grid;
+
+MGF_API mgf_status mgf_device_alloc(mgf_context ctx, int64_t bytes, void** out);
+MGF_API mgf_status mgf_device_free(mgf_context ctx, void* ptr);
+MGF_API mgf_status mgf_memcpy_h2d(mgf_context ctx, void* dst, const void* src, int64_t bytes);
+MGF_API mgf_status mgf_memcpy_d2h(mgf_context ctx, void* dst, const void* src, int64_t bytes);
+
+/* CSemiLagrangian::backTrace (MPhysics/SemiLagrangian.cpp:188-347): moves
+ * `count` points (SoA px,py,pz, in place) back through the face-centred
+ * velocity (u,v,w on `grid`) by dt with the given RK order. */
+MGF_API mgf_status mgf_op_backtrace(mgf_context ctx, const mgf_grid* grid,
+        const float* u, const float* v, const float* w, float dt, int32_t rk_order,
+        float* px, float* py, float* pz, int64_t count);
+/* Trilinear sample of a cell-centred scalar field at `count` points
+ * (sampleCellCenteredScalarFieldTrilerp, MPhysics/FieldMathTool.cu:52-104). */
+MGF_API mgf_status mgf_op_sample_cc(mgf_context ctx, const mgf_grid* grid,
+        const float* field, const float* px, const float* py, const float* pz,
+        float* out, int64_t count);
+/* CSemiLagrangian::advect for a face-centred field (SemiLagrangian.cpp:169-186):
+ * positions are this grid's face centres. Input and output must not alias. */
+MGF_API mgf_status mgf_op_advect_fc(mgf_context ctx, const mgf_grid* grid,
+        const float* in_u, const float* in_v, const float* in_w,
+        const float* vel_u, const float* vel_v, const float* vel_w,
+        float dt, int32_t rk_order,
+        float* out_u, float* out_v, float* out_w);
+/* CSemiLagrangian::advect for a cell-centred scalar (SemiLagrangian.cpp:135-150). */
+MGF_API mgf_status mgf_op_advect_cc(mgf_context ctx, const mgf_grid* grid,
+        const float* in, const float* vel_u, const float* vel_v, const float* vel_w,
+        float dt, int32_t rk_order, float* out);
+/* The CG operator callback seam: PressureAxCGProd -> fdmMatrixVectorMul
+ * (MPhysics/PressureSolver.cpp:6-10, MPhysics/EulerSolverTool.cu:199-227)
+ * with the compact matrix recomputed from cell markers (uint8: 0 air,
+ * 1 fluid, 2 solid) and scale = dt/(density*h^2) per axis. */
+MGF_API mgf_status mgf_op_pressure_matvec(mgf_context ctx, const mgf_grid* grid,
+        const float scale[3], const uint8_t* markers, const float* x, float* y);
+/* extrapolatingDataInvoker for a face-centred field
+ * (MPhysics/EulerSolverTool.cu:450-557, 904-969): in place. */
+MGF_API mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid,
+        float* u, float* v, float* w, int32_t layers, int32_t* layers_used);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGF_H */
