@@ -1,0 +1,211 @@
+"""Benchmark of the MAC-grid step on one MI355X.
+
+Workload: 256^3 interior cells (258^3 with the solid ghost layer), a seeded
+fluid block and three seeded solid spheres, RK1 semi-Lagrangian advection,
+gravity, mixed-precision PCG to a 1e-6 relative FP64 residual, projection and
+1000-layer extrapolation per step.  A step counts only when its solve meets
+the tolerance.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--seed 0]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hybrid-fluid-simulation_amd", "python"))
+
+METRIC = "cell-steps/s at 256³ (div resid ≤1e-6) vs plain FP32; p50/p99 batched step latency"
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic bytes per cell of the two fused CG kernels (FP32 = 4 B, uint8
+# marker = 1 B): p/Ap reads r, p_old, marker and writes p_new; the update
+# reads x, p, r, marker and writes x, r.
+BYTES_P_AP = 13
+BYTES_UPDATE = 21
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_steps(scene, steps, tol):
+    times, iters, sweeps, res, conv = [], [], [], [], 0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        st = scene.step(1, allow_unconverged=True)
+        times.append(time.perf_counter() - t0)
+        iters.append(st["cg_iterations"])
+        sweeps.append(st["refinement_sweeps"])
+        res.append(st["residual_rel"])
+        if st["converged"] == 1:
+            conv += 1
+    return dict(times=np.array(times), iters=np.array(iters), sweeps=np.array(sweeps),
+                res=np.array(res), converged=conv)
+
+
+def cpu_baseline(spec, plain_iters_per_step, seconds_hint):
+    """The FP32 oracle (C++ CPU restatement of the reference step) on the host
+    cores: one 256^3 step's stages are timed, the CG at two iteration caps to
+    separate its per-iteration cost; the step time is assembled with the
+    plain GPU path's iteration count for the same scene."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mac_oracle as orc
+    import scenes
+    s = scenes.build_oracle_scene(orc, spec)
+    p = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-6)
+    t0 = time.perf_counter()
+    s.update_boundaries(p)
+    s.advect(p)
+    s.apply_forces(p)
+    t_adv = time.perf_counter() - t0
+    res, o, h = spec.res, spec.origin, spec.spacing
+    fields = [s.download(f) for f in range(8)]
+    u, v, w, phi, solid, us, vs, ws = fields
+    k_lo, k_hi = 5, 25
+    t0 = time.perf_counter()
+    orc.pressure_solve(res, o, h, spec.dt, u, v, w, us, vs, ws, solid, phi, tol_mode=1, tol=1e-30,
+                       max_it=k_lo)
+    t_lo = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = orc.pressure_solve(res, o, h, spec.dt, u, v, w, us, vs, ws, solid, phi, tol_mode=1, tol=1e-30,
+                             max_it=k_hi)
+    t_hi = time.perf_counter() - t0
+    per_iter = (t_hi - t_lo) / (k_hi - k_lo)
+    fixed = max(t_lo - k_lo * per_iter, 0.0)
+    s.upload(0, out["u"]); s.upload(1, out["v"]); s.upload(2, out["w"])
+    t0 = time.perf_counter()
+    s.extrapolate(p)
+    t_ext = time.perf_counter() - t0
+    s.close()
+    step_s = t_adv + fixed + plain_iters_per_step * per_iter + t_ext
+    return dict(value=spec.interior_cells / step_s, unit="cell-steps/s", cores=orc.num_threads(),
+                kind="port",
+                sample=("FP32 oracle (oracle/mac_oracle.hpp, OpenMP) on one %d^3 step of the same "
+                        "scene: boundaries+advection+forces %.2fs, CG %.3fs/iteration (timed at %d "
+                        "and %d iterations) x %.0f iterations/step (the plain GPU path's mean), "
+                        "fixed solve cost %.2fs, extrapolation %.2fs; step %.1fs")
+                % (spec.n, t_adv, per_iter, k_lo, k_hi, plain_iters_per_step, fixed, t_ext, step_s),
+                step_seconds=step_s)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=5)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cg-timing-iters", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: this build runs the MAC-grid step on a single GPU "
+              "(multi-GPU sharding is described in DESIGN.md, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import mgf
+    import scenes
+
+    spec = scenes.obstacles(args.n, seed=args.seed)
+    ncell_padded = int(np.prod(spec.res))
+    tol = 1e-6
+    with mgf.Context(0) as ctx:
+        log("scene %s: %d^3 interior, %d padded cells, dt %.3g" % (spec.name, spec.n, ncell_padded, spec.dt))
+        s = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=tol,
+                                   precision=mgf.PRECISION_MIXED)
+        run_steps(s, args.warmup, tol)
+        s.sync()
+        t0 = time.perf_counter()
+        r = run_steps(s, args.steps, tol)
+        s.sync()
+        wall = time.perf_counter() - t0
+        log("mixed: %d steps in %.3fs, %d converged, CG iters/step %.1f, sweeps/step %.2f, max resid %.2e"
+            % (args.steps, wall, r["converged"], r["iters"].mean(), r["sweeps"].mean(), r["res"].max()))
+        value = spec.interior_cells * r["converged"] / wall
+        # live kernel timing (HIP events on the scene's stream)
+        ms_pap, ms_upd = s.time_cg_kernels(args.cg_timing_iters)
+        gbs_pap = BYTES_P_AP * ncell_padded / (ms_pap * 1e-3) / 1e9
+        gbs_upd = BYTES_UPDATE * ncell_padded / (ms_upd * 1e-3) / 1e9
+        log("cg kernels: p/Ap %.4f ms (%.0f GB/s), update %.4f ms (%.0f GB/s)" % (ms_pap, gbs_pap, ms_upd, gbs_upd))
+        s.close()
+
+        plain = None
+        if not args.no_plain and args.plain_steps > 0:
+            sp = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=tol,
+                                        precision=mgf.PRECISION_PLAIN)
+            run_steps(sp, 1, tol)
+            sp.sync()
+            t0 = time.perf_counter()
+            rp = run_steps(sp, args.plain_steps, tol)
+            sp.sync()
+            wp = time.perf_counter() - t0
+            sp.close()
+            plain = dict(value=spec.interior_cells * rp["converged"] / wp, unit="cell-steps/s",
+                         steps=args.plain_steps, ms_per_step=1e3 * wp / args.plain_steps,
+                         cg_iterations_per_step=float(rp["iters"].mean()),
+                         residual_rel_max=float(rp["res"].max()), converged_steps=rp["converged"],
+                         path="plain FP32 CG: one kernel per BLAS-1 op, scalars read by the host")
+            log("plain: %.1f ms/step, CG iters/step %.1f" % (plain["ms_per_step"], plain["cg_iterations_per_step"]))
+
+    cpu = None
+    if not args.no_cpu:
+        k_plain = plain["cg_iterations_per_step"] if plain else float(r["iters"].mean())
+        cpu = cpu_baseline(spec, k_plain, 20)
+        log("cpu: " + cpu["sample"])
+
+    ms = np.array(r["times"]) * 1e3
+    k = float(r["iters"].mean())
+    rs = float(r["sweeps"].mean())
+    step_bytes = spec.interior_cells * (109 + 34 * k + 25 * rs)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields, f64 reductions and pressure",
+        "data": "synthetic",
+        "config": {
+            "workload": "obstacles %d^3 (%d^3 padded), seed %d, 3 spheres, RK1, gravity, mixed-precision "
+                        "PCG to 1e-6 relative FP64 residual, 1000-layer extrapolation" % (spec.n, spec.res[0], args.seed),
+            "grid": list(spec.res), "dt": spec.dt, "tolerance": tol,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_cg_update (x += alpha p, r -= alpha A p, r.r)",
+            "achieved": gbs_upd,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": gbs_upd / HBM_PEAK_GBPS,
+            "traffic": None,
+            "bytes_per_launch": BYTES_UPDATE * ncell_padded,
+            "ms_per_launch": ms_upd,
+            "other": {"k_cg_p_ap": {"ms": ms_pap, "GBps": gbs_pap, "bytes_per_launch": BYTES_P_AP * ncell_padded}},
+        },
+        "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),
+        "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
+                       "note": "single-scene step latency (batched scenes not built yet)"},
+        "cg_iterations_per_step": k,
+        "refinement_sweeps_per_step": rs,
+        "residual_rel_max": float(r["res"].max()),
+        "converged_steps": r["converged"],
+        "plain_fp32_gpu": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,944 @@
+// mgf_capi.cpp — the extern "C" boundary (include/mgf.h): contexts, scenes,
+// the stages of CGridFluidSolver::update() and the operator seams.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mgf_internal.h"
+
+using namespace mgf;
+
+namespace mgf {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace mgf
+
+namespace {
+
+constexpr int kPlainDotBlocks = 1024;
+constexpr int kResidualBlocks = 1024;
+constexpr int kLayerChunk = 32;
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+mgf_status fail(mgf_status s, const std::string& msg) {
+    set_error(msg);
+    return s;
+}
+
+template <class T>
+mgf_status dalloc(mgf_scene sc, T** out, size_t count) {
+    void* p = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    sc->allocations.push_back(p);
+    sc->device_bytes += int64_t(bytes);
+    *out = static_cast<T*>(p);
+    return MGF_OK;
+}
+
+#define TRY(expr)                       \
+    do {                                \
+        mgf_status s_ = (expr);         \
+        if (s_ != MGF_OK) return s_;    \
+    } while (0)
+
+Vel vel_of(const float* u, const float* v, const float* w) { return Vel{u, v, w}; }
+
+bool finite3(const float* a) {
+    return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]);
+}
+
+mgf_status validate_desc(const mgf_scene_desc* d) {
+    if (!d) return fail(MGF_ERR_INVALID, "null descriptor");
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(MGF_ERR_INVALID, "resolution must be >= 1");
+    const double cells = double(d->nx) * d->ny * d->nz;
+    const double faces = double(d->nx + 1) * d->ny * d->nz + double(d->nx) * (d->ny + 1) * d->nz +
+                         double(d->nx) * d->ny * (d->nz + 1);
+    if (cells >= 2147483648.0 || faces >= 1073741824.0)
+        return fail(MGF_ERR_INVALID, "grid too large (faces must be < 2^30)");
+    for (int k = 0; k < 3; ++k)
+        if (!(d->spacing[k] != 0.0f) || !std::isfinite(d->spacing[k]) || !std::isfinite(d->origin[k]))
+            return fail(MGF_ERR_INVALID, "spacing must be finite and non-zero, origin finite");
+    if (!std::isfinite(d->dt)) return fail(MGF_ERR_INVALID, "dt must be finite");
+    if (!(d->density > 0.0f)) return fail(MGF_ERR_INVALID, "density must be > 0");
+    if (!finite3(d->gravity) || !finite3(d->external_force))
+        return fail(MGF_ERR_INVALID, "forces must be finite");
+    if (d->rk_order < 1 || d->rk_order > 3) return fail(MGF_ERR_INVALID, "rk_order must be 1..3");
+    if (d->tol_mode != MGF_TOL_ABS_SQRT_NORM && d->tol_mode != MGF_TOL_RELATIVE)
+        return fail(MGF_ERR_INVALID, "unknown tol_mode");
+    if (!(d->tolerance > 0.0)) return fail(MGF_ERR_INVALID, "tolerance must be > 0");
+    if (d->precision != MGF_PRECISION_PLAIN && d->precision != MGF_PRECISION_MIXED)
+        return fail(MGF_ERR_INVALID, "unknown precision");
+    if (d->preconditioner != MGF_PRECOND_IDENTITY)
+        return fail(MGF_ERR_UNSUPPORTED, "only the identity preconditioner is built");
+    if (d->extrapolation_layers < 0 || d->extrapolation_layers > 1000000)
+        return fail(MGF_ERR_INVALID, "extrapolation_layers out of range");
+    return MGF_OK;
+}
+
+size_t field_count(const mgf_scene_s* s, int field) {
+    switch (field) {
+        case MGF_FIELD_VEL_X: case MGF_FIELD_SOLID_VEL_X: return s->nfx;
+        case MGF_FIELD_VEL_Y: case MGF_FIELD_SOLID_VEL_Y: return s->nfy;
+        case MGF_FIELD_VEL_Z: case MGF_FIELD_SOLID_VEL_Z: return s->nfz;
+        case MGF_FIELD_MATRIX_A: return 4 * s->ncell;
+        default: return s->ncell;
+    }
+}
+
+void scales(const mgf_scene_s* s, float sa[3], float sg[3]) {
+    const float dt = s->desc.dt, rho = s->desc.density;
+    const float h[3] = {s->desc.spacing[0], s->desc.spacing[1], s->desc.spacing[2]};
+    for (int k = 0; k < 3; ++k) {
+        // CPressureSolver::__buildMatrix / __applyPressureGradient
+        // (MPhysics/PressureSolver.cpp:148-164), evaluated in float.
+        sa[k] = dt / (rho * h[k] * h[k]);
+        sg[k] = dt / (rho * h[k]);
+    }
+}
+
+mgf_status sync(mgf_scene s) {
+    MGF_HIP(hipStreamSynchronize(s->ctx->stream));
+    return MGF_OK;
+}
+
+// Host-side sum of FP64 partials (fixed order).
+double host_sum(const double* p, int n, int stride, int k) {
+    double acc = 0;
+    for (int i = 0; i < n; ++i) acc += p[size_t(i) * stride + k];
+    return acc;
+}
+
+mgf_status dot_host(mgf_scene s, const float* a, const float* b, size_t n, double* out) {
+    hipStream_t st = s->ctx->stream;
+    MGF_HIP(launch_dot_partials(st, a, b, n, s->red.partials, kPlainDotBlocks));
+    MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * kPlainDotBlocks,
+                           hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipStreamSynchronize(st));
+    *out = host_sum(s->host_scratch, kPlainDotBlocks, 1, 0);
+    return MGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Boundaries: CBoundarys::updateBoundarys (MPhysics/Boundary.cpp:115-150).
+mgf_status refresh_boundaries(mgf_scene s) {
+    bool moving = false;
+    for (auto& so : s->solids)
+        if (so.vel[0] != 0.0f || so.vel[1] != 0.0f || so.vel[2] != 0.0f) moving = true;
+    // __moveBoundarys: translation += -dt * velocity (cuBLAS axpy, no skip)
+    for (auto& so : s->solids)
+        for (int k = 0; k < 3; ++k) so.trans[k] = std::fma(-s->desc.dt, so.vel[k], so.trans[k]);
+    if (!s->boundary_dirty && !moving) return MGF_OK;
+    hipStream_t st = s->ctx->stream;
+    // zero-filled marker, then each solid in order
+    MGF_HIP(launch_fill(st, s->solid_total, s->ncell, 0.0f));
+    float* tmp = s->ap;
+    std::vector<float> vel3;
+    for (size_t k = 0; k < s->solids.size(); ++k) {
+        MGF_HIP(launch_resample_solid(st, s->g, s->solids[k].sdf0, s->solids[k].trans, tmp));
+        MGF_HIP(launch_solids_marker(st, s->ncell, tmp, int(k) + 1, s->solid_total));
+        for (int q = 0; q < 3; ++q) vel3.push_back(s->solids[k].vel[q]);
+    }
+    MGF_HIP(launch_solid_mask(st, s->ncell, s->solid_total, s->solid_mask));
+    if (vel3.empty()) vel3.assign(3, 0.0f);
+    MGF_HIP(launch_solids_vel(st, s->g, s->solid_total, vel3.data(), int(s->solids.size()), s->us,
+                              s->vs, s->ws));
+    s->boundary_dirty = false;
+    return MGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// _computeAdvection + (optionally) _computeExternalForces.
+mgf_status advect(mgf_scene s, bool with_forces) {
+    hipStream_t st = s->ctx->stream;
+    const int c = s->cur, n = 1 - s->cur;
+    const float dt = s->desc.dt;
+    float g[3], f[3];
+    int addg = 0, addf = 0;
+    if (with_forces) {
+        for (int k = 0; k < 3; ++k) {
+            g[k] = s->desc.gravity[k] * dt;
+            f[k] = s->desc.external_force[k] * dt;
+        }
+        auto on = [](const float* a) {
+            return !(std::fabs(a[0]) < MGF_FLT_EPSILON && std::fabs(a[1]) < MGF_FLT_EPSILON &&
+                     std::fabs(a[2]) < MGF_FLT_EPSILON);
+        };
+        addg = on(g);
+        addf = on(f);
+    }
+    Vel V = vel_of(s->u[c], s->v[c], s->w[c]);
+    MGF_HIP(launch_advect_fc(st, s->g, V, V, dt, s->desc.rk_order, s->u[n], s->v[n], s->w[n], addg,
+                             g, addf, f));
+    MGF_HIP(launch_advect_cc(st, s->g, s->phi[c], V, dt, s->desc.rk_order, s->phi[n]));
+    s->cur = n;
+    return MGF_OK;
+}
+
+mgf_status apply_forces(mgf_scene s) {
+    hipStream_t st = s->ctx->stream;
+    const float dt = s->desc.dt;
+    const float* incs[2] = {s->desc.gravity, s->desc.external_force};
+    for (int q = 0; q < 2; ++q) {
+        float a[3] = {incs[q][0] * dt, incs[q][1] * dt, incs[q][2] * dt};
+        if (std::fabs(a[0]) < MGF_FLT_EPSILON && std::fabs(a[1]) < MGF_FLT_EPSILON &&
+            std::fabs(a[2]) < MGF_FLT_EPSILON)
+            continue;
+        MGF_HIP(launch_add_uniform(st, s->u[s->cur], s->nfx, a[0]));
+        MGF_HIP(launch_add_uniform(st, s->v[s->cur], s->nfy, a[1]));
+        MGF_HIP(launch_add_uniform(st, s->w[s->cur], s->nfz, a[2]));
+    }
+    return MGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain CG: CCuMatrixFreePCG::solvePCGInvDia (MPhysics/CuMatrixFreePCG.cpp:
+// 22-65) with the identity preconditioner, scalars on the host.
+mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_stats* acc) {
+    hipStream_t st = s->ctx->stream;
+    const size_t n = s->ncell;
+    const auto& d = s->desc;
+    double bb;
+    TRY(dot_host(s, s->b, s->b, n, &bb));
+    const float bn = float(std::sqrt(bb));
+    auto stop = [&](float rn) {
+        if (d.tol_mode == MGF_TOL_ABS_SQRT_NORM) return std::sqrt(rn) < float(d.tolerance);
+        return double(rn) <= d.tolerance * double(bn);
+    };
+    MGF_HIP(launch_fill(st, s->x, n, 0.0f));
+    s->x_is_64 = false;
+    int64_t it = 0;
+    bool conv = false;
+    double rn_last = bn;
+    if (d.tol_mode == MGF_TOL_ABS_SQRT_NORM ? std::sqrt(bn) < float(d.tolerance) : bn == 0.0f) {
+        conv = true;
+    } else {
+        // r = A*0 = 0 -> scale(-1) -> += b, i.e. r = b; z = r; p = z.
+        MGF_HIP(hipMemcpyAsync(s->r, s->b, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        MGF_HIP(hipMemcpyAsync(s->p0, s->r, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        // r . z with z = r; the value carries over from the previous
+        // iteration's r . z (the same vector, the same sum).
+        double t;
+        TRY(dot_host(s, s->r, s->r, n, &t));
+        float rz = float(t);
+        for (; it < max_it;) {
+            MGF_HIP(launch_matvec(st, s->g, sa, s->markers, s->p0, s->ap));
+            ++it;
+            TRY(dot_host(s, s->p0, s->ap, n, &t));
+            const float pap = float(t);
+            const float alpha = rz / pap;
+            if (!(std::fabs(alpha) < MGF_FLT_EPSILON)) {
+                MGF_HIP(launch_axpy(st, alpha, s->p0, s->x, n));
+                MGF_HIP(launch_axpy(st, -alpha, s->ap, s->r, n));
+            }
+            TRY(dot_host(s, s->r, s->r, n, &t));
+            const float rn = float(std::sqrt(t));
+            rn_last = rn;
+            if (stop(rn)) { conv = true; break; }
+            if (!std::isfinite(rn)) break;   // reported as non-finite below
+            const float rz_new = float(t);
+            const float beta = rz_new / rz;
+            rz = rz_new;
+            MGF_HIP(launch_scal_add(st, beta, s->p0, s->r, n));
+        }
+    }
+    // FP64 true residual of the FP32 pressure
+    MGF_HIP(launch_true_residual32(st, s->g, sa, s->markers, s->x, s->b, s->red.partials,
+                                   kResidualBlocks));
+    MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * 2 * kResidualBlocks,
+                           hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipStreamSynchronize(st));
+    const double rr = host_sum(s->host_scratch, kResidualBlocks, 2, 0);
+    const double b2 = host_sum(s->host_scratch, kResidualBlocks, 2, 1);
+    acc->cg_iterations += it;
+    acc->converged = acc->converged && conv;
+    acc->residual_rel = b2 > 0 ? std::sqrt(rr / b2) : std::sqrt(rr);
+    acc->residual_recurrence = rn_last;
+    acc->b_norm = std::sqrt(b2);
+    acc->refinement_sweeps = 0;
+    return MGF_OK;
+}
+
+// Mixed CG: FP32 vectors in two fused kernels per iteration, FP64 dots and
+// scalars on the device, FP64 pressure accumulated over refinement sweeps
+// whose residual b - A x is formed in FP64.
+mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_stats* acc) {
+    hipStream_t st = s->ctx->stream;
+    const auto& d = s->desc;
+    const double inner = d.inner_tolerance > 0 ? d.inner_tolerance : 1e-3;
+    const int max_sweeps = d.max_refinements > 0 ? d.max_refinements : 8;
+    const int chunk = s->ncell < (1u << 20) ? 32 : 16;
+    // ||b||^2 and the overall target
+    MGF_HIP(launch_fill64(st, s->x64, s->ncell, 0.0));
+    MGF_HIP(launch_refine(st, s->g, sa, s->markers, nullptr, s->x64, s->b, nullptr, s->red.partials,
+                          kResidualBlocks, 0));
+    MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * 2 * kResidualBlocks,
+                           hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipStreamSynchronize(st));
+    const double b2 = host_sum(s->host_scratch, kResidualBlocks, 2, 1);
+    double target2;
+    if (d.tol_mode == MGF_TOL_ABS_SQRT_NORM) {
+        const double t = d.tolerance * d.tolerance;  // sqrt(||r||) < tol
+        target2 = t * t;
+    } else {
+        target2 = d.tolerance * d.tolerance * b2;
+    }
+    s->x_is_64 = true;
+    int64_t it_total = 0;
+    int sweeps = 0;
+    bool conv = b2 <= target2 || b2 == 0.0;
+    double rr_true = b2;
+    const float* rsrc = s->b;
+    CgScalars hs{};
+    while (!conv && sweeps < max_sweeps && it_total < max_it) {
+        MGF_HIP(launch_cg_init(st, s->g, rsrc, s->r, nullptr, s->x, s->scal, s->red, inner, target2,
+                               sweeps == 0));
+        float* pold = s->p0;
+        float* pnew = s->p1;
+        for (;;) {
+            const int64_t left = max_it - it_total;
+            const int k = int(std::min<int64_t>(chunk, std::max<int64_t>(left, 1)));
+            for (int q = 0; q < k; ++q) {
+                MGF_HIP(launch_cg_p_ap(st, s->g, sa, s->markers, s->r, pold, pnew, s->scal, s->red));
+                MGF_HIP(launch_cg_update(st, s->g, sa, s->markers, pnew, s->x, s->r, s->scal, s->red));
+                std::swap(pold, pnew);
+            }
+            MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+            MGF_HIP(hipStreamSynchronize(st));
+            if (hs.done || it_total + hs.iters >= max_it) break;
+        }
+        it_total += hs.iters;
+        if (hs.done == 2) { rr_true = std::nan(""); break; }
+        // x64 (+)= x; r = b - A x64 in FP64
+        MGF_HIP(launch_refine(st, s->g, sa, s->markers, s->x, s->x64, s->b, s->r, s->red.partials,
+                              kResidualBlocks, sweeps > 0));
+        MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * 2 * kResidualBlocks,
+                               hipMemcpyDeviceToHost, st));
+        MGF_HIP(hipStreamSynchronize(st));
+        rr_true = host_sum(s->host_scratch, kResidualBlocks, 2, 0);
+        ++sweeps;
+        rsrc = s->r;
+        if (rr_true <= target2) conv = true;
+        if (!std::isfinite(rr_true)) break;
+    }
+    acc->cg_iterations += it_total;
+    acc->refinement_sweeps += sweeps;
+    acc->converged = acc->converged && conv;
+    acc->residual_rel = b2 > 0 ? std::sqrt(rr_true / b2) : std::sqrt(rr_true);
+    acc->residual_recurrence = std::sqrt(hs.rr);
+    acc->b_norm = std::sqrt(b2);
+    return MGF_OK;
+}
+
+mgf_status pressure(mgf_scene s, mgf_step_stats* acc) {
+    hipStream_t st = s->ctx->stream;
+    const int c = s->cur;
+    float sa[3], sg[3];
+    scales(s, sa, sg);
+    MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
+    MGF_HIP(launch_rhs(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), vel_of(s->us, s->vs, s->ws),
+                       s->markers, s->b));
+    const int64_t max_it = s->desc.max_iterations > 0 ? s->desc.max_iterations : int64_t(s->ncell);
+    if (s->desc.precision == MGF_PRECISION_PLAIN) TRY(cg_plain(s, sa, max_it, acc));
+    else TRY(cg_mixed(s, sa, max_it, acc));
+    MGF_HIP(launch_project(st, s->g, sg, s->markers, s->x_is_64 ? nullptr : s->x,
+                           s->x_is_64 ? s->x64 : nullptr, s->u[c], s->v[c], s->w[c], s->us, s->vs,
+                           s->ws));
+    if (!std::isfinite(acc->residual_rel)) return fail(MGF_ERR_NONFINITE, "pressure solve produced NaN/Inf");
+    return MGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Extrapolation by frontiers; returns the last non-empty layer.
+mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, int* used) {
+    hipStream_t st = s->ctx->stream;
+    *used = 0;
+    if (layers <= 0) return MGF_OK;
+    // counts live in a ring of kLayerChunk+1 slots: slot L % (chunk+1)
+    const int ring = kLayerChunk + 1;
+    MGF_HIP(launch_fill(st, reinterpret_cast<float*>(s->counts), size_t(ring), 0.0f));
+    MGF_HIP(launch_extrap_init(st, s->g, u, v, w, s->layer, s->list[0], &s->counts[0]));
+    std::vector<uint32_t> hc(ring);
+    int L = 1;
+    bool empty = false;
+    while (L < layers && !empty) {
+        const int k = std::min(kLayerChunk, layers - L);
+        for (int q = 0; q < k; ++q, ++L) {
+            // clear the slot this layer writes
+            MGF_HIP(launch_fill(st, reinterpret_cast<float*>(&s->counts[L % ring]), 1, 0.0f));
+            MGF_HIP(launch_extrap_layer(st, s->g, L, u, v, w, s->layer, s->list[(L - 1) & 1],
+                                        &s->counts[(L - 1) % ring], s->list[L & 1],
+                                        &s->counts[L % ring]));
+        }
+        MGF_HIP(hipMemcpyAsync(hc.data(), s->counts, sizeof(uint32_t) * ring, hipMemcpyDeviceToHost, st));
+        MGF_HIP(hipStreamSynchronize(st));
+        for (int q = L - k; q < L; ++q) {
+            if (hc[q % ring] == 0) { empty = true; break; }
+            *used = q;
+        }
+    }
+    return MGF_OK;
+}
+
+mgf_status step_once(mgf_scene s, mgf_step_stats* acc) {
+    const bool t = s->timing;
+    double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    if (t) { TRY(sync(s)); t0 = now_ms(); }
+    TRY(refresh_boundaries(s));
+    TRY(advect(s, true));
+    if (t) { TRY(sync(s)); t1 = now_ms(); }
+    TRY(pressure(s, acc));
+    if (t) { TRY(sync(s)); t2 = now_ms(); }
+    int used = 0;
+    TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
+    acc->extrapolation_layers = used;
+    if (t) {
+        TRY(sync(s));
+        t3 = now_ms();
+        acc->ms_advect += t1 - t0;
+        acc->ms_pressure += t2 - t1;
+        acc->ms_extrapolate += t3 - t2;
+        acc->ms_total += t3 - t0;
+    }
+    acc->steps += 1;
+    return MGF_OK;
+}
+
+void stats_reset(mgf_step_stats* st) {
+    std::memset(st, 0, sizeof(*st));
+    st->converged = 1;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* mgf_status_string(mgf_status s) {
+    switch (s) {
+        case MGF_OK: return "ok";
+        case MGF_ERR_INVALID: return "invalid argument";
+        case MGF_ERR_DEVICE: return "device error";
+        case MGF_ERR_NOMEM: return "out of device memory";
+        case MGF_ERR_NOT_CONVERGED: return "not converged";
+        case MGF_ERR_NONFINITE: return "non-finite value";
+        case MGF_ERR_UNSUPPORTED: return "unsupported";
+        default: return "unknown status";
+    }
+}
+
+const char* mgf_last_error(void) { return g_last_error.c_str(); }
+
+mgf_status mgf_init(int32_t device, mgf_context* out) {
+    if (!out) return fail(MGF_ERR_INVALID, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_DEVICE, "no HIP device");
+    }
+    if (device < 0 || device >= n) return fail(MGF_ERR_INVALID, "device index out of range");
+    MGF_HIP(hipSetDevice(device));
+    auto* c = new mgf_context_s;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    *out = c;
+    return MGF_OK;
+}
+
+mgf_status mgf_shutdown(mgf_context ctx) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_desc_defaults(mgf_scene_desc* d, int32_t nx, int32_t ny, int32_t nz,
+                                   const float origin[3], const float spacing[3], float dt) {
+    if (!d) return fail(MGF_ERR_INVALID, "null descriptor");
+    std::memset(d, 0, sizeof(*d));
+    d->nx = nx; d->ny = ny; d->nz = nz;
+    for (int k = 0; k < 3; ++k) {
+        d->origin[k] = origin ? origin[k] : 0.0f;
+        d->spacing[k] = spacing ? spacing[k] : 1.0f;
+    }
+    d->dt = dt;
+    d->density = 1.0f;
+    d->gravity[1] = -9.8f;
+    d->rk_order = MGF_RK3;
+    d->tol_mode = MGF_TOL_ABS_SQRT_NORM;
+    d->tolerance = 1e-3;
+    d->max_iterations = 0;
+    d->precision = MGF_PRECISION_PLAIN;
+    d->preconditioner = MGF_PRECOND_IDENTITY;
+    d->extrapolation_layers = 1000;
+    d->max_refinements = 8;
+    d->inner_tolerance = 1e-3;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_scene* out) {
+    if (!ctx || !out) return fail(MGF_ERR_INVALID, "null context or output");
+    *out = nullptr;
+    TRY(validate_desc(desc));
+    MGF_HIP(hipSetDevice(ctx->device));
+    auto* s = new mgf_scene_s;
+    s->ctx = ctx;
+    s->desc = *desc;
+    s->g.nx = desc->nx; s->g.ny = desc->ny; s->g.nz = desc->nz;
+    s->g.ox = desc->origin[0]; s->g.oy = desc->origin[1]; s->g.oz = desc->origin[2];
+    s->g.hx = desc->spacing[0]; s->g.hy = desc->spacing[1]; s->g.hz = desc->spacing[2];
+    s->ncell = size_t(desc->nx) * desc->ny * desc->nz;
+    s->nfx = size_t(desc->nx + 1) * desc->ny * desc->nz;
+    s->nfy = size_t(desc->nx) * (desc->ny + 1) * desc->nz;
+    s->nfz = size_t(desc->nx) * desc->ny * (desc->nz + 1);
+    s->nvec = s->ncell;
+    const size_t nface = s->nfx + s->nfy + s->nfz;
+    mgf_status st = MGF_OK;
+    auto A = [&](auto** p, size_t n) {
+        if (st == MGF_OK) st = dalloc(s, p, n);
+    };
+    for (int k = 0; k < 2; ++k) {
+        A(&s->u[k], s->nfx); A(&s->v[k], s->nfy); A(&s->w[k], s->nfz); A(&s->phi[k], s->ncell);
+    }
+    A(&s->solid_total, s->ncell); A(&s->solid_mask, s->ncell);
+    A(&s->us, s->nfx); A(&s->vs, s->nfy); A(&s->ws, s->nfz);
+    A(&s->markers, s->ncell);
+    A(&s->b, s->ncell); A(&s->x, s->ncell); A(&s->x64, s->ncell);
+    A(&s->r, s->ncell); A(&s->p0, s->ncell); A(&s->p1, s->ncell); A(&s->ap, s->ncell);
+    A(&s->scal, 1);
+    A(&s->red.partials, size_t(kMaxBlocks) * 4);
+    A(&s->red.ticket, 4);
+    A(&s->layer, nface); A(&s->list[0], nface); A(&s->list[1], nface);
+    A(&s->counts, kLayerChunk + 1);
+    if (st != MGF_OK) {
+        mgf_scene_destroy(s);
+        return st;
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->host_scratch), sizeof(double) * kMaxBlocks * 4) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        mgf_scene_destroy(s);
+        return fail(MGF_ERR_NOMEM, "hipHostMalloc failed");
+    }
+    // reduction geometry of the fused kernels
+    size_t blocks = (s->ncell + kBlock - 1) / kBlock;
+    blocks = std::min<size_t>(std::max<size_t>(blocks, 8), kMaxBlocks);
+    blocks = (blocks + 7) / 8 * 8;
+    s->red.blocks = int(std::min<size_t>(blocks, kMaxBlocks));
+    hipStream_t sm = ctx->stream;
+    // resize() zero-fills every field (MPhysics/ScalarGrid.cpp:33-37)
+    hipError_t e = hipSuccess;
+    auto F = [&](float* p, size_t n) {
+        if (e == hipSuccess) e = launch_fill(sm, p, n, 0.0f);
+    };
+    for (int k = 0; k < 2; ++k) {
+        F(s->u[k], s->nfx); F(s->v[k], s->nfy); F(s->w[k], s->nfz); F(s->phi[k], s->ncell);
+    }
+    F(s->solid_total, s->ncell);
+    F(s->us, s->nfx); F(s->vs, s->nfy); F(s->ws, s->nfz);
+    F(s->x, s->ncell); F(s->b, s->ncell);
+    F(reinterpret_cast<float*>(s->red.ticket), 4);
+    if (e == hipSuccess) e = hipMemsetAsync(s->solid_mask, 0, s->ncell, sm);
+    if (e == hipSuccess) e = hipMemsetAsync(s->markers, 0, s->ncell, sm);
+    if (e == hipSuccess) e = hipStreamSynchronize(sm);
+    if (e != hipSuccess) {
+        mgf_scene_destroy(s);
+        return fail(MGF_ERR_DEVICE, std::string("scene init: ") + hipGetErrorString(e));
+    }
+    *out = s;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_destroy(mgf_scene s) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (void* p : s->allocations) (void)hipFree(p);
+    for (auto& so : s->solids) (void)hipFree(so.sdf0);
+    if (s->host_scratch) (void)hipHostFree(s->host_scratch);
+    delete s;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_get_desc(mgf_scene s, mgf_scene_desc* out) {
+    if (!s || !out) return fail(MGF_ERR_INVALID, "null argument");
+    *out = s->desc;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_set_params(mgf_scene s, const mgf_scene_desc* d) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    TRY(validate_desc(d));
+    if (d->nx != s->desc.nx || d->ny != s->desc.ny || d->nz != s->desc.nz)
+        return fail(MGF_ERR_INVALID, "resolution cannot change");
+    for (int k = 0; k < 3; ++k)
+        if (d->origin[k] != s->desc.origin[k] || d->spacing[k] != s->desc.spacing[k])
+            return fail(MGF_ERR_INVALID, "origin/spacing cannot change");
+    s->desc = *d;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_device_bytes(mgf_scene s, int64_t* out) {
+    if (!s || !out) return fail(MGF_ERR_INVALID, "null argument");
+    *out = s->device_bytes + int64_t(s->solids.size() * s->ncell * sizeof(float));
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_add_solid(mgf_scene s, const float* sdf, int64_t count, const float velocity[3]) {
+    if (!s || !sdf) return fail(MGF_ERR_INVALID, "null argument");
+    if (size_t(count) != s->ncell) return fail(MGF_ERR_INVALID, "solid SDF must have nx*ny*nz values");
+    if (s->solids.size() >= 64) return fail(MGF_ERR_UNSUPPORTED, "at most 64 solids");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    Solid so;
+    if (hipMalloc(reinterpret_cast<void**>(&so.sdf0), s->ncell * sizeof(float)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_NOMEM, "hipMalloc of a solid SDF failed");
+    }
+    if (hipMemcpy(so.sdf0, sdf, s->ncell * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(so.sdf0);
+        return fail(MGF_ERR_DEVICE, "copy of a solid SDF failed");
+    }
+    for (int k = 0; k < 3; ++k) so.vel[k] = velocity ? velocity[k] : 0.0f;
+    s->solids.push_back(so);
+    s->boundary_dirty = true;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_fluid_box(mgf_scene s, const float mn[3], const float mx[3]) {
+    if (!s || !mn || !mx) return fail(MGF_ERR_INVALID, "null argument");
+    hipStream_t st = s->ctx->stream;
+    MGF_HIP(launch_fluid_box(st, s->g, mn, mx, s->phi[s->cur]));
+    // generateFluidDomainFromBBox copies domain 1 into domain 2
+    MGF_HIP(hipMemcpyAsync(s->phi[1 - s->cur], s->phi[s->cur], s->ncell * sizeof(float),
+                           hipMemcpyDeviceToDevice, st));
+    return sync(s);
+}
+
+mgf_status mgf_scene_field_count(mgf_scene s, int32_t field, int64_t* out) {
+    if (!s || !out || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
+    *out = int64_t(field_count(s, field));
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64_t count) {
+    if (!s || !host || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
+    if (size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
+    hipStream_t st = s->ctx->stream;
+    float* dst = nullptr;
+    const int c = s->cur;
+    switch (field) {
+        case MGF_FIELD_VEL_X: dst = s->u[c]; break;
+        case MGF_FIELD_VEL_Y: dst = s->v[c]; break;
+        case MGF_FIELD_VEL_Z: dst = s->w[c]; break;
+        case MGF_FIELD_FLUID_DOMAIN: dst = s->phi[c]; break;
+        case MGF_FIELD_SOLID_SDF: dst = s->solid_total; break;
+        case MGF_FIELD_SOLID_VEL_X: dst = s->us; break;
+        case MGF_FIELD_SOLID_VEL_Y: dst = s->vs; break;
+        case MGF_FIELD_SOLID_VEL_Z: dst = s->ws; break;
+        default: return fail(MGF_ERR_INVALID, "field is download-only");
+    }
+    MGF_HIP(hipMemcpyAsync(dst, host, size_t(count) * sizeof(float), hipMemcpyHostToDevice, st));
+    if (field == MGF_FIELD_SOLID_SDF) MGF_HIP(launch_solid_mask(st, s->ncell, s->solid_total, s->solid_mask));
+    // The next boundary refresh (mgf_step) recomputes these from the solids,
+    // as CBoundarys::updateBoundarys overwrites setSolidVelField.
+    if (field >= MGF_FIELD_SOLID_SDF && field <= MGF_FIELD_SOLID_VEL_Z) s->boundary_dirty = true;
+    return sync(s);
+}
+
+mgf_status mgf_scene_download(mgf_scene s, int32_t field, float* host, int64_t count) {
+    if (!s || !host || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
+    if (size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
+    hipStream_t st = s->ctx->stream;
+    const int c = s->cur;
+    const float* src = nullptr;
+    float* tmp = nullptr;
+    switch (field) {
+        case MGF_FIELD_VEL_X: src = s->u[c]; break;
+        case MGF_FIELD_VEL_Y: src = s->v[c]; break;
+        case MGF_FIELD_VEL_Z: src = s->w[c]; break;
+        case MGF_FIELD_FLUID_DOMAIN: src = s->phi[c]; break;
+        case MGF_FIELD_SOLID_SDF: src = s->solid_total; break;
+        case MGF_FIELD_SOLID_VEL_X: src = s->us; break;
+        case MGF_FIELD_SOLID_VEL_Y: src = s->vs; break;
+        case MGF_FIELD_SOLID_VEL_Z: src = s->ws; break;
+        case MGF_FIELD_RHS: src = s->b; break;
+        case MGF_FIELD_PRESSURE:
+            if (s->x_is_64) {
+                std::vector<double> h(s->ncell);
+                MGF_HIP(hipMemcpyAsync(h.data(), s->x64, s->ncell * sizeof(double), hipMemcpyDeviceToHost, st));
+                TRY(sync(s));
+                for (size_t i = 0; i < s->ncell; ++i) host[i] = float(h[i]);
+                return MGF_OK;
+            }
+            src = s->x;
+            break;
+        case MGF_FIELD_MARKERS:
+            MGF_HIP(launch_u8_to_f32(st, s->markers, s->ap, s->ncell));
+            src = s->ap;
+            break;
+        case MGF_FIELD_DIVERGENCE:
+            MGF_HIP(launch_divergence(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), s->ap));
+            src = s->ap;
+            break;
+        case MGF_FIELD_MATRIX_A: {
+            float sa[3], sg[3];
+            scales(s, sa, sg);
+            if (hipMalloc(reinterpret_cast<void**>(&tmp), 4 * s->ncell * sizeof(float)) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(MGF_ERR_NOMEM, "temporary matrix buffer");
+            }
+            MGF_HIP(launch_matrix_a(st, s->g, sa, s->markers, tmp));
+            src = tmp;
+            break;
+        }
+        default: return fail(MGF_ERR_INVALID, "unknown field");
+    }
+    hipError_t e = hipMemcpyAsync(host, src, size_t(count) * sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(MGF_ERR_DEVICE, std::string("download: ") + hipGetErrorString(e));
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_update_boundaries(mgf_scene s) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    TRY(refresh_boundaries(s));
+    return sync(s);
+}
+
+mgf_status mgf_scene_advect(mgf_scene s) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    TRY(advect(s, false));
+    return sync(s);
+}
+
+mgf_status mgf_scene_apply_forces(mgf_scene s) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    TRY(apply_forces(s));
+    return sync(s);
+}
+
+mgf_status mgf_scene_pressure_solve(mgf_scene s, mgf_step_stats* stats) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    mgf_step_stats acc;
+    stats_reset(&acc);
+    TRY(pressure(s, &acc));
+    TRY(sync(s));
+    if (stats) *stats = acc;
+    return acc.converged ? MGF_OK : MGF_ERR_NOT_CONVERGED;
+}
+
+mgf_status mgf_scene_extrapolate_velocity(mgf_scene s, int32_t* layers_used) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    int used = 0;
+    TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
+    if (layers_used) *layers_used = used;
+    return sync(s);
+}
+
+mgf_status mgf_step(mgf_scene s, int32_t nsteps, mgf_step_stats* stats) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    if (nsteps < 0) return fail(MGF_ERR_INVALID, "nsteps < 0");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    mgf_step_stats acc;
+    stats_reset(&acc);
+    for (int i = 0; i < nsteps; ++i) TRY(step_once(s, &acc));
+    TRY(sync(s));
+    if (stats) *stats = acc;
+    return acc.converged ? MGF_OK : MGF_ERR_NOT_CONVERGED;
+}
+
+mgf_status mgf_scene_sync(mgf_scene s) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    return sync(s);
+}
+
+mgf_status mgf_scene_set_timing(mgf_scene s, int32_t enabled) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    s->timing = enabled != 0;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_time_cg_kernels(mgf_scene s, int32_t iters, double* ms_p_ap, double* ms_update) {
+    if (!s || iters < 1 || !ms_p_ap || !ms_update) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->ctx->stream;
+    float sa[3], sg[3];
+    scales(s, sa, sg);
+    std::vector<hipEvent_t> ev(size_t(iters) * 4);
+    for (auto& e : ev) MGF_HIP(hipEventCreate(&e));
+    // target 0: never stops on its own
+    hipError_t err = launch_cg_init(st, s->g, s->b, s->r, nullptr, s->x, s->scal, s->red, 0.0, 0.0, 1);
+    float* pold = s->p0;
+    float* pnew = s->p1;
+    for (int i = 0; i < iters && err == hipSuccess; ++i) {
+        err = hipEventRecord(ev[4 * i], st);
+        if (err == hipSuccess) err = launch_cg_p_ap(st, s->g, sa, s->markers, s->r, pold, pnew, s->scal, s->red);
+        if (err == hipSuccess) err = hipEventRecord(ev[4 * i + 1], st);
+        if (err == hipSuccess) err = hipEventRecord(ev[4 * i + 2], st);
+        if (err == hipSuccess) err = launch_cg_update(st, s->g, sa, s->markers, pnew, s->x, s->r, s->scal, s->red);
+        if (err == hipSuccess) err = hipEventRecord(ev[4 * i + 3], st);
+        std::swap(pold, pnew);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    double a = 0, b = 0;
+    for (int i = 0; i < iters && err == hipSuccess; ++i) {
+        float t0 = 0, t1 = 0;
+        err = hipEventElapsedTime(&t0, ev[4 * i], ev[4 * i + 1]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&t1, ev[4 * i + 2], ev[4 * i + 3]);
+        a += t0;
+        b += t1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (err != hipSuccess) return fail(MGF_ERR_DEVICE, std::string("time_cg: ") + hipGetErrorString(err));
+    CgScalars hs{};
+    MGF_HIP(hipMemcpy(&hs, s->scal, sizeof(hs), hipMemcpyDeviceToHost));
+    if (hs.done) return fail(MGF_ERR_INVALID, "the system converged before the timed iterations ended");
+    *ms_p_ap = a / iters;
+    *ms_update = b / iters;
+    return MGF_OK;
+}
+
+// ---- operator entry points --------------------------------------------------
+mgf_status mgf_device_alloc(mgf_context ctx, int64_t bytes, void** out) {
+    if (!ctx || !out || bytes < 0) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipSetDevice(ctx->device));
+    if (hipMalloc(out, size_t(std::max<int64_t>(bytes, 1))) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_NOMEM, "hipMalloc failed");
+    }
+    return MGF_OK;
+}
+
+mgf_status mgf_device_free(mgf_context ctx, void* p) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    MGF_HIP(hipFree(p));
+    return MGF_OK;
+}
+
+mgf_status mgf_memcpy_h2d(mgf_context ctx, void* dst, const void* src, int64_t bytes) {
+    if (!ctx || !dst || !src || bytes < 0) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyHostToDevice, ctx->stream));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_memcpy_d2h(mgf_context ctx, void* dst, const void* src, int64_t bytes) {
+    if (!ctx || !dst || !src || bytes < 0) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToHost, ctx->stream));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+static mgf_status check_grid(const mgf_grid* g) {
+    if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(MGF_ERR_INVALID, "bad grid");
+    const double faces = double(g->nx + 1) * g->ny * g->nz + double(g->nx) * (g->ny + 1) * g->nz +
+                         double(g->nx) * g->ny * (g->nz + 1);
+    if (faces >= 1073741824.0) return fail(MGF_ERR_INVALID, "grid too large");
+    for (int k = 0; k < 3; ++k)
+        if (!(g->spacing[k] != 0.0f)) return fail(MGF_ERR_INVALID, "zero spacing");
+    return MGF_OK;
+}
+
+mgf_status mgf_op_backtrace(mgf_context ctx, const mgf_grid* grid, const float* u, const float* v,
+                            const float* w, float dt, int32_t rk_order, float* px, float* py,
+                            float* pz, int64_t count) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    TRY(check_grid(grid));
+    if (rk_order < 1 || rk_order > 3 || count < 0) return fail(MGF_ERR_INVALID, "bad order or count");
+    MGF_HIP(launch_backtrace(ctx->stream, make_grid(grid), vel_of(u, v, w), dt, rk_order, px, py, pz, count));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_sample_cc(mgf_context ctx, const mgf_grid* grid, const float* f, const float* px,
+                            const float* py, const float* pz, float* out, int64_t count) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    TRY(check_grid(grid));
+    if (count < 0) return fail(MGF_ERR_INVALID, "bad count");
+    MGF_HIP(launch_sample_cc(ctx->stream, make_grid(grid), f, px, py, pz, out, count));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_advect_fc(mgf_context ctx, const mgf_grid* grid, const float* iu, const float* iv,
+                            const float* iw, const float* u, const float* v, const float* w, float dt,
+                            int32_t rk_order, float* ou, float* ov, float* ow) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    TRY(check_grid(grid));
+    if (rk_order < 1 || rk_order > 3) return fail(MGF_ERR_INVALID, "bad order");
+    if (ou == iu || ov == iv || ow == iw || ou == u || ov == v || ow == w)
+        return fail(MGF_ERR_INVALID, "output must not alias an input");
+    MGF_HIP(launch_advect_fc(ctx->stream, make_grid(grid), vel_of(iu, iv, iw), vel_of(u, v, w), dt,
+                             rk_order, ou, ov, ow, 0, nullptr, 0, nullptr));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_advect_cc(mgf_context ctx, const mgf_grid* grid, const float* in, const float* u,
+                            const float* v, const float* w, float dt, int32_t rk_order, float* out) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    TRY(check_grid(grid));
+    if (rk_order < 1 || rk_order > 3) return fail(MGF_ERR_INVALID, "bad order");
+    if (out == in) return fail(MGF_ERR_INVALID, "output must not alias the input");
+    MGF_HIP(launch_advect_cc(ctx->stream, make_grid(grid), in, vel_of(u, v, w), dt, rk_order, out));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_pressure_matvec(mgf_context ctx, const mgf_grid* grid, const float scale[3],
+                                  const uint8_t* markers, const float* x, float* y) {
+    if (!ctx || !scale) return fail(MGF_ERR_INVALID, "null argument");
+    TRY(check_grid(grid));
+    if (x == y) return fail(MGF_ERR_INVALID, "y must not alias x");
+    MGF_HIP(launch_matvec(ctx->stream, make_grid(grid), scale, markers, x, y));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid, float* u, float* v, float* w,
+                                 int32_t layers, int32_t* layers_used) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    TRY(check_grid(grid));
+    if (layers < 0) return fail(MGF_ERR_INVALID, "layers < 0");
+    // A transient scene-shaped scratch holder for the frontier buffers.
+    mgf_scene_s tmp;
+    tmp.ctx = ctx;
+    tmp.g = make_grid(grid);
+    const size_t nface = size_t(grid->nx + 1) * grid->ny * grid->nz +
+                         size_t(grid->nx) * (grid->ny + 1) * grid->nz +
+                         size_t(grid->nx) * grid->ny * (grid->nz + 1);
+    mgf_status st = MGF_OK;
+    st = dalloc(&tmp, &tmp.layer, nface);
+    if (st == MGF_OK) st = dalloc(&tmp, &tmp.list[0], nface);
+    if (st == MGF_OK) st = dalloc(&tmp, &tmp.list[1], nface);
+    if (st == MGF_OK) st = dalloc(&tmp, &tmp.counts, kLayerChunk + 1);
+    int used = 0;
+    if (st == MGF_OK) st = extrapolate(&tmp, u, v, w, layers, &used);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : tmp.allocations) (void)hipFree(p);
+    if (layers_used) *layers_used = used;
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// mgf_internal.h — host-side state of the library and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mgf.h"
+#include "mgf_device.h"
+
+namespace mgf {
+
+// Device scalars of the fused CG (one struct per scene, in device memory).
+struct CgScalars {
+    double rz;        // r . z of the current iteration
+    double rz_new;
+    double pap;       // p . A p
+    double alpha;
+    double beta;
+    double rr;        // ||r||^2 (recurrence)
+    double target2;   // stop when rr <= target2
+    double sweep_r0;  // ||r||^2 at the start of the sweep
+    int32_t done;     // 1 once rr <= target2 (or pap == 0)
+    int32_t iters;    // iterations completed in this sweep
+    int32_t pad0, pad1;
+};
+
+struct ReduceBuf {
+    double* partials = nullptr;     // [kMaxBlocks * 4]
+    unsigned int* ticket = nullptr; // one counter per reduction site
+    int blocks = 0;                 // blocks used by the fused kernels
+};
+
+struct Solid {
+    float* sdf0 = nullptr;   // original SDF on the device
+    float vel[3] = {0, 0, 0};
+    float trans[3] = {0, 0, 0};
+};
+
+}  // namespace mgf
+
+struct mgf_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+};
+
+struct mgf_scene_s {
+    mgf_context ctx = nullptr;
+    mgf_scene_desc desc{};
+    mgf::GridDev g{};
+    size_t ncell = 0, nfx = 0, nfy = 0, nfz = 0, nvec = 0;
+
+    // current velocity / fluid domain are u[cur] etc.; advection writes 1-cur.
+    float* u[2] = {nullptr, nullptr};
+    float* v[2] = {nullptr, nullptr};
+    float* w[2] = {nullptr, nullptr};
+    float* phi[2] = {nullptr, nullptr};
+    int cur = 0;
+
+    float* solid_total = nullptr;  // CBoundarys total marker: -k or UNKNOWN (0 with no solid)
+    uint8_t* solid_mask = nullptr; // 1 where solid_total < 0
+    float* us = nullptr;
+    float* vs = nullptr;
+    float* ws = nullptr;
+    uint8_t* markers = nullptr;    // 0 air / 1 fluid / 2 solid
+
+    // pressure system
+    float* b = nullptr;
+    float* x = nullptr;            // FP32 pressure (plain) / inner correction (mixed)
+    double* x64 = nullptr;         // FP64 pressure (mixed)
+    float* r = nullptr;
+    float* p0 = nullptr;
+    float* p1 = nullptr;
+    float* ap = nullptr;           // plain: A p
+    mgf::CgScalars* scal = nullptr;
+    mgf::ReduceBuf red;
+    double* host_scratch = nullptr; // pinned host mirror of small reductions
+    bool x_is_64 = false;          // which pressure the last solve produced
+
+    // extrapolation scratch
+    int32_t* layer = nullptr;      // nfx+nfy+nfz
+    uint32_t* list[2] = {nullptr, nullptr};
+    uint32_t* counts = nullptr;    // per-layer list sizes [kMaxLayerChunk+2]
+
+    std::vector<mgf::Solid> solids;
+    bool boundary_dirty = false;   // solids changed since the last refresh
+    bool boundary_user = false;    // SOLID_SDF / SOLID_VEL uploaded directly
+    bool timing = false;
+
+    int64_t device_bytes = 0;
+    std::vector<void*> allocations;
+};
+
+namespace mgf {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBlocks = 4096;   // fused reductions use at most this many blocks
+
+// Error helpers.
+void set_error(const std::string& msg);
+#define MGF_HIP(expr)                                                           \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::mgf::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return MGF_ERR_DEVICE;                                              \
+        }                                                                       \
+    } while (0)
+
+inline GridDev make_grid(const mgf_grid* g) {
+    GridDev d;
+    d.nx = g->nx; d.ny = g->ny; d.nz = g->nz;
+    d.ox = g->origin[0]; d.oy = g->origin[1]; d.oz = g->origin[2];
+    d.hx = g->spacing[0]; d.hy = g->spacing[1]; d.hz = g->spacing[2];
+    return d;
+}
+
+RkCoef rk_coef(float dt);
+
+// ---- launchers (mgf_kernels.hip) ------------------------------------------
+hipError_t launch_backtrace(hipStream_t s, const GridDev& g, Vel V, float dt, int order,
+                            float* px, float* py, float* pz, int64_t n);
+hipError_t launch_sample_cc(hipStream_t s, const GridDev& g, const float* f, const float* px,
+                            const float* py, const float* pz, float* out, int64_t n);
+// Advects the three face components and applies the uniform increments
+// inc0 (gravity*dt) and inc1 (force*dt) when enabled.
+hipError_t launch_advect_fc(hipStream_t s, const GridDev& g, Vel in, Vel V, float dt, int order,
+                            float* ou, float* ov, float* ow, int add0, const float inc0[3],
+                            int add1, const float inc1[3]);
+hipError_t launch_advect_cc(hipStream_t s, const GridDev& g, const float* in, Vel V, float dt,
+                            int order, float* out);
+hipError_t launch_add_uniform(hipStream_t s, float* a, size_t n, float inc);
+hipError_t launch_markers(hipStream_t s, const GridDev& g, const uint8_t* solid_mask,
+                          const float* fluid, uint8_t* markers);
+hipError_t launch_rhs(hipStream_t s, const GridDev& g, Vel U, Vel S, const uint8_t* markers,
+                      float* b);
+hipError_t launch_matrix_a(hipStream_t s, const GridDev& g, const float scale[3],
+                           const uint8_t* markers, float* A);
+hipError_t launch_divergence(hipStream_t s, const GridDev& g, Vel U, float* div);
+hipError_t launch_matvec(hipStream_t s, const GridDev& g, const float scale[3],
+                         const uint8_t* markers, const float* x, float* y);
+// Plain BLAS-1 (FP64 partial sums; result written to partials, reduced on host)
+hipError_t launch_dot_partials(hipStream_t s, const float* a, const float* b, size_t n,
+                               double* partials, int blocks);
+hipError_t launch_axpy(hipStream_t s, float alpha, const float* x, float* y, size_t n);
+hipError_t launch_scal_add(hipStream_t s, float beta, float* p, const float* z, size_t n);
+hipError_t launch_fill(hipStream_t s, float* a, size_t n, float v);
+hipError_t launch_fill64(hipStream_t s, double* a, size_t n, double v);
+// Fused mixed-precision CG.
+hipError_t launch_cg_init(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* p,
+                          float* x, CgScalars* sc, ReduceBuf red, double rel_inner,
+                          double abs_target2, int first_sweep);
+hipError_t launch_cg_p_ap(hipStream_t s, const GridDev& g, const float scale[3],
+                          const uint8_t* markers, const float* r, const float* p_old,
+                          float* p_new, CgScalars* sc, ReduceBuf red);
+hipError_t launch_cg_update(hipStream_t s, const GridDev& g, const float scale[3],
+                            const uint8_t* markers, const float* p, float* x, float* r,
+                            CgScalars* sc, ReduceBuf red);
+// x64 += x (FP32 correction), then r = b - A x64 in FP64 (stored FP32) and
+// ||r||^2 into partial sums (reduced by the host).
+hipError_t launch_refine(hipStream_t s, const GridDev& g, const float scale[3],
+                         const uint8_t* markers, const float* xcorr, double* x64,
+                         const float* b, float* r, double* partials, int blocks,
+                         int accumulate);
+// FP64 ||b - A x|| and ||b|| partials for an FP32 pressure.
+hipError_t launch_true_residual32(hipStream_t s, const GridDev& g, const float scale[3],
+                                  const uint8_t* markers, const float* x, const float* b,
+                                  double* partials, int blocks);
+hipError_t launch_project(hipStream_t s, const GridDev& g, const float sg[3],
+                          const uint8_t* markers, const float* p32, const double* p64,
+                          float* u, float* v, float* w, const float* us, const float* vs,
+                          const float* ws);
+// Extrapolation by frontiers.
+hipError_t launch_extrap_init(hipStream_t s, const GridDev& g, const float* u, const float* v,
+                              const float* w, int32_t* layer, uint32_t* list, uint32_t* count);
+hipError_t launch_extrap_layer(hipStream_t s, const GridDev& g, int layer_id, float* u, float* v,
+                               float* w, int32_t* layer, const uint32_t* in_list,
+                               const uint32_t* in_count, uint32_t* out_list,
+                               uint32_t* out_count);
+// Boundaries.
+hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0,
+                                 const float t[3], float* out);
+hipError_t launch_solids_marker(hipStream_t s, size_t n, const float* sdf, int index1,
+                                float* marker);
+hipError_t launch_solid_mask(hipStream_t s, size_t n, const float* total, uint8_t* mask);
+hipError_t launch_solids_vel(hipStream_t s, const GridDev& g, const float* total,
+                             const float* vel3, int nsolids, float* us, float* vs, float* ws);
+hipError_t launch_fluid_box(hipStream_t s, const GridDev& g, const float mn[3],
+                            const float mx[3], float* phi);
+hipError_t launch_u8_to_f32(hipStream_t s, const uint8_t* a, float* b, size_t n);
+
+}  // namespace mgf

@@ -1,0 +1,1041 @@
+// mgf_kernels.hip — HIP kernels of the MAC-grid step for gfx950 (wave64).
+//
+// Layout: structure-of-arrays, x fastest, face arrays with one extra layer on
+// their own axis (the reference's CFaceCenteredVectorField layout).  Markers
+// are uint8 (0 air, 1 fluid, 2 solid).  Every kernel is memory-bound; blocks
+// are 256 work-items and block IDs are remapped so that each of the 8 XCDs
+// walks one contiguous slab of the grid (neighbouring planes then meet in the
+// same L2).
+#include "mgf_internal.h"
+
+namespace mgf {
+
+constexpr uint32_t kXcds = 8;
+
+// Workgroups are dispatched round-robin over the XCDs: physical block b runs on
+// XCD b % 8.  Map it to logical block (b % 8) * (G / 8) + b / 8 so that XCD k
+// owns logical blocks [k G/8, (k+1) G/8).  Requires G % 8 == 0.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    return (b % kXcds) * (G / kXcds) + b / kXcds;
+}
+
+static inline uint32_t grid_for(size_t n) {
+    size_t g = (n + kBlock - 1) / kBlock;
+    g = (g + kXcds - 1) / kXcds * kXcds;
+    if (g == 0) g = kXcds;
+    return uint32_t(g);
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 / block reductions in FP64.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sums `n` doubles of a block (blockDim.x == kBlock); result valid in thread 0.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+    __shared__ double sh[N][kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < kBlock / 64; ++q) s += sh[k][q];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// One-pass grid reduction: every block stores its partials, the last block to
+// arrive (ticket) sums all partials in block order and returns true in all of
+// its threads; sums are then in out[] (valid in thread 0).
+template <int N>
+__device__ __forceinline__ bool grid_sum(double (&v)[N], double* partials, unsigned int* ticket,
+                                         double (&out)[N]) {
+    block_sum<N>(v);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            __hip_atomic_store(&partials[size_t(blockIdx.x) * N + k], v[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned int t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0;
+    for (uint32_t bI = threadIdx.x; bI < gridDim.x; bI += kBlock) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            acc[k] += __hip_atomic_load(&partials[size_t(bI) * N + k], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<N>(acc);
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = acc[k];
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Back-trace of arbitrary points and cell-centred sampling (operator seams).
+__global__ void __launch_bounds__(kBlock) k_backtrace(GridDev g, Vel V, RkCoef k, int order,
+                                                      float* px, float* py, float* pz, int64_t n) {
+    const int64_t i = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float x = px[i], y = py[i], z = pz[i];
+    backtrace(V, g, order, k, x, y, z);
+    px[i] = x; py[i] = y; pz[i] = z;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sample_cc(GridDev g, const float* f, const float* px,
+                                                      const float* py, const float* pz, float* out,
+                                                      int64_t n) {
+    const int64_t i = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    out[i] = trilerp(f, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px[i], py[i], pz[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Semi-Lagrangian advection of the face-centred velocity
+// (CSemiLagrangian::advect(FCV), MPhysics/SemiLagrangian.cpp:169-186) with
+// the uniform body-force increments of CExternalForcesSolver fused after the
+// sample (MPhysics/ExternalForcesSolver.cpp:50-59).
+struct Inc {
+    float a[3];
+    float b[3];
+    int add_a, add_b;
+};
+
+__global__ void __launch_bounds__(kBlock) k_advect_fc(GridDev g, Vel in, Vel V, RkCoef k, int order,
+                                                      float* __restrict__ ou, float* __restrict__ ov,
+                                                      float* __restrict__ ow, uint32_t nfx,
+                                                      uint32_t nfy, uint32_t nfz, Inc inc) {
+    uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    int c;
+    if (i < nfx) c = 0;
+    else if (i < nfx + nfy) { c = 1; i -= nfx; }
+    else if (i < nfx + nfy + nfz) { c = 2; i -= nfx + nfy; }
+    else return;
+    const uint32_t ex = uint32_t(g.nx + (c == 0)), ey = uint32_t(g.ny + (c == 1));
+    const int x = int(i % ex), y = int((i / ex) % ey), z = int(i / (ex * ey));
+    float px = c == 0 ? fpos(x, g.hx, g.ox) : cpos(x, g.hx, g.ox);
+    float py = c == 1 ? fpos(y, g.hy, g.oy) : cpos(y, g.hy, g.oy);
+    float pz = c == 2 ? fpos(z, g.hz, g.oz) : cpos(z, g.hz, g.oz);
+    backtrace(V, g, order, k, px, py, pz);
+    const float* src = c == 0 ? in.u : (c == 1 ? in.v : in.w);
+    float val = sample_comp(src, c, g, px, py, pz);
+    if (inc.add_a) val = val + inc.a[c];
+    if (inc.add_b) val = val + inc.b[c];
+    float* dst = c == 0 ? ou : (c == 1 ? ov : ow);
+    dst[i] = val;
+}
+
+// CSemiLagrangian::advect(CCS) (MPhysics/SemiLagrangian.cpp:135-150).
+__global__ void __launch_bounds__(kBlock) k_advect_cc(GridDev g, const float* __restrict__ in, Vel V,
+                                                      RkCoef k, int order, float* __restrict__ out,
+                                                      uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
+    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    float px = cpos(x, g.hx, g.ox), py = cpos(y, g.hy, g.oy), pz = cpos(z, g.hz, g.oz);
+    backtrace(V, g, order, k, px, py, pz);
+    out[i] = trilerp(in, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px, py, pz);
+}
+
+// plusAlphaX with a unit field: a[i] = inc*1 + a[i].
+__global__ void __launch_bounds__(kBlock) k_add_uniform(float* a, size_t n, float inc) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) a[i] = a[i] + inc;
+}
+
+// ---------------------------------------------------------------------------
+// buildFluidMarkers (MPhysics/EulerSolverTool.cu:30-56).
+__global__ void __launch_bounds__(kBlock) k_markers(uint32_t n, const uint8_t* __restrict__ solid,
+                                                    const float* __restrict__ fluid,
+                                                    uint8_t* __restrict__ m) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    m[i] = solid[i] ? kSolid : (fluid[i] < 0.0f ? kFluid : kAir);
+}
+
+struct MarkerView {
+    const uint8_t* __restrict__ m;
+    int nx, ny, nz;
+    __device__ __forceinline__ uint8_t operator()(int x, int y, int z) const {
+        return m[lin3(x, y, z, nx, ny, nz)];
+    }
+};
+
+// Divergence + buildPressureVectorb (MPhysics/FieldMathTool.cu:370-395,
+// MPhysics/EulerSolverTool.cu:128-197): b = -div on fluid cells, corrected by
+// (u - u_solid)/h on each face shared with a solid (clamped) neighbour.
+__global__ void __launch_bounds__(kBlock) k_rhs(GridDev g, Vel U, Vel S, const uint8_t* __restrict__ m,
+                                                float sx, float sy, float sz, float* __restrict__ b,
+                                                uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    if (m[i] != kFluid) { b[i] = 0.0f; return; }
+    const uint32_t fl = lin3(x, y, z, nx + 1, ny, nz), fr = lin3(x + 1, y, z, nx + 1, ny, nz);
+    const uint32_t fd = lin3(x, y, z, nx, ny + 1, nz), fu = lin3(x, y + 1, z, nx, ny + 1, nz);
+    const uint32_t fb = lin3(x, y, z, nx, ny, nz + 1), ff = lin3(x, y, z + 1, nx, ny, nz + 1);
+    const float ul = U.u[fl], ur = U.u[fr], vd = U.v[fd], vu = U.v[fu], wb = U.w[fb], wf = U.w[ff];
+    const float div = (ur - ul) / g.hx + (vu - vd) / g.hy + (wf - wb) / g.hz;
+    float val = -div;
+    if (m[lin3(x - 1, y, z, nx, ny, nz)] == kSolid) val -= sx * (ul - S.u[fl]);
+    if (m[lin3(x + 1, y, z, nx, ny, nz)] == kSolid) val += sx * (ur - S.u[fr]);
+    if (m[lin3(x, y - 1, z, nx, ny, nz)] == kSolid) val -= sy * (vd - S.v[fd]);
+    if (m[lin3(x, y + 1, z, nx, ny, nz)] == kSolid) val += sy * (vu - S.v[fu]);
+    if (m[lin3(x, y, z - 1, nx, ny, nz)] == kSolid) val -= sz * (wb - S.w[fb]);
+    if (m[lin3(x, y, z + 1, nx, ny, nz)] == kSolid) val += sz * (wf - S.w[ff]);
+    b[i] = val;
+}
+
+// CUDADivergence alone (download of MGF_FIELD_DIVERGENCE).
+__global__ void __launch_bounds__(kBlock) k_divergence(GridDev g, Vel U, float* __restrict__ d,
+                                                       uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    const float ul = U.u[lin3(x, y, z, nx + 1, ny, nz)], ur = U.u[lin3(x + 1, y, z, nx + 1, ny, nz)];
+    const float vd = U.v[lin3(x, y, z, nx, ny + 1, nz)], vu = U.v[lin3(x, y + 1, z, nx, ny + 1, nz)];
+    const float wb = U.w[lin3(x, y, z, nx, ny, nz + 1)], wf = U.w[lin3(x, y, z + 1, nx, ny, nz + 1)];
+    d[i] = (ur - ul) / g.hx + (vu - vd) / g.hy + (wf - wb) / g.hz;
+}
+
+// The compact matrix [diag,+x,+y,+z] (download of MGF_FIELD_MATRIX_A).
+__global__ void __launch_bounds__(kBlock) k_matrix_a(GridDev g, const uint8_t* __restrict__ m,
+                                                     float sx, float sy, float sz,
+                                                     float* __restrict__ A, uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    MarkerView mv{m, nx, ny, nz};
+    const Row r = matrix_row(mv, x, y, z, nx, ny, nz, sx, sy, sz);
+    A[4 * size_t(i)] = r.d;
+    A[4 * size_t(i) + 1] = r.ax;
+    A[4 * size_t(i) + 2] = r.ay;
+    A[4 * size_t(i) + 3] = r.az;
+}
+
+// ---------------------------------------------------------------------------
+// Stencil of the pressure matrix at cell (x,y,z) given a value accessor.
+// Returns the seven (coefficient, value) pairs in the reference's order:
+// centre, -x, +x, -y, +y, -z, +z; out-of-grid terms are flagged absent.
+struct Stencil {
+    float coef[7];
+    bool on[7];
+    uint32_t idx[7];
+};
+
+__device__ __forceinline__ Stencil stencil_at(const uint8_t* __restrict__ m, int x, int y, int z,
+                                              int nx, int ny, int nz, float sx, float sy, float sz) {
+    Stencil st;
+    const uint32_t c = lin3(x, y, z, nx, ny, nz);
+    st.idx[0] = c;
+    st.idx[1] = lin3(x - 1, y, z, nx, ny, nz);
+    st.idx[2] = lin3(x + 1, y, z, nx, ny, nz);
+    st.idx[3] = lin3(x, y - 1, z, nx, ny, nz);
+    st.idx[4] = lin3(x, y + 1, z, nx, ny, nz);
+    st.idx[5] = lin3(x, y, z - 1, nx, ny, nz);
+    st.idx[6] = lin3(x, y, z + 1, nx, ny, nz);
+    st.on[0] = true;
+    st.on[1] = x > 0; st.on[2] = x + 1 < nx;
+    st.on[3] = y > 0; st.on[4] = y + 1 < ny;
+    st.on[5] = z > 0; st.on[6] = z + 1 < nz;
+    const uint8_t mc = m[c];
+    uint8_t mn[7];
+#pragma unroll
+    for (int q = 1; q < 7; ++q) mn[q] = m[st.idx[q]];
+    float d = 0.0f;
+    if (mc == kFluid) {
+        if (st.on[1] && mn[1] != kSolid) d += sx;
+        if (st.on[2] && mn[2] != kSolid) d += sx;
+        if (st.on[3] && mn[3] != kSolid) d += sy;
+        if (st.on[4] && mn[4] != kSolid) d += sy;
+        if (st.on[5] && mn[5] != kSolid) d += sz;
+        if (st.on[6] && mn[6] != kSolid) d += sz;
+    } else {
+        d = 1.0f;
+    }
+    st.coef[0] = d;
+    st.coef[1] = plus_coupling(mn[1], mc, sx);
+    st.coef[2] = plus_coupling(mc, mn[2], sx);
+    st.coef[3] = plus_coupling(mn[3], mc, sy);
+    st.coef[4] = plus_coupling(mc, mn[4], sy);
+    st.coef[5] = plus_coupling(mn[5], mc, sz);
+    st.coef[6] = plus_coupling(mc, mn[6], sz);
+    return st;
+}
+
+// fdmMatrixVectorMul (MPhysics/EulerSolverTool.cu:199-227): products in FP32,
+// summed in FP64 in the reference's order, stored as FP32.
+__global__ void __launch_bounds__(kBlock) k_matvec(GridDev g, const uint8_t* __restrict__ m, float sx,
+                                                   float sy, float sz, const float* __restrict__ xv,
+                                                   float* __restrict__ y, uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int x = int(i % uint32_t(nx)), yy = int((i / uint32_t(nx)) % uint32_t(ny)),
+              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    const Stencil st = stencil_at(m, x, yy, z, nx, ny, nz, sx, sy, sz);
+    double acc = double(st.coef[0] * xv[st.idx[0]]);
+#pragma unroll
+    for (int q = 1; q < 7; ++q) acc = acc + (st.on[q] ? double(st.coef[q] * xv[st.idx[q]]) : 0.0);
+    y[i] = float(acc);
+}
+
+// ---------------------------------------------------------------------------
+// Plain BLAS-1 (the reference's cuBLAS calls, CuDenseVector.cpp:120-240).
+__global__ void __launch_bounds__(kBlock) k_dot_partials(const float* __restrict__ a,
+                                                         const float* __restrict__ b, size_t n,
+                                                         double* partials) {
+    double s[1] = {0.0};
+    const size_t stride = size_t(gridDim.x) * kBlock;
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        s[0] += double(a[i]) * double(b[i]);
+    block_sum<1>(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_axpy(float a, const float* __restrict__ x,
+                                                 float* __restrict__ y, size_t n) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) y[i] = __builtin_fmaf(a, x[i], y[i]);
+}
+
+// m_P *= beta; m_P += z  (scale, then axpy with alpha 1: two roundings)
+__global__ void __launch_bounds__(kBlock) k_scal_add(float beta, float* __restrict__ p,
+                                                     const float* __restrict__ z, size_t n) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) {
+        const float t = beta * p[i];
+        p[i] = __builtin_fmaf(1.0f, z[i], t);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill(float* a, size_t n, float v) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill64(double* a, size_t n, double v) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_u8_to_f32(const uint8_t* a, float* b, size_t n) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) b[i] = float(a[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused mixed-precision CG.  Each block owns `chunk` consecutive cells.
+struct Span {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ Span block_span(uint32_t n) {
+    const uint32_t G = gridDim.x;
+    const uint32_t per = ((n + G - 1) / G + kBlock - 1) / kBlock * kBlock;
+    const uint32_t b = xcd_block();
+    const uint32_t lo = b * per;
+    uint32_t hi = lo + per;
+    if (lo > n) return {n, n};
+    if (hi > n) hi = n;
+    return {lo, hi};
+}
+
+// Sweep start: r from rsrc (b on the first sweep, the FP64-formed residual
+// afterwards), x = 0, p unused until the first iteration; sets rz = rr and
+// the stop target.
+__global__ void __launch_bounds__(kBlock) k_cg_init(uint32_t n, const float* __restrict__ rsrc,
+                                                    float* __restrict__ r, float* __restrict__ x,
+                                                    CgScalars* sc, double* partials,
+                                                    unsigned int* ticket, double rel_inner,
+                                                    double abs_target2) {
+    const Span sp = block_span(n);
+    double s[1] = {0.0};
+    for (uint32_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBlock) {
+        const float v = rsrc[i];
+        if (r != rsrc) r[i] = v;
+        x[i] = 0.0f;
+        s[0] += double(v) * double(v);
+    }
+    double tot[1];
+    if (grid_sum<1>(s, partials, ticket, tot) && threadIdx.x == 0) {
+        sc->rz = tot[0];
+        sc->rr = tot[0];
+        sc->sweep_r0 = tot[0];
+        const double rel2 = rel_inner * rel_inner * tot[0];
+        sc->target2 = rel2 > abs_target2 ? rel2 : abs_target2;
+        sc->done = (tot[0] <= sc->target2 || tot[0] == 0.0) ? 1 : 0;
+        sc->iters = 0;
+        sc->beta = 0.0;
+        sc->alpha = 0.0;
+    }
+}
+
+// p_new = r + beta * p_old (p_new = r on the first iteration); A p_new with
+// the neighbours' p_new recomputed in registers; p . A p reduced; alpha.
+__global__ void __launch_bounds__(kBlock) k_cg_p_ap(GridDev g, const uint8_t* __restrict__ m,
+                                                    float sx, float sy, float sz,
+                                                    const float* __restrict__ r,
+                                                    const float* __restrict__ p_old,
+                                                    float* __restrict__ p_new, CgScalars* sc,
+                                                    double* partials, unsigned int* ticket) {
+    if (sc->done) return;
+    const double beta = sc->beta;
+    const bool first = sc->iters == 0;
+    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
+    const Span sp = block_span(n);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    double s[1] = {0.0};
+    for (uint32_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBlock) {
+        const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
+        const Stencil st = stencil_at(m, x, y, z, nx, ny, nz, sx, sy, sz);
+        double pv[7];
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            const uint32_t j = st.idx[q];
+            pv[q] = first ? double(r[j]) : double(float(__builtin_fma(beta, double(p_old[j]), double(r[j]))));
+        }
+        double ap = double(st.coef[0]) * pv[0];
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+            if (st.on[q]) ap += double(st.coef[q]) * pv[q];
+        p_new[i] = float(pv[0]);
+        s[0] += pv[0] * ap;
+    }
+    double tot[1];
+    if (grid_sum<1>(s, partials, ticket, tot) && threadIdx.x == 0) {
+        sc->pap = tot[0];
+        if (!isfinite(tot[0])) {
+            sc->done = 2;          // non-finite: stop, reported by the host
+            sc->alpha = 0.0;
+        } else if (tot[0] == 0.0) {
+            sc->done = 1;
+            sc->alpha = 0.0;
+        } else {
+            sc->alpha = sc->rz / tot[0];
+        }
+    }
+}
+
+// x += alpha p; r -= alpha A p (A p recomputed); r . r reduced; beta; stop test.
+__global__ void __launch_bounds__(kBlock) k_cg_update(GridDev g, const uint8_t* __restrict__ m,
+                                                      float sx, float sy, float sz,
+                                                      const float* __restrict__ p,
+                                                      float* __restrict__ x, float* __restrict__ r,
+                                                      CgScalars* sc, double* partials,
+                                                      unsigned int* ticket) {
+    if (sc->done) return;
+    const double alpha = sc->alpha;
+    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
+    const Span sp = block_span(n);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    double s[1] = {0.0};
+    for (uint32_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBlock) {
+        const int xx = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
+        const Stencil st = stencil_at(m, xx, y, z, nx, ny, nz, sx, sy, sz);
+        double ap = double(st.coef[0]) * double(p[st.idx[0]]);
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+            if (st.on[q]) ap += double(st.coef[q]) * double(p[st.idx[q]]);
+        const float xn = float(__builtin_fma(alpha, double(p[i]), double(x[i])));
+        const float rn = float(__builtin_fma(-alpha, ap, double(r[i])));
+        x[i] = xn;
+        r[i] = rn;
+        s[0] += double(rn) * double(rn);
+    }
+    double tot[1];
+    if (grid_sum<1>(s, partials, ticket, tot) && threadIdx.x == 0) {
+        const double rz_new = tot[0];
+        sc->rr = rz_new;
+        sc->beta = rz_new / sc->rz;
+        sc->rz = rz_new;
+        sc->iters = sc->iters + 1;
+        if (rz_new <= sc->target2) sc->done = 1;
+        else if (!isfinite(rz_new)) sc->done = 2;
+    }
+}
+
+// x64 (+)= x32 correction.
+__global__ void __launch_bounds__(kBlock) k_accumulate(uint32_t n, const float* __restrict__ xc,
+                                                       double* __restrict__ x64, int accumulate) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    x64[i] = accumulate ? x64[i] + double(xc[i]) : double(xc[i]);
+}
+
+// r = b - A x64 in FP64, stored FP32; partials of ||r||^2 and ||b||^2.
+__global__ void __launch_bounds__(kBlock) k_residual64(GridDev g, const uint8_t* __restrict__ m,
+                                                       float sx, float sy, float sz,
+                                                       const double* __restrict__ x64,
+                                                       const float* __restrict__ b,
+                                                       float* __restrict__ r, double* partials) {
+    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    double s[2] = {0.0, 0.0};
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
+        const Stencil st = stencil_at(m, x, y, z, nx, ny, nz, sx, sy, sz);
+        double ax = double(st.coef[0]) * x64[st.idx[0]];
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+            if (st.on[q]) ax += double(st.coef[q]) * x64[st.idx[q]];
+        const double bi = double(b[i]);
+        const double ri = bi - ax;
+        if (r) r[i] = float(ri);
+        s[0] += ri * ri;
+        s[1] += bi * bi;
+    }
+    block_sum<2>(s);
+    if (threadIdx.x == 0) {
+        partials[2 * size_t(blockIdx.x)] = s[0];
+        partials[2 * size_t(blockIdx.x) + 1] = s[1];
+    }
+}
+
+// FP64 ||b - A x||^2 and ||b||^2 partials for an FP32 pressure.
+__global__ void __launch_bounds__(kBlock) k_residual32(GridDev g, const uint8_t* __restrict__ m,
+                                                       float sx, float sy, float sz,
+                                                       const float* __restrict__ xv,
+                                                       const float* __restrict__ b,
+                                                       double* partials) {
+    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    double s[2] = {0.0, 0.0};
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
+        const Stencil st = stencil_at(m, x, y, z, nx, ny, nz, sx, sy, sz);
+        double ax = double(st.coef[0]) * double(xv[st.idx[0]]);
+#pragma unroll
+        for (int q = 1; q < 7; ++q)
+            if (st.on[q]) ax += double(st.coef[q]) * double(xv[st.idx[q]]);
+        const double bi = double(b[i]);
+        const double ri = bi - ax;
+        s[0] += ri * ri;
+        s[1] += bi * bi;
+    }
+    block_sum<2>(s);
+    if (threadIdx.x == 0) {
+        partials[2 * size_t(blockIdx.x)] = s[0];
+        partials[2 * size_t(blockIdx.x) + 1] = s[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// applyPressureGradient (MPhysics/EulerSolverTool.cu:229-322).  With an FP64
+// pressure the update is formed in FP64 and rounded once.
+__global__ void __launch_bounds__(kBlock) k_project(GridDev g, const uint8_t* __restrict__ m,
+                                                    float sx, float sy, float sz,
+                                                    const float* __restrict__ p32,
+                                                    const double* __restrict__ p64,
+                                                    float* __restrict__ u, float* __restrict__ v,
+                                                    float* __restrict__ w,
+                                                    const float* __restrict__ us,
+                                                    const float* __restrict__ vs,
+                                                    const float* __restrict__ ws, uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
+              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    const uint32_t l = lin3(x - 1, y, z, nx, ny, nz), d = lin3(x, y - 1, z, nx, ny, nz),
+                   bk = lin3(x, y, z - 1, nx, ny, nz);
+    const uint8_t mc = m[i], ml = m[l], md = m[d], mb = m[bk];
+    const uint32_t fu = lin3(x, y, z, nx + 1, ny, nz), fv = lin3(x, y, z, nx, ny + 1, nz),
+                   fw = lin3(x, y, z, nx, ny, nz + 1);
+    struct F {
+        uint8_t mo;
+        uint32_t o;
+        float s;
+        float* a;
+        const float* as;
+        uint32_t f;
+    } faces[3] = {{ml, l, sx, u, us, fu}, {md, d, sy, v, vs, fv}, {mb, bk, sz, w, ws, fw}};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const F& F_ = faces[q];
+        if (F_.mo == kFluid || mc == kFluid) {
+            if (F_.mo == kSolid || mc == kSolid) {
+                F_.a[F_.f] = F_.as[F_.f];
+            } else if (p64) {
+                F_.a[F_.f] = float(double(F_.a[F_.f]) - double(F_.s) * (p64[i] - p64[F_.o]));
+            } else {
+                F_.a[F_.f] = F_.a[F_.f] - F_.s * (p32[i] - p32[F_.o]);
+            }
+        } else {
+            F_.a[F_.f] = MGF_UNKNOWN;
+        }
+    }
+    if (x == nx - 1) u[lin3(x + 1, y, z, nx + 1, ny, nz)] = MGF_UNKNOWN;
+    if (y == ny - 1) v[lin3(x, y + 1, z, nx, ny + 1, nz)] = MGF_UNKNOWN;
+    if (z == nz - 1) w[lin3(x, y, z + 1, nx, ny, nz + 1)] = MGF_UNKNOWN;
+}
+
+// ---------------------------------------------------------------------------
+// Extrapolation (MPhysics/EulerSolverTool.cu:450-557, 904-969) by frontiers.
+// Entries of the three face grids are addressed as (component << 30) | index.
+constexpr int32_t kLayerUnknown = 0x7fffffff;
+
+struct FaceGrids {
+    float* f[3];
+    int ex[3], ey[3], ez[3];
+    uint32_t off[3];   // start of each component in the layer array
+};
+
+__device__ __forceinline__ void decode(const FaceGrids& G, uint32_t e, int& c, int& x, int& y, int& z) {
+    c = int(e >> 30);
+    const uint32_t i = e & 0x3fffffffu;
+    const uint32_t ex = uint32_t(G.ex[c]), ey = uint32_t(G.ey[c]);
+    x = int(i % ex);
+    y = int((i / ex) % ey);
+    z = int(i / (ex * ey));
+}
+
+// Wave-aggregated append of k (<= 6) entries per lane to list/count.
+__device__ __forceinline__ void wave_append(uint32_t* list, uint32_t* count, const uint32_t* items,
+                                            int k) {
+    const int lane = threadIdx.x & 63;
+    int incl = k;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(count, uint32_t(total));
+    base = __shfl(base, 63, 64);
+    const uint32_t start = base + uint32_t(incl - k);
+    for (int q = 0; q < k; ++q) list[start + q] = items[q];
+}
+
+// Layer 0: known entries (value != UNKNOWN); the frontier keeps those with an
+// unknown neighbour.
+__global__ void __launch_bounds__(kBlock) k_extrap_init(FaceGrids G, int32_t* layer, uint32_t* list,
+                                                        uint32_t* count, uint32_t total) {
+    const uint32_t gi = xcd_block() * kBlock + threadIdx.x;
+    uint32_t item[1];
+    int k = 0;
+    if (gi < total) {
+        const int c = gi < G.off[1] ? 0 : (gi < G.off[2] ? 1 : 2);
+        const uint32_t i = gi - G.off[c];
+        const int ex = G.ex[c], ey = G.ey[c], ez = G.ez[c];
+        const int x = int(i % uint32_t(ex)), y = int((i / uint32_t(ex)) % uint32_t(ey)),
+                  z = int(i / (uint32_t(ex) * uint32_t(ey)));
+        const float* f = G.f[c];
+        const bool known = f[i] != MGF_UNKNOWN;
+        layer[gi] = known ? 0 : kLayerUnknown;
+        if (known) {
+            const bool edge = f[lin3(x - 1, y, z, ex, ey, ez)] == MGF_UNKNOWN ||
+                              f[lin3(x + 1, y, z, ex, ey, ez)] == MGF_UNKNOWN ||
+                              f[lin3(x, y - 1, z, ex, ey, ez)] == MGF_UNKNOWN ||
+                              f[lin3(x, y + 1, z, ex, ey, ez)] == MGF_UNKNOWN ||
+                              f[lin3(x, y, z - 1, ex, ey, ez)] == MGF_UNKNOWN ||
+                              f[lin3(x, y, z + 1, ex, ey, ez)] == MGF_UNKNOWN;
+            if (edge) item[k++] = (uint32_t(c) << 30) | i;
+        }
+    }
+    wave_append(list, count, item, k);
+}
+
+// Layer L: every unknown neighbour of a layer L-1 entry joins layer L (the
+// claiming lane is the one whose compare-and-swap succeeds) and takes the mean
+// of its neighbours in layers < L, summed in the order -x,+x,-y,+y,-z,+z.
+__global__ void __launch_bounds__(kBlock) k_extrap_layer(FaceGrids G, int L, int32_t* layer,
+                                                         const uint32_t* in_list,
+                                                         const uint32_t* in_count,
+                                                         uint32_t* out_list, uint32_t* out_count) {
+    const uint32_t nin = *in_count;
+    const uint32_t stride = gridDim.x * kBlock;
+    // every lane of a wave runs the same number of rounds (wave_append is collective)
+    const uint32_t rounds = (nin + stride - 1) / stride;
+    for (uint32_t rd = 0; rd < rounds; ++rd) {
+        const uint32_t t = rd * stride + xcd_block() * kBlock + threadIdx.x;
+        uint32_t items[6];
+        int k = 0;
+        if (t < nin) {
+            int c, x, y, z;
+            decode(G, in_list[t], c, x, y, z);
+            const int ex = G.ex[c], ey = G.ey[c], ez = G.ez[c];
+            int32_t* lay = layer + G.off[c];
+            float* f = G.f[c];
+            const int nb[6][3] = {{x - 1, y, z}, {x + 1, y, z}, {x, y - 1, z},
+                                  {x, y + 1, z}, {x, y, z - 1}, {x, y, z + 1}};
+            for (int q = 0; q < 6; ++q) {
+                const uint32_t j = lin3(nb[q][0], nb[q][1], nb[q][2], ex, ey, ez);
+                if (lay[j] != kLayerUnknown) continue;
+                if (atomicCAS(&lay[j], kLayerUnknown, L) != kLayerUnknown) continue;
+                const int jx = int(j % uint32_t(ex)), jy = int((j / uint32_t(ex)) % uint32_t(ey)),
+                          jz = int(j / (uint32_t(ex) * uint32_t(ey)));
+                const uint32_t s[6] = {lin3(jx - 1, jy, jz, ex, ey, ez), lin3(jx + 1, jy, jz, ex, ey, ez),
+                                       lin3(jx, jy - 1, jz, ex, ey, ez), lin3(jx, jy + 1, jz, ex, ey, ez),
+                                       lin3(jx, jy, jz - 1, ex, ey, ez), lin3(jx, jy, jz + 1, ex, ey, ez)};
+                float acc = 0.0f, cnt = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) {
+                    const int32_t ls = __hip_atomic_load(&lay[s[e]], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                    if (ls < L) {
+                        acc += f[s[e]];
+                        cnt += 1.0f;
+                    }
+                }
+                if (cnt != 0.0f) acc /= cnt;
+                f[j] = acc;
+                items[k++] = (uint32_t(c) << 30) | j;
+            }
+        }
+        wave_append(out_list, out_count, items, k);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundaries (MPhysics/Boundary.cpp:115-150, MPhysics/BoundaryTool.cu:3-113).
+__global__ void __launch_bounds__(kBlock) k_resample_solid(GridDev g, const float* __restrict__ sdf0,
+                                                           float tx, float ty, float tz,
+                                                           float* __restrict__ out, uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
+    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    const float cx = g.ox + 0.5f * g.hx + g.hx * float(x);
+    const float cy = g.oy + 0.5f * g.hy + g.hy * float(y);
+    const float cz = g.oz + 0.5f * g.hz + g.hz * float(z);
+    out[i] = trilerp(sdf0, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, cx + tx, cy + ty,
+                     cz + tz);
+}
+
+__global__ void __launch_bounds__(kBlock) k_solids_marker(size_t n, const float* __restrict__ sdf,
+                                                          int index1, float* __restrict__ marker) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float mk = marker[i];
+    if (index1 == 1) mk = MGF_UNKNOWN;
+    if (sdf[i] < 0.0f && mk == MGF_UNKNOWN) mk = float(-index1);
+    marker[i] = mk;
+}
+
+__global__ void __launch_bounds__(kBlock) k_solid_mask(size_t n, const float* __restrict__ total,
+                                                       uint8_t* __restrict__ mask) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) mask[i] = total[i] < 0.0f ? 1 : 0;
+}
+
+struct Vel3Table {
+    float v[3 * 64];
+};
+
+// Solid face velocity: a face takes its higher-index adjacent solid cell's
+// velocity, else its lower-index one's, else 0 (the order in which the
+// reference's racing writes land when cells are visited by index).
+__global__ void __launch_bounds__(kBlock) k_solids_vel(GridDev g, const float* __restrict__ total,
+                                                       Vel3Table tab, int nsolids,
+                                                       float* __restrict__ us,
+                                                       float* __restrict__ vs,
+                                                       float* __restrict__ ws, uint32_t nfx,
+                                                       uint32_t nfy, uint32_t nfz) {
+    uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    int c;
+    if (i < nfx) c = 0;
+    else if (i < nfx + nfy) { c = 1; i -= nfx; }
+    else if (i < nfx + nfy + nfz) { c = 2; i -= nfx + nfy; }
+    else return;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const uint32_t ex = uint32_t(nx + (c == 0)), ey = uint32_t(ny + (c == 1));
+    const int x = int(i % ex), y = int((i / ex) % ey), z = int(i / (ex * ey));
+    int cell[2][3] = {{x, y, z}, {x, y, z}};  // [0] upper (same index), [1] lower
+    cell[1][c] -= 1;
+    const int up_lim = (c == 0 ? nx : (c == 1 ? ny : nz));
+    float val = 0.0f;
+    bool set = false;
+    for (int q = 0; q < 2 && !set; ++q) {
+        const int a = cell[q][c];
+        if (a < 0 || a >= up_lim) continue;
+        const float mk = total[(uint32_t(cell[q][2]) * uint32_t(ny) + uint32_t(cell[q][1])) * uint32_t(nx) +
+                               uint32_t(cell[q][0])];
+        if (!(mk < 0.0f)) continue;
+        const int s = int(-(1.0f + mk));
+        if (s >= nsolids || s < 0) continue;
+        val = tab.v[3 * s + c];
+        set = true;
+    }
+    float* dst = c == 0 ? us : (c == 1 ? vs : ws);
+    dst[i] = val;
+}
+
+// generateFluidDomainFromBBox (MPhysics/EulerSolverTool.cu:581-608).
+__global__ void __launch_bounds__(kBlock) k_fluid_box(GridDev g, float ax, float ay, float az,
+                                                      float bx, float by, float bz,
+                                                      float* __restrict__ phi, uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
+    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    const float cx = g.ox + 0.5f * g.hx + g.hx * float(x);
+    const float cy = g.oy + 0.5f * g.hy + g.hy * float(y);
+    const float cz = g.oz + 0.5f * g.hz + g.hz * float(z);
+    const bool in = cx > ax && cy > ay && cz > az && cx < bx && cy < by && cz < bz;
+    phi[i] = in ? -500.0f : 500.0f;
+}
+
+// ===========================================================================
+// Launchers.
+RkCoef rk_coef(float dt) {
+    RkCoef k;
+    k.c[0] = -dt;
+    k.c[1] = -0.5f * dt;
+    k.c[2] = -0.75f * dt;
+    k.c[3] = -float(2.0 / 9.0) * dt;
+    k.c[4] = -float(3.0 / 9.0) * dt;
+    k.c[5] = -float(4.0 / 9.0) * dt;
+    return k;
+}
+
+template <class K, class... A>
+static hipError_t launch_k(K kern, uint32_t grid, hipStream_t s, A... args) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, args...);
+    return hipGetLastError();
+}
+#define LAUNCH(kern, grid, ...) launch_k(kern, uint32_t(grid), s, __VA_ARGS__)
+
+hipError_t launch_backtrace(hipStream_t s, const GridDev& g, Vel V, float dt, int order, float* px,
+                            float* py, float* pz, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    return LAUNCH(k_backtrace, grid_for(size_t(n)), g, V, rk_coef(dt), order, px, py, pz, n);
+}
+
+hipError_t launch_sample_cc(hipStream_t s, const GridDev& g, const float* f, const float* px,
+                            const float* py, const float* pz, float* out, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    return LAUNCH(k_sample_cc, grid_for(size_t(n)), g, f, px, py, pz, out, n);
+}
+
+hipError_t launch_advect_fc(hipStream_t s, const GridDev& g, Vel in, Vel V, float dt, int order,
+                            float* ou, float* ov, float* ow, int add0, const float inc0[3], int add1,
+                            const float inc1[3]) {
+    const uint32_t nfx = uint32_t(g.nx + 1) * g.ny * g.nz, nfy = uint32_t(g.nx) * (g.ny + 1) * g.nz,
+                   nfz = uint32_t(g.nx) * g.ny * (g.nz + 1);
+    Inc inc{};
+    for (int q = 0; q < 3; ++q) {
+        inc.a[q] = inc0 ? inc0[q] : 0.0f;
+        inc.b[q] = inc1 ? inc1[q] : 0.0f;
+    }
+    inc.add_a = add0;
+    inc.add_b = add1;
+    return LAUNCH(k_advect_fc, grid_for(size_t(nfx) + nfy + nfz), g, in, V, rk_coef(dt), order, ou, ov,
+                  ow, nfx, nfy, nfz, inc);
+}
+
+hipError_t launch_advect_cc(hipStream_t s, const GridDev& g, const float* in, Vel V, float dt,
+                            int order, float* out) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_advect_cc, grid_for(n), g, in, V, rk_coef(dt), order, out, n);
+}
+
+hipError_t launch_add_uniform(hipStream_t s, float* a, size_t n, float inc) {
+    return LAUNCH(k_add_uniform, grid_for(n), a, n, inc);
+}
+
+hipError_t launch_markers(hipStream_t s, const GridDev& g, const uint8_t* solid_mask,
+                          const float* fluid, uint8_t* markers) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_markers, grid_for(n), n, solid_mask, fluid, markers);
+}
+
+hipError_t launch_rhs(hipStream_t s, const GridDev& g, Vel U, Vel S, const uint8_t* markers,
+                      float* b) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_rhs, grid_for(n), g, U, S, markers, 1.0f / g.hx, 1.0f / g.hy, 1.0f / g.hz, b, n);
+}
+
+hipError_t launch_matrix_a(hipStream_t s, const GridDev& g, const float scale[3],
+                           const uint8_t* markers, float* A) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_matrix_a, grid_for(n), g, markers, scale[0], scale[1], scale[2], A, n);
+}
+
+hipError_t launch_divergence(hipStream_t s, const GridDev& g, Vel U, float* div) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_divergence, grid_for(n), g, U, div, n);
+}
+
+hipError_t launch_matvec(hipStream_t s, const GridDev& g, const float scale[3],
+                         const uint8_t* markers, const float* x, float* y) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_matvec, grid_for(n), g, markers, scale[0], scale[1], scale[2], x, y, n);
+}
+
+hipError_t launch_dot_partials(hipStream_t s, const float* a, const float* b, size_t n,
+                               double* partials, int blocks) {
+    return LAUNCH(k_dot_partials, blocks, a, b, n, partials);
+}
+
+hipError_t launch_axpy(hipStream_t s, float alpha, const float* x, float* y, size_t n) {
+    return LAUNCH(k_axpy, grid_for(n), alpha, x, y, n);
+}
+
+hipError_t launch_scal_add(hipStream_t s, float beta, float* p, const float* z, size_t n) {
+    return LAUNCH(k_scal_add, grid_for(n), beta, p, z, n);
+}
+
+hipError_t launch_fill(hipStream_t s, float* a, size_t n, float v) {
+    if (n == 0) return hipSuccess;
+    return LAUNCH(k_fill, grid_for(n), a, n, v);
+}
+
+hipError_t launch_fill64(hipStream_t s, double* a, size_t n, double v) {
+    if (n == 0) return hipSuccess;
+    return LAUNCH(k_fill64, grid_for(n), a, n, v);
+}
+
+hipError_t launch_u8_to_f32(hipStream_t s, const uint8_t* a, float* b, size_t n) {
+    return LAUNCH(k_u8_to_f32, grid_for(n), a, b, n);
+}
+
+hipError_t launch_cg_init(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* p,
+                          float* x, CgScalars* sc, ReduceBuf red, double rel_inner,
+                          double abs_target2, int first_sweep) {
+    (void)p;
+    (void)first_sweep;
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_cg_init, red.blocks, n, rsrc, r, x, sc, red.partials, red.ticket, rel_inner,
+                  abs_target2);
+}
+
+hipError_t launch_cg_p_ap(hipStream_t s, const GridDev& g, const float scale[3],
+                          const uint8_t* markers, const float* r, const float* p_old, float* p_new,
+                          CgScalars* sc, ReduceBuf red) {
+    return LAUNCH(k_cg_p_ap, red.blocks, g, markers, scale[0], scale[1], scale[2], r, p_old, p_new,
+                  sc, red.partials, red.ticket);
+}
+
+hipError_t launch_cg_update(hipStream_t s, const GridDev& g, const float scale[3],
+                            const uint8_t* markers, const float* p, float* x, float* r,
+                            CgScalars* sc, ReduceBuf red) {
+    return LAUNCH(k_cg_update, red.blocks, g, markers, scale[0], scale[1], scale[2], p, x, r, sc,
+                  red.partials, red.ticket);
+}
+
+hipError_t launch_refine(hipStream_t s, const GridDev& g, const float scale[3],
+                         const uint8_t* markers, const float* xcorr, double* x64, const float* b,
+                         float* r, double* partials, int blocks, int accumulate) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    if (xcorr) {
+        hipError_t e = LAUNCH(k_accumulate, grid_for(n), n, xcorr, x64, accumulate);
+        if (e != hipSuccess) return e;
+    }
+    return LAUNCH(k_residual64, blocks, g, markers, scale[0], scale[1], scale[2], x64, b, r,
+                  partials);
+}
+
+hipError_t launch_true_residual32(hipStream_t s, const GridDev& g, const float scale[3],
+                                  const uint8_t* markers, const float* x, const float* b,
+                                  double* partials, int blocks) {
+    return LAUNCH(k_residual32, blocks, g, markers, scale[0], scale[1], scale[2], x, b, partials);
+}
+
+hipError_t launch_project(hipStream_t s, const GridDev& g, const float sg[3], const uint8_t* markers,
+                          const float* p32, const double* p64, float* u, float* v, float* w,
+                          const float* us, const float* vs, const float* ws) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_project, grid_for(n), g, markers, sg[0], sg[1], sg[2], p32, p64, u, v, w, us, vs,
+                  ws, n);
+}
+
+static FaceGrids face_grids(const GridDev& g, float* u, float* v, float* w) {
+    FaceGrids G;
+    G.f[0] = u; G.f[1] = v; G.f[2] = w;
+    for (int c = 0; c < 3; ++c) {
+        G.ex[c] = g.nx + (c == 0);
+        G.ey[c] = g.ny + (c == 1);
+        G.ez[c] = g.nz + (c == 2);
+    }
+    G.off[0] = 0;
+    G.off[1] = uint32_t(G.ex[0]) * G.ey[0] * G.ez[0];
+    G.off[2] = G.off[1] + uint32_t(G.ex[1]) * G.ey[1] * G.ez[1];
+    return G;
+}
+
+hipError_t launch_extrap_init(hipStream_t s, const GridDev& g, const float* u, const float* v,
+                              const float* w, int32_t* layer, uint32_t* list, uint32_t* count) {
+    FaceGrids G = face_grids(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
+    const uint32_t total = G.off[2] + uint32_t(g.nx) * g.ny * (g.nz + 1);
+    return LAUNCH(k_extrap_init, grid_for(total), G, layer, list, count, total);
+}
+
+hipError_t launch_extrap_layer(hipStream_t s, const GridDev& g, int layer_id, float* u, float* v,
+                               float* w, int32_t* layer, const uint32_t* in_list,
+                               const uint32_t* in_count, uint32_t* out_list, uint32_t* out_count) {
+    FaceGrids G = face_grids(g, u, v, w);
+    const uint32_t total = G.off[2] + uint32_t(g.nx) * g.ny * (g.nz + 1);
+    uint32_t blocks = grid_for(total);
+    if (blocks > 2048) blocks = 2048;
+    return LAUNCH(k_extrap_layer, blocks, G, layer_id, layer, in_list, in_count, out_list, out_count);
+}
+
+hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0, const float t[3],
+                                 float* out) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_resample_solid, grid_for(n), g, sdf0, t[0], t[1], t[2], out, n);
+}
+
+hipError_t launch_solids_marker(hipStream_t s, size_t n, const float* sdf, int index1, float* marker) {
+    return LAUNCH(k_solids_marker, grid_for(n), n, sdf, index1, marker);
+}
+
+hipError_t launch_solid_mask(hipStream_t s, size_t n, const float* total, uint8_t* mask) {
+    return LAUNCH(k_solid_mask, grid_for(n), n, total, mask);
+}
+
+hipError_t launch_solids_vel(hipStream_t s, const GridDev& g, const float* total, const float* vel3,
+                             int nsolids, float* us, float* vs, float* ws) {
+    Vel3Table tab{};
+    const int m = nsolids < 64 ? nsolids : 64;
+    for (int q = 0; q < 3 * m; ++q) tab.v[q] = vel3[q];
+    const uint32_t nfx = uint32_t(g.nx + 1) * g.ny * g.nz, nfy = uint32_t(g.nx) * (g.ny + 1) * g.nz,
+                   nfz = uint32_t(g.nx) * g.ny * (g.nz + 1);
+    return LAUNCH(k_solids_vel, grid_for(size_t(nfx) + nfy + nfz), g, total, tab, m, us, vs, ws, nfx,
+                  nfy, nfz);
+}
+
+hipError_t launch_fluid_box(hipStream_t s, const GridDev& g, const float mn[3], const float mx[3],
+                            float* phi) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    return LAUNCH(k_fluid_box, grid_for(n), g, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], phi, n);
+}
+
+}  // namespace mgf

@@ -1,0 +1,315 @@
+"""ctypes binding of libmgf.so (include/mgf.h) for Python callers.
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  This
+module only marshals arguments; it raises if the library is missing or a call
+fails, and never computes anything itself.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libmgf.so")
+HEADER = os.path.join(os.path.dirname(PKG), "include", "mgf.h")
+
+OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_UNSUPPORTED = range(7)
+RK1, RK2, RK3 = 1, 2, 3
+TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
+PRECISION_PLAIN, PRECISION_MIXED = 0, 1
+PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
+FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
+             solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
+             divergence=12)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("origin", C.c_float * 3), ("spacing", C.c_float * 3), ("dt", C.c_float),
+                ("density", C.c_float), ("gravity", C.c_float * 3),
+                ("external_force", C.c_float * 3), ("rk_order", C.c_int32),
+                ("tol_mode", C.c_int32), ("tolerance", C.c_double),
+                ("max_iterations", C.c_int64), ("precision", C.c_int32),
+                ("preconditioner", C.c_int32), ("extrapolation_layers", C.c_int32),
+                ("max_refinements", C.c_int32), ("inner_tolerance", C.c_double)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("cg_iterations", C.c_int64), ("refinement_sweeps", C.c_int32),
+                ("converged", C.c_int32), ("residual_rel", C.c_double),
+                ("residual_recurrence", C.c_double), ("b_norm", C.c_double),
+                ("extrapolation_layers", C.c_int32), ("steps", C.c_int32),
+                ("ms_advect", C.c_double), ("ms_pressure", C.c_double),
+                ("ms_extrapolate", C.c_double), ("ms_total", C.c_double),
+                ("fluid_cells", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("origin", C.c_float * 3), ("spacing", C.c_float * 3)]
+
+
+class MgfError(RuntimeError):
+    def __init__(self, status, where, msg):
+        super().__init__("%s failed: %s (%s)" % (where, msg, status))
+        self.status = status
+
+
+_lib = None
+_F = C.POINTER(C.c_float)
+_VP = C.c_void_p
+
+# name -> argtypes (all return mgf_status = int32 unless listed in _RESTYPE)
+_SIGS = {
+    "mgf_init": [C.c_int32, C.POINTER(C.c_void_p)],
+    "mgf_shutdown": [_VP],
+    "mgf_status_string": [C.c_int32],
+    "mgf_last_error": [],
+    "mgf_scene_desc_defaults": [C.POINTER(SceneDesc), C.c_int32, C.c_int32, C.c_int32, _F, _F,
+                                C.c_float],
+    "mgf_scene_create": [_VP, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)],
+    "mgf_scene_destroy": [_VP],
+    "mgf_scene_get_desc": [_VP, C.POINTER(SceneDesc)],
+    "mgf_scene_set_params": [_VP, C.POINTER(SceneDesc)],
+    "mgf_scene_device_bytes": [_VP, C.POINTER(C.c_int64)],
+    "mgf_scene_add_solid": [_VP, _F, C.c_int64, _F],
+    "mgf_scene_fluid_box": [_VP, _F, _F],
+    "mgf_scene_upload": [_VP, C.c_int32, _F, C.c_int64],
+    "mgf_scene_download": [_VP, C.c_int32, _F, C.c_int64],
+    "mgf_scene_field_count": [_VP, C.c_int32, C.POINTER(C.c_int64)],
+    "mgf_scene_update_boundaries": [_VP],
+    "mgf_scene_advect": [_VP],
+    "mgf_scene_apply_forces": [_VP],
+    "mgf_scene_pressure_solve": [_VP, C.POINTER(StepStats)],
+    "mgf_scene_extrapolate_velocity": [_VP, C.POINTER(C.c_int32)],
+    "mgf_step": [_VP, C.c_int32, C.POINTER(StepStats)],
+    "mgf_scene_sync": [_VP],
+    "mgf_scene_set_timing": [_VP, C.c_int32],
+    "mgf_scene_time_cg_kernels": [_VP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "mgf_device_alloc": [_VP, C.c_int64, C.POINTER(C.c_void_p)],
+    "mgf_device_free": [_VP, _VP],
+    "mgf_memcpy_h2d": [_VP, _VP, _VP, C.c_int64],
+    "mgf_memcpy_d2h": [_VP, _VP, _VP, C.c_int64],
+    "mgf_op_backtrace": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_float, C.c_int32, _VP, _VP, _VP,
+                         C.c_int64],
+    "mgf_op_sample_cc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _VP, _VP, C.c_int64],
+    "mgf_op_advect_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_int32,
+                         _VP, _VP, _VP],
+    "mgf_op_advect_cc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _VP, C.c_float, C.c_int32, _VP],
+    "mgf_op_pressure_matvec": [_VP, C.POINTER(Grid), _F, _VP, _VP, _VP],
+    "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
+}
+_RESTYPE = {"mgf_status_string": C.c_char_p, "mgf_last_error": C.c_char_p}
+
+
+def exported_names():
+    """Function names declared with MGF_API in include/mgf.h."""
+    import re
+    names = []
+    with open(HEADER) as f:
+        for line in f:
+            m = re.match(r"MGF_API\s+[\w\s\*]+?\b(mgf_\w+)\s*\(", line)
+            if m:
+                names.append(m.group(1))
+    return names
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmgf.so is not built (%s): run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPE.get(name, C.c_int32)
+        _lib = L
+    return _lib
+
+
+def check(status, where):
+    if status != OK:
+        msg = lib().mgf_last_error().decode(errors="replace")
+        raise MgfError(status, where, msg)
+    return status
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def fp(a):
+    return a.ctypes.data_as(_F)
+
+
+def farr3(v):
+    return (C.c_float * 3)(*[float(x) for x in v])
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().mgf_init(device, C.byref(h)), "mgf_init")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().mgf_shutdown(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # device buffers for the operator entry points
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        check(lib().mgf_device_alloc(self.h, int(nbytes), C.byref(p)), "mgf_device_alloc")
+        return p
+
+    def free(self, p):
+        check(lib().mgf_device_free(self.h, p), "mgf_device_free")
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        check(lib().mgf_memcpy_h2d(self.h, p, a.ctypes.data_as(C.c_void_p), a.nbytes), "h2d")
+        return p
+
+    def to_host(self, p, n, dtype=np.float32):
+        out = np.empty(n, dtype)
+        check(lib().mgf_memcpy_d2h(self.h, out.ctypes.data_as(C.c_void_p), p, out.nbytes), "d2h")
+        return out
+
+
+def desc_defaults(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0):
+    d = SceneDesc()
+    o, s = f32(origin), f32(spacing)
+    check(lib().mgf_scene_desc_defaults(C.byref(d), int(res[0]), int(res[1]), int(res[2]), fp(o), fp(s),
+                                        C.c_float(dt)), "mgf_scene_desc_defaults")
+    return d
+
+
+def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode=TOL_ABS_SQRT_NORM,
+              tol=1e-3, max_it=0, precision=PRECISION_PLAIN, layers=1000, gravity=(0, -9.8, 0),
+              force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3):
+    d = desc_defaults(res, origin, spacing, dt)
+    d.rk_order = rk
+    d.tol_mode = tol_mode
+    d.tolerance = tol
+    d.max_iterations = max_it
+    d.precision = precision
+    d.extrapolation_layers = layers
+    d.gravity = farr3(gravity)
+    d.external_force = farr3(force)
+    d.density = density
+    d.max_refinements = max_refinements
+    d.inner_tolerance = inner_tol
+    return d
+
+
+class Scene:
+    """CGridFluidSolver on one MI355X: fields resident in HBM."""
+
+    def __init__(self, ctx, desc):
+        self.ctx = ctx
+        self.desc = desc
+        h = C.c_void_p()
+        check(lib().mgf_scene_create(ctx.h, C.byref(desc), C.byref(h)), "mgf_scene_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgf_scene_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def count(self, field):
+        n = C.c_int64()
+        check(lib().mgf_scene_field_count(self.h, field, C.byref(n)), "mgf_scene_field_count")
+        return n.value
+
+    def add_solid(self, sdf, vel=(0, 0, 0)):
+        sdf = f32(sdf).ravel()
+        check(lib().mgf_scene_add_solid(self.h, fp(sdf), sdf.size, fp(f32(vel))), "mgf_scene_add_solid")
+
+    def fluid_box(self, mn, mx):
+        check(lib().mgf_scene_fluid_box(self.h, fp(f32(mn)), fp(f32(mx))), "mgf_scene_fluid_box")
+
+    def upload(self, field, data):
+        data = f32(data).ravel()
+        check(lib().mgf_scene_upload(self.h, field, fp(data), data.size), "mgf_scene_upload")
+
+    def download(self, field):
+        out = np.empty(self.count(field), np.float32)
+        check(lib().mgf_scene_download(self.h, field, fp(out), out.size), "mgf_scene_download")
+        return out
+
+    def set_params(self, desc):
+        check(lib().mgf_scene_set_params(self.h, C.byref(desc)), "mgf_scene_set_params")
+        self.desc = desc
+
+    def device_bytes(self):
+        n = C.c_int64()
+        check(lib().mgf_scene_device_bytes(self.h, C.byref(n)), "mgf_scene_device_bytes")
+        return n.value
+
+    def update_boundaries(self):
+        check(lib().mgf_scene_update_boundaries(self.h), "mgf_scene_update_boundaries")
+
+    def advect(self):
+        check(lib().mgf_scene_advect(self.h), "mgf_scene_advect")
+
+    def apply_forces(self):
+        check(lib().mgf_scene_apply_forces(self.h), "mgf_scene_apply_forces")
+
+    def pressure_solve(self, allow_unconverged=False):
+        st = StepStats()
+        rc = lib().mgf_scene_pressure_solve(self.h, C.byref(st))
+        if not (allow_unconverged and rc == ERR_NOT_CONVERGED):
+            check(rc, "mgf_scene_pressure_solve")
+        return st.as_dict()
+
+    def extrapolate(self):
+        n = C.c_int32()
+        check(lib().mgf_scene_extrapolate_velocity(self.h, C.byref(n)), "mgf_scene_extrapolate_velocity")
+        return n.value
+
+    def step(self, n=1, allow_unconverged=False):
+        st = StepStats()
+        rc = lib().mgf_step(self.h, n, C.byref(st))
+        if not (allow_unconverged and rc == ERR_NOT_CONVERGED):
+            check(rc, "mgf_step")
+        return st.as_dict()
+
+    def sync(self):
+        check(lib().mgf_scene_sync(self.h), "mgf_scene_sync")
+
+    def time_cg_kernels(self, iters=50):
+        a, b = C.c_double(), C.c_double()
+        check(lib().mgf_scene_time_cg_kernels(self.h, iters, C.byref(a), C.byref(b)),
+              "mgf_scene_time_cg_kernels")
+        return a.value, b.value
+
+    def set_timing(self, on):
+        check(lib().mgf_scene_set_timing(self.h, 1 if on else 0), "mgf_scene_set_timing")
+
+
+def grid(res, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    g = Grid()
+    g.nx, g.ny, g.nz = [int(r) for r in res]
+    g.origin = farr3(origin)
+    g.spacing = farr3(spacing)
+    return g

@@ -206,6 +206,15 @@ MGF_API mgf_status mgf_scene_sync(mgf_scene scene);
 /* Host wall-clock timing of stages in mgf_step (adds synchronisation). */
 MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
 
+/* Measurement hook (no counterpart in the reference): runs `iters` fused CG
+ * iterations of the MIXED solver on the scene's last pressure system (markers
+ * and b of the last solve) with no stopping test, each kernel bracketed by HIP
+ * events on the scene's stream, and returns the mean duration in ms of the
+ * p/Ap kernel and of the x/r update kernel.  The pressure and CG work vectors
+ * are overwritten; call it after the steps being measured. */
+MGF_API mgf_status mgf_scene_time_cg_kernels(mgf_scene scene, int32_t iters,
+        double* ms_p_ap, double* ms_update);
+
 /* ---- operator entry points on caller device memory ---------------------- */
 /* These run one reference operator on device buffers the caller owns (for
  * example buffers from mgf_device_alloc).  All pointers are device pointers.
