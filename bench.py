@@ -21,18 +21,34 @@ sys.path.insert(0, os.path.join(ROOT, "hybrid-fluid-simulation_amd", "python"))
 
 METRIC = "cell-steps/s at 256³ (div resid ≤1e-6) vs plain FP32; p50/p99 batched step latency"
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic bytes per cell of the two fused CG kernels (FP32 = 4 B, uint8
-# marker = 1 B): p/Ap reads r, p_old, marker and writes p_new; the update
-# reads x, p, r, marker and writes x, r.
-BYTES_P_AP = 13
-BYTES_UPDATE = 21
+# Algorithmic bytes per padded cell of the fused CG kernels (FP32 = 4 B,
+# uint16 stencil code = 2 B): p/Ap reads z, p_old, code and writes p_new; the
+# update reads x, p, r, code and writes x, r.
+BYTES_P_AP = 14
+BYTES_UPDATE = 22
+PRECONDS = {"identity": 0, "jacobi": 1, "multigrid": 2}
+
+
+def precond_bytes(name, sweeps=2):
+    """Algorithmic bytes per padded cell of one preconditioner application.
+    Jacobi: read r, code; write z.  V-cycle per level: first sweep (b, code ->
+    x: 10), later sweeps (b, x, code -> x': 14), restriction (b, x, code of
+    the fine level + 1/8 of a coarse b and code: 10.75), prolongation (code,
+    x read and written, 1/8 of a coarse x: 10.5); the levels shrink by 8, so
+    the hierarchy costs 8/7 of level 0."""
+    if name == "identity":
+        return 0.0
+    if name == "jacobi":
+        return 10.0
+    lvl0 = 10 + 14 * (sweeps - 1) + 10.75 + 10.5 + 14 * sweeps
+    return lvl0 * 8.0 / 7.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_steps(scene, steps, tol):
+def run_steps(scene, steps):
     times, iters, sweeps, res, conv = [], [], [], [], 0
     for _ in range(steps):
         t0 = time.perf_counter()
@@ -103,6 +119,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cg-timing-iters", type=int, default=40)
+    ap.add_argument("--precond", choices=sorted(PRECONDS), default="multigrid")
+    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--mg-sweeps", type=int, default=2)
+    ap.add_argument("--mg-omega", type=float, default=0.8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs the MAC-grid step on a single GPU "
@@ -118,31 +138,35 @@ def main():
     with mgf.Context(0) as ctx:
         log("scene %s: %d^3 interior, %d padded cells, dt %.3g" % (spec.name, spec.n, ncell_padded, spec.dt))
         s = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=tol,
-                                   precision=mgf.PRECISION_MIXED)
-        run_steps(s, args.warmup, tol)
+                                   precision=mgf.PRECISION_MIXED, precond=PRECONDS[args.precond],
+                                   inner_tol=args.inner_tol, mg_sweeps=args.mg_sweeps, mg_omega=args.mg_omega)
+        run_steps(s, args.warmup)
         s.sync()
         t0 = time.perf_counter()
-        r = run_steps(s, args.steps, tol)
+        r = run_steps(s, args.steps)
         s.sync()
         wall = time.perf_counter() - t0
         log("mixed: %d steps in %.3fs, %d converged, CG iters/step %.1f, sweeps/step %.2f, max resid %.2e"
             % (args.steps, wall, r["converged"], r["iters"].mean(), r["sweeps"].mean(), r["res"].max()))
         value = spec.interior_cells * r["converged"] / wall
         # live kernel timing (HIP events on the scene's stream)
-        ms_pap, ms_upd = s.time_cg_kernels(args.cg_timing_iters)
+        ms_pap, ms_upd, ms_pc = s.time_cg_kernels(args.cg_timing_iters)
         gbs_pap = BYTES_P_AP * ncell_padded / (ms_pap * 1e-3) / 1e9
         gbs_upd = BYTES_UPDATE * ncell_padded / (ms_upd * 1e-3) / 1e9
-        log("cg kernels: p/Ap %.4f ms (%.0f GB/s), update %.4f ms (%.0f GB/s)" % (ms_pap, gbs_pap, ms_upd, gbs_upd))
+        pcb = precond_bytes(args.precond, args.mg_sweeps)
+        gbs_pc = pcb * ncell_padded / (ms_pc * 1e-3) / 1e9 if ms_pc > 0 else None
+        log("cg kernels: p/Ap %.4f ms (%.0f GB/s), update %.4f ms (%.0f GB/s), preconditioner %.4f ms (%s GB/s)"
+            % (ms_pap, gbs_pap, ms_upd, gbs_upd, ms_pc, "%.0f" % gbs_pc if gbs_pc else "-"))
         s.close()
 
         plain = None
         if not args.no_plain and args.plain_steps > 0:
             sp = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=tol,
                                         precision=mgf.PRECISION_PLAIN)
-            run_steps(sp, 1, tol)
+            run_steps(sp, 1)
             sp.sync()
             t0 = time.perf_counter()
-            rp = run_steps(sp, args.plain_steps, tol)
+            rp = run_steps(sp, args.plain_steps)
             sp.sync()
             wp = time.perf_counter() - t0
             sp.close()
@@ -162,7 +186,10 @@ def main():
     ms = np.array(r["times"]) * 1e3
     k = float(r["iters"].mean())
     rs = float(r["sweeps"].mean())
-    step_bytes = spec.interior_cells * (109 + 34 * k + 25 * rs)
+    # per interior cell: fixed stages 109 B, per CG iteration 36 B + the
+    # preconditioner, per refinement sweep 25 B (cell counts of the padded and
+    # interior grids differ by 2%; the interior count is used throughout)
+    step_bytes = spec.interior_cells * (109 + (BYTES_P_AP + BYTES_UPDATE + pcb) * k + 25 * rs)
     out = {
         "metric": METRIC,
         "value": value,
@@ -178,12 +205,13 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "obstacles %d^3 (%d^3 padded), seed %d, 3 spheres, RK1, gravity, mixed-precision "
-                        "PCG to 1e-6 relative FP64 residual, 1000-layer extrapolation" % (spec.n, spec.res[0], args.seed),
+                        "PCG (%s preconditioner) to 1e-6 relative FP64 residual, 1000-layer extrapolation"
+                        % (spec.n, spec.res[0], args.seed, args.precond),
             "grid": list(spec.res), "dt": spec.dt, "tolerance": tol,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_update (x += alpha p, r -= alpha A p, r.r)",
+            "kernel": "k_col_update (x += alpha p, r -= alpha A p, r.r)",
             "achieved": gbs_upd,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
@@ -191,7 +219,9 @@ def main():
             "traffic": None,
             "bytes_per_launch": BYTES_UPDATE * ncell_padded,
             "ms_per_launch": ms_upd,
-            "other": {"k_cg_p_ap": {"ms": ms_pap, "GBps": gbs_pap, "bytes_per_launch": BYTES_P_AP * ncell_padded}},
+            "other": {"k_col_p_ap": {"ms": ms_pap, "GBps": gbs_pap, "bytes_per_launch": BYTES_P_AP * ncell_padded},
+                      "preconditioner": {"name": args.precond, "ms": ms_pc, "GBps": gbs_pc,
+                                         "bytes_per_application": pcb * ncell_padded}},
         },
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),
         "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),

@@ -22,7 +22,6 @@ namespace {
 
 constexpr int kPlainDotBlocks = 1024;
 constexpr int kResidualBlocks = 1024;
-constexpr int kLayerChunk = 32;
 
 double now_ms() {
     using namespace std::chrono;
@@ -81,8 +80,13 @@ mgf_status validate_desc(const mgf_scene_desc* d) {
     if (!(d->tolerance > 0.0)) return fail(MGF_ERR_INVALID, "tolerance must be > 0");
     if (d->precision != MGF_PRECISION_PLAIN && d->precision != MGF_PRECISION_MIXED)
         return fail(MGF_ERR_INVALID, "unknown precision");
-    if (d->preconditioner != MGF_PRECOND_IDENTITY)
-        return fail(MGF_ERR_UNSUPPORTED, "only the identity preconditioner is built");
+    if (d->preconditioner != MGF_PRECOND_IDENTITY && d->preconditioner != MGF_PRECOND_JACOBI &&
+        d->preconditioner != MGF_PRECOND_MULTIGRID)
+        return fail(MGF_ERR_INVALID, "unknown preconditioner");
+    if (d->precision == MGF_PRECISION_PLAIN && d->preconditioner != MGF_PRECOND_IDENTITY)
+        return fail(MGF_ERR_UNSUPPORTED, "PLAIN precision runs the reference's identity-preconditioned CG only");
+    if (d->mg_sweeps > 8) return fail(MGF_ERR_INVALID, "mg_sweeps must be <= 8");
+    if (!(d->mg_omega < 2.0f) || !std::isfinite(d->mg_omega)) return fail(MGF_ERR_INVALID, "mg_omega must be < 2");
     if (d->extrapolation_layers < 0 || d->extrapolation_layers > 1000000)
         return fail(MGF_ERR_INVALID, "extrapolation_layers out of range");
     return MGF_OK;
@@ -271,15 +275,121 @@ mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     return MGF_OK;
 }
 
-// Mixed CG: FP32 vectors in two fused kernels per iteration, FP64 dots and
-// scalars on the device, FP64 pressure accumulated over refinement sweeps
-// whose residual b - A x is formed in FP64.
+// ---------------------------------------------------------------------------
+// Preconditioners of the MIXED solver.
+constexpr int kCoarsestSweeps = 8;
+
+mgf_status ensure_precond(mgf_scene s) {
+    if (!s->z) {
+        TRY(dalloc(s, &s->z, s->ncell));
+        TRY(dalloc(s, &s->zt, s->ncell));
+    }
+    if (s->desc.preconditioner != MGF_PRECOND_MULTIGRID || s->mg_nx >= 0) return MGF_OK;
+    // hierarchy: halve every axis longer than one cell until none exceeds 4
+    int nx = s->g.nx, ny = s->g.ny, nz = s->g.nz;
+    while (std::max(nx, std::max(ny, nz)) > 4 && s->mg.size() < 16) {
+        MgLevelBuf L;
+        L.d.nx = nx > 1 ? (nx + 1) / 2 : 1;
+        L.d.ny = ny > 1 ? (ny + 1) / 2 : 1;
+        L.d.nz = nz > 1 ? (nz + 1) / 2 : 1;
+        const size_t n = size_t(L.d.nx) * L.d.ny * L.d.nz;
+        TRY(dalloc(s, &L.markers, n));
+        TRY(dalloc(s, &L.codes, n));
+        TRY(dalloc(s, &L.b, n));
+        TRY(dalloc(s, &L.x, n));
+        TRY(dalloc(s, &L.t, n));
+        s->mg.push_back(L);
+        nx = L.d.nx; ny = L.d.ny; nz = L.d.nz;
+    }
+    s->mg_nx = nx; s->mg_ny = ny; s->mg_nz = nz;
+    return MGF_OK;
+}
+
+// Coarse markers and stencil codes of every level for the current markers;
+// the operator scale of an axis drops by 4 where the axis is halved.
+mgf_status mg_setup(mgf_scene s, const float sa[3]) {
+    hipStream_t st = s->ctx->stream;
+    MgDims f{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    const uint8_t* mf = s->markers;
+    for (auto& L : s->mg) {
+        L.d.sx = f.nx > 1 ? f.sx * 0.25f : f.sx;
+        L.d.sy = f.ny > 1 ? f.sy * 0.25f : f.sy;
+        L.d.sz = f.nz > 1 ? f.sz * 0.25f : f.sz;
+        // a coarse cell with any air child is air (Dirichlet), else fluid if
+        // any child is fluid, else solid
+        MGF_HIP(launch_mg_coarsen(st, f, L.d, mf, L.markers, 0));
+        MGF_HIP(launch_codes(st, L.d.nx, L.d.ny, L.d.nz, L.markers, L.codes));
+        f = L.d;
+        mf = L.markers;
+    }
+    return MGF_OK;
+}
+
+// z = M^-1 r; *zout receives the buffer that holds z.  The last level-0
+// kernel also reduces r . z and sets beta (see k_mg_smooth).
+mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
+    hipStream_t st = s->ctx->stream;
+    const auto& d = s->desc;
+    const MgDims d0{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    if (d.preconditioner == MGF_PRECOND_JACOBI) {
+        MGF_HIP(launch_mg_smooth(st, d0, s->codes, s->r, s->r, s->z, 1.0f, s->scal, s->red, 1, 1));
+        *zout = s->z;
+        return MGF_OK;
+    }
+    const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 2;
+    const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.8f;
+    const int L = 1 + int(s->mg.size());
+    struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; };
+    Lv lv[17];
+    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt};
+    for (int l = 1; l < L; ++l) {
+        auto& m = s->mg[size_t(l - 1)];
+        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t};
+    }
+    // descent: pre-smoothing from zero, restriction of the residual
+    for (int l = 0; l < L; ++l) {
+        Lv& v = lv[l];
+        const bool coarsest = l == L - 1;
+        const int sweeps = coarsest ? kCoarsestSweeps : nu;
+        for (int k = 0; k < sweeps; ++k) {
+            const int rz = (l == 0 && coarsest && k == sweeps - 1) ? 1 : 0;
+            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, k == 0 ? v.b : v.x, k == 0 ? v.x : v.t, omega,
+                                     s->scal, s->red, k == 0 ? 1 : 0, rz));
+            if (k > 0) std::swap(v.x, v.t);
+        }
+        if (!coarsest)
+            MGF_HIP(launch_mg_restrict(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
+                                       const_cast<float*>(lv[l + 1].b), s->scal));
+    }
+    // ascent: prolongation (injection) and post-smoothing
+    for (int l = L - 2; l >= 0; --l) {
+        Lv& v = lv[l];
+        MGF_HIP(launch_mg_prolong(st, v.d, lv[l + 1].d, v.codes, lv[l + 1].x, v.x, s->scal));
+        for (int k = 0; k < nu; ++k) {
+            const int rz = (l == 0 && k == nu - 1) ? 1 : 0;
+            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz));
+            std::swap(v.x, v.t);
+        }
+    }
+    *zout = lv[0].x;
+    return MGF_OK;
+}
+
+// Mixed CG: FP32 vectors in two fused column kernels per iteration (plus the
+// preconditioner), FP64 dots and scalars on the device, FP64 pressure
+// accumulated over refinement sweeps whose residual b - A x is formed in FP64.
 mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_stats* acc) {
     hipStream_t st = s->ctx->stream;
     const auto& d = s->desc;
+    const bool ident = d.preconditioner == MGF_PRECOND_IDENTITY;
     const double inner = d.inner_tolerance > 0 ? d.inner_tolerance : 1e-3;
     const int max_sweeps = d.max_refinements > 0 ? d.max_refinements : 8;
-    const int chunk = s->ncell < (1u << 20) ? 32 : 16;
+    const int chunk = d.preconditioner == MGF_PRECOND_MULTIGRID ? 4 : (s->ncell < (1u << 20) ? 32 : 16);
+    MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
+    if (!ident) {
+        TRY(ensure_precond(s));
+        TRY(mg_setup(s, sa));
+    }
     // ||b||^2 and the overall target
     MGF_HIP(launch_fill64(st, s->x64, s->ncell, 0.0));
     MGF_HIP(launch_refine(st, s->g, sa, s->markers, nullptr, s->x64, s->b, nullptr, s->red.partials,
@@ -305,14 +415,18 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     while (!conv && sweeps < max_sweeps && it_total < max_it) {
         MGF_HIP(launch_cg_init(st, s->g, rsrc, s->r, nullptr, s->x, s->scal, s->red, inner, target2,
                                sweeps == 0));
+        const float* zv = s->r;
+        if (!ident) TRY(apply_precond(s, sa, &zv));
         float* pold = s->p0;
         float* pnew = s->p1;
         for (;;) {
             const int64_t left = max_it - it_total;
             const int k = int(std::min<int64_t>(chunk, std::max<int64_t>(left, 1)));
             for (int q = 0; q < k; ++q) {
-                MGF_HIP(launch_cg_p_ap(st, s->g, sa, s->markers, s->r, pold, pnew, s->scal, s->red));
-                MGF_HIP(launch_cg_update(st, s->g, sa, s->markers, pnew, s->x, s->r, s->scal, s->red));
+                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, pold, pnew, s->scal, s->red));
+                MGF_HIP(launch_col_update(st, s->g, sa, s->codes, pnew, s->x, s->r, s->scal, s->red,
+                                          ident ? 1 : 0));
+                if (!ident) TRY(apply_precond(s, sa, &zv));
                 std::swap(pold, pnew);
             }
             MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
@@ -361,34 +475,45 @@ mgf_status pressure(mgf_scene s, mgf_step_stats* acc) {
 }
 
 // ---------------------------------------------------------------------------
-// Extrapolation by frontiers; returns the last non-empty layer.
+// Extrapolation: distances, counting sort by distance, one gather kernel per
+// layer; returns the largest non-empty layer (< layers).
+mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
+    const size_t K = size_t(ext_max_distance(s->g)) + 2;
+    TRY(dalloc(s, &s->layer, nface));
+    TRY(dalloc(s, &s->list, nface));
+    TRY(dalloc(s, &s->counts, 3 * K + 1));
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->host_offs), sizeof(uint32_t) * (K + 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        s->host_offs = nullptr;
+        return fail(MGF_ERR_NOMEM, "hipHostMalloc failed");
+    }
+    return MGF_OK;
+}
+
 mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, int* used) {
     hipStream_t st = s->ctx->stream;
     *used = 0;
-    if (layers <= 0) return MGF_OK;
-    // counts live in a ring of kLayerChunk+1 slots: slot L % (chunk+1)
-    const int ring = kLayerChunk + 1;
-    MGF_HIP(launch_fill(st, reinterpret_cast<float*>(s->counts), size_t(ring), 0.0f));
-    MGF_HIP(launch_extrap_init(st, s->g, u, v, w, s->layer, s->list[0], &s->counts[0]));
-    std::vector<uint32_t> hc(ring);
-    int L = 1;
-    bool empty = false;
-    while (L < layers && !empty) {
-        const int k = std::min(kLayerChunk, layers - L);
-        for (int q = 0; q < k; ++q, ++L) {
-            // clear the slot this layer writes
-            MGF_HIP(launch_fill(st, reinterpret_cast<float*>(&s->counts[L % ring]), 1, 0.0f));
-            MGF_HIP(launch_extrap_layer(st, s->g, L, u, v, w, s->layer, s->list[(L - 1) & 1],
-                                        &s->counts[(L - 1) % ring], s->list[L & 1],
-                                        &s->counts[L % ring]));
-        }
-        MGF_HIP(hipMemcpyAsync(hc.data(), s->counts, sizeof(uint32_t) * ring, hipMemcpyDeviceToHost, st));
-        MGF_HIP(hipStreamSynchronize(st));
-        for (int q = L - k; q < L; ++q) {
-            if (hc[q % ring] == 0) { empty = true; break; }
-            *used = q;
-        }
+    const int32_t K = ext_max_distance(s->g);
+    const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
+    if (cap <= 0) return MGF_OK;   // layer 0 only: the reference's fill loop is empty
+    uint32_t* hist = s->counts;
+    uint32_t* offs = hist + (K + 2);
+    uint32_t* cursor = offs + (K + 2);
+    uint32_t* maxd = cursor + (K + 2);
+    MGF_HIP(launch_ext_distance(st, s->g, u, v, w, s->layer));
+    MGF_HIP(launch_fill(st, reinterpret_cast<float*>(hist), size_t(cap) + 2, 0.0f));
+    MGF_HIP(launch_fill(st, reinterpret_cast<float*>(maxd), 1, 0.0f));
+    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, hist, offs, cursor, maxd, s->list));
+    // offsets of layers 1..cap+1, then the largest distance present
+    MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipStreamSynchronize(st));
+    const int32_t top = int32_t(s->host_offs[cap + 2]);
+    for (int32_t L = 1; L <= top; ++L) {
+        const uint32_t b = s->host_offs[L], e = s->host_offs[L + 1];
+        MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->layer, s->list, b, e - b, L));
     }
+    *used = top;
     return MGF_OK;
 }
 
@@ -493,6 +618,8 @@ mgf_status mgf_scene_desc_defaults(mgf_scene_desc* d, int32_t nx, int32_t ny, in
     d->extrapolation_layers = 1000;
     d->max_refinements = 8;
     d->inner_tolerance = 1e-3;
+    d->mg_sweeps = 2;
+    d->mg_omega = 0.8f;
     return MGF_OK;
 }
 
@@ -525,11 +652,11 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
     A(&s->markers, s->ncell);
     A(&s->b, s->ncell); A(&s->x, s->ncell); A(&s->x64, s->ncell);
     A(&s->r, s->ncell); A(&s->p0, s->ncell); A(&s->p1, s->ncell); A(&s->ap, s->ncell);
+    A(&s->codes, s->ncell);
     A(&s->scal, 1);
-    A(&s->red.partials, size_t(kMaxBlocks) * 4);
+    A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 4);
     A(&s->red.ticket, 4);
-    A(&s->layer, nface); A(&s->list[0], nface); A(&s->list[1], nface);
-    A(&s->counts, kLayerChunk + 1);
+    if (st == MGF_OK) st = alloc_extrapolation(s, nface);
     if (st != MGF_OK) {
         mgf_scene_destroy(s);
         return st;
@@ -576,6 +703,7 @@ mgf_status mgf_scene_destroy(mgf_scene s) {
     for (void* p : s->allocations) (void)hipFree(p);
     for (auto& so : s->solids) (void)hipFree(so.sdf0);
     if (s->host_scratch) (void)hipHostFree(s->host_scratch);
+    if (s->host_offs) (void)hipHostFree(s->host_offs);
     delete s;
     return MGF_OK;
 }
@@ -745,7 +873,8 @@ mgf_status mgf_scene_pressure_solve(mgf_scene s, mgf_step_stats* stats) {
     TRY(pressure(s, &acc));
     TRY(sync(s));
     if (stats) *stats = acc;
-    return acc.converged ? MGF_OK : MGF_ERR_NOT_CONVERGED;
+    if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "pressure solve stopped before its tolerance");
+    return MGF_OK;
 }
 
 mgf_status mgf_scene_extrapolate_velocity(mgf_scene s, int32_t* layers_used) {
@@ -765,7 +894,8 @@ mgf_status mgf_step(mgf_scene s, int32_t nsteps, mgf_step_stats* stats) {
     for (int i = 0; i < nsteps; ++i) TRY(step_once(s, &acc));
     TRY(sync(s));
     if (stats) *stats = acc;
-    return acc.converged ? MGF_OK : MGF_ERR_NOT_CONVERGED;
+    if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "a pressure solve stopped before its tolerance");
+    return MGF_OK;
 }
 
 mgf_status mgf_scene_sync(mgf_scene s) {
@@ -779,43 +909,64 @@ mgf_status mgf_scene_set_timing(mgf_scene s, int32_t enabled) {
     return MGF_OK;
 }
 
-mgf_status mgf_scene_time_cg_kernels(mgf_scene s, int32_t iters, double* ms_p_ap, double* ms_update) {
-    if (!s || iters < 1 || !ms_p_ap || !ms_update) return fail(MGF_ERR_INVALID, "bad argument");
+mgf_status mgf_scene_time_cg_kernels(mgf_scene s, int32_t iters, double* ms) {
+    if (!s || iters < 1 || !ms) return fail(MGF_ERR_INVALID, "bad argument");
+    if (s->desc.precision != MGF_PRECISION_MIXED)
+        return fail(MGF_ERR_INVALID, "the fused kernels belong to MIXED precision");
     MGF_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
     float sa[3], sg[3];
     scales(s, sa, sg);
-    std::vector<hipEvent_t> ev(size_t(iters) * 4);
+    const bool ident = s->desc.preconditioner == MGF_PRECOND_IDENTITY;
+    MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
+    if (!ident) {
+        TRY(ensure_precond(s));
+        TRY(mg_setup(s, sa));
+    }
+    std::vector<hipEvent_t> ev(size_t(iters) * 6);
     for (auto& e : ev) MGF_HIP(hipEventCreate(&e));
     // target 0: never stops on its own
     hipError_t err = launch_cg_init(st, s->g, s->b, s->r, nullptr, s->x, s->scal, s->red, 0.0, 0.0, 1);
+    const float* zv = s->r;
+    mgf_status pst = MGF_OK;
+    if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);
     float* pold = s->p0;
     float* pnew = s->p1;
-    for (int i = 0; i < iters && err == hipSuccess; ++i) {
-        err = hipEventRecord(ev[4 * i], st);
-        if (err == hipSuccess) err = launch_cg_p_ap(st, s->g, sa, s->markers, s->r, pold, pnew, s->scal, s->red);
-        if (err == hipSuccess) err = hipEventRecord(ev[4 * i + 1], st);
-        if (err == hipSuccess) err = hipEventRecord(ev[4 * i + 2], st);
-        if (err == hipSuccess) err = launch_cg_update(st, s->g, sa, s->markers, pnew, s->x, s->r, s->scal, s->red);
-        if (err == hipSuccess) err = hipEventRecord(ev[4 * i + 3], st);
+    for (int i = 0; i < iters && err == hipSuccess && pst == MGF_OK; ++i) {
+        hipEvent_t* e = &ev[6 * size_t(i)];
+        err = hipEventRecord(e[0], st);
+        if (err == hipSuccess) err = launch_col_p_ap(st, s->g, sa, s->codes, zv, pold, pnew, s->scal, s->red);
+        if (err == hipSuccess) err = hipEventRecord(e[1], st);
+        if (err == hipSuccess) err = hipEventRecord(e[2], st);
+        if (err == hipSuccess)
+            err = launch_col_update(st, s->g, sa, s->codes, pnew, s->x, s->r, s->scal, s->red, ident ? 1 : 0);
+        if (err == hipSuccess) err = hipEventRecord(e[3], st);
+        if (err == hipSuccess) err = hipEventRecord(e[4], st);
+        if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);
+        if (err == hipSuccess) err = hipEventRecord(e[5], st);
         std::swap(pold, pnew);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(st);
-    double a = 0, b = 0;
-    for (int i = 0; i < iters && err == hipSuccess; ++i) {
-        float t0 = 0, t1 = 0;
-        err = hipEventElapsedTime(&t0, ev[4 * i], ev[4 * i + 1]);
-        if (err == hipSuccess) err = hipEventElapsedTime(&t1, ev[4 * i + 2], ev[4 * i + 3]);
+    double a = 0, b = 0, c = 0;
+    for (int i = 0; i < iters && err == hipSuccess && pst == MGF_OK; ++i) {
+        const hipEvent_t* e = &ev[6 * size_t(i)];
+        float t0 = 0, t1 = 0, t2 = 0;
+        err = hipEventElapsedTime(&t0, e[0], e[1]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&t1, e[2], e[3]);
+        if (err == hipSuccess) err = hipEventElapsedTime(&t2, e[4], e[5]);
         a += t0;
         b += t1;
+        c += t2;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
+    if (pst != MGF_OK) return pst;
     if (err != hipSuccess) return fail(MGF_ERR_DEVICE, std::string("time_cg: ") + hipGetErrorString(err));
     CgScalars hs{};
     MGF_HIP(hipMemcpy(&hs, s->scal, sizeof(hs), hipMemcpyDeviceToHost));
     if (hs.done) return fail(MGF_ERR_INVALID, "the system converged before the timed iterations ended");
-    *ms_p_ap = a / iters;
-    *ms_update = b / iters;
+    ms[0] = a / iters;
+    ms[1] = b / iters;
+    ms[2] = ident ? 0.0 : c / iters;
     return MGF_OK;
 }
 
@@ -921,22 +1072,20 @@ mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid, float* u
     if (!ctx) return fail(MGF_ERR_INVALID, "null context");
     TRY(check_grid(grid));
     if (layers < 0) return fail(MGF_ERR_INVALID, "layers < 0");
-    // A transient scene-shaped scratch holder for the frontier buffers.
+    // A transient scene-shaped holder of the extrapolation scratch.
+    MGF_HIP(hipSetDevice(ctx->device));
     mgf_scene_s tmp;
     tmp.ctx = ctx;
     tmp.g = make_grid(grid);
     const size_t nface = size_t(grid->nx + 1) * grid->ny * grid->nz +
                          size_t(grid->nx) * (grid->ny + 1) * grid->nz +
                          size_t(grid->nx) * grid->ny * (grid->nz + 1);
-    mgf_status st = MGF_OK;
-    st = dalloc(&tmp, &tmp.layer, nface);
-    if (st == MGF_OK) st = dalloc(&tmp, &tmp.list[0], nface);
-    if (st == MGF_OK) st = dalloc(&tmp, &tmp.list[1], nface);
-    if (st == MGF_OK) st = dalloc(&tmp, &tmp.counts, kLayerChunk + 1);
+    mgf_status st = alloc_extrapolation(&tmp, nface);
     int used = 0;
     if (st == MGF_OK) st = extrapolate(&tmp, u, v, w, layers, &used);
     (void)hipStreamSynchronize(ctx->stream);
     for (void* p : tmp.allocations) (void)hipFree(p);
+    if (tmp.host_offs) (void)hipHostFree(tmp.host_offs);
     if (layers_used) *layers_used = used;
     return st;
 }

@@ -34,6 +34,22 @@ struct ReduceBuf {
     int blocks = 0;                 // blocks used by the fused kernels
 };
 
+// One multigrid level: extents and the per-axis operator scales.
+struct MgDims {
+    int nx, ny, nz;
+    float sx, sy, sz;
+};
+
+// Device buffers of a coarse multigrid level (level 0 is the pressure grid).
+struct MgLevelBuf {
+    MgDims d{};
+    uint8_t* markers = nullptr;
+    uint16_t* codes = nullptr;
+    float* b = nullptr;
+    float* x = nullptr;
+    float* t = nullptr;     // Jacobi ping-pong partner of x
+};
+
 struct Solid {
     float* sdf0 = nullptr;   // original SDF on the device
     float vel[3] = {0, 0, 0};
@@ -79,12 +95,19 @@ struct mgf_scene_s {
     mgf::CgScalars* scal = nullptr;
     mgf::ReduceBuf red;
     double* host_scratch = nullptr; // pinned host mirror of small reductions
+    uint16_t* codes = nullptr;     // per-cell stencil codes of the pressure matrix
+    // preconditioner (allocated on first use)
+    float* z = nullptr;            // M^-1 r
+    float* zt = nullptr;           // Jacobi ping-pong partner of z
+    std::vector<mgf::MgLevelBuf> mg;   // coarse levels 1..L-1
+    int mg_nx = -1, mg_ny = -1, mg_nz = -1;
     bool x_is_64 = false;          // which pressure the last solve produced
 
     // extrapolation scratch
-    int32_t* layer = nullptr;      // nfx+nfy+nfz
-    uint32_t* list[2] = {nullptr, nullptr};
-    uint32_t* counts = nullptr;    // per-layer list sizes [kMaxLayerChunk+2]
+    int32_t* layer = nullptr;      // distance of every face (nfx+nfy+nfz)
+    uint32_t* list = nullptr;      // faces sorted by distance (nfx+nfy+nfz)
+    uint32_t* counts = nullptr;    // histogram, offsets, cursors [3 (K+2) + 1]
+    uint32_t* host_offs = nullptr; // pinned mirror of the offsets and the max distance
 
     std::vector<mgf::Solid> solids;
     bool boundary_dirty = false;   // solids changed since the last refresh
@@ -154,12 +177,6 @@ hipError_t launch_fill64(hipStream_t s, double* a, size_t n, double v);
 hipError_t launch_cg_init(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* p,
                           float* x, CgScalars* sc, ReduceBuf red, double rel_inner,
                           double abs_target2, int first_sweep);
-hipError_t launch_cg_p_ap(hipStream_t s, const GridDev& g, const float scale[3],
-                          const uint8_t* markers, const float* r, const float* p_old,
-                          float* p_new, CgScalars* sc, ReduceBuf red);
-hipError_t launch_cg_update(hipStream_t s, const GridDev& g, const float scale[3],
-                            const uint8_t* markers, const float* p, float* x, float* r,
-                            CgScalars* sc, ReduceBuf red);
 // x64 += x (FP32 correction), then r = b - A x64 in FP64 (stored FP32) and
 // ||r||^2 into partial sums (reduced by the host).
 hipError_t launch_refine(hipStream_t s, const GridDev& g, const float scale[3],
@@ -174,13 +191,41 @@ hipError_t launch_project(hipStream_t s, const GridDev& g, const float sg[3],
                           const uint8_t* markers, const float* p32, const double* p64,
                           float* u, float* v, float* w, const float* us, const float* vs,
                           const float* ws);
-// Extrapolation by frontiers.
-hipError_t launch_extrap_init(hipStream_t s, const GridDev& g, const float* u, const float* v,
-                              const float* w, int32_t* layer, uint32_t* list, uint32_t* count);
-hipError_t launch_extrap_layer(hipStream_t s, const GridDev& g, int layer_id, float* u, float* v,
-                               float* w, int32_t* layer, const uint32_t* in_list,
-                               const uint32_t* in_count, uint32_t* out_list,
-                               uint32_t* out_count);
+// ---- column-marching CG and multigrid (mgf_kernels_mg.hip) -----------------
+hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* markers, uint16_t* codes);
+// Blocks of the column kernels on grid g (the partial-sum buffer must hold
+// four doubles per block).
+uint32_t col_blocks(const GridDev& g);
+// p_new = z + beta p_old (z on the first iteration), p . A p, alpha.
+hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
+                           const float* zv, const float* p_old, float* p_new, CgScalars* sc, ReduceBuf red);
+// x += alpha p, r -= alpha A p, r . r; identity != 0: beta = rr / rz, rz = rr.
+hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
+                             const float* p, float* x, float* r, CgScalars* sc, ReduceBuf red, int identity);
+hipError_t launch_mg_coarsen(hipStream_t s, const MgDims& f, const MgDims& c, const uint8_t* mf, uint8_t* mc,
+                             int fluid_first);
+hipError_t launch_mg_smooth(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
+                            float* xo, float omega, CgScalars* sc, ReduceBuf red, int first, int with_rz);
+hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
+                              const float* bf, const float* xf, const uint16_t* codec, float* bc,
+                              const CgScalars* sc);
+hipError_t launch_mg_prolong(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
+                             const float* xc, float* xf, const CgScalars* sc);
+// ---- extrapolation (mgf_kernels_ext.hip) ------------------------------------
+// Largest L1 distance between two faces of one component grid (+ margin).
+int32_t ext_max_distance(const GridDev& g);
+// dist = L1 distance of every face to the nearest known face (1 << 29 where
+// a component has none); faces indexed u, then v, then w.
+hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
+                               int32_t* dist);
+// Counting sort of the faces with 1 <= dist <= cap: hist (zeroed by the
+// caller) counts, offs[L] = first slot of distance L in list (offs[cap+1] =
+// total), *maxd (zeroed by the caller) = largest distance present.
+hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, uint32_t* hist,
+                           uint32_t* offs, uint32_t* cursor, uint32_t* maxd, uint32_t* list);
+// Layer L: list[begin, begin+count) take the mean of their neighbours below L.
+hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const int32_t* dist,
+                            const uint32_t* list, uint32_t begin, uint32_t count, int32_t L);
 // Boundaries.
 hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0,
                                  const float t[3], float* out);

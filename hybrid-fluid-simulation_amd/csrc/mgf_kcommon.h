@@ -1,0 +1,101 @@
+// mgf_kcommon.h — helpers shared by the kernel translation units: XCD-aware
+// block numbering, FP64 wave/block/grid reductions, launch wrapper.
+#pragma once
+
+#include "mgf_internal.h"
+
+namespace mgf {
+
+constexpr uint32_t kXcds = 8;
+
+// Workgroups are dispatched round-robin over the XCDs: physical block b runs on
+// XCD b % 8.  Map it to logical block (b % 8) * (G / 8) + b / 8 so that XCD k
+// owns logical blocks [k G/8, (k+1) G/8).  Requires G % 8 == 0.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t b = blockIdx.x, G = gridDim.x;
+    return (b % kXcds) * (G / kXcds) + b / kXcds;
+}
+
+inline uint32_t grid_for(size_t n) {
+    size_t g = (n + kBlock - 1) / kBlock;
+    g = (g + kXcds - 1) / kXcds * kXcds;
+    if (g == 0) g = kXcds;
+    return uint32_t(g);
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 / block reductions in FP64.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sums `n` doubles of a block (blockDim.x == kBlock); result valid in thread 0.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+    __shared__ double sh[N][kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < kBlock / 64; ++q) s += sh[k][q];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// One-pass grid reduction: every block stores its partials, the last block to
+// arrive (ticket) sums all partials in block order and returns true in all of
+// its threads; sums are then in out[] (valid in thread 0).
+template <int N>
+__device__ __forceinline__ bool grid_sum(double (&v)[N], double* partials, unsigned int* ticket,
+                                         double (&out)[N]) {
+    block_sum<N>(v);
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            __hip_atomic_store(&partials[size_t(blockIdx.x) * N + k], v[k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned int t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0;
+    for (uint32_t bI = threadIdx.x; bI < gridDim.x; bI += kBlock) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            acc[k] += __hip_atomic_load(&partials[size_t(bI) * N + k], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<N>(acc);
+#pragma unroll
+    for (int k = 0; k < N; ++k) out[k] = acc[k];
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+
+template <class K, class... A>
+inline hipError_t launch_k(K kern, uint32_t grid, hipStream_t s, A... args) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, args...);
+    return hipGetLastError();
+}
+#define LAUNCH(kern, grid, ...) launch_k(kern, uint32_t(grid), s, __VA_ARGS__)
+
+}  // namespace mgf

@@ -7,92 +7,9 @@
 // walks one contiguous slab of the grid (neighbouring planes then meet in the
 // same L2).
 #include "mgf_internal.h"
+#include "mgf_kcommon.h"
 
 namespace mgf {
-
-constexpr uint32_t kXcds = 8;
-
-// Workgroups are dispatched round-robin over the XCDs: physical block b runs on
-// XCD b % 8.  Map it to logical block (b % 8) * (G / 8) + b / 8 so that XCD k
-// owns logical blocks [k G/8, (k+1) G/8).  Requires G % 8 == 0.
-__device__ __forceinline__ uint32_t xcd_block() {
-    const uint32_t b = blockIdx.x, G = gridDim.x;
-    return (b % kXcds) * (G / kXcds) + b / kXcds;
-}
-
-static inline uint32_t grid_for(size_t n) {
-    size_t g = (n + kBlock - 1) / kBlock;
-    g = (g + kXcds - 1) / kXcds * kXcds;
-    if (g == 0) g = kXcds;
-    return uint32_t(g);
-}
-
-// ---------------------------------------------------------------------------
-// Wave64 / block reductions in FP64.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Sums `n` doubles of a block (blockDim.x == kBlock); result valid in thread 0.
-template <int N>
-__device__ __forceinline__ void block_sum(double (&v)[N]) {
-    __shared__ double sh[N][kBlock / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) sh[k][wv] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            double s = 0;
-#pragma unroll
-            for (int q = 0; q < kBlock / 64; ++q) s += sh[k][q];
-            v[k] = s;
-        }
-    }
-    __syncthreads();
-}
-
-// One-pass grid reduction: every block stores its partials, the last block to
-// arrive (ticket) sums all partials in block order and returns true in all of
-// its threads; sums are then in out[] (valid in thread 0).
-template <int N>
-__device__ __forceinline__ bool grid_sum(double (&v)[N], double* partials, unsigned int* ticket,
-                                         double (&out)[N]) {
-    block_sum<N>(v);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            __hip_atomic_store(&partials[size_t(blockIdx.x) * N + k], v[k], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) acc[k] = 0;
-    for (uint32_t bI = threadIdx.x; bI < gridDim.x; bI += kBlock) {
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            acc[k] += __hip_atomic_load(&partials[size_t(bI) * N + k], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<N>(acc);
-#pragma unroll
-    for (int k = 0; k < N; ++k) out[k] = acc[k];
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
-}
 
 // ---------------------------------------------------------------------------
 // Back-trace of arbitrary points and cell-centred sampling (operator seams).
@@ -400,92 +317,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(uint32_t n, const float* __r
     }
 }
 
-// p_new = r + beta * p_old (p_new = r on the first iteration); A p_new with
-// the neighbours' p_new recomputed in registers; p . A p reduced; alpha.
-__global__ void __launch_bounds__(kBlock) k_cg_p_ap(GridDev g, const uint8_t* __restrict__ m,
-                                                    float sx, float sy, float sz,
-                                                    const float* __restrict__ r,
-                                                    const float* __restrict__ p_old,
-                                                    float* __restrict__ p_new, CgScalars* sc,
-                                                    double* partials, unsigned int* ticket) {
-    if (sc->done) return;
-    const double beta = sc->beta;
-    const bool first = sc->iters == 0;
-    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
-    const Span sp = block_span(n);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    double s[1] = {0.0};
-    for (uint32_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBlock) {
-        const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
-        const Stencil st = stencil_at(m, x, y, z, nx, ny, nz, sx, sy, sz);
-        double pv[7];
-#pragma unroll
-        for (int q = 0; q < 7; ++q) {
-            const uint32_t j = st.idx[q];
-            pv[q] = first ? double(r[j]) : double(float(__builtin_fma(beta, double(p_old[j]), double(r[j]))));
-        }
-        double ap = double(st.coef[0]) * pv[0];
-#pragma unroll
-        for (int q = 1; q < 7; ++q)
-            if (st.on[q]) ap += double(st.coef[q]) * pv[q];
-        p_new[i] = float(pv[0]);
-        s[0] += pv[0] * ap;
-    }
-    double tot[1];
-    if (grid_sum<1>(s, partials, ticket, tot) && threadIdx.x == 0) {
-        sc->pap = tot[0];
-        if (!isfinite(tot[0])) {
-            sc->done = 2;          // non-finite: stop, reported by the host
-            sc->alpha = 0.0;
-        } else if (tot[0] == 0.0) {
-            sc->done = 1;
-            sc->alpha = 0.0;
-        } else {
-            sc->alpha = sc->rz / tot[0];
-        }
-    }
-}
-
-// x += alpha p; r -= alpha A p (A p recomputed); r . r reduced; beta; stop test.
-__global__ void __launch_bounds__(kBlock) k_cg_update(GridDev g, const uint8_t* __restrict__ m,
-                                                      float sx, float sy, float sz,
-                                                      const float* __restrict__ p,
-                                                      float* __restrict__ x, float* __restrict__ r,
-                                                      CgScalars* sc, double* partials,
-                                                      unsigned int* ticket) {
-    if (sc->done) return;
-    const double alpha = sc->alpha;
-    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
-    const Span sp = block_span(n);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    double s[1] = {0.0};
-    for (uint32_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBlock) {
-        const int xx = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
-        const Stencil st = stencil_at(m, xx, y, z, nx, ny, nz, sx, sy, sz);
-        double ap = double(st.coef[0]) * double(p[st.idx[0]]);
-#pragma unroll
-        for (int q = 1; q < 7; ++q)
-            if (st.on[q]) ap += double(st.coef[q]) * double(p[st.idx[q]]);
-        const float xn = float(__builtin_fma(alpha, double(p[i]), double(x[i])));
-        const float rn = float(__builtin_fma(-alpha, ap, double(r[i])));
-        x[i] = xn;
-        r[i] = rn;
-        s[0] += double(rn) * double(rn);
-    }
-    double tot[1];
-    if (grid_sum<1>(s, partials, ticket, tot) && threadIdx.x == 0) {
-        const double rz_new = tot[0];
-        sc->rr = rz_new;
-        sc->beta = rz_new / sc->rz;
-        sc->rz = rz_new;
-        sc->iters = sc->iters + 1;
-        if (rz_new <= sc->target2) sc->done = 1;
-        else if (!isfinite(rz_new)) sc->done = 2;
-    }
-}
-
 // x64 (+)= x32 correction.
 __global__ void __launch_bounds__(kBlock) k_accumulate(uint32_t n, const float* __restrict__ xc,
                                                        double* __restrict__ x64, int accumulate) {
@@ -606,123 +437,6 @@ __global__ void __launch_bounds__(kBlock) k_project(GridDev g, const uint8_t* __
 }
 
 // ---------------------------------------------------------------------------
-// Extrapolation (MPhysics/EulerSolverTool.cu:450-557, 904-969) by frontiers.
-// Entries of the three face grids are addressed as (component << 30) | index.
-constexpr int32_t kLayerUnknown = 0x7fffffff;
-
-struct FaceGrids {
-    float* f[3];
-    int ex[3], ey[3], ez[3];
-    uint32_t off[3];   // start of each component in the layer array
-};
-
-__device__ __forceinline__ void decode(const FaceGrids& G, uint32_t e, int& c, int& x, int& y, int& z) {
-    c = int(e >> 30);
-    const uint32_t i = e & 0x3fffffffu;
-    const uint32_t ex = uint32_t(G.ex[c]), ey = uint32_t(G.ey[c]);
-    x = int(i % ex);
-    y = int((i / ex) % ey);
-    z = int(i / (ex * ey));
-}
-
-// Wave-aggregated append of k (<= 6) entries per lane to list/count.
-__device__ __forceinline__ void wave_append(uint32_t* list, uint32_t* count, const uint32_t* items,
-                                            int k) {
-    const int lane = threadIdx.x & 63;
-    int incl = k;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    const int total = __shfl(incl, 63, 64);
-    uint32_t base = 0;
-    if (lane == 63 && total > 0) base = atomicAdd(count, uint32_t(total));
-    base = __shfl(base, 63, 64);
-    const uint32_t start = base + uint32_t(incl - k);
-    for (int q = 0; q < k; ++q) list[start + q] = items[q];
-}
-
-// Layer 0: known entries (value != UNKNOWN); the frontier keeps those with an
-// unknown neighbour.
-__global__ void __launch_bounds__(kBlock) k_extrap_init(FaceGrids G, int32_t* layer, uint32_t* list,
-                                                        uint32_t* count, uint32_t total) {
-    const uint32_t gi = xcd_block() * kBlock + threadIdx.x;
-    uint32_t item[1];
-    int k = 0;
-    if (gi < total) {
-        const int c = gi < G.off[1] ? 0 : (gi < G.off[2] ? 1 : 2);
-        const uint32_t i = gi - G.off[c];
-        const int ex = G.ex[c], ey = G.ey[c], ez = G.ez[c];
-        const int x = int(i % uint32_t(ex)), y = int((i / uint32_t(ex)) % uint32_t(ey)),
-                  z = int(i / (uint32_t(ex) * uint32_t(ey)));
-        const float* f = G.f[c];
-        const bool known = f[i] != MGF_UNKNOWN;
-        layer[gi] = known ? 0 : kLayerUnknown;
-        if (known) {
-            const bool edge = f[lin3(x - 1, y, z, ex, ey, ez)] == MGF_UNKNOWN ||
-                              f[lin3(x + 1, y, z, ex, ey, ez)] == MGF_UNKNOWN ||
-                              f[lin3(x, y - 1, z, ex, ey, ez)] == MGF_UNKNOWN ||
-                              f[lin3(x, y + 1, z, ex, ey, ez)] == MGF_UNKNOWN ||
-                              f[lin3(x, y, z - 1, ex, ey, ez)] == MGF_UNKNOWN ||
-                              f[lin3(x, y, z + 1, ex, ey, ez)] == MGF_UNKNOWN;
-            if (edge) item[k++] = (uint32_t(c) << 30) | i;
-        }
-    }
-    wave_append(list, count, item, k);
-}
-
-// Layer L: every unknown neighbour of a layer L-1 entry joins layer L (the
-// claiming lane is the one whose compare-and-swap succeeds) and takes the mean
-// of its neighbours in layers < L, summed in the order -x,+x,-y,+y,-z,+z.
-__global__ void __launch_bounds__(kBlock) k_extrap_layer(FaceGrids G, int L, int32_t* layer,
-                                                         const uint32_t* in_list,
-                                                         const uint32_t* in_count,
-                                                         uint32_t* out_list, uint32_t* out_count) {
-    const uint32_t nin = *in_count;
-    const uint32_t stride = gridDim.x * kBlock;
-    // every lane of a wave runs the same number of rounds (wave_append is collective)
-    const uint32_t rounds = (nin + stride - 1) / stride;
-    for (uint32_t rd = 0; rd < rounds; ++rd) {
-        const uint32_t t = rd * stride + xcd_block() * kBlock + threadIdx.x;
-        uint32_t items[6];
-        int k = 0;
-        if (t < nin) {
-            int c, x, y, z;
-            decode(G, in_list[t], c, x, y, z);
-            const int ex = G.ex[c], ey = G.ey[c], ez = G.ez[c];
-            int32_t* lay = layer + G.off[c];
-            float* f = G.f[c];
-            const int nb[6][3] = {{x - 1, y, z}, {x + 1, y, z}, {x, y - 1, z},
-                                  {x, y + 1, z}, {x, y, z - 1}, {x, y, z + 1}};
-            for (int q = 0; q < 6; ++q) {
-                const uint32_t j = lin3(nb[q][0], nb[q][1], nb[q][2], ex, ey, ez);
-                if (lay[j] != kLayerUnknown) continue;
-                if (atomicCAS(&lay[j], kLayerUnknown, L) != kLayerUnknown) continue;
-                const int jx = int(j % uint32_t(ex)), jy = int((j / uint32_t(ex)) % uint32_t(ey)),
-                          jz = int(j / (uint32_t(ex) * uint32_t(ey)));
-                const uint32_t s[6] = {lin3(jx - 1, jy, jz, ex, ey, ez), lin3(jx + 1, jy, jz, ex, ey, ez),
-                                       lin3(jx, jy - 1, jz, ex, ey, ez), lin3(jx, jy + 1, jz, ex, ey, ez),
-                                       lin3(jx, jy, jz - 1, ex, ey, ez), lin3(jx, jy, jz + 1, ex, ey, ez)};
-                float acc = 0.0f, cnt = 0.0f;
-#pragma unroll
-                for (int e = 0; e < 6; ++e) {
-                    const int32_t ls = __hip_atomic_load(&lay[s[e]], __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-                    if (ls < L) {
-                        acc += f[s[e]];
-                        cnt += 1.0f;
-                    }
-                }
-                if (cnt != 0.0f) acc /= cnt;
-                f[j] = acc;
-                items[k++] = (uint32_t(c) << 30) | j;
-            }
-        }
-        wave_append(out_list, out_count, items, k);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Boundaries (MPhysics/Boundary.cpp:115-150, MPhysics/BoundaryTool.cu:3-113).
 __global__ void __launch_bounds__(kBlock) k_resample_solid(GridDev g, const float* __restrict__ sdf0,
@@ -825,12 +539,7 @@ RkCoef rk_coef(float dt) {
     return k;
 }
 
-template <class K, class... A>
-static hipError_t launch_k(K kern, uint32_t grid, hipStream_t s, A... args) {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, args...);
-    return hipGetLastError();
-}
-#define LAUNCH(kern, grid, ...) launch_k(kern, uint32_t(grid), s, __VA_ARGS__)
+
 
 hipError_t launch_backtrace(hipStream_t s, const GridDev& g, Vel V, float dt, int order, float* px,
                             float* py, float* pz, int64_t n) {
@@ -936,20 +645,6 @@ hipError_t launch_cg_init(hipStream_t s, const GridDev& g, const float* rsrc, fl
                   abs_target2);
 }
 
-hipError_t launch_cg_p_ap(hipStream_t s, const GridDev& g, const float scale[3],
-                          const uint8_t* markers, const float* r, const float* p_old, float* p_new,
-                          CgScalars* sc, ReduceBuf red) {
-    return LAUNCH(k_cg_p_ap, red.blocks, g, markers, scale[0], scale[1], scale[2], r, p_old, p_new,
-                  sc, red.partials, red.ticket);
-}
-
-hipError_t launch_cg_update(hipStream_t s, const GridDev& g, const float scale[3],
-                            const uint8_t* markers, const float* p, float* x, float* r,
-                            CgScalars* sc, ReduceBuf red) {
-    return LAUNCH(k_cg_update, red.blocks, g, markers, scale[0], scale[1], scale[2], p, x, r, sc,
-                  red.partials, red.ticket);
-}
-
 hipError_t launch_refine(hipStream_t s, const GridDev& g, const float scale[3],
                          const uint8_t* markers, const float* xcorr, double* x64, const float* b,
                          float* r, double* partials, int blocks, int accumulate) {
@@ -974,37 +669,6 @@ hipError_t launch_project(hipStream_t s, const GridDev& g, const float sg[3], co
     const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
     return LAUNCH(k_project, grid_for(n), g, markers, sg[0], sg[1], sg[2], p32, p64, u, v, w, us, vs,
                   ws, n);
-}
-
-static FaceGrids face_grids(const GridDev& g, float* u, float* v, float* w) {
-    FaceGrids G;
-    G.f[0] = u; G.f[1] = v; G.f[2] = w;
-    for (int c = 0; c < 3; ++c) {
-        G.ex[c] = g.nx + (c == 0);
-        G.ey[c] = g.ny + (c == 1);
-        G.ez[c] = g.nz + (c == 2);
-    }
-    G.off[0] = 0;
-    G.off[1] = uint32_t(G.ex[0]) * G.ey[0] * G.ez[0];
-    G.off[2] = G.off[1] + uint32_t(G.ex[1]) * G.ey[1] * G.ez[1];
-    return G;
-}
-
-hipError_t launch_extrap_init(hipStream_t s, const GridDev& g, const float* u, const float* v,
-                              const float* w, int32_t* layer, uint32_t* list, uint32_t* count) {
-    FaceGrids G = face_grids(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
-    const uint32_t total = G.off[2] + uint32_t(g.nx) * g.ny * (g.nz + 1);
-    return LAUNCH(k_extrap_init, grid_for(total), G, layer, list, count, total);
-}
-
-hipError_t launch_extrap_layer(hipStream_t s, const GridDev& g, int layer_id, float* u, float* v,
-                               float* w, int32_t* layer, const uint32_t* in_list,
-                               const uint32_t* in_count, uint32_t* out_list, uint32_t* out_count) {
-    FaceGrids G = face_grids(g, u, v, w);
-    const uint32_t total = G.off[2] + uint32_t(g.nx) * g.ny * (g.nz + 1);
-    uint32_t blocks = grid_for(total);
-    if (blocks > 2048) blocks = 2048;
-    return LAUNCH(k_extrap_layer, blocks, G, layer_id, layer, in_list, in_count, out_list, out_count);
 }
 
 hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0, const float t[3],

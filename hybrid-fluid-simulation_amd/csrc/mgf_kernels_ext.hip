@@ -1,0 +1,347 @@
+// mgf_kernels_ext.hip — velocity extrapolation of extrapolatingDataInvoker
+// (MPhysics/EulerSolverTool.cu:450-557, 904-969).
+//
+// The reference assigns every face of a component a distance marker by
+// breadth-first layers (buildExtrapolationMarkers: layer 0 = known faces,
+// layer L = unknown faces with a layer L-1 neighbour, for L < N) and then, for
+// L = 1 .. N-1 in order, sets each layer-L face to the mean of its neighbours
+// whose marker is below L (extrapolatingData).  The breadth-first search runs
+// on the full face grid with no obstacles, so the marker of a face is its L1
+// distance to the nearest known face (unknown beyond N-1).  Here:
+//
+//   1. distance transform, exact and separable: along x a wave-wide
+//      min-plus scan per row, along y and z a forward and a backward sweep per
+//      line (d = min(d, neighbour + 1));
+//   2. counting sort of the faces with 1 <= d <= N-1 by d (LDS histogram per
+//      block, one global atomic per block and distance);
+//   3. one gather kernel per layer, over that layer's faces only, in layer
+//      order: the neighbours in layers below L are final when layer L runs.
+//
+// The mean is summed in the reference's neighbour order (-x, +x, -y, +y, -z,
+// +z) in FP32 and divided once, so the values are bitwise the reference's.
+#include "mgf_internal.h"
+#include "mgf_kcommon.h"
+
+namespace mgf {
+
+constexpr int32_t kDistInf = 1 << 29;   // unreached
+
+struct Faces {
+    float* f[3];
+    uint32_t ex[3], ey[3], ez[3];
+    uint32_t off[4];      // first global face index of each component; off[3] = total
+    uint32_t rows[4];     // prefix counts of x-rows (ey * ez per component)
+    uint32_t ylines[4];   // prefix counts of y-lines (ex * ez)
+    uint32_t zlines[4];   // prefix counts of z-lines (ex * ey)
+    double rex[3], rexy[3];   // 1/ex, 1/(ex*ey)
+};
+
+__device__ __forceinline__ int comp_of(const uint32_t (&pre)[4], uint32_t t) {
+    return t < pre[1] ? 0 : (t < pre[2] ? 1 : 2);
+}
+
+__device__ __forceinline__ uint32_t udiv_r(uint32_t a, uint32_t d, double rd) {
+    uint32_t q = uint32_t(double(a) * rd);
+    const uint32_t r = a - q * d;
+    if (int32_t(r) < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+}
+
+// ---------------------------------------------------------------------------
+// 1a. Rows along x: d = 0 on known faces, INF elsewhere; then the 1-D L1
+// distance: forward d(x) = x + min_{k<=x} (d(k) - k), backward
+// d(x) = min_{k>=x} (d(k) + k) - x.  One wave per row.
+__global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restrict__ dist) {
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wave >= F.rows[3]) return;
+    const int c = comp_of(F.rows, wave);
+    const uint32_t ex = F.ex[c];
+    const uint32_t local = (wave - F.rows[c]) * ex;     // first face of the row in component c
+    const float* f = F.f[c];
+    int32_t* d = dist + F.off[c] + local;
+    // forward
+    int32_t carry = kDistInf;
+    for (uint32_t x0 = 0; x0 < ex; x0 += 64) {
+        const uint32_t x = x0 + uint32_t(lane);
+        const bool ok = x < ex;
+        const int32_t d0 = ok ? (f[local + x] != MGF_UNKNOWN ? 0 : kDistInf) : kDistInf;
+        int32_t g = d0 - int32_t(x);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t t = __shfl_up(g, o, 64);
+            if (lane >= o) g = min(g, t);
+        }
+        g = min(g, carry);
+        carry = __shfl(g, 63, 64);
+        if (ok) d[x] = min(g + int32_t(x), kDistInf);
+    }
+    // backward (each lane re-reads what it wrote)
+    carry = 2 * kDistInf;
+    const uint32_t nch = (ex + 63) / 64;
+    for (uint32_t ch = nch; ch-- > 0;) {
+        const uint32_t x = ch * 64 + uint32_t(lane);
+        const bool ok = x < ex;
+        int32_t h = ok ? d[x] + int32_t(x) : 2 * kDistInf;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t t = __shfl_down(h, o, 64);
+            if (lane + o < 64) h = min(h, t);
+        }
+        h = min(h, carry);
+        carry = __shfl(h, 0, 64);
+        if (ok) d[x] = min(h - int32_t(x), kDistInf);
+    }
+}
+
+// 1b. Lines along y (axis 1) or z (axis 2): d = min(d, d(prev) + 1) forward
+// then backward; one work-item per line, lines x-fastest (coalesced), eight
+// loads in flight per chunk.
+template <int AXIS>
+__global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restrict__ dist) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
+    if (t >= pre[3]) return;
+    const int c = comp_of(pre, t);
+    const uint32_t l = t - pre[c];
+    const uint32_t ex = F.ex[c], ey = F.ey[c];
+    const uint32_t w = udiv_r(l, ex, F.rex[c]);
+    const uint32_t x = l - w * ex;
+    uint32_t start, stride, len;
+    if (AXIS == 1) { start = w * ex * ey + x; stride = ex; len = ey; }
+    else { start = w * ex + x; stride = ex * ey; len = F.ez[c]; }
+    int32_t* d = dist + F.off[c] + start;
+    constexpr int K = 8;
+    int32_t prev = kDistInf;
+    for (uint32_t k0 = 0; k0 < len; k0 += K) {
+        int32_t v[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = (k0 + q < len) ? d[size_t(k0 + q) * stride] : 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (k0 + q < len) {
+                prev = min(v[q], prev + 1);
+                d[size_t(k0 + q) * stride] = prev;
+            }
+        }
+    }
+    prev = kDistInf;
+    for (uint32_t k1 = len; k1 > 0;) {
+        const uint32_t k0 = k1 >= K ? k1 - K : 0;
+        const uint32_t m = k1 - k0;
+        int32_t v[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = (uint32_t(q) < m) ? d[size_t(k1 - 1 - q) * stride] : 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (uint32_t(q) < m) {
+                prev = min(v[q], prev + 1);
+                d[size_t(k1 - 1 - q) * stride] = min(prev, kDistInf);
+            }
+        }
+        k1 = k0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Counting sort by distance.  Bins 1..cap; a block counts its faces in LDS
+// (kLdsBins bins; larger distances go straight to the global counters).
+constexpr int kLdsBins = 2048;
+constexpr int kSortCpt = 8;   // faces per work-item
+
+__global__ void __launch_bounds__(kBlock) k_ext_count(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
+                                                      uint32_t* __restrict__ hist, uint32_t* __restrict__ maxd) {
+    __shared__ uint32_t h[kLdsBins];
+    __shared__ int32_t bmax;
+    const int32_t nb = min(cap + 1, kLdsBins);
+    for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
+    if (threadIdx.x == 0) bmax = 0;
+    __syncthreads();
+    const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
+    int32_t mymax = 0;
+#pragma unroll
+    for (int k = 0; k < kSortCpt; ++k) {
+        const uint32_t i = base + uint32_t(k) * kBlock;
+        const int32_t dv = i < total ? dist[i] : 0;
+        if (dv >= 1 && dv <= cap) {
+            if (dv < kLdsBins) atomicAdd(&h[dv], 1u);
+            else atomicAdd(&hist[dv], 1u);
+            mymax = max(mymax, dv);
+        }
+    }
+    atomicMax(&bmax, mymax);
+    __syncthreads();
+    for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
+    if (threadIdx.x == 0 && bmax > 0) atomicMax(maxd, uint32_t(bmax));
+}
+
+// Exclusive scan of hist[1..cap] into offs[1..cap+1] (offs[1] = 0); cursor
+// cleared.  One block.
+__global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ hist, int32_t cap,
+                                                   uint32_t* __restrict__ offs, uint32_t* __restrict__ cursor) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int32_t b0 = 1; b0 <= cap; b0 += 1024) {
+        const int32_t b = b0 + int32_t(threadIdx.x);
+        const uint32_t v = b <= cap ? hist[b] : 0u;
+        uint32_t incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        uint32_t before = carry;
+        for (int q = 0; q < wv; ++q) before += wsum[q];
+        if (b <= cap) {
+            offs[b] = before + incl - v;
+            cursor[b] = 0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) offs[cap + 1] = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ext_scatter(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
+                                                        const uint32_t* __restrict__ offs, uint32_t* cursor,
+                                                        uint32_t* __restrict__ list) {
+    __shared__ uint32_t h[kLdsBins];
+    const int32_t nb = min(cap + 1, kLdsBins);
+    for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
+    __syncthreads();
+    const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
+    int32_t dv[kSortCpt];
+    uint32_t rank[kSortCpt];
+#pragma unroll
+    for (int k = 0; k < kSortCpt; ++k) {
+        const uint32_t i = base + uint32_t(k) * kBlock;
+        dv[k] = i < total ? dist[i] : 0;
+        rank[k] = 0;
+        if (dv[k] >= 1 && dv[k] <= cap) {
+            if (dv[k] < kLdsBins) rank[k] = atomicAdd(&h[dv[k]], 1u);
+            else rank[k] = atomicAdd(&cursor[dv[k]], 1u);
+        }
+    }
+    __syncthreads();
+    // reserve each bin's range: the bin's slot now holds its base
+    for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
+        if (h[b]) h[b] = atomicAdd(&cursor[b], h[b]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kSortCpt; ++k) {
+        const uint32_t i = base + uint32_t(k) * kBlock;
+        if (dv[k] >= 1 && dv[k] <= cap) {
+            const uint32_t pos = offs[dv[k]] + (dv[k] < kLdsBins ? h[dv[k]] : 0u) + rank[k];
+            list[pos] = i;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. Layer L: each listed face takes the mean of its in-grid neighbours with
+// distance below L.
+__global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const int32_t* __restrict__ dist,
+                                                      const uint32_t* __restrict__ list, uint32_t begin,
+                                                      uint32_t count, int32_t L) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t gi = list[begin + t];
+    const int c = comp_of(F.off, gi);
+    const uint32_t i = gi - F.off[c];
+    const uint32_t ex = F.ex[c], ey = F.ey[c], ez = F.ez[c], sz = ex * ey;
+    const uint32_t z = udiv_r(i, sz, F.rexy[c]);
+    const uint32_t rem = i - z * sz;
+    const uint32_t y = udiv_r(rem, ex, F.rex[c]);
+    const uint32_t x = rem - y * ex;
+    const int32_t* dc = dist + F.off[c];
+    float* f = F.f[c];
+    const bool in[6] = {x > 0, x + 1 < ex, y > 0, y + 1 < ey, z > 0, z + 1 < ez};
+    const uint32_t j[6] = {i - 1, i + 1, i - ex, i + ex, i - sz, i + sz};
+    int32_t dn[6];
+    float fv[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const uint32_t jj = in[q] ? j[q] : i;
+        dn[q] = dc[jj];
+        fv[q] = f[jj];
+    }
+    float acc = 0.0f, cnt = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (in[q] && dn[q] < L) {
+            acc += fv[q];
+            cnt += 1.0f;
+        }
+    }
+    if (cnt != 0.0f) acc /= cnt;
+    f[i] = acc;
+}
+
+// ===========================================================================
+static Faces faces_of(const GridDev& g, float* u, float* v, float* w) {
+    Faces F;
+    F.f[0] = u; F.f[1] = v; F.f[2] = w;
+    F.off[0] = F.rows[0] = F.ylines[0] = F.zlines[0] = 0;
+    for (int c = 0; c < 3; ++c) {
+        F.ex[c] = uint32_t(g.nx + (c == 0));
+        F.ey[c] = uint32_t(g.ny + (c == 1));
+        F.ez[c] = uint32_t(g.nz + (c == 2));
+        F.off[c + 1] = F.off[c] + F.ex[c] * F.ey[c] * F.ez[c];
+        F.rows[c + 1] = F.rows[c] + F.ey[c] * F.ez[c];
+        F.ylines[c + 1] = F.ylines[c] + F.ex[c] * F.ez[c];
+        F.zlines[c + 1] = F.zlines[c] + F.ex[c] * F.ey[c];
+        F.rex[c] = 1.0 / double(F.ex[c]);
+        F.rexy[c] = 1.0 / (double(F.ex[c]) * double(F.ey[c]));
+    }
+    return F;
+}
+
+int32_t ext_max_distance(const GridDev& g) { return g.nx + g.ny + g.nz + 3; }
+
+hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
+                               int32_t* dist) {
+    const Faces F = faces_of(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
+    hipLaunchKernelGGL(k_ext_rows, dim3((F.rows[3] + 3) / 4), dim3(kBlock), 0, s, F, dist);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ext_lines<1>, dim3((F.ylines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ext_lines<2>, dim3((F.zlines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, uint32_t* hist,
+                           uint32_t* offs, uint32_t* cursor, uint32_t* maxd, uint32_t* list) {
+    const Faces F = faces_of(g, nullptr, nullptr, nullptr);
+    const uint32_t total = F.off[3];
+    size_t G = (size_t(total) + size_t(kBlock) * kSortCpt - 1) / (size_t(kBlock) * kSortCpt);
+    G = (G + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(k_ext_count, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, hist, maxd);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, hist, cap, offs, cursor);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, offs, cursor, list);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const int32_t* dist,
+                            const uint32_t* list, uint32_t begin, uint32_t count, int32_t L) {
+    if (count == 0) return hipSuccess;
+    const Faces F = faces_of(g, u, v, w);
+    hipLaunchKernelGGL(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist, list, begin,
+                       count, L);
+    return hipGetLastError();
+}
+
+}  // namespace mgf

@@ -18,7 +18,7 @@ OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_UN
 RK1, RK2, RK3 = 1, 2, 3
 TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
-PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
+PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
              solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
              divergence=12)
@@ -32,7 +32,8 @@ class SceneDesc(C.Structure):
                 ("tol_mode", C.c_int32), ("tolerance", C.c_double),
                 ("max_iterations", C.c_int64), ("precision", C.c_int32),
                 ("preconditioner", C.c_int32), ("extrapolation_layers", C.c_int32),
-                ("max_refinements", C.c_int32), ("inner_tolerance", C.c_double)]
+                ("max_refinements", C.c_int32), ("inner_tolerance", C.c_double),
+                ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float)]
 
 
 class StepStats(C.Structure):
@@ -89,7 +90,7 @@ _SIGS = {
     "mgf_step": [_VP, C.c_int32, C.POINTER(StepStats)],
     "mgf_scene_sync": [_VP],
     "mgf_scene_set_timing": [_VP, C.c_int32],
-    "mgf_scene_time_cg_kernels": [_VP, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    "mgf_scene_time_cg_kernels": [_VP, C.c_int32, C.POINTER(C.c_double)],
     "mgf_device_alloc": [_VP, C.c_int64, C.POINTER(C.c_void_p)],
     "mgf_device_free": [_VP, _VP],
     "mgf_memcpy_h2d": [_VP, _VP, _VP, C.c_int64],
@@ -199,7 +200,8 @@ def desc_defaults(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0):
 
 def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode=TOL_ABS_SQRT_NORM,
               tol=1e-3, max_it=0, precision=PRECISION_PLAIN, layers=1000, gravity=(0, -9.8, 0),
-              force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3):
+              force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3,
+              precond=PRECOND_IDENTITY, mg_sweeps=2, mg_omega=0.8):
     d = desc_defaults(res, origin, spacing, dt)
     d.rk_order = rk
     d.tol_mode = tol_mode
@@ -212,6 +214,9 @@ def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode
     d.density = density
     d.max_refinements = max_refinements
     d.inner_tolerance = inner_tol
+    d.preconditioner = precond
+    d.mg_sweeps = mg_sweeps
+    d.mg_omega = mg_omega
     return d
 
 
@@ -298,10 +303,11 @@ class Scene:
         check(lib().mgf_scene_sync(self.h), "mgf_scene_sync")
 
     def time_cg_kernels(self, iters=50):
-        a, b = C.c_double(), C.c_double()
-        check(lib().mgf_scene_time_cg_kernels(self.h, iters, C.byref(a), C.byref(b)),
-              "mgf_scene_time_cg_kernels")
-        return a.value, b.value
+        """Mean ms of the p/Ap kernel, the x/r update kernel and the
+        preconditioner application (0 with the identity)."""
+        ms = (C.c_double * 3)()
+        check(lib().mgf_scene_time_cg_kernels(self.h, iters, ms), "mgf_scene_time_cg_kernels")
+        return ms[0], ms[1], ms[2]
 
     def set_timing(self, on):
         check(lib().mgf_scene_set_timing(self.h, 1 if on else 0), "mgf_scene_set_timing")

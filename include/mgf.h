@@ -70,9 +70,16 @@ typedef int32_t mgf_status;
 #define MGF_PRECISION_MIXED 1
 
 /* Preconditioner.  IDENTITY is the reference's (MPhysics/PressureSolver.cpp:
- * 12-15); JACOBI uses the inverse diagonal recomputed from the markers. */
-#define MGF_PRECOND_IDENTITY 0
-#define MGF_PRECOND_JACOBI   1
+ * 12-15) and the only one of PLAIN precision.  MIXED precision also offers
+ * JACOBI (the inverse diagonal recomputed from the markers) and MULTIGRID: one
+ * geometric V-cycle per iteration (2x2x2 cell coarsening down to <= 4 cells per
+ * axis, damped Jacobi smoothing with mg_sweeps sweeps before and after the
+ * coarse correction, restriction = average over the fluid children,
+ * prolongation = injection), a symmetric positive definite operator, so the
+ * iteration stays a preconditioned CG on the same system and tolerance. */
+#define MGF_PRECOND_IDENTITY  0
+#define MGF_PRECOND_JACOBI    1
+#define MGF_PRECOND_MULTIGRID 2
 
 /* Field identifiers for upload/download. */
 #define MGF_FIELD_VEL_X        0  /* (nx+1)*ny*nz  face-centred velocity      */
@@ -113,6 +120,9 @@ typedef struct mgf_scene_desc {
     int32_t max_refinements;   /* MIXED: outer sweeps cap (<= 0: 8)          */
     double  inner_tolerance;   /* MIXED: relative tolerance of one inner CG
                                   (<= 0: 1e-3)                                 */
+    int32_t mg_sweeps;         /* MULTIGRID: Jacobi sweeps before and after the
+                                  coarse correction (<= 0: 2; at most 8)       */
+    float   mg_omega;          /* MULTIGRID: Jacobi damping (<= 0: 0.8; < 2)  */
 } mgf_scene_desc;
 
 /* Per-call statistics (summed over the steps of one mgf_step call). */
@@ -207,13 +217,14 @@ MGF_API mgf_status mgf_scene_sync(mgf_scene scene);
 MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
 
 /* Measurement hook (no counterpart in the reference): runs `iters` fused CG
- * iterations of the MIXED solver on the scene's last pressure system (markers
- * and b of the last solve) with no stopping test, each kernel bracketed by HIP
- * events on the scene's stream, and returns the mean duration in ms of the
- * p/Ap kernel and of the x/r update kernel.  The pressure and CG work vectors
- * are overwritten; call it after the steps being measured. */
-MGF_API mgf_status mgf_scene_time_cg_kernels(mgf_scene scene, int32_t iters,
-        double* ms_p_ap, double* ms_update);
+ * iterations of the MIXED solver, with the scene's preconditioner, on the
+ * scene's last pressure system (markers and b of the last solve) with no
+ * stopping test, the p/Ap kernel, the x/r update kernel and the whole
+ * preconditioner application each bracketed by HIP events on the scene's
+ * stream.  ms[0], ms[1], ms[2] receive their mean durations in ms (ms[2] = 0
+ * with the identity).  The pressure and CG work vectors are overwritten;
+ * call it after the steps being measured. */
+MGF_API mgf_status mgf_scene_time_cg_kernels(mgf_scene scene, int32_t iters, double* ms);
 
 /* ---- operator entry points on caller device memory ---------------------- */
 /* These run one reference operator on device buffers the caller owns (for

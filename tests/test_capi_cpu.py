@@ -38,6 +38,8 @@ def test_desc_defaults_are_the_reference_defaults():
     assert np.float32(d.gravity[1]) == np.float32(-9.8)     # ExternalForcesSolver.h:25
     assert d.density == 1.0                                 # PressureSolver.h:61
     assert d.precision == mgf.PRECISION_PLAIN
+    assert d.preconditioner == mgf.PRECOND_IDENTITY                  # PressureSolver.cpp:12-15
+    assert d.mg_sweeps == 2 and np.float32(d.mg_omega) == np.float32(0.8)
 
 
 def test_status_strings():

@@ -332,8 +332,12 @@ def test_reference_integration_goldens(ctx, cases, precision):
 
 # ---------------------------------------------------------------------------
 # Whole steps on the dam-break scene (64^3, dx = 1/64, dt = 2e-3, RK1).
-def dam_desc_kw(precision):
-    return dict(rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=precision)
+def dam_desc_kw(precision, precond=mgf.PRECOND_IDENTITY):
+    return dict(rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=precision, precond=precond)
+
+
+PRECONDS = [pytest.param(mgf.PRECOND_IDENTITY, id="identity"), pytest.param(mgf.PRECOND_JACOBI, id="jacobi"),
+            pytest.param(mgf.PRECOND_MULTIGRID, id="multigrid")]
 
 
 def sync_state(s, o):
@@ -405,11 +409,12 @@ def test_dam_break_64_plain_matches_fp32_oracle(ctx, orc):
     s.close(); o.close()
 
 
-def test_dam_break_64_mixed(ctx, orc):
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_dam_break_64_mixed(ctx, orc, precond):
     """Mixed precision: true FP64 residual <= 1e-6; velocity within 1e-4
     relative L2 of both the FP32 and the FP64 oracle trajectories."""
     spec = scenes.dam_break(64)
-    s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED))
+    s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED, precond))
     o32 = scenes.build_oracle_scene(orc, spec)
     o64 = scenes.build_oracle_scene(orc, spec, f64=True)
     p = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-6)
@@ -426,11 +431,12 @@ def test_dam_break_64_mixed(ctx, orc):
     s.close(); o32.close(); o64.close()
 
 
-def test_single_step_mixed_pressure_vs_fp64_oracle(ctx, orc):
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_single_step_mixed_pressure_vs_fp64_oracle(ctx, orc, precond):
     """From one identical state: the mixed solve's pressure agrees with the
-    FP64 oracle's to the 1e-6 residual, and better than the plain FP32 CG."""
+    FP64 oracle's to the 1e-6 residual."""
     spec = scenes.obstacles(40, seed=1)
-    s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED))
+    s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED, precond))
     o64 = scenes.build_oracle_scene(orc, spec, f64=True)
     p64 = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-13)
     s.update_boundaries(); s.advect(); s.apply_forces()
@@ -445,11 +451,12 @@ def test_single_step_mixed_pressure_vs_fp64_oracle(ctx, orc):
     s.close(); o64.close()
 
 
-def test_determinism(ctx):
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_determinism(ctx, precond):
     spec = scenes.obstacles(48, seed=3)
     outs = []
     for _ in range(2):
-        s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED))
+        s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED, precond))
         s.step(2)
         outs.append([s.download(f) for f in (0, 1, 2, 3)])
         s.close()
@@ -487,9 +494,90 @@ def test_ragged_and_tiny_grids(ctx, orc, res):
     s.close(); o.close()
 
 
-def test_no_fluid_and_zero_rhs(ctx):
+def ragged_scene(ctx, orc, res, psolid, seed, **kw):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(res))
+    solid = np.where(rng.random(n) < psolid, -1.0, 1.0).astype(np.float32)
+    fluid = np.where(rng.random(n) < 0.7, -1.0, 1.0).astype(np.float32)
+    u, v, w = rand_vel(rng, res, 0.3)
+    d = mgf.make_desc(res, (0, 0, 0), (0.1, 0.2, 0.3), 0.01, rk=2, tol_mode=mgf.TOL_RELATIVE, tol=1e-6,
+                      precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID, **kw)
+    s = mgf.Scene(ctx, d)
+    o = orc.Scene(res, (0, 0, 0), (0.1, 0.2, 0.3), f64=True)
+    for sc in (s, o):
+        sc.add_solid(solid)
+        sc.upload(3, fluid)
+        sc.upload(0, u); sc.upload(1, v); sc.upload(2, w)
+    return s, o
+
+
+@pytest.mark.parametrize("res", [(5, 7, 3), (1, 9, 4), (17, 2, 1), (33, 9, 70)])
+def test_multigrid_ragged_grids(ctx, orc, res):
+    """Multigrid on grids whose axes are odd, 1 or 2 cells long (axes of one
+    cell are not coarsened), 10% random solids: the FP64 residual meets 1e-6
+    and the projected velocity matches the FP64 oracle's."""
+    s, o = ragged_scene(ctx, orc, res, 0.1, 7 + sum(res), max_it=10000)
+    p = orc.params(0.01, rk=2, tol_mode=1, tol=1e-13, max_it=100000)
+    st = s.step(1)
+    ost = o.step(p, 1)
+    assert st["converged"] == 1 and st["residual_rel"] <= 1e-6, st
+    assert ost["converged"] == 1
+    for fid in (0, 1, 2):
+        g_, e_ = s.download(fid), o.download(fid, f64=True)
+        assert np.array_equal(g_ == FLT_MAX, np.float32(e_) == FLT_MAX)
+        assert rel_l2(g_, e_, g_ != FLT_MAX) <= 1e-4
+    s.close(); o.close()
+
+
+def test_multigrid_inconsistent_system_reports_not_converged(ctx, orc):
+    """A 1x1x1 fluid cell has an all-zero matrix row (every neighbour outside
+    the grid) and a non-zero right-hand side: no solver can meet the
+    tolerance.  The step reports NOT_CONVERGED within its iteration cap and
+    leaves finite velocities, as the FP64 oracle's CG does not converge
+    either."""
+    s, o = ragged_scene(ctx, orc, (1, 1, 1), 0.0, 11, max_it=50)
+    p = orc.params(0.01, rk=2, tol_mode=1, tol=1e-13, max_it=1000)
+    with pytest.raises(mgf.MgfError) as ei:
+        s.step(1)
+    assert ei.value.status == mgf.ERR_NOT_CONVERGED
+    ost = o.step(p, 1)
+    assert ost["converged"] == 0
+    for fid in (0, 1, 2):
+        assert np.all(np.isfinite(s.download(fid)))
+    s.close(); o.close()
+
+
+def test_multigrid_cuts_iterations(ctx):
+    """The V-cycle preconditioner leaves the system and tolerance unchanged
+    and needs far fewer iterations than the identity on an obstacle scene."""
+    spec = scenes.obstacles(96, seed=2)
+    its = {}
+    for pc in (mgf.PRECOND_IDENTITY, mgf.PRECOND_MULTIGRID):
+        s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED, pc))
+        st = s.step(1)
+        assert st["converged"] == 1 and st["residual_rel"] <= 1e-6, st
+        its[pc] = st["cg_iterations"]
+        s.close()
+    assert its[mgf.PRECOND_MULTIGRID] * 8 <= its[mgf.PRECOND_IDENTITY], its
+
+
+def test_preconditioner_options_are_checked(ctx):
+    d = mgf.make_desc((8, 8, 8), precision=mgf.PRECISION_PLAIN, precond=mgf.PRECOND_MULTIGRID)
+    with pytest.raises(mgf.MgfError):
+        mgf.Scene(ctx, d)           # PLAIN is the reference's identity CG only
+    d = mgf.make_desc((8, 8, 8), precision=mgf.PRECISION_MIXED, precond=7)
+    with pytest.raises(mgf.MgfError):
+        mgf.Scene(ctx, d)
+    d = mgf.make_desc((8, 8, 8), precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID, mg_omega=2.5)
+    with pytest.raises(mgf.MgfError):
+        mgf.Scene(ctx, d)
+
+
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_no_fluid_and_zero_rhs(ctx, precond):
     res = (8, 8, 8)
-    d = mgf.make_desc(res, dt=0.01, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=mgf.PRECISION_MIXED)
+    d = mgf.make_desc(res, dt=0.01, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=mgf.PRECISION_MIXED,
+                      precond=precond)
     s = mgf.Scene(ctx, d)
     s.upload(3, np.full(512, 1.0, np.float32))   # all air
     st = s.step(1)
@@ -515,10 +603,11 @@ def test_invalid_arguments(ctx):
 
 # ---------------------------------------------------------------------------
 # Full size (256^3 with obstacles): size-independent properties.
-def test_obstacles_256_one_step_properties(ctx):
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_obstacles_256_one_step_properties(ctx, precond):
     spec = scenes.obstacles(256, seed=0)
     s = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6,
-                               precision=mgf.PRECISION_MIXED)
+                               precision=mgf.PRECISION_MIXED, precond=precond)
     s.update_boundaries()
     s.advect()
     s.apply_forces()
