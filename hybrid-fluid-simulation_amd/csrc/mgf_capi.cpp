@@ -278,6 +278,9 @@ mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
 // ---------------------------------------------------------------------------
 // Preconditioners of the MIXED solver.
 constexpr int kCoarsestSweeps = 8;
+constexpr size_t kCoarseCells = 5000;    // levels this small run in one work-group
+constexpr int kCoarseLevelsMax = 8;
+constexpr size_t kCoarseLdsMax = 65536;
 
 mgf_status ensure_precond(mgf_scene s) {
     if (!s->z) {
@@ -302,6 +305,21 @@ mgf_status ensure_precond(mgf_scene s) {
         nx = L.d.nx; ny = L.d.ny; nz = L.d.nz;
     }
     s->mg_nx = nx; s->mg_ny = ny; s->mg_nz = nz;
+    // the first level (below the pressure grid) from which the rest of the
+    // hierarchy fits one work-group's LDS
+    const int L = 1 + int(s->mg.size());
+    s->mg_l0 = L;
+    const size_t lds_cap = std::min<size_t>(kCoarseLdsMax, s->ctx->lds_per_block);
+    for (int l = 1; l < L; ++l) {
+        const MgDims& d = s->mg[size_t(l - 1)].d;
+        if (size_t(d.nx) * d.ny * d.nz > kCoarseCells || L - l > kCoarseLevelsMax) continue;
+        std::vector<MgDims> dims;
+        for (int k = l; k < L; ++k) dims.push_back(s->mg[size_t(k - 1)].d);
+        if (mg_coarse_lds_bytes(dims.data(), int(dims.size())) <= lds_cap) {
+            s->mg_l0 = l;
+            break;
+        }
+    }
     return MGF_OK;
 }
 
@@ -336,8 +354,8 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         *zout = s->z;
         return MGF_OK;
     }
-    const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 2;
-    const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.8f;
+    const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
+    const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.9f;
     const int L = 1 + int(s->mg.size());
     struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; };
     Lv lv[17];
@@ -346,28 +364,51 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         auto& m = s->mg[size_t(l - 1)];
         lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t};
     }
-    // descent: pre-smoothing from zero, restriction of the residual
-    for (int l = 0; l < L; ++l) {
+    // levels [l0, L) run in one work-group
+    const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
+    // descent: pre-smoothing from zero (the first two sweeps in one pass),
+    // restriction of the residual
+    for (int l = 0; l < l0; ++l) {
         Lv& v = lv[l];
         const bool coarsest = l == L - 1;
         const int sweeps = coarsest ? kCoarsestSweeps : nu;
-        for (int k = 0; k < sweeps; ++k) {
-            const int rz = (l == 0 && coarsest && k == sweeps - 1) ? 1 : 0;
-            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, k == 0 ? v.b : v.x, k == 0 ? v.x : v.t, omega,
-                                     s->scal, s->red, k == 0 ? 1 : 0, rz));
-            if (k > 0) std::swap(v.x, v.t);
+        auto rz_at = [&](int k) { return (l == 0 && coarsest && k == sweeps - 1) ? 1 : 0; };
+        int k;
+        if (sweeps >= 2) {
+            MGF_HIP(launch_mg_smooth2_first(st, v.d, v.codes, v.b, v.x, omega, s->scal, s->red, rz_at(1)));
+            k = 2;
+        } else {
+            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.b, v.x, omega, s->scal, s->red, 1, rz_at(0)));
+            k = 1;
+        }
+        for (; k < sweeps; ++k) {
+            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k)));
+            std::swap(v.x, v.t);
         }
         if (!coarsest)
             MGF_HIP(launch_mg_restrict(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
                                        const_cast<float*>(lv[l + 1].b), s->scal));
     }
-    // ascent: prolongation (injection) and post-smoothing
-    for (int l = L - 2; l >= 0; --l) {
+    if (l0 < L) {
+        MgDims dims[kCoarseLevelsMax];
+        const uint16_t* codes[kCoarseLevelsMax];
+        for (int l = l0; l < L; ++l) {
+            dims[l - l0] = lv[l].d;
+            codes[l - l0] = lv[l].codes;
+        }
+        MGF_HIP(launch_mg_coarse_cycle(st, dims, codes, L - l0, lv[l0].b, lv[l0].x, nu, kCoarsestSweeps, omega,
+                                       s->scal));
+    }
+    // ascent: the first post-sweep applies the prolongation (injection) on
+    // the fly, then the remaining sweeps
+    for (int l = std::min(L - 2, l0 - 1); l >= 0; --l) {
         Lv& v = lv[l];
-        MGF_HIP(launch_mg_prolong(st, v.d, lv[l + 1].d, v.codes, lv[l + 1].x, v.x, s->scal));
-        for (int k = 0; k < nu; ++k) {
-            const int rz = (l == 0 && k == nu - 1) ? 1 : 0;
-            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz));
+        auto rz_at = [&](int k) { return (l == 0 && k == nu - 1) ? 1 : 0; };
+        MGF_HIP(launch_mg_smooth_prolong(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
+                                         s->red, rz_at(0)));
+        std::swap(v.x, v.t);
+        for (int k = 1; k < nu; ++k) {
+            MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k)));
             std::swap(v.x, v.t);
         }
     }
@@ -502,7 +543,6 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     uint32_t* maxd = cursor + (K + 2);
     MGF_HIP(launch_ext_distance(st, s->g, u, v, w, s->layer));
     MGF_HIP(launch_fill(st, reinterpret_cast<float*>(hist), size_t(cap) + 2, 0.0f));
-    MGF_HIP(launch_fill(st, reinterpret_cast<float*>(maxd), 1, 0.0f));
     MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, hist, offs, cursor, maxd, s->list));
     // offsets of layers 1..cap+1, then the largest distance present
     MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
@@ -583,7 +623,10 @@ mgf_status mgf_init(int32_t device, mgf_context* out) {
         return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->cu_count = prop.multiProcessorCount;
+        c->lds_per_block = prop.sharedMemPerBlock;
+    }
     *out = c;
     return MGF_OK;
 }
@@ -618,8 +661,8 @@ mgf_status mgf_scene_desc_defaults(mgf_scene_desc* d, int32_t nx, int32_t ny, in
     d->extrapolation_layers = 1000;
     d->max_refinements = 8;
     d->inner_tolerance = 1e-3;
-    d->mg_sweeps = 2;
-    d->mg_omega = 0.8f;
+    d->mg_sweeps = 3;
+    d->mg_omega = 0.9f;
     return MGF_OK;
 }
 

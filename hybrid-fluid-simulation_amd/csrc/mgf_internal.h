@@ -62,6 +62,7 @@ struct mgf_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
+    size_t lds_per_block = 65536;
 };
 
 struct mgf_scene_s {
@@ -101,6 +102,7 @@ struct mgf_scene_s {
     float* zt = nullptr;           // Jacobi ping-pong partner of z
     std::vector<mgf::MgLevelBuf> mg;   // coarse levels 1..L-1
     int mg_nx = -1, mg_ny = -1, mg_nz = -1;
+    int mg_l0 = 0;                 // levels [mg_l0, L) run in one work-group
     bool x_is_64 = false;          // which pressure the last solve produced
 
     // extrapolation scratch
@@ -206,11 +208,22 @@ hipError_t launch_mg_coarsen(hipStream_t s, const MgDims& f, const MgDims& c, co
                              int fluid_first);
 hipError_t launch_mg_smooth(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
                             float* xo, float omega, CgScalars* sc, ReduceBuf red, int first, int with_rz);
+// The first two sweeps from x = 0 fused (x1 = omega D^-1 b is pointwise).
+hipError_t launch_mg_smooth2_first(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, float* xo,
+                                   float omega, CgScalars* sc, ReduceBuf red, int with_rz);
+// A sweep on xi + (injection of the coarse xc) without storing the sum.
+hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code,
+                                    const float* b, const float* xi, const float* xc, float* xo, float omega,
+                                    CgScalars* sc, ReduceBuf red, int with_rz);
+// The V-cycle of levels[0..nlev) in one work-group with the iterates in LDS:
+// b0 = right-hand side of levels[0] (global), x0 = its result (global).
+size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev);
+hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
+                                  const float* b0, float* x0, int nu, int last_sweeps, float omega,
+                                  const CgScalars* sc);
 hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
                               const float* bf, const float* xf, const uint16_t* codec, float* bc,
                               const CgScalars* sc);
-hipError_t launch_mg_prolong(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
-                             const float* xc, float* xf, const CgScalars* sc);
 // ---- extrapolation (mgf_kernels_ext.hip) ------------------------------------
 // Largest L1 distance between two faces of one component grid (+ margin).
 int32_t ext_max_distance(const GridDev& g);
@@ -220,7 +233,7 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
                                int32_t* dist);
 // Counting sort of the faces with 1 <= dist <= cap: hist (zeroed by the
 // caller) counts, offs[L] = first slot of distance L in list (offs[cap+1] =
-// total), *maxd (zeroed by the caller) = largest distance present.
+// total), *maxd = largest distance present (0 if none).
 hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, uint32_t* hist,
                            uint32_t* offs, uint32_t* cursor, uint32_t* maxd, uint32_t* list);
 // Layer L: list[begin, begin+count) take the mean of their neighbours below L.

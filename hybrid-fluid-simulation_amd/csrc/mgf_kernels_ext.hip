@@ -13,7 +13,8 @@
 //      min-plus scan per row, along y and z a forward and a backward sweep per
 //      line (d = min(d, neighbour + 1));
 //   2. counting sort of the faces with 1 <= d <= N-1 by d (LDS histogram per
-//      block, one global atomic per block and distance);
+//      block with one LDS atomic per wave and distinct distance, one global
+//      atomic per block and distance);
 //   3. one gather kernel per layer, over that layer's faces only, in layer
 //      order: the neighbours in layers below L are final when layer L runs.
 //
@@ -150,40 +151,56 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
 constexpr int kLdsBins = 2048;
 constexpr int kSortCpt = 8;   // faces per work-item
 
+// Counts the active lanes of a wave by key: for each distinct key, the lowest
+// active lane holding it adds the number of active lanes holding it to
+// ctr_of_key[key] with one atomic.  Each active lane gets that counter's old
+// value plus its rank among the lanes holding its key (lane order).
+__device__ __forceinline__ uint32_t wave_count(uint32_t* ctr_of_key, int32_t key, bool active) {
+    uint32_t out = 0;
+    uint64_t todo = __ballot(active);
+    const int lane = threadIdx.x & 63;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int32_t k = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(active && key == k);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ctr_of_key[k], uint32_t(__popcll(same)));
+        base = __shfl(base, leader, 64);
+        if (active && key == k) out = base + uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
+        todo &= ~same;
+    }
+    return out;
+}
+
 __global__ void __launch_bounds__(kBlock) k_ext_count(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
-                                                      uint32_t* __restrict__ hist, uint32_t* __restrict__ maxd) {
+                                                      uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[kLdsBins];
-    __shared__ int32_t bmax;
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
-    if (threadIdx.x == 0) bmax = 0;
     __syncthreads();
     const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
-    int32_t mymax = 0;
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
         const int32_t dv = i < total ? dist[i] : 0;
-        if (dv >= 1 && dv <= cap) {
-            if (dv < kLdsBins) atomicAdd(&h[dv], 1u);
-            else atomicAdd(&hist[dv], 1u);
-            mymax = max(mymax, dv);
-        }
+        const bool in = dv >= 1 && dv <= cap;
+        (void)wave_count(h, dv < kLdsBins ? dv : 0, in && dv < kLdsBins);
+        if (in && dv >= kLdsBins) atomicAdd(&hist[dv], 1u);
     }
-    atomicMax(&bmax, mymax);
     __syncthreads();
     for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
         if (h[b]) atomicAdd(&hist[b], h[b]);
-    if (threadIdx.x == 0 && bmax > 0) atomicMax(maxd, uint32_t(bmax));
 }
 
 // Exclusive scan of hist[1..cap] into offs[1..cap+1] (offs[1] = 0); cursor
-// cleared.  One block.
+// cleared; *maxd = the largest distance with a non-zero count.  One block.
 __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ hist, int32_t cap,
-                                                   uint32_t* __restrict__ offs, uint32_t* __restrict__ cursor) {
+                                                   uint32_t* __restrict__ offs, uint32_t* __restrict__ cursor,
+                                                   uint32_t* __restrict__ maxd) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
-    if (threadIdx.x == 0) carry = 0;
+    __shared__ int32_t top;
+    if (threadIdx.x == 0) { carry = 0; top = 0; }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int32_t b0 = 1; b0 <= cap; b0 += 1024) {
@@ -202,12 +219,16 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
         if (b <= cap) {
             offs[b] = before + incl - v;
             cursor[b] = 0;
+            if (v) atomicMax(&top, b);
         }
         __syncthreads();
         if (threadIdx.x == 1023) carry = before + incl;
         __syncthreads();
     }
-    if (threadIdx.x == 0) offs[cap + 1] = carry;
+    if (threadIdx.x == 0) {
+        offs[cap + 1] = carry;
+        *maxd = uint32_t(top);
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_ext_scatter(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
@@ -224,11 +245,9 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(const int32_t* __restric
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
         dv[k] = i < total ? dist[i] : 0;
-        rank[k] = 0;
-        if (dv[k] >= 1 && dv[k] <= cap) {
-            if (dv[k] < kLdsBins) rank[k] = atomicAdd(&h[dv[k]], 1u);
-            else rank[k] = atomicAdd(&cursor[dv[k]], 1u);
-        }
+        const bool in = dv[k] >= 1 && dv[k] <= cap;
+        rank[k] = wave_count(h, dv[k] < kLdsBins ? dv[k] : 0, in && dv[k] < kLdsBins);
+        if (in && dv[k] >= kLdsBins) rank[k] = atomicAdd(&cursor[dv[k]], 1u);
     }
     __syncthreads();
     // reserve each bin's range: the bin's slot now holds its base
@@ -325,10 +344,10 @@ hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist,
     const uint32_t total = F.off[3];
     size_t G = (size_t(total) + size_t(kBlock) * kSortCpt - 1) / (size_t(kBlock) * kSortCpt);
     G = (G + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(k_ext_count, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, hist, maxd);
+    hipLaunchKernelGGL(k_ext_count, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, hist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, hist, cap, offs, cursor);
+    hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, hist, cap, offs, cursor, maxd);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, offs, cursor, list);

@@ -341,6 +341,93 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth(Flat F, MgLevel L, const u
     if (RZ) block_partial(acc, part);
 }
 
+// The first two sweeps from x = 0 in one pass: x1 = omega D^-1 b is
+// pointwise, so x2 = x1 + omega D^-1 (b - A x1) needs only b and the codes at
+// the seven points.
+template <bool RZ>
+__global__ void __launch_bounds__(kBlock) k_mg_smooth2_first(Flat F, MgLevel L, const uint16_t* __restrict__ code,
+                                                             const float* __restrict__ b, float* __restrict__ xo,
+                                                             float omega, const CgScalars* __restrict__ sc,
+                                                             double* __restrict__ part) {
+    if (sc->done) return;
+    const uint32_t c = xcd_block() * kBlock + threadIdx.x;
+    const bool ok = c < F.n;
+    const uint32_t cc = ok ? c : F.n - 1;
+    uint32_t j[6];
+    nbrs(cc, F, j);
+    const uint16_t cd = code[cc];
+    const float bc = b[cc];
+    uint16_t cn[6];
+    float bn[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        cn[m] = code[j[m]];
+        bn[m] = b[j[m]];
+    }
+    const float d = code_diag(cd, L.sx, L.sy, L.sz);
+    const bool live = (cd & kCodeFluid) && d != 0.0f;
+    const float wd = omega * __builtin_amdgcn_rcpf(d);
+    const float x1c = live ? wd * bc : 0.0f;
+    float x1[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const float dn = code_diag(cn[m], L.sx, L.sy, L.sz);
+        const bool ln = (cn[m] & kCodeFluid) && dn != 0.0f;
+        x1[m] = ln ? omega * __builtin_amdgcn_rcpf(dn) * bn[m] : 0.0f;
+    }
+    const float ax = arow_f(cd, x1c, x1, L.sx, L.sy, L.sz);
+    const float out = live ? x1c + wd * (bc - ax) : 0.0f;
+    if (ok) xo[c] = out;
+    if (RZ) block_partial(ok ? double(bc) * double(out) : 0.0, part);
+}
+
+// The first sweep after the coarse correction, with the prolongation by
+// injection applied on the fly: x_in = x + xc[parent] at the seven points
+// (only coupled, hence fluid, neighbours are used), xo = x_in + omega D^-1
+// (b - A x_in); x_in is not stored.  h[xyz] = 1 where the axis was halved.
+template <bool RZ>
+__global__ void __launch_bounds__(kBlock) k_mg_smooth_prolong(Flat F, MgLevel L, Flat FC, int hx, int hy, int hz,
+                                                              double rnx, double rplane,
+                                                              const uint16_t* __restrict__ code,
+                                                              const float* __restrict__ b,
+                                                              const float* __restrict__ xi,
+                                                              const float* __restrict__ xc, float* __restrict__ xo,
+                                                              float omega, const CgScalars* __restrict__ sc,
+                                                              double* __restrict__ part) {
+    if (sc->done) return;
+    const uint32_t c = xcd_block() * kBlock + threadIdx.x;
+    const bool ok = c < F.n;
+    const uint32_t cc = ok ? c : F.n - 1;
+    const uint32_t z = udiv(cc, F.sz, rplane);
+    const uint32_t rem = cc - z * F.sz;
+    const uint32_t y = udiv(rem, F.sy, rnx);
+    const uint32_t x = rem - y * F.sy;
+    const uint32_t nx = uint32_t(L.nx), ny = uint32_t(L.ny), nz = uint32_t(L.nz);
+    const uint32_t X = x >> hx, Y = y >> hy, Z = z >> hz;
+    const uint32_t Xm = (x > 0 ? x - 1 : 0) >> hx, Xp = (x + 1 < nx ? x + 1 : x) >> hx;
+    const uint32_t Ym = (y > 0 ? y - 1 : 0) >> hy, Yp = (y + 1 < ny ? y + 1 : y) >> hy;
+    const uint32_t Zm = (z > 0 ? z - 1 : 0) >> hz, Zp = (z + 1 < nz ? z + 1 : z) >> hz;
+    const uint32_t pz = Z * FC.sz, py = Y * FC.sy;
+    const uint32_t pc = pz + py + X;
+    const uint32_t p[6] = {pz + py + Xm, pz + py + Xp, pz + Ym * FC.sy + X,
+                           pz + Yp * FC.sy + X, Zm * FC.sz + py + X, Zp * FC.sz + py + X};
+    uint32_t j[6];
+    nbrs(cc, F, j);
+    const uint16_t cd = code[cc];
+    const float bc = b[cc];
+    const float vc = xi[cc] + xc[pc];
+    float v[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) v[m] = xi[j[m]] + xc[p[m]];
+    const float d = code_diag(cd, L.sx, L.sy, L.sz);
+    const bool live = (cd & kCodeFluid) && d != 0.0f;
+    const float wd = omega * __builtin_amdgcn_rcpf(d);
+    const float ax = arow_f(cd, vc, v, L.sx, L.sy, L.sz);
+    const float out = live ? vc + wd * (bc - ax) : 0.0f;
+    if (ok) xo[c] = out;
+    if (RZ) block_partial(ok ? double(bc) * double(out) : 0.0, part);
+}
+
 // Restriction of the residual, one work-item per coarse cell:
 // bc = weight * sum over fluid children of (b - A x).
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Flat FF, MgLevel Fd,
@@ -384,23 +471,140 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Fla
     if (ok) bc[c] = (codec[c] & kCodeFluid) ? sum * weight : 0.0f;
 }
 
-// Prolongation by injection: xf += xc[parent] on fluid fine cells.
-__global__ void __launch_bounds__(kBlock) k_mg_prolong(Flat FF, MgLevel Fd, Flat FC,
-                                                       const uint16_t* __restrict__ codef,
-                                                       const float* __restrict__ xc, float* __restrict__ xf,
-                                                       double rnx, double rplane, const CgScalars* __restrict__ sc) {
+// ---------------------------------------------------------------------------
+// The V-cycle on the smallest levels in one work-group, iterates in LDS.
+// Level 0 of the argument list takes its right-hand side from global memory
+// (written by the restriction of the level above) and writes its result back
+// (read by the prolongation of the level above); deeper levels keep b, x and
+// the Jacobi partner in LDS.  The arithmetic is the multi-kernel cycle's.
+constexpr int kCoarseThreads = 1024;
+constexpr int kCoarseMaxLevels = 8;
+
+struct CoarseLevel {
+    int nx, ny, nz;
+    float sx, sy, sz;
+    const uint16_t* code;
+};
+
+struct CoarseArgs {
+    int nlev, nu, last_sweeps;
+    float omega;
+    const float* b0;
+    float* x0;
+    CoarseLevel lv[kCoarseMaxLevels];
+};
+
+__device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* b, const float* xi, float* xo,
+                                          float omega, bool first) {
+    const uint32_t n = uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz);
+    const Flat F{n, uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
+    for (uint32_t c = threadIdx.x; c < n; c += kCoarseThreads) {
+        const uint16_t cd = L.code[c];
+        const float d = code_diag(cd, L.sx, L.sy, L.sz);
+        const bool live = (cd & kCodeFluid) && d != 0.0f;
+        const float wd = omega * __builtin_amdgcn_rcpf(d);
+        const float bc = b[c];
+        float out;
+        if (first) {
+            out = live ? wd * bc : 0.0f;
+        } else {
+            uint32_t j[6];
+            nbrs(c, F, j);
+            float v[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) v[m] = xi[j[m]];
+            const float vc = xi[c];
+            out = live ? vc + wd * (bc - arow_f(cd, vc, v, L.sx, L.sy, L.sz)) : 0.0f;
+        }
+        xo[c] = out;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A, const CgScalars* __restrict__ sc) {
     if (sc->done) return;
-    const uint32_t c = xcd_block() * kBlock + threadIdx.x;
-    if (c >= FF.n) return;
-    const uint32_t z = udiv(c, FF.sz, rplane);
-    const uint32_t rem = c - z * FF.sz;
-    const uint32_t y = udiv(rem, FF.sy, rnx);
-    const uint32_t x = rem - y * FF.sy;
-    const uint32_t X = Fd.nx > 1 ? x >> 1 : x, Y = Fd.ny > 1 ? y >> 1 : y, Z = Fd.nz > 1 ? z >> 1 : z;
-    const float add = xc[Z * FC.sz + Y * FC.sy + X];
-    const uint16_t cd = codef[c];
-    const float x0 = xf[c];
-    if (cd & kCodeFluid) xf[c] = x0 + add;
+    extern __shared__ float lds[];
+    float* xs[kCoarseMaxLevels];
+    float* ts[kCoarseMaxLevels];
+    const float* bs[kCoarseMaxLevels];
+    // carve LDS: level 0: x, t; deeper levels: b, x, t
+    float* q = lds;
+    for (int l = 0; l < A.nlev; ++l) {
+        const uint32_t n = uint32_t(A.lv[l].nx) * uint32_t(A.lv[l].ny) * uint32_t(A.lv[l].nz);
+        if (l == 0) {
+            bs[0] = A.b0;
+        } else {
+            bs[l] = q;
+            q += n;
+        }
+        xs[l] = q; q += n;
+        ts[l] = q; q += n;
+    }
+    // descent
+    for (int l = 0; l < A.nlev; ++l) {
+        const CoarseLevel& L = A.lv[l];
+        const int sweeps = l == A.nlev - 1 ? A.last_sweeps : A.nu;
+        lds_sweep(L, bs[l], xs[l], xs[l], A.omega, true);
+        for (int k = 1; k < sweeps; ++k) {
+            lds_sweep(L, bs[l], xs[l], ts[l], A.omega, false);
+            float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
+        }
+        if (l == A.nlev - 1) break;
+        // restriction of the residual into level l+1
+        const CoarseLevel& C = A.lv[l + 1];
+        const uint32_t nc = uint32_t(C.nx) * uint32_t(C.ny) * uint32_t(C.nz);
+        const int fx = L.nx > 1 ? 2 : 1, fy = L.ny > 1 ? 2 : 1, fz = L.nz > 1 ? 2 : 1;
+        const float weight = 1.0f / float(fx * fy * fz);
+        const Flat FF{uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz), uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
+        float* bc = const_cast<float*>(bs[l + 1]);
+        const float* xf = xs[l];
+        const float* bf = bs[l];
+        for (uint32_t c = threadIdx.x; c < nc; c += kCoarseThreads) {
+            const uint32_t X = c % uint32_t(C.nx), Y = (c / uint32_t(C.nx)) % uint32_t(C.ny),
+                           Z = c / (uint32_t(C.nx) * uint32_t(C.ny));
+            float sum = 0.0f;
+            for (int dz = 0; dz < fz; ++dz)
+                for (int dy = 0; dy < fy; ++dy)
+                    for (int dx = 0; dx < fx; ++dx) {
+                        const uint32_t x = X * fx + dx, y = Y * fy + dy, z = Z * fz + dz;
+                        if (x >= uint32_t(L.nx) || y >= uint32_t(L.ny) || z >= uint32_t(L.nz)) continue;
+                        const uint32_t f = z * FF.sz + y * FF.sy + x;
+                        const uint16_t cd = L.code[f];
+                        if (!(cd & kCodeFluid)) continue;
+                        uint32_t j[6];
+                        nbrs(f, FF, j);
+                        float v[6];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) v[m] = xf[j[m]];
+                        sum += bf[f] - arow_f(cd, xf[f], v, L.sx, L.sy, L.sz);
+                    }
+            bc[c] = (C.code[c] & kCodeFluid) ? sum * weight : 0.0f;
+        }
+        __syncthreads();
+    }
+    // ascent
+    for (int l = A.nlev - 2; l >= 0; --l) {
+        const CoarseLevel& L = A.lv[l];
+        const CoarseLevel& C = A.lv[l + 1];
+        const uint32_t n = uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz);
+        const int hx = L.nx > 1 ? 1 : 0, hy = L.ny > 1 ? 1 : 0, hz = L.nz > 1 ? 1 : 0;
+        const float* xc = xs[l + 1];
+        float* xf = xs[l];
+        for (uint32_t c = threadIdx.x; c < n; c += kCoarseThreads) {
+            const uint32_t x = c % uint32_t(L.nx), y = (c / uint32_t(L.nx)) % uint32_t(L.ny),
+                           z = c / (uint32_t(L.nx) * uint32_t(L.ny));
+            if (L.code[c] & kCodeFluid)
+                xf[c] += xc[((z >> hz) * uint32_t(C.ny) + (y >> hy)) * uint32_t(C.nx) + (x >> hx)];
+        }
+        __syncthreads();
+        for (int k = 0; k < A.nu; ++k) {
+            lds_sweep(L, bs[l], xs[l], ts[l], A.omega, false);
+            float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
+        }
+    }
+    const CoarseLevel& L0 = A.lv[0];
+    const uint32_t n0 = uint32_t(L0.nx) * uint32_t(L0.ny) * uint32_t(L0.nz);
+    for (uint32_t c = threadIdx.x; c < n0; c += kCoarseThreads) A.x0[c] = xs[0][c];
 }
 
 // ===========================================================================
@@ -471,6 +675,65 @@ hipError_t launch_mg_smooth(hipStream_t s, const MgDims& l, const uint16_t* code
     return finish(s, red.partials, G, sc, kFinishRz);
 }
 
+hipError_t launch_mg_smooth2_first(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, float* xo,
+                                   float omega, CgScalars* sc, ReduceBuf red, int with_rz) {
+    const Flat F = make_flat(l.nx, l.ny, l.nz);
+    const uint32_t G = flat_grid(F.n, 1);
+    hipError_t e;
+    if (with_rz) e = LAUNCH(k_mg_smooth2_first<true>, G, F, l, code, b, xo, omega, sc, red.partials);
+    else e = LAUNCH(k_mg_smooth2_first<false>, G, F, l, code, b, xo, omega, sc, red.partials);
+    if (e != hipSuccess || !with_rz) return e;
+    return finish(s, red.partials, G, sc, kFinishRz);
+}
+
+hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code,
+                                    const float* b, const float* xi, const float* xc, float* xo, float omega,
+                                    CgScalars* sc, ReduceBuf red, int with_rz) {
+    const Flat F = make_flat(f.nx, f.ny, f.nz), FC = make_flat(c.nx, c.ny, c.nz);
+    const uint32_t G = flat_grid(F.n, 1);
+    const int hx = f.nx > 1 ? 1 : 0, hy = f.ny > 1 ? 1 : 0, hz = f.nz > 1 ? 1 : 0;
+    const double rnx = 1.0 / double(F.sy), rplane = 1.0 / double(F.sz);
+    hipError_t e;
+    if (with_rz)
+        e = LAUNCH(k_mg_smooth_prolong<true>, G, F, f, FC, hx, hy, hz, rnx, rplane, code, b, xi, xc, xo, omega, sc,
+                   red.partials);
+    else
+        e = LAUNCH(k_mg_smooth_prolong<false>, G, F, f, FC, hx, hy, hz, rnx, rplane, code, b, xi, xc, xo, omega, sc,
+                   red.partials);
+    if (e != hipSuccess || !with_rz) return e;
+    return finish(s, red.partials, G, sc, kFinishRz);
+}
+
+size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev) {
+    size_t floats = 0;
+    for (int l = 0; l < nlev; ++l) {
+        const size_t n = size_t(levels[l].nx) * levels[l].ny * levels[l].nz;
+        floats += (l == 0 ? 2 : 3) * n;
+    }
+    return floats * sizeof(float);
+}
+
+hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
+                                  const float* b0, float* x0, int nu, int last_sweeps, float omega,
+                                  const CgScalars* sc) {
+    if (nlev < 1 || nlev > kCoarseMaxLevels) return hipErrorInvalidValue;
+    CoarseArgs A;
+    A.nlev = nlev;
+    A.nu = nu;
+    A.last_sweeps = last_sweeps;
+    A.omega = omega;
+    A.b0 = b0;
+    A.x0 = x0;
+    for (int l = 0; l < nlev; ++l) {
+        A.lv[l].nx = levels[l].nx; A.lv[l].ny = levels[l].ny; A.lv[l].nz = levels[l].nz;
+        A.lv[l].sx = levels[l].sx; A.lv[l].sy = levels[l].sy; A.lv[l].sz = levels[l].sz;
+        A.lv[l].code = codes[l];
+    }
+    const size_t lds = mg_coarse_lds_bytes(levels, nlev);
+    hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(kCoarseThreads), lds, s, A, sc);
+    return hipGetLastError();
+}
+
 hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
                               const float* bf, const float* xf, const uint16_t* codec, float* bc,
                               const CgScalars* sc) {
@@ -480,11 +743,5 @@ hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, c
                   1.0 / double(FC.sy), 1.0 / double(FC.sz), sc);
 }
 
-hipError_t launch_mg_prolong(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
-                             const float* xc, float* xf, const CgScalars* sc) {
-    const Flat FF = make_flat(f.nx, f.ny, f.nz), FC = make_flat(c.nx, c.ny, c.nz);
-    return LAUNCH(k_mg_prolong, grid_for(FF.n), FF, f, FC, codef, xc, xf, 1.0 / double(FF.sy),
-                  1.0 / double(FF.sz), sc);
-}
 
 }  // namespace mgf

@@ -201,7 +201,7 @@ def desc_defaults(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0):
 def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode=TOL_ABS_SQRT_NORM,
               tol=1e-3, max_it=0, precision=PRECISION_PLAIN, layers=1000, gravity=(0, -9.8, 0),
               force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3,
-              precond=PRECOND_IDENTITY, mg_sweeps=2, mg_omega=0.8):
+              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9):
     d = desc_defaults(res, origin, spacing, dt)
     d.rk_order = rk
     d.tol_mode = tol_mode

@@ -121,8 +121,8 @@ typedef struct mgf_scene_desc {
     double  inner_tolerance;   /* MIXED: relative tolerance of one inner CG
                                   (<= 0: 1e-3)                                 */
     int32_t mg_sweeps;         /* MULTIGRID: Jacobi sweeps before and after the
-                                  coarse correction (<= 0: 2; at most 8)       */
-    float   mg_omega;          /* MULTIGRID: Jacobi damping (<= 0: 0.8; < 2)  */
+                                  coarse correction (<= 0: 3; at most 8)       */
+    float   mg_omega;          /* MULTIGRID: Jacobi damping (<= 0: 0.9; < 2)  */
 } mgf_scene_desc;
 
 /* Per-call statistics (summed over the steps of one mgf_step call). */

@@ -39,7 +39,7 @@ def test_desc_defaults_are_the_reference_defaults():
     assert d.density == 1.0                                 # PressureSolver.h:61
     assert d.precision == mgf.PRECISION_PLAIN
     assert d.preconditioner == mgf.PRECOND_IDENTITY                  # PressureSolver.cpp:12-15
-    assert d.mg_sweeps == 2 and np.float32(d.mg_omega) == np.float32(0.8)
+    assert d.mg_sweeps == 3 and np.float32(d.mg_omega) == np.float32(0.9)
 
 
 def test_status_strings():
