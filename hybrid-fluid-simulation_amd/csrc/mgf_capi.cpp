@@ -519,11 +519,11 @@ mgf_status pressure(mgf_scene s, mgf_step_stats* acc) {
 // Extrapolation: distances, counting sort by distance, one gather kernel per
 // layer; returns the largest non-empty layer (< layers).
 mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
-    const size_t K = size_t(ext_max_distance(s->g)) + 2;
+    const int32_t K = ext_max_distance(s->g);
     TRY(dalloc(s, &s->layer, nface));
     TRY(dalloc(s, &s->list, nface));
-    TRY(dalloc(s, &s->counts, 3 * K + 1));
-    if (hipHostMalloc(reinterpret_cast<void**>(&s->host_offs), sizeof(uint32_t) * (K + 1)) != hipSuccess) {
+    TRY(dalloc(s, &s->counts, ext_counts_size(K)));
+    if (hipHostMalloc(reinterpret_cast<void**>(&s->host_offs), sizeof(uint32_t) * (size_t(K) + 3)) != hipSuccess) {
         (void)hipGetLastError();
         s->host_offs = nullptr;
         return fail(MGF_ERR_NOMEM, "hipHostMalloc failed");
@@ -537,14 +537,11 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     const int32_t K = ext_max_distance(s->g);
     const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
     if (cap <= 0) return MGF_OK;   // layer 0 only: the reference's fill loop is empty
-    uint32_t* hist = s->counts;
-    uint32_t* offs = hist + (K + 2);
-    uint32_t* cursor = offs + (K + 2);
-    uint32_t* maxd = cursor + (K + 2);
     MGF_HIP(launch_ext_distance(st, s->g, u, v, w, s->layer));
-    MGF_HIP(launch_fill(st, reinterpret_cast<float*>(hist), size_t(cap) + 2, 0.0f));
-    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, hist, offs, cursor, maxd, s->list));
-    // offsets of layers 1..cap+1, then the largest distance present
+    const uint32_t* offs = nullptr;
+    const uint32_t* maxd = nullptr;
+    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, K, s->counts, s->list, &offs, &maxd));
+    // offsets of layers 0..cap+1, then the largest distance present
     MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipStreamSynchronize(st));

@@ -108,7 +108,7 @@ struct mgf_scene_s {
     // extrapolation scratch
     int32_t* layer = nullptr;      // distance of every face (nfx+nfy+nfz)
     uint32_t* list = nullptr;      // faces sorted by distance (nfx+nfy+nfz)
-    uint32_t* counts = nullptr;    // histogram, offsets, cursors [3 (K+2) + 1]
+    uint32_t* counts = nullptr;    // counting-sort scratch [ext_counts_size(K)]
     uint32_t* host_offs = nullptr; // pinned mirror of the offsets and the max distance
 
     std::vector<mgf::Solid> solids;
@@ -231,11 +231,13 @@ int32_t ext_max_distance(const GridDev& g);
 // a component has none); faces indexed u, then v, then w.
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
                                int32_t* dist);
-// Counting sort of the faces with 1 <= dist <= cap: hist (zeroed by the
-// caller) counts, offs[L] = first slot of distance L in list (offs[cap+1] =
-// total), *maxd = largest distance present (0 if none).
-hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, uint32_t* hist,
-                           uint32_t* offs, uint32_t* cursor, uint32_t* maxd, uint32_t* list);
+// Counting sort of the faces with 1 <= dist <= cap (cap <= K) into list, by
+// distance.  counts: ext_counts_size(K) entries of scratch; *offs receives
+// the device address of offs[0..cap+1] (offs[L] = first slot of distance L,
+// offs[cap+1] = total) and *maxd that of the largest distance present.
+size_t ext_counts_size(int32_t K);
+hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
+                           uint32_t* counts, uint32_t* list, const uint32_t** offs, const uint32_t** maxd);
 // Layer L: list[begin, begin+count) take the mean of their neighbours below L.
 hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const int32_t* dist,
                             const uint32_t* list, uint32_t begin, uint32_t count, int32_t L);

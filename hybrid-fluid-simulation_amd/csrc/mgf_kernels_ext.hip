@@ -13,8 +13,8 @@
 //      min-plus scan per row, along y and z a forward and a backward sweep per
 //      line (d = min(d, neighbour + 1));
 //   2. counting sort of the faces with 1 <= d <= N-1 by d (LDS histogram per
-//      block with one LDS atomic per wave and distinct distance, one global
-//      atomic per block and distance);
+//      block, one global atomic per block and distance into one of 32
+//      private copies);
 //   3. one gather kernel per layer, over that layer's faces only, in layer
 //      order: the neighbours in layers below L are final when layer L runs.
 //
@@ -41,6 +41,17 @@ __device__ __forceinline__ int comp_of(const uint32_t (&pre)[4], uint32_t t) {
     return t < pre[1] ? 0 : (t < pre[2] ? 1 : 2);
 }
 
+// Per-component values picked with selects: an index into the argument
+// arrays with a per-lane component would load them from memory.
+template <class T>
+__device__ __forceinline__ T pick(const T (&a)[3], int c) {
+    return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]);
+}
+template <class T>
+__device__ __forceinline__ T pick4(const T (&a)[4], int c) {
+    return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]);
+}
+
 __device__ __forceinline__ uint32_t udiv_r(uint32_t a, uint32_t d, double rd) {
     uint32_t q = uint32_t(double(a) * rd);
     const uint32_t r = a - q * d;
@@ -58,10 +69,10 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restric
     const int lane = threadIdx.x & 63;
     if (wave >= F.rows[3]) return;
     const int c = comp_of(F.rows, wave);
-    const uint32_t ex = F.ex[c];
-    const uint32_t local = (wave - F.rows[c]) * ex;     // first face of the row in component c
-    const float* f = F.f[c];
-    int32_t* d = dist + F.off[c] + local;
+    const uint32_t ex = pick(F.ex, c);
+    const uint32_t local = (wave - pick4(F.rows, c)) * ex;     // first face of the row in component c
+    const float* f = pick(F.f, c);
+    int32_t* d = dist + pick4(F.off, c) + local;
     // forward
     int32_t carry = kDistInf;
     for (uint32_t x0 = 0; x0 < ex; x0 += 64) {
@@ -105,14 +116,14 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
     const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
     if (t >= pre[3]) return;
     const int c = comp_of(pre, t);
-    const uint32_t l = t - pre[c];
-    const uint32_t ex = F.ex[c], ey = F.ey[c];
-    const uint32_t w = udiv_r(l, ex, F.rex[c]);
+    const uint32_t l = t - pick4(pre, c);
+    const uint32_t ex = pick(F.ex, c), ey = pick(F.ey, c);
+    const uint32_t w = udiv_r(l, ex, pick(F.rex, c));
     const uint32_t x = l - w * ex;
     uint32_t start, stride, len;
     if (AXIS == 1) { start = w * ex * ey + x; stride = ex; len = ey; }
-    else { start = w * ex + x; stride = ex * ey; len = F.ez[c]; }
-    int32_t* d = dist + F.off[c] + start;
+    else { start = w * ex + x; stride = ex * ey; len = pick(F.ez, c); }
+    int32_t* d = dist + pick4(F.off, c) + start;
     constexpr int K = 8;
     int32_t prev = kDistInf;
     for (uint32_t k0 = 0; k0 < len; k0 += K) {
@@ -146,57 +157,91 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
 }
 
 // ---------------------------------------------------------------------------
-// 2. Counting sort by distance.  Bins 1..cap; a block counts its faces in LDS
-// (kLdsBins bins; larger distances go straight to the global counters).
+// 2. Counting sort by distance.  Bins 1..cap.  A block counts its faces in
+// LDS (kLdsBins bins; larger distances go straight to the global counters),
+// then adds its non-zero bins to one of kCopies private copies of the global
+// histogram (copy = block % kCopies): the atomics of different blocks on one
+// distance are spread over kCopies addresses instead of serialising on one.
 constexpr int kLdsBins = 2048;
 constexpr int kSortCpt = 8;   // faces per work-item
+constexpr int kCopies = 32;
 
-// Counts the active lanes of a wave by key: for each distinct key, the lowest
-// active lane holding it adds the number of active lanes holding it to
-// ctr_of_key[key] with one atomic.  Each active lane gets that counter's old
-// value plus its rank among the lanes holding its key (lane order).
-__device__ __forceinline__ uint32_t wave_count(uint32_t* ctr_of_key, int32_t key, bool active) {
-    uint32_t out = 0;
-    uint64_t todo = __ballot(active);
+// Layout of the counter buffer for bins 0..K+1 (K2 = K + 2 entries a row):
+// hist[kCopies][K2], cursor[kCopies][K2], poff[kCopies][K2], offs[K2], maxd.
+struct SortBufs {
+    uint32_t *hist, *cursor, *poff, *offs, *maxd;
+};
+static SortBufs sort_bufs(uint32_t* counts, int32_t K) {
+    const size_t K2 = size_t(K) + 2;
+    SortBufs b;
+    b.hist = counts;
+    b.cursor = b.hist + kCopies * K2;
+    b.poff = b.cursor + kCopies * K2;
+    b.offs = b.poff + kCopies * K2;
+    b.maxd = b.offs + K2;
+    return b;
+}
+
+// Counts the active lanes of a wave by key into LDS counters and returns, per
+// active lane, a slot in [old counter, old counter + lanes with its key).
+// Where every active lane holds the same key (the common case: a row of faces
+// at one distance) one lane adds them all with one atomic; otherwise each lane
+// adds itself (distances along a row differ, so the addresses mostly differ).
+__device__ __forceinline__ uint32_t lds_count(uint32_t* ctr_of_key, int32_t key, bool active) {
+    const uint64_t act = __ballot(active);
+    if (act == 0) return 0;
     const int lane = threadIdx.x & 63;
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const int32_t k = __shfl(key, leader, 64);
-        const uint64_t same = __ballot(active && key == k);
+    const int leader = __ffsll((unsigned long long)act) - 1;
+    const int32_t k0 = __shfl(key, leader, 64);
+    const uint64_t same = __ballot(active && key == k0);
+    if (same == act) {
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&ctr_of_key[k], uint32_t(__popcll(same)));
+        if (lane == leader) base = atomicAdd(&ctr_of_key[k0], uint32_t(__popcll(act)));
         base = __shfl(base, leader, 64);
-        if (active && key == k) out = base + uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
-        todo &= ~same;
+        return base + uint32_t(__popcll(act & ((1ull << lane) - 1ull)));
     }
-    return out;
+    return active ? atomicAdd(&ctr_of_key[key], 1u) : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ext_zero(uint32_t* __restrict__ a, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) a[i] = 0;
 }
 
 __global__ void __launch_bounds__(kBlock) k_ext_count(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
-                                                      uint32_t* __restrict__ hist) {
+                                                      uint32_t* __restrict__ hist, uint32_t K2) {
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
     __syncthreads();
+    uint32_t* hp = hist + size_t(blockIdx.x % kCopies) * K2;
     const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
+    // all loads first (clamped index, no branch), then the counting
+    int32_t dv[kSortCpt];
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
-        const int32_t dv = i < total ? dist[i] : 0;
-        const bool in = dv >= 1 && dv <= cap;
-        (void)wave_count(h, dv < kLdsBins ? dv : 0, in && dv < kLdsBins);
-        if (in && dv >= kLdsBins) atomicAdd(&hist[dv], 1u);
+        dv[k] = dist[i < total ? i : total - 1];
+        if (i >= total) dv[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortCpt; ++k) {
+        const bool in = dv[k] >= 1 && dv[k] <= cap;
+        (void)lds_count(h, dv[k] < kLdsBins ? dv[k] : 0, in && dv[k] < kLdsBins);
+        if (in && dv[k] >= kLdsBins) atomicAdd(&hp[dv[k]], 1u);
     }
     __syncthreads();
     for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
-        if (h[b]) atomicAdd(&hist[b], h[b]);
+        if (h[b]) atomicAdd(&hp[b], h[b]);
 }
 
-// Exclusive scan of hist[1..cap] into offs[1..cap+1] (offs[1] = 0); cursor
-// cleared; *maxd = the largest distance with a non-zero count.  One block.
-__global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ hist, int32_t cap,
-                                                   uint32_t* __restrict__ offs, uint32_t* __restrict__ cursor,
-                                                   uint32_t* __restrict__ maxd) {
+// Per distance b: T = sum over copies; offs[b] = exclusive scan of T over
+// b = 1..cap (offs[cap+1] = total); poff[p][b] = offs[b] + counts of the
+// copies before p; cursors cleared; *maxd = the largest non-empty distance.
+// One block.
+__global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ hist, int32_t cap, uint32_t K2,
+                                                   uint32_t* __restrict__ offs, uint32_t* __restrict__ poff,
+                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ maxd) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
     __shared__ int32_t top;
@@ -205,7 +250,11 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int32_t b0 = 1; b0 <= cap; b0 += 1024) {
         const int32_t b = b0 + int32_t(threadIdx.x);
-        const uint32_t v = b <= cap ? hist[b] : 0u;
+        uint32_t v = 0;
+        if (b <= cap) {
+#pragma unroll 8
+            for (int p = 0; p < kCopies; ++p) v += hist[size_t(p) * K2 + b];
+        }
         uint32_t incl = v;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -217,8 +266,14 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
         uint32_t before = carry;
         for (int q = 0; q < wv; ++q) before += wsum[q];
         if (b <= cap) {
-            offs[b] = before + incl - v;
-            cursor[b] = 0;
+            const uint32_t o = before + incl - v;
+            offs[b] = o;
+            uint32_t run = o;
+            for (int p = 0; p < kCopies; ++p) {
+                poff[size_t(p) * K2 + b] = run;
+                cursor[size_t(p) * K2 + b] = 0;
+                run += hist[size_t(p) * K2 + b];
+            }
             if (v) atomicMax(&top, b);
         }
         __syncthreads();
@@ -232,33 +287,40 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
 }
 
 __global__ void __launch_bounds__(kBlock) k_ext_scatter(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
-                                                        const uint32_t* __restrict__ offs, uint32_t* cursor,
-                                                        uint32_t* __restrict__ list) {
+                                                        uint32_t K2, const uint32_t* __restrict__ poff,
+                                                        uint32_t* cursor, uint32_t* __restrict__ list) {
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
     __syncthreads();
+    const size_t copy = size_t(blockIdx.x % kCopies) * K2;
+    const uint32_t* po = poff + copy;
+    uint32_t* cu = cursor + copy;
     const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
     int32_t dv[kSortCpt];
     uint32_t rank[kSortCpt];
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
-        dv[k] = i < total ? dist[i] : 0;
+        dv[k] = dist[i < total ? i : total - 1];
+        if (i >= total) dv[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kSortCpt; ++k) {
         const bool in = dv[k] >= 1 && dv[k] <= cap;
-        rank[k] = wave_count(h, dv[k] < kLdsBins ? dv[k] : 0, in && dv[k] < kLdsBins);
-        if (in && dv[k] >= kLdsBins) rank[k] = atomicAdd(&cursor[dv[k]], 1u);
+        rank[k] = lds_count(h, dv[k] < kLdsBins ? dv[k] : 0, in && dv[k] < kLdsBins);
+        if (in && dv[k] >= kLdsBins) rank[k] = atomicAdd(&cu[dv[k]], 1u);
     }
     __syncthreads();
-    // reserve each bin's range: the bin's slot now holds its base
+    // reserve each bin's range in this block's copy: the bin's slot now holds its base
     for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
-        if (h[b]) h[b] = atomicAdd(&cursor[b], h[b]);
+        if (h[b]) h[b] = atomicAdd(&cu[b], h[b]);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
         if (dv[k] >= 1 && dv[k] <= cap) {
-            const uint32_t pos = offs[dv[k]] + (dv[k] < kLdsBins ? h[dv[k]] : 0u) + rank[k];
+            const uint32_t pos = po[dv[k]] + (dv[k] < kLdsBins ? h[dv[k]] : 0u) + rank[k];
             list[pos] = i;
         }
     }
@@ -274,31 +336,32 @@ __global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const int32_t* __
     if (t >= count) return;
     const uint32_t gi = list[begin + t];
     const int c = comp_of(F.off, gi);
-    const uint32_t i = gi - F.off[c];
-    const uint32_t ex = F.ex[c], ey = F.ey[c], ez = F.ez[c], sz = ex * ey;
-    const uint32_t z = udiv_r(i, sz, F.rexy[c]);
+    const uint32_t i = gi - pick4(F.off, c);
+    const uint32_t ex = pick(F.ex, c), ey = pick(F.ey, c), ez = pick(F.ez, c), sz = ex * ey;
+    const uint32_t z = udiv_r(i, sz, pick(F.rexy, c));
     const uint32_t rem = i - z * sz;
-    const uint32_t y = udiv_r(rem, ex, F.rex[c]);
+    const uint32_t y = udiv_r(rem, ex, pick(F.rex, c));
     const uint32_t x = rem - y * ex;
-    const int32_t* dc = dist + F.off[c];
-    float* f = F.f[c];
+    const int32_t* dc = dist + pick4(F.off, c);
+    float* f = pick(F.f, c);
     const bool in[6] = {x > 0, x + 1 < ex, y > 0, y + 1 < ey, z > 0, z + 1 < ez};
-    const uint32_t j[6] = {i - 1, i + 1, i - ex, i + ex, i - sz, i + sz};
+    const uint32_t j[6] = {in[0] ? i - 1 : i, in[1] ? i + 1 : i, in[2] ? i - ex : i,
+                           in[3] ? i + ex : i, in[4] ? i - sz : i, in[5] ? i + sz : i};
     int32_t dn[6];
     float fv[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        const uint32_t jj = in[q] ? j[q] : i;
-        dn[q] = dc[jj];
-        fv[q] = f[jj];
+        dn[q] = dc[j[q]];
+        fv[q] = f[j[q]];
     }
+    // the reference's conditional sums as selects (an unused neighbour leaves
+    // the sum untouched, a -0.0 included)
     float acc = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        if (in[q] && dn[q] < L) {
-            acc += fv[q];
-            cnt += 1.0f;
-        }
+        const bool use = in[q] && dn[q] < L;
+        acc = use ? acc + fv[q] : acc;
+        cnt = use ? cnt + 1.0f : cnt;
     }
     if (cnt != 0.0f) acc /= cnt;
     f[i] = acc;
@@ -338,19 +401,30 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
     return hipGetLastError();
 }
 
-hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, uint32_t* hist,
-                           uint32_t* offs, uint32_t* cursor, uint32_t* maxd, uint32_t* list) {
+size_t ext_counts_size(int32_t K) { return (3 * size_t(kCopies) + 1) * (size_t(K) + 2) + 1; }
+
+hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
+                           uint32_t* counts, uint32_t* list, const uint32_t** offs, const uint32_t** maxd) {
     const Faces F = faces_of(g, nullptr, nullptr, nullptr);
     const uint32_t total = F.off[3];
-    size_t G = (size_t(total) + size_t(kBlock) * kSortCpt - 1) / (size_t(kBlock) * kSortCpt);
-    G = (G + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(k_ext_count, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, hist);
+    const uint32_t K2 = uint32_t(K) + 2;
+    const SortBufs B = sort_bufs(counts, K);
+    *offs = B.offs;
+    *maxd = B.maxd;
+    const uint32_t nh = kCopies * K2;
+    hipLaunchKernelGGL(k_ext_zero, dim3((nh + kBlock - 1) / kBlock), dim3(kBlock), 0, s, B.hist, nh);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, hist, cap, offs, cursor, maxd);
+    size_t G = (size_t(total) + size_t(kBlock) * kSortCpt - 1) / (size_t(kBlock) * kSortCpt);
+    G = (G + kXcds - 1) / kXcds * kXcds;
+    hipLaunchKernelGGL(k_ext_count, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, B.hist, K2);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, offs, cursor, list);
+    hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, B.hist, cap, K2, B.offs, B.poff, B.cursor, B.maxd);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, K2, B.poff,
+                       B.cursor, list);
     return hipGetLastError();
 }
 

@@ -1,12 +1,19 @@
 // mgf_kernels_mg.hip — stencil codes, the fused kernels of the mixed-precision
 // CG and the geometric multigrid V-cycle used as its preconditioner.
 //
-// Layout and access pattern: one work-item per cell (CPT cells per work-item,
-// 256 apart, so every load of a wave is one coalesced row), cells x-fastest.
-// Every load of a stencil is issued unconditionally — a neighbour outside the
-// grid is read at a clamped index and its value discarded by a select on the
-// cell's stencil code — so all loads of a cell are in flight together instead
-// of one guarded load after another.
+// Layout and access pattern: one work-item per cell, cells x-fastest, so every
+// load of a wave is one coalesced row.  Every load of a stencil is issued
+// unconditionally — a neighbour outside the grid is read at a clamped index
+// and its value discarded by a mask taken from the cell's stencil code — so
+// all loads of a cell are in flight together.
+//
+// Arithmetic: (A v)_c = aw_c v_c + sum over fluid neighbours k of s_k (v_c -
+// v_k) in FP32, where aw_c = sum of s_k over the air neighbours.  This is the
+// matrix row of buildPressureMatrixA (diagonal = sum of s_k over non-solid
+// neighbours, -s_k to fluid neighbours) rearranged so that no term of the
+// size of the diagonal cancels: for smooth v the differences are exact and
+// the result is accurate to FP32 rounding of itself.  Dot products and the
+// x / r updates of the CG are formed in FP64.
 //
 // Reductions: each block writes one FP64 partial sum with a plain store; a
 // one-block kernel sums the partials in a fixed order and updates the CG
@@ -18,35 +25,18 @@
 namespace mgf {
 
 // ---------------------------------------------------------------------------
-// Stencil code of a cell (uint16): the matrix row of buildPressureMatrixA
-// (MPhysics/EulerSolverTool.cu:58-126) in a form that needs no neighbour
-// marker at apply time.  bit 0: fluid; bits 1-2, 3-4, 5-6: number of
-// non-solid in-grid neighbours along x, y, z (the diagonal terms); bits 7-12:
-// coupling to the -x, +x, -y, +y, -z, +z neighbour (both cells fluid).
+// Stencil code of a cell (uint16), from the markers of the cell and its six
+// neighbours, in the order -x, +x, -y, +y, -z, +z:
+//   bit 0      fluid cell
+//   bits 1-6   the neighbour is in the grid and air (Dirichlet term only)
+//   bits 7-12  the neighbour is in the grid and fluid (coupled)
+// Non-fluid cells have code 0 (identity rows of the reference matrix).
+// (MPhysics/EulerSolverTool.cu:58-126 builds the same rows.)
 enum : uint16_t {
     kCodeFluid = 1u << 0,
-    kCplMX = 1u << 7, kCplPX = 1u << 8,
-    kCplMY = 1u << 9, kCplPY = 1u << 10,
-    kCplMZ = 1u << 11, kCplPZ = 1u << 12,
+    kAirShift = 1,
+    kCplShift = 7,
 };
-__device__ __forceinline__ int code_cx(uint16_t c) { return (c >> 1) & 3; }
-__device__ __forceinline__ int code_cy(uint16_t c) { return (c >> 3) & 3; }
-__device__ __forceinline__ int code_cz(uint16_t c) { return (c >> 5) & 3; }
-
-// The diagonal in the reference's accumulation order: the x terms, then y,
-// then z — bitwise the value buildPressureMatrixA stores.  Non-fluid rows are
-// identity rows.
-__device__ __forceinline__ float code_diag(uint16_t c, float sx, float sy, float sz) {
-    const int cx = code_cx(c), cy = code_cy(c), cz = code_cz(c);
-    float d = 0.0f;
-    d = cx > 0 ? d + sx : d;
-    d = cx > 1 ? d + sx : d;
-    d = cy > 0 ? d + sy : d;
-    d = cy > 1 ? d + sy : d;
-    d = cz > 0 ? d + sz : d;
-    d = cz > 1 ? d + sz : d;
-    return (c & kCodeFluid) ? d : 1.0f;
-}
 
 __global__ void __launch_bounds__(kBlock) k_codes(int nx, int ny, int nz, const uint8_t* __restrict__ m,
                                                   uint16_t* __restrict__ code, uint32_t n) {
@@ -56,16 +46,68 @@ __global__ void __launch_bounds__(kBlock) k_codes(int nx, int ny, int nz, const 
               z = int(i / (uint32_t(nx) * uint32_t(ny)));
     if (m[i] != kFluid) { code[i] = 0; return; }
     const uint32_t sy = uint32_t(nx), sz = uint32_t(nx) * uint32_t(ny);
-    uint16_t c = kCodeFluid;
-    int cx = 0, cy = 0, cz = 0;
-    if (x > 0) { const uint8_t q = m[i - 1]; if (q != kSolid) ++cx; if (q == kFluid) c |= kCplMX; }
-    if (x + 1 < nx) { const uint8_t q = m[i + 1]; if (q != kSolid) ++cx; if (q == kFluid) c |= kCplPX; }
-    if (y > 0) { const uint8_t q = m[i - sy]; if (q != kSolid) ++cy; if (q == kFluid) c |= kCplMY; }
-    if (y + 1 < ny) { const uint8_t q = m[i + sy]; if (q != kSolid) ++cy; if (q == kFluid) c |= kCplPY; }
-    if (z > 0) { const uint8_t q = m[i - sz]; if (q != kSolid) ++cz; if (q == kFluid) c |= kCplMZ; }
-    if (z + 1 < nz) { const uint8_t q = m[i + sz]; if (q != kSolid) ++cz; if (q == kFluid) c |= kCplPZ; }
-    c |= uint16_t(cx << 1) | uint16_t(cy << 3) | uint16_t(cz << 5);
-    code[i] = c;
+    const bool in[6] = {x > 0, x + 1 < nx, y > 0, y + 1 < ny, z > 0, z + 1 < nz};
+    const uint32_t j[6] = {i - 1, i + 1, i - sy, i + sy, i - sz, i + sz};
+    uint32_t c = kCodeFluid;
+    for (int k = 0; k < 6; ++k) {
+        if (!in[k]) continue;
+        const uint8_t q = m[j[k]];
+        if (q == kAir) c |= 1u << (kAirShift + k);
+        if (q == kFluid) c |= 1u << (kCplShift + k);
+    }
+    code[i] = uint16_t(c);
+}
+
+// Per-level tables indexed by a 6-bit neighbour mask (-x,+x,-y,+y,-z,+z):
+// sum of the scales of the masked directions, added in the reference's order
+// (x terms, then y, then z: MPhysics/EulerSolverTool.cu:58-126), so that
+// table[air | coupled] is bitwise the reference's diagonal.
+__device__ __forceinline__ float mask_scale_sum(uint32_t m, float sx, float sy, float sz) {
+    float d = 0.0f;
+    if (m & 1u) d += sx;
+    if (m & 2u) d += sx;
+    if (m & 4u) d += sy;
+    if (m & 8u) d += sy;
+    if (m & 16u) d += sz;
+    if (m & 32u) d += sz;
+    return d;
+}
+
+// Block-shared tables: aw[mask] (air weights) and, with W, w[mask] = omega / d
+// for mask = the non-solid neighbours (0 where d == 0).  All threads call it.
+template <bool W>
+__device__ __forceinline__ void build_tables(float* aw, float* w, float sx, float sy, float sz, float omega) {
+    if (threadIdx.x < 64) {
+        const float d = mask_scale_sum(threadIdx.x, sx, sy, sz);
+        aw[threadIdx.x] = d;
+        if (W) w[threadIdx.x] = d != 0.0f ? omega / d : 0.0f;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t air_mask(uint16_t cd) { return (uint32_t(cd) >> kAirShift) & 63u; }
+__device__ __forceinline__ uint32_t cpl_mask(uint16_t cd) { return (uint32_t(cd) >> kCplShift) & 63u; }
+
+// v if bit `b` of the code is set, else +0 (bit pattern AND with 0 / ~0).
+__device__ __forceinline__ float keep_if(float v, uint16_t cd, int b) {
+    const int32_t m = __builtin_amdgcn_sbfe(int32_t(cd), b, 1);
+    return __int_as_float(__float_as_int(v) & m);
+}
+
+// (A v)_c, FP32, difference form; identity row for a non-fluid cell.
+__device__ __forceinline__ float arow(uint16_t cd, float vc, const float (&v)[6], float sx, float sy, float sz,
+                                      const float* aw) {
+    const float ex = keep_if(vc - v[0], cd, kCplShift + 0) + keep_if(vc - v[1], cd, kCplShift + 1);
+    const float ey = keep_if(vc - v[2], cd, kCplShift + 2) + keep_if(vc - v[3], cd, kCplShift + 3);
+    const float ez = keep_if(vc - v[4], cd, kCplShift + 4) + keep_if(vc - v[5], cd, kCplShift + 5);
+    const float a = aw[air_mask(cd)] * vc + sx * ex + sy * ey + sz * ez;
+    return (cd & kCodeFluid) ? a : vc;
+}
+
+// omega / d of a cell: 0 for non-fluid cells (code 0, entry 0 of the table)
+// and for zero rows.  No test of the code, so no branch holds up the loads.
+__device__ __forceinline__ float wdiag(uint16_t cd, const float* w) {
+    return w[air_mask(cd) | cpl_mask(cd)];
 }
 
 // ---------------------------------------------------------------------------
@@ -94,34 +136,6 @@ __device__ __forceinline__ void nbrs(uint32_t c, const Flat& F, uint32_t (&j)[6]
     j[5] = c + F.sz <= last ? c + F.sz : last;
 }
 
-// (A v)_c from the code, the centre value and the six neighbour values
-// (FP64 accumulation; the order -x, +x, -y, +y, -z, +z).  Branch-free, so the
-// neighbour loads are not sunk below a test of the code.
-__device__ __forceinline__ double arow(uint16_t cd, double vc, const double (&v)[6], float sx, float sy,
-                                       float sz) {
-    double a = double(code_diag(cd, sx, sy, sz)) * vc;
-    a -= (cd & kCplMX) ? double(sx) * v[0] : 0.0;
-    a -= (cd & kCplPX) ? double(sx) * v[1] : 0.0;
-    a -= (cd & kCplMY) ? double(sy) * v[2] : 0.0;
-    a -= (cd & kCplPY) ? double(sy) * v[3] : 0.0;
-    a -= (cd & kCplMZ) ? double(sz) * v[4] : 0.0;
-    a -= (cd & kCplPZ) ? double(sz) * v[5] : 0.0;
-    return (cd & kCodeFluid) ? a : vc;
-}
-
-// FP32 version for the multigrid smoother and residual.
-__device__ __forceinline__ float arow_f(uint16_t cd, float vc, const float (&v)[6], float sx, float sy,
-                                        float sz) {
-    float a = code_diag(cd, sx, sy, sz) * vc;
-    a -= (cd & kCplMX) ? sx * v[0] : 0.0f;
-    a -= (cd & kCplPX) ? sx * v[1] : 0.0f;
-    a -= (cd & kCplMY) ? sy * v[2] : 0.0f;
-    a -= (cd & kCplPY) ? sy * v[3] : 0.0f;
-    a -= (cd & kCplMZ) ? sz * v[4] : 0.0f;
-    a -= (cd & kCplPZ) ? sz * v[5] : 0.0f;
-    return (cd & kCodeFluid) ? a : vc;
-}
-
 // Block sum of one double, stored to part[blockIdx.x] by thread 0.
 __device__ __forceinline__ void block_partial(double v, double* part) {
     double a[1] = {v};
@@ -139,79 +153,134 @@ __device__ __forceinline__ uint32_t udiv(uint32_t a, uint32_t d, double rd) {
     return q;
 }
 
+// Cells per work-item of the fine-grid kernels (256 apart): fewer block
+// partials to finish, the LDS tables amortised, more loads in flight.
+constexpr int kCpt = 4;
+
 // ---------------------------------------------------------------------------
-// Fused CG.
+// Fused CG.  Each work-item handles kCpt cells 256 apart: first all loads of
+// its cells, then the arithmetic and the stores.  Only the last block can
+// hold cells past the end; the others run the body without the range test
+// (a block-uniform branch), so nothing stands between the loads.
+#define MGF_CELLS(CHECK, F, base)                                                  \
+    uint32_t cidx[kCpt];                                                           \
+    _Pragma("unroll") for (int k = 0; k < kCpt; ++k) {                             \
+        const uint32_t c_ = (base) + uint32_t(k) * kBlock;                         \
+        cidx[k] = (CHECK && c_ >= (F).n) ? (F).n - 1 : c_;                         \
+    }
+
 // K_I: p_new = z + beta p_old (p_new = z on the first iteration), with the
 // neighbours' p_new recomputed from z and p_old; block partials of p . A p.
-template <int CPT>
+template <bool CHECK>
+__device__ __forceinline__ double cg_pap_body(const Flat& F, uint32_t base, const uint16_t* __restrict__ code,
+                                              float sx, float sy, float sz, const float* __restrict__ zv,
+                                              const float* __restrict__ p_old, float* __restrict__ p_new,
+                                              float beta, bool first, const float* aw) {
+    MGF_CELLS(CHECK, F, base)
+    uint16_t cd[kCpt];
+    float zc[kCpt], pc[kCpt], zn[kCpt][6], pn[kCpt][6];
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        uint32_t j[6];
+        nbrs(cidx[k], F, j);
+        cd[k] = code[cidx[k]];
+        zc[k] = zv[cidx[k]];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) zn[k][m] = zv[j[m]];
+        if (!first) {
+            pc[k] = p_old[cidx[k]];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) pn[k][m] = p_old[j[m]];
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        float vc = zc[k], v[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) v[m] = zn[k][m];
+        if (!first) {
+            vc = __builtin_fmaf(beta, pc[k], vc);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) v[m] = __builtin_fmaf(beta, pn[k][m], v[m]);
+        }
+        const float a = arow(cd[k], vc, v, sx, sy, sz, aw);
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (!CHECK || c < F.n) {
+            p_new[c] = vc;
+            acc += double(vc) * double(a);
+        }
+    }
+    return acc;
+}
+
 __global__ void __launch_bounds__(kBlock) k_cg_pap(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
                                                    float sz, const float* __restrict__ zv,
                                                    const float* __restrict__ p_old, float* __restrict__ p_new,
                                                    const CgScalars* __restrict__ sc, double* __restrict__ part) {
+    __shared__ float aw[64];
     if (sc->done) return;
-    const double beta = sc->beta;
+    build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
+    const float beta = float(sc->beta);
     const bool first = sc->iters == 0;
-    const uint32_t base = xcd_block() * (kBlock * CPT) + threadIdx.x;
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const uint32_t c = base + uint32_t(k) * kBlock;
-        const bool ok = c < F.n;
-        const uint32_t cc = ok ? c : F.n - 1;
-        uint32_t j[6];
-        nbrs(cc, F, j);
-        const uint16_t cd = code[cc];
-        double vc, v[6];
-        if (first) {
-            vc = double(zv[cc]);
-#pragma unroll
-            for (int m = 0; m < 6; ++m) v[m] = double(zv[j[m]]);
-        } else {
-            vc = double(float(__builtin_fma(beta, double(p_old[cc]), double(zv[cc]))));
-#pragma unroll
-            for (int m = 0; m < 6; ++m)
-                v[m] = double(float(__builtin_fma(beta, double(p_old[j[m]]), double(zv[j[m]]))));
-        }
-        const double a = arow(cd, vc, v, sx, sy, sz);
-        if (ok) p_new[c] = float(vc);
-        acc += ok ? vc * a : 0.0;
-    }
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n)
+                           ? cg_pap_body<false>(F, base, code, sx, sy, sz, zv, p_old, p_new, beta, first, aw)
+                           : cg_pap_body<true>(F, base, code, sx, sy, sz, zv, p_old, p_new, beta, first, aw);
     block_partial(acc, part);
 }
 
 // K_II: A p recomputed from the stored p; x += alpha p; r -= alpha A p (each
 // formed in FP64, stored FP32); block partials of r . r.
-template <int CPT>
+template <bool CHECK>
+__device__ __forceinline__ double cg_update_body(const Flat& F, uint32_t base, const uint16_t* __restrict__ code,
+                                                 float sx, float sy, float sz, const float* __restrict__ p,
+                                                 float* __restrict__ x, float* __restrict__ r, double alpha,
+                                                 const float* aw) {
+    MGF_CELLS(CHECK, F, base)
+    uint16_t cd[kCpt];
+    float pc[kCpt], pn[kCpt][6], x0[kCpt], r0[kCpt];
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        uint32_t j[6];
+        nbrs(cidx[k], F, j);
+        cd[k] = code[cidx[k]];
+        pc[k] = p[cidx[k]];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pn[k][m] = p[j[m]];
+        x0[k] = x[cidx[k]];
+        r0[k] = r[cidx[k]];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const float a = arow(cd[k], pc[k], pn[k], sx, sy, sz, aw);
+        const float xn = float(__builtin_fma(alpha, double(pc[k]), double(x0[k])));
+        const float rn = float(__builtin_fma(-alpha, double(a), double(r0[k])));
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (!CHECK || c < F.n) {
+            x[c] = xn;
+            r[c] = rn;
+            acc += double(rn) * double(rn);
+        }
+    }
+    return acc;
+}
+
 __global__ void __launch_bounds__(kBlock) k_cg_update(Flat F, const uint16_t* __restrict__ code, float sx,
                                                       float sy, float sz, const float* __restrict__ p,
                                                       float* __restrict__ x, float* __restrict__ r,
                                                       const CgScalars* __restrict__ sc, double* __restrict__ part) {
+    __shared__ float aw[64];
     if (sc->done) return;
+    build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
     const double alpha = sc->alpha;
-    const uint32_t base = xcd_block() * (kBlock * CPT) + threadIdx.x;
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const uint32_t c = base + uint32_t(k) * kBlock;
-        const bool ok = c < F.n;
-        const uint32_t cc = ok ? c : F.n - 1;
-        uint32_t j[6];
-        nbrs(cc, F, j);
-        const uint16_t cd = code[cc];
-        const double vc = double(p[cc]);
-        double v[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v[m] = double(p[j[m]]);
-        const float x0 = x[cc], r0 = r[cc];
-        const double a = arow(cd, vc, v, sx, sy, sz);
-        const float xn = float(__builtin_fma(alpha, vc, double(x0)));
-        const float rn = float(__builtin_fma(-alpha, a, double(r0)));
-        if (ok) {
-            x[c] = xn;
-            r[c] = rn;
-        }
-        acc += ok ? double(rn) * double(rn) : 0.0;
-    }
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n)
+                           ? cg_update_body<false>(F, base, code, sx, sy, sz, p, x, r, alpha, aw)
+                           : cg_update_body<true>(F, base, code, sx, sy, sz, p, x, r, alpha, aw);
     block_partial(acc, part);
 }
 
@@ -228,8 +297,15 @@ constexpr int kFinishBlock = 1024;
 __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restrict__ part, uint32_t count,
                                                          CgScalars* sc, int kind) {
     if (sc->done) return;
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < count; i += kFinishBlock) s += part[i];
+    // four independent chains: the loads of a thread are in flight together
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    uint32_t i = threadIdx.x;
+    for (; i + 3 * kFinishBlock < count; i += 4 * kFinishBlock) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s4[q] += part[i + uint32_t(q) * kFinishBlock];
+    }
+    for (; i < count; i += kFinishBlock) s4[0] += part[i];
+    double s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
     __shared__ double sh[kFinishBlock / 64];
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
@@ -267,10 +343,11 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restric
 // ---------------------------------------------------------------------------
 // Multigrid (preconditioner).  Level l: cells with code_l, right-hand side
 // b_l, iterate x_l; a coarse cell is a 2x2x2 block of fine cells (one layer
-// on an axis whose extent is 1).  Damped Jacobi smoothing, restriction =
-// weight * sum over the fluid children's residuals, prolongation = injection
-// into fluid children: with as many sweeps after the coarse correction as
-// before it, the V-cycle is a symmetric operator.
+// on an axis whose extent is 1).  Damped Jacobi smoothing with the diagonal
+// D (zero rows are left at 0), restriction = weight * sum over the fluid
+// children's residuals, prolongation = injection into fluid children: with as
+// many sweeps after the coarse correction as before it, the V-cycle is a
+// symmetric operator.
 using MgLevel = MgDims;
 
 // Coarse markers: with fluid_first, fluid if any child is fluid, else air if
@@ -298,134 +375,196 @@ __global__ void __launch_bounds__(kBlock) k_mg_coarsen(MgLevel F, MgLevel Cc, co
     mc[i] = out;
 }
 
-// One damped Jacobi sweep xo = xi + omega (b - A xi) / d on fluid cells (0
-// elsewhere, and on a fluid cell whose diagonal is 0: every neighbour solid or
-// outside the grid, an all-zero row of the singular system); FIRST: the sweep
-// from xi = 0, xo = omega b / d.  RZ (the level-0 final sweep, b = r there)
-// also writes block partials of r . z.
-template <bool FIRST, bool RZ, int CPT>
+// One damped Jacobi sweep xo = xi + omega D^-1 (b - A xi) on fluid cells (0
+// elsewhere, and on zero rows: a fluid cell whose neighbours are all solid
+// or outside the grid); FIRST: the sweep from xi = 0, xo = omega D^-1 b.  RZ
+// (the level-0 final sweep, b = r there) also writes block partials of r . z.
+template <bool FIRST, bool CHECK>
+__device__ __forceinline__ double smooth_body(const Flat& F, const MgLevel& L, uint32_t base,
+                                              const uint16_t* __restrict__ code, const float* __restrict__ b,
+                                              const float* __restrict__ xi, float* __restrict__ xo,
+                                              const float* aw, const float* w) {
+    MGF_CELLS(CHECK, F, base)
+    uint16_t cd[kCpt];
+    float bc[kCpt], vc[kCpt], vn[kCpt][6];
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        cd[k] = code[cidx[k]];
+        bc[k] = b[cidx[k]];
+        if (!FIRST) {
+            uint32_t j[6];
+            nbrs(cidx[k], F, j);
+            vc[k] = xi[cidx[k]];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) vn[k][m] = xi[j[m]];
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const float wd = wdiag(cd[k], w);
+        float out;
+        if (FIRST) {
+            out = wd * bc[k];
+        } else {
+            const float ax = arow(cd[k], vc[k], vn[k], L.sx, L.sy, L.sz, aw);
+            out = (cd[k] & kCodeFluid) ? vc[k] + wd * (bc[k] - ax) : 0.0f;
+        }
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (!CHECK || c < F.n) {
+            xo[c] = out;
+            acc += double(bc[k]) * double(out);
+        }
+    }
+    return acc;
+}
+
+template <bool FIRST, bool RZ>
 __global__ void __launch_bounds__(kBlock) k_mg_smooth(Flat F, MgLevel L, const uint16_t* __restrict__ code,
                                                       const float* __restrict__ b, const float* __restrict__ xi,
                                                       float* __restrict__ xo, float omega,
                                                       const CgScalars* __restrict__ sc, double* __restrict__ part) {
+    __shared__ float aw[64], w[64];
     if (sc->done) return;
-    const uint32_t base = xcd_block() * (kBlock * CPT) + threadIdx.x;
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-        const uint32_t c = base + uint32_t(k) * kBlock;
-        const bool ok = c < F.n;
-        const uint32_t cc = ok ? c : F.n - 1;
-        const uint16_t cd = code[cc];
-        const float bc = b[cc];
-        const float d = code_diag(cd, L.sx, L.sy, L.sz);
-        const bool live = (cd & kCodeFluid) && d != 0.0f;
-        // the hardware reciprocal: a smoother needs no correctly rounded divide
-        const float wd = omega * __builtin_amdgcn_rcpf(d);
-        float out;
-        if (FIRST) {
-            out = live ? wd * bc : 0.0f;
-        } else {
-            uint32_t j[6];
-            nbrs(cc, F, j);
-            const float vc = xi[cc];
-            float v[6];
-#pragma unroll
-            for (int m = 0; m < 6; ++m) v[m] = xi[j[m]];
-            const float ax = arow_f(cd, vc, v, L.sx, L.sy, L.sz);
-            out = live ? vc + wd * (bc - ax) : 0.0f;
-        }
-        if (ok) xo[c] = out;
-        if (RZ) acc += ok ? double(bc) * double(out) : 0.0;
-    }
+    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n)
+                           ? smooth_body<FIRST, false>(F, L, base, code, b, xi, xo, aw, w)
+                           : smooth_body<FIRST, true>(F, L, base, code, b, xi, xo, aw, w);
     if (RZ) block_partial(acc, part);
 }
 
 // The first two sweeps from x = 0 in one pass: x1 = omega D^-1 b is
 // pointwise, so x2 = x1 + omega D^-1 (b - A x1) needs only b and the codes at
 // the seven points.
+template <bool CHECK>
+__device__ __forceinline__ double smooth2_body(const Flat& F, const MgLevel& L, uint32_t base,
+                                               const uint16_t* __restrict__ code, const float* __restrict__ b,
+                                               float* __restrict__ xo, const float* aw, const float* w) {
+    MGF_CELLS(CHECK, F, base)
+    uint16_t cd[kCpt], cn[kCpt][6];
+    float bc[kCpt], bn[kCpt][6];
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        uint32_t j[6];
+        nbrs(cidx[k], F, j);
+        cd[k] = code[cidx[k]];
+        bc[k] = b[cidx[k]];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            cn[k][m] = code[j[m]];
+            bn[k][m] = b[j[m]];
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const float wd = wdiag(cd[k], w);
+        const float x1c = wd * bc[k];
+        float x1[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) x1[m] = wdiag(cn[k][m], w) * bn[k][m];
+        const float ax = arow(cd[k], x1c, x1, L.sx, L.sy, L.sz, aw);
+        const float out = (cd[k] & kCodeFluid) ? x1c + wd * (bc[k] - ax) : 0.0f;
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (!CHECK || c < F.n) {
+            xo[c] = out;
+            acc += double(bc[k]) * double(out);
+        }
+    }
+    return acc;
+}
+
 template <bool RZ>
 __global__ void __launch_bounds__(kBlock) k_mg_smooth2_first(Flat F, MgLevel L, const uint16_t* __restrict__ code,
                                                              const float* __restrict__ b, float* __restrict__ xo,
                                                              float omega, const CgScalars* __restrict__ sc,
                                                              double* __restrict__ part) {
+    __shared__ float aw[64], w[64];
     if (sc->done) return;
-    const uint32_t c = xcd_block() * kBlock + threadIdx.x;
-    const bool ok = c < F.n;
-    const uint32_t cc = ok ? c : F.n - 1;
-    uint32_t j[6];
-    nbrs(cc, F, j);
-    const uint16_t cd = code[cc];
-    const float bc = b[cc];
-    uint16_t cn[6];
-    float bn[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        cn[m] = code[j[m]];
-        bn[m] = b[j[m]];
-    }
-    const float d = code_diag(cd, L.sx, L.sy, L.sz);
-    const bool live = (cd & kCodeFluid) && d != 0.0f;
-    const float wd = omega * __builtin_amdgcn_rcpf(d);
-    const float x1c = live ? wd * bc : 0.0f;
-    float x1[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        const float dn = code_diag(cn[m], L.sx, L.sy, L.sz);
-        const bool ln = (cn[m] & kCodeFluid) && dn != 0.0f;
-        x1[m] = ln ? omega * __builtin_amdgcn_rcpf(dn) * bn[m] : 0.0f;
-    }
-    const float ax = arow_f(cd, x1c, x1, L.sx, L.sy, L.sz);
-    const float out = live ? x1c + wd * (bc - ax) : 0.0f;
-    if (ok) xo[c] = out;
-    if (RZ) block_partial(ok ? double(bc) * double(out) : 0.0, part);
+    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n) ? smooth2_body<false>(F, L, base, code, b, xo, aw, w)
+                                                   : smooth2_body<true>(F, L, base, code, b, xo, aw, w);
+    if (RZ) block_partial(acc, part);
 }
 
 // The first sweep after the coarse correction, with the prolongation by
 // injection applied on the fly: x_in = x + xc[parent] at the seven points
 // (only coupled, hence fluid, neighbours are used), xo = x_in + omega D^-1
 // (b - A x_in); x_in is not stored.  h[xyz] = 1 where the axis was halved.
+struct Prolong {
+    Flat FC;
+    int hx, hy, hz;
+    double rnx, rplane;
+};
+
+template <bool CHECK>
+__device__ __forceinline__ double smooth_prolong_body(const Flat& F, const MgLevel& L, const Prolong& P,
+                                                      uint32_t base, const uint16_t* __restrict__ code,
+                                                      const float* __restrict__ b, const float* __restrict__ xi,
+                                                      const float* __restrict__ xc, float* __restrict__ xo,
+                                                      const float* aw, const float* w) {
+    MGF_CELLS(CHECK, F, base)
+    uint16_t cd[kCpt];
+    float bc[kCpt], vc[kCpt], vn[kCpt][6];
+    const uint32_t nx = uint32_t(L.nx), ny = uint32_t(L.ny), nz = uint32_t(L.nz);
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const uint32_t cc = cidx[k];
+        const uint32_t z = udiv(cc, F.sz, P.rplane);
+        const uint32_t rem = cc - z * F.sz;
+        const uint32_t y = udiv(rem, F.sy, P.rnx);
+        const uint32_t x = rem - y * F.sy;
+        const uint32_t X = x >> P.hx, Y = y >> P.hy, Z = z >> P.hz;
+        const uint32_t Xm = (x > 0 ? x - 1 : 0) >> P.hx, Xp = (x + 1 < nx ? x + 1 : x) >> P.hx;
+        const uint32_t Ym = (y > 0 ? y - 1 : 0) >> P.hy, Yp = (y + 1 < ny ? y + 1 : y) >> P.hy;
+        const uint32_t Zm = (z > 0 ? z - 1 : 0) >> P.hz, Zp = (z + 1 < nz ? z + 1 : z) >> P.hz;
+        const uint32_t pz = Z * P.FC.sz, py = Y * P.FC.sy;
+        const uint32_t pp[6] = {pz + py + Xm, pz + py + Xp, pz + Ym * P.FC.sy + X,
+                                pz + Yp * P.FC.sy + X, Zm * P.FC.sz + py + X, Zp * P.FC.sz + py + X};
+        uint32_t j[6];
+        nbrs(cc, F, j);
+        cd[k] = code[cc];
+        bc[k] = b[cc];
+        vc[k] = xi[cc] + xc[pz + py + X];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) vn[k][m] = xi[j[m]] + xc[pp[m]];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const float wd = wdiag(cd[k], w);
+        const float ax = arow(cd[k], vc[k], vn[k], L.sx, L.sy, L.sz, aw);
+        const float out = (cd[k] & kCodeFluid) ? vc[k] + wd * (bc[k] - ax) : 0.0f;
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (!CHECK || c < F.n) {
+            xo[c] = out;
+            acc += double(bc[k]) * double(out);
+        }
+    }
+    return acc;
+}
+
 template <bool RZ>
-__global__ void __launch_bounds__(kBlock) k_mg_smooth_prolong(Flat F, MgLevel L, Flat FC, int hx, int hy, int hz,
-                                                              double rnx, double rplane,
+__global__ void __launch_bounds__(kBlock) k_mg_smooth_prolong(Flat F, MgLevel L, Prolong P,
                                                               const uint16_t* __restrict__ code,
                                                               const float* __restrict__ b,
                                                               const float* __restrict__ xi,
                                                               const float* __restrict__ xc, float* __restrict__ xo,
                                                               float omega, const CgScalars* __restrict__ sc,
                                                               double* __restrict__ part) {
+    __shared__ float aw[64], w[64];
     if (sc->done) return;
-    const uint32_t c = xcd_block() * kBlock + threadIdx.x;
-    const bool ok = c < F.n;
-    const uint32_t cc = ok ? c : F.n - 1;
-    const uint32_t z = udiv(cc, F.sz, rplane);
-    const uint32_t rem = cc - z * F.sz;
-    const uint32_t y = udiv(rem, F.sy, rnx);
-    const uint32_t x = rem - y * F.sy;
-    const uint32_t nx = uint32_t(L.nx), ny = uint32_t(L.ny), nz = uint32_t(L.nz);
-    const uint32_t X = x >> hx, Y = y >> hy, Z = z >> hz;
-    const uint32_t Xm = (x > 0 ? x - 1 : 0) >> hx, Xp = (x + 1 < nx ? x + 1 : x) >> hx;
-    const uint32_t Ym = (y > 0 ? y - 1 : 0) >> hy, Yp = (y + 1 < ny ? y + 1 : y) >> hy;
-    const uint32_t Zm = (z > 0 ? z - 1 : 0) >> hz, Zp = (z + 1 < nz ? z + 1 : z) >> hz;
-    const uint32_t pz = Z * FC.sz, py = Y * FC.sy;
-    const uint32_t pc = pz + py + X;
-    const uint32_t p[6] = {pz + py + Xm, pz + py + Xp, pz + Ym * FC.sy + X,
-                           pz + Yp * FC.sy + X, Zm * FC.sz + py + X, Zp * FC.sz + py + X};
-    uint32_t j[6];
-    nbrs(cc, F, j);
-    const uint16_t cd = code[cc];
-    const float bc = b[cc];
-    const float vc = xi[cc] + xc[pc];
-    float v[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) v[m] = xi[j[m]] + xc[p[m]];
-    const float d = code_diag(cd, L.sx, L.sy, L.sz);
-    const bool live = (cd & kCodeFluid) && d != 0.0f;
-    const float wd = omega * __builtin_amdgcn_rcpf(d);
-    const float ax = arow_f(cd, vc, v, L.sx, L.sy, L.sz);
-    const float out = live ? vc + wd * (bc - ax) : 0.0f;
-    if (ok) xo[c] = out;
-    if (RZ) block_partial(ok ? double(bc) * double(out) : 0.0, part);
+    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n)
+                           ? smooth_prolong_body<false>(F, L, P, base, code, b, xi, xc, xo, aw, w)
+                           : smooth_prolong_body<true>(F, L, P, base, code, b, xi, xc, xo, aw, w);
+    if (RZ) block_partial(acc, part);
 }
 
 // Restriction of the residual, one work-item per coarse cell:
@@ -436,7 +575,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Fla
                                                         const uint16_t* __restrict__ codec, float* __restrict__ bc,
                                                         float weight, double rnx, double rplane,
                                                         const CgScalars* __restrict__ sc) {
+    __shared__ float aw[64];
     if (sc->done) return;
+    build_tables<false>(aw, nullptr, Fd.sx, Fd.sy, Fd.sz, 0.0f);
     const uint32_t c = xcd_block() * kBlock + threadIdx.x;
     const bool ok = c < FC.n;
     const uint32_t cc = ok ? c : FC.n - 1;
@@ -465,7 +606,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Fla
                 float v[6];
 #pragma unroll
                 for (int m = 0; m < 6; ++m) v[m] = xf[j[m]];
-                const float res = bv - arow_f(cd, vc, v, Fd.sx, Fd.sy, Fd.sz);
+                const float res = bv - arow(cd, vc, v, Fd.sx, Fd.sy, Fd.sz, aw);
                 sum += (in && (cd & kCodeFluid)) ? res : 0.0f;
             }
     if (ok) bc[c] = (codec[c] & kCodeFluid) ? sum * weight : 0.0f;
@@ -494,19 +635,17 @@ struct CoarseArgs {
     CoarseLevel lv[kCoarseMaxLevels];
 };
 
-__device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* b, const float* xi, float* xo,
-                                          float omega, bool first) {
+__device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* aw, const float* w, const float* b,
+                                          const float* xi, float* xo, bool first) {
     const uint32_t n = uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz);
     const Flat F{n, uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
     for (uint32_t c = threadIdx.x; c < n; c += kCoarseThreads) {
         const uint16_t cd = L.code[c];
-        const float d = code_diag(cd, L.sx, L.sy, L.sz);
-        const bool live = (cd & kCodeFluid) && d != 0.0f;
-        const float wd = omega * __builtin_amdgcn_rcpf(d);
+        const float wd = wdiag(cd, w);
         const float bc = b[c];
         float out;
         if (first) {
-            out = live ? wd * bc : 0.0f;
+            out = wd * bc;
         } else {
             uint32_t j[6];
             nbrs(c, F, j);
@@ -514,7 +653,7 @@ __device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* b, 
 #pragma unroll
             for (int m = 0; m < 6; ++m) v[m] = xi[j[m]];
             const float vc = xi[c];
-            out = live ? vc + wd * (bc - arow_f(cd, vc, v, L.sx, L.sy, L.sz)) : 0.0f;
+            out = (cd & kCodeFluid) ? vc + wd * (bc - arow(cd, vc, v, L.sx, L.sy, L.sz, aw)) : 0.0f;
         }
         xo[c] = out;
     }
@@ -522,7 +661,15 @@ __device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* b, 
 }
 
 __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A, const CgScalars* __restrict__ sc) {
+    __shared__ float tab_aw[kCoarseMaxLevels][64], tab_w[kCoarseMaxLevels][64];
     if (sc->done) return;
+    for (int l = 0; l < A.nlev; ++l) {
+        if (threadIdx.x < 64) {
+            const float d = mask_scale_sum(threadIdx.x, A.lv[l].sx, A.lv[l].sy, A.lv[l].sz);
+            tab_aw[l][threadIdx.x] = d;
+            tab_w[l][threadIdx.x] = d != 0.0f ? A.omega / d : 0.0f;
+        }
+    }
     extern __shared__ float lds[];
     float* xs[kCoarseMaxLevels];
     float* ts[kCoarseMaxLevels];
@@ -540,13 +687,14 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
         xs[l] = q; q += n;
         ts[l] = q; q += n;
     }
+    __syncthreads();
     // descent
     for (int l = 0; l < A.nlev; ++l) {
         const CoarseLevel& L = A.lv[l];
         const int sweeps = l == A.nlev - 1 ? A.last_sweeps : A.nu;
-        lds_sweep(L, bs[l], xs[l], xs[l], A.omega, true);
+        lds_sweep(L, tab_aw[l], tab_w[l], bs[l], xs[l], xs[l], true);
         for (int k = 1; k < sweeps; ++k) {
-            lds_sweep(L, bs[l], xs[l], ts[l], A.omega, false);
+            lds_sweep(L, tab_aw[l], tab_w[l], bs[l], xs[l], ts[l], false);
             float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
         }
         if (l == A.nlev - 1) break;
@@ -576,7 +724,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
                         float v[6];
 #pragma unroll
                         for (int m = 0; m < 6; ++m) v[m] = xf[j[m]];
-                        sum += bf[f] - arow_f(cd, xf[f], v, L.sx, L.sy, L.sz);
+                        sum += bf[f] - arow(cd, xf[f], v, L.sx, L.sy, L.sz, tab_aw[l]);
                     }
             bc[c] = (C.code[c] & kCodeFluid) ? sum * weight : 0.0f;
         }
@@ -598,7 +746,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
         }
         __syncthreads();
         for (int k = 0; k < A.nu; ++k) {
-            lds_sweep(L, bs[l], xs[l], ts[l], A.omega, false);
+            lds_sweep(L, tab_aw[l], tab_w[l], bs[l], xs[l], ts[l], false);
             float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
         }
     }
@@ -609,17 +757,15 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
 
 // ===========================================================================
 // Launchers.
-constexpr int kCpt = 1;   // cells per work-item of the fine-grid kernels
-
-static uint32_t flat_grid(uint32_t n, int cpt) {
-    size_t g = (size_t(n) + size_t(kBlock) * cpt - 1) / (size_t(kBlock) * cpt);
+static uint32_t flat_grid(uint32_t n) {
+    size_t g = (size_t(n) + size_t(kBlock) * kCpt - 1) / (size_t(kBlock) * kCpt);
     g = (g + kXcds - 1) / kXcds * kXcds;
     if (g == 0) g = kXcds;
     return uint32_t(g);
 }
 
 uint32_t col_blocks(const GridDev& g) {
-    return flat_grid(make_flat(g.nx, g.ny, g.nz).n, 1);
+    return flat_grid(make_flat(g.nx, g.ny, g.nz).n);
 }
 
 hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* markers, uint16_t* codes) {
@@ -635,9 +781,8 @@ static hipError_t finish(hipStream_t s, const double* part, uint32_t count, CgSc
 hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
                            const float* zv, const float* p_old, float* p_new, CgScalars* sc, ReduceBuf red) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
-    const uint32_t G = flat_grid(F.n, kCpt);
-    hipError_t e = LAUNCH(k_cg_pap<kCpt>, G, F, codes, scale[0], scale[1], scale[2], zv, p_old, p_new, sc,
-                          red.partials);
+    const uint32_t G = flat_grid(F.n);
+    hipError_t e = LAUNCH(k_cg_pap, G, F, codes, scale[0], scale[1], scale[2], zv, p_old, p_new, sc, red.partials);
     if (e != hipSuccess) return e;
     return finish(s, red.partials, G, sc, kFinishAlpha);
 }
@@ -645,9 +790,8 @@ hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3]
 hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
                              const float* p, float* x, float* r, CgScalars* sc, ReduceBuf red, int identity) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
-    const uint32_t G = flat_grid(F.n, kCpt);
-    hipError_t e = LAUNCH(k_cg_update<kCpt>, G, F, codes, scale[0], scale[1], scale[2], p, x, r, sc,
-                          red.partials);
+    const uint32_t G = flat_grid(F.n);
+    hipError_t e = LAUNCH(k_cg_update, G, F, codes, scale[0], scale[1], scale[2], p, x, r, sc, red.partials);
     if (e != hipSuccess) return e;
     return finish(s, red.partials, G, sc, identity ? kFinishRrIdent : kFinishRr);
 }
@@ -661,16 +805,16 @@ hipError_t launch_mg_coarsen(hipStream_t s, const MgDims& f, const MgDims& c, co
 hipError_t launch_mg_smooth(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
                             float* xo, float omega, CgScalars* sc, ReduceBuf red, int first, int with_rz) {
     const Flat F = make_flat(l.nx, l.ny, l.nz);
-    const uint32_t G = flat_grid(F.n, kCpt);
+    const uint32_t G = flat_grid(F.n);
     hipError_t e;
     if (first && with_rz)
-        e = LAUNCH((k_mg_smooth<true, true, kCpt>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
+        e = LAUNCH((k_mg_smooth<true, true>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
     else if (first)
-        e = LAUNCH((k_mg_smooth<true, false, kCpt>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
+        e = LAUNCH((k_mg_smooth<true, false>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
     else if (with_rz)
-        e = LAUNCH((k_mg_smooth<false, true, kCpt>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
+        e = LAUNCH((k_mg_smooth<false, true>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
     else
-        e = LAUNCH((k_mg_smooth<false, false, kCpt>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
+        e = LAUNCH((k_mg_smooth<false, false>), G, F, l, code, b, xi, xo, omega, sc, red.partials);
     if (e != hipSuccess || !with_rz) return e;
     return finish(s, red.partials, G, sc, kFinishRz);
 }
@@ -678,7 +822,7 @@ hipError_t launch_mg_smooth(hipStream_t s, const MgDims& l, const uint16_t* code
 hipError_t launch_mg_smooth2_first(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, float* xo,
                                    float omega, CgScalars* sc, ReduceBuf red, int with_rz) {
     const Flat F = make_flat(l.nx, l.ny, l.nz);
-    const uint32_t G = flat_grid(F.n, 1);
+    const uint32_t G = flat_grid(F.n);
     hipError_t e;
     if (with_rz) e = LAUNCH(k_mg_smooth2_first<true>, G, F, l, code, b, xo, omega, sc, red.partials);
     else e = LAUNCH(k_mg_smooth2_first<false>, G, F, l, code, b, xo, omega, sc, red.partials);
@@ -689,17 +833,20 @@ hipError_t launch_mg_smooth2_first(hipStream_t s, const MgDims& l, const uint16_
 hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code,
                                     const float* b, const float* xi, const float* xc, float* xo, float omega,
                                     CgScalars* sc, ReduceBuf red, int with_rz) {
-    const Flat F = make_flat(f.nx, f.ny, f.nz), FC = make_flat(c.nx, c.ny, c.nz);
-    const uint32_t G = flat_grid(F.n, 1);
-    const int hx = f.nx > 1 ? 1 : 0, hy = f.ny > 1 ? 1 : 0, hz = f.nz > 1 ? 1 : 0;
-    const double rnx = 1.0 / double(F.sy), rplane = 1.0 / double(F.sz);
+    const Flat F = make_flat(f.nx, f.ny, f.nz);
+    const uint32_t G = flat_grid(F.n);
+    Prolong P;
+    P.FC = make_flat(c.nx, c.ny, c.nz);
+    P.hx = f.nx > 1 ? 1 : 0;
+    P.hy = f.ny > 1 ? 1 : 0;
+    P.hz = f.nz > 1 ? 1 : 0;
+    P.rnx = 1.0 / double(F.sy);
+    P.rplane = 1.0 / double(F.sz);
     hipError_t e;
     if (with_rz)
-        e = LAUNCH(k_mg_smooth_prolong<true>, G, F, f, FC, hx, hy, hz, rnx, rplane, code, b, xi, xc, xo, omega, sc,
-                   red.partials);
+        e = LAUNCH(k_mg_smooth_prolong<true>, G, F, f, P, code, b, xi, xc, xo, omega, sc, red.partials);
     else
-        e = LAUNCH(k_mg_smooth_prolong<false>, G, F, f, FC, hx, hy, hz, rnx, rplane, code, b, xi, xc, xo, omega, sc,
-                   red.partials);
+        e = LAUNCH(k_mg_smooth_prolong<false>, G, F, f, P, code, b, xi, xc, xo, omega, sc, red.partials);
     if (e != hipSuccess || !with_rz) return e;
     return finish(s, red.partials, G, sc, kFinishRz);
 }
@@ -710,7 +857,8 @@ size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev) {
         const size_t n = size_t(levels[l].nx) * levels[l].ny * levels[l].nz;
         floats += (l == 0 ? 2 : 3) * n;
     }
-    return floats * sizeof(float);
+    // + the static tables of the kernel
+    return floats * sizeof(float) + size_t(2) * kCoarseMaxLevels * 64 * sizeof(float);
 }
 
 hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
@@ -729,8 +877,8 @@ hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uin
         A.lv[l].sx = levels[l].sx; A.lv[l].sy = levels[l].sy; A.lv[l].sz = levels[l].sz;
         A.lv[l].code = codes[l];
     }
-    const size_t lds = mg_coarse_lds_bytes(levels, nlev);
-    hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(kCoarseThreads), lds, s, A, sc);
+    const size_t dyn = mg_coarse_lds_bytes(levels, nlev) - size_t(2) * kCoarseMaxLevels * 64 * sizeof(float);
+    hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(kCoarseThreads), dyn, s, A, sc);
     return hipGetLastError();
 }
 
@@ -742,6 +890,5 @@ hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, c
     return LAUNCH(k_mg_restrict, grid_for(FC.n), FC, c, FF, f, codef, bf, xf, codec, bc, 1.0f / float(slots),
                   1.0 / double(FC.sy), 1.0 / double(FC.sz), sc);
 }
-
 
 }  // namespace mgf
