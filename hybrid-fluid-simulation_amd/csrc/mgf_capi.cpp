@@ -431,14 +431,11 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         TRY(ensure_precond(s));
         TRY(mg_setup(s, sa));
     }
-    // ||b||^2 and the overall target
-    MGF_HIP(launch_fill64(st, s->x64, s->ncell, 0.0));
-    MGF_HIP(launch_refine(st, s->g, sa, s->markers, nullptr, s->x64, s->b, nullptr, s->red.partials,
-                          kResidualBlocks, 0));
-    MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * 2 * kResidualBlocks,
-                           hipMemcpyDeviceToHost, st));
+    // x64 = 0, ||b||^2 and the overall target
+    MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, nullptr, s->x64, s->b, nullptr, 0, s->red, s->red.out2));
+    MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.out2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
     MGF_HIP(hipStreamSynchronize(st));
-    const double b2 = host_sum(s->host_scratch, kResidualBlocks, 2, 1);
+    const double b2 = s->host_scratch[1];
     double target2;
     if (d.tol_mode == MGF_TOL_ABS_SQRT_NORM) {
         const double t = d.tolerance * d.tolerance;  // sqrt(||r||) < tol
@@ -454,8 +451,7 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     const float* rsrc = s->b;
     CgScalars hs{};
     while (!conv && sweeps < max_sweeps && it_total < max_it) {
-        MGF_HIP(launch_cg_init(st, s->g, rsrc, s->r, nullptr, s->x, s->scal, s->red, inner, target2,
-                               sweeps == 0));
+        MGF_HIP(launch_cg_start(st, s->g, rsrc, s->r, s->x, s->scal, s->red, inner, target2));
         const float* zv = s->r;
         if (!ident) TRY(apply_precond(s, sa, &zv));
         float* pold = s->p0;
@@ -477,12 +473,11 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         it_total += hs.iters;
         if (hs.done == 2) { rr_true = std::nan(""); break; }
         // x64 (+)= x; r = b - A x64 in FP64
-        MGF_HIP(launch_refine(st, s->g, sa, s->markers, s->x, s->x64, s->b, s->r, s->red.partials,
-                              kResidualBlocks, sweeps > 0));
-        MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * 2 * kResidualBlocks,
-                               hipMemcpyDeviceToHost, st));
+        MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, s->x, s->x64, s->b, s->r, sweeps > 0 ? 2 : 1, s->red,
+                                   s->red.out2));
+        MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.out2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
         MGF_HIP(hipStreamSynchronize(st));
-        rr_true = host_sum(s->host_scratch, kResidualBlocks, 2, 0);
+        rr_true = s->host_scratch[0];
         ++sweeps;
         rsrc = s->r;
         if (rr_true <= target2) conv = true;
@@ -522,6 +517,7 @@ mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
     const int32_t K = ext_max_distance(s->g);
     TRY(dalloc(s, &s->layer, nface));
     TRY(dalloc(s, &s->list, nface));
+    TRY(dalloc(s, &s->lmask, nface));
     TRY(dalloc(s, &s->counts, ext_counts_size(K)));
     if (hipHostMalloc(reinterpret_cast<void**>(&s->host_offs), sizeof(uint32_t) * (size_t(K) + 3)) != hipSuccess) {
         (void)hipGetLastError();
@@ -540,7 +536,7 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     MGF_HIP(launch_ext_distance(st, s->g, u, v, w, s->layer));
     const uint32_t* offs = nullptr;
     const uint32_t* maxd = nullptr;
-    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, K, s->counts, s->list, &offs, &maxd));
+    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, K, s->counts, s->list, s->lmask, &offs, &maxd));
     // offsets of layers 0..cap+1, then the largest distance present
     MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -548,7 +544,7 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     const int32_t top = int32_t(s->host_offs[cap + 2]);
     for (int32_t L = 1; L <= top; ++L) {
         const uint32_t b = s->host_offs[L], e = s->host_offs[L + 1];
-        MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->layer, s->list, b, e - b, L));
+        MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->list, s->lmask, b, e - b));
     }
     *used = top;
     return MGF_OK;
@@ -694,8 +690,8 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
     A(&s->r, s->ncell); A(&s->p0, s->ncell); A(&s->p1, s->ncell); A(&s->ap, s->ncell);
     A(&s->codes, s->ncell);
     A(&s->scal, 1);
-    A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 4);
-    A(&s->red.ticket, 4);
+    A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2);
+    A(&s->red.out2, 2);
     if (st == MGF_OK) st = alloc_extrapolation(s, nface);
     if (st != MGF_OK) {
         mgf_scene_destroy(s);
@@ -707,11 +703,6 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
         mgf_scene_destroy(s);
         return fail(MGF_ERR_NOMEM, "hipHostMalloc failed");
     }
-    // reduction geometry of the fused kernels
-    size_t blocks = (s->ncell + kBlock - 1) / kBlock;
-    blocks = std::min<size_t>(std::max<size_t>(blocks, 8), kMaxBlocks);
-    blocks = (blocks + 7) / 8 * 8;
-    s->red.blocks = int(std::min<size_t>(blocks, kMaxBlocks));
     hipStream_t sm = ctx->stream;
     // resize() zero-fills every field (MPhysics/ScalarGrid.cpp:33-37)
     hipError_t e = hipSuccess;
@@ -724,7 +715,6 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
     F(s->solid_total, s->ncell);
     F(s->us, s->nfx); F(s->vs, s->nfy); F(s->ws, s->nfz);
     F(s->x, s->ncell); F(s->b, s->ncell);
-    F(reinterpret_cast<float*>(s->red.ticket), 4);
     if (e == hipSuccess) e = hipMemsetAsync(s->solid_mask, 0, s->ncell, sm);
     if (e == hipSuccess) e = hipMemsetAsync(s->markers, 0, s->ncell, sm);
     if (e == hipSuccess) e = hipStreamSynchronize(sm);
@@ -966,7 +956,7 @@ mgf_status mgf_scene_time_cg_kernels(mgf_scene s, int32_t iters, double* ms) {
     std::vector<hipEvent_t> ev(size_t(iters) * 6);
     for (auto& e : ev) MGF_HIP(hipEventCreate(&e));
     // target 0: never stops on its own
-    hipError_t err = launch_cg_init(st, s->g, s->b, s->r, nullptr, s->x, s->scal, s->red, 0.0, 0.0, 1);
+    hipError_t err = launch_cg_start(st, s->g, s->b, s->r, s->x, s->scal, s->red, 0.0, 0.0);
     const float* zv = s->r;
     mgf_status pst = MGF_OK;
     if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);

@@ -29,9 +29,8 @@ struct CgScalars {
 };
 
 struct ReduceBuf {
-    double* partials = nullptr;     // [kMaxBlocks * 4]
-    unsigned int* ticket = nullptr; // one counter per reduction site
-    int blocks = 0;                 // blocks used by the fused kernels
+    double* partials = nullptr;     // block partial sums (two per block at most)
+    double* out2 = nullptr;         // two finished sums (refinement residual, ||b||^2)
 };
 
 // One multigrid level: extents and the per-axis operator scales.
@@ -108,6 +107,7 @@ struct mgf_scene_s {
     // extrapolation scratch
     int32_t* layer = nullptr;      // distance of every face (nfx+nfy+nfz)
     uint32_t* list = nullptr;      // faces sorted by distance (nfx+nfy+nfz)
+    uint8_t* lmask = nullptr;      // previous-layer neighbour mask of each list entry
     uint32_t* counts = nullptr;    // counting-sort scratch [ext_counts_size(K)]
     uint32_t* host_offs = nullptr; // pinned mirror of the offsets and the max distance
 
@@ -175,16 +175,6 @@ hipError_t launch_axpy(hipStream_t s, float alpha, const float* x, float* y, siz
 hipError_t launch_scal_add(hipStream_t s, float beta, float* p, const float* z, size_t n);
 hipError_t launch_fill(hipStream_t s, float* a, size_t n, float v);
 hipError_t launch_fill64(hipStream_t s, double* a, size_t n, double v);
-// Fused mixed-precision CG.
-hipError_t launch_cg_init(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* p,
-                          float* x, CgScalars* sc, ReduceBuf red, double rel_inner,
-                          double abs_target2, int first_sweep);
-// x64 += x (FP32 correction), then r = b - A x64 in FP64 (stored FP32) and
-// ||r||^2 into partial sums (reduced by the host).
-hipError_t launch_refine(hipStream_t s, const GridDev& g, const float scale[3],
-                         const uint8_t* markers, const float* xcorr, double* x64,
-                         const float* b, float* r, double* partials, int blocks,
-                         int accumulate);
 // FP64 ||b - A x|| and ||b|| partials for an FP32 pressure.
 hipError_t launch_true_residual32(hipStream_t s, const GridDev& g, const float scale[3],
                                   const uint8_t* markers, const float* x, const float* b,
@@ -204,6 +194,16 @@ hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3]
 // x += alpha p, r -= alpha A p, r . r; identity != 0: beta = rr / rz, rz = rr.
 hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
                              const float* p, float* x, float* r, CgScalars* sc, ReduceBuf red, int identity);
+// Start of an inner sweep: r = rsrc, x = 0, the scalars from r.r (target =
+// max(rel_inner^2 r.r, abs_target2)).
+hipError_t launch_cg_start(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,
+                           ReduceBuf red, double rel_inner, double abs_target2);
+// Refinement: mode 0: x64 = 0; mode 1: x64 = xcorr; mode 2: x64 += xcorr; then
+// (modes 1, 2) r = b - A x64 in FP64, stored FP32 when r != null.  out2 (device)
+// receives ||b - A x64||^2 (0 in mode 0) and ||b||^2.
+hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
+                              const float* xcorr, double* x64, const float* b, float* r, int mode, ReduceBuf red,
+                              double* out2);
 hipError_t launch_mg_coarsen(hipStream_t s, const MgDims& f, const MgDims& c, const uint8_t* mf, uint8_t* mc,
                              int fluid_first);
 hipError_t launch_mg_smooth(hipStream_t s, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
@@ -232,15 +232,17 @@ int32_t ext_max_distance(const GridDev& g);
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
                                int32_t* dist);
 // Counting sort of the faces with 1 <= dist <= cap (cap <= K) into list, by
-// distance.  counts: ext_counts_size(K) entries of scratch; *offs receives
+// distance; lmask[k] = the mask of list[k]'s neighbours one layer below.  counts: ext_counts_size(K) entries of scratch; *offs receives
 // the device address of offs[0..cap+1] (offs[L] = first slot of distance L,
 // offs[cap+1] = total) and *maxd that of the largest distance present.
 size_t ext_counts_size(int32_t K);
 hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
-                           uint32_t* counts, uint32_t* list, const uint32_t** offs, const uint32_t** maxd);
-// Layer L: list[begin, begin+count) take the mean of their neighbours below L.
-hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const int32_t* dist,
-                            const uint32_t* list, uint32_t begin, uint32_t count, int32_t L);
+                           uint32_t* counts, uint32_t* list, uint8_t* lmask, const uint32_t** offs,
+                           const uint32_t** maxd);
+// One layer: list[begin, begin+count) take the mean of their neighbours in
+// the previous layer (lmask).
+hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                            const uint8_t* lmask, uint32_t begin, uint32_t count);
 // Boundaries.
 hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0,
                                  const float t[3], float* out);

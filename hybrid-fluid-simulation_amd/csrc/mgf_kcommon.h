@@ -1,5 +1,5 @@
 // mgf_kcommon.h — helpers shared by the kernel translation units: XCD-aware
-// block numbering, FP64 wave/block/grid reductions, launch wrapper.
+// block numbering, FP64 wave/block reductions, launch wrapper.
 #pragma once
 
 #include "mgf_internal.h"
@@ -53,43 +53,6 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
     }
     __syncthreads();
 }
-
-// One-pass grid reduction: every block stores its partials, the last block to
-// arrive (ticket) sums all partials in block order and returns true in all of
-// its threads; sums are then in out[] (valid in thread 0).
-template <int N>
-__device__ __forceinline__ bool grid_sum(double (&v)[N], double* partials, unsigned int* ticket,
-                                         double (&out)[N]) {
-    block_sum<N>(v);
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            __hip_atomic_store(&partials[size_t(blockIdx.x) * N + k], v[k], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) acc[k] = 0;
-    for (uint32_t bI = threadIdx.x; bI < gridDim.x; bI += kBlock) {
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            acc[k] += __hip_atomic_load(&partials[size_t(bI) * N + k], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<N>(acc);
-#pragma unroll
-    for (int k = 0; k < N; ++k) out[k] = acc[k];
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
-}
-
 
 template <class K, class... A>
 inline hipError_t launch_k(K kern, uint32_t grid, hipStream_t s, A... args) {

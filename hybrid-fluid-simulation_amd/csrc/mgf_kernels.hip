@@ -271,90 +271,6 @@ __global__ void __launch_bounds__(kBlock) k_u8_to_f32(const uint8_t* a, float* b
     if (i < n) b[i] = float(a[i]);
 }
 
-// ---------------------------------------------------------------------------
-// Fused mixed-precision CG.  Each block owns `chunk` consecutive cells.
-struct Span {
-    uint32_t lo, hi;
-};
-__device__ __forceinline__ Span block_span(uint32_t n) {
-    const uint32_t G = gridDim.x;
-    const uint32_t per = ((n + G - 1) / G + kBlock - 1) / kBlock * kBlock;
-    const uint32_t b = xcd_block();
-    const uint32_t lo = b * per;
-    uint32_t hi = lo + per;
-    if (lo > n) return {n, n};
-    if (hi > n) hi = n;
-    return {lo, hi};
-}
-
-// Sweep start: r from rsrc (b on the first sweep, the FP64-formed residual
-// afterwards), x = 0, p unused until the first iteration; sets rz = rr and
-// the stop target.
-__global__ void __launch_bounds__(kBlock) k_cg_init(uint32_t n, const float* __restrict__ rsrc,
-                                                    float* __restrict__ r, float* __restrict__ x,
-                                                    CgScalars* sc, double* partials,
-                                                    unsigned int* ticket, double rel_inner,
-                                                    double abs_target2) {
-    const Span sp = block_span(n);
-    double s[1] = {0.0};
-    for (uint32_t i = sp.lo + threadIdx.x; i < sp.hi; i += kBlock) {
-        const float v = rsrc[i];
-        if (r != rsrc) r[i] = v;
-        x[i] = 0.0f;
-        s[0] += double(v) * double(v);
-    }
-    double tot[1];
-    if (grid_sum<1>(s, partials, ticket, tot) && threadIdx.x == 0) {
-        sc->rz = tot[0];
-        sc->rr = tot[0];
-        sc->sweep_r0 = tot[0];
-        const double rel2 = rel_inner * rel_inner * tot[0];
-        sc->target2 = rel2 > abs_target2 ? rel2 : abs_target2;
-        sc->done = (tot[0] <= sc->target2 || tot[0] == 0.0) ? 1 : 0;
-        sc->iters = 0;
-        sc->beta = 0.0;
-        sc->alpha = 0.0;
-    }
-}
-
-// x64 (+)= x32 correction.
-__global__ void __launch_bounds__(kBlock) k_accumulate(uint32_t n, const float* __restrict__ xc,
-                                                       double* __restrict__ x64, int accumulate) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= n) return;
-    x64[i] = accumulate ? x64[i] + double(xc[i]) : double(xc[i]);
-}
-
-// r = b - A x64 in FP64, stored FP32; partials of ||r||^2 and ||b||^2.
-__global__ void __launch_bounds__(kBlock) k_residual64(GridDev g, const uint8_t* __restrict__ m,
-                                                       float sx, float sy, float sz,
-                                                       const double* __restrict__ x64,
-                                                       const float* __restrict__ b,
-                                                       float* __restrict__ r, double* partials) {
-    const uint32_t n = uint32_t(g.nx) * uint32_t(g.ny) * uint32_t(g.nz);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    double s[2] = {0.0, 0.0};
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
-        const Stencil st = stencil_at(m, x, y, z, nx, ny, nz, sx, sy, sz);
-        double ax = double(st.coef[0]) * x64[st.idx[0]];
-#pragma unroll
-        for (int q = 1; q < 7; ++q)
-            if (st.on[q]) ax += double(st.coef[q]) * x64[st.idx[q]];
-        const double bi = double(b[i]);
-        const double ri = bi - ax;
-        if (r) r[i] = float(ri);
-        s[0] += ri * ri;
-        s[1] += bi * bi;
-    }
-    block_sum<2>(s);
-    if (threadIdx.x == 0) {
-        partials[2 * size_t(blockIdx.x)] = s[0];
-        partials[2 * size_t(blockIdx.x) + 1] = s[1];
-    }
-}
 
 // FP64 ||b - A x||^2 and ||b||^2 partials for an FP32 pressure.
 __global__ void __launch_bounds__(kBlock) k_residual32(GridDev g, const uint8_t* __restrict__ m,
@@ -633,28 +549,6 @@ hipError_t launch_fill64(hipStream_t s, double* a, size_t n, double v) {
 
 hipError_t launch_u8_to_f32(hipStream_t s, const uint8_t* a, float* b, size_t n) {
     return LAUNCH(k_u8_to_f32, grid_for(n), a, b, n);
-}
-
-hipError_t launch_cg_init(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* p,
-                          float* x, CgScalars* sc, ReduceBuf red, double rel_inner,
-                          double abs_target2, int first_sweep) {
-    (void)p;
-    (void)first_sweep;
-    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
-    return LAUNCH(k_cg_init, red.blocks, n, rsrc, r, x, sc, red.partials, red.ticket, rel_inner,
-                  abs_target2);
-}
-
-hipError_t launch_refine(hipStream_t s, const GridDev& g, const float scale[3],
-                         const uint8_t* markers, const float* xcorr, double* x64, const float* b,
-                         float* r, double* partials, int blocks, int accumulate) {
-    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
-    if (xcorr) {
-        hipError_t e = LAUNCH(k_accumulate, grid_for(n), n, xcorr, x64, accumulate);
-        if (e != hipSuccess) return e;
-    }
-    return LAUNCH(k_residual64, blocks, g, markers, scale[0], scale[1], scale[2], x64, b, r,
-                  partials);
 }
 
 hipError_t launch_true_residual32(hipStream_t s, const GridDev& g, const float scale[3],

@@ -16,7 +16,8 @@
 //      block, one global atomic per block and distance into one of 32
 //      private copies);
 //   3. one gather kernel per layer, over that layer's faces only, in layer
-//      order: the neighbours in layers below L are final when layer L runs.
+//      order: the neighbours in layers below L are final when layer L runs
+//      (the sort also stores each face's mask of previous-layer neighbours).
 //
 // The mean is summed in the reference's neighbour order (-x, +x, -y, +y, -z,
 // +z) in FP32 and divided once, so the values are bitwise the reference's.
@@ -286,9 +287,35 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_ext_scatter(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
-                                                        uint32_t K2, const uint32_t* __restrict__ poff,
-                                                        uint32_t* cursor, uint32_t* __restrict__ list) {
+// Bit q of the lower-neighbour mask of a face: its in-grid neighbour in
+// direction q (-x, +x, -y, +y, -z, +z) has a smaller distance, i.e. lies in
+// the previous layer.
+__device__ __forceinline__ uint32_t lower_mask(const Faces& F, const int32_t* __restrict__ dist, uint32_t gi,
+                                               int32_t dv) {
+    const int c = comp_of(F.off, gi);
+    const uint32_t o = pick4(F.off, c);
+    const uint32_t i = gi - o;
+    const uint32_t ex = pick(F.ex, c), ey = pick(F.ey, c), ez = pick(F.ez, c), sz = ex * ey;
+    const uint32_t z = udiv_r(i, sz, pick(F.rexy, c));
+    const uint32_t rem = i - z * sz;
+    const uint32_t y = udiv_r(rem, ex, pick(F.rex, c));
+    const uint32_t x = rem - y * ex;
+    const bool in[6] = {x > 0, x + 1 < ex, y > 0, y + 1 < ey, z > 0, z + 1 < ez};
+    const uint32_t j[6] = {in[0] ? gi - 1 : gi, in[1] ? gi + 1 : gi, in[2] ? gi - ex : gi,
+                           in[3] ? gi + ex : gi, in[4] ? gi - sz : gi, in[5] ? gi + sz : gi};
+    int32_t dn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) dn[q] = dist[j[q]];
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) m |= (in[q] && dn[q] < dv) ? (1u << q) : 0u;
+    return m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const int32_t* __restrict__ dist, uint32_t total,
+                                                        int32_t cap, uint32_t K2, const uint32_t* __restrict__ poff,
+                                                        uint32_t* cursor, uint32_t* __restrict__ list,
+                                                        uint8_t* __restrict__ lmask) {
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
@@ -316,50 +343,50 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(const int32_t* __restric
     for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
         if (h[b]) h[b] = atomicAdd(&cu[b], h[b]);
     __syncthreads();
+    uint32_t msk[kSortCpt];
+#pragma unroll
+    for (int k = 0; k < kSortCpt; ++k) {
+        const uint32_t i = base + uint32_t(k) * kBlock;
+        msk[k] = lower_mask(F, dist, i < total ? i : total - 1, dv[k]);
+    }
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
         if (dv[k] >= 1 && dv[k] <= cap) {
             const uint32_t pos = po[dv[k]] + (dv[k] < kLdsBins ? h[dv[k]] : 0u) + rank[k];
             list[pos] = i;
+            lmask[pos] = uint8_t(msk[k]);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// 3. Layer L: each listed face takes the mean of its in-grid neighbours with
-// distance below L.
-__global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const int32_t* __restrict__ dist,
-                                                      const uint32_t* __restrict__ list, uint32_t begin,
-                                                      uint32_t count, int32_t L) {
+// 3. Layer L: each listed face takes the mean of its neighbours in the
+// previous layer (lower-neighbour mask from the sort); the others are read at
+// the face itself and left out by the mask, so every load of a work-item is
+// issued at once and no distance is read.
+__global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const uint32_t* __restrict__ list,
+                                                      const uint8_t* __restrict__ lmask, uint32_t begin,
+                                                      uint32_t count) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= count) return;
     const uint32_t gi = list[begin + t];
+    const uint32_t m = lmask[begin + t];
     const int c = comp_of(F.off, gi);
     const uint32_t i = gi - pick4(F.off, c);
-    const uint32_t ex = pick(F.ex, c), ey = pick(F.ey, c), ez = pick(F.ez, c), sz = ex * ey;
-    const uint32_t z = udiv_r(i, sz, pick(F.rexy, c));
-    const uint32_t rem = i - z * sz;
-    const uint32_t y = udiv_r(rem, ex, pick(F.rex, c));
-    const uint32_t x = rem - y * ex;
-    const int32_t* dc = dist + pick4(F.off, c);
+    const uint32_t ex = pick(F.ex, c), sz = ex * pick(F.ey, c);
     float* f = pick(F.f, c);
-    const bool in[6] = {x > 0, x + 1 < ex, y > 0, y + 1 < ey, z > 0, z + 1 < ez};
-    const uint32_t j[6] = {in[0] ? i - 1 : i, in[1] ? i + 1 : i, in[2] ? i - ex : i,
-                           in[3] ? i + ex : i, in[4] ? i - sz : i, in[5] ? i + sz : i};
-    int32_t dn[6];
+    const uint32_t j[6] = {(m & 1u) ? i - 1 : i, (m & 2u) ? i + 1 : i, (m & 4u) ? i - ex : i,
+                           (m & 8u) ? i + ex : i, (m & 16u) ? i - sz : i, (m & 32u) ? i + sz : i};
     float fv[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        dn[q] = dc[j[q]];
-        fv[q] = f[j[q]];
-    }
+    for (int q = 0; q < 6; ++q) fv[q] = f[j[q]];
     // the reference's conditional sums as selects (an unused neighbour leaves
     // the sum untouched, a -0.0 included)
     float acc = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-        const bool use = in[q] && dn[q] < L;
+        const bool use = (m >> q) & 1u;
         acc = use ? acc + fv[q] : acc;
         cnt = use ? cnt + 1.0f : cnt;
     }
@@ -404,7 +431,8 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
 size_t ext_counts_size(int32_t K) { return (3 * size_t(kCopies) + 1) * (size_t(K) + 2) + 1; }
 
 hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
-                           uint32_t* counts, uint32_t* list, const uint32_t** offs, const uint32_t** maxd) {
+                           uint32_t* counts, uint32_t* list, uint8_t* lmask, const uint32_t** offs,
+                           const uint32_t** maxd) {
     const Faces F = faces_of(g, nullptr, nullptr, nullptr);
     const uint32_t total = F.off[3];
     const uint32_t K2 = uint32_t(K) + 2;
@@ -423,17 +451,17 @@ hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist,
     hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, B.hist, cap, K2, B.offs, B.poff, B.cursor, B.maxd);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, K2, B.poff,
-                       B.cursor, list);
+    hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, F, dist, total, cap, K2, B.poff,
+                       B.cursor, list, lmask);
     return hipGetLastError();
 }
 
-hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const int32_t* dist,
-                            const uint32_t* list, uint32_t begin, uint32_t count, int32_t L) {
+hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                            const uint8_t* lmask, uint32_t begin, uint32_t count) {
     if (count == 0) return hipSuccess;
     const Faces F = faces_of(g, u, v, w);
-    hipLaunchKernelGGL(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist, list, begin,
-                       count, L);
+    hipLaunchKernelGGL(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, list, lmask, begin,
+                       count);
     return hipGetLastError();
 }
 

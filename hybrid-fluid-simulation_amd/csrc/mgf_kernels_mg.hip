@@ -284,9 +284,162 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Flat F, const uint16_t* __
     block_partial(acc, part);
 }
 
+// Refinement (outer sweeps of the mixed solver).
+// MODE 0: x64 = 0; partials of ||b||^2.
+// MODE 1: x64 = xc (the FP32 correction of the first sweep; the neighbours'
+// new values come from xc, x64 is only written); MODE 2: x64 read only (the
+// correction was added by k_acc64 before).  Then r = b - A x64 formed in FP64
+// and stored FP32 when r != null; partials of ||r||^2 and ||b||^2.  (Reading
+// x64 at the neighbours while updating it in place would race.)
+__device__ __forceinline__ double arow64(uint16_t cd, double vc, const double (&v)[6], double sx, double sy,
+                                         double sz, const float* aw) {
+    auto keep = [&](double d, int b) { return ((cd >> b) & 1) ? d : 0.0; };
+    const double ex = keep(vc - v[0], kCplShift + 0) + keep(vc - v[1], kCplShift + 1);
+    const double ey = keep(vc - v[2], kCplShift + 2) + keep(vc - v[3], kCplShift + 3);
+    const double ez = keep(vc - v[4], kCplShift + 4) + keep(vc - v[5], kCplShift + 5);
+    const double a = double(aw[air_mask(cd)]) * vc + sx * ex + sy * ey + sz * ez;
+    return (cd & kCodeFluid) ? a : vc;
+}
+
+template <int MODE, bool CHECK>
+__device__ __forceinline__ void refine_body(const Flat& F, uint32_t base, const uint16_t* __restrict__ code,
+                                            float sx, float sy, float sz, const float* __restrict__ xc,
+                                            double* __restrict__ x64, const float* __restrict__ b,
+                                            float* __restrict__ r, const float* aw, double& rr, double& bb) {
+    MGF_CELLS(CHECK, F, base)
+    constexpr int K = kCpt;
+    uint16_t cd[K];
+    float bc[K], xcc[K], xcn[K][6];
+    double xo[K], xon[K][6];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        bc[k] = b[cidx[k]];
+        if (MODE != 0) {
+            uint32_t j[6];
+            nbrs(cidx[k], F, j);
+            cd[k] = code[cidx[k]];
+            if (MODE == 1) {
+                xcc[k] = xc[cidx[k]];
+#pragma unroll
+                for (int m = 0; m < 6; ++m) xcn[k][m] = xc[j[m]];
+            } else {
+                xo[k] = x64[cidx[k]];
+#pragma unroll
+                for (int m = 0; m < 6; ++m) xon[k][m] = x64[j[m]];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (CHECK && c >= F.n) continue;
+        bb += double(bc[k]) * double(bc[k]);
+        if (MODE == 0) {
+            x64[c] = 0.0;
+            continue;
+        }
+        double vc, v[6];
+        if (MODE == 1) {
+            vc = double(xcc[k]);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) v[m] = double(xcn[k][m]);
+            x64[c] = vc;
+        } else {
+            vc = xo[k];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) v[m] = xon[k][m];
+        }
+        const double res = double(bc[k]) - arow64(cd[k], vc, v, sx, sy, sz, aw);
+        if (r) r[c] = float(res);
+        rr += res * res;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_refine(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
+                                                   float sz, const float* __restrict__ xc, double* __restrict__ x64,
+                                                   const float* __restrict__ b, float* __restrict__ r,
+                                                   double* __restrict__ part) {
+    __shared__ float aw[64];
+    build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (b0 + kBlock * kCpt <= F.n)
+        refine_body<MODE, false>(F, base, code, sx, sy, sz, xc, x64, b, r, aw, v[0], v[1]);
+    else
+        refine_body<MODE, true>(F, base, code, sx, sy, sz, xc, x64, b, r, aw, v[0], v[1]);
+    block_sum<2>(v);
+    if (threadIdx.x == 0) {
+        part[2 * size_t(blockIdx.x)] = v[0];
+        part[2 * size_t(blockIdx.x) + 1] = v[1];
+    }
+}
+
+// x64 += xc, pointwise.
+__global__ void __launch_bounds__(kBlock) k_acc64(uint32_t n, const float* __restrict__ xc, double* __restrict__ x64) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i < n) x64[i] = x64[i] + double(xc[i]);
+}
+
+// Start of an inner sweep: r = rsrc (when distinct), x = 0; partials of r.r.
+template <bool CHECK>
+__device__ __forceinline__ double start_body(const Flat& F, uint32_t base, const float* __restrict__ rsrc,
+                                             float* __restrict__ r, float* __restrict__ x) {
+    MGF_CELLS(CHECK, F, base)
+    float v[kCpt];
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) v[k] = rsrc[cidx[k]];
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const uint32_t c = base + uint32_t(k) * kBlock;
+        if (CHECK && c >= F.n) continue;
+        if (r != rsrc) r[c] = v[k];
+        x[c] = 0.0f;
+        acc += double(v[k]) * double(v[k]);
+    }
+    return acc;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_start(Flat F, const float* __restrict__ rsrc, float* r,
+                                                     float* __restrict__ x, double* __restrict__ part) {
+    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n) ? start_body<false>(F, base, rsrc, r, x)
+                                                   : start_body<true>(F, base, rsrc, r, x);
+    block_partial(acc, part);
+}
+
+// Sums `count` pairs of partials (fixed order) into out[0], out[1].
+__global__ void __launch_bounds__(1024) k_sum2(const double* __restrict__ part, uint32_t count,
+                                               double* __restrict__ out) {
+    double s[2] = {0.0, 0.0};
+    for (uint32_t i = threadIdx.x; i < count; i += 1024) {
+        s[0] += part[2 * size_t(i)];
+        s[1] += part[2 * size_t(i) + 1];
+    }
+    __shared__ double sh[2][16];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double w = wave_sum(s[q]);
+        if ((threadIdx.x & 63) == 0) sh[q][threadIdx.x >> 6] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            double t = 0.0;
+            for (int w = 0; w < 16; ++w) t += sh[q][w];
+            out[q] = t;
+        }
+    }
+}
+
 // The one-block finishing kernel: sums `count` partials in a fixed order and
 // updates the scalars.
 enum FinishKind : int {
+    kFinishInit = 4,      // r.r -> rz, rr, sweep_r0, target2, done; iters, alpha, beta = 0
     kFinishAlpha = 0,     // pap -> alpha
     kFinishRrIdent = 1,   // rr -> iters, done; beta = rr / rz, rz = rr (identity)
     kFinishRr = 2,        // rr -> iters, done (preconditioned: beta comes later)
@@ -295,8 +448,9 @@ enum FinishKind : int {
 constexpr int kFinishBlock = 1024;
 
 __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restrict__ part, uint32_t count,
-                                                         CgScalars* sc, int kind) {
-    if (sc->done) return;
+                                                         CgScalars* sc, int kind, double rel_inner,
+                                                         double abs_target2) {
+    if (kind != kFinishInit && sc->done) return;
     // four independent chains: the loads of a thread are in flight together
     double s4[4] = {0.0, 0.0, 0.0, 0.0};
     uint32_t i = threadIdx.x;
@@ -315,6 +469,18 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restric
 #pragma unroll
     for (int q = 0; q < kFinishBlock / 64; ++q) t += sh[q];
     switch (kind) {
+        case kFinishInit: {
+            sc->rz = t;
+            sc->rr = t;
+            sc->sweep_r0 = t;
+            const double rel2 = rel_inner * rel_inner * t;
+            sc->target2 = rel2 > abs_target2 ? rel2 : abs_target2;
+            sc->done = (t <= sc->target2 || t == 0.0) ? 1 : 0;
+            sc->iters = 0;
+            sc->beta = 0.0;
+            sc->alpha = 0.0;
+            break;
+        }
         case kFinishAlpha:
             sc->pap = t;
             if (!isfinite(t)) { sc->done = 2; sc->alpha = 0.0; }
@@ -773,8 +939,38 @@ hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* ma
     return LAUNCH(k_codes, grid_for(n), nx, ny, nz, markers, codes, n);
 }
 
-static hipError_t finish(hipStream_t s, const double* part, uint32_t count, CgScalars* sc, int kind) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, part, count, sc, kind);
+static hipError_t finish(hipStream_t s, const double* part, uint32_t count, CgScalars* sc, int kind,
+                         double rel_inner = 0.0, double abs_target2 = 0.0) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, part, count, sc, kind, rel_inner, abs_target2);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_start(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,
+                           ReduceBuf red, double rel_inner, double abs_target2) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    hipError_t e = LAUNCH(k_cg_start, G, F, rsrc, r, x, red.partials);
+    if (e != hipSuccess) return e;
+    return finish(s, red.partials, G, sc, kFinishInit, rel_inner, abs_target2);
+}
+
+hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
+                              const float* xcorr, double* x64, const float* b, float* r, int mode, ReduceBuf red,
+                              double* out2) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    hipError_t e;
+    if (mode == 0)
+        e = LAUNCH(k_refine<0>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
+    else if (mode == 1)
+        e = LAUNCH(k_refine<1>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
+    else {
+        e = LAUNCH(k_acc64, grid_for(F.n), F.n, xcorr, x64);
+        if (e != hipSuccess) return e;
+        e = LAUNCH(k_refine<2>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, red.partials, G, out2);
     return hipGetLastError();
 }
 
