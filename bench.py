@@ -22,10 +22,11 @@ sys.path.insert(0, os.path.join(ROOT, "hybrid-fluid-simulation_amd", "python"))
 METRIC = "cell-steps/s at 256³ (div resid ≤1e-6) vs plain FP32; p50/p99 batched step latency"
 HBM_PEAK_GBPS = 8000.0
 # Algorithmic bytes per padded cell of the fused CG kernels (FP32 = 4 B,
-# uint16 stencil code = 2 B): p/Ap reads z, p_old, code and writes p_new; the
-# update reads x, p, r, code and writes x, r.
-BYTES_P_AP = 14
-BYTES_UPDATE = 22
+# uint16 stencil code = 2 B).  K_I (p/Ap): reads z, code, p, Ap and writes p,
+# Ap (A z from the stencil of z, p and Ap updated in place); K_II (update,
+# pointwise): reads p, Ap, x, r and writes x, r.
+BYTES_P_AP = 22
+BYTES_UPDATE = 24
 PRECONDS = {"identity": 0, "jacobi": 1, "multigrid": 2}
 
 
@@ -213,7 +214,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_col_update (x += alpha p, r -= alpha A p, r.r)",
+            "kernel": "k_cg_update (x += alpha p, r -= alpha Ap, r.r)",
             "achieved": gbs_upd,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
@@ -221,7 +222,7 @@ def main():
             "traffic": None,
             "bytes_per_launch": BYTES_UPDATE * ncell_padded,
             "ms_per_launch": ms_upd,
-            "other": {"k_col_p_ap": {"ms": ms_pap, "GBps": gbs_pap, "bytes_per_launch": BYTES_P_AP * ncell_padded},
+            "other": {"k_cg_pap": {"ms": ms_pap, "GBps": gbs_pap, "bytes_per_launch": BYTES_P_AP * ncell_padded},
                       "preconditioner": {"name": args.precond, "ms": ms_pc, "GBps": gbs_pc,
                                          "bytes_per_application": pcb * ncell_padded}},
         },

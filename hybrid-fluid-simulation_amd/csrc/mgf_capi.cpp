@@ -278,7 +278,7 @@ mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
 // ---------------------------------------------------------------------------
 // Preconditioners of the MIXED solver.
 constexpr int kCoarsestSweeps = 8;
-constexpr size_t kCoarseCells = 5000;    // levels this small run in one work-group
+constexpr size_t kCoarseCells = 1000;    // levels this small run in one work-group
 constexpr int kCoarseLevelsMax = 8;
 constexpr size_t kCoarseLdsMax = 65536;
 
@@ -454,17 +454,14 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         MGF_HIP(launch_cg_start(st, s->g, rsrc, s->r, s->x, s->scal, s->red, inner, target2));
         const float* zv = s->r;
         if (!ident) TRY(apply_precond(s, sa, &zv));
-        float* pold = s->p0;
-        float* pnew = s->p1;
+        // p and A p (p0, p1) are updated in place by the fused kernels
         for (;;) {
             const int64_t left = max_it - it_total;
             const int k = int(std::min<int64_t>(chunk, std::max<int64_t>(left, 1)));
             for (int q = 0; q < k; ++q) {
-                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, pold, pnew, s->scal, s->red));
-                MGF_HIP(launch_col_update(st, s->g, sa, s->codes, pnew, s->x, s->r, s->scal, s->red,
-                                          ident ? 1 : 0));
+                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red));
+                MGF_HIP(launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0));
                 if (!ident) TRY(apply_precond(s, sa, &zv));
-                std::swap(pold, pnew);
             }
             MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
             MGF_HIP(hipStreamSynchronize(st));
@@ -960,21 +957,18 @@ mgf_status mgf_scene_time_cg_kernels(mgf_scene s, int32_t iters, double* ms) {
     const float* zv = s->r;
     mgf_status pst = MGF_OK;
     if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);
-    float* pold = s->p0;
-    float* pnew = s->p1;
     for (int i = 0; i < iters && err == hipSuccess && pst == MGF_OK; ++i) {
         hipEvent_t* e = &ev[6 * size_t(i)];
         err = hipEventRecord(e[0], st);
-        if (err == hipSuccess) err = launch_col_p_ap(st, s->g, sa, s->codes, zv, pold, pnew, s->scal, s->red);
+        if (err == hipSuccess) err = launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red);
         if (err == hipSuccess) err = hipEventRecord(e[1], st);
         if (err == hipSuccess) err = hipEventRecord(e[2], st);
         if (err == hipSuccess)
-            err = launch_col_update(st, s->g, sa, s->codes, pnew, s->x, s->r, s->scal, s->red, ident ? 1 : 0);
+            err = launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0);
         if (err == hipSuccess) err = hipEventRecord(e[3], st);
         if (err == hipSuccess) err = hipEventRecord(e[4], st);
         if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);
         if (err == hipSuccess) err = hipEventRecord(e[5], st);
-        std::swap(pold, pnew);
     }
     if (err == hipSuccess) err = hipStreamSynchronize(st);
     double a = 0, b = 0, c = 0;

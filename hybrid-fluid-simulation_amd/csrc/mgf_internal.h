@@ -188,12 +188,13 @@ hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* ma
 // Blocks of the column kernels on grid g (the partial-sum buffer must hold
 // four doubles per block).
 uint32_t col_blocks(const GridDev& g);
-// p_new = z + beta p_old (z on the first iteration), p . A p, alpha.
+// K_I: p = z + beta p, Ap = A z + beta Ap in place (p = z, Ap = A z on the
+// first iteration), p . Ap, alpha.
 hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
-                           const float* zv, const float* p_old, float* p_new, CgScalars* sc, ReduceBuf red);
-// x += alpha p, r -= alpha A p, r . r; identity != 0: beta = rr / rz, rz = rr.
-hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
-                             const float* p, float* x, float* r, CgScalars* sc, ReduceBuf red, int identity);
+                           const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red);
+// K_II: x += alpha p, r -= alpha Ap, r . r; identity != 0: beta = rr / rz, rz = rr.
+hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float* p, const float* ap, float* x, float* r,
+                             CgScalars* sc, ReduceBuf red, int identity);
 // Start of an inner sweep: r = rsrc, x = 0, the scalars from r.r (target =
 // max(rel_inner^2 r.r, abs_target2)).
 hipError_t launch_cg_start(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,

@@ -169,16 +169,18 @@ constexpr int kCpt = 4;
         cidx[k] = (CHECK && c_ >= (F).n) ? (F).n - 1 : c_;                         \
     }
 
-// K_I: p_new = z + beta p_old (p_new = z on the first iteration), with the
-// neighbours' p_new recomputed from z and p_old; block partials of p . A p.
+// K_I: Az = A z (the stencil), then in place p = z + beta p and
+// Ap = Az + beta Ap (p = z, Ap = Az on the first iteration); block partials
+// of p . Ap.  Only z is read at the neighbours, so p and Ap are updated in
+// place without a race.
 template <bool CHECK>
 __device__ __forceinline__ double cg_pap_body(const Flat& F, uint32_t base, const uint16_t* __restrict__ code,
                                               float sx, float sy, float sz, const float* __restrict__ zv,
-                                              const float* __restrict__ p_old, float* __restrict__ p_new,
-                                              float beta, bool first, const float* aw) {
+                                              float* __restrict__ p, float* __restrict__ ap, float beta,
+                                              bool first, const float* aw) {
     MGF_CELLS(CHECK, F, base)
     uint16_t cd[kCpt];
-    float zc[kCpt], pc[kCpt], zn[kCpt][6], pn[kCpt][6];
+    float zc[kCpt], zn[kCpt][6], pc[kCpt], apc[kCpt];
 #pragma unroll
     for (int k = 0; k < kCpt; ++k) {
         uint32_t j[6];
@@ -188,36 +190,30 @@ __device__ __forceinline__ double cg_pap_body(const Flat& F, uint32_t base, cons
 #pragma unroll
         for (int m = 0; m < 6; ++m) zn[k][m] = zv[j[m]];
         if (!first) {
-            pc[k] = p_old[cidx[k]];
-#pragma unroll
-            for (int m = 0; m < 6; ++m) pn[k][m] = p_old[j[m]];
+            pc[k] = p[cidx[k]];
+            apc[k] = ap[cidx[k]];
         }
     }
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kCpt; ++k) {
-        float vc = zc[k], v[6];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) v[m] = zn[k][m];
-        if (!first) {
-            vc = __builtin_fmaf(beta, pc[k], vc);
-#pragma unroll
-            for (int m = 0; m < 6; ++m) v[m] = __builtin_fmaf(beta, pn[k][m], v[m]);
-        }
-        const float a = arow(cd[k], vc, v, sx, sy, sz, aw);
+        const float az = arow(cd[k], zc[k], zn[k], sx, sy, sz, aw);
+        const float pn = first ? zc[k] : __builtin_fmaf(beta, pc[k], zc[k]);
+        const float an = first ? az : __builtin_fmaf(beta, apc[k], az);
         const uint32_t c = base + uint32_t(k) * kBlock;
         if (!CHECK || c < F.n) {
-            p_new[c] = vc;
-            acc += double(vc) * double(a);
+            p[c] = pn;
+            ap[c] = an;
+            acc += double(pn) * double(an);
         }
     }
     return acc;
 }
 
 __global__ void __launch_bounds__(kBlock) k_cg_pap(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
-                                                   float sz, const float* __restrict__ zv,
-                                                   const float* __restrict__ p_old, float* __restrict__ p_new,
-                                                   const CgScalars* __restrict__ sc, double* __restrict__ part) {
+                                                   float sz, const float* __restrict__ zv, float* __restrict__ p,
+                                                   float* __restrict__ ap, const CgScalars* __restrict__ sc,
+                                                   double* __restrict__ part) {
     __shared__ float aw[64];
     if (sc->done) return;
     build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
@@ -226,38 +222,31 @@ __global__ void __launch_bounds__(kBlock) k_cg_pap(Flat F, const uint16_t* __res
     const uint32_t b0 = xcd_block() * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
     const double acc = (b0 + kBlock * kCpt <= F.n)
-                           ? cg_pap_body<false>(F, base, code, sx, sy, sz, zv, p_old, p_new, beta, first, aw)
-                           : cg_pap_body<true>(F, base, code, sx, sy, sz, zv, p_old, p_new, beta, first, aw);
+                           ? cg_pap_body<false>(F, base, code, sx, sy, sz, zv, p, ap, beta, first, aw)
+                           : cg_pap_body<true>(F, base, code, sx, sy, sz, zv, p, ap, beta, first, aw);
     block_partial(acc, part);
 }
 
-// K_II: A p recomputed from the stored p; x += alpha p; r -= alpha A p (each
-// formed in FP64, stored FP32); block partials of r . r.
+// K_II (pointwise): x += alpha p; r -= alpha Ap (each formed in FP64, stored
+// FP32); block partials of r . r.
 template <bool CHECK>
-__device__ __forceinline__ double cg_update_body(const Flat& F, uint32_t base, const uint16_t* __restrict__ code,
-                                                 float sx, float sy, float sz, const float* __restrict__ p,
-                                                 float* __restrict__ x, float* __restrict__ r, double alpha,
-                                                 const float* aw) {
+__device__ __forceinline__ double cg_update_body(const Flat& F, uint32_t base, const float* __restrict__ p,
+                                                 const float* __restrict__ ap, float* __restrict__ x,
+                                                 float* __restrict__ r, double alpha) {
     MGF_CELLS(CHECK, F, base)
-    uint16_t cd[kCpt];
-    float pc[kCpt], pn[kCpt][6], x0[kCpt], r0[kCpt];
+    float pc[kCpt], ac[kCpt], x0[kCpt], r0[kCpt];
 #pragma unroll
     for (int k = 0; k < kCpt; ++k) {
-        uint32_t j[6];
-        nbrs(cidx[k], F, j);
-        cd[k] = code[cidx[k]];
         pc[k] = p[cidx[k]];
-#pragma unroll
-        for (int m = 0; m < 6; ++m) pn[k][m] = p[j[m]];
+        ac[k] = ap[cidx[k]];
         x0[k] = x[cidx[k]];
         r0[k] = r[cidx[k]];
     }
     double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kCpt; ++k) {
-        const float a = arow(cd[k], pc[k], pn[k], sx, sy, sz, aw);
         const float xn = float(__builtin_fma(alpha, double(pc[k]), double(x0[k])));
-        const float rn = float(__builtin_fma(-alpha, double(a), double(r0[k])));
+        const float rn = float(__builtin_fma(-alpha, double(ac[k]), double(r0[k])));
         const uint32_t c = base + uint32_t(k) * kBlock;
         if (!CHECK || c < F.n) {
             x[c] = xn;
@@ -268,19 +257,16 @@ __device__ __forceinline__ double cg_update_body(const Flat& F, uint32_t base, c
     return acc;
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_update(Flat F, const uint16_t* __restrict__ code, float sx,
-                                                      float sy, float sz, const float* __restrict__ p,
-                                                      float* __restrict__ x, float* __restrict__ r,
-                                                      const CgScalars* __restrict__ sc, double* __restrict__ part) {
-    __shared__ float aw[64];
+__global__ void __launch_bounds__(kBlock) k_cg_update(Flat F, const float* __restrict__ p,
+                                                      const float* __restrict__ ap, float* __restrict__ x,
+                                                      float* __restrict__ r, const CgScalars* __restrict__ sc,
+                                                      double* __restrict__ part) {
     if (sc->done) return;
-    build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
     const double alpha = sc->alpha;
     const uint32_t b0 = xcd_block() * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
-    const double acc = (b0 + kBlock * kCpt <= F.n)
-                           ? cg_update_body<false>(F, base, code, sx, sy, sz, p, x, r, alpha, aw)
-                           : cg_update_body<true>(F, base, code, sx, sy, sz, p, x, r, alpha, aw);
+    const double acc = (b0 + kBlock * kCpt <= F.n) ? cg_update_body<false>(F, base, p, ap, x, r, alpha)
+                                                   : cg_update_body<true>(F, base, p, ap, x, r, alpha);
     block_partial(acc, part);
 }
 
@@ -778,6 +764,51 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Fla
     if (ok) bc[c] = (codec[c] & kCodeFluid) ? sum * weight : 0.0f;
 }
 
+// Restriction when every axis is halved: one work-item per fine cell.  A
+// block covers 64 fine cells along x in each of the 2 x 2 rows (y, z) of one
+// coarse row: each wave computes its row's residuals, sums x-pairs with a
+// lane shuffle, and 32 work-items add the four rows' pair sums (a fixed
+// order) into the 32 coarse cells.
+__global__ void __launch_bounds__(kBlock) k_mg_restrict2(Flat FF, MgLevel Fd, Flat FC, MgLevel Cd, uint32_t gx,
+                                                         const uint16_t* __restrict__ codef,
+                                                         const float* __restrict__ bf, const float* __restrict__ xf,
+                                                         const uint16_t* __restrict__ codec, float* __restrict__ bc,
+                                                         float weight, const CgScalars* __restrict__ sc) {
+    __shared__ float aw[64];
+    __shared__ float sh[4][32];
+    if (sc->done) return;
+    build_tables<false>(aw, nullptr, Fd.sx, Fd.sy, Fd.sz, 0.0f);
+    const uint32_t blk = xcd_block();
+    const uint32_t xcb = blk % gx;                  // 64-cell chunk along x
+    const uint32_t YZ = blk / gx;
+    const uint32_t Y = YZ % uint32_t(Cd.ny), Z = YZ / uint32_t(Cd.ny);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t x = xcb * 64 + uint32_t(lane);
+    const uint32_t y = 2 * Y + uint32_t(wv & 1), z = 2 * Z + uint32_t(wv >> 1);
+    const bool in = Z < uint32_t(Cd.nz) && x < uint32_t(Fd.nx) && y < uint32_t(Fd.ny) && z < uint32_t(Fd.nz);
+    const uint32_t f = in ? (z * FF.sz + y * FF.sy + x) : 0u;
+    uint32_t j[6];
+    nbrs(f, FF, j);
+    const uint16_t cd = codef[f];
+    const float bv = bf[f];
+    const float vc = xf[f];
+    float v[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) v[m] = xf[j[m]];
+    float res = (in && (cd & kCodeFluid)) ? bv - arow(cd, vc, v, Fd.sx, Fd.sy, Fd.sz, aw) : 0.0f;
+    res += __shfl_xor(res, 1, 64);
+    if ((lane & 1) == 0) sh[wv][lane >> 1] = res;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const uint32_t X = xcb * 32 + threadIdx.x;
+        if (Z < uint32_t(Cd.nz) && X < uint32_t(Cd.nx)) {
+            const float sum = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+            const uint32_t c = Z * FC.sz + Y * FC.sy + X;
+            bc[c] = (codec[c] & kCodeFluid) ? sum * weight : 0.0f;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The V-cycle on the smallest levels in one work-group, iterates in LDS.
 // Level 0 of the argument list takes its right-hand side from global memory
@@ -801,12 +832,12 @@ struct CoarseArgs {
     CoarseLevel lv[kCoarseMaxLevels];
 };
 
-__device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* aw, const float* w, const float* b,
-                                          const float* xi, float* xo, bool first) {
+__device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const uint16_t* code, const float* aw,
+                                          const float* w, const float* b, const float* xi, float* xo, bool first) {
     const uint32_t n = uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz);
     const Flat F{n, uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
     for (uint32_t c = threadIdx.x; c < n; c += kCoarseThreads) {
-        const uint16_t cd = L.code[c];
+        const uint16_t cd = code[c];
         const float wd = wdiag(cd, w);
         const float bc = b[c];
         float out;
@@ -826,50 +857,62 @@ __device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const float* aw,
     __syncthreads();
 }
 
+// LDS layout (floats): per level the tables aw[64], w[64]; then per level the
+// codes (uint16, each level padded to an even count); then level 0: x, t;
+// deeper levels: b, x, t.
 __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A, const CgScalars* __restrict__ sc) {
-    __shared__ float tab_aw[kCoarseMaxLevels][64], tab_w[kCoarseMaxLevels][64];
     if (sc->done) return;
-    for (int l = 0; l < A.nlev; ++l) {
-        if (threadIdx.x < 64) {
-            const float d = mask_scale_sum(threadIdx.x, A.lv[l].sx, A.lv[l].sy, A.lv[l].sz);
-            tab_aw[l][threadIdx.x] = d;
-            tab_w[l][threadIdx.x] = d != 0.0f ? A.omega / d : 0.0f;
-        }
-    }
     extern __shared__ float lds[];
+    float* tab = lds;
+    uint16_t* codes[kCoarseMaxLevels];
     float* xs[kCoarseMaxLevels];
     float* ts[kCoarseMaxLevels];
     const float* bs[kCoarseMaxLevels];
-    // carve LDS: level 0: x, t; deeper levels: b, x, t
-    float* q = lds;
+    uint32_t ns[kCoarseMaxLevels];
+    float* q = lds + 128 * A.nlev;
     for (int l = 0; l < A.nlev; ++l) {
-        const uint32_t n = uint32_t(A.lv[l].nx) * uint32_t(A.lv[l].ny) * uint32_t(A.lv[l].nz);
+        ns[l] = uint32_t(A.lv[l].nx) * uint32_t(A.lv[l].ny) * uint32_t(A.lv[l].nz);
+        codes[l] = reinterpret_cast<uint16_t*>(q);
+        q += (ns[l] + 1) / 2;
+    }
+    for (int l = 0; l < A.nlev; ++l) {
         if (l == 0) {
             bs[0] = A.b0;
         } else {
             bs[l] = q;
-            q += n;
+            q += ns[l];
         }
-        xs[l] = q; q += n;
-        ts[l] = q; q += n;
+        xs[l] = q; q += ns[l];
+        ts[l] = q; q += ns[l];
+    }
+    // tables and codes of every level
+    for (int l = 0; l < A.nlev; ++l) {
+        if (threadIdx.x < 64) {
+            const float d = mask_scale_sum(threadIdx.x, A.lv[l].sx, A.lv[l].sy, A.lv[l].sz);
+            tab[128 * l + threadIdx.x] = d;
+            tab[128 * l + 64 + threadIdx.x] = d != 0.0f ? A.omega / d : 0.0f;
+        }
+        for (uint32_t c = threadIdx.x; c < ns[l]; c += kCoarseThreads) codes[l][c] = A.lv[l].code[c];
     }
     __syncthreads();
     // descent
     for (int l = 0; l < A.nlev; ++l) {
         const CoarseLevel& L = A.lv[l];
+        const float* aw = tab + 128 * l;
+        const float* w = aw + 64;
         const int sweeps = l == A.nlev - 1 ? A.last_sweeps : A.nu;
-        lds_sweep(L, tab_aw[l], tab_w[l], bs[l], xs[l], xs[l], true);
+        lds_sweep(L, codes[l], aw, w, bs[l], xs[l], xs[l], true);
         for (int k = 1; k < sweeps; ++k) {
-            lds_sweep(L, tab_aw[l], tab_w[l], bs[l], xs[l], ts[l], false);
+            lds_sweep(L, codes[l], aw, w, bs[l], xs[l], ts[l], false);
             float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
         }
         if (l == A.nlev - 1) break;
         // restriction of the residual into level l+1
         const CoarseLevel& C = A.lv[l + 1];
-        const uint32_t nc = uint32_t(C.nx) * uint32_t(C.ny) * uint32_t(C.nz);
+        const uint32_t nc = ns[l + 1];
         const int fx = L.nx > 1 ? 2 : 1, fy = L.ny > 1 ? 2 : 1, fz = L.nz > 1 ? 2 : 1;
         const float weight = 1.0f / float(fx * fy * fz);
-        const Flat FF{uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz), uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
+        const Flat FF{ns[l], uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
         float* bc = const_cast<float*>(bs[l + 1]);
         const float* xf = xs[l];
         const float* bf = bs[l];
@@ -883,16 +926,16 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
                         const uint32_t x = X * fx + dx, y = Y * fy + dy, z = Z * fz + dz;
                         if (x >= uint32_t(L.nx) || y >= uint32_t(L.ny) || z >= uint32_t(L.nz)) continue;
                         const uint32_t f = z * FF.sz + y * FF.sy + x;
-                        const uint16_t cd = L.code[f];
+                        const uint16_t cd = codes[l][f];
                         if (!(cd & kCodeFluid)) continue;
                         uint32_t j[6];
                         nbrs(f, FF, j);
                         float v[6];
 #pragma unroll
                         for (int m = 0; m < 6; ++m) v[m] = xf[j[m]];
-                        sum += bf[f] - arow(cd, xf[f], v, L.sx, L.sy, L.sz, tab_aw[l]);
+                        sum += bf[f] - arow(cd, xf[f], v, L.sx, L.sy, L.sz, aw);
                     }
-            bc[c] = (C.code[c] & kCodeFluid) ? sum * weight : 0.0f;
+            bc[c] = (codes[l + 1][c] & kCodeFluid) ? sum * weight : 0.0f;
         }
         __syncthreads();
     }
@@ -900,25 +943,24 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
     for (int l = A.nlev - 2; l >= 0; --l) {
         const CoarseLevel& L = A.lv[l];
         const CoarseLevel& C = A.lv[l + 1];
-        const uint32_t n = uint32_t(L.nx) * uint32_t(L.ny) * uint32_t(L.nz);
+        const float* aw = tab + 128 * l;
+        const float* w = aw + 64;
         const int hx = L.nx > 1 ? 1 : 0, hy = L.ny > 1 ? 1 : 0, hz = L.nz > 1 ? 1 : 0;
         const float* xc = xs[l + 1];
         float* xf = xs[l];
-        for (uint32_t c = threadIdx.x; c < n; c += kCoarseThreads) {
+        for (uint32_t c = threadIdx.x; c < ns[l]; c += kCoarseThreads) {
             const uint32_t x = c % uint32_t(L.nx), y = (c / uint32_t(L.nx)) % uint32_t(L.ny),
                            z = c / (uint32_t(L.nx) * uint32_t(L.ny));
-            if (L.code[c] & kCodeFluid)
+            if (codes[l][c] & kCodeFluid)
                 xf[c] += xc[((z >> hz) * uint32_t(C.ny) + (y >> hy)) * uint32_t(C.nx) + (x >> hx)];
         }
         __syncthreads();
         for (int k = 0; k < A.nu; ++k) {
-            lds_sweep(L, tab_aw[l], tab_w[l], bs[l], xs[l], ts[l], false);
+            lds_sweep(L, codes[l], aw, w, bs[l], xs[l], ts[l], false);
             float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
         }
     }
-    const CoarseLevel& L0 = A.lv[0];
-    const uint32_t n0 = uint32_t(L0.nx) * uint32_t(L0.ny) * uint32_t(L0.nz);
-    for (uint32_t c = threadIdx.x; c < n0; c += kCoarseThreads) A.x0[c] = xs[0][c];
+    for (uint32_t c = threadIdx.x; c < ns[0]; c += kCoarseThreads) A.x0[c] = xs[0][c];
 }
 
 // ===========================================================================
@@ -975,19 +1017,19 @@ hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale
 }
 
 hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
-                           const float* zv, const float* p_old, float* p_new, CgScalars* sc, ReduceBuf red) {
+                           const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
     const uint32_t G = flat_grid(F.n);
-    hipError_t e = LAUNCH(k_cg_pap, G, F, codes, scale[0], scale[1], scale[2], zv, p_old, p_new, sc, red.partials);
+    hipError_t e = LAUNCH(k_cg_pap, G, F, codes, scale[0], scale[1], scale[2], zv, p, ap, sc, red.partials);
     if (e != hipSuccess) return e;
     return finish(s, red.partials, G, sc, kFinishAlpha);
 }
 
-hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
-                             const float* p, float* x, float* r, CgScalars* sc, ReduceBuf red, int identity) {
+hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float* p, const float* ap, float* x, float* r,
+                             CgScalars* sc, ReduceBuf red, int identity) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
     const uint32_t G = flat_grid(F.n);
-    hipError_t e = LAUNCH(k_cg_update, G, F, codes, scale[0], scale[1], scale[2], p, x, r, sc, red.partials);
+    hipError_t e = LAUNCH(k_cg_update, G, F, p, ap, x, r, sc, red.partials);
     if (e != hipSuccess) return e;
     return finish(s, red.partials, G, sc, identity ? kFinishRrIdent : kFinishRr);
 }
@@ -1048,13 +1090,12 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
 }
 
 size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev) {
-    size_t floats = 0;
+    size_t floats = 128 * size_t(nlev);
     for (int l = 0; l < nlev; ++l) {
         const size_t n = size_t(levels[l].nx) * levels[l].ny * levels[l].nz;
-        floats += (l == 0 ? 2 : 3) * n;
+        floats += (n + 1) / 2 + (l == 0 ? 2 : 3) * n;
     }
-    // + the static tables of the kernel
-    return floats * sizeof(float) + size_t(2) * kCoarseMaxLevels * 64 * sizeof(float);
+    return floats * sizeof(float);
 }
 
 hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
@@ -1073,8 +1114,8 @@ hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uin
         A.lv[l].sx = levels[l].sx; A.lv[l].sy = levels[l].sy; A.lv[l].sz = levels[l].sz;
         A.lv[l].code = codes[l];
     }
-    const size_t dyn = mg_coarse_lds_bytes(levels, nlev) - size_t(2) * kCoarseMaxLevels * 64 * sizeof(float);
-    hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(kCoarseThreads), dyn, s, A, sc);
+    hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(kCoarseThreads), mg_coarse_lds_bytes(levels, nlev), s, A,
+                       sc);
     return hipGetLastError();
 }
 
@@ -1083,6 +1124,11 @@ hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, c
                               const CgScalars* sc) {
     const Flat FC = make_flat(c.nx, c.ny, c.nz), FF = make_flat(f.nx, f.ny, f.nz);
     const int slots = (f.nx > 1 ? 2 : 1) * (f.ny > 1 ? 2 : 1) * (f.nz > 1 ? 2 : 1);
+    if (slots == 8) {
+        const uint32_t gx = (uint32_t(f.nx) + 63) / 64;
+        return LAUNCH(k_mg_restrict2, size_t(gx) * c.ny * c.nz + (8 - (size_t(gx) * c.ny * c.nz) % 8) % 8, FF, f,
+                      FC, c, gx, codef, bf, xf, codec, bc, 0.125f, sc);
+    }
     return LAUNCH(k_mg_restrict, grid_for(FC.n), FC, c, FF, f, codef, bf, xf, codec, bc, 1.0f / float(slots),
                   1.0 / double(FC.sy), 1.0 / double(FC.sz), sc);
 }
