@@ -152,6 +152,14 @@ def main():
         log("mixed: %d steps in %.3fs, %d converged, CG iters/step %.1f, sweeps/step %.2f, max resid %.2e"
             % (args.steps, wall, r["converged"], r["iters"].mean(), r["sweeps"].mean(), r["res"].max()))
         value = spec.interior_cells * r["converged"] / wall
+        # per-stage breakdown: a few more steps with the stage clock on (a
+        # synchronisation between stages), outside the timed region
+        s.set_timing(True)
+        stg = s.step(3, allow_unconverged=True)
+        s.set_timing(False)
+        stages = {k: stg[k] / 3.0 for k in ("ms_advect", "ms_pressure", "ms_extrapolate", "ms_total")}
+        log("stages (ms/step): advect+forces %.2f, pressure %.2f, extrapolation %.2f, total %.2f"
+            % (stages["ms_advect"], stages["ms_pressure"], stages["ms_extrapolate"], stages["ms_total"]))
         # live kernel timing (HIP events on the scene's stream)
         ms_pap, ms_upd, ms_pc = s.time_cg_kernels(args.cg_timing_iters)
         gbs_pap = BYTES_P_AP * ncell_padded / (ms_pap * 1e-3) / 1e9
@@ -229,6 +237,7 @@ def main():
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),
         "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
                        "note": "single-scene step latency (batched scenes not built yet)"},
+        "stages_ms_per_step": stages,
         "cg_iterations_per_step": k,
         "refinement_sweeps_per_step": rs,
         "residual_rel_max": float(r["res"].max()),
