@@ -425,7 +425,17 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     const bool ident = d.preconditioner == MGF_PRECOND_IDENTITY;
     const double inner = d.inner_tolerance > 0 ? d.inner_tolerance : 1e-3;
     const int max_sweeps = d.max_refinements > 0 ? d.max_refinements : 8;
-    const int chunk = d.preconditioner == MGF_PRECOND_MULTIGRID ? 4 : (s->ncell < (1u << 20) ? 32 : 16);
+    // iterations launched between two reads of the scalars: predicted from the
+    // observed contraction of r.r (converged iterations still cost their
+    // launches), bounded by `chunk`
+    const int chunk = d.preconditioner == MGF_PRECOND_MULTIGRID ? 16 : (s->ncell < (1u << 20) ? 32 : 16);
+    const int first_chunk = d.preconditioner == MGF_PRECOND_MULTIGRID ? 4 : chunk;
+    double log_rho = 0.0;   // log of the per-iteration factor of r.r, 0 = unknown
+    auto predict = [&](double rr_now, double target) {
+        if (!(log_rho < 0.0) || !(rr_now > target) || !(target > 0.0)) return first_chunk;
+        const double n = std::ceil(std::log(target / rr_now) / log_rho);
+        return int(std::max(1.0, std::min(double(chunk), n)));
+    };
     MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
     if (!ident) {
         TRY(ensure_precond(s));
@@ -454,10 +464,15 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         MGF_HIP(launch_cg_start(st, s->g, rsrc, s->r, s->x, s->scal, s->red, inner, target2));
         const float* zv = s->r;
         if (!ident) TRY(apply_precond(s, sa, &zv));
+        // r.r at the sweep start and the sweep's target, as the device sets them
+        const double rr0 = rr_true;
+        const double sweep_target = std::max(inner * inner * rr0, target2);
+        double rr_now = rr0;
+        int done_before = 0;
         // p and A p (p0, p1) are updated in place by the fused kernels
         for (;;) {
-            const int64_t left = max_it - it_total;
-            const int k = int(std::min<int64_t>(chunk, std::max<int64_t>(left, 1)));
+            const int64_t left = max_it - it_total - done_before;
+            const int k = int(std::min<int64_t>(predict(rr_now, sweep_target), std::max<int64_t>(left, 1)));
             for (int q = 0; q < k; ++q) {
                 MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red));
                 MGF_HIP(launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0));
@@ -465,6 +480,10 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
             }
             MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
             MGF_HIP(hipStreamSynchronize(st));
+            if (hs.iters > 0 && hs.rr > 0.0 && rr0 > 0.0 && hs.rr < rr0)
+                log_rho = std::log(hs.rr / rr0) / double(hs.iters);
+            rr_now = hs.rr;
+            done_before = hs.iters;
             if (hs.done || it_total + hs.iters >= max_it) break;
         }
         it_total += hs.iters;
