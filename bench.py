@@ -47,6 +47,23 @@ def precond_bytes(name, sweeps=3):
     return lvl0 * 8.0 / 7.0
 
 
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE passes, calibrated against a streaming
+    kernel of known size; see the summary file), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_obstacles256.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch") and kernel.split()[0] in line:
+                try:
+                    return float(line.split("=")[-1])
+                except ValueError:
+                    return None
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -121,7 +138,6 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cg-timing-iters", type=int, default=40)
     ap.add_argument("--precond", choices=sorted(PRECONDS), default="multigrid")
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--mg-sweeps", type=int, default=3)
@@ -152,22 +168,32 @@ def main():
         log("mixed: %d steps in %.3fs, %d converged, CG iters/step %.1f, sweeps/step %.2f, max resid %.2e"
             % (args.steps, wall, r["converged"], r["iters"].mean(), r["sweeps"].mean(), r["res"].max()))
         value = spec.interior_cells * r["converged"] / wall
-        # per-stage breakdown: a few more steps with the stage clock on (a
-        # synchronisation between stages), outside the timed region
+        # per-stage breakdown and kernel probes: a few more steps with the stage
+        # clock on (a synchronisation between stages) and HIP events around
+        # the probed kernels' launches, outside the timed region
+        probe_steps = 3
         s.set_timing(True)
-        stg = s.step(3, allow_unconverged=True)
+        s.set_kernel_probes(True)
+        s.kernel_probes(reset=True)
+        stg = s.step(probe_steps, allow_unconverged=True)
+        probes = s.kernel_probes(reset=True)
+        s.set_kernel_probes(False)
         s.set_timing(False)
-        stages = {k: stg[k] / 3.0 for k in ("ms_advect", "ms_pressure", "ms_extrapolate", "ms_total")}
+        stages = {k: stg[k] / probe_steps for k in ("ms_advect", "ms_pressure", "ms_extrapolate", "ms_total")}
         log("stages (ms/step): advect+forces %.2f, pressure %.2f, extrapolation %.2f, total %.2f"
             % (stages["ms_advect"], stages["ms_pressure"], stages["ms_extrapolate"], stages["ms_total"]))
-        # live kernel timing (HIP events on the scene's stream)
-        ms_pap, ms_upd, ms_pc = s.time_cg_kernels(args.cg_timing_iters)
-        gbs_pap = BYTES_P_AP * ncell_padded / (ms_pap * 1e-3) / 1e9
-        gbs_upd = BYTES_UPDATE * ncell_padded / (ms_upd * 1e-3) / 1e9
+        ktab = {}
+        for name, p in probes.items():
+            if p["launches"] > 0 and p["ms"] > 0:
+                ktab[name] = {"ms_per_launch": p["ms"] / p["launches"], "launches_per_step": p["launches"] / probe_steps,
+                              "ms_per_step": p["ms"] / probe_steps,
+                              "bytes_per_launch": p["bytes"] / p["launches"],
+                              "GBps": p["bytes"] / (p["ms"] * 1e-3) / 1e9}
+        for name, v in sorted(ktab.items(), key=lambda kv: -kv[1]["ms_per_step"]):
+            log("  %-22s %8.4f ms/launch %6.1f launches/step %7.3f ms/step %6.0f GB/s"
+                % (name, v["ms_per_launch"], v["launches_per_step"], v["ms_per_step"], v["GBps"]))
+        dominant = max(ktab.items(), key=lambda kv: kv[1]["ms_per_step"])
         pcb = precond_bytes(args.precond, args.mg_sweeps)
-        gbs_pc = pcb * ncell_padded / (ms_pc * 1e-3) / 1e9 if ms_pc > 0 else None
-        log("cg kernels: p/Ap %.4f ms (%.0f GB/s), update %.4f ms (%.0f GB/s), preconditioner %.4f ms (%s GB/s)"
-            % (ms_pap, gbs_pap, ms_upd, gbs_upd, ms_pc, "%.0f" % gbs_pc if gbs_pc else "-"))
         s.close()
 
         plain = None
@@ -222,17 +248,16 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_update (x += alpha p, r -= alpha Ap, r.r)",
-            "achieved": gbs_upd,
+            "kernel": dominant[0],
+            "achieved": dominant[1]["GBps"],
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
-            "frac": gbs_upd / HBM_PEAK_GBPS,
-            "traffic": None,
-            "bytes_per_launch": BYTES_UPDATE * ncell_padded,
-            "ms_per_launch": ms_upd,
-            "other": {"k_cg_pap": {"ms": ms_pap, "GBps": gbs_pap, "bytes_per_launch": BYTES_P_AP * ncell_padded},
-                      "preconditioner": {"name": args.precond, "ms": ms_pc, "GBps": gbs_pc,
-                                         "bytes_per_application": pcb * ncell_padded}},
+            "frac": dominant[1]["GBps"] / HBM_PEAK_GBPS,
+            "traffic": traffic_for(dominant[0]),
+            "bytes_per_launch": dominant[1]["bytes_per_launch"],
+            "ms_per_launch": dominant[1]["ms_per_launch"],
+            "note": "HIP events around the kernel's launches in %d ordinary steps after the timed ones" % probe_steps,
+            "kernels": ktab,
         },
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),
         "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),

@@ -113,8 +113,60 @@ void scales(const mgf_scene_s* s, float sa[3], float sg[3]) {
     }
 }
 
+// ---- kernel probes: HIP event pairs around selected launches, read at the
+// next synchronisation of a stage entry point.
+int probe_begin(mgf_scene s) {
+    if (!s->probes) return -1;
+    const size_t k = s->probe_used;
+    if (s->probe_ev.size() < 2 * (k + 1)) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            (void)hipGetLastError();
+            if (a) (void)hipEventDestroy(a);
+            return -1;
+        }
+        s->probe_ev.push_back(a);
+        s->probe_ev.push_back(b);
+        s->probe_kind.push_back(0);
+        s->probe_bytes.push_back(0.0);
+    }
+    if (hipEventRecord(s->probe_ev[2 * k], s->ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    return int(k);
+}
+
+void probe_end(mgf_scene s, int k, int kind, double bytes) {
+    if (k < 0) return;
+    if (hipEventRecord(s->probe_ev[2 * size_t(k) + 1], s->ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    s->probe_kind[size_t(k)] = kind;
+    s->probe_bytes[size_t(k)] = bytes;
+    s->probe_used = size_t(k) + 1;
+}
+
+// The stream is synchronised.
+void probe_collect(mgf_scene s) {
+    for (size_t k = 0; k < s->probe_used; ++k) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, s->probe_ev[2 * k], s->probe_ev[2 * k + 1]) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        const int kind = s->probe_kind[k];
+        s->probe_ms[kind] += t;
+        s->probe_b[kind] += s->probe_bytes[k];
+        s->probe_n[kind] += 1;
+    }
+    s->probe_used = 0;
+}
+
 mgf_status sync(mgf_scene s) {
     MGF_HIP(hipStreamSynchronize(s->ctx->stream));
+    probe_collect(s);
     return MGF_OK;
 }
 
@@ -184,8 +236,10 @@ mgf_status advect(mgf_scene s, bool with_forces) {
         addf = on(f);
     }
     Vel V = vel_of(s->u[c], s->v[c], s->w[c]);
+    const int pk = probe_begin(s);
     MGF_HIP(launch_advect_fc(st, s->g, V, V, dt, s->desc.rk_order, s->u[n], s->v[n], s->w[n], addg,
                              g, addf, f));
+    probe_end(s, pk, MGF_PROBE_ADVECT_FC, 12.0 * double(s->nfx + s->nfy + s->nfz));
     MGF_HIP(launch_advect_cc(st, s->g, s->phi[c], V, dt, s->desc.rk_order, s->phi[n]));
     s->cur = n;
     return MGF_OK;
@@ -374,20 +428,28 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         const int sweeps = coarsest ? kCoarsestSweeps : nu;
         auto rz_at = [&](int k) { return (l == 0 && coarsest && k == sweeps - 1) ? 1 : 0; };
         int k;
+        const double ncell = double(v.d.nx) * v.d.ny * v.d.nz;
         if (sweeps >= 2) {
+            const int pk = l == 0 ? probe_begin(s) : -1;
             MGF_HIP(launch_mg_smooth2_first(st, v.d, v.codes, v.b, v.x, omega, s->scal, s->red, rz_at(1)));
+            probe_end(s, pk, MGF_PROBE_MG_SMOOTH2, 10.0 * ncell);
             k = 2;
         } else {
             MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.b, v.x, omega, s->scal, s->red, 1, rz_at(0)));
             k = 1;
         }
         for (; k < sweeps; ++k) {
+            const int pk = (l == 0 && !rz_at(k)) ? probe_begin(s) : -1;
             MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k)));
+            probe_end(s, pk, MGF_PROBE_MG_SMOOTH, 14.0 * ncell);
             std::swap(v.x, v.t);
         }
-        if (!coarsest)
+        if (!coarsest) {
+            const int pk = l == 0 ? probe_begin(s) : -1;
             MGF_HIP(launch_mg_restrict(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
                                        const_cast<float*>(lv[l + 1].b), s->scal));
+            probe_end(s, pk, MGF_PROBE_MG_RESTRICT, 10.5 * ncell);
+        }
     }
     if (l0 < L) {
         MgDims dims[kCoarseLevelsMax];
@@ -404,11 +466,16 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     for (int l = std::min(L - 2, l0 - 1); l >= 0; --l) {
         Lv& v = lv[l];
         auto rz_at = [&](int k) { return (l == 0 && k == nu - 1) ? 1 : 0; };
+        const double ncell = double(v.d.nx) * v.d.ny * v.d.nz;
+        int pk = (l == 0 && !rz_at(0)) ? probe_begin(s) : -1;
         MGF_HIP(launch_mg_smooth_prolong(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
                                          s->red, rz_at(0)));
+        probe_end(s, pk, MGF_PROBE_MG_PROLONG, 14.5 * ncell);
         std::swap(v.x, v.t);
         for (int k = 1; k < nu; ++k) {
+            pk = (l == 0 && !rz_at(k)) ? probe_begin(s) : -1;
             MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k)));
+            probe_end(s, pk, MGF_PROBE_MG_SMOOTH, 14.0 * ncell);
             std::swap(v.x, v.t);
         }
     }
@@ -474,8 +541,12 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
             const int64_t left = max_it - it_total - done_before;
             const int k = int(std::min<int64_t>(predict(rr_now, sweep_target), std::max<int64_t>(left, 1)));
             for (int q = 0; q < k; ++q) {
+                int pk = probe_begin(s);
                 MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red));
+                probe_end(s, pk, MGF_PROBE_CG_PAP, 22.0 * double(s->ncell));
+                pk = probe_begin(s);
                 MGF_HIP(launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0));
+                probe_end(s, pk, MGF_PROBE_CG_UPDATE, 24.0 * double(s->ncell));
                 if (!ident) TRY(apply_precond(s, sa, &zv));
             }
             MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
@@ -560,7 +631,10 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     const int32_t top = int32_t(s->host_offs[cap + 2]);
     for (int32_t L = 1; L <= top; ++L) {
         const uint32_t b = s->host_offs[L], e = s->host_offs[L + 1];
+        const int pk = probe_begin(s);
         MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->list, s->lmask, b, e - b));
+        // list entry 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
+        probe_end(s, pk, MGF_PROBE_EXT_LAYER, 13.0 * double(e - b));
     }
     *used = top;
     return MGF_OK;
@@ -750,6 +824,7 @@ mgf_status mgf_scene_destroy(mgf_scene s) {
     for (auto& so : s->solids) (void)hipFree(so.sdf0);
     if (s->host_scratch) (void)hipHostFree(s->host_scratch);
     if (s->host_offs) (void)hipHostFree(s->host_offs);
+    for (hipEvent_t e : s->probe_ev) (void)hipEventDestroy(e);
     delete s;
     return MGF_OK;
 }
@@ -955,61 +1030,26 @@ mgf_status mgf_scene_set_timing(mgf_scene s, int32_t enabled) {
     return MGF_OK;
 }
 
-mgf_status mgf_scene_time_cg_kernels(mgf_scene s, int32_t iters, double* ms) {
-    if (!s || iters < 1 || !ms) return fail(MGF_ERR_INVALID, "bad argument");
-    if (s->desc.precision != MGF_PRECISION_MIXED)
-        return fail(MGF_ERR_INVALID, "the fused kernels belong to MIXED precision");
+mgf_status mgf_scene_set_kernel_probes(mgf_scene s, int32_t enabled) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    s->probes = enabled != 0;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_kernel_probes(mgf_scene s, double* ms, int64_t* launches, double* bytes, int32_t reset) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
     MGF_HIP(hipSetDevice(s->ctx->device));
-    hipStream_t st = s->ctx->stream;
-    float sa[3], sg[3];
-    scales(s, sa, sg);
-    const bool ident = s->desc.preconditioner == MGF_PRECOND_IDENTITY;
-    MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
-    if (!ident) {
-        TRY(ensure_precond(s));
-        TRY(mg_setup(s, sa));
+    TRY(sync(s));
+    for (int k = 0; k < MGF_PROBE_COUNT; ++k) {
+        if (ms) ms[k] = s->probe_ms[k];
+        if (launches) launches[k] = s->probe_n[k];
+        if (bytes) bytes[k] = s->probe_b[k];
+        if (reset) {
+            s->probe_ms[k] = 0.0;
+            s->probe_b[k] = 0.0;
+            s->probe_n[k] = 0;
+        }
     }
-    std::vector<hipEvent_t> ev(size_t(iters) * 6);
-    for (auto& e : ev) MGF_HIP(hipEventCreate(&e));
-    // target 0: never stops on its own
-    hipError_t err = launch_cg_start(st, s->g, s->b, s->r, s->x, s->scal, s->red, 0.0, 0.0);
-    const float* zv = s->r;
-    mgf_status pst = MGF_OK;
-    if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);
-    for (int i = 0; i < iters && err == hipSuccess && pst == MGF_OK; ++i) {
-        hipEvent_t* e = &ev[6 * size_t(i)];
-        err = hipEventRecord(e[0], st);
-        if (err == hipSuccess) err = launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red);
-        if (err == hipSuccess) err = hipEventRecord(e[1], st);
-        if (err == hipSuccess) err = hipEventRecord(e[2], st);
-        if (err == hipSuccess)
-            err = launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0);
-        if (err == hipSuccess) err = hipEventRecord(e[3], st);
-        if (err == hipSuccess) err = hipEventRecord(e[4], st);
-        if (err == hipSuccess && !ident) pst = apply_precond(s, sa, &zv);
-        if (err == hipSuccess) err = hipEventRecord(e[5], st);
-    }
-    if (err == hipSuccess) err = hipStreamSynchronize(st);
-    double a = 0, b = 0, c = 0;
-    for (int i = 0; i < iters && err == hipSuccess && pst == MGF_OK; ++i) {
-        const hipEvent_t* e = &ev[6 * size_t(i)];
-        float t0 = 0, t1 = 0, t2 = 0;
-        err = hipEventElapsedTime(&t0, e[0], e[1]);
-        if (err == hipSuccess) err = hipEventElapsedTime(&t1, e[2], e[3]);
-        if (err == hipSuccess) err = hipEventElapsedTime(&t2, e[4], e[5]);
-        a += t0;
-        b += t1;
-        c += t2;
-    }
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    if (pst != MGF_OK) return pst;
-    if (err != hipSuccess) return fail(MGF_ERR_DEVICE, std::string("time_cg: ") + hipGetErrorString(err));
-    CgScalars hs{};
-    MGF_HIP(hipMemcpy(&hs, s->scal, sizeof(hs), hipMemcpyDeviceToHost));
-    if (hs.done) return fail(MGF_ERR_INVALID, "the system converged before the timed iterations ended");
-    ms[0] = a / iters;
-    ms[1] = b / iters;
-    ms[2] = ident ? 0.0 : c / iters;
     return MGF_OK;
 }
 

@@ -115,6 +115,15 @@ struct mgf_scene_s {
     bool boundary_dirty = false;   // solids changed since the last refresh
     bool boundary_user = false;    // SOLID_SDF / SOLID_VEL uploaded directly
     bool timing = false;
+    // kernel probes: event pairs recorded around selected launches
+    bool probes = false;
+    std::vector<hipEvent_t> probe_ev;   // 2 per pair
+    std::vector<int> probe_kind;
+    std::vector<double> probe_bytes;
+    size_t probe_used = 0;              // pairs recorded since the last collection
+    double probe_ms[MGF_PROBE_COUNT] = {};
+    double probe_b[MGF_PROBE_COUNT] = {};
+    int64_t probe_n[MGF_PROBE_COUNT] = {};
 
     int64_t device_bytes = 0;
     std::vector<void*> allocations;

@@ -19,6 +19,9 @@ RK1, RK2, RK3 = 1, 2, 3
 TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
+PROBE_COUNT = 8  # MGF_PROBE_COUNT
+PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "k_mg_smooth_prolong",
+               "k_mg_restrict", "k_ext_layer", "k_advect_fc")
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
              solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
              divergence=12)
@@ -90,7 +93,9 @@ _SIGS = {
     "mgf_step": [_VP, C.c_int32, C.POINTER(StepStats)],
     "mgf_scene_sync": [_VP],
     "mgf_scene_set_timing": [_VP, C.c_int32],
-    "mgf_scene_time_cg_kernels": [_VP, C.c_int32, C.POINTER(C.c_double)],
+    "mgf_scene_set_kernel_probes": [_VP, C.c_int32],
+    "mgf_scene_kernel_probes": [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                C.c_int32],
     "mgf_device_alloc": [_VP, C.c_int64, C.POINTER(C.c_void_p)],
     "mgf_device_free": [_VP, _VP],
     "mgf_memcpy_h2d": [_VP, _VP, _VP, C.c_int64],
@@ -302,12 +307,18 @@ class Scene:
     def sync(self):
         check(lib().mgf_scene_sync(self.h), "mgf_scene_sync")
 
-    def time_cg_kernels(self, iters=50):
-        """Mean ms of the p/Ap kernel, the x/r update kernel and the
-        preconditioner application (0 with the identity)."""
-        ms = (C.c_double * 3)()
-        check(lib().mgf_scene_time_cg_kernels(self.h, iters, ms), "mgf_scene_time_cg_kernels")
-        return ms[0], ms[1], ms[2]
+    def set_kernel_probes(self, on):
+        check(lib().mgf_scene_set_kernel_probes(self.h, 1 if on else 0), "mgf_scene_set_kernel_probes")
+
+    def kernel_probes(self, reset=True):
+        """Per probed kernel kind: summed ms, launches and algorithmic bytes
+        (HIP events on the scene's stream around the launches of ordinary
+        steps while probes are on)."""
+        ms = (C.c_double * PROBE_COUNT)()
+        n = (C.c_int64 * PROBE_COUNT)()
+        by = (C.c_double * PROBE_COUNT)()
+        check(lib().mgf_scene_kernel_probes(self.h, ms, n, by, 1 if reset else 0), "mgf_scene_kernel_probes")
+        return {name: {"ms": ms[i], "launches": n[i], "bytes": by[i]} for i, name in enumerate(PROBE_NAMES)}
 
     def set_timing(self, on):
         check(lib().mgf_scene_set_timing(self.h, 1 if on else 0), "mgf_scene_set_timing")

@@ -216,15 +216,25 @@ MGF_API mgf_status mgf_scene_sync(mgf_scene scene);
 /* Host wall-clock timing of stages in mgf_step (adds synchronisation). */
 MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
 
-/* Measurement hook (no counterpart in the reference): runs `iters` fused CG
- * iterations of the MIXED solver, with the scene's preconditioner, on the
- * scene's last pressure system (markers and b of the last solve) with no
- * stopping test, the p/Ap kernel, the x/r update kernel and the whole
- * preconditioner application each bracketed by HIP events on the scene's
- * stream.  ms[0], ms[1], ms[2] receive their mean durations in ms (ms[2] = 0
- * with the identity).  The pressure and CG work vectors are overwritten;
- * call it after the steps being measured. */
-MGF_API mgf_status mgf_scene_time_cg_kernels(mgf_scene scene, int32_t iters, double* ms);
+/* Kernel probes (measurement; no counterpart in the reference).  While
+ * enabled, the launches of the kinds below are bracketed by HIP events on the
+ * scene's stream during ordinary steps and stage calls.  mgf_scene_kernel_probes
+ * returns, per kind, the summed duration in ms, the number of launches and
+ * the algorithmic bytes of those launches (each value read or written once;
+ * padded-grid cells), accumulated since the last reset.  Arrays have
+ * MGF_PROBE_COUNT entries; any pointer may be null. */
+#define MGF_PROBE_CG_PAP      0  /* p / A p kernel with its scalar finish      */
+#define MGF_PROBE_CG_UPDATE   1  /* x / r update kernel with its scalar finish */
+#define MGF_PROBE_MG_SMOOTH   2  /* level-0 Jacobi sweep (not the last one)    */
+#define MGF_PROBE_MG_SMOOTH2  3  /* level-0 first two sweeps from zero          */
+#define MGF_PROBE_MG_PROLONG  4  /* level-0 sweep with the prolongation        */
+#define MGF_PROBE_MG_RESTRICT 5  /* level-0 restriction                        */
+#define MGF_PROBE_EXT_LAYER   6  /* one extrapolation layer                    */
+#define MGF_PROBE_ADVECT_FC   7  /* face-velocity advection (+ forces)         */
+#define MGF_PROBE_COUNT       8
+MGF_API mgf_status mgf_scene_set_kernel_probes(mgf_scene scene, int32_t enabled);
+MGF_API mgf_status mgf_scene_kernel_probes(mgf_scene scene, double* ms, int64_t* launches, double* bytes,
+                                           int32_t reset);
 
 /* ---- operator entry points on caller device memory ---------------------- */
 /* These run one reference operator on device buffers the caller owns (for
