@@ -56,9 +56,10 @@ def traffic_for(kernel):
         return None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch") and kernel.split()[0] in line:
+            tok = line.split()
+            if len(tok) == 4 and tok[0] == "traffic_bytes_per_launch" and tok[1] == kernel:
                 try:
-                    return float(line.split("=")[-1])
+                    return float(tok[3])
                 except ValueError:
                     return None
     return None
