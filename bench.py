@@ -129,6 +129,143 @@ def cpu_baseline(spec, plain_iters_per_step, seconds_hint):
                 step_seconds=step_s)
 
 
+MIXED_KW = dict(rk=1, tol_mode=1, tol=1e-6, precision=1, precond=2)   # RK1, relative 1e-6, MIXED, MULTIGRID
+
+
+def plain_one_at_a_time(mgf, scenes, ctx, specs):
+    """The plain FP32 path on the given scenes, one scene at a time (one warm
+    step, then one timed step each): aggregate interior cell-steps/s."""
+    cells, secs = 0, 0.0
+    for p in specs:
+        s = scenes.build_gpu_scene(mgf, ctx, p, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6,
+                                   precision=mgf.PRECISION_PLAIN)
+        s.step(1, allow_unconverged=True)
+        s.sync()
+        t0 = time.perf_counter()
+        st = s.step(1, allow_unconverged=True)
+        s.sync()
+        secs += time.perf_counter() - t0
+        cells += p.interior_cells if st["converged"] == 1 else 0
+        s.close()
+    return cells / secs if secs > 0 else 0.0
+
+
+def batched_leg(mgf, scenes, ctx, args):
+    """Independent scenes of uneven sizes (n_x, n_y, n_z drawn from
+    {32,48,64,96,128}, fluid block, 0-3 spheres, seeds 0..n-1) advanced
+    together by mgf_batch_step: aggregate cell-steps/s over converged steps
+    and per-step latency percentiles."""
+    specs = [scenes.ragged_scene(i) for i in range(args.batch_scenes)]
+    t0 = time.perf_counter()
+    sc = [scenes.build_gpu_scene(mgf, ctx, p, **MIXED_KW) for p in specs]
+    t_setup = time.perf_counter() - t0
+    cells = np.array([p.interior_cells for p in specs], dtype=np.float64)
+    mgf.batch_step(ctx, sc, 1, workers=args.batch_workers)      # warm: first-step allocations, code paths
+    t0 = time.perf_counter()
+    status, stats, ms = mgf.batch_step(ctx, sc, args.batch_steps, workers=args.batch_workers, step_latency=True)
+    wall = time.perf_counter() - t0
+    conv = np.array([st.converged == 1 and st.steps == args.batch_steps for st in stats])
+    value = float((cells * conv).sum() * args.batch_steps / wall)
+    iters = float(np.mean([st.cg_iterations for st in stats]) / max(args.batch_steps, 1))
+    res_max = float(max(st.residual_rel for st in stats))
+    for s in sc:
+        s.close()
+    nplain = min(args.plain_sample, len(specs))
+    plain = plain_one_at_a_time(mgf, scenes, ctx, specs[:nplain]) if nplain > 0 else None
+    out = {"workload": "%d scenes, n_x,n_y,n_z from {32,48,64,96,128} (seeds 0..%d), h=1/64, dt=2e-3, 0-3 "
+                       "spheres, mixed-precision multigrid PCG to 1e-6 per scene, 1000-layer extrapolation"
+                       % (len(specs), len(specs) - 1),
+           "value": value, "unit": "cell-steps/s", "steps": args.batch_steps, "workers": args.batch_workers,
+           "interior_cells": float(cells.sum()), "wall_s": wall, "converged_scenes": int(conv.sum()),
+           "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
+                          "max": float(ms.max())},
+           "cg_iterations_per_step": iters, "residual_rel_max": res_max, "setup_s": t_setup,
+           "plain_one_at_a_time": plain, "plain_sample_scenes": nplain,
+           "speedup_vs_plain": (value / plain) if plain else None}
+    log("batched: %d scenes x %d steps in %.2fs: %.3g cell-steps/s, step latency p50 %.2f ms p99 %.2f ms, "
+        "%d/%d scenes converged every step; plain one at a time %.3g cell-steps/s"
+        % (len(specs), args.batch_steps, wall, value, out["latency_ms"]["p50"], out["latency_ms"]["p99"],
+           conv.sum(), len(specs), plain or 0.0))
+    return out
+
+
+def served_leg(mgf, scenes, ctx, args):
+    """Step requests arriving Poisson at rate lambda for a pool of resident
+    scenes (sizes per axis log-uniform in [32,256], 0-4 spheres) on the
+    served loop; lambda is swept as fractions of the measured saturated
+    request rate.  Latency = completion - arrival of a one-step request."""
+    rng = np.random.default_rng(12345)
+    specs = [scenes.ragged_scene(1000 + i, log_uniform=(32, 256), max_spheres=4)
+             for i in range(args.served_scenes)]
+    cells = np.array([p.interior_cells for p in specs], dtype=np.float64)
+    out = {"workload": "%d resident scenes, n per axis log-uniform in [32,256] (seeds 1000..%d), h=1/64, "
+                       "dt=2e-3, 0-4 spheres, mixed-precision multigrid PCG to 1e-6, one step per request, "
+                       "200 GB admission budget" % (len(specs), 999 + len(specs)),
+           "sweep": []}
+    with mgf.Server(ctx, budget_bytes=200 * 10**9, slots=args.batch_workers) as srv:
+        sc = []
+        for p in specs:
+            s = srv.open_scene(mgf.make_desc(p.res, p.origin, p.spacing, p.dt, **MIXED_KW))
+            if s is None:
+                break
+            scenes.setup_gpu_scene(s, p)
+            sc.append(s)
+        out["admitted"] = len(sc)
+        out["resident_bytes"] = srv.state()[0]
+
+        def drain(n):
+            got = []
+            while len(got) < n:
+                got += srv.poll(cap=1024, wait_ms=1000)
+            return got
+
+        # warm every scene once, then the saturated rate: all scenes x 2 requests at once
+        for s in sc:
+            srv.submit(s, 1)
+        drain(len(sc))
+        t0 = time.perf_counter()
+        for _ in range(2):
+            for s in sc:
+                srv.submit(s, 1)
+        drain(2 * len(sc))
+        sat = 2 * len(sc) / (time.perf_counter() - t0)
+        out["saturated_requests_per_s"] = sat
+        idx_of = {s.h.value: i for i, s in enumerate(sc)}
+        for frac in args.served_load:
+            lam = frac * sat
+            n_req = max(int(lam * args.served_seconds), 8)
+            arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req))
+            pick = rng.integers(0, len(sc), n_req)
+            base = time.perf_counter()
+            got = []
+            for t_arr, i in zip(arrivals, pick):
+                while True:
+                    now = time.perf_counter() - base
+                    if now >= t_arr:
+                        break
+                    got += srv.poll(cap=1024, wait_ms=0)
+                    time.sleep(min(t_arr - now, 0.0005))
+                srv.submit(sc[i], 1)
+            got += drain(n_req - len(got))
+            span = (max(r.t_end_ms for r in got) - min(r.t_submit_ms for r in got)) * 1e-3
+            lat = np.array([r.t_end_ms - r.t_submit_ms for r in got])
+            ok = [r for r in got if r.status == 0 and r.stats.converged == 1]
+            cs = sum(cells[idx_of[r.scene]] for r in ok)
+            row = {"load": frac, "lambda_per_s": lam, "requests": n_req, "converged": len(ok),
+                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                   "cell_steps_per_s": cs / span if span > 0 else 0.0}
+            out["sweep"].append(row)
+            log("served: load %.2f (lambda %.1f/s, %d requests): p50 %.1f ms p99 %.1f ms, %.3g cell-steps/s"
+                % (frac, lam, n_req, row["p50_ms"], row["p99_ms"], row["cell_steps_per_s"]))
+    nplain = min(args.plain_sample, len(specs))
+    plain = plain_one_at_a_time(mgf, scenes, ctx, specs[:nplain]) if nplain > 0 else None
+    out["plain_one_at_a_time"] = plain
+    out["plain_sample_scenes"] = nplain
+    if plain:
+        out["best_speedup_vs_plain"] = max(r["cell_steps_per_s"] for r in out["sweep"]) / plain
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +280,15 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--mg-sweeps", type=int, default=3)
     ap.add_argument("--mg-omega", type=float, default=0.9)
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-workers", type=int, default=8)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--served-scenes", type=int, default=32)
+    ap.add_argument("--served-seconds", type=float, default=3.0)
+    ap.add_argument("--served-load", type=float, nargs="+", default=[0.5, 0.8, 1.0])
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--plain-sample", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs the MAC-grid step on a single GPU "
@@ -215,6 +361,9 @@ def main():
                          path="plain FP32 CG: one kernel per BLAS-1 op, scalars read by the host")
             log("plain: %.1f ms/step, CG iters/step %.1f" % (plain["ms_per_step"], plain["cg_iterations_per_step"]))
 
+        batched = None if args.no_batch or args.batch_scenes <= 0 else batched_leg(mgf, scenes, ctx, args)
+        served = None if args.no_served or args.served_scenes <= 0 else served_leg(mgf, scenes, ctx, args)
+
     cpu = None
     if not args.no_cpu:
         k_plain = plain["cg_iterations_per_step"] if plain else float(r["iters"].mean())
@@ -261,8 +410,11 @@ def main():
             "kernels": ktab,
         },
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),
-        "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
-                       "note": "single-scene step latency (batched scenes not built yet)"},
+        "latency_ms": ({"p50": batched["latency_ms"]["p50"], "p99": batched["latency_ms"]["p99"],
+                        "note": "batched step latency (see batched)"} if batched else
+                       {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
+                        "note": "single-scene step latency"}),
+        "single_scene_latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
         "stages_ms_per_step": stages,
         "cg_iterations_per_step": k,
         "refinement_sweeps_per_step": rs,
@@ -271,6 +423,8 @@ def main():
         "plain_fp32_gpu": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out))
 

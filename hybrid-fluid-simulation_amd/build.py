@@ -15,13 +15,13 @@ LIB = os.path.join(LIB_DIR, "libmgf.so")
 # -ffp-contract=off: one rounding per written operation (explicit fmaf where an
 # axpy is meant); correctly rounded FP32 divide/sqrt; no packed FP32 ops.
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-pthread",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["mgf_kernels.hip", "mgf_kernels_mg.hip", "mgf_kernels_ext.hip", "mgf_capi.cpp"]
+SOURCES = ["mgf_kernels.hip", "mgf_kernels_mg.hip", "mgf_kernels_ext.hip", "mgf_capi.cpp", "mgf_batch.cpp"]
 
 
 def hipcc():
@@ -42,7 +42,7 @@ def build(verbose=False):
         subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

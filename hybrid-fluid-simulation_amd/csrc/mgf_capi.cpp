@@ -33,10 +33,16 @@ mgf_status fail(mgf_status s, const std::string& msg) {
     return s;
 }
 
+// Device allocation owned by the scene; a dry scene only counts the bytes.
 template <class T>
 mgf_status dalloc(mgf_scene sc, T** out, size_t count) {
     void* p = nullptr;
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (sc->dry) {
+        sc->device_bytes += int64_t(bytes);
+        *out = nullptr;
+        return MGF_OK;
+    }
     if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return fail(MGF_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
@@ -130,7 +136,7 @@ int probe_begin(mgf_scene s) {
         s->probe_kind.push_back(0);
         s->probe_bytes.push_back(0.0);
     }
-    if (hipEventRecord(s->probe_ev[2 * k], s->ctx->stream) != hipSuccess) {
+    if (hipEventRecord(s->probe_ev[2 * k], s->stream) != hipSuccess) {
         (void)hipGetLastError();
         return -1;
     }
@@ -139,7 +145,7 @@ int probe_begin(mgf_scene s) {
 
 void probe_end(mgf_scene s, int k, int kind, double bytes) {
     if (k < 0) return;
-    if (hipEventRecord(s->probe_ev[2 * size_t(k) + 1], s->ctx->stream) != hipSuccess) {
+    if (hipEventRecord(s->probe_ev[2 * size_t(k) + 1], s->stream) != hipSuccess) {
         (void)hipGetLastError();
         return;
     }
@@ -165,7 +171,7 @@ void probe_collect(mgf_scene s) {
 }
 
 mgf_status sync(mgf_scene s) {
-    MGF_HIP(hipStreamSynchronize(s->ctx->stream));
+    MGF_HIP(hipStreamSynchronize(s->stream));
     probe_collect(s);
     return MGF_OK;
 }
@@ -178,7 +184,7 @@ double host_sum(const double* p, int n, int stride, int k) {
 }
 
 mgf_status dot_host(mgf_scene s, const float* a, const float* b, size_t n, double* out) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     MGF_HIP(launch_dot_partials(st, a, b, n, s->red.partials, kPlainDotBlocks));
     MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.partials, sizeof(double) * kPlainDotBlocks,
                            hipMemcpyDeviceToHost, st));
@@ -197,7 +203,7 @@ mgf_status refresh_boundaries(mgf_scene s) {
     for (auto& so : s->solids)
         for (int k = 0; k < 3; ++k) so.trans[k] = std::fma(-s->desc.dt, so.vel[k], so.trans[k]);
     if (!s->boundary_dirty && !moving) return MGF_OK;
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     // zero-filled marker, then each solid in order
     MGF_HIP(launch_fill(st, s->solid_total, s->ncell, 0.0f));
     float* tmp = s->ap;
@@ -218,7 +224,7 @@ mgf_status refresh_boundaries(mgf_scene s) {
 // ---------------------------------------------------------------------------
 // _computeAdvection + (optionally) _computeExternalForces.
 mgf_status advect(mgf_scene s, bool with_forces) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const int c = s->cur, n = 1 - s->cur;
     const float dt = s->desc.dt;
     float g[3], f[3];
@@ -246,7 +252,7 @@ mgf_status advect(mgf_scene s, bool with_forces) {
 }
 
 mgf_status apply_forces(mgf_scene s) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const float dt = s->desc.dt;
     const float* incs[2] = {s->desc.gravity, s->desc.external_force};
     for (int q = 0; q < 2; ++q) {
@@ -265,7 +271,7 @@ mgf_status apply_forces(mgf_scene s) {
 // Plain CG: CCuMatrixFreePCG::solvePCGInvDia (MPhysics/CuMatrixFreePCG.cpp:
 // 22-65) with the identity preconditioner, scalars on the host.
 mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_stats* acc) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const size_t n = s->ncell;
     const auto& d = s->desc;
     double bb;
@@ -380,7 +386,7 @@ mgf_status ensure_precond(mgf_scene s) {
 // Coarse markers and stencil codes of every level for the current markers;
 // the operator scale of an axis drops by 4 where the axis is halved.
 mgf_status mg_setup(mgf_scene s, const float sa[3]) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     MgDims f{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
     const uint8_t* mf = s->markers;
     for (auto& L : s->mg) {
@@ -400,7 +406,7 @@ mgf_status mg_setup(mgf_scene s, const float sa[3]) {
 // z = M^-1 r; *zout receives the buffer that holds z.  The last level-0
 // kernel also reduces r . z and sets beta (see k_mg_smooth).
 mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const auto& d = s->desc;
     const MgDims d0{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
     if (d.preconditioner == MGF_PRECOND_JACOBI) {
@@ -487,7 +493,7 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
 // preconditioner), FP64 dots and scalars on the device, FP64 pressure
 // accumulated over refinement sweeps whose residual b - A x is formed in FP64.
 mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_stats* acc) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const auto& d = s->desc;
     const bool ident = d.preconditioner == MGF_PRECOND_IDENTITY;
     const double inner = d.inner_tolerance > 0 ? d.inner_tolerance : 1e-3;
@@ -580,7 +586,7 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
 }
 
 mgf_status pressure(mgf_scene s, mgf_step_stats* acc) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const int c = s->cur;
     float sa[3], sg[3];
     scales(s, sa, sg);
@@ -597,6 +603,43 @@ mgf_status pressure(mgf_scene s, mgf_step_stats* acc) {
     return MGF_OK;
 }
 
+// Resolution-derived sizes of a scene.
+void shape_scene(mgf_scene s, const mgf_scene_desc* desc) {
+    s->desc = *desc;
+    s->g.nx = desc->nx; s->g.ny = desc->ny; s->g.nz = desc->nz;
+    s->g.ox = desc->origin[0]; s->g.oy = desc->origin[1]; s->g.oz = desc->origin[2];
+    s->g.hx = desc->spacing[0]; s->g.hy = desc->spacing[1]; s->g.hz = desc->spacing[2];
+    s->ncell = size_t(desc->nx) * desc->ny * desc->nz;
+    s->nfx = size_t(desc->nx + 1) * desc->ny * desc->nz;
+    s->nfy = size_t(desc->nx) * (desc->ny + 1) * desc->nz;
+    s->nfz = size_t(desc->nx) * desc->ny * (desc->nz + 1);
+    s->nvec = s->ncell;
+}
+
+mgf_status alloc_extrapolation(mgf_scene s, size_t nface);
+
+// Every buffer a scene holds from creation on.
+mgf_status alloc_scene(mgf_scene s) {
+    mgf_status st = MGF_OK;
+    auto A = [&](auto** p, size_t n) {
+        if (st == MGF_OK) st = dalloc(s, p, n);
+    };
+    for (int k = 0; k < 2; ++k) {
+        A(&s->u[k], s->nfx); A(&s->v[k], s->nfy); A(&s->w[k], s->nfz); A(&s->phi[k], s->ncell);
+    }
+    A(&s->solid_total, s->ncell); A(&s->solid_mask, s->ncell);
+    A(&s->us, s->nfx); A(&s->vs, s->nfy); A(&s->ws, s->nfz);
+    A(&s->markers, s->ncell);
+    A(&s->b, s->ncell); A(&s->x, s->ncell); A(&s->x64, s->ncell);
+    A(&s->r, s->ncell); A(&s->p0, s->ncell); A(&s->p1, s->ncell); A(&s->ap, s->ncell);
+    A(&s->codes, s->ncell);
+    A(&s->scal, 1);
+    A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2);
+    A(&s->red.out2, 2);
+    if (st == MGF_OK) st = alloc_extrapolation(s, s->nfx + s->nfy + s->nfz);
+    return st;
+}
+
 // ---------------------------------------------------------------------------
 // Extrapolation: distances, counting sort by distance, one gather kernel per
 // layer; returns the largest non-empty layer (< layers).
@@ -606,6 +649,7 @@ mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
     TRY(dalloc(s, &s->list, nface));
     TRY(dalloc(s, &s->lmask, nface));
     TRY(dalloc(s, &s->counts, ext_counts_size(K)));
+    if (s->dry) return MGF_OK;
     if (hipHostMalloc(reinterpret_cast<void**>(&s->host_offs), sizeof(uint32_t) * (size_t(K) + 3)) != hipSuccess) {
         (void)hipGetLastError();
         s->host_offs = nullptr;
@@ -615,7 +659,7 @@ mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
 }
 
 mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, int* used) {
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     *used = 0;
     const int32_t K = ext_max_distance(s->g);
     const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
@@ -671,6 +715,32 @@ void stats_reset(mgf_step_stats* st) {
 
 }  // namespace
 
+namespace mgf {
+
+mgf_status scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t* out) {
+    if (!ctx || !out) return fail(MGF_ERR_INVALID, "null argument");
+    TRY(validate_desc(desc));
+    mgf_scene_s dry;
+    dry.ctx = ctx;
+    dry.dry = true;
+    shape_scene(&dry, desc);
+    TRY(alloc_scene(&dry));
+    if (desc->precision == MGF_PRECISION_MIXED && desc->preconditioner != MGF_PRECOND_IDENTITY)
+        TRY(ensure_precond(&dry));
+    *out = dry.device_bytes;
+    return MGF_OK;
+}
+
+mgf_status scene_prepare(mgf_scene s) {
+    if (s->desc.precision == MGF_PRECISION_MIXED && s->desc.preconditioner != MGF_PRECOND_IDENTITY) {
+        MGF_HIP(hipSetDevice(s->ctx->device));
+        TRY(ensure_precond(s));
+    }
+    return MGF_OK;
+}
+
+}  // namespace mgf
+
 // ===========================================================================
 extern "C" {
 
@@ -683,6 +753,7 @@ const char* mgf_status_string(mgf_status s) {
         case MGF_ERR_NOT_CONVERGED: return "not converged";
         case MGF_ERR_NONFINITE: return "non-finite value";
         case MGF_ERR_UNSUPPORTED: return "unsupported";
+        case MGF_ERR_BUDGET: return "HBM budget full";
         default: return "unknown status";
     }
 }
@@ -756,33 +827,14 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
     MGF_HIP(hipSetDevice(ctx->device));
     auto* s = new mgf_scene_s;
     s->ctx = ctx;
-    s->desc = *desc;
-    s->g.nx = desc->nx; s->g.ny = desc->ny; s->g.nz = desc->nz;
-    s->g.ox = desc->origin[0]; s->g.oy = desc->origin[1]; s->g.oz = desc->origin[2];
-    s->g.hx = desc->spacing[0]; s->g.hy = desc->spacing[1]; s->g.hz = desc->spacing[2];
-    s->ncell = size_t(desc->nx) * desc->ny * desc->nz;
-    s->nfx = size_t(desc->nx + 1) * desc->ny * desc->nz;
-    s->nfy = size_t(desc->nx) * (desc->ny + 1) * desc->nz;
-    s->nfz = size_t(desc->nx) * desc->ny * (desc->nz + 1);
-    s->nvec = s->ncell;
-    const size_t nface = s->nfx + s->nfy + s->nfz;
-    mgf_status st = MGF_OK;
-    auto A = [&](auto** p, size_t n) {
-        if (st == MGF_OK) st = dalloc(s, p, n);
-    };
-    for (int k = 0; k < 2; ++k) {
-        A(&s->u[k], s->nfx); A(&s->v[k], s->nfy); A(&s->w[k], s->nfz); A(&s->phi[k], s->ncell);
+    shape_scene(s, desc);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        s->stream = nullptr;
+        mgf_scene_destroy(s);
+        return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
     }
-    A(&s->solid_total, s->ncell); A(&s->solid_mask, s->ncell);
-    A(&s->us, s->nfx); A(&s->vs, s->nfy); A(&s->ws, s->nfz);
-    A(&s->markers, s->ncell);
-    A(&s->b, s->ncell); A(&s->x, s->ncell); A(&s->x64, s->ncell);
-    A(&s->r, s->ncell); A(&s->p0, s->ncell); A(&s->p1, s->ncell); A(&s->ap, s->ncell);
-    A(&s->codes, s->ncell);
-    A(&s->scal, 1);
-    A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2);
-    A(&s->red.out2, 2);
-    if (st == MGF_OK) st = alloc_extrapolation(s, nface);
+    mgf_status st = alloc_scene(s);
     if (st != MGF_OK) {
         mgf_scene_destroy(s);
         return st;
@@ -793,7 +845,7 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
         mgf_scene_destroy(s);
         return fail(MGF_ERR_NOMEM, "hipHostMalloc failed");
     }
-    hipStream_t sm = ctx->stream;
+    hipStream_t sm = s->stream;
     // resize() zero-fills every field (MPhysics/ScalarGrid.cpp:33-37)
     hipError_t e = hipSuccess;
     auto F = [&](float* p, size_t n) {
@@ -819,12 +871,13 @@ mgf_status mgf_scene_create(mgf_context ctx, const mgf_scene_desc* desc, mgf_sce
 mgf_status mgf_scene_destroy(mgf_scene s) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
     (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void* p : s->allocations) (void)hipFree(p);
     for (auto& so : s->solids) (void)hipFree(so.sdf0);
     if (s->host_scratch) (void)hipHostFree(s->host_scratch);
     if (s->host_offs) (void)hipHostFree(s->host_offs);
     for (hipEvent_t e : s->probe_ev) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MGF_OK;
 }
@@ -875,7 +928,7 @@ mgf_status mgf_scene_add_solid(mgf_scene s, const float* sdf, int64_t count, con
 
 mgf_status mgf_scene_fluid_box(mgf_scene s, const float mn[3], const float mx[3]) {
     if (!s || !mn || !mx) return fail(MGF_ERR_INVALID, "null argument");
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     MGF_HIP(launch_fluid_box(st, s->g, mn, mx, s->phi[s->cur]));
     // generateFluidDomainFromBBox copies domain 1 into domain 2
     MGF_HIP(hipMemcpyAsync(s->phi[1 - s->cur], s->phi[s->cur], s->ncell * sizeof(float),
@@ -892,7 +945,7 @@ mgf_status mgf_scene_field_count(mgf_scene s, int32_t field, int64_t* out) {
 mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64_t count) {
     if (!s || !host || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
     if (size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     float* dst = nullptr;
     const int c = s->cur;
     switch (field) {
@@ -917,7 +970,7 @@ mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64
 mgf_status mgf_scene_download(mgf_scene s, int32_t field, float* host, int64_t count) {
     if (!s || !host || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
     if (size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
-    hipStream_t st = s->ctx->stream;
+    hipStream_t st = s->stream;
     const int c = s->cur;
     const float* src = nullptr;
     float* tmp = nullptr;
@@ -1159,6 +1212,7 @@ mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid, float* u
     MGF_HIP(hipSetDevice(ctx->device));
     mgf_scene_s tmp;
     tmp.ctx = ctx;
+    tmp.stream = ctx->stream;
     tmp.g = make_grid(grid);
     const size_t nface = size_t(grid->nx + 1) * grid->ny * grid->nz +
                          size_t(grid->nx) * (grid->ny + 1) * grid->nz +

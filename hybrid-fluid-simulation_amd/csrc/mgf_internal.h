@@ -66,6 +66,8 @@ struct mgf_context_s {
 
 struct mgf_scene_s {
     mgf_context ctx = nullptr;
+    hipStream_t stream = nullptr;  // the scene's own stream: scenes overlap on the device
+    bool dry = false;              // sizing only: allocations are counted, not made
     mgf_scene_desc desc{};
     mgf::GridDev g{};
     size_t ncell = 0, nfx = 0, nfy = 0, nfz = 0, nvec = 0;
@@ -136,6 +138,13 @@ constexpr int kMaxBlocks = 4096;   // fused reductions use at most this many blo
 
 // Error helpers.
 void set_error(const std::string& msg);
+
+// Device bytes a scene with this descriptor holds once it has stepped (the
+// preconditioner's levels included, solids excluded); nothing is allocated.
+mgf_status scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t* out);
+// Makes the allocations a scene's first step would make, so that stepping
+// allocates nothing.
+mgf_status scene_prepare(mgf_scene s);
 #define MGF_HIP(expr)                                                           \
     do {                                                                        \
         hipError_t e_ = (expr);                                                 \

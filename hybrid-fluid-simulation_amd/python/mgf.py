@@ -14,7 +14,7 @@ PKG = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG, "lib", "libmgf.so")
 HEADER = os.path.join(os.path.dirname(PKG), "include", "mgf.h")
 
-OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_UNSUPPORTED = range(7)
+OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_NOT_CONVERGED, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_BUDGET = range(8)
 RK1, RK2, RK3 = 1, 2, 3
 TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
@@ -50,6 +50,12 @@ class StepStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("ticket", C.c_int64), ("scene", C.c_void_p), ("status", C.c_int32),
+                ("steps", C.c_int32), ("t_submit_ms", C.c_double), ("t_start_ms", C.c_double),
+                ("t_end_ms", C.c_double), ("stats", StepStats)]
 
 
 class Grid(C.Structure):
@@ -96,6 +102,16 @@ _SIGS = {
     "mgf_scene_set_kernel_probes": [_VP, C.c_int32],
     "mgf_scene_kernel_probes": [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                 C.c_int32],
+    "mgf_batch_step": [_VP, C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32, C.POINTER(StepStats),
+                       C.POINTER(C.c_int32), C.POINTER(C.c_double)],
+    "mgf_scene_bytes_for": [_VP, C.POINTER(SceneDesc), C.POINTER(C.c_int64)],
+    "mgf_server_create": [_VP, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)],
+    "mgf_server_destroy": [_VP],
+    "mgf_server_open_scene": [_VP, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)],
+    "mgf_server_close_scene": [_VP, _VP],
+    "mgf_server_submit": [_VP, _VP, C.c_int32, C.POINTER(C.c_int64)],
+    "mgf_server_poll": [_VP, C.POINTER(RequestResult), C.c_int32, C.c_int32, C.POINTER(C.c_int32)],
+    "mgf_server_state": [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
     "mgf_device_alloc": [_VP, C.c_int64, C.POINTER(C.c_void_p)],
     "mgf_device_free": [_VP, _VP],
     "mgf_memcpy_h2d": [_VP, _VP, _VP, C.c_int64],
@@ -228,16 +244,22 @@ def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode
 class Scene:
     """CGridFluidSolver on one MI355X: fields resident in HBM."""
 
-    def __init__(self, ctx, desc):
+    def __init__(self, ctx, desc, handle=None):
         self.ctx = ctx
         self.desc = desc
+        if handle is not None:          # a scene a Server admitted (the server owns it)
+            self.h = handle
+            self.owned = False
+            return
         h = C.c_void_p()
         check(lib().mgf_scene_create(ctx.h, C.byref(desc), C.byref(h)), "mgf_scene_create")
         self.h = h
+        self.owned = True
 
     def close(self):
         if getattr(self, "h", None):
-            lib().mgf_scene_destroy(self.h)
+            if self.owned:
+                lib().mgf_scene_destroy(self.h)
             self.h = None
 
     def __enter__(self):
@@ -330,3 +352,75 @@ def grid(res, origin=(0, 0, 0), spacing=(1, 1, 1)):
     g.origin = farr3(origin)
     g.spacing = farr3(spacing)
     return g
+
+
+def batch_step(ctx, scenes, nsteps, workers=0, step_latency=False):
+    """mgf_batch_step over Scene objects: returns (statuses, stats, step_ms)
+    with step_ms an (n, nsteps) array when step_latency, else None."""
+    n = len(scenes)
+    arr = (C.c_void_p * n)(*[sc.h.value if isinstance(sc.h, C.c_void_p) else sc.h for sc in scenes])
+    stats = (StepStats * n)()
+    status = (C.c_int32 * n)()
+    ms = np.zeros((n, nsteps), dtype=np.float64) if step_latency else None
+    rc = lib().mgf_batch_step(ctx.h, arr, n, nsteps, workers, stats, status,
+                              ms.ctypes.data_as(C.POINTER(C.c_double)) if ms is not None else None)
+    if rc not in (OK, ERR_NOT_CONVERGED):
+        check(rc, "mgf_batch_step")
+    return list(status), [st for st in stats], ms
+
+
+def scene_bytes_for(ctx, desc):
+    out = C.c_int64()
+    check(lib().mgf_scene_bytes_for(ctx.h, C.byref(desc), C.byref(out)), "mgf_scene_bytes_for")
+    return out.value
+
+
+class Server:
+    """The served request loop (mgf_server_*): admitted scenes stay resident;
+    step requests run on `slots` workers and overlap on the device."""
+
+    def __init__(self, ctx, budget_bytes, slots=0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().mgf_server_create(ctx.h, int(budget_bytes), int(slots), C.byref(h)), "mgf_server_create")
+        self.h = h
+
+    def open_scene(self, desc):
+        """Returns a Scene, or None when the HBM budget refuses admission."""
+        h = C.c_void_p()
+        rc = lib().mgf_server_open_scene(self.h, C.byref(desc), C.byref(h))
+        if rc == ERR_BUDGET:
+            return None
+        check(rc, "mgf_server_open_scene")
+        return Scene(self.ctx, desc, handle=h)
+
+    def close_scene(self, scene):
+        check(lib().mgf_server_close_scene(self.h, scene.h), "mgf_server_close_scene")
+        scene.h = None
+
+    def submit(self, scene, nsteps=1):
+        t = C.c_int64()
+        check(lib().mgf_server_submit(self.h, scene.h, int(nsteps), C.byref(t)), "mgf_server_submit")
+        return t.value
+
+    def poll(self, cap=256, wait_ms=0):
+        out = (RequestResult * cap)()
+        n = C.c_int32()
+        check(lib().mgf_server_poll(self.h, out, cap, int(wait_ms), C.byref(n)), "mgf_server_poll")
+        return [out[i] for i in range(n.value)]
+
+    def state(self):
+        b, q, r = C.c_int64(), C.c_int32(), C.c_int32()
+        check(lib().mgf_server_state(self.h, C.byref(b), C.byref(q), C.byref(r)), "mgf_server_state")
+        return b.value, q.value, r.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().mgf_server_destroy(self.h), "mgf_server_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -12,10 +12,21 @@ import numpy as np
 
 
 class SceneSpec:
-    def __init__(self, n, dt, fluid_min, fluid_max, spheres, name):
-        self.n = n
-        self.res = (n + 2, n + 2, n + 2)
-        self.h = np.float32(1.0) / np.float32(n)
+    """n interior cells per axis (an int, or (nx, ny, nz)) of spacing h over
+    [0, L_x] x [0, L_y] x [0, L_z]; by default h = 1/n on a cubic [0,1]^3."""
+
+    def __init__(self, n, dt, fluid_min, fluid_max, spheres, name, h=None):
+        nxyz = (n, n, n) if np.isscalar(n) else tuple(int(v) for v in n)
+        self.nxyz = nxyz
+        self.n = nxyz[0] if len(set(nxyz)) == 1 else None
+        self.res = tuple(v + 2 for v in nxyz)
+        if h is None:
+            assert self.n is not None, "non-cubic scenes need an explicit spacing"
+            self.h = np.float32(1.0) / np.float32(self.n)
+            self.extent = (1.0, 1.0, 1.0)
+        else:
+            self.h = np.float32(h)
+            self.extent = tuple(float(v) * float(self.h) for v in nxyz)
         self.origin = (-float(self.h),) * 3
         self.spacing = (float(self.h),) * 3
         self.dt = dt
@@ -26,42 +37,51 @@ class SceneSpec:
 
     @property
     def interior_cells(self):
-        return self.n ** 3
+        return self.nxyz[0] * self.nxyz[1] * self.nxyz[2]
+
+    @property
+    def padded_cells(self):
+        return self.res[0] * self.res[1] * self.res[2]
+
+    def _axis_centres(self, k):
+        h = np.float32(self.h)
+        o = np.float32(self.origin[k])
+        idx = np.arange(self.res[k], dtype=np.float32)
+        return (o + np.float32(0.5) * h) + h * idx        # cell centres, as the reference forms them
 
     def _centres(self, z0, z1):
-        g = self.res[0]
-        h = np.float32(self.h)
-        o = np.float32(self.origin[0])
-        idx = np.arange(g, dtype=np.float32)
-        c = (o + np.float32(0.5) * h) + h * idx        # cell centres, as the reference forms them
-        cz = c[z0:z1][:, None, None]
-        cy = c[None, :, None]
-        cx = c[None, None, :]
+        cx = self._axis_centres(0)[None, None, :]
+        cy = self._axis_centres(1)[None, :, None]
+        cz = self._axis_centres(2)[z0:z1][:, None, None]
         return cx, cy, cz
 
+    def _shape(self, z0, z1):
+        return (z1 - z0, self.res[1], self.res[0])
+
     def wall_sdf(self):
-        """min distance to the walls of [0,1]^3: < 0 exactly on the ghost layer."""
-        g = self.res[0]
-        out = np.empty((g, g, g), np.float32)
-        for z0 in range(0, g, 64):
-            z1 = min(g, z0 + 64)
+        """min distance to the walls of the domain: < 0 exactly on the ghost layer."""
+        gz = self.res[2]
+        lx, ly, lz = self.extent
+        out = np.empty((gz, self.res[1], self.res[0]), np.float32)
+        for z0 in range(0, gz, 64):
+            z1 = min(gz, z0 + 64)
             cx, cy, cz = self._centres(z0, z1)
             cx = cx.astype(np.float64); cy = cy.astype(np.float64); cz = cz.astype(np.float64)
-            d = np.minimum(np.minimum(np.minimum(cx, 1 - cx), np.minimum(cy, 1 - cy)),
-                           np.minimum(cz, 1 - cz))
-            out[z0:z1] = np.broadcast_to(d, (z1 - z0, g, g)).astype(np.float32)
+            d = np.minimum(np.minimum(np.minimum(cx, lx - cx), np.minimum(cy, ly - cy)),
+                           np.minimum(cz, lz - cz))
+            out[z0:z1] = np.broadcast_to(d, self._shape(z0, z1)).astype(np.float32)
         return out
 
     def sphere_sdf(self, k):
         cx0, cy0, cz0, r = self.spheres[k]
-        g = self.res[0]
-        out = np.empty((g, g, g), np.float32)
-        for z0 in range(0, g, 64):
-            z1 = min(g, z0 + 64)
+        gz = self.res[2]
+        out = np.empty((gz, self.res[1], self.res[0]), np.float32)
+        for z0 in range(0, gz, 64):
+            z1 = min(gz, z0 + 64)
             cx, cy, cz = self._centres(z0, z1)
             d = np.sqrt((cx.astype(np.float64) - cx0) ** 2 + (cy.astype(np.float64) - cy0) ** 2 +
                         (cz.astype(np.float64) - cz0) ** 2) - r
-            out[z0:z1] = d.astype(np.float32)
+            out[z0:z1] = np.broadcast_to(d, self._shape(z0, z1)).astype(np.float32)
         return out
 
     def solids(self):
@@ -93,13 +113,47 @@ def obstacles(n=256, seed=0, nspheres=3, dt=None):
     return SceneSpec(n, dt, (0.0, 0.0, 0.0), (a, b, 1.0), spheres, "obstacles_%d_s%d" % (n, seed))
 
 
+BATCH_SIZES = (32, 48, 64, 96, 128)
+
+
+def ragged_scene(seed, sizes=BATCH_SIZES, max_spheres=3, h=1.0 / 64, dt=2e-3, log_uniform=None):
+    """One scene of the batched and served workloads: nx, ny, nz drawn
+    independently from `sizes` (or log-uniform in [lo, hi] when
+    log_uniform=(lo, hi)), spacing h, a seeded fluid block [0,a)x[0,b)x[0,L_z)
+    of volume fraction f in [0.2,0.5] of the domain, and 0..max_spheres spheres
+    with centres in [0.15,0.85] of each extent and radii in [0.05,0.15] of the
+    shortest extent."""
+    rng = np.random.default_rng(seed)
+    if log_uniform is None:
+        nxyz = tuple(int(v) for v in rng.choice(sizes, 3))
+    else:
+        lo, hi = log_uniform
+        nxyz = tuple(int(round(np.exp(rng.uniform(np.log(lo), np.log(hi))))) for _ in range(3))
+    ext = [v * float(np.float32(h)) for v in nxyz]
+    f = rng.uniform(0.2, 0.5)
+    a = rng.uniform(f, 1.0)
+    b = f / a
+    spheres = []
+    for _ in range(int(rng.integers(0, max_spheres + 1))):
+        c = rng.uniform(0.15, 0.85, 3)
+        r = rng.uniform(0.05, 0.15) * min(ext)
+        spheres.append((float(c[0] * ext[0]), float(c[1] * ext[1]), float(c[2] * ext[2]), float(r)))
+    return SceneSpec(nxyz, dt, (0.0, 0.0, 0.0), (a * ext[0], b * ext[1], ext[2]), spheres,
+                     "ragged_%dx%dx%d_s%d" % (nxyz + (seed,)), h=h)
+
+
 def build_gpu_scene(mgf, ctx, spec, **desc_kw):
     d = mgf.make_desc(spec.res, spec.origin, spec.spacing, spec.dt, **desc_kw)
     s = mgf.Scene(ctx, d)
+    setup_gpu_scene(s, spec)
+    return s
+
+
+def setup_gpu_scene(s, spec):
+    """Solids and fluid block of `spec` on an existing mgf.Scene."""
     for sdf in spec.solids():
         s.add_solid(sdf)
     s.fluid_box(spec.fluid_min, spec.fluid_max)
-    return s
 
 
 def build_oracle_scene(orc, spec, f64=False):

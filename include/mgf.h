@@ -20,8 +20,9 @@
  *
  * Every function returns an mgf_status (0 = success) and never throws.
  * Host buffers passed in are caller-owned and copied; device state is owned by
- * the library.  A scene handle is not thread-safe; calls on one context are
- * serialised on that context's HIP stream.
+ * the library.  A scene handle is not thread-safe.  Each scene runs on its
+ * own HIP stream (different scenes may be driven from different threads); the
+ * mgf_op_* operators run on the context's stream.
  */
 #ifndef MGF_H
 #define MGF_H
@@ -44,6 +45,7 @@ typedef int32_t mgf_status;
 #define MGF_ERR_NOT_CONVERGED  4  /* CG hit its iteration cap (result kept)    */
 #define MGF_ERR_NONFINITE      5  /* NaN/Inf found in an input or the solve    */
 #define MGF_ERR_UNSUPPORTED    6  /* feature outside this build                */
+#define MGF_ERR_BUDGET         7  /* server admission refused: HBM budget full */
 
 /* ---- enumerations ------------------------------------------------------ */
 /* Back-trace order; values match EAdvectionAccuracy RK1/RK2/RK3
@@ -235,6 +237,74 @@ MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
 MGF_API mgf_status mgf_scene_set_kernel_probes(mgf_scene scene, int32_t enabled);
 MGF_API mgf_status mgf_scene_kernel_probes(mgf_scene scene, double* ms, int64_t* launches, double* bytes,
                                            int32_t reset);
+
+/* ---- batched scenes ------------------------------------------------------ */
+/* No counterpart in the reference, which steps one CGridFluidSolver at a time
+ * (GridFluidSolver.cpp:336-397): callers that own many independent scenes.
+ *
+ * mgf_batch_step advances each of the n scenes by nsteps steps.  Every scene
+ * runs on its own HIP stream, so the scenes' kernels overlap on the device;
+ * `workers` host threads (<= 0: 8; at most 64) submit them, largest scene
+ * first.  Each scene computes exactly what mgf_step(scene, nsteps) computes
+ * alone (same kernels in the same order on its own buffers): results do not
+ * depend on the other scenes.  stats (n entries) and status (n entries) may
+ * be null; step_ms (n * nsteps entries, scene-major, may be null) receives the
+ * host wall time of each step from its first launch to its completion.
+ * Returns MGF_OK when every scene returned MGF_OK, else the first failing
+ * scene's status.  The scenes must be distinct and belong to ctx; no other
+ * call may use them until this one returns. */
+MGF_API mgf_status mgf_batch_step(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t nsteps,
+                                  int32_t workers, mgf_step_stats* stats, mgf_status* status,
+                                  double* step_ms);
+
+/* Device bytes a scene with this descriptor would hold once it has stepped
+ * (solids excluded); allocates nothing. */
+MGF_API mgf_status mgf_scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t* out);
+
+/* ---- served loop ----------------------------------------------------------
+ * No counterpart in the reference: callers that stream step requests for
+ * resident scenes (continuous batching).
+ *
+ * A server owns the scenes it admits and runs step requests on `slots` host
+ * worker threads, each driving its scene's own stream, so requests that
+ * arrive while others run start as soon as a slot is free and overlap with
+ * them on the device.  Requests for one scene run in submission order, one at
+ * a time.  Admission is by HBM budget: mgf_server_open_scene refuses with
+ * MGF_ERR_BUDGET when the resident bytes plus the new scene's
+ * (mgf_scene_bytes_for) would exceed hbm_budget_bytes; solids added to an
+ * admitted scene count from then on.  Between open and the first submit, and
+ * while a scene has no request pending, the caller may set the scene up or
+ * read it with the ordinary mgf_scene_* calls. */
+typedef struct mgf_server_s* mgf_server;
+typedef struct mgf_request_result {
+    int64_t        ticket;
+    mgf_scene      scene;
+    mgf_status     status;
+    int32_t        steps;        /* steps requested                           */
+    double         t_submit_ms;  /* host steady clock (ms) at submit          */
+    double         t_start_ms;   /* when a slot started it                    */
+    double         t_end_ms;     /* when its last kernel had finished         */
+    mgf_step_stats stats;
+} mgf_request_result;
+
+MGF_API mgf_status mgf_server_create(mgf_context ctx, int64_t hbm_budget_bytes, int32_t slots,
+                                     mgf_server* out);
+/* Waits for every pending request, then destroys the admitted scenes. */
+MGF_API mgf_status mgf_server_destroy(mgf_server srv);
+MGF_API mgf_status mgf_server_open_scene(mgf_server srv, const mgf_scene_desc* desc, mgf_scene* out);
+/* Waits for the scene's pending requests, then destroys it (its results stay
+ * pollable). */
+MGF_API mgf_status mgf_server_close_scene(mgf_server srv, mgf_scene scene);
+MGF_API mgf_status mgf_server_submit(mgf_server srv, mgf_scene scene, int32_t nsteps, int64_t* ticket);
+/* Moves up to cap finished requests into out (oldest first) and sets *count;
+ * waits up to wait_ms (< 0: until one finishes or nothing is pending) when none
+ * has finished yet. */
+MGF_API mgf_status mgf_server_poll(mgf_server srv, mgf_request_result* out, int32_t cap, int32_t wait_ms,
+                                   int32_t* count);
+/* Resident device bytes of the admitted scenes, requests waiting for a slot,
+ * requests running; any pointer may be null. */
+MGF_API mgf_status mgf_server_state(mgf_server srv, int64_t* resident_bytes, int32_t* queued,
+                                    int32_t* running);
 
 /* ---- operator entry points on caller device memory ---------------------- */
 /* These run one reference operator on device buffers the caller owns (for

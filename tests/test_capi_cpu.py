@@ -54,6 +54,42 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.mgf_step(None, 1, None) == mgf.ERR_INVALID
     assert L.mgf_shutdown(None) == mgf.ERR_INVALID
     assert L.mgf_init(0, None) == mgf.ERR_INVALID
+    assert L.mgf_batch_step(None, None, 0, 1, 0, None, None, None) == mgf.ERR_INVALID
+    assert L.mgf_server_create(None, 1 << 30, 2, None) == mgf.ERR_INVALID
+    assert L.mgf_server_destroy(None) == mgf.ERR_INVALID
+    assert L.mgf_server_submit(None, None, 1, None) == mgf.ERR_INVALID
+    assert L.mgf_server_poll(None, None, 0, 0, None) == mgf.ERR_INVALID
+    assert L.mgf_scene_bytes_for(None, None, None) == mgf.ERR_INVALID
+    assert L.mgf_status_string(mgf.ERR_BUDGET) == b"HBM budget full"
+
+
+def test_ragged_scene_generator():
+    """The batched/served workloads' scenes: seeded, per-axis sizes from the
+    set, walls negative exactly on the ghost layer, fluid block and spheres
+    inside the domain."""
+    import scenes
+    a, b = scenes.ragged_scene(11), scenes.ragged_scene(11)
+    assert a.name == b.name and a.spheres == b.spheres and a.fluid_max == b.fluid_max
+    seen = set()
+    for seed in range(40):
+        sp = scenes.ragged_scene(seed)
+        assert all(n in scenes.BATCH_SIZES for n in sp.nxyz)
+        assert 0 <= len(sp.spheres) <= 3
+        seen.add(sp.nxyz)
+        for k in range(3):
+            assert 0 < sp.fluid_max[k] <= sp.extent[k] + 1e-12
+        vol = np.prod(sp.fluid_max) / np.prod(sp.extent)
+        assert 0.2 - 1e-9 <= vol <= 0.5 + 1e-9
+    assert len(seen) > 20
+    sp = scenes.ragged_scene(3, sizes=(8, 12), h=1.0 / 16)
+    w = sp.wall_sdf()
+    assert w.shape == sp.res[::-1]
+    inner = np.zeros(w.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    assert np.array_equal(w < 0, ~inner)
+    for seed in range(20):
+        sp = scenes.ragged_scene(seed, log_uniform=(32, 256), max_spheres=4)
+        assert all(32 <= n <= 256 for n in sp.nxyz) and len(sp.spheres) <= 4
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK),
