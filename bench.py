@@ -167,6 +167,12 @@ def batched_leg(mgf, scenes, ctx, args):
     conv = np.array([st.converged == 1 and st.steps == args.batch_steps for st in stats])
     value = float((cells * conv).sum() * args.batch_steps / wall)
     iters = float(np.mean([st.cg_iterations for st in stats]) / max(args.batch_steps, 1))
+    # the step-byte model per scene with its own iteration and sweep counts
+    pcb = precond_bytes("multigrid", 3)
+    nst = max(args.batch_steps, 1)
+    moved = sum(c * nst * (109 + (BYTES_P_AP + BYTES_UPDATE + pcb) * st.cg_iterations / nst
+                           + 25 * st.refinement_sweeps / nst) for c, st in zip(cells, stats))
+    frac = moved / (wall * HBM_PEAK_GBPS * 1e9)
     res_max = float(max(st.residual_rel for st in stats))
     for s in sc:
         s.close()
@@ -180,6 +186,7 @@ def batched_leg(mgf, scenes, ctx, args):
            "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
                           "max": float(ms.max())},
            "cg_iterations_per_step": iters, "residual_rel_max": res_max, "setup_s": t_setup,
+           "step_roofline_frac": frac,
            "plain_one_at_a_time": plain, "plain_sample_scenes": nplain,
            "speedup_vs_plain": (value / plain) if plain else None}
     log("batched: %d scenes x %d steps in %.2fs: %.3g cell-steps/s, step latency p50 %.2f ms p99 %.2f ms, "
