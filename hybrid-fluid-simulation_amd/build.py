@@ -49,7 +49,27 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_host_test(verbose)
     return LIB
+
+
+HOST_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "host_classes_test.cpp")
+HOST_TEST = os.path.join(LIB_DIR, "host_classes_test")
+
+
+def build_host_test(verbose=False):
+    """The C++ driver of the reference-shaped classes (host/mgf_fluid.hpp):
+    host code only, linked against libmgf.so."""
+    if not os.path.exists(HOST_TEST_SRC):
+        return None
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra",
+           "-I", os.path.join(os.path.dirname(HERE), "include"), "-I", os.path.join(HERE, "host"),
+           HOST_TEST_SRC, "-L", LIB_DIR, "-lmgf", "-Wl,-rpath,$ORIGIN", "-o", HOST_TEST + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(HOST_TEST + ".tmp", HOST_TEST)
+    return HOST_TEST
 
 
 if __name__ == "__main__":

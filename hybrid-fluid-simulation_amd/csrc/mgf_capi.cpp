@@ -246,7 +246,13 @@ mgf_status advect(mgf_scene s, bool with_forces) {
     MGF_HIP(launch_advect_fc(st, s->g, V, V, dt, s->desc.rk_order, s->u[n], s->v[n], s->w[n], addg,
                              g, addf, f));
     probe_end(s, pk, MGF_PROBE_ADVECT_FC, 12.0 * double(s->nfx + s->nfy + s->nfz));
-    MGF_HIP(launch_advect_cc(st, s->g, s->phi[c], V, dt, s->desc.rk_order, s->phi[n]));
+    if (s->color[0][0]) {
+        const float* ic[3] = {s->color[c][0], s->color[c][1], s->color[c][2]};
+        float* oc[3] = {s->color[n][0], s->color[n][1], s->color[n][2]};
+        MGF_HIP(launch_advect_cc4(st, s->g, s->phi[c], ic, V, dt, s->desc.rk_order, s->phi[n], oc));
+    } else {
+        MGF_HIP(launch_advect_cc(st, s->g, s->phi[c], V, dt, s->desc.rk_order, s->phi[n]));
+    }
     s->cur = n;
     return MGF_OK;
 }
@@ -942,9 +948,16 @@ mgf_status mgf_scene_field_count(mgf_scene s, int32_t field, int64_t* out) {
     return MGF_OK;
 }
 
-mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64_t count) {
-    if (!s || !host || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
-    if (size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
+}  // extern "C"
+
+namespace {
+
+// Upload from host (kind = H2D) or device (D2D) memory; the same side effects
+// either way.
+mgf_status upload_field(mgf_scene s, int32_t field, const float* src, int64_t count, hipMemcpyKind kind) {
+    if (!s || !src || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
+    if (count < 0 || size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
+    MGF_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->stream;
     float* dst = nullptr;
     const int c = s->cur;
@@ -957,9 +970,20 @@ mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64
         case MGF_FIELD_SOLID_VEL_X: dst = s->us; break;
         case MGF_FIELD_SOLID_VEL_Y: dst = s->vs; break;
         case MGF_FIELD_SOLID_VEL_Z: dst = s->ws; break;
+        case MGF_FIELD_COLOR_X: case MGF_FIELD_COLOR_Y: case MGF_FIELD_COLOR_Z:
+            if (!s->color[0][0]) {
+                // both buffers of the three components, zero-filled as resize() fills
+                for (int b = 0; b < 2; ++b)
+                    for (int k = 0; k < 3; ++k) {
+                        TRY(dalloc(s, &s->color[b][k], s->ncell));
+                        MGF_HIP(launch_fill(st, s->color[b][k], s->ncell, 0.0f));
+                    }
+            }
+            dst = s->color[c][field - MGF_FIELD_COLOR_X];
+            break;
         default: return fail(MGF_ERR_INVALID, "field is download-only");
     }
-    MGF_HIP(hipMemcpyAsync(dst, host, size_t(count) * sizeof(float), hipMemcpyHostToDevice, st));
+    MGF_HIP(hipMemcpyAsync(dst, src, size_t(count) * sizeof(float), kind, st));
     if (field == MGF_FIELD_SOLID_SDF) MGF_HIP(launch_solid_mask(st, s->ncell, s->solid_total, s->solid_mask));
     // The next boundary refresh (mgf_step) recomputes these from the solids,
     // as CBoundarys::updateBoundarys overwrites setSolidVelField.
@@ -967,9 +991,10 @@ mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64
     return sync(s);
 }
 
-mgf_status mgf_scene_download(mgf_scene s, int32_t field, float* host, int64_t count) {
-    if (!s || !host || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
-    if (size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
+mgf_status download_field(mgf_scene s, int32_t field, float* out, int64_t count, hipMemcpyKind kind) {
+    if (!s || !out || field < 0 || field >= MGF_FIELD_COUNT) return fail(MGF_ERR_INVALID, "bad argument");
+    if (count < 0 || size_t(count) != field_count(s, field)) return fail(MGF_ERR_INVALID, "element count mismatch");
+    MGF_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->stream;
     const int c = s->cur;
     const float* src = nullptr;
@@ -984,15 +1009,25 @@ mgf_status mgf_scene_download(mgf_scene s, int32_t field, float* host, int64_t c
         case MGF_FIELD_SOLID_VEL_Y: src = s->vs; break;
         case MGF_FIELD_SOLID_VEL_Z: src = s->ws; break;
         case MGF_FIELD_RHS: src = s->b; break;
+        case MGF_FIELD_COLOR_X: case MGF_FIELD_COLOR_Y: case MGF_FIELD_COLOR_Z:
+            if (!s->color[0][0]) {
+                if (kind == hipMemcpyDeviceToHost) {
+                    std::fill(out, out + count, 0.0f);
+                    return MGF_OK;
+                }
+                MGF_HIP(launch_fill(st, out, size_t(count), 0.0f));
+                return sync(s);
+            }
+            src = s->color[c][field - MGF_FIELD_COLOR_X];
+            break;
         case MGF_FIELD_PRESSURE:
             if (s->x_is_64) {
-                std::vector<double> h(s->ncell);
-                MGF_HIP(hipMemcpyAsync(h.data(), s->x64, s->ncell * sizeof(double), hipMemcpyDeviceToHost, st));
-                TRY(sync(s));
-                for (size_t i = 0; i < s->ncell; ++i) host[i] = float(h[i]);
-                return MGF_OK;
+                // the FP64 pressure rounded to FP32 (in the scratch vector ap)
+                MGF_HIP(launch_f64_to_f32(st, s->x64, s->ap, s->ncell));
+                src = s->ap;
+            } else {
+                src = s->x;
             }
-            src = s->x;
             break;
         case MGF_FIELD_MARKERS:
             MGF_HIP(launch_u8_to_f32(st, s->markers, s->ap, s->ncell));
@@ -1015,11 +1050,31 @@ mgf_status mgf_scene_download(mgf_scene s, int32_t field, float* host, int64_t c
         }
         default: return fail(MGF_ERR_INVALID, "unknown field");
     }
-    hipError_t e = hipMemcpyAsync(host, src, size_t(count) * sizeof(float), hipMemcpyDeviceToHost, st);
+    hipError_t e = hipMemcpyAsync(out, src, size_t(count) * sizeof(float), kind, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess) return fail(MGF_ERR_DEVICE, std::string("download: ") + hipGetErrorString(e));
     return MGF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mgf_status mgf_scene_upload(mgf_scene s, int32_t field, const float* host, int64_t count) {
+    return upload_field(s, field, host, count, hipMemcpyHostToDevice);
+}
+
+mgf_status mgf_scene_download(mgf_scene s, int32_t field, float* host, int64_t count) {
+    return download_field(s, field, host, count, hipMemcpyDeviceToHost);
+}
+
+mgf_status mgf_scene_upload_device(mgf_scene s, int32_t field, const float* dev, int64_t count) {
+    return upload_field(s, field, dev, count, hipMemcpyDeviceToDevice);
+}
+
+mgf_status mgf_scene_download_device(mgf_scene s, int32_t field, float* dev, int64_t count) {
+    return download_field(s, field, dev, count, hipMemcpyDeviceToDevice);
 }
 
 mgf_status mgf_scene_update_boundaries(mgf_scene s) {
@@ -1130,6 +1185,14 @@ mgf_status mgf_memcpy_h2d(mgf_context ctx, void* dst, const void* src, int64_t b
     return MGF_OK;
 }
 
+mgf_status mgf_memcpy_d2d(mgf_context ctx, void* dst, const void* src, int64_t bytes) {
+    if (!ctx || !dst || !src || bytes < 0) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipSetDevice(ctx->device));
+    MGF_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToDevice, ctx->stream));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
 mgf_status mgf_memcpy_d2h(mgf_context ctx, void* dst, const void* src, int64_t bytes) {
     if (!ctx || !dst || !src || bytes < 0) return fail(MGF_ERR_INVALID, "bad argument");
     MGF_HIP(hipMemcpyAsync(dst, src, size_t(bytes), hipMemcpyDeviceToHost, ctx->stream));
@@ -1199,6 +1262,29 @@ mgf_status mgf_op_pressure_matvec(mgf_context ctx, const mgf_grid* grid, const f
     TRY(check_grid(grid));
     if (x == y) return fail(MGF_ERR_INVALID, "y must not alias x");
     MGF_HIP(launch_matvec(ctx->stream, make_grid(grid), scale, markers, x, y));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_apply_forces(mgf_context ctx, const mgf_grid* grid, float* u, float* v, float* w,
+                               const float gravity[3], const float force[3], float dt) {
+    if (!ctx || !u || !v || !w || !gravity || !force) return fail(MGF_ERR_INVALID, "null argument");
+    TRY(check_grid(grid));
+    MGF_HIP(hipSetDevice(ctx->device));
+    const size_t nx = size_t(grid->nx), ny = size_t(grid->ny), nz = size_t(grid->nz);
+    const size_t n[3] = {(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)};
+    float* f[3] = {u, v, w};
+    // CExternalForcesSolver::applyExternalForces (ExternalForcesSolver.cpp:50-59): gravity,
+    // then the external forces, each as plusAlphaX with the unit field, skipped when every
+    // component of the increment is below FLT_EPSILON (FaceCenteredVectorGrid.cpp:162-175)
+    const float* incs[2] = {gravity, force};
+    for (int q = 0; q < 2; ++q) {
+        const float a[3] = {incs[q][0] * dt, incs[q][1] * dt, incs[q][2] * dt};
+        if (std::fabs(a[0]) < MGF_FLT_EPSILON && std::fabs(a[1]) < MGF_FLT_EPSILON &&
+            std::fabs(a[2]) < MGF_FLT_EPSILON)
+            continue;
+        for (int k = 0; k < 3; ++k) MGF_HIP(launch_add_uniform(ctx->stream, f[k], n[k], a[k]));
+    }
     MGF_HIP(hipStreamSynchronize(ctx->stream));
     return MGF_OK;
 }

@@ -77,6 +77,7 @@ struct mgf_scene_s {
     float* v[2] = {nullptr, nullptr};
     float* w[2] = {nullptr, nullptr};
     float* phi[2] = {nullptr, nullptr};
+    float* color[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};  // from the first upload on
     int cur = 0;
 
     float* solid_total = nullptr;  // CBoundarys total marker: -k or UNKNOWN (0 with no solid)
@@ -176,6 +177,10 @@ hipError_t launch_advect_fc(hipStream_t s, const GridDev& g, Vel in, Vel V, floa
                             int add1, const float inc1[3]);
 hipError_t launch_advect_cc(hipStream_t s, const GridDev& g, const float* in, Vel V, float dt,
                             int order, float* out);
+// The fluid domain and the three colour components sampled at one back-traced
+// point per cell.
+hipError_t launch_advect_cc4(hipStream_t s, const GridDev& g, const float* phi, const float* const c[3], Vel V,
+                             float dt, int order, float* ophi, float* const oc[3]);
 hipError_t launch_add_uniform(hipStream_t s, float* a, size_t n, float inc);
 hipError_t launch_markers(hipStream_t s, const GridDev& g, const uint8_t* solid_mask,
                           const float* fluid, uint8_t* markers);
@@ -273,5 +278,6 @@ hipError_t launch_solids_vel(hipStream_t s, const GridDev& g, const float* total
 hipError_t launch_fluid_box(hipStream_t s, const GridDev& g, const float mn[3],
                             const float mx[3], float* phi);
 hipError_t launch_u8_to_f32(hipStream_t s, const uint8_t* a, float* b, size_t n);
+hipError_t launch_f64_to_f32(hipStream_t s, const double* a, float* b, size_t n);
 
 }  // namespace mgf

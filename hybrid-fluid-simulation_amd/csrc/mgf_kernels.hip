@@ -78,6 +78,27 @@ __global__ void __launch_bounds__(kBlock) k_advect_cc(GridDev g, const float* __
     out[i] = trilerp(in, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px, py, pz);
 }
 
+// CSemiLagrangian::advect(CCS) of the fluid domain and advect(CCV) of the
+// colour (MPhysics/SemiLagrangian.cpp:135-167): the four advects of one step
+// back-trace the same cell centre through the same velocity, so one back-trace
+// serves all four trilinear samples (sampleCellCenteredVectorFieldInvoker,
+// FieldMathTool.cu:575-640, samples each component like a scalar field).
+struct Cc4 {
+    const float* in[4];
+    float* out[4];
+};
+__global__ void __launch_bounds__(kBlock) k_advect_cc4(GridDev g, Cc4 F, Vel V, RkCoef k, int order, uint32_t n) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
+    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    float px = cpos(x, g.hx, g.ox), py = cpos(y, g.hy, g.oy), pz = cpos(z, g.hz, g.oz);
+    backtrace(V, g, order, k, px, py, pz);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        F.out[q][i] = trilerp(F.in[q], g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px, py, pz);
+}
+
 // plusAlphaX with a unit field: a[i] = inc*1 + a[i].
 __global__ void __launch_bounds__(kBlock) k_add_uniform(float* a, size_t n, float inc) {
     const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
@@ -267,6 +288,11 @@ __global__ void __launch_bounds__(kBlock) k_fill64(double* a, size_t n, double v
 }
 
 __global__ void __launch_bounds__(kBlock) k_u8_to_f32(const uint8_t* a, float* b, size_t n) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) b[i] = float(a[i]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_f64_to_f32(const double* a, float* b, size_t n) {
     const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
     if (i < n) b[i] = float(a[i]);
 }
@@ -491,6 +517,19 @@ hipError_t launch_advect_cc(hipStream_t s, const GridDev& g, const float* in, Ve
     return LAUNCH(k_advect_cc, grid_for(n), g, in, V, rk_coef(dt), order, out, n);
 }
 
+hipError_t launch_advect_cc4(hipStream_t s, const GridDev& g, const float* phi, const float* const c[3], Vel V,
+                             float dt, int order, float* ophi, float* const oc[3]) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    Cc4 F;
+    F.in[0] = phi;
+    F.out[0] = ophi;
+    for (int q = 0; q < 3; ++q) {
+        F.in[q + 1] = c[q];
+        F.out[q + 1] = oc[q];
+    }
+    return LAUNCH(k_advect_cc4, grid_for(n), g, F, V, rk_coef(dt), order, n);
+}
+
 hipError_t launch_add_uniform(hipStream_t s, float* a, size_t n, float inc) {
     return LAUNCH(k_add_uniform, grid_for(n), a, n, inc);
 }
@@ -549,6 +588,10 @@ hipError_t launch_fill64(hipStream_t s, double* a, size_t n, double v) {
 
 hipError_t launch_u8_to_f32(hipStream_t s, const uint8_t* a, float* b, size_t n) {
     return LAUNCH(k_u8_to_f32, grid_for(n), a, b, n);
+}
+
+hipError_t launch_f64_to_f32(hipStream_t s, const double* a, float* b, size_t n) {
+    return LAUNCH(k_f64_to_f32, grid_for(n), a, b, n);
 }
 
 hipError_t launch_true_residual32(hipStream_t s, const GridDev& g, const float scale[3],

@@ -24,7 +24,7 @@ PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "
                "k_mg_restrict", "k_ext_layer", "k_advect_fc")
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
              solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
-             divergence=12)
+             divergence=12, color_x=13, color_y=14, color_z=15)
 
 
 class SceneDesc(C.Structure):
@@ -90,6 +90,8 @@ _SIGS = {
     "mgf_scene_fluid_box": [_VP, _F, _F],
     "mgf_scene_upload": [_VP, C.c_int32, _F, C.c_int64],
     "mgf_scene_download": [_VP, C.c_int32, _F, C.c_int64],
+    "mgf_scene_upload_device": [_VP, C.c_int32, _VP, C.c_int64],
+    "mgf_scene_download_device": [_VP, C.c_int32, _VP, C.c_int64],
     "mgf_scene_field_count": [_VP, C.c_int32, C.POINTER(C.c_int64)],
     "mgf_scene_update_boundaries": [_VP],
     "mgf_scene_advect": [_VP],
@@ -116,6 +118,7 @@ _SIGS = {
     "mgf_device_free": [_VP, _VP],
     "mgf_memcpy_h2d": [_VP, _VP, _VP, C.c_int64],
     "mgf_memcpy_d2h": [_VP, _VP, _VP, C.c_int64],
+    "mgf_memcpy_d2d": [_VP, _VP, _VP, C.c_int64],
     "mgf_op_backtrace": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_float, C.c_int32, _VP, _VP, _VP,
                          C.c_int64],
     "mgf_op_sample_cc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _VP, _VP, C.c_int64],
@@ -123,6 +126,7 @@ _SIGS = {
                          _VP, _VP, _VP],
     "mgf_op_advect_cc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _VP, C.c_float, C.c_int32, _VP],
     "mgf_op_pressure_matvec": [_VP, C.POINTER(Grid), _F, _VP, _VP, _VP],
+    "mgf_op_apply_forces": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _F, _F, C.c_float],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
 _RESTYPE = {"mgf_status_string": C.c_char_p, "mgf_last_error": C.c_char_p}

@@ -101,7 +101,16 @@ typedef int32_t mgf_status;
                                      built from the markers on request)        */
 #define MGF_FIELD_DIVERGENCE  12  /* nx*ny*nz  divergence of the velocity
                                      (download only; computed on request)      */
-#define MGF_FIELD_COUNT       13
+#define MGF_FIELD_COLOR_X     13  /* nx*ny*nz  colour field, one component each
+                                     (setColorField / getColorField,
+                                     GridFluidSolver.cpp:208-213); advected with
+                                     the velocity the step starts from, as
+                                     CSemiLagrangian::advect(CCV)
+                                     (SemiLagrangian.cpp:152-167).  Held from
+                                     the first upload on (zeros before).       */
+#define MGF_FIELD_COLOR_Y     14
+#define MGF_FIELD_COLOR_Z     15
+#define MGF_FIELD_COUNT       16
 
 /* ---- descriptors -------------------------------------------------------- */
 typedef struct mgf_scene_desc {
@@ -191,6 +200,12 @@ MGF_API mgf_status mgf_scene_upload(mgf_scene scene, int32_t field,
         const float* host, int64_t count);
 MGF_API mgf_status mgf_scene_download(mgf_scene scene, int32_t field,
         float* host, int64_t count);
+/* The same from / into device memory the caller owns (the fields of the
+ * reference live on the device: VectorGrid.h:16-21); same side effects. */
+MGF_API mgf_status mgf_scene_upload_device(mgf_scene scene, int32_t field,
+        const float* dev, int64_t count);
+MGF_API mgf_status mgf_scene_download_device(mgf_scene scene, int32_t field,
+        float* dev, int64_t count);
 MGF_API mgf_status mgf_scene_field_count(mgf_scene scene, int32_t field,
         int64_t* out);
 
@@ -318,6 +333,7 @@ typedef struct mgf_grid {
 
 MGF_API mgf_status mgf_device_alloc(mgf_context ctx, int64_t bytes, void** out);
 MGF_API mgf_status mgf_device_free(mgf_context ctx, void* ptr);
+MGF_API mgf_status mgf_memcpy_d2d(mgf_context ctx, void* dst, const void* src, int64_t bytes);
 MGF_API mgf_status mgf_memcpy_h2d(mgf_context ctx, void* dst, const void* src, int64_t bytes);
 MGF_API mgf_status mgf_memcpy_d2h(mgf_context ctx, void* dst, const void* src, int64_t bytes);
 
@@ -349,6 +365,12 @@ MGF_API mgf_status mgf_op_advect_cc(mgf_context ctx, const mgf_grid* grid,
  * 1 fluid, 2 solid) and scale = dt/(density*h^2) per axis. */
 MGF_API mgf_status mgf_op_pressure_matvec(mgf_context ctx, const mgf_grid* grid,
         const float scale[3], const uint8_t* markers, const float* x, float* y);
+/* CExternalForcesSolver::applyExternalForces (MPhysics/ExternalForcesSolver.cpp:
+ * 50-59): u_c += gravity_c*dt, then u_c += force_c*dt, on every face; each
+ * increment is skipped when all three components are below FLT_EPSILON
+ * (FaceCenteredVectorGrid.cpp:162-175). */
+MGF_API mgf_status mgf_op_apply_forces(mgf_context ctx, const mgf_grid* grid,
+        float* u, float* v, float* w, const float gravity[3], const float force[3], float dt);
 /* extrapolatingDataInvoker for a face-centred field
  * (MPhysics/EulerSolverTool.cu:450-557, 904-969): in place. */
 MGF_API mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid,

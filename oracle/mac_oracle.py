@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libmac_oracle.so")
 
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4,
              solid_vel_x=5, solid_vel_y=6, solid_vel_z=7, markers=8,
-             pressure=9, rhs=10, matrix_a=11)
+             pressure=9, rhs=10, matrix_a=11, color_x=13, color_y=14, color_z=15)
 
 
 class OrcGrid(C.Structure):

@@ -90,6 +90,7 @@ template <class R> struct Scene {
     std::vector<R> solid_marker, us, vs, ws;
     std::vector<std::vector<R>> sdf_cur;  // last re-sampled SDFs
     std::vector<R> last_markers, last_x, last_b, last_A;
+    std::vector<R> color[3];            // CCV colour field, empty until uploaded
     bool boundary_from_solids = true;
 
     explicit Scene(const Grid<R>& gr) : g(gr) {
@@ -123,6 +124,19 @@ template <class R> struct Scene {
         std::vector<R> nu(u.size()), nv(v.size()), nw(w.size()), nphi(phi.size());
         advect_fc(g, g, u.data(), v.data(), w.data(), u.data(), v.data(), w.data(),
                   R(p.dt), p.rk_order, nu.data(), nv.data(), nw.data());
+        // CSemiLagrangian::advect(CCV) (MPhysics/SemiLagrangian.cpp:152-167): one
+        // back-trace from the cell centres, then the CCS trilinear sample of each
+        // component (sampleCellCenteredVectorFieldInvoker, FieldMathTool.cu:575-640);
+        // the colour is advected between the velocity and the fluid domain
+        // (GridFluidSolver.cpp:405-406, 413-414), all with the old velocity
+        if (!color[0].empty()) {
+            for (int k = 0; k < 3; ++k) {
+                std::vector<R> nc(color[k].size());
+                advect_cc(g, g, color[k].data(), u.data(), v.data(), w.data(), R(p.dt), p.rk_order,
+                          nc.data());
+                color[k].swap(nc);
+            }
+        }
         advect_cc(g, g, phi.data(), u.data(), v.data(), w.data(), R(p.dt), p.rk_order,
                   nphi.data());
         u.swap(nu); v.swap(nv); w.swap(nw); phi.swap(nphi);
@@ -159,8 +173,7 @@ template <class R> struct Scene {
         return extrapolate_fc(g, u.data(), v.data(), w.data(), p.extrapolation_layers);
     }
 
-    // _onAdvanceTimeStep (MPhysics/GridFluidSolver.cpp:359-370); the colour
-    // field is not carried (it has no effect on the velocity or the domain).
+    // _onAdvanceTimeStep (MPhysics/GridFluidSolver.cpp:359-370).
     void step(const orc_params& p, orc_stats* st) {
         double t0 = now_ms();
         update_boundaries(R(p.dt));
@@ -206,6 +219,12 @@ template <class R> std::vector<R>* field_ptr(Scene<R>& s, int field) {
         case 9: return &s.last_x;
         case 10: return &s.last_b;
         case 11: return &s.last_A;
+        case 13: case 14: case 15:
+            // the colour field exists from its first upload on (zero-filled, as
+            // resize() fills, MPhysics/ScalarGrid.cpp:33-37)
+            if (s.color[0].empty())
+                for (auto& c : s.color) c.assign(s.g.cells(), 0);
+            return &s.color[field - 13];
         default: return nullptr;
     }
 }

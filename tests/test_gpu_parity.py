@@ -380,6 +380,38 @@ def test_step_stages_match_oracle(ctx, orc):
     s.close(); o.close()
 
 
+@pytest.mark.parametrize("rk", [1, 3])
+def test_colour_advection_bitexact(ctx, orc, rk):
+    """The colour field (setColorField, GridFluidSolver.cpp:208-212) is advected
+    with the velocity the step starts from, as advect(CCV) does: bit-exact
+    against the oracle.  The full step carries the same colour as the stage
+    path, and a scene without colour reads zeros."""
+    spec = scenes.obstacles(24, seed=4)
+    kw = dict(rk=rk, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=mgf.PRECISION_MIXED,
+              precond=mgf.PRECOND_MULTIGRID)
+    s = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
+    full = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
+    o = scenes.build_oracle_scene(orc, spec)
+    assert not np.any(s.download(13))
+    rng = np.random.default_rng(rk)
+    for f in (0, 1, 2):
+        v = (rng.standard_normal(s.count(f)) * 0.5).astype(np.float32)
+        s.upload(f, v); full.upload(f, v); o.upload(f, v)
+    for k in range(3):
+        c = rng.uniform(0, 1, spec.padded_cells).astype(np.float32)
+        s.upload(13 + k, c); full.upload(13 + k, c); o.upload(13 + k, c)
+    p = orc.params(spec.dt, rk=rk, tol_mode=1, tol=1e-6)
+    s.update_boundaries(); o.update_boundaries(p)
+    s.advect(); o.advect(p)
+    for k in range(3):
+        assert_bitexact(s.download(13 + k), o.download(13 + k), "colour %d" % k)
+    assert_bitexact(s.download(3), o.download(3), "fluid domain advected beside the colour")
+    full.step(1)
+    for k in range(3):
+        assert_bitexact(full.download(13 + k), s.download(13 + k), "colour after a full step %d" % k)
+    s.close(); full.close(); o.close()
+
+
 def compare_trajectory(s, o, step, tol_v, what):
     phi_g, phi_o = s.download(3), o.download(3)
     if step == 0:

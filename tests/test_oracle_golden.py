@@ -426,3 +426,40 @@ def test_extrapolation_layers_are_l1_distance(orc, seed):
                 if ids[Z, Y, X] < L:
                     nb.append(vals[Z, Y, X])
             assert abs(vals[z, y, x] - np.mean(nb)) < 1e-5
+
+
+def test_scene_colour_is_advected_with_the_old_velocity(orc):
+    """The oracle step carries the colour field as CSemiLagrangian::advect(CCV)
+    does (SemiLagrangian.cpp:152-167): each channel is the CCS advection of the
+    channel by the velocity the step started from; a constant colour stays
+    constant, and a scene without colour keeps none."""
+    import scenes
+    rng = np.random.default_rng(5)
+    spec = scenes.obstacles(10, seed=2)
+    s = scenes.build_oracle_scene(orc, spec)
+    n = int(np.prod(spec.res))
+    col = [rng.standard_normal(n).astype(np.float32) for _ in range(3)]
+    for k in range(3):
+        s.upload(13 + k, col[k])
+    vel = [s.download(f) for f in (0, 1, 2)]
+    for order in (1, 3):
+        p = orc.params(spec.dt, rk=order, tol_mode=1, tol=1e-6)
+        s.update_boundaries(p)
+        vel = [s.download(f) for f in (0, 1, 2)]
+        exp = [orc.advect_cc(spec.res, spec.origin, spec.spacing, col[k], *vel, spec.dt, order) for k in range(3)]
+        s.advect(p)
+        got = [s.download(13 + k) for k in range(3)]
+        for k in range(3):
+            assert np.array_equal(got[k].view(np.uint32), exp[k].view(np.uint32)), (order, k)
+        col = got
+        # give the next round a non-zero velocity
+        for f in (0, 1, 2):
+            s.upload(f, rng.standard_normal(vel[f].size).astype(np.float32) * 0.3)
+    c = scenes.build_oracle_scene(orc, spec)
+    for k in range(3):
+        c.upload(13 + k, np.full(n, 0.25 * (k + 1), np.float32))
+    p = orc.params(spec.dt, rk=3, tol_mode=1, tol=1e-6)
+    c.step(p, 2)
+    for k in range(3):
+        assert np.allclose(c.download(13 + k), 0.25 * (k + 1), atol=1e-6)
+    s.close(); c.close()
