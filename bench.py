@@ -280,6 +280,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--spheres", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -305,7 +306,7 @@ def main():
     import mgf
     import scenes
 
-    spec = scenes.obstacles(args.n, seed=args.seed)
+    spec = scenes.obstacles(args.n, seed=args.seed, nspheres=args.spheres)
     ncell_padded = int(np.prod(spec.res))
     tol = 1e-6
     with mgf.Context(0) as ctx:
@@ -398,9 +399,9 @@ def main():
         "dtype": "f32 fields, f64 reductions and pressure",
         "data": "synthetic",
         "config": {
-            "workload": "obstacles %d^3 (%d^3 padded), seed %d, 3 spheres, RK1, gravity, mixed-precision "
+            "workload": "obstacles %d^3 (%d^3 padded), seed %d, %d spheres, RK1, gravity, mixed-precision "
                         "PCG (%s preconditioner) to 1e-6 relative FP64 residual, 1000-layer extrapolation"
-                        % (spec.n, spec.res[0], args.seed, args.precond),
+                        % (spec.n, spec.res[0], args.seed, args.spheres, args.precond),
             "grid": list(spec.res), "dt": spec.dt, "tolerance": tol,
         },
         "roofline": {

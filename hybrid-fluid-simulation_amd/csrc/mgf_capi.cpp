@@ -615,6 +615,7 @@ void shape_scene(mgf_scene s, const mgf_scene_desc* desc) {
     s->g.nx = desc->nx; s->g.ny = desc->ny; s->g.nz = desc->nz;
     s->g.ox = desc->origin[0]; s->g.oy = desc->origin[1]; s->g.oz = desc->origin[2];
     s->g.hx = desc->spacing[0]; s->g.hy = desc->spacing[1]; s->g.hz = desc->spacing[2];
+    set_recip(s->g);
     s->ncell = size_t(desc->nx) * desc->ny * desc->nz;
     s->nfx = size_t(desc->nx + 1) * desc->ny * desc->nz;
     s->nfy = size_t(desc->nx) * (desc->ny + 1) * desc->nz;
@@ -654,6 +655,7 @@ mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
     TRY(dalloc(s, &s->layer, nface));
     TRY(dalloc(s, &s->list, nface));
     TRY(dalloc(s, &s->lmask, nface));
+    TRY(dalloc(s, &s->fmask, nface));
     TRY(dalloc(s, &s->counts, ext_counts_size(K)));
     if (s->dry) return MGF_OK;
     if (hipHostMalloc(reinterpret_cast<void**>(&s->host_offs), sizeof(uint32_t) * (size_t(K) + 3)) != hipSuccess) {
@@ -673,17 +675,22 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     MGF_HIP(launch_ext_distance(st, s->g, u, v, w, s->layer));
     const uint32_t* offs = nullptr;
     const uint32_t* maxd = nullptr;
-    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, K, s->counts, s->list, s->lmask, &offs, &maxd));
+    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, K, s->counts, s->list, s->lmask, s->fmask, &offs, &maxd));
     // offsets of layers 0..cap+1, then the largest distance present
     MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipStreamSynchronize(st));
     const int32_t top = int32_t(s->host_offs[cap + 2]);
-    for (int32_t L = 1; L <= top; ++L) {
-        const uint32_t b = s->host_offs[L], e = s->host_offs[L + 1];
+    // two layers per launch; the last one alone when their count is odd
+    for (int32_t L = 1; L <= top; L += 2) {
+        const uint32_t b = s->host_offs[L], m = s->host_offs[L + 1];
+        const uint32_t e = L + 1 <= top ? s->host_offs[L + 2] : m;
         const int pk = probe_begin(s);
-        MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->list, s->lmask, b, e - b));
-        // list entry 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
+        if (L + 1 <= top)
+            MGF_HIP(launch_ext_layer2(st, s->g, u, v, w, s->list, s->lmask, s->fmask, b, m - b, e - m));
+        else
+            MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->list, s->lmask, b, m - b));
+        // per entry: list 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
         probe_end(s, pk, MGF_PROBE_EXT_LAYER, 13.0 * double(e - b));
     }
     *used = top;
@@ -1035,6 +1042,17 @@ mgf_status download_field(mgf_scene s, int32_t field, float* out, int64_t count,
             break;
         case MGF_FIELD_DIVERGENCE:
             MGF_HIP(launch_divergence(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), s->ap));
+            src = s->ap;
+            break;
+        case MGF_FIELD_FLUID_DENSITY:
+            MGF_HIP(launch_fluid_density(st, s->g, s->phi[c], s->solid_total, s->ap));
+            src = s->ap;
+            break;
+        case MGF_FIELD_FLUID_SDF:
+            if (s->desc.extrapolation_layers > 0)
+                MGF_HIP(launch_fluid_sdf(st, s->g, s->phi[c], s->desc.extrapolation_layers, s->layer, s->ap));
+            else   // no pass: the freshly resized field's zeros (ScalarGrid.cpp:33-37)
+                MGF_HIP(launch_fill(st, s->ap, s->ncell, 0.0f));
             src = s->ap;
             break;
         case MGF_FIELD_MATRIX_A: {

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #define MGF_UNKNOWN 3.40282346638528859812e+38f   /* FLT_MAX, MMathExtra.h:58 */
 #define MGF_FLT_EPSILON 1.19209290e-07f          /* FLT_EPSILON, MMathExtra.h:55 */
 
@@ -18,7 +20,30 @@ struct GridDev {
     int nx, ny, nz;
     float ox, oy, oz;
     float hx, hy, hz;
+    // 1/h per axis and whether every spacing is a power of two whose
+    // reciprocal is a normal float (set by set_recip)
+    float rhx = 0, rhy = 0, rhz = 0;
+    int pow2 = 0;
 };
+
+// Division by a spacing h = 2^k and multiplication by 2^-k round the same
+// real number once, so they give the same float (overflow, underflow, zero
+// signs and NaN included) while the multiplication avoids the correctly
+// rounded division's instruction sequence.  Other spacings divide.
+inline void set_recip(GridDev& g) {
+    auto p2 = [](float h, float& r) {
+        int e = 0;
+        const float m = std::frexp(h, &e);
+        r = 1.0f / h;
+        return h > 0.0f && m == 0.5f && std::isnormal(h) && std::isnormal(r);
+    };
+    const bool a = p2(g.hx, g.rhx), b = p2(g.hy, g.rhy), c = p2(g.hz, g.rhz);
+    g.pow2 = (a && b && c) ? 1 : 0;
+}
+
+__device__ __forceinline__ float div_h(float x, float h, float rh, int pow2) {
+    return pow2 ? x * rh : x / h;
+}
 
 // limitIndex (MPhysics/EulerMathTool.cu:16-27).
 __device__ __forceinline__ int clampi(int i, int n) {
@@ -49,17 +74,18 @@ __device__ __forceinline__ int floor_ref(float v) {
 // spacing h): sampleCellCenteredScalarFieldTrilerp
 // (MPhysics/FieldMathTool.cu:52-104).
 __device__ __forceinline__ float trilerp(const float* __restrict__ f, int nx, int ny, int nz,
-                                         float ox, float oy, float oz, float hx, float hy,
-                                         float hz, float px, float py, float pz) {
+                                         float ox, float oy, float oz, const GridDev& g,
+                                         float px, float py, float pz) {
+    const float hx = g.hx, hy = g.hy, hz = g.hz;
     const float rx = px - ox - hx / 2;
     const float ry = py - oy - hy / 2;
     const float rz = pz - oz - hz / 2;
-    const int ix = floor_ref(rx / hx);
-    const int iy = floor_ref(ry / hy);
-    const int iz = floor_ref(rz / hz);
-    const float fx = (rx - float(ix) * hx) / hx;
-    const float fy = (ry - float(iy) * hy) / hy;
-    const float fz = (rz - float(iz) * hz) / hz;
+    const int ix = floor_ref(div_h(rx, hx, g.rhx, g.pow2));
+    const int iy = floor_ref(div_h(ry, hy, g.rhy, g.pow2));
+    const int iz = floor_ref(div_h(rz, hz, g.rhz, g.pow2));
+    const float fx = div_h(rx - float(ix) * hx, hx, g.rhx, g.pow2);
+    const float fy = div_h(ry - float(iy) * hy, hy, g.rhy, g.pow2);
+    const float fz = div_h(rz - float(iz) * hz, hz, g.rhz, g.pow2);
     const int x0 = clampi(ix, nx), x1 = clampi(ix + 1, nx);
     const int y0 = clampi(iy, ny), y1 = clampi(iy + 1, ny);
     const int z0 = clampi(iz, nz), z1 = clampi(iz + 1, nz);
@@ -87,11 +113,9 @@ __device__ __forceinline__ float trilerp(const float* __restrict__ f, int nx, in
 // (MPhysics/FieldMathTool.cu:783-892).
 __device__ __forceinline__ float sample_comp(const float* __restrict__ f, int c, const GridDev& g,
                                              float px, float py, float pz) {
-    if (c == 0)
-        return trilerp(f, g.nx + 1, g.ny, g.nz, g.ox - g.hx / 2, g.oy, g.oz, g.hx, g.hy, g.hz, px, py, pz);
-    if (c == 1)
-        return trilerp(f, g.nx, g.ny + 1, g.nz, g.ox, g.oy - g.hy / 2, g.oz, g.hx, g.hy, g.hz, px, py, pz);
-    return trilerp(f, g.nx, g.ny, g.nz + 1, g.ox, g.oy, g.oz - g.hz / 2, g.hx, g.hy, g.hz, px, py, pz);
+    if (c == 0) return trilerp(f, g.nx + 1, g.ny, g.nz, g.ox - g.hx / 2, g.oy, g.oz, g, px, py, pz);
+    if (c == 1) return trilerp(f, g.nx, g.ny + 1, g.nz, g.ox, g.oy - g.hy / 2, g.oz, g, px, py, pz);
+    return trilerp(f, g.nx, g.ny, g.nz + 1, g.ox, g.oy, g.oz - g.hz / 2, g, px, py, pz);
 }
 
 struct Vel {

@@ -111,6 +111,7 @@ struct mgf_scene_s {
     int32_t* layer = nullptr;      // distance of every face (nfx+nfy+nfz)
     uint32_t* list = nullptr;      // faces sorted by distance (nfx+nfy+nfz)
     uint8_t* lmask = nullptr;      // previous-layer neighbour mask of each list entry
+    uint8_t* fmask = nullptr;      // the same mask per face (read for neighbours of the next layer)
     uint32_t* counts = nullptr;    // counting-sort scratch [ext_counts_size(K)]
     uint32_t* host_offs = nullptr; // pinned mirror of the offsets and the max distance
 
@@ -160,6 +161,7 @@ inline GridDev make_grid(const mgf_grid* g) {
     d.nx = g->nx; d.ny = g->ny; d.nz = g->nz;
     d.ox = g->origin[0]; d.oy = g->origin[1]; d.oz = g->origin[2];
     d.hx = g->spacing[0]; d.hy = g->spacing[1]; d.hz = g->spacing[2];
+    set_recip(d);
     return d;
 }
 
@@ -261,12 +263,21 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
 // offs[cap+1] = total) and *maxd that of the largest distance present.
 size_t ext_counts_size(int32_t K);
 hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
-                           uint32_t* counts, uint32_t* list, uint8_t* lmask, const uint32_t** offs,
+                           uint32_t* counts, uint32_t* list, uint8_t* lmask, uint8_t* fmask, const uint32_t** offs,
                            const uint32_t** maxd);
 // One layer: list[begin, begin+count) take the mean of their neighbours in
 // the previous layer (lmask).
 hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
                             const uint8_t* lmask, uint32_t begin, uint32_t count);
+// Layers L and L+1 (n0 then n1 list entries from `begin`) in one launch;
+// fmask = every face's lower-neighbour mask.
+hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                             const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1);
+// buildFluidDensity / buildFluidSDF (EulerSolverTool.cu:324-448) on the
+// cell-centred grid; scratch holds ncell int32.
+hipError_t launch_fluid_density(hipStream_t s, const GridDev& g, const float* phi, const float* solid, float* out);
+hipError_t launch_fluid_sdf(hipStream_t s, const GridDev& g, const float* phi, int32_t passes, int32_t* scratch,
+                            float* out);
 // Boundaries.
 hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0,
                                  const float t[3], float* out);

@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(kBlock) k_sample_cc(GridDev g, const float* f,
                                                       int64_t n) {
     const int64_t i = int64_t(xcd_block()) * kBlock + threadIdx.x;
     if (i >= n) return;
-    out[i] = trilerp(f, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px[i], py[i], pz[i]);
+    out[i] = trilerp(f, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, px[i], py[i], pz[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kBlock) k_advect_cc(GridDev g, const float* __
     const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
     float px = cpos(x, g.hx, g.ox), py = cpos(y, g.hy, g.oy), pz = cpos(z, g.hz, g.oz);
     backtrace(V, g, order, k, px, py, pz);
-    out[i] = trilerp(in, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px, py, pz);
+    out[i] = trilerp(in, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, px, py, pz);
 }
 
 // CSemiLagrangian::advect(CCS) of the fluid domain and advect(CCV) of the
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(kBlock) k_advect_cc4(GridDev g, Cc4 F, Vel V, 
     backtrace(V, g, order, k, px, py, pz);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        F.out[q][i] = trilerp(F.in[q], g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, px, py, pz);
+        F.out[q][i] = trilerp(F.in[q], g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, px, py, pz);
 }
 
 // plusAlphaX with a unit field: a[i] = inc*1 + a[i].
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) k_resample_solid(GridDev g, const floa
     const float cx = g.ox + 0.5f * g.hx + g.hx * float(x);
     const float cy = g.oy + 0.5f * g.hy + g.hy * float(y);
     const float cz = g.oz + 0.5f * g.hz + g.hz * float(z);
-    out[i] = trilerp(sdf0, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, cx + tx, cy + ty,
+    out[i] = trilerp(sdf0, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, cx + tx, cy + ty,
                      cz + tz);
 }
 

@@ -15,9 +15,11 @@
 //   2. counting sort of the faces with 1 <= d <= N-1 by d (LDS histogram per
 //      block, one global atomic per block and distance into one of 32
 //      private copies);
-//   3. one gather kernel per layer, over that layer's faces only, in layer
-//      order: the neighbours in layers below L are final when layer L runs
-//      (the sort also stores each face's mask of previous-layer neighbours).
+//   3. one gather kernel per pair of layers, over those layers' faces only,
+//      in layer order: the neighbours in layers below L are final when layer
+//      L runs, and a layer-L+1 face recomputes its layer-L neighbours' values
+//      (the sort also stores each face's mask of previous-layer neighbours,
+//      per list entry and per face).
 //
 // The mean is summed in the reference's neighbour order (-x, +x, -y, +y, -z,
 // +z) in FP32 and divided once, so the values are bitwise the reference's.
@@ -109,8 +111,9 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restric
 }
 
 // 1b. Lines along y (axis 1) or z (axis 2): d = min(d, d(prev) + 1) forward
-// then backward; one work-item per line, lines x-fastest (coalesced), eight
-// loads in flight per chunk.
+// then backward; one work-item per line, lines x-fastest (coalesced), 32
+// loads in flight per chunk (a line is a chain of dependent minima, so the
+// loads of a chunk are issued before its first minimum).
 template <int AXIS>
 __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restrict__ dist) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
@@ -125,7 +128,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
     if (AXIS == 1) { start = w * ex * ey + x; stride = ex; len = ey; }
     else { start = w * ex + x; stride = ex * ey; len = pick(F.ez, c); }
     int32_t* d = dist + pick4(F.off, c) + start;
-    constexpr int K = 8;
+    constexpr int K = 32;
     int32_t prev = kDistInf;
     for (uint32_t k0 = 0; k0 < len; k0 += K) {
         int32_t v[K];
@@ -315,7 +318,7 @@ __device__ __forceinline__ uint32_t lower_mask(const Faces& F, const int32_t* __
 __global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const int32_t* __restrict__ dist, uint32_t total,
                                                         int32_t cap, uint32_t K2, const uint32_t* __restrict__ poff,
                                                         uint32_t* cursor, uint32_t* __restrict__ list,
-                                                        uint8_t* __restrict__ lmask) {
+                                                        uint8_t* __restrict__ lmask, uint8_t* __restrict__ fmask) {
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
@@ -356,6 +359,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const int32_t* 
             const uint32_t pos = po[dv[k]] + (dv[k] < kLdsBins ? h[dv[k]] : 0u) + rank[k];
             list[pos] = i;
             lmask[pos] = uint8_t(msk[k]);
+            fmask[i] = uint8_t(msk[k]);
         }
     }
 }
@@ -365,6 +369,28 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const int32_t* 
 // previous layer (lower-neighbour mask from the sort); the others are read at
 // the face itself and left out by the mask, so every load of a work-item is
 // issued at once and no distance is read.
+//
+// The mean of face i (component-local) over the neighbours flagged in m, in
+// the reference's order (-x, +x, -y, +y, -z, +z) with one divide; the
+// conditional sums as selects (an unused neighbour leaves the sum untouched,
+// a -0.0 included).
+__device__ __forceinline__ float layer_mean(const float* f, uint32_t i, uint32_t m, uint32_t ex, uint32_t sz) {
+    const uint32_t j[6] = {(m & 1u) ? i - 1 : i, (m & 2u) ? i + 1 : i, (m & 4u) ? i - ex : i,
+                           (m & 8u) ? i + ex : i, (m & 16u) ? i - sz : i, (m & 32u) ? i + sz : i};
+    float fv[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) fv[q] = f[j[q]];
+    float acc = 0.0f, cnt = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const bool use = (m >> q) & 1u;
+        acc = use ? acc + fv[q] : acc;
+        cnt = use ? cnt + 1.0f : cnt;
+    }
+    if (cnt != 0.0f) acc /= cnt;
+    return acc;
+}
+
 __global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const uint32_t* __restrict__ list,
                                                       const uint8_t* __restrict__ lmask, uint32_t begin,
                                                       uint32_t count) {
@@ -376,13 +402,41 @@ __global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const uint32_t* _
     const uint32_t i = gi - pick4(F.off, c);
     const uint32_t ex = pick(F.ex, c), sz = ex * pick(F.ey, c);
     float* f = pick(F.f, c);
+    f[i] = layer_mean(f, i, m, ex, sz);
+}
+
+// Layers L and L+1 in one launch (their list segments are adjacent: n0 faces
+// of layer L from `begin`, then n1 of layer L+1).  A layer-L+1 face needs the
+// new values of its layer-L neighbours; it recomputes each of them with the
+// same operations from the final layer-(L-1) values (the neighbour's own
+// lower mask is in the per-face mask array fmask) instead of reading what the
+// layer-L work-items write in this launch, so no work-item reads a value
+// written in the launch and the results are bitwise those of two launches.
+__global__ void __launch_bounds__(kBlock) k_ext_layer2(Faces F, const uint32_t* __restrict__ list,
+                                                       const uint8_t* __restrict__ lmask,
+                                                       const uint8_t* __restrict__ fmask, uint32_t begin,
+                                                       uint32_t n0, uint32_t n1) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n0 + n1) return;
+    const uint32_t gi = list[begin + t];
+    const uint32_t m = lmask[begin + t];
+    const int c = comp_of(F.off, gi);
+    const uint32_t o = pick4(F.off, c);
+    const uint32_t i = gi - o;
+    const uint32_t ex = pick(F.ex, c), sz = ex * pick(F.ey, c);
+    float* f = pick(F.f, c);
+    if (t < n0) {
+        f[i] = layer_mean(f, i, m, ex, sz);
+        return;
+    }
     const uint32_t j[6] = {(m & 1u) ? i - 1 : i, (m & 2u) ? i + 1 : i, (m & 4u) ? i - ex : i,
                            (m & 8u) ? i + ex : i, (m & 16u) ? i - sz : i, (m & 32u) ? i + sz : i};
+    uint32_t mn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) mn[q] = ((m >> q) & 1u) ? uint32_t(fmask[o + j[q]]) : 0u;
     float fv[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) fv[q] = f[j[q]];
-    // the reference's conditional sums as selects (an unused neighbour leaves
-    // the sum untouched, a -0.0 included)
+    for (int q = 0; q < 6; ++q) fv[q] = layer_mean(f, j[q], mn[q], ex, sz);
     float acc = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -392,6 +446,183 @@ __global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const uint32_t* _
     }
     if (cnt != 0.0f) acc /= cnt;
     f[i] = acc;
+}
+
+// ===========================================================================
+// Fields the hybrid coupling reads (GridFluidSolver.cpp:571-613) on the
+// cell-centred grid; FluidSurfaceDensity = 0.2 (Common.h:244).
+
+struct Cells {
+    uint32_t nx, ny, nz, n;
+};
+
+__device__ __forceinline__ void cell_nbrs(const Cells& C, uint32_t i, uint32_t (&j)[6]) {
+    const uint32_t sxy = C.nx * C.ny;
+    const uint32_t z = i / sxy, rem = i - z * sxy, y = rem / C.nx, x = rem - y * C.nx;
+    j[0] = x > 0 ? i - 1 : i;
+    j[1] = x + 1 < C.nx ? i + 1 : i;
+    j[2] = y > 0 ? i - C.nx : i;
+    j[3] = y + 1 < C.ny ? i + C.nx : i;
+    j[4] = z > 0 ? i - sxy : i;
+    j[5] = z + 1 < C.nz ? i + sxy : i;
+}
+
+// buildFluidDensity (MPhysics/EulerSolverTool.cu:408-448): fluid (phi < 0)
+// outside the solids (marker >= 0) is 0.2 next to an air cell (phi >= 0 and
+// marker >= 0, isInsideAir EulerMathTool.cu:344-348), else 1; the rest 0.
+__global__ void __launch_bounds__(kBlock) k_fluid_density(Cells C, const float* __restrict__ phi,
+                                                          const float* __restrict__ solid, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= C.n) return;
+    uint32_t j[6];
+    cell_nbrs(C, i, j);
+    bool air = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) air = air || (phi[j[q]] >= 0.0f && solid[j[q]] >= 0.0f);
+    const bool fluid = phi[i] < 0.0f && !(solid[i] < 0.0f);
+    out[i] = fluid ? (air ? 0.2f : 1.0f) : 0.0f;
+}
+
+// buildFluidSDF (MPhysics/EulerSolverTool.cu:324-406) by distance: pass i of
+// the reference assigns the cells at breadth-first distance i from the
+// boundary layer of their own region (fluid cells with a non-fluid clamped
+// neighbour, non-fluid cells with a fluid neighbour).  The nearest boundary
+// cell of either region lies in a cell's own region (a path into the other
+// region crosses an own boundary cell first), so one L1 distance transform
+// from all boundary cells gives both distances.
+__global__ void __launch_bounds__(kBlock) k_sdf_seed(Cells C, const float* __restrict__ phi,
+                                                     int32_t* __restrict__ d) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= C.n) return;
+    uint32_t j[6];
+    cell_nbrs(C, i, j);
+    const bool in = phi[i] < 0.0f;
+    bool other = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) other = other || ((phi[j[q]] < 0.0f) != in);
+    d[i] = other ? 0 : kDistInf;
+}
+
+// 1-D L1 distance along x rows in place (one wave per row): forward and
+// backward min-plus scans as in k_ext_rows.
+__global__ void __launch_bounds__(kBlock) k_cdt_rows(Cells C, int32_t* __restrict__ dist) {
+    const uint32_t row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= C.ny * C.nz) return;
+    int32_t* d = dist + size_t(row) * C.nx;
+    const uint32_t ex = C.nx;
+    int32_t carry = kDistInf;
+    for (uint32_t x0 = 0; x0 < ex; x0 += 64) {
+        const uint32_t x = x0 + uint32_t(lane);
+        const bool ok = x < ex;
+        int32_t g = (ok ? d[x] : kDistInf) - int32_t(x);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t t = __shfl_up(g, o, 64);
+            if (lane >= o) g = min(g, t);
+        }
+        g = min(g, carry);
+        carry = __shfl(g, 63, 64);
+        if (ok) d[x] = min(g + int32_t(x), kDistInf);
+    }
+    carry = 2 * kDistInf;
+    for (uint32_t ch = (ex + 63) / 64; ch-- > 0;) {
+        const uint32_t x = ch * 64 + uint32_t(lane);
+        const bool ok = x < ex;
+        int32_t h = ok ? d[x] + int32_t(x) : 2 * kDistInf;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t t = __shfl_down(h, o, 64);
+            if (lane + o < 64) h = min(h, t);
+        }
+        h = min(h, carry);
+        carry = __shfl(h, 0, 64);
+        if (ok) d[x] = min(h - int32_t(x), kDistInf);
+    }
+}
+
+// Lines along y (AXIS 1) or z (AXIS 2), one work-item per line, x fastest.
+template <int AXIS>
+__global__ void __launch_bounds__(kBlock) k_cdt_lines(Cells C, int32_t* __restrict__ dist) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t lines = AXIS == 1 ? C.nx * C.nz : C.nx * C.ny;
+    if (t >= lines) return;
+    const uint32_t w = t / C.nx, x = t - w * C.nx;
+    uint32_t start, stride, len;
+    if (AXIS == 1) { start = w * C.nx * C.ny + x; stride = C.nx; len = C.ny; }
+    else { start = w * C.nx + x; stride = C.nx * C.ny; len = C.nz; }
+    int32_t* d = dist + start;
+    constexpr int K = 32;
+    int32_t prev = kDistInf;
+    for (uint32_t k0 = 0; k0 < len; k0 += K) {
+        int32_t v[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = (k0 + q < len) ? d[size_t(k0 + q) * stride] : 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (k0 + q < len) {
+                prev = min(v[q], prev + 1);
+                d[size_t(k0 + q) * stride] = prev;
+            }
+        }
+    }
+    prev = kDistInf;
+    for (uint32_t k1 = len; k1 > 0;) {
+        const uint32_t k0 = k1 >= K ? k1 - K : 0;
+        const uint32_t m = k1 - k0;
+        int32_t v[K];
+#pragma unroll
+        for (int q = 0; q < K; ++q) v[q] = (uint32_t(q) < m) ? d[size_t(k1 - 1 - q) * stride] : 0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (uint32_t(q) < m) {
+                prev = min(v[q], prev + 1);
+                d[size_t(k1 - 1 - q) * stride] = min(prev, kDistInf);
+            }
+        }
+        k1 = k0;
+    }
+}
+
+// Values of buildFluidSDF from the distance d and the region: fluid d = 0 ->
+// 0.5-0.2, fluid d -> -(d - (0.5-0.2)), non-fluid d -> d + (1.5-0.2), each
+// formed in double and rounded once as the reference's double literals make
+// them; d >= passes keeps -UNKNOWN (fluid) or UNKNOWN.
+__global__ void __launch_bounds__(kBlock) k_sdf_value(Cells C, const float* __restrict__ phi,
+                                                      const int32_t* __restrict__ d, int32_t passes,
+                                                      float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= C.n) return;
+    const bool in = phi[i] < 0.0f;
+    const int32_t k = d[i];
+    float v;
+    if (k >= passes) v = in ? -MGF_UNKNOWN : MGF_UNKNOWN;
+    else if (in) v = k == 0 ? float(0.5 - 0.2) : float(-(double(k) - (0.5 - 0.2)));
+    else v = float(double(k) + (1.5 - 0.2));
+    out[i] = v;
+}
+
+hipError_t launch_fluid_density(hipStream_t s, const GridDev& g, const float* phi, const float* solid, float* out) {
+    const Cells C{uint32_t(g.nx), uint32_t(g.ny), uint32_t(g.nz), uint32_t(g.nx) * g.ny * g.nz};
+    hipLaunchKernelGGL(k_fluid_density, dim3((C.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, phi, solid, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fluid_sdf(hipStream_t s, const GridDev& g, const float* phi, int32_t passes, int32_t* scratch,
+                            float* out) {
+    const Cells C{uint32_t(g.nx), uint32_t(g.ny), uint32_t(g.nz), uint32_t(g.nx) * g.ny * g.nz};
+    const uint32_t G = (C.n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_sdf_seed, dim3(G), dim3(kBlock), 0, s, C, phi, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cdt_rows, dim3((C.ny * C.nz + 3) / 4), dim3(kBlock), 0, s, C, scratch);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cdt_lines<1>, dim3((C.nx * C.nz + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, scratch);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cdt_lines<2>, dim3((C.nx * C.ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, scratch);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sdf_value, dim3(G), dim3(kBlock), 0, s, C, phi, scratch, passes, out);
+    return hipGetLastError();
 }
 
 // ===========================================================================
@@ -431,7 +662,7 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
 size_t ext_counts_size(int32_t K) { return (3 * size_t(kCopies) + 1) * (size_t(K) + 2) + 1; }
 
 hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
-                           uint32_t* counts, uint32_t* list, uint8_t* lmask, const uint32_t** offs,
+                           uint32_t* counts, uint32_t* list, uint8_t* lmask, uint8_t* fmask, const uint32_t** offs,
                            const uint32_t** maxd) {
     const Faces F = faces_of(g, nullptr, nullptr, nullptr);
     const uint32_t total = F.off[3];
@@ -452,7 +683,7 @@ hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, F, dist, total, cap, K2, B.poff,
-                       B.cursor, list, lmask);
+                       B.cursor, list, lmask, fmask);
     return hipGetLastError();
 }
 
@@ -462,6 +693,15 @@ hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v,
     const Faces F = faces_of(g, u, v, w);
     hipLaunchKernelGGL(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, list, lmask, begin,
                        count);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                             const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1) {
+    if (n0 + n1 == 0) return hipSuccess;
+    const Faces F = faces_of(g, u, v, w);
+    hipLaunchKernelGGL(k_ext_layer2, dim3((n0 + n1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, list, lmask, fmask,
+                       begin, n0, n1);
     return hipGetLastError();
 }
 

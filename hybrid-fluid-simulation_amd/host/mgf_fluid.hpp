@@ -621,6 +621,13 @@ public:
         m_CurSimulationTime += m_DeltaT;
     }
 
+    // generateCurFluidSDF / generateCurFluidDensity (GridFluidSolver.cpp:571-613)
+    // fill the fields getFluidSDFField / getFluidDensityField return.
+    void generateCurFluidSDF() { refresh(m_FluidSDFField, MGF_FIELD_FLUID_SDF); }
+    void generateCurFluidDensity() { refresh(m_FluidDensityField, MGF_FIELD_FLUID_DENSITY); }
+    const CCellCenteredScalarField& getFluidSDFField() const { return m_FluidSDFField; }
+    const CCellCenteredScalarField& getFluidDensityField() const { return m_FluidDensityField; }
+
     void generateFluidDomainFromBBox(Vector3 mn, Vector3 mx) {
         const float a[3] = {mn.x, mn.y, mn.z}, b[3] = {mx.x, mx.y, mx.z};
         check(mgf_scene_fluid_box(scene(), a, b), "mgf_scene_fluid_box");
@@ -687,6 +694,7 @@ private:
     ExternalForcesSolverPtr m_ExternalForcesSolver;
     PressureSolverPtr m_PressureSolver;
     mgf_step_stats m_LastStats{};
+    CCellCenteredScalarField m_FluidSDFField, m_FluidDensityField;
     mutable CCellCenteredScalarField m_SolidView, m_DomainView;
     mutable CCellCenteredVectorField m_ColorView;
     mutable CFaceCenteredVectorField m_VelView;

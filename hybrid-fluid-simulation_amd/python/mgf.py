@@ -24,7 +24,7 @@ PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "
                "k_mg_restrict", "k_ext_layer", "k_advect_fc")
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
              solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
-             divergence=12, color_x=13, color_y=14, color_z=15)
+             divergence=12, color_x=13, color_y=14, color_z=15, fluid_sdf=16, fluid_density=17)
 
 
 class SceneDesc(C.Structure):

@@ -110,7 +110,18 @@ typedef int32_t mgf_status;
                                      the first upload on (zeros before).       */
 #define MGF_FIELD_COLOR_Y     14
 #define MGF_FIELD_COLOR_Z     15
-#define MGF_FIELD_COUNT       16
+#define MGF_FIELD_FLUID_SDF   16  /* nx*ny*nz  generateCurFluidSDF
+                                     (GridFluidSolver.cpp:571-588,
+                                     EulerSolverTool.cu:324-406): layered
+                                     distance from the fluid surface over
+                                     extrapolation_layers passes (download
+                                     only; computed on request)                */
+#define MGF_FIELD_FLUID_DENSITY 17 /* nx*ny*nz generateCurFluidDensity
+                                     (GridFluidSolver.cpp:590-613,
+                                     EulerSolverTool.cu:408-448): 1 in the
+                                     fluid, 0.2 at its air surface, 0 elsewhere
+                                     (download only; computed on request)      */
+#define MGF_FIELD_COUNT       18
 
 /* ---- descriptors -------------------------------------------------------- */
 typedef struct mgf_scene_desc {

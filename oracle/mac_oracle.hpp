@@ -682,6 +682,83 @@ PressureOut<R> pressure_solve(const Grid<R>& g, R dt, R density, R* u, R* v, R* 
     return o;
 }
 
+// Fields the hybrid coupling reads from the grid step (GridFluidSolver.cpp:
+// 571-613).  FluidSurfaceDensity = 0.2 (Common.h:244).
+//
+// buildFluidDensity (MPhysics/EulerSolverTool.cu:408-448): a cell inside the
+// fluid (phi < 0, isInsideSDF EulerMathTool.cu:129) and outside every solid
+// (solid marker >= 0) is FluidSurfaceDensity when a clamped neighbour is air
+// (phi >= 0 and solid >= 0, isInsideAir EulerMathTool.cu:344-348), else 1;
+// every other cell 0.
+template <class R>
+void build_fluid_density(const Grid<R>& g, const R* phi, const R* solid, R* out) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    auto air = [&](size_t i) { return phi[i] >= R(0) && solid[i] >= R(0); };
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t c = lin(x, y, z, nx, ny, nz);
+                if (phi[c] < R(0) && !(solid[c] < R(0))) {
+                    const bool surf = air(lin(x - 1, y, z, nx, ny, nz)) || air(lin(x + 1, y, z, nx, ny, nz)) ||
+                                      air(lin(x, y - 1, z, nx, ny, nz)) || air(lin(x, y + 1, z, nx, ny, nz)) ||
+                                      air(lin(x, y, z - 1, nx, ny, nz)) || air(lin(x, y, z + 1, nx, ny, nz));
+                    out[c] = surf ? R(0.2) : R(1.0);
+                } else {
+                    out[c] = R(0);
+                }
+            }
+}
+
+// buildFluidSDFInvoker / buildFluidSDF (MPhysics/EulerSolverTool.cu:324-406,
+// 760-785): passes i = 0 .. passes-1 over the grid, in place.  Pass 0: a
+// fluid cell (phi < 0) with a non-fluid clamped neighbour gets 0.5 - 0.2, other
+// fluid cells -UNKNOWN; a non-fluid cell with a fluid neighbour gets
+// 1.5 - 0.2, others UNKNOWN.  Pass i: an UNKNOWN cell with a neighbour equal
+// (|a - b| < FLT_MIN, isRealEQ EulerMathTool.cu:88-91) to i + (1.5 - 0.2) - 1
+// becomes i + (1.5 - 0.2); a -UNKNOWN cell with a neighbour equal to
+// -(i + 0.2 - 1.5) becomes -(i - (0.5 - 0.2)).  The constants are evaluated in
+// double and rounded to Real, as the reference's double literals make them.
+// A cell written in pass i holds a pass-i value, which no other cell of pass
+// i compares against, so the reference's in-place parallel passes equal this
+// sequential loop; a pass that writes nothing ends the loop (the later passes
+// would find nothing either).
+template <class R>
+void build_fluid_sdf(const Grid<R>& g, const R* phi, R* out, int passes) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const R unknown = Limits<R>::unknown();
+    const R tiny = std::numeric_limits<R>::min();
+    auto eq = [&](R a, R b) { return std::abs(a - b) < tiny; };
+    for (int i = 0; i < passes; ++i) {
+        bool wrote = false;
+        const R prev_out = R(double(i) + (1.5 - 0.2) - 1), cur_out = R(double(i) + (1.5 - 0.2));
+        const R prev_in = R(-(double(i) + 0.2 - 1.5)), cur_in = R(-(double(i) - (0.5 - 0.2)));
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const size_t c = lin(x, y, z, nx, ny, nz);
+                    const size_t nb[6] = {lin(x - 1, y, z, nx, ny, nz), lin(x + 1, y, z, nx, ny, nz),
+                                          lin(x, y - 1, z, nx, ny, nz), lin(x, y + 1, z, nx, ny, nz),
+                                          lin(x, y, z - 1, nx, ny, nz), lin(x, y, z + 1, nx, ny, nz)};
+                    if (i == 0) {
+                        const bool in = phi[c] < R(0);
+                        bool other = false;
+                        for (size_t n : nb) other = other || ((phi[n] < R(0)) != in);
+                        out[c] = in ? (other ? R(0.5 - 0.2) : -unknown) : (other ? R(1.5 - 0.2) : unknown);
+                        wrote = true;
+                    } else if (out[c] == unknown) {
+                        bool hit = false;
+                        for (size_t n : nb) hit = hit || eq(out[n], prev_out);
+                        if (hit) { out[c] = cur_out; wrote = true; }
+                    } else if (out[c] == -unknown) {
+                        bool hit = false;
+                        for (size_t n : nb) hit = hit || eq(out[n], prev_in);
+                        if (hit) { out[c] = cur_in; wrote = true; }
+                    }
+                }
+        if (!wrote) break;
+    }
+}
+
 // FP64 true residual ||b - A x|| / ||b|| with A from the markers.
 template <class R>
 double true_residual_rel(const Grid<R>& g, const R* A, const R* b, const R* x) {

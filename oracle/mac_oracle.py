@@ -81,6 +81,8 @@ def _setup(L):
                                    C.c_int64, C.POINTER(C.c_int64)]
     L.orc_extrapolate_grid_f32.argtypes = [_F, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I]
     L.orc_extrapolate_fc_f32.argtypes = [g, _F, _F, _F, C.c_int32]
+    L.orc_fluid_density_f32.argtypes = [g, _F, _F, _F]
+    L.orc_fluid_sdf_f32.argtypes = [g, _F, _F, C.c_int32]
     L.orc_pressure_solve_f32.argtypes = [g, C.c_float, C.c_float, _F, _F, _F, _F, _F, _F, _F,
                                          _F, C.c_int32, C.c_double, C.c_int64, C.c_int32,
                                          _F, _F, _F, _F, C.POINTER(OrcStats)]
@@ -253,6 +255,22 @@ def extrapolate_fc(res, u, v, w, layers):
     u, v, w = f32(u).copy(), f32(v).copy(), f32(w).copy()
     used = lib().orc_extrapolate_fc_f32(C.byref(g), fp(u), fp(v), fp(w), layers)
     return u, v, w, used
+
+
+def fluid_density(res, phi, solid):
+    """buildFluidDensity (EulerSolverTool.cu:408-448) on host arrays."""
+    g = grid(res)
+    out = np.empty(int(np.prod(res)), np.float32)
+    lib().orc_fluid_density_f32(C.byref(g), fp(f32(phi)), fp(f32(solid)), fp(out))
+    return out
+
+
+def fluid_sdf(res, phi, passes=1000):
+    """buildFluidSDFInvoker (EulerSolverTool.cu:324-406, 760-785) on host arrays."""
+    g = grid(res)
+    out = np.empty(int(np.prod(res)), np.float32)
+    lib().orc_fluid_sdf_f32(C.byref(g), fp(f32(phi)), fp(out), passes)
+    return out
 
 
 def pressure_solve(res, origin, spacing, dt, u, v, w, us, vs, ws, solid, fluid,

@@ -361,6 +361,18 @@ int orc_extrapolate_fc_f32(const orc_grid* gr, float* u, float* v, float* w, int
     return extrapolate_fc(g, u, v, w, layers);
 }
 
+int orc_fluid_density_f32(const orc_grid* gr, const float* phi, const float* solid, float* out) {
+    Grid<float> g = to_grid<float>(gr);
+    build_fluid_density(g, phi, solid, out);
+    return 0;
+}
+
+int orc_fluid_sdf_f32(const orc_grid* gr, const float* phi, float* out, int32_t passes) {
+    Grid<float> g = to_grid<float>(gr);
+    build_fluid_sdf(g, phi, out, passes);
+    return 0;
+}
+
 // CPressureSolver::solvePressure on host arrays; u,v,w updated in place.
 // markers/A/b/x/div outputs may be null.
 int orc_pressure_solve_f32(const orc_grid* gr, float dt, float density, float* u, float* v,

@@ -250,7 +250,15 @@ int main() {
           "advect_fc");
     for (int k = 0; k < 3; ++k) EXPECT(same_bits(adv.getHostData(k), adv2.getHostData(k)), "advect FCV %d", k);
 
-    // 7. Field copies are deep.
+    // 7. generateCurFluidSDF / generateCurFluidDensity read the library's fields.
+    fast.generateCurFluidSDF();
+    fast.generateCurFluidDensity();
+    EXPECT(same_bits(fast.getFluidSDFField().getHostData(), host_field(ref.get(), MGF_FIELD_FLUID_SDF)),
+           "fluid SDF");
+    EXPECT(same_bits(fast.getFluidDensityField().getHostData(), host_field(ref.get(), MGF_FIELD_FLUID_DENSITY)),
+           "fluid density");
+
+    // 8. Field copies are deep.
     CCellCenteredScalarField a = d.wallField(), b = a;
     std::vector<float> ones(d.wall.size(), 1.0f);
     a.resize(d.res, d.o, d.h, ones.data());

@@ -412,6 +412,23 @@ def test_colour_advection_bitexact(ctx, orc, rk):
     s.close(); full.close(); o.close()
 
 
+@pytest.mark.parametrize("layers", [1000, 4])
+def test_fluid_sdf_and_density_bitexact(ctx, orc, layers):
+    """generateCurFluidSDF / generateCurFluidDensity (GridFluidSolver.cpp:
+    571-613) from the scene's current fluid domain and solid marker, bit-exact
+    against the oracle's restatement of the reference's passes."""
+    spec = scenes.obstacles(30, seed=5)
+    s = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6,
+                               precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID, layers=layers)
+    for step in range(2):
+        if step:
+            s.step(1)
+        phi, solid = s.download(3), s.download(4)
+        assert_bitexact(s.download(17), orc.fluid_density(spec.res, phi, solid), "density step %d" % step)
+        assert_bitexact(s.download(16), orc.fluid_sdf(spec.res, phi, layers), "sdf step %d" % step)
+    s.close()
+
+
 def compare_trajectory(s, o, step, tol_v, what):
     phi_g, phi_o = s.download(3), o.download(3)
     if step == 0:
@@ -661,6 +678,41 @@ def test_obstacles_256_one_step_properties(ctx, precond):
     assert np.array_equal(m, exp_m)
     used = s.extrapolate()
     assert used > 0
+    for fid in (0, 1, 2):
+        a = s.download(fid)
+        assert np.all(np.isfinite(a)) and not np.any(a == FLT_MAX)
+    s.close()
+
+
+def test_large_512_steps_properties(ctx):
+    """The large single scene: 512^3 interior (514^3 padded, 135.8M cells,
+    408M faces), a seeded fluid block and 8 spheres, mixed-precision
+    multigrid PCG.  One step by stages with the projection invariant checked,
+    then a full step: both converge to the 1e-6 FP64 residual, markers follow
+    their definition, and every face holds a finite extrapolated value."""
+    spec = scenes.obstacles(512, seed=0, nspheres=8)
+    s = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6,
+                               precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID)
+    s.update_boundaries()
+    s.advect()
+    s.apply_forces()
+    st = s.pressure_solve()
+    assert st["converged"] == 1 and st["residual_rel"] <= 1e-6, st
+    m = s.download(8)
+    phi = s.download(3)
+    solid = s.download(4) < 0
+    assert np.array_equal(m, np.where(solid, 2, np.where(phi < 0, 1, 0)).astype(np.float32))
+    del phi, solid
+    bn = np.linalg.norm(s.download(10).astype(np.float64))
+    div1 = s.download(12)
+    g = spec.res[0]
+    inner = (m == 1).reshape(g, g, g)
+    inner[-1, :, :] = False; inner[:, -1, :] = False; inner[:, :, -1] = False
+    assert bn > 0 and np.linalg.norm(div1.reshape(g, g, g)[inner].astype(np.float64)) <= 1e-4 * bn
+    del m, div1, inner
+    assert s.extrapolate() > 0
+    st2 = s.step(1)
+    assert st2["converged"] == 1 and st2["residual_rel"] <= 1e-6, st2
     for fid in (0, 1, 2):
         a = s.download(fid)
         assert np.all(np.isfinite(a)) and not np.any(a == FLT_MAX)

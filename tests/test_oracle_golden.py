@@ -463,3 +463,53 @@ def test_scene_colour_is_advected_with_the_old_velocity(orc):
     for k in range(3):
         assert np.allclose(c.download(13 + k), 0.25 * (k + 1), atol=1e-6)
     s.close(); c.close()
+
+
+def test_fluid_density_closed_form(orc):
+    """buildFluidDensity on a fluid block against walls: fluid cells touching
+    air are FluidSurfaceDensity (0.2), fluid cells whose neighbours are fluid
+    or solid are 1, solid and air cells 0 (a solid neighbour is not air)."""
+    n = 10
+    res = (n, n, n)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    wall = (x == 0) | (y == 0) | (z == 0) | (x == n - 1) | (y == n - 1) | (z == n - 1)
+    solid = np.where(wall, -1.0, np.finfo(np.float32).max).astype(np.float32)
+    fluid = (~wall) & (x <= 4) & (y <= 6)
+    phi = np.where(fluid, -500.0, 500.0).astype(np.float32)
+    got = orc.fluid_density(res, phi.ravel(), solid.ravel()).reshape(res)
+    surface = fluid & ((x == 4) | (y == 6))
+    exp = np.where(surface, np.float32(0.2), np.where(fluid, 1.0, 0.0)).astype(np.float32)
+    assert np.array_equal(got, exp)
+    # a fluid cell inside a solid is not fluid
+    phi2 = np.full(n ** 3, -500.0, np.float32)
+    assert np.all(orc.fluid_density(res, phi2, solid.ravel()).reshape(res)[wall] == 0)
+
+
+@pytest.mark.parametrize("passes", [1000, 3])
+def test_fluid_sdf_closed_form(orc, passes):
+    """buildFluidSDF on a fluid slab z < k: inside cells at distance d from the
+    last fluid layer hold 0.5-0.2 (d = 0) and -(d - (0.5-0.2)) below; outside
+    cells d layers beyond the first air layer hold d + (1.5-0.2), up to
+    passes-1 layers (the rest keep +-UNKNOWN).  Around one fluid cell the
+    outside value grows with the L1 distance."""
+    U = np.finfo(np.float32).max
+    nx, ny, nz, k = 4, 3, 16, 6
+    phi = np.where(np.arange(nz)[:, None, None] < k, -500.0, 500.0) * np.ones((nz, ny, nx))
+    got = orc.fluid_sdf((nx, ny, nz), phi.astype(np.float32).ravel(), passes).reshape(nz, ny, nx)
+    for z in range(nz):
+        if z < k:
+            d = k - 1 - z
+            e = np.float32(0.5 - 0.2) if d == 0 else (np.float32(-(d - (0.5 - 0.2))) if d < passes else -U)
+        else:
+            d = z - k
+            e = np.float32(d + (1.5 - 0.2)) if d < passes else U
+        assert np.all(got[z] == e), (z, got[z].ravel()[:3], e)
+    n = 9
+    phi = np.full((n, n, n), 500.0, np.float32)
+    phi[4, 4, 4] = -500.0
+    got = orc.fluid_sdf((n, n, n), phi.ravel(), 1000).reshape(n, n, n)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    l1 = np.abs(x - 4) + np.abs(y - 4) + np.abs(z - 4)
+    exp = (l1 - 1 + (1.5 - 0.2)).astype(np.float32)
+    exp[4, 4, 4] = np.float32(0.5 - 0.2)
+    assert np.array_equal(got, exp)
