@@ -414,7 +414,9 @@ def main():
             "traffic": traffic_for(dominant[0]),
             "bytes_per_launch": dominant[1]["bytes_per_launch"],
             "ms_per_launch": dominant[1]["ms_per_launch"],
-            "note": "HIP events around the kernel's launches in %d ordinary steps after the timed ones" % probe_steps,
+            "note": ("HIP events recorded at each launch's own start and end (hipExtLaunchKernelGGL) during %d "
+                     "ordinary steps after the timed ones; CG/multigrid launches that returned at once after "
+                     "convergence are left out" % probe_steps),
             "kernels": ktab,
         },
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),

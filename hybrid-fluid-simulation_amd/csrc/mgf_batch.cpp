@@ -205,9 +205,15 @@ mgf_status mgf_batch_step(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t
                                       step_ms ? step_ms + size_t(i) * size_t(nsteps) : nullptr, &err[size_t(i)]);
         }
     };
+    // Threads that cannot be started leave their share to the ones that run
+    // (the calling thread always works), so a thread-creation failure only
+    // narrows the overlap; nothing crosses the C boundary as an exception.
     std::vector<std::thread> pool;
-    pool.reserve(size_t(W - 1));
-    for (int w = 1; w < W; ++w) pool.emplace_back(body);
+    try {
+        pool.reserve(size_t(W - 1));
+        for (int w = 1; w < W; ++w) pool.emplace_back(body);
+    } catch (...) {
+    }
     body();
     for (auto& t : pool) t.join();
     mgf_status first = MGF_OK;
@@ -227,10 +233,25 @@ mgf_status mgf_server_create(mgf_context ctx, int64_t hbm_budget_bytes, int32_t 
     *out = nullptr;
     if (hbm_budget_bytes <= 0) return fail(MGF_ERR_INVALID, "budget must be > 0");
     const int W = slots <= 0 ? kDefaultWorkers : std::min<int>(slots, kMaxWorkers);
-    auto* srv = new mgf_server_s;
-    srv->ctx = ctx;
-    srv->budget = hbm_budget_bytes;
-    for (int w = 0; w < W; ++w) srv->workers.emplace_back([srv] { srv->work(); });
+    mgf_server_s* srv = nullptr;
+    try {
+        srv = new mgf_server_s;
+        srv->ctx = ctx;
+        srv->budget = hbm_budget_bytes;
+        for (int w = 0; w < W; ++w) srv->workers.emplace_back([srv] { srv->work(); });
+    } catch (...) {
+        // no exception crosses the C boundary: stop the workers that started
+        if (srv) {
+            {
+                std::lock_guard<std::mutex> lk(srv->mu);
+                srv->stop = true;
+            }
+            srv->cv_work.notify_all();
+            for (auto& t : srv->workers) t.join();
+            delete srv;
+        }
+        return fail(MGF_ERR_DEVICE, "could not start the server's worker threads");
+    }
     *out = srv;
     return MGF_OK;
 }
@@ -311,13 +332,17 @@ mgf_status mgf_server_submit(mgf_server srv, mgf_scene scene, int32_t nsteps, in
     auto& r = it->second;
     srv->resident += bytes - r.bytes;
     r.bytes = bytes;
-    const int64_t t = srv->next_ticket++;
-    r.pending.push_back({t, nsteps, now_ms()});
-    ++srv->queued;
-    if (!r.running && r.pending.size() == 1) {
-        srv->ready.push_back(scene);
-        srv->cv_work.notify_one();
+    const int64_t t = srv->next_ticket;
+    try {
+        r.pending.push_back({t, nsteps, now_ms()});
+        if (!r.running && r.pending.size() == 1) srv->ready.push_back(scene);
+    } catch (...) {
+        if (!r.pending.empty() && r.pending.back().ticket == t) r.pending.pop_back();
+        return fail(MGF_ERR_NOMEM, "request queue allocation failed");
     }
+    ++srv->next_ticket;
+    ++srv->queued;
+    srv->cv_work.notify_one();
     if (ticket) *ticket = t;
     return MGF_OK;
 }

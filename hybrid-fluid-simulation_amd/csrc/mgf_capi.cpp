@@ -16,6 +16,8 @@ using namespace mgf;
 namespace mgf {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+static thread_local LaunchProbe g_launch_probe;
+LaunchProbe& launch_probe() { return g_launch_probe; }
 }  // namespace mgf
 
 namespace {
@@ -136,25 +138,29 @@ int probe_begin(mgf_scene s) {
         s->probe_kind.push_back(0);
         s->probe_bytes.push_back(0.0);
     }
-    if (hipEventRecord(s->probe_ev[2 * k], s->stream) != hipSuccess) {
-        (void)hipGetLastError();
-        return -1;
-    }
+    // armed for the next launch of this thread: the probed kernel
+    LaunchProbe& p = launch_probe();
+    p.start = s->probe_ev[2 * k];
+    p.stop = s->probe_ev[2 * k + 1];
+    p.used = false;
     return int(k);
 }
 
 void probe_end(mgf_scene s, int k, int kind, double bytes) {
     if (k < 0) return;
-    if (hipEventRecord(s->probe_ev[2 * size_t(k) + 1], s->stream) != hipSuccess) {
-        (void)hipGetLastError();
-        return;
-    }
+    LaunchProbe& p = launch_probe();
+    const bool launched = p.used;
+    p.start = p.stop = nullptr;
+    p.used = false;
+    if (!launched) return;   // nothing was launched (an empty layer)
     s->probe_kind[size_t(k)] = kind;
     s->probe_bytes[size_t(k)] = bytes;
     s->probe_used = size_t(k) + 1;
 }
 
-// The stream is synchronised.
+// The stream is synchronised.  A CG or multigrid launch that returned at once
+// because the solve had converged (shorter than a quarter of the longest
+// launch of its kind seen so far) moved no data and is left out.
 void probe_collect(mgf_scene s) {
     for (size_t k = 0; k < s->probe_used; ++k) {
         float t = 0.0f;
@@ -163,6 +169,9 @@ void probe_collect(mgf_scene s) {
             continue;
         }
         const int kind = s->probe_kind[k];
+        s->probe_max[kind] = std::max(s->probe_max[kind], double(t));
+        const bool may_exit_early = kind != MGF_PROBE_EXT_LAYER && kind != MGF_PROBE_ADVECT_FC;
+        if (may_exit_early && double(t) < 0.25 * s->probe_max[kind]) continue;
         s->probe_ms[kind] += t;
         s->probe_b[kind] += s->probe_bytes[k];
         s->probe_n[kind] += 1;
@@ -509,10 +518,13 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     // launches), bounded by `chunk`
     const int chunk = d.preconditioner == MGF_PRECOND_MULTIGRID ? 16 : (s->ncell < (1u << 20) ? 32 : 16);
     const int first_chunk = d.preconditioner == MGF_PRECOND_MULTIGRID ? 4 : chunk;
+    // The estimate is rounded to the nearest count: an iteration launched past
+    // convergence costs its whole V-cycle of (early-returning) launches, while
+    // one too few costs only another read of the scalars.
     double log_rho = 0.0;   // log of the per-iteration factor of r.r, 0 = unknown
     auto predict = [&](double rr_now, double target) {
         if (!(log_rho < 0.0) || !(rr_now > target) || !(target > 0.0)) return first_chunk;
-        const double n = std::ceil(std::log(target / rr_now) / log_rho);
+        const double n = std::floor(std::log(target / rr_now) / log_rho + 0.5);
         return int(std::max(1.0, std::min(double(chunk), n)));
     };
     MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));

@@ -128,6 +128,7 @@ struct mgf_scene_s {
     double probe_ms[MGF_PROBE_COUNT] = {};
     double probe_b[MGF_PROBE_COUNT] = {};
     int64_t probe_n[MGF_PROBE_COUNT] = {};
+    double probe_max[MGF_PROBE_COUNT] = {};   // longest launch of each kind seen
 
     int64_t device_bytes = 0;
     std::vector<void*> allocations;
@@ -140,6 +141,16 @@ constexpr int kMaxBlocks = 4096;   // fused reductions use at most this many blo
 
 // Error helpers.
 void set_error(const std::string& msg);
+
+// Kernel probes: start/stop events armed for the next kernel launch of this
+// host thread.  The launch records them at the kernel's own start and end,
+// so their interval is the kernel's execution alone, without the queue's
+// dispatch gaps; `used` tells that a launch consumed them.
+struct LaunchProbe {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool used = false;
+};
+LaunchProbe& launch_probe();
 
 // Device bytes a scene with this descriptor holds once it has stepped (the
 // preconditioner's levels included, solids excluded); nothing is allocated.

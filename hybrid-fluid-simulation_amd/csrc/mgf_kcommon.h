@@ -2,6 +2,8 @@
 // block numbering, FP64 wave/block reductions, launch wrapper.
 #pragma once
 
+#include <hip/hip_ext.h>
+
 #include "mgf_internal.h"
 
 namespace mgf {
@@ -54,10 +56,24 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
     __syncthreads();
 }
 
+// Kernel probes (LaunchProbe, mgf_internal.h): the armed events are recorded
+// at the kernel's own start and end by hipExtLaunchKernelGGL.
+template <class K, class... A>
+inline hipError_t launch_dims(K kern, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, A... args) {
+    LaunchProbe& p = launch_probe();
+    if (p.start) {
+        hipExtLaunchKernelGGL(kern, grid, block, shmem, s, p.start, p.stop, 0u, args...);
+        p.start = p.stop = nullptr;
+        p.used = true;
+    } else {
+        hipLaunchKernelGGL(kern, grid, block, shmem, s, args...);
+    }
+    return hipGetLastError();
+}
+
 template <class K, class... A>
 inline hipError_t launch_k(K kern, uint32_t grid, hipStream_t s, A... args) {
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, args...);
-    return hipGetLastError();
+    return launch_dims(kern, dim3(grid), dim3(kBlock), 0u, s, args...);
 }
 #define LAUNCH(kern, grid, ...) launch_k(kern, uint32_t(grid), s, __VA_ARGS__)
 

@@ -691,18 +691,16 @@ hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v,
                             const uint8_t* lmask, uint32_t begin, uint32_t count) {
     if (count == 0) return hipSuccess;
     const Faces F = faces_of(g, u, v, w);
-    hipLaunchKernelGGL(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, list, lmask, begin,
+    return launch_dims(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask, begin,
                        count);
-    return hipGetLastError();
 }
 
 hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
                              const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1) {
     if (n0 + n1 == 0) return hipSuccess;
     const Faces F = faces_of(g, u, v, w);
-    hipLaunchKernelGGL(k_ext_layer2, dim3((n0 + n1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, list, lmask, fmask,
-                       begin, n0, n1);
-    return hipGetLastError();
+    return launch_dims(k_ext_layer2, dim3((n0 + n1 + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask,
+                       fmask, begin, n0, n1);
 }
 
 }  // namespace mgf

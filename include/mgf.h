@@ -245,12 +245,15 @@ MGF_API mgf_status mgf_scene_sync(mgf_scene scene);
 MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
 
 /* Kernel probes (measurement; no counterpart in the reference).  While
- * enabled, the launches of the kinds below are bracketed by HIP events on the
- * scene's stream during ordinary steps and stage calls.  mgf_scene_kernel_probes
- * returns, per kind, the summed duration in ms, the number of launches and
- * the algorithmic bytes of those launches (each value read or written once;
- * padded-grid cells), accumulated since the last reset.  Arrays have
- * MGF_PROBE_COUNT entries; any pointer may be null. */
+ * enabled, the launches of the kinds below record HIP events at the kernel's
+ * own start and end (hipExtLaunchKernelGGL on the scene's stream) during
+ * ordinary steps and stage calls.  mgf_scene_kernel_probes returns, per kind,
+ * the summed duration in ms, the number of launches and the algorithmic bytes
+ * of those launches (each value read or written once; padded-grid cells),
+ * accumulated since the last reset.  CG and multigrid launches that returned
+ * at once after convergence (under a quarter of the kind's longest launch)
+ * are not counted.  Arrays have MGF_PROBE_COUNT entries; any pointer may be
+ * null. */
 #define MGF_PROBE_CG_PAP      0  /* p / A p kernel with its scalar finish      */
 #define MGF_PROBE_CG_UPDATE   1  /* x / r update kernel with its scalar finish */
 #define MGF_PROBE_MG_SMOOTH   2  /* level-0 Jacobi sweep (not the last one)    */
