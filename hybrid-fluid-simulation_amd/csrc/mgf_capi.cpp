@@ -693,17 +693,23 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipStreamSynchronize(st));
     const int32_t top = int32_t(s->host_offs[cap + 2]);
-    // two layers per launch; the last one alone when their count is odd
-    for (int32_t L = 1; L <= top; L += 2) {
+    // Small layers are launch-bound: two of them share a launch (the second
+    // recomputes its previous-layer neighbours, several loads per face).
+    // Large layers are bound by their gathers, which the pairing would
+    // multiply, so they run one per launch.
+    constexpr uint32_t kPairMax = 65536;   // faces of two layers launched together
+    for (int32_t L = 1; L <= top;) {
         const uint32_t b = s->host_offs[L], m = s->host_offs[L + 1];
         const uint32_t e = L + 1 <= top ? s->host_offs[L + 2] : m;
+        const bool pair = L + 1 <= top && e - b <= kPairMax;
         const int pk = probe_begin(s);
-        if (L + 1 <= top)
+        if (pair)
             MGF_HIP(launch_ext_layer2(st, s->g, u, v, w, s->list, s->lmask, s->fmask, b, m - b, e - m));
         else
             MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->list, s->lmask, b, m - b));
         // per entry: list 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
-        probe_end(s, pk, MGF_PROBE_EXT_LAYER, 13.0 * double(e - b));
+        probe_end(s, pk, MGF_PROBE_EXT_LAYER, 13.0 * double((pair ? e : m) - b));
+        L += pair ? 2 : 1;
     }
     *used = top;
     return MGF_OK;

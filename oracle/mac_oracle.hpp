@@ -759,6 +759,231 @@ void build_fluid_sdf(const Grid<R>& g, const R* phi, R* out, int passes) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Particles (MPhysics/EulerParticles.cpp, MPhysics/EulerParticlesTool.cu):
+// positions and per-particle vectors are interleaved x,y,z (3*i + k).
+//
+// Transfer kernels (EulerMathTool.cu:268-294).  The reference writes them with
+// double literals, so each is evaluated in double from the Real argument and
+// rounded to Real once.
+template <class R> inline R pg_linear(R v) {
+    const R a = std::abs(v);
+    return a < R(1) ? R(1.0 - double(a)) : R(0);
+}
+template <class R> inline R pg_quadratic(R v) {
+    const R a = std::abs(v);
+    if (a < R(0.5)) return R(0.75 - double(R(a * a)));
+    if (a < R(1.5)) return R(0.5 * (1.5 - double(a)) * (1.5 - double(a)));
+    return R(0);
+}
+template <class R> inline R pg_cubic(R v) {
+    const R a = std::abs(v);
+    if (a < R(1)) return R(0.5 * double(a) * double(a) * double(a) - double(R(a * a)) + 2.0 / 3.0);
+    if (a < R(2)) return R(1.0 / 6.0 * (2.0 - double(a)) * (2.0 - double(a)) * (2.0 - double(a)));
+    return R(0);
+}
+
+// EPGTransferAlgorithm (MPhysics/EnumType.h:19-26).
+enum PgAlg { PG_P2GSUM = 0, PG_G2PNEAREST = 1, PG_LINEAR = 2, PG_QUADRATIC = 3, PG_CUBIC = 4 };
+
+// A cell-centred grid of extents n, origin o, spacing h (a face component is
+// one with the component's extents and origin shifted by -h/2).
+template <class R> struct PGrid {
+    int nx, ny, nz;
+    R ox, oy, oz, hx, hy, hz;
+};
+
+// transPos2Index (EulerMathTool.cu:29-40): truncation of (p - o) / h,
+// clamped to the grid.
+template <class R> inline size_t pos_cell(const PGrid<R>& g, R px, R py, R pz) {
+    const int ix = limit_index(int((px - g.ox) / g.hx), g.nx);
+    const int iy = limit_index(int((py - g.oy) / g.hy), g.ny);
+    const int iz = limit_index(int((pz - g.oz) / g.hz), g.nz);
+    return lin(ix, iy, iz, g.nx, g.ny, g.nz);
+}
+
+// The support of a particle: lower corner index and per-axis weights of the
+// kernel (EulerParticlesTool.cu:79-198 and 198-290 share this prologue).
+template <class R> struct Stencil {
+    int i0[3];
+    R w[3][4];
+    int n;   // 2 (linear) or 4 (quadratic, cubic): corners i0-1 .. i0+2 for n = 4
+};
+template <class R> inline Stencil<R> pg_stencil(const PGrid<R>& g, R px, R py, R pz, int alg) {
+    const R rel[3] = {px - g.ox - g.hx * R(0.5), py - g.oy - g.hy * R(0.5), pz - g.oz - g.hz * R(0.5)};
+    const R h[3] = {g.hx, g.hy, g.hz};
+    Stencil<R> st{};
+    for (int k = 0; k < 3; ++k) {
+        st.i0[k] = floor_cuda(rel[k] / h[k]);
+        const R f = (rel[k] - R(st.i0[k]) * h[k]) / h[k];
+        if (alg == PG_LINEAR) {
+            st.n = 2;
+            st.w[k][0] = pg_linear(f);
+            st.w[k][1] = pg_linear(R(double(f) - 1.0));
+        } else {
+            st.n = 4;
+            for (int q = 0; q < 4; ++q) {
+                const R a = R(double(f) + 1.0 - double(q));
+                st.w[k][q] = alg == PG_QUADRATIC ? pg_quadratic(a) : pg_cubic(a);
+            }
+        }
+    }
+    return st;
+}
+
+// accumulateParticles2CCSField (EulerParticlesTool.cu:79-196): each particle
+// adds weight * value to the clamped corners of its support and the weight
+// to the weight field (P2GSUM: the value to its own cell, no weight).  The
+// reference adds with atomics in no fixed order; here particles are added in
+// index order, corners z, y, x, so sums agree to rounding.  Then, when
+// normalising, value /= weight where weight > 0 (normalizeCCSField :290-308).
+template <class R>
+void p2g(const PGrid<R>& g, size_t np, const R* pos, const R* val, int span, int off, int alg, R* field,
+         R* weight, bool normalize) {
+    const size_t n = size_t(g.nx) * g.ny * g.nz;
+    std::fill(field, field + n, R(0));
+    std::fill(weight, weight + n, R(0));
+    for (size_t p = 0; p < np; ++p) {
+        const R px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+        const R v = val[size_t(span) * p + size_t(off)];
+        if (alg == PG_P2GSUM) {
+            field[pos_cell(g, px, py, pz)] += v;
+            continue;
+        }
+        if (alg != PG_LINEAR && alg != PG_QUADRATIC && alg != PG_CUBIC) continue;
+        const Stencil<R> st = pg_stencil(g, px, py, pz, alg);
+        const int lo = st.n == 2 ? 0 : -1;
+        for (int z = 0; z < st.n; ++z)
+            for (int y = 0; y < st.n; ++y)
+                for (int x = 0; x < st.n; ++x) {
+                    const R wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
+                    const size_t c = lin(st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo, g.nx, g.ny, g.nz);
+                    field[c] += wt * v;
+                    weight[c] += wt;
+                }
+    }
+    if (normalize)
+        for (size_t c = 0; c < n; ++c)
+            if (weight[c] > R(0)) field[c] /= weight[c];
+}
+
+// accumulateCCSField2Particles (EulerParticlesTool.cu:198-288): the value at
+// the particle, summed from 0 in corner order z, y, x (G2PNEAREST: the value
+// of its cell).
+template <class R>
+void g2p(const PGrid<R>& g, size_t np, const R* pos, const R* field, int alg, R* val, int span, int off) {
+    for (size_t p = 0; p < np; ++p) {
+        const R px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+        R acc = R(0);
+        if (alg == PG_G2PNEAREST) {
+            acc = field[pos_cell(g, px, py, pz)];
+        } else if (alg == PG_LINEAR || alg == PG_QUADRATIC || alg == PG_CUBIC) {
+            const Stencil<R> st = pg_stencil(g, px, py, pz, alg);
+            const int lo = st.n == 2 ? 0 : -1;
+            for (int z = 0; z < st.n; ++z)
+                for (int y = 0; y < st.n; ++y)
+                    for (int x = 0; x < st.n; ++x) {
+                        const R wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
+                        acc += wt * field[lin(st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo, g.nx, g.ny, g.nz)];
+                    }
+        }
+        val[size_t(span) * p + size_t(off)] = acc;
+    }
+}
+
+// The three components of a face-centred field as cell-centred grids
+// (tranferParticles2FCVFieldInvoker, EulerParticlesTool.cu:601-711).
+template <class R> inline PGrid<R> face_grid(const Grid<R>& g, int c) {
+    PGrid<R> f{g.nx + (c == 0), g.ny + (c == 1), g.nz + (c == 2), g.ox, g.oy, g.oz, g.hx, g.hy, g.hz};
+    if (c == 0) f.ox = g.ox - g.hx / 2;
+    if (c == 1) f.oy = g.oy - g.hy / 2;
+    if (c == 2) f.oz = g.oz - g.hz / 2;
+    return f;
+}
+template <class R> inline PGrid<R> cell_grid(const Grid<R>& g) {
+    return PGrid<R>{g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz};
+}
+
+// statisticalFluidDensity (EulerParticlesTool.cu:27-46): +1 per particle in
+// its cell; buildFluidMarkers (:48-77, 366-411): 2 where the solid marker is
+// < 0, else 1 where a particle lies, else 0.
+template <class R>
+void particle_counts(const Grid<R>& g, size_t np, const R* pos, R* out) {
+    const PGrid<R> c = cell_grid(g);
+    std::fill(out, out + g.cells(), R(0));
+    for (size_t p = 0; p < np; ++p) out[pos_cell(c, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2])] += R(1);
+}
+template <class R>
+void particle_markers(const Grid<R>& g, const R* solid, size_t np, const R* pos, R* out) {
+    particle_counts(g, np, pos, out);
+    for (size_t c = 0; c < g.cells(); ++c) out[c] = solid[c] < R(0) ? R(2) : (out[c] > R(0) ? R(1) : R(0));
+}
+
+// CEulerParticles::advectParticlesInVelField (EulerParticles.cpp:231-325):
+// substeps of cfl * min(h) / max|v| (v = the velocity components at the
+// particles), the last two shortened to end at dt; RK1/2/3 per substep with
+// the axpys of the positions (axpyReal: no early return), then the positions
+// clamped to the grid box when a boundary SDF of the grid's resolution is
+// given (fixParticlesPosWithBoundarys, EulerParticlesTool.cu:310-333, 870-894:
+// the reference launches one thread per *cell*, so only the first
+// min(cells, particles) particles are clamped; here the same ones).
+template <class R>
+void advect_particles(const Grid<R>& g, const R* u, const R* v, const R* w, size_t np, R* pos, R dt, R cfl,
+                      int order, bool fix) {
+    std::vector<R> v1(3 * np), v2(3 * np), v3(3 * np), mid(3 * np), tf(3 * np);
+    auto sample = [&](const R* at, R* out) {
+        for (size_t p = 0; p < np; ++p)
+            sample_fc(g, u, v, w, at[3 * p], at[3 * p + 1], at[3 * p + 2], out[3 * p], out[3 * p + 1], out[3 * p + 2]);
+    };
+    auto axpy = [&](const R* x, R* y, R a) {
+        for (size_t i = 0; i < 3 * np; ++i) y[i] = fma_r(a, x[i], y[i]);
+    };
+    const R minh = std::min(std::min(g.hx, g.hy), g.hz);
+    R t = R(0);
+    bool finished = false;
+    while (!finished && np > 0) {
+        sample(pos, v1.data());
+        R vmax = R(0);
+        for (size_t i = 0; i < 3 * np; ++i) vmax = std::max(vmax, std::abs(v1[i]));
+        R step = vmax != R(0) ? cfl * minh / vmax : dt;
+        if (t + step >= dt) {
+            step = dt - t;
+            finished = true;
+        } else if (t + 2 * step >= dt) {
+            step = R(0.5 * double(dt - t));
+        }
+        if (order == 1) {
+            axpy(v1.data(), pos, step);
+        } else if (order == 2) {
+            const R half = R(0.5 * double(step));
+            mid.assign(pos, pos + 3 * np);
+            axpy(v1.data(), mid.data(), half);
+            sample(mid.data(), v2.data());
+            axpy(v2.data(), pos, step);
+        } else if (order == 3) {
+            const R half = R(0.5 * double(step)), tq = R(0.75 * double(step));
+            const R c1 = R(2.0 / 9.0 * double(step)), c2 = R(3.0 / 9.0 * double(step)), c3 = R(4.0 / 9.0 * double(step));
+            mid.assign(pos, pos + 3 * np);
+            tf.assign(pos, pos + 3 * np);
+            axpy(v1.data(), mid.data(), half);
+            sample(mid.data(), v2.data());
+            axpy(v2.data(), tf.data(), tq);
+            sample(tf.data(), v3.data());
+            axpy(v1.data(), pos, c1);
+            axpy(v2.data(), pos, c2);
+            axpy(v3.data(), pos, c3);
+        }
+        t += step;
+    }
+    if (fix) {
+        const R mn[3] = {g.ox, g.oy, g.oz};
+        const R mx[3] = {g.ox + R(g.nx) * g.hx, g.oy + R(g.ny) * g.hy, g.oz + R(g.nz) * g.hz};
+        const size_t m = std::min(np, g.cells());
+        for (size_t p = 0; p < m; ++p)
+            for (int k = 0; k < 3; ++k) pos[3 * p + k] = std::max(mn[k], std::min(pos[3 * p + k], mx[k]));
+    }
+}
+
 // FP64 true residual ||b - A x|| / ||b|| with A from the markers.
 template <class R>
 double true_residual_rel(const Grid<R>& g, const R* A, const R* b, const R* x) {

@@ -82,6 +82,11 @@ def _setup(L):
     L.orc_extrapolate_grid_f32.argtypes = [_F, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I]
     L.orc_extrapolate_fc_f32.argtypes = [g, _F, _F, _F, C.c_int32]
     L.orc_fluid_density_f32.argtypes = [g, _F, _F, _F]
+    L.orc_p2g_f32.argtypes = [g, C.c_int32, C.c_int64, _F, _F, C.c_int32, C.c_int32, C.c_int32, _F, _F, C.c_int32]
+    L.orc_g2p_f32.argtypes = [g, C.c_int32, C.c_int64, _F, _F, C.c_int32, _F, C.c_int32, C.c_int32]
+    L.orc_particle_markers_f32.argtypes = [g, _F, C.c_int64, _F, _F, C.c_int32]
+    L.orc_advect_particles_f32.argtypes = [g, _F, _F, _F, C.c_int64, _F, C.c_float, C.c_float, C.c_int32,
+                                           C.c_int32]
     L.orc_fluid_sdf_f32.argtypes = [g, _F, _F, C.c_int32]
     L.orc_pressure_solve_f32.argtypes = [g, C.c_float, C.c_float, _F, _F, _F, _F, _F, _F, _F,
                                          _F, C.c_int32, C.c_double, C.c_int64, C.c_int32,
@@ -255,6 +260,52 @@ def extrapolate_fc(res, u, v, w, layers):
     u, v, w = f32(u).copy(), f32(v).copy(), f32(w).copy()
     used = lib().orc_extrapolate_fc_f32(C.byref(g), fp(u), fp(v), fp(w), layers)
     return u, v, w, used
+
+
+PG = dict(p2gsum=0, g2pnearest=1, linear=2, quadratic=3, cubic=4)   # EPGTransferAlgorithm
+
+
+def _field_extent(res, kind):
+    nx, ny, nz = res
+    return {0: nx * ny * nz, 1: (nx + 1) * ny * nz, 2: nx * (ny + 1) * nz, 3: nx * ny * (nz + 1)}[kind]
+
+
+def p2g(res, origin, spacing, kind, pos, val, alg, span=1, off=0, normalize=True):
+    """Particles to a cell-centred grid (kind 0) or to the x/y/z face grid
+    (kind 1/2/3) of the grid res; returns (field, weight)."""
+    g = grid(res, origin, spacing)
+    n = _field_extent(res, kind)
+    f, wt = np.empty(n, np.float32), np.empty(n, np.float32)
+    pos = f32(pos).ravel()
+    lib().orc_p2g_f32(C.byref(g), kind, pos.size // 3, fp(pos), fp(f32(val).ravel()), span, off, PG[alg], fp(f),
+                      fp(wt), 1 if normalize else 0)
+    return f, wt
+
+
+def g2p(res, origin, spacing, kind, pos, field, alg, span=1, off=0, out=None):
+    g = grid(res, origin, spacing)
+    pos = f32(pos).ravel()
+    np_ = pos.size // 3
+    out = np.zeros(span * np_, np.float32) if out is None else f32(out).copy()
+    lib().orc_g2p_f32(C.byref(g), kind, np_, fp(pos), fp(f32(field)), PG[alg], fp(out), span, off)
+    return out
+
+
+def particle_markers(res, origin, spacing, solid, pos, counts_only=False):
+    g = grid(res, origin, spacing)
+    pos = f32(pos).ravel()
+    out = np.empty(int(np.prod(res)), np.float32)
+    lib().orc_particle_markers_f32(C.byref(g), fp(f32(solid) if solid is not None else out), pos.size // 3,
+                                   fp(pos), fp(out), 1 if counts_only else 0)
+    return out
+
+
+def advect_particles(res, origin, spacing, u, v, w, pos, dt, cfl, order, fix=False):
+    g = grid(res, origin, spacing)
+    pos = f32(pos).ravel().copy()
+    lib().orc_advect_particles_f32(C.byref(g), fp(f32(u)), fp(f32(v)), fp(f32(w)), pos.size // 3, fp(pos),
+                                   C.c_float(dt), C.c_float(cfl), order, 1 if fix else 0)
+    return pos
 
 
 def fluid_density(res, phi, solid):

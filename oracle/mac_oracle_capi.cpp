@@ -361,6 +361,39 @@ int orc_extrapolate_fc_f32(const orc_grid* gr, float* u, float* v, float* w, int
     return extrapolate_fc(g, u, v, w, layers);
 }
 
+// ---- particles, FP32 ------------------------------------------------------
+// field_kind: 0 = cell-centred grid `gr`; 1, 2, 3 = the x, y, z face grid of `gr`.
+static PGrid<float> pgrid_of(const orc_grid* gr, int field_kind) {
+    Grid<float> g = to_grid<float>(gr);
+    return field_kind == 0 ? cell_grid(g) : face_grid(g, field_kind - 1);
+}
+
+int orc_p2g_f32(const orc_grid* gr, int32_t field_kind, int64_t np, const float* pos, const float* val,
+                int32_t span, int32_t off, int32_t alg, float* field, float* weight, int32_t normalize) {
+    p2g(pgrid_of(gr, field_kind), size_t(np), pos, val, span, off, alg, field, weight, normalize != 0);
+    return 0;
+}
+
+int orc_g2p_f32(const orc_grid* gr, int32_t field_kind, int64_t np, const float* pos, const float* field,
+                int32_t alg, float* val, int32_t span, int32_t off) {
+    g2p(pgrid_of(gr, field_kind), size_t(np), pos, field, alg, val, span, off);
+    return 0;
+}
+
+int orc_particle_markers_f32(const orc_grid* gr, const float* solid, int64_t np, const float* pos, float* out,
+                             int32_t counts_only) {
+    Grid<float> g = to_grid<float>(gr);
+    if (counts_only) particle_counts(g, size_t(np), pos, out);
+    else particle_markers(g, solid, size_t(np), pos, out);
+    return 0;
+}
+
+int orc_advect_particles_f32(const orc_grid* gr, const float* u, const float* v, const float* w, int64_t np,
+                             float* pos, float dt, float cfl, int32_t order, int32_t fix) {
+    advect_particles(to_grid<float>(gr), u, v, w, size_t(np), pos, dt, cfl, order, fix != 0);
+    return 0;
+}
+
 int orc_fluid_density_f32(const orc_grid* gr, const float* phi, const float* solid, float* out) {
     Grid<float> g = to_grid<float>(gr);
     build_fluid_density(g, phi, solid, out);

@@ -535,11 +535,163 @@ def integration_cases():
     return [c1, c2]
 
 
+# ---------------------------------------------------------------------------
+# Particle transfers (Euler/UnitTest-ParticlesAdvection/UnitTest-ParticlesAdvection.cu).
+PT = T + "Euler/UnitTest-ParticlesAdvection/UnitTest-ParticlesAdvection.cu"
+A_RES, A_O, A_H = (4, 4, 1), (0.0, 20.0, -30.0), (30.0, 20.0, 10.0)
+B_RES = (1, 4, 4)
+FIELD_4x4 = [64.0, 64.0, 64.0, 0.0, 64.0, 64.0, 96.0, 128.0, 64.0, 96.0, 128.0, 128.0, 0.0, 128.0, 128.0, 128.0]
+FIELD_4x4_B = [36.0, 36.0, 36.0, 0.0, 36.0, 36.0, 54.0, 72.0, 36.0, 54.0, 72.0, 72.0, 0.0, 72.0, 72.0, 72.0]
+
+
+def cells_particles(res, origin, spacing, cells, per_cell):
+    """Particles inside the given cells, per_cell each, in the order of
+    CEulerParticles::generateParticlesInFluid (cells x-fastest, then the
+    particles of a cell).  The tests that use generated particles check values
+    that do not depend on where in its cell a particle lies; the fractions
+    here are fixed instead of the reference's rand()."""
+    nx, ny, _ = res
+    pos = []
+    for c in sorted(cells):
+        x, y, z = c % nx, (c // nx) % ny, c // (nx * ny)
+        for i in range(per_cell):
+            f = [(0.13 + 0.71 * ((i * 3 + a) % 7) / 7.0) for a in range(3)]
+            pos += [(x + f[0]) * spacing[0] + origin[0], (y + f[1]) * spacing[1] + origin[1],
+                    (z + f[2]) * spacing[2] + origin[2]]
+    return pos
+
+
+def particle_cases():
+    out = []
+    eight = [21, 22, 25, 26, 37, 38, 41, 42]
+    o8, h8 = (-10.0, -20.0, -30.0), (10.0, 20.0, 30.0)
+    # P2G to a cell-centred scalar field
+    pos = cells_particles((4, 4, 4), o8, h8, eight, 8)
+    out.append(dict(name="p2g_ccs_sum1", src=PT + ":116-196", kind="p2g", field=0, alg="p2gsum", res=[4, 4, 4],
+                    origin=list(o8), spacing=list(h8), pos=pos, val=[10.0] * 64, span=1,
+                    expect={str(i): (80.0 if i in eight else 0.0) for i in range(64)}, tol=EPS))
+    pos = cells_particles((2, 2, 1), A_O, A_H, [0], 5)
+    out.append(dict(name="p2g_ccs_sum2", src=PT + ":198-251", kind="p2g", field=0, alg="p2gsum", res=[2, 2, 1],
+                    origin=list(A_O), spacing=list(A_H), pos=pos, val=[1.0, 2.0, 3.0, 4.0, 5.0], span=1,
+                    expect={"0": 15.0, "1": 0.0, "2": 0.0, "3": 0.0}, tol=EPS))
+    posA = [45.0, 50.0, -25.0, 75.0, 70.0, -15.0]
+    posB = [15.0, 60.0, -10.0, 15.0, 60.0, -10.0]
+    e = {str(i): 0.0 for i in range(16)}
+    e.update({"5": 64.0, "10": 128.0})
+    out.append(dict(name="p2g_ccs_linear1", src=PT + ":253-305", kind="p2g", field=0, alg="linear",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=posA, val=[64.0, 128.0], span=1,
+                    expect=e, tol=EPS))
+    e = {str(i): (96.0 if i in (5, 6, 9, 10) else 0.0) for i in range(16)}
+    out.append(dict(name="p2g_ccs_linear2", src=PT + ":307-354", kind="p2g", field=0, alg="linear",
+                    res=list(B_RES), origin=list(A_O), spacing=list(A_H), pos=posB, val=[64.0, 128.0], span=1,
+                    expect=e, tol=EPS))
+    e = {"0": 64.0, "1": 64.0, "2": 64.0, "4": 64.0, "5": 38.0 * 64.0 / 37.0, "6": 96.0, "7": 128.0,
+         "8": 64.0, "9": 96.0, "10": 73.0 * 64.0 / 37.0, "11": 128.0, "13": 128.0, "14": 128.0, "15": 128.0}
+    out.append(dict(name="p2g_ccs_quadratic1", src=PT + ":356-416", kind="p2g", field=0, alg="quadratic",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=posA, val=[64.0, 128.0], span=1,
+                    expect=e, tol=EPS))
+    e = {str(i): (96.0 if i in (5, 6, 9, 10) else 0.0) for i in range(16)}
+    out.append(dict(name="p2g_ccs_quadratic2", src=PT + ":418-465", kind="p2g", field=0, alg="quadratic",
+                    res=list(B_RES), origin=list(A_O), spacing=list(A_H), pos=posB, val=[64.0, 128.0], span=1,
+                    expect=e, tol=EPS))
+    e = {"0": 64.0, "1": 64.0, "2": 64.0, "4": 64.0, "5": (64.0 * 16.0 + 128.0) / 17.0, "6": 96.0, "7": 128.0,
+         "8": 64.0, "9": 96.0, "10": (64.0 + 128.0 * 16.0) / 17.0, "11": 128.0, "13": 128.0, "14": 128.0,
+         "15": 128.0}
+    out.append(dict(name="p2g_ccs_cubic1", src=PT + ":467-519", kind="p2g", field=0, alg="cubic",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=posA, val=[64.0, 128.0], span=1,
+                    expect=e, tol=EPS))
+    out.append(dict(name="p2g_ccs_cubic2", src=PT + ":521-562", kind="p2g", field=0, alg="cubic",
+                    res=list(B_RES), origin=list(A_O), spacing=list(A_H), pos=posB, val=[64.0, 128.0], span=1,
+                    expect={str(i): 96.0 for i in range(16)}, tol=EPS))
+    # G2P from a cell-centred scalar field
+    pos = cells_particles((4, 4, 4), o8, h8, eight, 8)
+    out.append(dict(name="g2p_ccs_nearest1", src=PT + ":1771-1840", kind="g2p", field=0, alg="g2pnearest",
+                    res=[4, 4, 4], origin=list(o8), spacing=list(h8), pos=pos, grid=[float(i) for i in range(64)],
+                    span=1, expect={str(8 * k + q): float(c) for k, c in enumerate(eight) for q in range(8)},
+                    tol=EPS))
+    for name, src, res, ps, fld, alg, ev in [
+            ("g2p_ccs_linear1", "1842-1894", A_RES, [45.0, 50.0, -24.0, 75.0, 70.0, -26.0], FIELD_4x4, "linear",
+             [64.0, 128.0]),
+            ("g2p_ccs_linear2", "1896-1948", B_RES, [15.0, 60.0, -10.0, 15.0, 70.0, -5.0], FIELD_4x4, "linear",
+             [96.0, 128.0]),
+            ("g2p_ccs_quadratic1", "1950-2010", A_RES, [45.0, 50.0, -24.0, 75.0, 70.0, -26.0], FIELD_4x4,
+             "quadratic", [71.0, 121.0]),
+            ("g2p_ccs_quadratic2", "2012-2065", B_RES, [15.0, 50.0, -15.0, 15.0, 70.0, -5.0], FIELD_4x4,
+             "quadratic", [71.0, 121.0]),
+            ("g2p_ccs_cubic1", "2067-2120", A_RES, [45.0, 50.0, -24.0, 75.0, 70.0, -26.0], FIELD_4x4_B, "cubic",
+             [41.0, 67.0]),
+            ("g2p_ccs_cubic2", "2122-2175", B_RES, [15.0, 50.0, -15.0, 15.0, 70.0, -5.0], FIELD_4x4_B, "cubic",
+             [41.0, 67.0])]:
+        out.append(dict(name=name, src=PT + ":" + src, kind="g2p", field=0, alg=alg, res=list(res),
+                        origin=list(A_O), spacing=list(A_H), pos=ps, grid=fld, span=1,
+                        expect={"0": ev[0], "1": ev[1]}, tol=EPS))
+    # P2G to a face-centred vector field (each particle's 3 components to the 3 face grids)
+    o8, h8 = (-10.0, -20.0, -30.0), (10.0, 20.0, 30.0)
+    base = (o8[0] + 1.25 * h8[0], o8[1] + 1.25 * h8[1], -16.0)
+    pos = []
+    for y in range(4):
+        for x in range(4):
+            pos += [base[0] + x * 0.5 * h8[0], base[1] + y * 0.5 * h8[1], base[2]]
+    ex = {str(i): (2.0 if i in (6, 8, 11, 13) else 4.0 if i in (7, 12) else 0.0) for i in range(20)}
+    ey = {str(i): (4.0 if i in (5, 6, 13, 14) else 8.0 if i in (9, 10) else 0.0) for i in range(20)}
+    ez = {str(i): (12.0 if i in (5, 6, 9, 10) else 0.0) for i in range(32)}
+    out.append(dict(name="p2g_fcv_sum1", src=PT + ":1074-1189", kind="p2g_fcv", alg="p2gsum", res=[4, 4, 1],
+                    origin=list(o8), spacing=list(h8), pos=pos, val=[1.0, 2.0, 3.0] * 16,
+                    expect=dict(x=ex, y=ey, z=ez), tol=EPS))
+    base = (-6.0, o8[1] + 1.25 * h8[1], o8[2] + 1.25 * h8[2])
+    pos = []
+    for z in range(4):
+        for y in range(4):
+            pos += [base[0], base[1] + y * 0.5 * h8[1], base[2] + z * 0.5 * h8[2]]
+    ex = {str(i): (4.0 if i in (10, 12, 18, 20) else 0.0) for i in range(32)}
+    ey = {str(i): (4.0 if i in (6, 8, 11, 13) else 8.0 if i in (7, 12) else 0.0) for i in range(20)}
+    ez = {str(i): (6.0 if i in (5, 6, 13, 14) else 12.0 if i in (9, 10) else 0.0) for i in range(20)}
+    out.append(dict(name="p2g_fcv_sum2", src=PT + ":1191-1306", kind="p2g_fcv", alg="p2gsum", res=[1, 4, 4],
+                    origin=list(o8), spacing=list(h8), pos=pos, val=[1.0, 2.0, 3.0] * 16,
+                    expect=dict(x=ex, y=ey, z=ez), tol=EPS))
+    vel = [1.0, 3.0, 5.0, 2.0, 4.0, 6.0]
+    p1 = [30.0, 50.0, -24.0, 60.0, 70.0, -26.0]
+    p2 = [45.0, 40.0, -24.0, 75.0, 60.0, -26.0]
+    p3 = [15.0, 50.0, -20.0, 15.0, 70.0, -10.0]
+
+    def sparse(n, d):
+        e = {str(i): 0.0 for i in range(n)}
+        e.update({str(k): v for k, v in d.items()})
+        return e
+    out.append(dict(name="p2g_fcv_linear1", src=PT + ":1308-1354", kind="p2g_fcv", alg="linear",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=p1, val=vel,
+                    expect=dict(x=sparse(20, {6: 1.0, 12: 2.0})), tol=EPS))
+    out.append(dict(name="p2g_fcv_linear2", src=PT + ":1356-1400", kind="p2g_fcv", alg="linear",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=p2, val=vel,
+                    expect=dict(y=sparse(20, {5: 3.0, 10: 4.0})), tol=EPS))
+    out.append(dict(name="p2g_fcv_linear3", src=PT + ":1402-1446", kind="p2g_fcv", alg="linear",
+                    res=list(B_RES), origin=list(A_O), spacing=list(A_H), pos=p3, val=vel,
+                    expect=dict(z=sparse(20, {5: 5.0, 10: 6.0})), tol=EPS))
+
+    def quad(a, b):
+        return {"0": a, "1": a, "2": a, "4": a, "5": (9.0 / 16.0 * a + 1.0 / 64.0 * b) * 64.0 / 37.0,
+                "6": (a + b) / 2, "7": b, "8": a, "9": (a + b) / 2,
+                "10": (1.0 / 64.0 * a + 9.0 / 16.0 * b) * 64.0 / 37.0, "11": b, "13": b, "14": b, "15": b}
+    qx = {"0": 1.0, "1": 1.0, "2": 1.0, "5": 1.0, "6": (9.0 / 16.0 * 1.0 + 1.0 / 64.0 * 2.0) * 64.0 / 37.0,
+          "7": 1.5, "8": 2.0, "10": 1.0, "11": 1.5, "12": (1.0 / 64.0 * 1.0 + 9.0 / 16.0 * 2.0) * 64.0 / 37.0,
+          "13": 2.0, "16": 2.0, "17": 2.0, "18": 2.0}
+    out.append(dict(name="p2g_fcv_quadratic1", src=PT + ":1448-1500", kind="p2g_fcv", alg="quadratic",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=p1, val=vel, expect=dict(x=qx),
+                    tol=EPS))
+    out.append(dict(name="p2g_fcv_quadratic2", src=PT + ":1502-1553", kind="p2g_fcv", alg="quadratic",
+                    res=list(A_RES), origin=list(A_O), spacing=list(A_H), pos=p2, val=vel,
+                    expect=dict(y=quad(3.0, 4.0)), tol=EPS))
+    out.append(dict(name="p2g_fcv_quadratic3", src=PT + ":1555-1604", kind="p2g_fcv", alg="quadratic",
+                    res=list(B_RES), origin=list(A_O), spacing=list(A_H), pos=p3, val=vel,
+                    expect=dict(z=quad(5.0, 6.0)), tol=EPS))
+    return out
+
+
 def main():
     cases = dict(pressure=pressure_cases(), semilagrangian=semilagrangian_cases(),
                  external_forces=external_forces_cases(), extrapolation=extrapolation_cases(),
                  boundary=boundary_cases(), cg=cg_cases(), fields=field_cases(),
-                 integration=integration_cases())
+                 integration=integration_cases(), particles=particle_cases())
     path = os.path.join(HERE, "reference_cases.json.gz")
     with gzip.open(path, "wt", compresslevel=9) as f:
         json.dump(cases, f, separators=(",", ":"))

@@ -513,3 +513,25 @@ def test_fluid_sdf_closed_form(orc, passes):
     exp = (l1 - 1 + (1.5 - 0.2)).astype(np.float32)
     exp[4, 4, 4] = np.float32(0.5 - 0.2)
     assert np.array_equal(got, exp)
+
+
+def test_particle_transfer_goldens(cases, orc):
+    """Particle <-> grid transfers of the reference's particle tests
+    (UnitTest-ParticlesAdvection.cu): P2G with P2GSUM / LINEAR / QUADRATIC /
+    CUBIC kernels (normalised by the weights), G2P with G2PNEAREST and the same
+    kernels."""
+    for c in cases["particles"]:
+        if c["kind"] == "p2g":
+            f, _ = orc.p2g(c["res"], c["origin"], c["spacing"], c["field"], c["pos"], c["val"], c["alg"],
+                           span=c["span"])
+            check_map(f, c["expect"], c["tol"], c["name"])
+        elif c["kind"] == "g2p":
+            v = orc.g2p(c["res"], c["origin"], c["spacing"], c["field"], c["pos"], c["grid"], c["alg"],
+                        span=c["span"])
+            check_map(v, c["expect"], c["tol"], c["name"])
+        elif c["kind"] == "p2g_fcv":
+            for k, comp in enumerate("xyz"):
+                f, _ = orc.p2g(c["res"], c["origin"], c["spacing"], k + 1, c["pos"], c["val"], c["alg"],
+                               span=3, off=k)
+                if comp in c["expect"]:
+                    check_map(f, c["expect"][comp], c["tol"], c["name"] + ":" + comp)
