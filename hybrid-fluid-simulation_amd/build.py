@@ -21,7 +21,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["mgf_kernels.hip", "mgf_kernels_mg.hip", "mgf_kernels_ext.hip", "mgf_capi.cpp", "mgf_batch.cpp"]
+SOURCES = ["mgf_kernels.hip", "mgf_kernels_mg.hip", "mgf_kernels_ext.hip", "mgf_kernels_particles.hip",
+           "mgf_capi.cpp", "mgf_batch.cpp"]
 
 
 def hipcc():

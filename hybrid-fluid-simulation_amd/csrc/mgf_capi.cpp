@@ -1325,6 +1325,142 @@ mgf_status mgf_op_apply_forces(mgf_context ctx, const mgf_grid* grid, float* u, 
     return MGF_OK;
 }
 
+// ---- particles ---------------------------------------------------------------
+static bool pg_alg_ok(int32_t alg) { return alg >= MGF_PG_P2GSUM && alg <= MGF_PG_CUBIC; }
+
+mgf_status mgf_op_particles_to_grid(mgf_context ctx, const mgf_grid* grid, int32_t field_kind, int64_t n,
+                                    const float* pos, const float* values, int32_t span, int32_t offset, int32_t alg,
+                                    float* field, float* weight, int32_t normalize) {
+    if (!ctx || !field || !weight || (n > 0 && (!pos || !values))) return fail(MGF_ERR_INVALID, "null argument");
+    TRY(check_grid(grid));
+    if (field_kind < 0 || field_kind > 3 || n < 0 || span < 1 || offset < 0 || offset >= span)
+        return fail(MGF_ERR_INVALID, "bad field kind, count, span or offset");
+    if (!pg_alg_ok(alg) || alg == MGF_PG_G2PNEAREST)
+        return fail(MGF_ERR_INVALID, "particle-to-grid algorithm must be P2GSUM, LINEAR, QUADRATIC or CUBIC");
+    MGF_HIP(hipSetDevice(ctx->device));
+    MGF_HIP(launch_p2g(ctx->stream, make_grid(grid), field_kind, n, pos, values, span, offset, alg, field, weight,
+                       normalize));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_grid_to_particles(mgf_context ctx, const mgf_grid* grid, int32_t field_kind, int64_t n,
+                                    const float* pos, const float* field, int32_t alg, float* values, int32_t span,
+                                    int32_t offset) {
+    if (!ctx || (n > 0 && (!pos || !field || !values))) return fail(MGF_ERR_INVALID, "null argument");
+    TRY(check_grid(grid));
+    if (field_kind < 0 || field_kind > 3 || n < 0 || span < 1 || offset < 0 || offset >= span)
+        return fail(MGF_ERR_INVALID, "bad field kind, count, span or offset");
+    if (!pg_alg_ok(alg) || alg == MGF_PG_P2GSUM)
+        return fail(MGF_ERR_INVALID, "grid-to-particle algorithm must be G2PNEAREST, LINEAR, QUADRATIC or CUBIC");
+    MGF_HIP(hipSetDevice(ctx->device));
+    MGF_HIP(launch_g2p(ctx->stream, make_grid(grid), field_kind, n, pos, field, alg, values, span, offset));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_particle_markers(mgf_context ctx, const mgf_grid* grid, const float* solid, int64_t n,
+                                   const float* pos, float* out) {
+    if (!ctx || !out || (n > 0 && !pos) || n < 0) return fail(MGF_ERR_INVALID, "bad argument");
+    TRY(check_grid(grid));
+    MGF_HIP(hipSetDevice(ctx->device));
+    MGF_HIP(launch_particle_markers(ctx->stream, make_grid(grid), solid, n, pos, out));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_axpy(mgf_context ctx, int64_t n, float a, const float* x, float* y) {
+    if (!ctx || n < 0 || (n > 0 && (!x || !y))) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipSetDevice(ctx->device));
+    MGF_HIP(launch_axpy_n(ctx->stream, size_t(n), a, x, y));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_scale(mgf_context ctx, int64_t n, float a, float* x) {
+    if (!ctx || n < 0 || (n > 0 && !x)) return fail(MGF_ERR_INVALID, "bad argument");
+    MGF_HIP(hipSetDevice(ctx->device));
+    MGF_HIP(launch_scale_n(ctx->stream, size_t(n), a, x));
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    return MGF_OK;
+}
+
+mgf_status mgf_op_advect_particles(mgf_context ctx, const mgf_grid* grid, const float* u, const float* v,
+                                   const float* w, int64_t n, float* pos, float dt, float cfl, int32_t order,
+                                   int32_t fix_to_grid) {
+    if (!ctx || !u || !v || !w || n < 0 || (n > 0 && !pos)) return fail(MGF_ERR_INVALID, "bad argument");
+    TRY(check_grid(grid));
+    if (!std::isfinite(dt) || !std::isfinite(cfl)) return fail(MGF_ERR_INVALID, "dt and cfl must be finite");
+    if (n == 0) return MGF_OK;
+    MGF_HIP(hipSetDevice(ctx->device));
+    const GridDev g = make_grid(grid);
+    const Vel V = vel_of(u, v, w);
+    hipStream_t st = ctx->stream;
+    const size_t m = 3 * size_t(n);
+    // v1, v2, v3, mid, three-quarter positions, then the max word
+    float* buf = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&buf), 5 * m * sizeof(float) + 64) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_NOMEM, "particle advection scratch");
+    }
+    float *v1 = buf, *v2 = buf + m, *v3 = buf + 2 * m, *mid = buf + 3 * m, *tq = buf + 4 * m;
+    uint32_t* vmax_bits = reinterpret_cast<uint32_t*>(buf + 5 * m);
+    auto run = [&]() -> mgf_status {
+        const float minh = std::min(std::min(g.hx, g.hy), g.hz);
+        float t = 0.0f;
+        bool finished = false;
+        for (int64_t sub = 0; !finished; ++sub) {
+            if (sub > 10000000) return fail(MGF_ERR_NONFINITE, "particle advection did not reach dt");
+            MGF_HIP(launch_sample_particles(st, g, V, n, pos, v1));
+            MGF_HIP(launch_absmax(st, m, v1, vmax_bits));
+            uint32_t bits = 0;
+            MGF_HIP(hipMemcpyAsync(&bits, vmax_bits, sizeof bits, hipMemcpyDeviceToHost, st));
+            MGF_HIP(hipStreamSynchronize(st));
+            float vmax;
+            std::memcpy(&vmax, &bits, sizeof vmax);
+            float step = vmax != 0.0f ? cfl * minh / vmax : dt;
+            if (t + step >= dt) {
+                step = dt - t;
+                finished = true;
+            } else if (t + 2 * step >= dt) {
+                step = float(0.5 * double(dt - t));
+            }
+            if (!finished && !(step > 0.0f && std::isfinite(step)))
+                return fail(MGF_ERR_NONFINITE, "particle velocity gives no finite substep");
+            if (order == 1) {
+                MGF_HIP(launch_axpy_n(st, m, step, v1, pos));
+            } else if (order == 2) {
+                const float half = float(0.5 * double(step));
+                MGF_HIP(hipMemcpyAsync(mid, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
+                MGF_HIP(launch_axpy_n(st, m, half, v1, mid));
+                MGF_HIP(launch_sample_particles(st, g, V, n, mid, v2));
+                MGF_HIP(launch_axpy_n(st, m, step, v2, pos));
+            } else if (order == 3) {
+                const float half = float(0.5 * double(step)), three_q = float(0.75 * double(step));
+                const float c1 = float(2.0 / 9.0 * double(step)), c2 = float(3.0 / 9.0 * double(step));
+                const float c3 = float(4.0 / 9.0 * double(step));
+                MGF_HIP(hipMemcpyAsync(mid, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
+                MGF_HIP(hipMemcpyAsync(tq, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
+                MGF_HIP(launch_axpy_n(st, m, half, v1, mid));
+                MGF_HIP(launch_sample_particles(st, g, V, n, mid, v2));
+                MGF_HIP(launch_axpy_n(st, m, three_q, v2, tq));
+                MGF_HIP(launch_sample_particles(st, g, V, n, tq, v3));
+                MGF_HIP(launch_axpy_n(st, m, c1, v1, pos));
+                MGF_HIP(launch_axpy_n(st, m, c2, v2, pos));
+                MGF_HIP(launch_axpy_n(st, m, c3, v3, pos));
+            }
+            t += step;
+        }
+        if (fix_to_grid) MGF_HIP(launch_clamp_particles(st, g, n, pos));
+        MGF_HIP(hipStreamSynchronize(st));
+        return MGF_OK;
+    };
+    const mgf_status r = run();
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    return r;
+}
+
 mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid, float* u, float* v, float* w,
                                  int32_t layers, int32_t* layers_used) {
     if (!ctx) return fail(MGF_ERR_INVALID, "null context");

@@ -286,6 +286,19 @@ hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v
                              const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1);
 // buildFluidDensity / buildFluidSDF (EulerSolverTool.cu:324-448) on the
 // cell-centred grid; scratch holds ncell int32.
+// ---- particles (mgf_kernels_particles.hip); kind 0 cells, 1..3 face grids ----
+size_t pg_field_count(const GridDev& g, int kind);
+hipError_t launch_p2g(hipStream_t s, const GridDev& g, int kind, int64_t np, const float* pos, const float* val,
+                      int span, int off, int alg, float* field, float* weight, int normalize);
+hipError_t launch_g2p(hipStream_t s, const GridDev& g, int kind, int64_t np, const float* pos, const float* field,
+                      int alg, float* val, int span, int off);
+hipError_t launch_particle_markers(hipStream_t s, const GridDev& g, const float* solid, int64_t np, const float* pos,
+                                   float* out);
+hipError_t launch_sample_particles(hipStream_t s, const GridDev& g, Vel V, int64_t np, const float* pos, float* out);
+hipError_t launch_absmax(hipStream_t s, size_t n, const float* v, uint32_t* out);
+hipError_t launch_axpy_n(hipStream_t s, size_t n, float a, const float* x, float* y);
+hipError_t launch_scale_n(hipStream_t s, size_t n, float a, float* x);
+hipError_t launch_clamp_particles(hipStream_t s, const GridDev& g, int64_t np, float* pos);
 hipError_t launch_fluid_density(hipStream_t s, const GridDev& g, const float* phi, const float* solid, float* out);
 hipError_t launch_fluid_sdf(hipStream_t s, const GridDev& g, const float* phi, int32_t passes, int32_t* scratch,
                             float* out);

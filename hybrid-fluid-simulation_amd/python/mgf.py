@@ -20,6 +20,7 @@ TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
 PROBE_COUNT = 8  # MGF_PROBE_COUNT
+PG = dict(p2gsum=0, g2pnearest=1, linear=2, quadratic=3, cubic=4)   # MGF_PG_*
 PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "k_mg_smooth_prolong",
                "k_mg_restrict", "k_ext_layer", "k_advect_fc")
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
@@ -127,6 +128,15 @@ _SIGS = {
     "mgf_op_advect_cc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _VP, C.c_float, C.c_int32, _VP],
     "mgf_op_pressure_matvec": [_VP, C.POINTER(Grid), _F, _VP, _VP, _VP],
     "mgf_op_apply_forces": [_VP, C.POINTER(Grid), _VP, _VP, _VP, _F, _F, C.c_float],
+    "mgf_op_particles_to_grid": [_VP, C.POINTER(Grid), C.c_int32, C.c_int64, _VP, _VP, C.c_int32, C.c_int32,
+                                 C.c_int32, _VP, _VP, C.c_int32],
+    "mgf_op_grid_to_particles": [_VP, C.POINTER(Grid), C.c_int32, C.c_int64, _VP, _VP, C.c_int32, _VP, C.c_int32,
+                                 C.c_int32],
+    "mgf_op_advect_particles": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int64, _VP, C.c_float, C.c_float,
+                                C.c_int32, C.c_int32],
+    "mgf_op_particle_markers": [_VP, C.POINTER(Grid), _VP, C.c_int64, _VP, _VP],
+    "mgf_op_axpy": [_VP, C.c_int64, C.c_float, _VP, _VP],
+    "mgf_op_scale": [_VP, C.c_int64, C.c_float, _VP],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
 _RESTYPE = {"mgf_status_string": C.c_char_p, "mgf_last_error": C.c_char_p}
@@ -428,3 +438,67 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def field_extent(res, kind):
+    """Values of a field on the cells (kind 0) or the x/y/z face grid (1/2/3)."""
+    nx, ny, nz = res
+    return {0: nx * ny * nz, 1: (nx + 1) * ny * nz, 2: nx * (ny + 1) * nz, 3: nx * ny * (nz + 1)}[kind]
+
+
+def particles_to_grid(ctx, res, origin, spacing, kind, pos, values, alg, span=1, off=0, normalize=True):
+    """mgf_op_particles_to_grid on host arrays (copied to the device and back)."""
+    pos = f32(pos).ravel()
+    n = pos.size // 3
+    m = field_extent(res, kind) if kind in (0, 1, 2, 3) else 1   # the library rejects other kinds
+    P, V = ctx.to_device(pos), ctx.to_device(f32(values).ravel())
+    F, W = ctx.alloc(4 * m), ctx.alloc(4 * m)
+    try:
+        check(lib().mgf_op_particles_to_grid(ctx.h, grid(res, origin, spacing), kind, n, P, V, span, off, PG[alg],
+                                             F, W, 1 if normalize else 0), "mgf_op_particles_to_grid")
+        return ctx.to_host(F, m), ctx.to_host(W, m)
+    finally:
+        for p in (P, V, F, W):
+            ctx.free(p)
+
+
+def grid_to_particles(ctx, res, origin, spacing, kind, pos, field, alg, span=1, off=0, out=None):
+    pos = f32(pos).ravel()
+    n = pos.size // 3
+    out = np.zeros(span * n, np.float32) if out is None else f32(out).copy()
+    P, Fd, O = ctx.to_device(pos), ctx.to_device(f32(field).ravel()), ctx.to_device(out)
+    try:
+        check(lib().mgf_op_grid_to_particles(ctx.h, grid(res, origin, spacing), kind, n, P, Fd, PG[alg], O, span,
+                                             off), "mgf_op_grid_to_particles")
+        return ctx.to_host(O, out.size)
+    finally:
+        for p in (P, Fd, O):
+            ctx.free(p)
+
+
+def advect_particles(ctx, res, origin, spacing, u, v, w, pos, dt, cfl, order, fix=False):
+    pos = f32(pos).ravel()
+    D = [ctx.to_device(f32(a).ravel()) for a in (u, v, w)]
+    P = ctx.to_device(pos)
+    try:
+        check(lib().mgf_op_advect_particles(ctx.h, grid(res, origin, spacing), D[0], D[1], D[2], pos.size // 3, P,
+                                            C.c_float(dt), C.c_float(cfl), order, 1 if fix else 0),
+              "mgf_op_advect_particles")
+        return ctx.to_host(P, pos.size)
+    finally:
+        for p in D + [P]:
+            ctx.free(p)
+
+
+def particle_markers(ctx, res, origin, spacing, solid, pos):
+    pos = f32(pos).ravel()
+    m = int(np.prod(res))
+    P, O = ctx.to_device(pos), ctx.alloc(4 * m)
+    S = ctx.to_device(f32(solid).ravel()) if solid is not None else None
+    try:
+        check(lib().mgf_op_particle_markers(ctx.h, grid(res, origin, spacing), S, pos.size // 3, P, O),
+              "mgf_op_particle_markers")
+        return ctx.to_host(O, m)
+    finally:
+        for p in (P, O) + ((S,) if S is not None else ()):
+            ctx.free(p)

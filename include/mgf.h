@@ -385,6 +385,51 @@ MGF_API mgf_status mgf_op_pressure_matvec(mgf_context ctx, const mgf_grid* grid,
  * (FaceCenteredVectorGrid.cpp:162-175). */
 MGF_API mgf_status mgf_op_apply_forces(mgf_context ctx, const mgf_grid* grid,
         float* u, float* v, float* w, const float gravity[3], const float force[3], float dt);
+/* ---- particles (CEulerParticles, MPhysics/EulerParticles.cpp; kernels
+ * MPhysics/EulerParticlesTool.cu) -----------------------------------------
+ * The primitives of the PIC / FLIP / mixed advection solvers.  Positions and
+ * per-particle vectors are interleaved x,y,z (3*i + k) in caller device
+ * memory.  field_kind selects the grid a field lives on: 0 the cells of
+ * `grid`, 1/2/3 its x/y/z face grid (extents +1 on that axis, origin -h/2). */
+#define MGF_PG_P2GSUM     0   /* EPGTransferAlgorithm (MPhysics/EnumType.h:19-26) */
+#define MGF_PG_G2PNEAREST 1
+#define MGF_PG_LINEAR     2
+#define MGF_PG_QUADRATIC  3
+#define MGF_PG_CUBIC      4
+/* tranferParticles2{CCS,CCV,FCV}FieldInvoker (EulerParticlesTool.cu:443-711):
+ * field and weight (same extents) are zeroed, every particle adds
+ * weight*values[span*i + offset] to the clamped corners of its kernel support
+ * and the weight to `weight` (P2GSUM: the value to its own cell), then, with
+ * normalize, field /= weight where weight > 0.  The sums are FP32 atomics, as
+ * in the reference, so their rounding depends on the order they land in. */
+MGF_API mgf_status mgf_op_particles_to_grid(mgf_context ctx, const mgf_grid* grid, int32_t field_kind,
+        int64_t n, const float* pos, const float* values, int32_t span, int32_t offset, int32_t alg,
+        float* field, float* weight, int32_t normalize);
+/* tranfer{CCS,CCV,FCV}Field2ParticlesInvoker (EulerParticlesTool.cu:713-868):
+ * values[span*i + offset] = the kernel-weighted sum of the field around
+ * particle i (G2PNEAREST: the value of its cell). */
+MGF_API mgf_status mgf_op_grid_to_particles(mgf_context ctx, const mgf_grid* grid, int32_t field_kind,
+        int64_t n, const float* pos, const float* field, int32_t alg, float* values, int32_t span,
+        int32_t offset);
+/* CEulerParticles::advectParticlesInVelField (EulerParticles.cpp:231-325):
+ * substeps of cfl*min(h)/max|v| ending at dt, RK1/2/3 through the face-centred
+ * velocity (u,v,w on `grid`); with fix_to_grid the positions are then clamped
+ * to the grid box (fixParticlesPosWithBoundarys, applied, as there, to the
+ * first min(cells, n) particles).  A non-finite or zero substep returns
+ * MGF_ERR_NONFINITE (the reference would not terminate). */
+MGF_API mgf_status mgf_op_advect_particles(mgf_context ctx, const mgf_grid* grid, const float* u,
+        const float* v, const float* w, int64_t n, float* pos, float dt, float cfl, int32_t order,
+        int32_t fix_to_grid);
+/* statisticalFluidDensityInvoker (solid == NULL: particles per cell) and
+ * buildFluidMarkersInvoker (2 where solid < 0, else 1 where a particle lies,
+ * else 0) (EulerParticlesTool.cu:366-441). */
+MGF_API mgf_status mgf_op_particle_markers(mgf_context ctx, const mgf_grid* grid, const float* solid,
+        int64_t n, const float* pos, float* out);
+/* axpyReal / scaleReal (MPhysics/ThrustWapper.cu:281-304): y = a*x + y (one
+ * fused multiply-add), x = a*x, on n device floats. */
+MGF_API mgf_status mgf_op_axpy(mgf_context ctx, int64_t n, float a, const float* x, float* y);
+MGF_API mgf_status mgf_op_scale(mgf_context ctx, int64_t n, float a, float* x);
+
 /* extrapolatingDataInvoker for a face-centred field
  * (MPhysics/EulerSolverTool.cu:450-557, 904-969): in place. */
 MGF_API mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid,
