@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function",
 ]
 SOURCES = ["mgf_kernels.hip", "mgf_kernels_mg.hip", "mgf_kernels_ext.hip", "mgf_kernels_particles.hip",
-           "mgf_capi.cpp", "mgf_batch.cpp"]
+           "mgf_capi.cpp", "mgf_batch.cpp", "mgf_particles.cpp"]
 
 
 def hipcc():

@@ -603,12 +603,15 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     return MGF_OK;
 }
 
-mgf_status pressure(mgf_scene s, mgf_step_stats* acc) {
+// With particles, the markers come from their positions
+// (CPressureSolver::solvePressure, PressureSolver.cpp:109-134).
+mgf_status pressure(mgf_scene s, mgf_step_stats* acc, mgf_particles parts = nullptr) {
     hipStream_t st = s->stream;
     const int c = s->cur;
     float sa[3], sg[3];
     scales(s, sa, sg);
-    MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
+    if (parts) MGF_HIP(launch_particle_markers_u8(st, s->g, s->solid_mask, parts->n, parts->pos, s->markers));
+    else MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
     MGF_HIP(launch_rhs(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), vel_of(s->us, s->vs, s->ws),
                        s->markers, s->b));
     const int64_t max_it = s->desc.max_iterations > 0 ? s->desc.max_iterations : int64_t(s->ncell);
@@ -723,6 +726,47 @@ mgf_status step_once(mgf_scene s, mgf_step_stats* acc) {
     TRY(advect(s, true));
     if (t) { TRY(sync(s)); t1 = now_ms(); }
     TRY(pressure(s, acc));
+    if (t) { TRY(sync(s)); t2 = now_ms(); }
+    int used = 0;
+    TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
+    acc->extrapolation_layers = used;
+    if (t) {
+        TRY(sync(s));
+        t3 = now_ms();
+        acc->ms_advect += t1 - t0;
+        acc->ms_pressure += t2 - t1;
+        acc->ms_extrapolate += t3 - t2;
+        acc->ms_total += t3 - t0;
+    }
+    acc->steps += 1;
+    return MGF_OK;
+}
+
+// _onAdvanceTimeStep with a particle advection solver: _computeAdvection
+// (GridFluidSolver.cpp:399-416: velocity, colour, fluid domain, all through
+// the old velocity, the boundary SDF of the grid's resolution passed so the
+// particles are clamped), forces, the projection with particle markers
+// (:434-525), extrapolation.
+mgf_status step_particles_once(mgf_scene s, mgf_psolver ps, mgf_step_stats* acc) {
+    const bool t = s->timing;
+    double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    if (t) { TRY(sync(s)); t0 = now_ms(); }
+    TRY(refresh_boundaries(s));
+    hipStream_t st = s->stream;
+    const int c = s->cur, n = 1 - s->cur;
+    const float* in[3] = {s->u[c], s->v[c], s->w[c]};
+    float* out[3] = {s->u[n], s->v[n], s->w[n]};
+    TRY(psolver_velocity(ps, st, in, in, s->desc.dt, s->desc.rk_order, 1, out));
+    if (s->color[0][0]) {
+        const float* ic[3] = {s->color[c][0], s->color[c][1], s->color[c][2]};
+        float* oc[3] = {s->color[n][0], s->color[n][1], s->color[n][2]};
+        TRY(psolver_vector(ps, st, ic, oc));
+    }
+    TRY(psolver_scalar(ps, st, s->phi[c], s->phi[n]));
+    s->cur = n;
+    TRY(apply_forces(s));
+    if (t) { TRY(sync(s)); t1 = now_ms(); }
+    TRY(pressure(s, acc, ps->p));
     if (t) { TRY(sync(s)); t2 = now_ms(); }
     int used = 0;
     TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
@@ -1142,6 +1186,46 @@ mgf_status mgf_scene_pressure_solve(mgf_scene s, mgf_step_stats* stats) {
     return MGF_OK;
 }
 
+mgf_status mgf_scene_pressure_solve_particles(mgf_scene s, mgf_particles parts, mgf_step_stats* stats) {
+    if (!s || !parts) return fail(MGF_ERR_INVALID, "null argument");
+    if (parts->ctx->device != s->ctx->device) return fail(MGF_ERR_INVALID, "particles on another device");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    mgf_step_stats acc;
+    stats_reset(&acc);
+    TRY(pressure(s, &acc, parts));
+    TRY(sync(s));
+    if (stats) *stats = acc;
+    if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "pressure solve stopped before its tolerance");
+    return MGF_OK;
+}
+
+static bool same_grid(const mgf_scene_desc& d, const mgf_grid& g) {
+    return d.nx == g.nx && d.ny == g.ny && d.nz == g.nz && d.origin[0] == g.origin[0] &&
+           d.origin[1] == g.origin[1] && d.origin[2] == g.origin[2] && d.spacing[0] == g.spacing[0] &&
+           d.spacing[1] == g.spacing[1] && d.spacing[2] == g.spacing[2];
+}
+
+mgf_status mgf_step_particles(mgf_scene s, mgf_psolver ps, int32_t nsteps, mgf_step_stats* stats) {
+    if (!s || !ps) return fail(MGF_ERR_INVALID, "null argument");
+    if (nsteps < 0) return fail(MGF_ERR_INVALID, "nsteps < 0");
+    if (!same_grid(s->desc, ps->grid)) return fail(MGF_ERR_INVALID, "the solver's grid is not the scene's");
+    if (ps->ctx->device != s->ctx->device) return fail(MGF_ERR_INVALID, "solver on another device");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    mgf_step_stats acc;
+    stats_reset(&acc);
+    for (int i = 0; i < nsteps; ++i) {
+        const mgf_status r = step_particles_once(s, ps, &acc);
+        if (r != MGF_OK) {
+            (void)hipStreamSynchronize(s->stream);
+            return r;
+        }
+    }
+    TRY(sync(s));
+    if (stats) *stats = acc;
+    if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "a pressure solve stopped before its tolerance");
+    return MGF_OK;
+}
+
 mgf_status mgf_scene_extrapolate_velocity(mgf_scene s, int32_t* layers_used) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
     int used = 0;
@@ -1393,8 +1477,6 @@ mgf_status mgf_op_advect_particles(mgf_context ctx, const mgf_grid* grid, const 
     if (!std::isfinite(dt) || !std::isfinite(cfl)) return fail(MGF_ERR_INVALID, "dt and cfl must be finite");
     if (n == 0) return MGF_OK;
     MGF_HIP(hipSetDevice(ctx->device));
-    const GridDev g = make_grid(grid);
-    const Vel V = vel_of(u, v, w);
     hipStream_t st = ctx->stream;
     const size_t m = 3 * size_t(n);
     // v1, v2, v3, mid, three-quarter positions, then the max word
@@ -1403,59 +1485,8 @@ mgf_status mgf_op_advect_particles(mgf_context ctx, const mgf_grid* grid, const 
         (void)hipGetLastError();
         return fail(MGF_ERR_NOMEM, "particle advection scratch");
     }
-    float *v1 = buf, *v2 = buf + m, *v3 = buf + 2 * m, *mid = buf + 3 * m, *tq = buf + 4 * m;
-    uint32_t* vmax_bits = reinterpret_cast<uint32_t*>(buf + 5 * m);
-    auto run = [&]() -> mgf_status {
-        const float minh = std::min(std::min(g.hx, g.hy), g.hz);
-        float t = 0.0f;
-        bool finished = false;
-        for (int64_t sub = 0; !finished; ++sub) {
-            if (sub > 10000000) return fail(MGF_ERR_NONFINITE, "particle advection did not reach dt");
-            MGF_HIP(launch_sample_particles(st, g, V, n, pos, v1));
-            MGF_HIP(launch_absmax(st, m, v1, vmax_bits));
-            uint32_t bits = 0;
-            MGF_HIP(hipMemcpyAsync(&bits, vmax_bits, sizeof bits, hipMemcpyDeviceToHost, st));
-            MGF_HIP(hipStreamSynchronize(st));
-            float vmax;
-            std::memcpy(&vmax, &bits, sizeof vmax);
-            float step = vmax != 0.0f ? cfl * minh / vmax : dt;
-            if (t + step >= dt) {
-                step = dt - t;
-                finished = true;
-            } else if (t + 2 * step >= dt) {
-                step = float(0.5 * double(dt - t));
-            }
-            if (!finished && !(step > 0.0f && std::isfinite(step)))
-                return fail(MGF_ERR_NONFINITE, "particle velocity gives no finite substep");
-            if (order == 1) {
-                MGF_HIP(launch_axpy_n(st, m, step, v1, pos));
-            } else if (order == 2) {
-                const float half = float(0.5 * double(step));
-                MGF_HIP(hipMemcpyAsync(mid, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
-                MGF_HIP(launch_axpy_n(st, m, half, v1, mid));
-                MGF_HIP(launch_sample_particles(st, g, V, n, mid, v2));
-                MGF_HIP(launch_axpy_n(st, m, step, v2, pos));
-            } else if (order == 3) {
-                const float half = float(0.5 * double(step)), three_q = float(0.75 * double(step));
-                const float c1 = float(2.0 / 9.0 * double(step)), c2 = float(3.0 / 9.0 * double(step));
-                const float c3 = float(4.0 / 9.0 * double(step));
-                MGF_HIP(hipMemcpyAsync(mid, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
-                MGF_HIP(hipMemcpyAsync(tq, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
-                MGF_HIP(launch_axpy_n(st, m, half, v1, mid));
-                MGF_HIP(launch_sample_particles(st, g, V, n, mid, v2));
-                MGF_HIP(launch_axpy_n(st, m, three_q, v2, tq));
-                MGF_HIP(launch_sample_particles(st, g, V, n, tq, v3));
-                MGF_HIP(launch_axpy_n(st, m, c1, v1, pos));
-                MGF_HIP(launch_axpy_n(st, m, c2, v2, pos));
-                MGF_HIP(launch_axpy_n(st, m, c3, v3, pos));
-            }
-            t += step;
-        }
-        if (fix_to_grid) MGF_HIP(launch_clamp_particles(st, g, n, pos));
-        MGF_HIP(hipStreamSynchronize(st));
-        return MGF_OK;
-    };
-    const mgf_status r = run();
+    const mgf_status r = particles_advect_run(st, make_grid(grid), vel_of(u, v, w), n, pos, dt, cfl, order,
+                                              fix_to_grid, buf, reinterpret_cast<uint32_t*>(buf + 5 * m));
     (void)hipStreamSynchronize(st);
     (void)hipFree(buf);
     return r;

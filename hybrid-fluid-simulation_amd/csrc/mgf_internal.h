@@ -134,6 +134,36 @@ struct mgf_scene_s {
     std::vector<void*> allocations;
 };
 
+// A particle set: interleaved xyz per particle; scratch = the advection
+// temporaries (v1, v2, v3, mid, three-quarter positions: 3n floats each).
+struct mgf_particles_s {
+    mgf_context ctx = nullptr;
+    int64_t n = 0;
+    float* pos = nullptr;
+    float* vel = nullptr;
+    float* color = nullptr;
+    float* scalar = nullptr;   // n
+    float* vector = nullptr;   // 3n
+    float* scratch = nullptr;  // 15n
+    uint32_t* vmax = nullptr;  // max |v| bits
+    std::vector<void*> allocations;
+};
+
+// A particle advection solver on a borrowed particle set.
+struct mgf_psolver_s {
+    mgf_context ctx = nullptr;
+    mgf_particles p = nullptr;
+    mgf_grid grid{};
+    int kind = 0, alg = 0;
+    float cfl = 1.0f, mixing = 0.1f;
+    float* delta[3] = {nullptr, nullptr, nullptr};   // previous output velocity / grid change (FLIP, MIX)
+    float* weight[3] = {nullptr, nullptr, nullptr};  // weight fields (face extents; cells fit in them)
+    float* dvel = nullptr;                           // 3n: grid change at the particles
+    float* picvel = nullptr;                         // 3n: PIC velocity at the particles (MIX)
+    bool scalar_seeded = false, vector_seeded = false, vel_seeded = false;
+    std::vector<void*> allocations;
+};
+
 namespace mgf {
 
 constexpr int kBlock = 256;
@@ -286,6 +316,9 @@ hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v
                              const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1);
 // buildFluidDensity / buildFluidSDF (EulerSolverTool.cu:324-448) on the
 // cell-centred grid; scratch holds ncell int32.
+hipError_t launch_fluid_density(hipStream_t s, const GridDev& g, const float* phi, const float* solid, float* out);
+hipError_t launch_fluid_sdf(hipStream_t s, const GridDev& g, const float* phi, int32_t passes, int32_t* scratch,
+                            float* out);
 // ---- particles (mgf_kernels_particles.hip); kind 0 cells, 1..3 face grids ----
 size_t pg_field_count(const GridDev& g, int kind);
 hipError_t launch_p2g(hipStream_t s, const GridDev& g, int kind, int64_t np, const float* pos, const float* val,
@@ -299,9 +332,23 @@ hipError_t launch_absmax(hipStream_t s, size_t n, const float* v, uint32_t* out)
 hipError_t launch_axpy_n(hipStream_t s, size_t n, float a, const float* x, float* y);
 hipError_t launch_scale_n(hipStream_t s, size_t n, float a, float* x);
 hipError_t launch_clamp_particles(hipStream_t s, const GridDev& g, int64_t np, float* pos);
-hipError_t launch_fluid_density(hipStream_t s, const GridDev& g, const float* phi, const float* solid, float* out);
-hipError_t launch_fluid_sdf(hipStream_t s, const GridDev& g, const float* phi, int32_t passes, int32_t* scratch,
-                            float* out);
+// Particle generation: tile sums of the fluid cells scanned in place
+// (gen_tiles(ncell) + 1 words, the last the total), then the emit pass.
+uint32_t gen_tiles(size_t ncell);
+hipError_t launch_gen_count(hipStream_t s, size_t ncell, const float* fluid, const float* solid, uint32_t* tiles);
+hipError_t launch_gen_emit(hipStream_t s, const GridDev& g, const float* fluid, const float* solid,
+                           const uint32_t* tiles, int per_cell, uint64_t seed, float* pos);
+hipError_t launch_flip_delta(hipStream_t s, size_t n, const float* in, float* delta);
+hipError_t launch_particle_markers_u8(hipStream_t s, const GridDev& g, const uint8_t* solid_mask, int64_t np,
+                                      const float* pos, uint8_t* markers);
+// The particle advection loop (mgf_particles.cpp): scratch holds 15 n floats,
+// vmax one word.
+mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t n, float* pos, float dt, float cfl,
+                                int order, int fix, float* scratch, uint32_t* vmax);
+mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in[3], const float* const vel[3],
+                            float dt, int order, int fix, float* const out[3]);
+mgf_status psolver_scalar(mgf_psolver s, hipStream_t st, const float* in, float* out);
+mgf_status psolver_vector(mgf_psolver s, hipStream_t st, const float* const in[3], float* const out[3]);
 // Boundaries.
 hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0,
                                  const float t[3], float* out);

@@ -203,6 +203,138 @@ __global__ void __launch_bounds__(kBlock) k_clamp_particles(int64_t m, float* po
     pos[3 * p + 2] = fmaxf(lz, fminf(pos[3 * p + 2], hz));
 }
 
+// ---- particle generation (generateParticlesInFluid, EulerParticles.cpp:
+// 163-201; findFluidGrid, EulerParticlesTool.cu:5-25) -----------------------
+// A tile of kGenTile consecutive cells per work-group: the count pass sums the
+// fluid cells of each tile, one work-group scans the tile sums, and the emit
+// pass ranks the fluid cells inside its tile (wave ballots, x-fastest order)
+// and writes their particles.
+constexpr int kGenRows = 8;
+constexpr int kGenTile = kBlock * kGenRows;
+
+__device__ __forceinline__ bool fluid_cell(const float* fluid, const float* solid, size_t c) {
+    return fluid[c] < 0.0f && !(solid && solid[c] < 0.0f);
+}
+
+// Fluid cells of this work-group's tile among rows [0, kGenRows): for each row
+// the rank of this work-item's cell among the row's fluid cells and the row
+// total.  wsum holds one count per wave.
+__device__ __forceinline__ void row_rank(bool f, uint32_t* wsum, uint32_t& before, uint32_t& total) {
+    const uint64_t b = __ballot(f);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wsum[wave] = uint32_t(__popcll(b));
+    __syncthreads();
+    uint32_t pre = uint32_t(__popcll(b & ((uint64_t(1) << lane) - 1)));
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+        if (q < wave) pre += wsum[q];
+        total += wsum[q];
+    }
+    before = pre;
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBlock) k_gen_count(size_t n, const float* __restrict__ fluid,
+                                                      const float* __restrict__ solid, uint32_t* __restrict__ tiles) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const size_t base = size_t(blockIdx.x) * kGenTile;
+    uint32_t sum = 0;
+    for (int j = 0; j < kGenRows; ++j) {
+        const size_t c = base + size_t(j) * kBlock + threadIdx.x;
+        uint32_t before, total;
+        row_rank(c < n && fluid_cell(fluid, solid, c), wsum, before, total);
+        sum += total;
+    }
+    if (threadIdx.x == 0) tiles[blockIdx.x] = sum;
+}
+
+// In place: tiles[0..nt) -> exclusive prefix sums, tiles[nt] = the total.
+__global__ void __launch_bounds__(1024) k_gen_scan(uint32_t nt, uint32_t* tiles) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x, per = (nt + 1023) / 1024;
+    const uint32_t b = min(nt, t * per), e = min(nt, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += tiles[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {   // inclusive Hillis-Steele scan
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = tiles[i];
+        tiles[i] = run;
+        run += v;
+    }
+    if (t == 1023) tiles[nt] = part[1023];
+}
+
+// SplitMix64's output function (Steele, Lea, Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ float particle_jitter(uint64_t seed, uint64_t index, int axis) {
+    return float(splitmix64(seed ^ splitmix64(3 * index + uint64_t(axis))) >> 40) * (1.0f / 16777216.0f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_gen_emit(GridDev g, size_t n, const float* __restrict__ fluid,
+                                                     const float* __restrict__ solid,
+                                                     const uint32_t* __restrict__ tiles, int per_cell, uint64_t seed,
+                                                     float* __restrict__ pos) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const size_t base = size_t(blockIdx.x) * kGenTile;
+    uint32_t rank = tiles[blockIdx.x];
+    for (int j = 0; j < kGenRows; ++j) {
+        const size_t c = base + size_t(j) * kBlock + threadIdx.x;
+        const bool f = c < n && fluid_cell(fluid, solid, c);
+        uint32_t before, total;
+        row_rank(f, wsum, before, total);
+        if (f) {
+            const uint32_t x = uint32_t(c % uint32_t(g.nx));
+            const uint32_t y = uint32_t((c / uint32_t(g.nx)) % uint32_t(g.ny));
+            const uint32_t z = uint32_t(c / (size_t(g.nx) * uint32_t(g.ny)));
+            const uint64_t first = uint64_t(rank + before) * uint64_t(per_cell);
+            for (int i = 0; i < per_cell; ++i) {
+                const uint64_t idx = first + uint64_t(i);
+                pos[3 * idx] = (float(x) + particle_jitter(seed, idx, 0)) * g.hx + g.ox;
+                pos[3 * idx + 1] = (float(y) + particle_jitter(seed, idx, 1)) * g.hy + g.oy;
+                pos[3 * idx + 2] = (float(z) + particle_jitter(seed, idx, 2)) * g.hz + g.oz;
+            }
+        }
+        rank += total;
+    }
+}
+
+// m_DeltaGridVelField.plusAlphaX(in, -1); m_DeltaGridVelField *= -1
+// (FluidImplicitParticle.cpp:105-106): one axpy with alpha -1, then a sign flip.
+__global__ void __launch_bounds__(kBlock) k_flip_delta(size_t n, const float* __restrict__ in, float* delta) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) delta[i] = -__builtin_fmaf(-1.0f, in[i], delta[i]);
+}
+
+// Pressure markers from particles (buildFluidMarkers, EulerParticlesTool.cu:
+// 48-77): 2 in solid cells, then 1 in every other cell holding a particle.
+__global__ void __launch_bounds__(kBlock) k_markers_solid(size_t n, const uint8_t* __restrict__ solid_mask,
+                                                          uint8_t* __restrict__ markers) {
+    const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (i < n) markers[i] = solid_mask[i] ? uint8_t(2) : uint8_t(0);
+}
+__global__ void __launch_bounds__(kBlock) k_markers_particles(PGrid g, int64_t np, const float* __restrict__ pos,
+                                                              uint8_t* markers) {
+    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const uint32_t c = pos_cell(g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+    if (markers[c] != 2) markers[c] = 1;
+}
+
 // ===========================================================================
 static PGrid pgrid_of(const GridDev& g, int kind) {
     PGrid p{g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz};
@@ -271,6 +403,41 @@ hipError_t launch_clamp_particles(hipStream_t s, const GridDev& g, int64_t np, f
     if (m <= 0) return hipSuccess;
     const float hx = g.ox + float(g.nx) * g.hx, hy = g.oy + float(g.ny) * g.hy, hz = g.oz + float(g.nz) * g.hz;
     return LAUNCH(k_clamp_particles, grid_for(size_t(m)), m, pos, g.ox, g.oy, g.oz, hx, hy, hz);
+}
+
+}  // namespace mgf
+
+namespace mgf {
+
+uint32_t gen_tiles(size_t ncell) { return uint32_t((ncell + kGenTile - 1) / kGenTile); }
+
+hipError_t launch_gen_count(hipStream_t s, size_t ncell, const float* fluid, const float* solid, uint32_t* tiles) {
+    const uint32_t nt = gen_tiles(ncell);
+    hipLaunchKernelGGL(k_gen_count, dim3(nt), dim3(kBlock), 0, s, ncell, fluid, solid, tiles);
+    hipLaunchKernelGGL(k_gen_scan, dim3(1), dim3(1024), 0, s, nt, tiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_emit(hipStream_t s, const GridDev& g, const float* fluid, const float* solid,
+                           const uint32_t* tiles, int per_cell, uint64_t seed, float* pos) {
+    const size_t ncell = size_t(g.nx) * g.ny * g.nz;
+    hipLaunchKernelGGL(k_gen_emit, dim3(gen_tiles(ncell)), dim3(kBlock), 0, s, g, ncell, fluid, solid, tiles,
+                       per_cell, seed, pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_flip_delta(hipStream_t s, size_t n, const float* in, float* delta) {
+    if (n == 0) return hipSuccess;
+    return LAUNCH(k_flip_delta, grid_for(n), n, in, delta);
+}
+
+hipError_t launch_particle_markers_u8(hipStream_t s, const GridDev& g, const uint8_t* solid_mask, int64_t np,
+                                      const float* pos, uint8_t* markers) {
+    const size_t n = size_t(g.nx) * g.ny * g.nz;
+    hipError_t e = LAUNCH(k_markers_solid, grid_for(n), n, solid_mask, markers);
+    if (e == hipSuccess && np > 0)
+        e = LAUNCH(k_markers_particles, grid_for(size_t(np)), pgrid_of(g, 0), np, pos, markers);
+    return e;
 }
 
 }  // namespace mgf

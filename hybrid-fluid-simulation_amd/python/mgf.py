@@ -136,6 +136,23 @@ _SIGS = {
                                 C.c_int32, C.c_int32],
     "mgf_op_particle_markers": [_VP, C.POINTER(Grid), _VP, C.c_int64, _VP, _VP],
     "mgf_op_axpy": [_VP, C.c_int64, C.c_float, _VP, _VP],
+    "mgf_particles_create": [_VP, C.c_int64, _F, _F, _F, C.POINTER(C.c_void_p)],
+    "mgf_particles_generate": [_VP, C.POINTER(Grid), _VP, _VP, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)],
+    "mgf_particles_destroy": [_VP],
+    "mgf_particles_count": [_VP, C.POINTER(C.c_int64)],
+    "mgf_particles_data": [_VP, C.c_int32, C.POINTER(C.c_void_p)],
+    "mgf_particles_upload": [_VP, C.c_int32, _F, C.c_int64],
+    "mgf_particles_download": [_VP, C.c_int32, _F, C.c_int64],
+    "mgf_particles_advect": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_float, C.c_float, C.c_int32, C.c_int32],
+    "mgf_psolver_create": [_VP, C.POINTER(Grid), C.c_int32, _VP, C.c_float, C.c_int32, C.c_float,
+                           C.POINTER(C.c_void_p)],
+    "mgf_psolver_destroy": [_VP],
+    "mgf_psolver_advect_velocity": [_VP, _VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_int32, C.c_int32, _VP, _VP,
+                                    _VP],
+    "mgf_psolver_advect_scalar": [_VP, _VP, _VP],
+    "mgf_psolver_advect_vector": [_VP, _VP, _VP, _VP, _VP, _VP, _VP],
+    "mgf_scene_pressure_solve_particles": [_VP, _VP, C.POINTER(StepStats)],
+    "mgf_step_particles": [_VP, _VP, C.c_int32, C.POINTER(StepStats)],
     "mgf_op_scale": [_VP, C.c_int64, C.c_float, _VP],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
@@ -340,6 +357,20 @@ class Scene:
             check(rc, "mgf_step")
         return st.as_dict()
 
+    def pressure_solve_particles(self, particles, allow_unconverged=False):
+        st = StepStats()
+        rc = lib().mgf_scene_pressure_solve_particles(self.h, particles.h, C.byref(st))
+        if not (allow_unconverged and rc == ERR_NOT_CONVERGED):
+            check(rc, "mgf_scene_pressure_solve_particles")
+        return st.as_dict()
+
+    def step_particles(self, solver, n=1, allow_unconverged=False):
+        st = StepStats()
+        rc = lib().mgf_step_particles(self.h, solver.h, n, C.byref(st))
+        if not (allow_unconverged and rc == ERR_NOT_CONVERGED):
+            check(rc, "mgf_step_particles")
+        return st.as_dict()
+
     def sync(self):
         check(lib().mgf_scene_sync(self.h), "mgf_scene_sync")
 
@@ -502,3 +533,132 @@ def particle_markers(ctx, res, origin, spacing, solid, pos):
     finally:
         for p in (P, O) + ((S,) if S is not None else ()):
             ctx.free(p)
+
+
+PARTICLE = dict(pos=0, vel=1, color=2, scalar=3, vector=4)   # MGF_PARTICLE_*
+PSOLVER = dict(pic=0, flip=1, mix=2)                        # MGF_PSOLVER_*
+
+
+class Particles:
+    """CEulerParticles: a particle set resident in HBM."""
+
+    def __init__(self, ctx, pos=None, vel=None, color=None, n=None, handle=None):
+        self.ctx = ctx
+        if handle is not None:
+            self.h = handle
+            return
+        arrs = [None if a is None else f32(a).ravel() for a in (pos, vel, color)]
+        if n is None:
+            n = next(a.size // 3 for a in arrs if a is not None)
+        h = C.c_void_p()
+        check(lib().mgf_particles_create(ctx.h, n, *[fp(a) if a is not None else None for a in arrs], C.byref(h)),
+              "mgf_particles_create")
+        self.h = h
+
+    @classmethod
+    def generate(cls, ctx, res, origin, spacing, fluid, solid, per_cell, seed):
+        """generateParticlesInFluid from host fields (copied to the device)."""
+        F = ctx.to_device(f32(fluid).ravel())
+        S = ctx.to_device(f32(solid).ravel()) if solid is not None else None
+        try:
+            h = C.c_void_p()
+            check(lib().mgf_particles_generate(ctx.h, grid(res, origin, spacing), F, S, per_cell, seed, C.byref(h)),
+                  "mgf_particles_generate")
+        finally:
+            ctx.free(F)
+            if S is not None:
+                ctx.free(S)
+        return cls(ctx, handle=h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgf_particles_destroy(self.h)
+            self.h = None
+
+    def __len__(self):
+        n = C.c_int64()
+        check(lib().mgf_particles_count(self.h, C.byref(n)), "mgf_particles_count")
+        return n.value
+
+    def _count(self, which):
+        return len(self) * (1 if which == "scalar" else 3)
+
+    def get(self, which):
+        out = np.empty(self._count(which), np.float32)
+        check(lib().mgf_particles_download(self.h, PARTICLE[which], fp(out), out.size), "mgf_particles_download")
+        return out
+
+    def set(self, which, data):
+        data = f32(data).ravel()
+        check(lib().mgf_particles_upload(self.h, PARTICLE[which], fp(data), data.size), "mgf_particles_upload")
+
+    def data(self, which):
+        p = C.c_void_p()
+        check(lib().mgf_particles_data(self.h, PARTICLE[which], C.byref(p)), "mgf_particles_data")
+        return p
+
+    def advect(self, res, origin, spacing, u, v, w, dt, cfl, order, fix=False):
+        D = [self.ctx.to_device(f32(a).ravel()) for a in (u, v, w)]
+        try:
+            check(lib().mgf_particles_advect(self.h, grid(res, origin, spacing), D[0], D[1], D[2], C.c_float(dt),
+                                             C.c_float(cfl), order, 1 if fix else 0), "mgf_particles_advect")
+        finally:
+            for d in D:
+                self.ctx.free(d)
+
+
+class ParticleSolver:
+    """CParticleInCell / CFluidImplicitParticle / CMixPICAndFLIP on a
+    Particles set (borrowed; keep it alive)."""
+
+    def __init__(self, ctx, res, origin, spacing, kind, particles, cfl=1.0, alg="linear", mixing=0.1):
+        self.ctx = ctx
+        self.res = tuple(int(r) for r in res)
+        self.particles = particles
+        h = C.c_void_p()
+        check(lib().mgf_psolver_create(ctx.h, grid(res, origin, spacing), PSOLVER[kind], particles.h,
+                                       C.c_float(cfl), PG[alg], C.c_float(mixing), C.byref(h)), "mgf_psolver_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgf_psolver_destroy(self.h)
+            self.h = None
+
+    def advect_velocity(self, inp, vel, dt, order, fix):
+        """Host arrays in and out (copied through device buffers)."""
+        ctx = self.ctx
+        I = [ctx.to_device(f32(a).ravel()) for a in inp]
+        V = [ctx.to_device(f32(a).ravel()) for a in vel]
+        n = [field_extent(self.res, k) for k in (1, 2, 3)]
+        O = [ctx.alloc(4 * m) for m in n]
+        try:
+            check(lib().mgf_psolver_advect_velocity(self.h, *I, *V, C.c_float(dt), order, 1 if fix else 0, *O),
+                  "mgf_psolver_advect_velocity")
+            return [ctx.to_host(o, m) for o, m in zip(O, n)]
+        finally:
+            for p in I + V + O:
+                ctx.free(p)
+
+    def advect_scalar(self, inp):
+        ctx = self.ctx
+        m = field_extent(self.res, 0)
+        I, O = ctx.to_device(f32(inp).ravel()), ctx.alloc(4 * m)
+        try:
+            check(lib().mgf_psolver_advect_scalar(self.h, I, O), "mgf_psolver_advect_scalar")
+            return ctx.to_host(O, m)
+        finally:
+            ctx.free(I)
+            ctx.free(O)
+
+    def advect_vector(self, inp):
+        ctx = self.ctx
+        m = field_extent(self.res, 0)
+        I = [ctx.to_device(f32(a).ravel()) for a in inp]
+        O = [ctx.alloc(4 * m) for _ in range(3)]
+        try:
+            check(lib().mgf_psolver_advect_vector(self.h, *I, *O), "mgf_psolver_advect_vector")
+            return [ctx.to_host(o, m) for o in O]
+        finally:
+            for p in I + O:
+                ctx.free(p)

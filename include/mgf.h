@@ -430,6 +430,92 @@ MGF_API mgf_status mgf_op_particle_markers(mgf_context ctx, const mgf_grid* grid
 MGF_API mgf_status mgf_op_axpy(mgf_context ctx, int64_t n, float a, const float* x, float* y);
 MGF_API mgf_status mgf_op_scale(mgf_context ctx, int64_t n, float a, float* x);
 
+/* ---- particle systems and particle advection solvers -------------------- */
+/* A particle set (CEulerParticles, MPhysics/EulerParticles.h:5-64): positions,
+ * velocities and colours (3 floats each per particle, interleaved), the
+ * scalar value (1) and vector value (3) the particles carry, and the
+ * advection scratch, all in device memory the handle owns.  Not thread-safe;
+ * runs on its context's stream. */
+typedef struct mgf_particles_s* mgf_particles;
+#define MGF_PARTICLE_POS    0
+#define MGF_PARTICLE_VEL    1
+#define MGF_PARTICLE_COLOR  2
+#define MGF_PARTICLE_SCALAR 3   /* n floats  */
+#define MGF_PARTICLE_VECTOR 4   /* 3n floats */
+/* CEulerParticles::resize (EulerParticles.cpp:20-67): n particles from host
+ * arrays of 3n floats; a NULL array gives zeros. */
+MGF_API mgf_status mgf_particles_create(mgf_context ctx, int64_t n, const float* pos, const float* vel,
+        const float* color, mgf_particles* out);
+/* CEulerParticles::generateParticlesInFluid (EulerParticles.cpp:163-201,
+ * findFluidGrid EulerParticlesTool.cu:5-25): per_cell particles in every cell
+ * with fluid_sdf < 0 and not solid_sdf < 0 (solid_sdf may be NULL), cells in
+ * x-fastest order; particle i of a cell at (x + r) * h + o per axis.  The
+ * reference draws r with rand() seeded by the clock; here r is a
+ * counter-based hash of (seed, particle index, axis) in [0, 1), so a seed
+ * reproduces the set exactly.  The fields are device arrays on `grid`. */
+MGF_API mgf_status mgf_particles_generate(mgf_context ctx, const mgf_grid* grid, const float* fluid_sdf,
+        const float* solid_sdf, int32_t per_cell, uint64_t seed, mgf_particles* out);
+MGF_API mgf_status mgf_particles_destroy(mgf_particles p);
+MGF_API mgf_status mgf_particles_count(mgf_particles p, int64_t* n);
+/* getParticles*GPUPtr (EulerParticles.cpp:99-137): a borrowed device pointer,
+ * valid until the handle is destroyed. */
+MGF_API mgf_status mgf_particles_data(mgf_particles p, int32_t which, float** dev);
+/* setParticlesPos / Vel / Color and the getters, through host memory. */
+MGF_API mgf_status mgf_particles_upload(mgf_particles p, int32_t which, const float* host, int64_t count);
+MGF_API mgf_status mgf_particles_download(mgf_particles p, int32_t which, float* host, int64_t count);
+/* CEulerParticles::advectParticlesInVelField (EulerParticles.cpp:231-325),
+ * as mgf_op_advect_particles but with the handle's own scratch. */
+MGF_API mgf_status mgf_particles_advect(mgf_particles p, const mgf_grid* grid, const float* u, const float* v,
+        const float* w, float dt, float cfl, int32_t order, int32_t fix_to_grid);
+
+/* The particle CAdvectionSolver plug-ins: CParticleInCell
+ * (MPhysics/ParticleInCell.cpp:58-106), CFluidImplicitParticle
+ * (FluidImplicitParticle.cpp:60-125) and CMixPICAndFLIP
+ * (MixPICAndFLIP.cpp:64-134), on a particle set they borrow (it must outlive
+ * the solver).  The solver holds the grid-shaped state of the reference class:
+ * the previous output velocity (FLIP, MIX), the weight fields, and the flags
+ * that make the first scalar / vector call seed the particles' values. */
+typedef struct mgf_psolver_s* mgf_psolver;
+#define MGF_PSOLVER_PIC  0
+#define MGF_PSOLVER_FLIP 1
+#define MGF_PSOLVER_MIX  2
+/* cfl: the substep CFL number (1.0 in the reference constructors); alg: the
+ * particle<->grid kernel, MGF_PG_LINEAR .. MGF_PG_CUBIC; mixing: the PIC share
+ * of MIX (0.1 in the reference), ignored by the others. */
+MGF_API mgf_status mgf_psolver_create(mgf_context ctx, const mgf_grid* grid, int32_t kind, mgf_particles particles,
+        float cfl, int32_t alg, float mixing, mgf_psolver* out);
+MGF_API mgf_status mgf_psolver_destroy(mgf_psolver s);
+/* advect(CFaceCenteredVectorField): the input velocity (in_*) is
+ * transferred to the particles (PIC: replaced; FLIP: the grid change since
+ * the last call is added; MIX: the blend (1-mixing) FLIP + mixing PIC), the
+ * particles move through vel_* (mgf_particles_advect), and their velocity is
+ * splatted into out_* (normalised by the weights).  fix_to_grid stands for a
+ * boundary SDF of the grid's resolution being passed (the particles are then
+ * clamped to the grid box).  All arrays are device face arrays on `grid`;
+ * out_* may not alias in_* or vel_*. */
+MGF_API mgf_status mgf_psolver_advect_velocity(mgf_psolver s, const float* in_u, const float* in_v,
+        const float* in_w, const float* vel_u, const float* vel_v, const float* vel_w, float dt, int32_t order,
+        int32_t fix_to_grid, float* out_u, float* out_v, float* out_w);
+/* advect(CCellCenteredScalarField) / advect(CCellCenteredVectorField): the
+ * first call samples `in` onto the particles; every call splats the
+ * particles' values into `out` (the particles are not moved here). */
+MGF_API mgf_status mgf_psolver_advect_scalar(mgf_psolver s, const float* in, float* out);
+MGF_API mgf_status mgf_psolver_advect_vector(mgf_psolver s, const float* in_x, const float* in_y,
+        const float* in_z, float* out_x, float* out_y, float* out_z);
+
+/* CPressureSolver::solvePressure with particle positions
+ * (MPhysics/PressureSolver.cpp:109-134): the markers come from the particles
+ * (2 solid, 1 a cell holding a particle, 0 else; EulerParticlesTool.cu:48-77)
+ * instead of the fluid domain; the rest of the projection is the same. */
+MGF_API mgf_status mgf_scene_pressure_solve_particles(mgf_scene scene, mgf_particles particles,
+        mgf_step_stats* stats);
+/* CGridFluidSolver::update() with a particle advection solver
+ * (GridFluidSolver.cpp:336-370, 399-416, 434-525): boundaries, then the
+ * velocity, the colour (when present) and the fluid domain through the
+ * solver, body forces, the projection with particle markers, extrapolation;
+ * nsteps times.  The solver's grid must be the scene's. */
+MGF_API mgf_status mgf_step_particles(mgf_scene scene, mgf_psolver solver, int32_t nsteps, mgf_step_stats* stats);
+
 /* extrapolatingDataInvoker for a face-centred field
  * (MPhysics/EulerSolverTool.cu:450-557, 904-969): in place. */
 MGF_API mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid,

@@ -664,12 +664,14 @@ template <class R>
 PressureOut<R> pressure_solve(const Grid<R>& g, R dt, R density, R* u, R* v, R* w,
                               const R* us, const R* vs, const R* ws, const R* solid,
                               const R* fluid, int tol_mode, double tol,
-                              long long max_it, bool jacobi) {
+                              long long max_it, bool jacobi, const R* markers_in = nullptr) {
     const size_t n = g.cells();
     PressureOut<R> o;
     o.markers.resize(n); o.A.resize(4 * n); o.b.resize(n); o.x.assign(n, R(0));
     o.div.resize(n);
-    build_fluid_markers(n, solid, fluid, o.markers.data());
+    // markers_in: markers built elsewhere (from particles, PressureSolver.cpp:109-134)
+    if (markers_in) o.markers.assign(markers_in, markers_in + n);
+    else build_fluid_markers(n, solid, fluid, o.markers.data());
     R sa[3] = {dt / (density * g.hx * g.hx), dt / (density * g.hy * g.hy),
                dt / (density * g.hz * g.hz)};
     build_matrix(g, o.markers.data(), sa, o.A.data());
@@ -983,6 +985,123 @@ void advect_particles(const Grid<R>& g, const R* u, const R* v, const R* w, size
             for (int k = 0; k < 3; ++k) pos[3 * p + k] = std::max(mn[k], std::min(pos[3 * p + k], mx[k]));
     }
 }
+
+// ---------------------------------------------------------------------------
+// CEulerParticles::generateParticlesInFluid (EulerParticles.cpp:163-201):
+// per_cell particles in every cell with fluid < 0 and not solid < 0
+// (findFluidGrid, EulerParticlesTool.cu:5-25), cells in x-fastest order,
+// particle i of the k-th such cell at index k * per_cell + i, placed at
+// (x + r) * h + o per axis.  The reference draws r = rand() / RAND_MAX with
+// the generator seeded by the clock (not reproducible); here r is drawn from
+// SplitMix64's output function (Steele, Lea, Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014) applied to (seed, 3 * index +
+// axis), top 24 bits, in [0, 1).
+inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline float particle_jitter(uint64_t seed, uint64_t index, int axis) {
+    return float(splitmix64(seed ^ splitmix64(3 * index + uint64_t(axis))) >> 40) * (1.0f / 16777216.0f);
+}
+template <class R>
+std::vector<R> generate_particles(const Grid<R>& g, const R* fluid, const R* solid, int per_cell, uint64_t seed) {
+    std::vector<R> pos;
+    size_t rank = 0;
+    for (int z = 0; z < g.nz; ++z)
+        for (int y = 0; y < g.ny; ++y)
+            for (int x = 0; x < g.nx; ++x) {
+                const size_t c = lin(x, y, z, g.nx, g.ny, g.nz);
+                if (!(fluid[c] < R(0)) || (solid && solid[c] < R(0))) continue;
+                for (int i = 0; i < per_cell; ++i) {
+                    const uint64_t idx = uint64_t(rank) * uint64_t(per_cell) + uint64_t(i);
+                    pos.push_back((R(x) + R(particle_jitter(seed, idx, 0))) * g.hx + g.ox);
+                    pos.push_back((R(y) + R(particle_jitter(seed, idx, 1))) * g.hy + g.oy);
+                    pos.push_back((R(z) + R(particle_jitter(seed, idx, 2))) * g.hz + g.oz);
+                }
+                ++rank;
+            }
+    return pos;
+}
+
+// The particle CAdvectionSolver plug-ins on one particle set:
+// CParticleInCell (ParticleInCell.cpp:58-106), CFluidImplicitParticle
+// (FluidImplicitParticle.cpp:60-125), CMixPICAndFLIP (MixPICAndFLIP.cpp:
+// 64-134).  The vector transfers are the three component transfers of
+// tranferFCVField2ParticlesInvoker / tranferParticles2FCVFieldInvoker
+// (EulerParticlesTool.cu:601-711, 820-868) with span 3, offset = component.
+enum PSolverKind { PS_PIC = 0, PS_FLIP = 1, PS_MIX = 2 };
+template <class R> struct ParticleSolver {
+    Grid<R> g;
+    int kind = PS_PIC, alg = PG_LINEAR;
+    R cfl = R(1), mixing = R(0.1);
+    std::vector<R> pos, vel, scalar, vec3, dvel, picvel;
+    std::vector<R> delta[3];                 // m_DeltaGridVelField
+    bool sflag = false, vflag = false, velflag = false;
+
+    size_t np() const { return pos.size() / 3; }
+    size_t faces(int c) const {
+        const PGrid<R> f = face_grid(g, c);
+        return size_t(f.nx) * f.ny * f.nz;
+    }
+    void init(size_t n) {
+        vel.assign(3 * n, R(0)); scalar.assign(n, R(0)); vec3.assign(3 * n, R(0));
+        dvel.assign(3 * n, R(0)); picvel.assign(3 * n, R(0));
+        for (int c = 0; c < 3; ++c) delta[c].assign(faces(c), R(0));
+    }
+    void g2p_fcv(const R* const f[3], R* out) const {
+        for (int c = 0; c < 3; ++c) g2p(face_grid(g, c), np(), pos.data(), f[c], alg, out, 3, c);
+    }
+    void p2g_fcv(const R* val, R* const out[3]) const {
+        for (int c = 0; c < 3; ++c) {
+            std::vector<R> wt(faces(c));
+            p2g(face_grid(g, c), np(), pos.data(), val, 3, c, alg, out[c], wt.data(), true);
+        }
+    }
+    // m_DeltaGridVelField.plusAlphaX(in, -1); *= -1 (the first call copies in).
+    void update_delta(const R* const in[3]) {
+        for (int c = 0; c < 3; ++c)
+            for (size_t i = 0; i < delta[c].size(); ++i)
+                delta[c][i] = velflag ? -fma_r(R(-1), in[c][i], delta[c][i]) : in[c][i];
+        velflag = true;
+    }
+    void advect_velocity(const R* const in[3], const R* const V[3], R dt, int order, bool fix, R* const out[3]) {
+        if (kind == PS_PIC) {
+            g2p_fcv(in, vel.data());
+        } else {
+            if (kind == PS_MIX) g2p_fcv(in, picvel.data());
+            update_delta(in);
+            const R* d[3] = {delta[0].data(), delta[1].data(), delta[2].data()};
+            g2p_fcv(d, dvel.data());
+            for (size_t i = 0; i < vel.size(); ++i) vel[i] = fma_r(R(1), dvel[i], vel[i]);
+            if (kind == PS_MIX) {
+                const R keep = R(1.0 - double(mixing));
+                for (size_t i = 0; i < vel.size(); ++i) vel[i] = keep * vel[i];
+                for (size_t i = 0; i < vel.size(); ++i) vel[i] = fma_r(mixing, picvel[i], vel[i]);
+            }
+        }
+        advect_particles(g, V[0], V[1], V[2], np(), pos.data(), dt, cfl, order, fix);
+        p2g_fcv(vel.data(), out);
+        if (kind != PS_PIC)
+            for (int c = 0; c < 3; ++c) delta[c].assign(out[c], out[c] + faces(c));
+    }
+    void advect_scalar(const R* in, R* out) {
+        const PGrid<R> cg = cell_grid(g);
+        if (!sflag) g2p(cg, np(), pos.data(), in, alg, scalar.data(), 1, 0);
+        sflag = true;
+        std::vector<R> wt(g.cells());
+        p2g(cg, np(), pos.data(), scalar.data(), 1, 0, alg, out, wt.data(), true);
+    }
+    void advect_vector(const R* const in[3], R* const out[3]) {
+        const PGrid<R> cg = cell_grid(g);
+        if (!vflag)
+            for (int c = 0; c < 3; ++c) g2p(cg, np(), pos.data(), in[c], alg, vec3.data(), 3, c);
+        vflag = true;
+        std::vector<R> wt(g.cells());
+        for (int c = 0; c < 3; ++c) p2g(cg, np(), pos.data(), vec3.data(), 3, c, alg, out[c], wt.data(), true);
+    }
+};
 
 // FP64 true residual ||b - A x|| / ||b|| with A from the markers.
 template <class R>

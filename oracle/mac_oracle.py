@@ -112,6 +112,19 @@ def _setup(L):
     L.orc_scene_step.argtypes = [C.c_void_p, P, C.c_int32, S]
     L.orc_scene_solid_state.argtypes = [C.c_void_p, C.c_int32, _F, _F, C.c_int64]
     L.orc_num_threads.restype = C.c_int
+    L.orc_splitmix64.argtypes = [C.c_uint64]
+    L.orc_splitmix64.restype = C.c_uint64
+    L.orc_generate_particles.argtypes = [g, _F, _F, C.c_int32, C.c_uint64, _F, C.c_int64]
+    L.orc_generate_particles.restype = C.c_int64
+    L.orc_psolver_create.argtypes = [g, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_int64, _F]
+    L.orc_psolver_create.restype = C.c_void_p
+    L.orc_psolver_destroy.argtypes = [C.c_void_p]
+    L.orc_psolver_advect_velocity.argtypes = [C.c_void_p] + [_F] * 6 + [C.c_float, C.c_int32, C.c_int32] + [_F] * 3
+    L.orc_psolver_advect_scalar.argtypes = [C.c_void_p, _F, _F]
+    L.orc_psolver_advect_vector.argtypes = [C.c_void_p] + [_F] * 6
+    L.orc_psolver_get.argtypes = [C.c_void_p, C.c_int32, _F, C.c_int64]
+    L.orc_scene_step_particles.argtypes = [C.c_void_p, C.c_void_p, P, C.c_int32, S]
+    L.orc_scene_pressure_solve_particles.argtypes = [C.c_void_p, C.c_void_p, P, S]
 
 
 def f32(a):
@@ -308,6 +321,69 @@ def advect_particles(res, origin, spacing, u, v, w, pos, dt, cfl, order, fix=Fal
     return pos
 
 
+def generate_particles(res, origin, spacing, fluid, solid, per_cell, seed):
+    """CEulerParticles::generateParticlesInFluid (EulerParticles.cpp:163-201)
+    with the seeded counter-based jitter; returns (n, 3) positions."""
+    g = grid(res, origin, spacing)
+    fluid = f32(fluid)
+    solid_p = fp(f32(solid)) if solid is not None else None
+    n = lib().orc_generate_particles(C.byref(g), fp(fluid), solid_p, per_cell, seed, None, 0)
+    out = np.empty(3 * n, np.float32)
+    lib().orc_generate_particles(C.byref(g), fp(fluid), solid_p, per_cell, seed, fp(out), n)
+    return out.reshape(-1, 3)
+
+
+PSOLVER = dict(pic=0, flip=1, mix=2)
+
+
+class ParticleSolver:
+    """CParticleInCell / CFluidImplicitParticle / CMixPICAndFLIP restated
+    (ParticleInCell.cpp:58-106, FluidImplicitParticle.cpp:60-125,
+    MixPICAndFLIP.cpp:64-134) on the CPU, FP32."""
+
+    def __init__(self, res, origin, spacing, kind, pos, cfl=1.0, alg="linear", mixing=0.1):
+        self.res = tuple(int(r) for r in res)
+        self.g = grid(res, origin, spacing)
+        pos = f32(pos).ravel()
+        self.n = pos.size // 3
+        self.h = lib().orc_psolver_create(C.byref(self.g), PSOLVER[kind], PG[alg], C.c_float(cfl),
+                                          C.c_float(mixing), self.n, fp(pos))
+
+    def close(self):
+        if self.h:
+            lib().orc_psolver_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def advect_velocity(self, inp, vel, dt, order, fix):
+        nx, ny, nz = self.res
+        out = [np.empty((nx + 1) * ny * nz, np.float32), np.empty(nx * (ny + 1) * nz, np.float32),
+               np.empty(nx * ny * (nz + 1), np.float32)]
+        a = [f32(x).ravel() for x in list(inp) + list(vel)]
+        lib().orc_psolver_advect_velocity(self.h, *[fp(x) for x in a], C.c_float(dt), order, 1 if fix else 0,
+                                          *[fp(x) for x in out])
+        return out
+
+    def advect_scalar(self, inp):
+        out = np.empty(int(np.prod(self.res)), np.float32)
+        lib().orc_psolver_advect_scalar(self.h, fp(f32(inp).ravel()), fp(out))
+        return out
+
+    def advect_vector(self, inp):
+        n = int(np.prod(self.res))
+        out = [np.empty(n, np.float32) for _ in range(3)]
+        a = [f32(x).ravel() for x in inp]
+        lib().orc_psolver_advect_vector(self.h, *[fp(x) for x in a], *[fp(x) for x in out])
+        return out
+
+    def get(self, which):
+        count = {0: 3, 1: 3, 3: 1, 4: 3}[which] * self.n
+        out = np.empty(count, np.float32)
+        assert lib().orc_psolver_get(self.h, which, fp(out), count) == 0
+        return out
+
+
 def fluid_density(res, phi, solid):
     """buildFluidDensity (EulerSolverTool.cu:408-448) on host arrays."""
     g = grid(res)
@@ -434,4 +510,15 @@ class Scene:
     def step(self, p, n=1):
         st = OrcStats()
         lib().orc_scene_step(self.h, C.byref(p), n, C.byref(st))
+        return st.as_dict()
+
+    def step_particles(self, ps, p, n=1):
+        """update() with a particle advection solver and particle markers."""
+        st = OrcStats()
+        assert lib().orc_scene_step_particles(self.h, ps.h, C.byref(p), n, C.byref(st)) == 0
+        return st.as_dict()
+
+    def pressure_solve_particles(self, ps, p):
+        st = OrcStats()
+        assert lib().orc_scene_pressure_solve_particles(self.h, ps.h, C.byref(p), C.byref(st)) == 0
         return st.as_dict()

@@ -148,11 +148,11 @@ template <class R> struct Scene {
         apply_external_forces(g, u.data(), v.data(), w.data(), gr, fo, R(p.dt));
     }
 
-    void pressure(const orc_params& p, orc_stats* st) {
+    void pressure(const orc_params& p, orc_stats* st, const R* markers_in = nullptr) {
         PressureOut<R> o = pressure_solve(g, R(p.dt), R(p.density), u.data(), v.data(),
                                           w.data(), us.data(), vs.data(), ws.data(),
                                           solid_marker.data(), phi.data(), p.tol_mode,
-                                          p.tolerance, p.max_iterations, p.jacobi != 0);
+                                          p.tolerance, p.max_iterations, p.jacobi != 0, markers_in);
         if (st) {
             st->cg_iterations += o.cg.iterations;
             st->converged = st->converged && o.cg.converged;
@@ -167,6 +167,43 @@ template <class R> struct Scene {
         last_x.swap(o.x);
         last_b.swap(o.b);
         last_A.swap(o.A);
+    }
+
+    // _computeAdvection through a particle solver (GridFluidSolver.cpp:399-416):
+    // velocity, colour, fluid domain, all with the old velocity; the boundary
+    // SDF passed there has the grid's resolution, so the particles are clamped.
+    void advect_particles_solver(ParticleSolver<R>& ps, const orc_params& p) {
+        std::vector<R> nu(u.size()), nv(v.size()), nw(w.size()), nphi(phi.size());
+        const R* in[3] = {u.data(), v.data(), w.data()};
+        R* out[3] = {nu.data(), nv.data(), nw.data()};
+        ps.advect_velocity(in, in, R(p.dt), p.rk_order, true, out);
+        if (!color[0].empty()) {
+            std::vector<R> nc[3];
+            for (auto& c : nc) c.resize(g.cells());
+            const R* ic[3] = {color[0].data(), color[1].data(), color[2].data()};
+            R* oc[3] = {nc[0].data(), nc[1].data(), nc[2].data()};
+            ps.advect_vector(ic, oc);
+            for (int k = 0; k < 3; ++k) color[k].swap(nc[k]);
+        }
+        ps.advect_scalar(phi.data(), nphi.data());
+        u.swap(nu); v.swap(nv); w.swap(nw); phi.swap(nphi);
+    }
+
+    // CPressureSolver::solvePressure with the particles' positions
+    // (PressureSolver.cpp:109-134, buildFluidMarkersInvoker EulerParticlesTool.cu:366-411).
+    void pressure_particles(const ParticleSolver<R>& ps, const orc_params& p, orc_stats* st) {
+        std::vector<R> m(g.cells());
+        particle_markers(g, solid_marker.data(), ps.np(), ps.pos.data(), m.data());
+        pressure(p, st, m.data());
+    }
+
+    void step_particles(ParticleSolver<R>& ps, const orc_params& p, orc_stats* st) {
+        update_boundaries(R(p.dt));
+        advect_particles_solver(ps, p);
+        forces(p);
+        pressure_particles(ps, p, st);
+        const int used = extrapolate(p);
+        if (st) st->extrapolation_layers = used;
     }
 
     int extrapolate(const orc_params& p) {
@@ -572,6 +609,81 @@ int orc_scene_solid_state(void* hp, int32_t idx, float* translation, float* sdf,
         }
         return 0;
     });
+}
+
+// ---- particle sets and particle solvers (FP32) -------------------------------
+uint64_t orc_splitmix64(uint64_t z) { return splitmix64(z); }
+
+int64_t orc_generate_particles(const orc_grid* gr, const float* fluid, const float* solid, int32_t per_cell,
+                               uint64_t seed, float* out, int64_t cap) {
+    const std::vector<float> p = generate_particles(to_grid<float>(gr), fluid, solid, per_cell, seed);
+    const int64_t n = int64_t(p.size() / 3);
+    for (int64_t i = 0; i < 3 * std::min(n, cap); ++i) out[i] = p[size_t(i)];
+    return n;
+}
+
+void* orc_psolver_create(const orc_grid* gr, int32_t kind, int32_t alg, float cfl, float mixing, int64_t n,
+                         const float* pos) {
+    auto* ps = new ParticleSolver<float>;
+    ps->g = to_grid<float>(gr);
+    ps->kind = kind; ps->alg = alg; ps->cfl = cfl; ps->mixing = mixing;
+    ps->pos.assign(pos, pos + 3 * n);
+    ps->init(size_t(n));
+    return ps;
+}
+
+void orc_psolver_destroy(void* h) { delete static_cast<ParticleSolver<float>*>(h); }
+
+int orc_psolver_advect_velocity(void* h, const float* iu, const float* iv, const float* iw, const float* vu,
+                                const float* vv, const float* vw, float dt, int32_t order, int32_t fix, float* ou,
+                                float* ov, float* ow) {
+    auto* ps = static_cast<ParticleSolver<float>*>(h);
+    const float* in[3] = {iu, iv, iw};
+    const float* V[3] = {vu, vv, vw};
+    float* out[3] = {ou, ov, ow};
+    ps->advect_velocity(in, V, dt, order, fix != 0, out);
+    return 0;
+}
+
+int orc_psolver_advect_scalar(void* h, const float* in, float* out) {
+    static_cast<ParticleSolver<float>*>(h)->advect_scalar(in, out);
+    return 0;
+}
+
+int orc_psolver_advect_vector(void* h, const float* ix, const float* iy, const float* iz, float* ox, float* oy,
+                              float* oz) {
+    const float* in[3] = {ix, iy, iz};
+    float* out[3] = {ox, oy, oz};
+    static_cast<ParticleSolver<float>*>(h)->advect_vector(in, out);
+    return 0;
+}
+
+// which: 0 positions, 1 velocities, 3 scalar values, 4 vector values (the
+// numbering of MGF_PARTICLE_*).
+int orc_psolver_get(void* h, int32_t which, float* out, int64_t count) {
+    auto* ps = static_cast<ParticleSolver<float>*>(h);
+    const std::vector<float>* v = which == 0 ? &ps->pos : which == 1 ? &ps->vel : which == 3 ? &ps->scalar
+                                : which == 4 ? &ps->vec3 : nullptr;
+    if (!v || int64_t(v->size()) != count) return 1;
+    std::copy(v->begin(), v->end(), out);
+    return 0;
+}
+
+int orc_scene_step_particles(void* hp, void* psh, const orc_params* p, int32_t nsteps, orc_stats* st) {
+    auto* h = static_cast<SceneHandle*>(hp);
+    if (h->f64) return 1;
+    if (st) { *st = orc_stats{}; st->converged = 1; }
+    auto* ps = static_cast<ParticleSolver<float>*>(psh);
+    for (int i = 0; i < nsteps; ++i) h->s32->step_particles(*ps, *p, st);
+    return 0;
+}
+
+int orc_scene_pressure_solve_particles(void* hp, void* psh, const orc_params* p, orc_stats* st) {
+    auto* h = static_cast<SceneHandle*>(hp);
+    if (h->f64) return 1;
+    if (st) { *st = orc_stats{}; st->converged = 1; }
+    h->s32->pressure_particles(*static_cast<ParticleSolver<float>*>(psh), *p, st);
+    return 0;
 }
 
 int orc_num_threads(void) {
