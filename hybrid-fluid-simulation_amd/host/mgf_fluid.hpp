@@ -7,6 +7,9 @@
 //   CAdvectionSolver          MPhysics/AdvectionSolver.h:12-40
 //   CSemiLagrangian           MPhysics/SemiLagrangian.h:4-50
 //   CExternalForcesSolver     MPhysics/ExternalForcesSolver.h:6-28
+//   CEulerParticles           MPhysics/EulerParticles.h:5-64
+//   CParticleInCell, CFluidImplicitParticle, CMixPICAndFLIP
+//                             MPhysics/ParticleInCell.h, FluidImplicitParticle.h, MixPICAndFLIP.h
 //   CCellCenteredScalarField, CCellCenteredVectorField, CFaceCenteredVectorField
 //                             (MPhysics/CellCenteredScalarField.h, CellCenteredVectorField.h,
 //                             FaceCenteredVectorField.h; layouts Grid.h:5-24, VectorGrid.h:34-36)
@@ -17,9 +20,11 @@
 #pragma once
 
 #include <cmath>
+#include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "mgf.h"
@@ -383,6 +388,323 @@ private:
     Vector3 m_ExternalForces = Vector3(0, 0, 0);
 };
 
+// ---------------------------------------------------------------------------
+// EPGTransferAlgorithm (MPhysics/EnumType.h:19-26).
+enum class EPGTransferAlgorithm {
+    P2GSUM = MGF_PG_P2GSUM, G2PNEAREST = MGF_PG_G2PNEAREST, LINEAR = MGF_PG_LINEAR,
+    QUADRATIC = MGF_PG_QUADRATIC, CUBIC = MGF_PG_CUBIC
+};
+using UInt = unsigned int;
+
+// CEulerParticles (MPhysics/EulerParticles.h:5-64): a particle set in device
+// memory the library owns.  Movable, not copyable (the reference's
+// thrust-vector copies are replaced by explicit get/set through the host).
+class CEulerParticles {
+public:
+    CEulerParticles() = default;
+    explicit CEulerParticles(UInt n, const Real* pos = nullptr, const Real* vel = nullptr,
+                             const Real* color = nullptr) {
+        resize(n, pos, vel, color);
+    }
+    CEulerParticles(const CEulerParticles&) = delete;
+    CEulerParticles& operator=(const CEulerParticles&) = delete;
+    CEulerParticles(CEulerParticles&& o) noexcept : m_P(o.m_P) { o.m_P = nullptr; }
+    CEulerParticles& operator=(CEulerParticles&& o) noexcept {
+        std::swap(m_P, o.m_P);
+        return *this;
+    }
+    ~CEulerParticles() { release(); }
+
+    // CEulerParticles::resize (EulerParticles.cpp:20-67): host arrays or zeros.
+    void resize(UInt n, const Real* pos = nullptr, const Real* vel = nullptr, const Real* color = nullptr) {
+        release();
+        check(mgf_particles_create(ctx(), int64_t(n), pos, vel, color, &m_P), "mgf_particles_create");
+    }
+    UInt getNumOfParticles() const {
+        if (!m_P) return 0;
+        int64_t n = 0;
+        check(mgf_particles_count(m_P, &n), "mgf_particles_count");
+        return UInt(n);
+    }
+    const Real* getConstParticlesPosGPUPtr() const { return array(MGF_PARTICLE_POS); }
+    const Real* getConstParticlesVelGPUPtr() const { return array(MGF_PARTICLE_VEL); }
+    const Real* getConstParticlesColorGPUPtr() const { return array(MGF_PARTICLE_COLOR); }
+    Real* getParticlesPosGPUPtr() { return array(MGF_PARTICLE_POS); }
+    Real* getParticlesVelGPUPtr() { return array(MGF_PARTICLE_VEL); }
+    Real* getParticlesColorGPUPtr() { return array(MGF_PARTICLE_COLOR); }
+    // getParticlesPos / Vel / Color as host copies.
+    std::vector<Real> getParticlesPos() const { return fetch(MGF_PARTICLE_POS); }
+    std::vector<Real> getParticlesVel() const { return fetch(MGF_PARTICLE_VEL); }
+    std::vector<Real> getParticlesColor() const { return fetch(MGF_PARTICLE_COLOR); }
+    void setParticlesPos(const Real* cpu) { put(MGF_PARTICLE_POS, cpu); }
+    void setParticlesVel(const Real* cpu) { put(MGF_PARTICLE_VEL, cpu); }
+    void setParticlesColor(const Real* cpu) { put(MGF_PARTICLE_COLOR, cpu); }
+
+    // generateParticlesInFluid (EulerParticles.cpp:163-201): seeded jitter in
+    // place of the reference's clock-seeded rand().
+    void generateParticlesInFluid(const CCellCenteredScalarField& vFluidSDF, const CCellCenteredScalarField& vSolidSDF,
+                                  UInt vNumOfPerGrid, uint64_t seed = 0) {
+        require_same_grid(vFluidSDF, vSolidSDF, "CEulerParticles::generateParticlesInFluid");
+        const mgf_grid g = vFluidSDF.toMgfGrid();
+        mgf_particles p = nullptr;
+        check(mgf_particles_generate(ctx(), &g, vFluidSDF.getConstGridDataGPUPtr(), vSolidSDF.getConstGridDataGPUPtr(),
+                                     int32_t(vNumOfPerGrid), seed, &p),
+              "mgf_particles_generate");
+        release();
+        m_P = p;
+    }
+    // statisticalFluidDensity (EulerParticles.cpp:203-206): particles per cell.
+    void statisticalFluidDensity(CCellCenteredScalarField& vo) const {
+        const mgf_grid g = vo.toMgfGrid();
+        check(mgf_op_particle_markers(ctx(), &g, nullptr, getNumOfParticles(), getConstParticlesPosGPUPtr(),
+                                      vo.getGridDataGPUPtr()),
+              "mgf_op_particle_markers");
+    }
+
+    // The transfers of EulerParticles.cpp:208-261 (tranfer* invokers of
+    // EulerParticlesTool.cu:443-868): particle values to grid fields,
+    // normalised by the weight fields, and grid fields to particle values.
+    void transferParticlesScalarValue2Field(CCellCenteredScalarField& vo, CCellCenteredScalarField& w,
+                                            EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) const {
+        p2g(vo, 0, array(MGF_PARTICLE_SCALAR), 1, 0, a, vo.getGridDataGPUPtr(), w, w.getGridDataGPUPtr());
+    }
+    void transferParticlesVectorValue2Field(CCellCenteredVectorField& vo, CCellCenteredVectorField& w,
+                                            EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) const {
+        for (int k = 0; k < 3; ++k)
+            p2g(vo, 0, array(MGF_PARTICLE_VECTOR), 3, k, a, vo.component(k), w, w.component(k));
+    }
+    void transferParticlesVectorValue2Field(CFaceCenteredVectorField& vo, CFaceCenteredVectorField& w,
+                                            EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) const {
+        for (int k = 0; k < 3; ++k)
+            p2g(vo, k + 1, array(MGF_PARTICLE_VECTOR), 3, k, a, vo.component(k), w, w.component(k));
+    }
+    void transferParticlesColor2Field(CCellCenteredVectorField& vo, CCellCenteredVectorField& w,
+                                      EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) const {
+        for (int k = 0; k < 3; ++k)
+            p2g(vo, 0, array(MGF_PARTICLE_COLOR), 3, k, a, vo.component(k), w, w.component(k));
+    }
+    void transferParticlesVel2Field(CFaceCenteredVectorField& vo, CFaceCenteredVectorField& w,
+                                    EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) const {
+        for (int k = 0; k < 3; ++k)
+            p2g(vo, k + 1, array(MGF_PARTICLE_VEL), 3, k, a, vo.component(k), w, w.component(k));
+    }
+    void transferScalarField2Particles(const CCellCenteredScalarField& f,
+                                       EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) {
+        g2p(f, 0, f.getConstGridDataGPUPtr(), a, MGF_PARTICLE_SCALAR, 1, 0);
+    }
+    void transferVectorField2Particles(const CCellCenteredVectorField& f,
+                                       EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) {
+        for (int k = 0; k < 3; ++k) g2p(f, 0, f.component(k), a, MGF_PARTICLE_VECTOR, 3, k);
+    }
+    void transferVectorField2Particles(const CFaceCenteredVectorField& f,
+                                       EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) {
+        for (int k = 0; k < 3; ++k) g2p(f, k + 1, f.component(k), a, MGF_PARTICLE_VECTOR, 3, k);
+    }
+    void transferVelField2Particles(const CFaceCenteredVectorField& f,
+                                    EPGTransferAlgorithm a = EPGTransferAlgorithm::LINEAR) {
+        for (int k = 0; k < 3; ++k) g2p(f, k + 1, f.component(k), a, MGF_PARTICLE_VEL, 3, k);
+    }
+
+    // advectParticlesInVelField (EulerParticles.cpp:231-325): a boundary SDF of
+    // the velocity's resolution clamps the particles to the grid box.
+    void advectParticlesInVelField(const CFaceCenteredVectorField& vel, Real dt, Real cfl,
+                                   const CCellCenteredScalarField& boundary = CCellCenteredScalarField(),
+                                   EAdvectionAccuracy acc = EAdvectionAccuracy::RK3) {
+        const mgf_grid g = vel.toMgfGrid();
+        const int fix = !boundary.empty() && boundary.getResolution() == vel.getResolution();
+        check(mgf_particles_advect(handle(), &g, vel.getConstGridDataXGPUPtr(), vel.getConstGridDataYGPUPtr(),
+                                   vel.getConstGridDataZGPUPtr(), dt, cfl, int32_t(acc), fix),
+              "mgf_particles_advect");
+    }
+
+    mgf_particles handle() const {
+        if (!m_P) throw std::logic_error("CEulerParticles: no particles (resize or generate first)");
+        return m_P;
+    }
+
+private:
+    Real* array(int which) const {
+        float* p = nullptr;
+        check(mgf_particles_data(handle(), which, &p), "mgf_particles_data");
+        return p;
+    }
+    size_t count(int which) const { return size_t(getNumOfParticles()) * (which == MGF_PARTICLE_SCALAR ? 1 : 3); }
+    std::vector<Real> fetch(int which) const {
+        std::vector<Real> h(count(which));
+        check(mgf_particles_download(handle(), which, h.data(), int64_t(h.size())), "mgf_particles_download");
+        return h;
+    }
+    void put(int which, const Real* cpu) {
+        check(mgf_particles_upload(handle(), which, cpu, int64_t(count(which))), "mgf_particles_upload");
+    }
+    void p2g(const CGrid& grid, int kind, const Real* values, int span, int off, EPGTransferAlgorithm a, Real* out,
+             const CGrid& wgrid, Real* weight) const {
+        require_same_grid(grid, wgrid, "CEulerParticles: field and weight field");
+        const mgf_grid g = grid.toMgfGrid();
+        check(mgf_op_particles_to_grid(ctx(), &g, kind, getNumOfParticles(), getConstParticlesPosGPUPtr(), values,
+                                       span, off, int32_t(a), out, weight, 1),
+              "mgf_op_particles_to_grid");
+    }
+    void g2p(const CGrid& grid, int kind, const Real* field, EPGTransferAlgorithm a, int which, int span, int off) {
+        const mgf_grid g = grid.toMgfGrid();
+        check(mgf_op_grid_to_particles(ctx(), &g, kind, getNumOfParticles(), getConstParticlesPosGPUPtr(), field,
+                                       int32_t(a), array(which), span, off),
+              "mgf_op_grid_to_particles");
+    }
+    void release() {
+        if (m_P) mgf_particles_destroy(m_P);
+        m_P = nullptr;
+    }
+    mgf_particles m_P = nullptr;
+};
+
+// The particle CAdvectionSolver plug-ins (CParticleInCell, ParticleInCell.cpp;
+// CFluidImplicitParticle, FluidImplicitParticle.cpp; CMixPICAndFLIP,
+// MixPICAndFLIP.cpp) share this base: particles generated in the fluid, the
+// solver state of mgf_psolver, and the three advect overloads.
+class CParticleAdvectionSolver : public CAdvectionSolver {
+public:
+    ~CParticleAdvectionSolver() override { release(); }
+    CParticleAdvectionSolver(const CParticleAdvectionSolver&) = delete;
+    CParticleAdvectionSolver& operator=(const CParticleAdvectionSolver&) = delete;
+
+    void advect(const CCellCenteredScalarField& in, const CFaceCenteredVectorField& vel, Real dt,
+                CCellCenteredScalarField& out, EAdvectionAccuracy acc = EAdvectionAccuracy::RK3,
+                const CCellCenteredScalarField& sdf = CCellCenteredScalarField()) override {
+        (void)vel; (void)dt; (void)acc; (void)sdf;
+        size_like(out, in);
+        check(mgf_psolver_advect_scalar(solver(), in.getConstGridDataGPUPtr(), out.getGridDataGPUPtr()),
+              "mgf_psolver_advect_scalar");
+    }
+    void advect(const CCellCenteredVectorField& in, const CFaceCenteredVectorField& vel, Real dt,
+                CCellCenteredVectorField& out, EAdvectionAccuracy acc = EAdvectionAccuracy::RK3,
+                const CCellCenteredScalarField& sdf = CCellCenteredScalarField()) override {
+        (void)vel; (void)dt; (void)acc; (void)sdf;
+        size_like(out, in);
+        check(mgf_psolver_advect_vector(solver(), in.component(0), in.component(1), in.component(2),
+                                        out.component(0), out.component(1), out.component(2)),
+              "mgf_psolver_advect_vector");
+    }
+    void advect(const CFaceCenteredVectorField& in, const CFaceCenteredVectorField& vel, Real dt,
+                CFaceCenteredVectorField& out, EAdvectionAccuracy acc = EAdvectionAccuracy::RK3,
+                const CCellCenteredScalarField& sdf = CCellCenteredScalarField()) override {
+        size_like(out, in);
+        const int fix = !sdf.empty() && sdf.getResolution() == vel.getResolution();
+        if (&out == &in || &out == &vel) {   // the library's output may not alias its inputs
+            CFaceCenteredVectorField tmp(in.getResolution(), in.getOrigin(), in.getSpacing());
+            run_velocity(in, vel, dt, acc, fix, tmp);
+            out = tmp;
+            return;
+        }
+        run_velocity(in, vel, dt, acc, fix, out);
+    }
+
+    const CEulerParticles& getEulerParticles() const { return m_EulerParticles; }
+    mgf_psolver solver() const {
+        if (!m_S) throw std::logic_error("particle advection solver: not sized");
+        return m_S;
+    }
+    // The generation seed (the reference seeds rand() with the clock).
+    void setSeed(uint64_t seed) { m_Seed = seed; }
+
+protected:
+    explicit CParticleAdvectionSolver(int kind) : m_Kind(kind) {}
+    void resize_solver(const CCellCenteredScalarField& fluid, const CCellCenteredScalarField& solid, UInt perGrid,
+                       const Vector3i& r, const Vector3& o, const Vector3& h, Real cfl, Real mixing,
+                       EPGTransferAlgorithm alg) {
+        release();
+        m_EulerParticles.generateParticlesInFluid(fluid, solid, perGrid, m_Seed);
+        CCellCenteredScalarField g(r, o, h);
+        const mgf_grid mg = g.toMgfGrid();
+        check(mgf_psolver_create(ctx(), &mg, m_Kind, m_EulerParticles.handle(), cfl, int32_t(alg), mixing, &m_S),
+              "mgf_psolver_create");
+    }
+
+private:
+    template <class F> static void size_like(F& out, const F& in) {
+        if (out.getResolution() != in.getResolution() || out.empty())
+            out.resize(in.getResolution(), in.getOrigin(), in.getSpacing());
+    }
+    void run_velocity(const CFaceCenteredVectorField& in, const CFaceCenteredVectorField& vel, Real dt,
+                      EAdvectionAccuracy acc, int fix, CFaceCenteredVectorField& out) {
+        check(mgf_psolver_advect_velocity(solver(), in.component(0), in.component(1), in.component(2),
+                                          vel.component(0), vel.component(1), vel.component(2), dt, int32_t(acc),
+                                          fix, out.component(0), out.component(1), out.component(2)),
+              "mgf_psolver_advect_velocity");
+    }
+    void release() {
+        if (m_S) mgf_psolver_destroy(m_S);
+        m_S = nullptr;
+    }
+    int m_Kind;
+    uint64_t m_Seed = 0;
+    CEulerParticles m_EulerParticles;
+    mgf_psolver m_S = nullptr;
+};
+
+class CParticleInCell : public CParticleAdvectionSolver {
+public:
+    CParticleInCell() : CParticleAdvectionSolver(MGF_PSOLVER_PIC) {}
+    CParticleInCell(const CCellCenteredScalarField& vFluidSDF, const CCellCenteredScalarField& vSolidSDF,
+                    UInt vNumOfPerGrid, const Vector3i& vResolution, const Vector3& vOrigin = Vector3(0, 0, 0),
+                    const Vector3& vSpacing = Vector3(1, 1, 1), Real vCFLNumber = 1.0,
+                    EPGTransferAlgorithm vPGTransferAlg = EPGTransferAlgorithm::LINEAR)
+        : CParticleAdvectionSolver(MGF_PSOLVER_PIC) {
+        resizeParticleInCell(vFluidSDF, vSolidSDF, vNumOfPerGrid, vResolution, vOrigin, vSpacing, vCFLNumber,
+                             vPGTransferAlg);
+    }
+    void resizeParticleInCell(const CCellCenteredScalarField& vFluidSDF, const CCellCenteredScalarField& vSolidSDF,
+                              UInt vNumOfPerGrid, const Vector3i& vResolution,
+                              const Vector3& vOrigin = Vector3(0, 0, 0), const Vector3& vSpacing = Vector3(1, 1, 1),
+                              Real vCFLNumber = 1.0,
+                              EPGTransferAlgorithm vPGTransferAlg = EPGTransferAlgorithm::LINEAR) {
+        resize_solver(vFluidSDF, vSolidSDF, vNumOfPerGrid, vResolution, vOrigin, vSpacing, vCFLNumber, 0.0f,
+                      vPGTransferAlg);
+    }
+};
+
+class CFluidImplicitParticle : public CParticleAdvectionSolver {
+public:
+    CFluidImplicitParticle() : CParticleAdvectionSolver(MGF_PSOLVER_FLIP) {}
+    CFluidImplicitParticle(const CCellCenteredScalarField& vFluidSDF, const CCellCenteredScalarField& vSolidSDF,
+                           UInt vNumOfPerGrid, const Vector3i& vResolution, const Vector3& vOrigin = Vector3(0, 0, 0),
+                           const Vector3& vSpacing = Vector3(1, 1, 1), Real vCFLNumber = 1.0,
+                           EPGTransferAlgorithm vPGTransferAlg = EPGTransferAlgorithm::LINEAR)
+        : CParticleAdvectionSolver(MGF_PSOLVER_FLIP) {
+        resizeFluidImplicitParticle(vFluidSDF, vSolidSDF, vNumOfPerGrid, vResolution, vOrigin, vSpacing, vCFLNumber,
+                                    vPGTransferAlg);
+    }
+    void resizeFluidImplicitParticle(const CCellCenteredScalarField& vFluidSDF,
+                                     const CCellCenteredScalarField& vSolidSDF, UInt vNumOfPerGrid,
+                                     const Vector3i& vResolution, const Vector3& vOrigin = Vector3(0, 0, 0),
+                                     const Vector3& vSpacing = Vector3(1, 1, 1), Real vCFLNumber = 1.0,
+                                     EPGTransferAlgorithm vPGTransferAlg = EPGTransferAlgorithm::LINEAR) {
+        resize_solver(vFluidSDF, vSolidSDF, vNumOfPerGrid, vResolution, vOrigin, vSpacing, vCFLNumber, 0.0f,
+                      vPGTransferAlg);
+    }
+};
+
+class CMixPICAndFLIP : public CParticleAdvectionSolver {
+public:
+    CMixPICAndFLIP() : CParticleAdvectionSolver(MGF_PSOLVER_MIX) {}
+    CMixPICAndFLIP(const CCellCenteredScalarField& vFluidSDF, const CCellCenteredScalarField& vSolidSDF,
+                   UInt vNumOfPerGrid, const Vector3i& vResolution, const Vector3& vOrigin = Vector3(0, 0, 0),
+                   const Vector3& vSpacing = Vector3(1, 1, 1), Real vCFLNumber = 1.0, Real vMixingCoefficient = 0.1f,
+                   EPGTransferAlgorithm vPGTransferAlg = EPGTransferAlgorithm::LINEAR)
+        : CParticleAdvectionSolver(MGF_PSOLVER_MIX) {
+        resizeMixPICAndFLIP(vFluidSDF, vSolidSDF, vNumOfPerGrid, vResolution, vOrigin, vSpacing, vCFLNumber,
+                            vMixingCoefficient, vPGTransferAlg);
+    }
+    void resizeMixPICAndFLIP(const CCellCenteredScalarField& vFluidSDF, const CCellCenteredScalarField& vSolidSDF,
+                             UInt vNumOfPerGrid, const Vector3i& vResolution,
+                             const Vector3& vOrigin = Vector3(0, 0, 0), const Vector3& vSpacing = Vector3(1, 1, 1),
+                             Real vCFLNumber = 1.0, Real vMixingCoefficient = 0.1f,
+                             EPGTransferAlgorithm vPGTransferAlg = EPGTransferAlgorithm::LINEAR) {
+        resize_solver(vFluidSDF, vSolidSDF, vNumOfPerGrid, vResolution, vOrigin, vSpacing, vCFLNumber,
+                      vMixingCoefficient, vPGTransferAlg);
+    }
+};
+
 // Precision and stopping rule of a pressure solve.  The defaults are the
 // reference's: plain FP32 CG, identity preconditioner, sqrt(||r||) < 1e-3,
 // at most one iteration per cell (PressureSolver.cpp:80-82, CuMatrixFreePCG.h:27-28).
@@ -466,21 +788,25 @@ public:
                        const CCellCenteredScalarField& vFluidSDFField) {
         if (!m_Sized) throw std::logic_error("CPressureSolver: resizePressureSolver first");
         require_same_grid(vioFluidVelField, vFluidSDFField, "CPressureSolver::solvePressure");
-        m_Desc.dt = vDeltaT;
-        if (!m_Scene)
-            m_Scene = SceneHandle(m_Desc);
-        else
-            check(mgf_scene_set_params(m_Scene.get(), &m_Desc), "mgf_scene_set_params");
-        mgf_scene s = m_Scene.get();
-        for (int k = 0; k < 3; ++k) {
-            upload(s, MGF_FIELD_VEL_X + k, vioFluidVelField.component(k), vioFluidVelField.faces(k));
-            upload(s, MGF_FIELD_SOLID_VEL_X + k, vSolidVelField.component(k), vSolidVelField.faces(k));
-        }
-        upload(s, MGF_FIELD_SOLID_SDF, vSolidSDFField.getConstGridDataGPUPtr(), vSolidSDFField.getDataSize());
+        mgf_scene s = stage(vioFluidVelField, vDeltaT, vSolidVelField, vSolidSDFField);
         upload(s, MGF_FIELD_FLUID_DOMAIN, vFluidSDFField.getConstGridDataGPUPtr(), vFluidSDFField.getDataSize());
         const mgf_status st = mgf_scene_pressure_solve(s, &m_LastStats);
         // the reference ignores the CG result (PressureSolver.cpp:105) and projects anyway
         if (st != MGF_OK && st != MGF_ERR_NOT_CONVERGED) check(st, "mgf_scene_pressure_solve");
+        for (int k = 0; k < 3; ++k)
+            download(s, MGF_FIELD_VEL_X + k, vioFluidVelField.component(k), vioFluidVelField.faces(k));
+    }
+
+    // solvePressure with particle positions (PressureSolver.cpp:109-134): the
+    // markers come from the particles.
+    void solvePressure(CFaceCenteredVectorField& vioFluidVelField, Real vDeltaT,
+                       const CFaceCenteredVectorField& vSolidVelField, const CCellCenteredScalarField& vSolidSDFField,
+                       const CEulerParticles& vParticles) {
+        if (!m_Sized) throw std::logic_error("CPressureSolver: resizePressureSolver first");
+        require_same_grid(vioFluidVelField, vSolidSDFField, "CPressureSolver::solvePressure");
+        mgf_scene s = stage(vioFluidVelField, vDeltaT, vSolidVelField, vSolidSDFField);
+        const mgf_status st = mgf_scene_pressure_solve_particles(s, vParticles.handle(), &m_LastStats);
+        if (st != MGF_OK && st != MGF_ERR_NOT_CONVERGED) check(st, "mgf_scene_pressure_solve_particles");
         for (int k = 0; k < 3; ++k)
             download(s, MGF_FIELD_VEL_X + k, vioFluidVelField.component(k), vioFluidVelField.faces(k));
     }
@@ -493,6 +819,22 @@ public:
     const mgf_step_stats& getLastStats() const { return m_LastStats; }
 
 private:
+    // The solve's scene with the caller's velocity, solid velocity and solid marker.
+    mgf_scene stage(const CFaceCenteredVectorField& vel, Real dt, const CFaceCenteredVectorField& svel,
+                    const CCellCenteredScalarField& solid) {
+        m_Desc.dt = dt;
+        if (!m_Scene)
+            m_Scene = SceneHandle(m_Desc);
+        else
+            check(mgf_scene_set_params(m_Scene.get(), &m_Desc), "mgf_scene_set_params");
+        mgf_scene s = m_Scene.get();
+        for (int k = 0; k < 3; ++k) {
+            upload(s, MGF_FIELD_VEL_X + k, vel.component(k), vel.faces(k));
+            upload(s, MGF_FIELD_SOLID_VEL_X + k, svel.component(k), svel.faces(k));
+        }
+        upload(s, MGF_FIELD_SOLID_SDF, solid.getConstGridDataGPUPtr(), solid.getDataSize());
+        return s;
+    }
     CCellCenteredScalarField fetch(int field) const {
         if (!m_Scene) throw std::logic_error("CPressureSolver: no solve yet");
         const Vector3i r(m_Desc.nx, m_Desc.ny, m_Desc.nz);
@@ -592,11 +934,20 @@ public:
     const mgf_step_stats& getLastStats() const { return m_LastStats; }
 
     // CGridFluidSolver::update (GridFluidSolver.cpp:336-370).
+    // _computePressure's typeid dispatch (GridFluidSolver.cpp:434-525): a
+    // particle advection solver makes the markers from its particles, and the
+    // whole step then runs in the library (mgf_step_particles).
     void update() {
         prepare();
         if (fast_path()) {
             const mgf_status st = mgf_step(scene(), 1, &m_LastStats);
             if (st != MGF_OK && st != MGF_ERR_NOT_CONVERGED) check(st, "mgf_step");
+            m_CurSimulationTime += m_DeltaT;
+            return;
+        }
+        if (const CParticleAdvectionSolver* ps = particle_solver()) {
+            const mgf_status st = mgf_step_particles(scene(), ps->solver(), 1, &m_LastStats);
+            if (st != MGF_OK && st != MGF_ERR_NOT_CONVERGED) check(st, "mgf_step_particles");
             m_CurSimulationTime += m_DeltaT;
             return;
         }
@@ -614,7 +965,10 @@ public:
         check(mgf_scene_apply_forces(scene()), "mgf_scene_apply_forces");
     }
     void solvePressure() {
-        const mgf_status st = mgf_scene_pressure_solve(scene(), &m_LastStats);
+        const CParticleAdvectionSolver* ps = particle_solver();
+        const mgf_status st = ps ? mgf_scene_pressure_solve_particles(scene(), ps->getEulerParticles().handle(),
+                                                                      &m_LastStats)
+                                 : mgf_scene_pressure_solve(scene(), &m_LastStats);
         if (st != MGF_OK && st != MGF_ERR_NOT_CONVERGED) check(st, "mgf_scene_pressure_solve");
         check(mgf_scene_extrapolate_velocity(scene(), &m_LastStats.extrapolation_layers),
               "mgf_scene_extrapolate_velocity");
@@ -640,6 +994,9 @@ public:
 
 private:
     bool fast_path() const { return dynamic_cast<CSemiLagrangian*>(m_AdvectionSolver.get()) != nullptr; }
+    const CParticleAdvectionSolver* particle_solver() const {
+        return dynamic_cast<const CParticleAdvectionSolver*>(m_AdvectionSolver.get());
+    }
 
     // The scene's parameters follow the current solvers: forces, pressure options, accuracy.
     void prepare() {

@@ -264,6 +264,87 @@ int main() {
     a.resize(d.res, d.o, d.h, ones.data());
     EXPECT(same_bits(b.getHostData(), d.wall), "copy is not deep");
 
+    // 9. Particle advection solvers in the strategy slot: the step with
+    //    particle markers (mgf_step_particles) and the split path through the
+    //    virtual advect both follow the library's own particle step.  The
+    //    particle-to-grid sums are FP32 atomics, so runs agree to rounding.
+    {
+        CGridFluidSolver pf;
+        build_solver(pf, d, true);
+        const CCellCenteredScalarField wall = d.wallField();
+        const CCellCenteredScalarField dom = pf.getFluidDomainField();
+        auto flip = std::make_shared<CFluidImplicitParticle>();
+        flip->setSeed(77);
+        flip->resizeFluidImplicitParticle(dom, wall, 4, d.res, d.o, d.h, 1.0f);
+        pf.setAdvectionSolver(flip);
+        for (int i = 0; i < steps; ++i) pf.update();
+
+        SceneHandle pref = build_scene(d, true);
+        mgf_particles parts = nullptr;
+        const mgf_grid gg = wall.toMgfGrid();
+        check(mgf_particles_generate(ctx(), &gg, dom.getConstGridDataGPUPtr(), wall.getConstGridDataGPUPtr(), 4, 77,
+                                     &parts), "generate");
+        EXPECT(flip->getEulerParticles().getNumOfParticles() > 0, "no particles generated");
+        mgf_psolver solver = nullptr;
+        check(mgf_psolver_create(ctx(), &gg, MGF_PSOLVER_FLIP, parts, 1.0f, MGF_PG_LINEAR, 0.1f, &solver), "psolver");
+        check(mgf_step_particles(pref.get(), solver, steps, &st), "mgf_step_particles");
+        const int vf[] = {MGF_FIELD_VEL_X, MGF_FIELD_VEL_Y, MGF_FIELD_VEL_Z, MGF_FIELD_COLOR_X};
+        for (int f : vf) {
+            const std::vector<float> a = host_field(pf.scene(), f), b = host_field(pref.get(), f);
+            double num = 0, den = 0;
+            for (size_t i = 0; i < a.size(); ++i) {
+                num += double(a[i] - b[i]) * double(a[i] - b[i]);
+                den += double(b[i]) * double(b[i]);
+            }
+            EXPECT(num <= 1e-8 * std::max(den, 1e-30), "particle step field %d: rel L2 %g", f,
+                   std::sqrt(num / std::max(den, 1e-30)));
+        }
+        std::vector<float> pa = flip->getEulerParticles().getParticlesPos(), pb(pa.size());
+        check(mgf_particles_download(parts, MGF_PARTICLE_POS, pb.data(), int64_t(pb.size())), "download");
+        float worst = 0;
+        for (size_t i = 0; i < pa.size(); ++i) worst = std::max(worst, std::fabs(pa[i] - pb[i]));
+        EXPECT(worst <= 1e-4f, "particle positions differ by %g", double(worst));
+        check(mgf_psolver_destroy(solver), "destroy");
+        check(mgf_particles_destroy(parts), "destroy");
+
+        // the split path: virtual advect on copies + pressure with particle markers
+        CGridFluidSolver ps2;
+        build_solver(ps2, d, true);
+        auto pic = std::make_shared<CParticleInCell>(dom, wall, 2, d.res, d.o, d.h, 1.0f);
+        ps2.setAdvectionSolver(pic);
+        CGridFluidSolver ps3;
+        build_solver(ps3, d, true);
+        auto pic3 = std::make_shared<CParticleInCell>(dom, wall, 2, d.res, d.o, d.h, 1.0f);
+        ps3.setAdvectionSolver(pic3);
+        for (int i = 0; i < 2; ++i) {
+            ps2.updateWithoutPressure();
+            ps2.solvePressure();
+            ps3.update();
+        }
+        for (int f : vf) {
+            const std::vector<float> a = host_field(ps2.scene(), f), b = host_field(ps3.scene(), f);
+            double num = 0, den = 0;
+            for (size_t i = 0; i < a.size(); ++i) {
+                num += double(a[i] - b[i]) * double(a[i] - b[i]);
+                den += double(b[i]) * double(b[i]);
+            }
+            EXPECT(num <= 1e-8 * std::max(den, 1e-30), "split particle step field %d: rel L2 %g", f,
+                   std::sqrt(num / std::max(den, 1e-30)));
+        }
+
+        // CEulerParticles alone: generation and the per-cell count
+        CEulerParticles ep;
+        ep.generateParticlesInFluid(dom, wall, 3, 5);
+        CCellCenteredScalarField count(d.res, d.o, d.h);
+        ep.statisticalFluidDensity(count);
+        double total = 0;
+        for (float c : count.getHostData()) total += c;
+        EXPECT(total == double(ep.getNumOfParticles()), "counted %g of %u particles", total, ep.getNumOfParticles());
+        std::vector<float> vel0(3 * size_t(ep.getNumOfParticles()), 0.5f);
+        ep.setParticlesVel(vel0.data());
+        EXPECT(same_bits(ep.getParticlesVel(), vel0), "particle velocities round trip");
+    }
+
     std::printf("host classes: %d checks, %d failures\n", g_checks, g_failures);
     return g_failures ? 1 : 0;
 }

@@ -684,6 +684,88 @@ def particle_cases():
     out.append(dict(name="p2g_fcv_quadratic3", src=PT + ":1555-1604", kind="p2g_fcv", alg="quadratic",
                     res=list(B_RES), origin=list(A_O), spacing=list(A_H), pos=p3, val=vel,
                     expect=dict(z=quad(5.0, 6.0)), tol=EPS))
+    out += particle_motion_cases()
+    return out
+
+
+def particle_motion_cases():
+    """advectParticlesInVelField, buildFluidMarkers and statisticalFluidDensity
+    (UnitTest-ParticlesAdvection.cu:3441-4196).  The advection cases use the
+    linear velocity u = 10 i, v = 20 j, w = 30 k on the faces (i, j, k) of a
+    4x4x1 grid, so the sampled velocity is 10, 20, 30 times the position
+    relative to the origin in cells and the expected positions are the
+    reference test's closed forms, evaluated here in FP64."""
+    out = []
+    o, h = (-10.0, -20.0, -30.0), (10.0, 20.0, 30.0)
+    res = (4, 4, 1)
+    base = (o[0] + 1.25 * h[0], o[1] + 1.25 * h[1], -15.0)
+    pos = []
+    for y in range(4):
+        for x in range(4):
+            pos += [base[0] + x * 0.5 * h[0], base[1] + y * 0.5 * h[1], base[2]]
+    vx = [10.0 * i for k in range(1) for j in range(4) for i in range(5)]
+    vy = [20.0 * j for k in range(1) for j in range(5) for i in range(4)]
+    vz = [30.0 * k for k in range(2) for j in range(4) for i in range(4)]
+    scale = (10.0, 20.0, 30.0)
+
+    def rel(p):   # position relative to the origin, in cells
+        return [(p[a] - o[a]) / h[a] for a in range(3)]
+
+    def vel(r):
+        return [scale[a] * r[a] for a in range(3)]
+
+    def expect(order, steps):
+        e = []
+        for q in range(16):
+            p = pos[3 * q:3 * q + 3]
+            for dt in steps:
+                r0 = rel(p)
+                if order == 1:
+                    k = vel(r0)
+                elif order == 2:
+                    mid = [p[a] + 0.5 * dt * vel(r0)[a] for a in range(3)]
+                    k = vel(rel(mid))
+                else:
+                    k1 = vel(r0)
+                    mid = [p[a] + 0.5 * dt * k1[a] for a in range(3)]
+                    k2 = vel(rel(mid))
+                    tq = [p[a] + 0.75 * dt * k2[a] for a in range(3)]
+                    k3 = vel(rel(tq))
+                    k = [(2.0 * k1[a] + 3.0 * k2[a] + 4.0 * k3[a]) / 9.0 for a in range(3)]
+                p = [p[a] + dt * k[a] for a in range(3)]
+            e += p
+        return {str(i): v for i, v in enumerate(e)}
+
+    for name, src, order, dt, cfl, steps in [
+            ("advect_rk1", ":3445-3537", 1, 0.1, 1.0, [0.1]),
+            ("advect_rk1_3substeps", ":3539-3637", 1, 0.4, 1.1, [0.2, 0.1, 0.1]),
+            ("advect_rk2", ":3639-3735", 2, 0.1, 1.0, [0.1]),
+            ("advect_rk3", ":3737-3838", 3, 0.1, 1.0, [0.1])]:
+        out.append(dict(name=name, src=PT + src, kind="advect", res=list(res), origin=list(o), spacing=list(h),
+                        pos=pos, u=vx, v=vy, w=vz, dt=dt, cfl=cfl, order=order, expect=expect(order, steps),
+                        tol=EPS))
+    # buildFluidMarkers: solid everywhere but cells 5, 6, 9, 10, 13, 14; the
+    # 4x4 particles (a 2x2 block per cell) lie in cells 5, 6, 9, 10
+    solid = [-1.0] * 64
+    for c in (5, 6, 9, 10, 13, 14):
+        solid[c] = 1.0
+    pos = []
+    for y in range(4):
+        for x in range(4):
+            pos += [base[0] + x * 0.5 * h[0], base[1] + y * 0.5 * h[1], base[2]]
+    e = {str(i): (1.0 if i in (5, 6, 9, 10) else 0.0 if i in (13, 14) else 2.0) for i in range(64)}
+    out.append(dict(name="fluid_markers", src=PT + ":3843-3917", kind="markers", res=[4, 4, 4], origin=list(o),
+                    spacing=list(h), solid=solid, pos=pos, expect=e, tol=EPS))
+    # statisticalFluidDensity of particles generated in the 8 central cells, 8 each
+    eight = [21, 22, 25, 26, 37, 38, 41, 42]
+    fluid = [1.0] * 64
+    solid = [-1.0] * 64
+    for c in eight:
+        fluid[c] = -1.0
+        solid[c] = 1.0
+    e = {str(i): (8.0 if i in eight else 0.0) for i in range(64)}
+    out.append(dict(name="fluid_density_particles", src=PT + ":4139-4196", kind="density", res=[4, 4, 4],
+                    origin=list(o), spacing=list(h), fluid=fluid, solid=solid, per_cell=8, expect=e, tol=EPS))
     return out
 
 

@@ -87,6 +87,21 @@ def test_reference_particle_goldens(ctx, cases):
                                              c["alg"], span=3, off=k)
                 if comp in c["expect"]:
                     check_map(f, c["expect"][comp], c["tol"], c["name"] + ":" + comp)
+        elif c["kind"] == "advect":
+            p = mgf.advect_particles(ctx, c["res"], c["origin"], c["spacing"], c["u"], c["v"], c["w"], c["pos"],
+                                     c["dt"], c["cfl"], c["order"])
+            check_map(p, c["expect"], c["tol"], c["name"])
+        elif c["kind"] == "markers":
+            m = mgf.particle_markers(ctx, c["res"], c["origin"], c["spacing"], c["solid"], c["pos"])
+            check_map(m, c["expect"], c["tol"], c["name"])
+        elif c["kind"] == "density":
+            P = mgf.Particles.generate(ctx, c["res"], c["origin"], c["spacing"], c["fluid"], c["solid"],
+                                       c["per_cell"], seed=2024)
+            d = mgf.particle_markers(ctx, c["res"], c["origin"], c["spacing"], None, P.get("pos"))
+            P.close()
+            check_map(d, c["expect"], c["tol"], c["name"])
+        else:
+            raise AssertionError(c["kind"])
 
 
 @pytest.mark.parametrize("setup", range(len(SETUPS)))

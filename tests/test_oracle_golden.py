@@ -516,10 +516,11 @@ def test_fluid_sdf_closed_form(orc, passes):
 
 
 def test_particle_transfer_goldens(cases, orc):
-    """Particle <-> grid transfers of the reference's particle tests
-    (UnitTest-ParticlesAdvection.cu): P2G with P2GSUM / LINEAR / QUADRATIC /
-    CUBIC kernels (normalised by the weights), G2P with G2PNEAREST and the same
-    kernels."""
+    """The reference's particle tests (UnitTest-ParticlesAdvection.cu): P2G
+    with P2GSUM / LINEAR / QUADRATIC / CUBIC kernels (normalised by the
+    weights), G2P with G2PNEAREST and the same kernels, particle advection
+    (RK1 with and without substeps, RK2, RK3), fluid markers from particles and
+    the particle count of generated particles."""
     for c in cases["particles"]:
         if c["kind"] == "p2g":
             f, _ = orc.p2g(c["res"], c["origin"], c["spacing"], c["field"], c["pos"], c["val"], c["alg"],
@@ -535,3 +536,17 @@ def test_particle_transfer_goldens(cases, orc):
                                span=3, off=k)
                 if comp in c["expect"]:
                     check_map(f, c["expect"][comp], c["tol"], c["name"] + ":" + comp)
+        elif c["kind"] == "advect":
+            p = orc.advect_particles(c["res"], c["origin"], c["spacing"], c["u"], c["v"], c["w"], c["pos"], c["dt"],
+                                     c["cfl"], c["order"])
+            check_map(p, c["expect"], c["tol"], c["name"])
+        elif c["kind"] == "markers":
+            m = orc.particle_markers(c["res"], c["origin"], c["spacing"], c["solid"], c["pos"])
+            check_map(m, c["expect"], c["tol"], c["name"])
+        elif c["kind"] == "density":
+            pos = orc.generate_particles(c["res"], c["origin"], c["spacing"], c["fluid"], c["solid"],
+                                         c["per_cell"], seed=2024)
+            d = orc.particle_markers(c["res"], c["origin"], c["spacing"], None, pos, counts_only=True)
+            check_map(d, c["expect"], c["tol"], c["name"])
+        else:
+            raise AssertionError(c["kind"])
