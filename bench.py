@@ -273,6 +273,56 @@ def served_leg(mgf, scenes, ctx, args):
     return out
 
 
+def particles_leg(mgf, scenes, ctx, args, spec):
+    """The step with a particle advection solver on the headline scene, in the
+    reference's scene-file configuration (Application/TestScene.json:
+    MixPICAndFLIP, CFL 1, mixing 0.1, LINEAR transfers, 8 particles per fluid
+    cell; RK3 particle advection as _computeAdvection passes it): velocity,
+    colour-free fluid domain through the particles, projection with particle
+    markers (mixed-precision multigrid PCG to 1e-6), extrapolation."""
+    kw = dict(MIXED_KW)
+    kw["rk"] = 3
+    s = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
+    t0 = time.perf_counter()
+    P = s.generate_particles(args.particles_per_cell, 12345)
+    res, origin, spacing = spec.res, spec.origin, spec.spacing
+    ps = mgf.ParticleSolver(ctx, res, origin, spacing, args.particles_solver, P, cfl=1.0, alg="linear", mixing=0.1)
+    t_setup = time.perf_counter() - t0
+    n = len(P)
+    s.step_particles(ps, 1, allow_unconverged=True)     # warm: first-step allocations
+    s.sync()
+    times, conv = [], 0
+    t0 = time.perf_counter()
+    for _ in range(args.particles_steps):
+        a = time.perf_counter()
+        st = s.step_particles(ps, 1, allow_unconverged=True)
+        times.append(time.perf_counter() - a)
+        conv += int(st["converged"] == 1)
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.set_timing(True)
+    stg = s.step_particles(ps, 2, allow_unconverged=True)
+    s.set_timing(False)
+    ms = np.array(times) * 1e3
+    out = {"workload": "%s, %s solver, %d particles per fluid cell (%d particles), CFL 1, LINEAR transfers, "
+                       "RK3 particle advection" % (spec.name, args.particles_solver, args.particles_per_cell, n),
+           "particles": n, "steps": args.particles_steps, "converged_steps": conv,
+           "ms_per_step": 1e3 * wall / args.particles_steps,
+           "cell_steps_per_s": spec.interior_cells * conv / wall,
+           "particle_steps_per_s": n * args.particles_steps / wall,
+           "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
+           "stages_ms_per_step": {k: stg[k] / 2 for k in ("ms_advect", "ms_pressure", "ms_extrapolate", "ms_total")},
+           "setup_s": t_setup}
+    log("particles: %d particles, %.2f ms/step (advect+forces %.2f, pressure %.2f, extrapolation %.2f), "
+        "%.3g particle-steps/s" % (n, out["ms_per_step"], out["stages_ms_per_step"]["ms_advect"],
+                                   out["stages_ms_per_step"]["ms_pressure"],
+                                   out["stages_ms_per_step"]["ms_extrapolate"], out["particle_steps_per_s"]))
+    ps.close()
+    P.close()
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +347,10 @@ def main():
     ap.add_argument("--served-load", type=float, nargs="+", default=[0.5, 0.8, 1.0])
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--particles-per-cell", type=int, default=8)
+    ap.add_argument("--particles-steps", type=int, default=10)
+    ap.add_argument("--particles-solver", choices=["pic", "flip", "mix"], default="mix")
+    ap.add_argument("--no-particles", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs the MAC-grid step on a single GPU "
@@ -371,6 +425,8 @@ def main():
 
         batched = None if args.no_batch or args.batch_scenes <= 0 else batched_leg(mgf, scenes, ctx, args)
         served = None if args.no_served or args.served_scenes <= 0 else served_leg(mgf, scenes, ctx, args)
+        particles = (None if args.no_particles or args.particles_steps <= 0
+                     else particles_leg(mgf, scenes, ctx, args, spec))
 
     cpu = None
     if not args.no_cpu:
@@ -435,6 +491,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "particles": particles,
     }
     print(json.dumps(out))
 

@@ -204,6 +204,7 @@ mgf_status dot_host(mgf_scene s, const float* a, const float* b, size_t n, doubl
 
 // ---------------------------------------------------------------------------
 // Boundaries: CBoundarys::updateBoundarys (MPhysics/Boundary.cpp:115-150).
+mgf_status rebuild_boundaries(mgf_scene s);
 mgf_status refresh_boundaries(mgf_scene s) {
     bool moving = false;
     for (auto& so : s->solids)
@@ -212,6 +213,12 @@ mgf_status refresh_boundaries(mgf_scene s) {
     for (auto& so : s->solids)
         for (int k = 0; k < 3; ++k) so.trans[k] = std::fma(-s->desc.dt, so.vel[k], so.trans[k]);
     if (!s->boundary_dirty && !moving) return MGF_OK;
+    return rebuild_boundaries(s);
+}
+
+// The solid marker, solid mask and solid face velocities from the solids at
+// their current translations.
+mgf_status rebuild_boundaries(mgf_scene s) {
     hipStream_t st = s->stream;
     // zero-filled marker, then each solid in order
     MGF_HIP(launch_fill(st, s->solid_total, s->ncell, 0.0f));
@@ -227,6 +234,7 @@ mgf_status refresh_boundaries(mgf_scene s) {
     MGF_HIP(launch_solids_vel(st, s->g, s->solid_total, vel3.data(), int(s->solids.size()), s->us,
                               s->vs, s->ws));
     s->boundary_dirty = false;
+    s->solids_unbuilt = false;
     return MGF_OK;
 }
 
@@ -998,6 +1006,7 @@ mgf_status mgf_scene_add_solid(mgf_scene s, const float* sdf, int64_t count, con
     for (int k = 0; k < 3; ++k) so.vel[k] = velocity ? velocity[k] : 0.0f;
     s->solids.push_back(so);
     s->boundary_dirty = true;
+    s->solids_unbuilt = true;
     return MGF_OK;
 }
 
@@ -1203,6 +1212,23 @@ static bool same_grid(const mgf_scene_desc& d, const mgf_grid& g) {
     return d.nx == g.nx && d.ny == g.ny && d.nz == g.nz && d.origin[0] == g.origin[0] &&
            d.origin[1] == g.origin[1] && d.origin[2] == g.origin[2] && d.spacing[0] == g.spacing[0] &&
            d.spacing[1] == g.spacing[1] && d.spacing[2] == g.spacing[2];
+}
+
+mgf_status mgf_scene_generate_particles(mgf_scene s, int32_t per_cell, uint64_t seed, mgf_particles* out) {
+    if (!s || !out) return fail(MGF_ERR_INVALID, "null argument");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    // solids added since the last step: their marker at the current translations
+    if (s->solids_unbuilt) {
+        TRY(rebuild_boundaries(s));
+        TRY(sync(s));
+    }
+    mgf_grid g;
+    g.nx = s->desc.nx; g.ny = s->desc.ny; g.nz = s->desc.nz;
+    for (int k = 0; k < 3; ++k) {
+        g.origin[k] = s->desc.origin[k];
+        g.spacing[k] = s->desc.spacing[k];
+    }
+    return mgf_particles_generate(s->ctx, &g, s->phi[s->cur], s->solid_total, per_cell, seed, out);
 }
 
 mgf_status mgf_step_particles(mgf_scene s, mgf_psolver ps, int32_t nsteps, mgf_step_stats* stats) {
@@ -1479,14 +1505,14 @@ mgf_status mgf_op_advect_particles(mgf_context ctx, const mgf_grid* grid, const 
     MGF_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const size_t m = 3 * size_t(n);
-    // v1, v2, v3, mid, three-quarter positions, then the max word
+    // the sampled velocities, then the max word
     float* buf = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&buf), 5 * m * sizeof(float) + 64) != hipSuccess) {
+    if (hipMalloc(reinterpret_cast<void**>(&buf), m * sizeof(float) + 64) != hipSuccess) {
         (void)hipGetLastError();
         return fail(MGF_ERR_NOMEM, "particle advection scratch");
     }
     const mgf_status r = particles_advect_run(st, make_grid(grid), vel_of(u, v, w), n, pos, dt, cfl, order,
-                                              fix_to_grid, buf, reinterpret_cast<uint32_t*>(buf + 5 * m));
+                                              fix_to_grid, buf, reinterpret_cast<uint32_t*>(buf + m));
     (void)hipStreamSynchronize(st);
     (void)hipFree(buf);
     return r;

@@ -117,6 +117,7 @@ struct mgf_scene_s {
 
     std::vector<mgf::Solid> solids;
     bool boundary_dirty = false;   // solids changed since the last refresh
+    bool solids_unbuilt = false;   // solids added since the marker was last built
     bool boundary_user = false;    // SOLID_SDF / SOLID_VEL uploaded directly
     bool timing = false;
     // kernel probes: event pairs recorded around selected launches
@@ -135,7 +136,7 @@ struct mgf_scene_s {
 };
 
 // A particle set: interleaved xyz per particle; scratch = the advection
-// temporaries (v1, v2, v3, mid, three-quarter positions: 3n floats each).
+// temporary (the sampled velocity, 3n floats).
 struct mgf_particles_s {
     mgf_context ctx = nullptr;
     int64_t n = 0;
@@ -144,7 +145,7 @@ struct mgf_particles_s {
     float* color = nullptr;
     float* scalar = nullptr;   // n
     float* vector = nullptr;   // 3n
-    float* scratch = nullptr;  // 15n
+    float* scratch = nullptr;  // 3n: the velocities sampled at the particles
     uint32_t* vmax = nullptr;  // max |v| bits
     std::vector<void*> allocations;
 };
@@ -158,8 +159,6 @@ struct mgf_psolver_s {
     float cfl = 1.0f, mixing = 0.1f;
     float* delta[3] = {nullptr, nullptr, nullptr};   // previous output velocity / grid change (FLIP, MIX)
     float* weight[3] = {nullptr, nullptr, nullptr};  // weight fields (face extents; cells fit in them)
-    float* dvel = nullptr;                           // 3n: grid change at the particles
-    float* picvel = nullptr;                         // 3n: PIC velocity at the particles (MIX)
     bool scalar_seeded = false, vector_seeded = false, vel_seeded = false;
     std::vector<void*> allocations;
 };
@@ -339,10 +338,24 @@ hipError_t launch_gen_count(hipStream_t s, size_t ncell, const float* fluid, con
 hipError_t launch_gen_emit(hipStream_t s, const GridDev& g, const float* fluid, const float* solid,
                            const uint32_t* tiles, int per_cell, uint64_t seed, float* pos);
 hipError_t launch_flip_delta(hipStream_t s, size_t n, const float* in, float* delta);
+// Fused particle kernels: the velocity splat of the three face grids; the
+// particle-velocity update of PIC / FLIP / MIX (delta = NULL for PIC); the
+// first velocity sample of a substep loop with its max |v|; one RK substep
+// (resample: the next substep's velocities and max |v|).
+hipError_t launch_p2g_velocity(hipStream_t s, const GridDev& g, int64_t np, const float* pos, const float* vel,
+                               int alg, float* const field[3], float* const weight[3]);
+hipError_t launch_transfer_vel(hipStream_t s, const GridDev& g, int64_t np, const float* pos,
+                               const float* const in[3], const float* const delta[3], int kind, int alg, float keep,
+                               float mixing, float* vel);
+hipError_t launch_sample_absmax(hipStream_t s, const GridDev& g, Vel V, int64_t np, const float* pos, float* out,
+                                uint32_t* vmax_bits);
+hipError_t launch_rk_substep(hipStream_t s, const GridDev& g, Vel V, int64_t np, float* pos, float* v1, int order,
+                             float step, float half, float three_q, float c1, float c2, float c3, int resample,
+                             uint32_t* vmax_bits);
 hipError_t launch_particle_markers_u8(hipStream_t s, const GridDev& g, const uint8_t* solid_mask, int64_t np,
                                       const float* pos, uint8_t* markers);
-// The particle advection loop (mgf_particles.cpp): scratch holds 15 n floats,
-// vmax one word.
+// The particle advection loop (mgf_particles.cpp): scratch holds at least 3 n
+// floats, vmax one word.
 mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t n, float* pos, float dt, float cfl,
                                 int order, int fix, float* scratch, uint32_t* vmax);
 mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in[3], const float* const vel[3],

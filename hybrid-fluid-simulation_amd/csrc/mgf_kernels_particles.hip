@@ -106,6 +106,93 @@ __global__ void __launch_bounds__(kBlock) k_p2g(PGrid g, int64_t np, const float
             }
 }
 
+// The same sums with a work-group's particles first merged in an LDS hash
+// table keyed by (component, node): particles of a work-group are neighbours
+// in index order (generation writes them cell by cell) and mostly share the
+// nodes of their supports, so a table entry gathers several contributions and
+// the flush issues one global atomic per distinct node.  Contributions that
+// find no slot within kProbe probes go straight to global memory.  NC
+// components (1: one field; 3: the three face grids of a velocity with
+// values[3 p + c]) share the table; the key's top two bits are the component
+// (every field holds fewer than 2^30 values).
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr int kProbe = 32;
+
+struct P2GOut {
+    float* field[3];
+    float* weight[3];
+};
+
+template <int HB>
+__device__ __forceinline__ void lds_accumulate(uint32_t* keys, float* sv, float* sw, uint32_t key, float a, float b,
+                                               bool with_w, float* gf, float* gw) {
+    constexpr uint32_t T = 1u << HB;
+    uint32_t h = (key * 2654435761u) >> (32 - HB);
+    for (int probe = 0; probe < kProbe; ++probe) {
+        // a slot once claimed keeps its key: read first, claim empty slots only
+        uint32_t k = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
+        if (k == kEmptyKey) k = atomicCAS(&keys[h], kEmptyKey, key);
+        if (k == kEmptyKey || k == key) {
+            atomicAdd(&sv[h], a);
+            if (with_w) atomicAdd(&sw[h], b);
+            return;
+        }
+        h = (h + 1) & (T - 1);
+    }
+    atomicAdd(gf, a);
+    if (with_w) atomicAdd(gw, b);
+}
+
+template <int NC, int HB>
+__global__ void __launch_bounds__(kBlock) k_p2g_lds(PGrid g0, PGrid g1, PGrid g2, int64_t np,
+                                                    const float* __restrict__ pos, const float* __restrict__ val,
+                                                    int span, int off, int alg, P2GOut out) {
+    constexpr uint32_t T = 1u << HB;
+    __shared__ uint32_t keys[T];
+    __shared__ float sv[T], sw[T];
+    for (uint32_t i = threadIdx.x; i < T; i += kBlock) {
+        keys[i] = kEmptyKey;
+        sv[i] = 0.0f;
+        sw[i] = 0.0f;
+    }
+    __syncthreads();
+    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    const bool with_w = alg != MGF_PG_P2GSUM;
+    if (p < np) {
+        const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const PGrid& g = c == 0 ? g0 : (c == 1 ? g1 : g2);
+            const float v = val[int64_t(span) * p + (NC == 1 ? off : c)];
+            const uint32_t tag = uint32_t(c) << 30;
+            if (alg == MGF_PG_P2GSUM) {
+                const uint32_t node = pos_cell(g, px, py, pz);
+                lds_accumulate<HB>(keys, sv, sw, tag | node, v, 0.0f, false, &out.field[c][node], nullptr);
+                continue;
+            }
+            PStencil st;
+            pg_stencil(g, px, py, pz, alg, st);
+            const int n = alg == MGF_PG_LINEAR ? 2 : 4, lo = alg == MGF_PG_LINEAR ? 0 : -1;
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y)
+                    for (int x = 0; x < n; ++x) {
+                        const float wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
+                        const uint32_t node = pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo);
+                        lds_accumulate<HB>(keys, sv, sw, tag | node, wt * v, wt, true, &out.field[c][node],
+                                           &out.weight[c][node]);
+                    }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < T; i += kBlock) {
+        const uint32_t k = keys[i];
+        if (k == kEmptyKey) continue;
+        const uint32_t c = k >> 30, node = k & 0x3FFFFFFFu;
+        atomicAdd(&out.field[c][node], sv[i]);
+        if (with_w) atomicAdd(&out.weight[c][node], sw[i]);
+    }
+}
+
 // normalizeCCSField (EulerParticlesTool.cu:290-308).
 __global__ void __launch_bounds__(kBlock) k_p2g_normalize(size_t n, float* __restrict__ field,
                                                           const float* __restrict__ weight) {
@@ -136,6 +223,128 @@ __global__ void __launch_bounds__(kBlock) k_g2p(PGrid g, int64_t np, const float
                 }
     }
     val[int64_t(span) * p + off] = acc;
+}
+
+// The value of one field at a particle (the body of k_g2p).
+__device__ __forceinline__ float g2p_one(const PGrid& g, float px, float py, float pz, const float* __restrict__ f,
+                                         int alg) {
+    if (alg == MGF_PG_G2PNEAREST) return f[pos_cell(g, px, py, pz)];
+    PStencil st;
+    pg_stencil(g, px, py, pz, alg, st);
+    const int n = alg == MGF_PG_LINEAR ? 2 : 4, lo = alg == MGF_PG_LINEAR ? 0 : -1;
+    float acc = 0.0f;
+    for (int z = 0; z < n; ++z)
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) {
+                const float wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
+                acc += wt * f[pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo)];
+            }
+    return acc;
+}
+
+// The particle-velocity update of the three solvers in one pass, with the
+// operations and rounding of the reference's separate calls:
+//   PIC  (ParticleInCell.cpp:101):  v = G2P(in)
+//   FLIP (FluidImplicitParticle.cpp:109-110): v = fma(1, G2P(delta), v)
+//   MIX  (MixPICAndFLIP.cpp:114-129): p = G2P(in); v = fma(1, G2P(delta), v);
+//        v = keep * v; v = fma(mixing, p, v)
+struct Faces3 {
+    const float* f[3];
+};
+__global__ void __launch_bounds__(kBlock) k_transfer_vel(PGrid g0, PGrid g1, PGrid g2, int64_t np,
+                                                         const float* __restrict__ pos, Faces3 in, Faces3 delta,
+                                                         int kind, int alg, float keep, float mixing,
+                                                         float* __restrict__ vel) {
+    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const PGrid& g = c == 0 ? g0 : (c == 1 ? g1 : g2);
+        if (kind == MGF_PSOLVER_PIC) {
+            vel[3 * p + c] = g2p_one(g, px, py, pz, in.f[c], alg);
+            continue;
+        }
+        const float pic = kind == MGF_PSOLVER_MIX ? g2p_one(g, px, py, pz, in.f[c], alg) : 0.0f;
+        float v = __builtin_fmaf(1.0f, g2p_one(g, px, py, pz, delta.f[c], alg), vel[3 * p + c]);
+        if (kind == MGF_PSOLVER_MIX) v = __builtin_fmaf(mixing, pic, keep * v);
+        vel[3 * p + c] = v;
+    }
+}
+
+// max over the work-group, then one atomic per work-group.
+__device__ __forceinline__ void block_max_atomic(uint32_t m, uint32_t* out) {
+    __shared__ uint32_t wmax[kBlock / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t b = wmax[0];
+#pragma unroll
+        for (int q = 1; q < kBlock / 64; ++q) b = max(b, wmax[q]);
+        atomicMax(out, b);
+    }
+}
+
+// One particle advection substep in one pass (EulerParticles.cpp:273-313:
+// the axpys of RK1/2/3 with the samples between them, same order and
+// rounding), v1 = the velocity sampled at the positions it starts from.  With
+// resample, the velocity at the new positions is written to v1 and its
+// largest magnitude to vmax_bits, for the next substep's length.
+__global__ void __launch_bounds__(kBlock) k_rk_substep(GridDev g, Vel V, int64_t np, float* __restrict__ pos,
+                                                       float* __restrict__ v1, int order, float step, float half,
+                                                       float three_q, float c1, float c2, float c3, int resample,
+                                                       uint32_t* vmax_bits) {
+    uint32_t m = 0;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x; p < np; p += stride) {
+        const float x0 = pos[3 * p], y0 = pos[3 * p + 1], z0 = pos[3 * p + 2];
+        const float ax = v1[3 * p], ay = v1[3 * p + 1], az = v1[3 * p + 2];
+        float x, y, z;
+        if (order == 1) {
+            x = __builtin_fmaf(step, ax, x0); y = __builtin_fmaf(step, ay, y0); z = __builtin_fmaf(step, az, z0);
+        } else if (order == 2) {
+            float bx, by, bz;
+            sample_vel(V, g, __builtin_fmaf(half, ax, x0), __builtin_fmaf(half, ay, y0), __builtin_fmaf(half, az, z0),
+                       bx, by, bz);
+            x = __builtin_fmaf(step, bx, x0); y = __builtin_fmaf(step, by, y0); z = __builtin_fmaf(step, bz, z0);
+        } else if (order == 3) {
+            float bx, by, bz, cx, cy, cz;
+            sample_vel(V, g, __builtin_fmaf(half, ax, x0), __builtin_fmaf(half, ay, y0), __builtin_fmaf(half, az, z0),
+                       bx, by, bz);
+            sample_vel(V, g, __builtin_fmaf(three_q, bx, x0), __builtin_fmaf(three_q, by, y0),
+                       __builtin_fmaf(three_q, bz, z0), cx, cy, cz);
+            x = __builtin_fmaf(c3, cx, __builtin_fmaf(c2, bx, __builtin_fmaf(c1, ax, x0)));
+            y = __builtin_fmaf(c3, cy, __builtin_fmaf(c2, by, __builtin_fmaf(c1, ay, y0)));
+            z = __builtin_fmaf(c3, cz, __builtin_fmaf(c2, bz, __builtin_fmaf(c1, az, z0)));
+        } else {
+            x = x0; y = y0; z = z0;
+        }
+        pos[3 * p] = x; pos[3 * p + 1] = y; pos[3 * p + 2] = z;
+        if (resample) {
+            float a, b, c;
+            sample_vel(V, g, x, y, z, a, b, c);
+            v1[3 * p] = a; v1[3 * p + 1] = b; v1[3 * p + 2] = c;
+            m = max(m, max(__float_as_uint(fabsf(a)), max(__float_as_uint(fabsf(b)), __float_as_uint(fabsf(c)))));
+        }
+    }
+    if (resample) block_max_atomic(m, vmax_bits);
+}
+
+// The velocity at each particle and its largest magnitude (the first
+// sample of a substep loop).
+__global__ void __launch_bounds__(kBlock) k_sample_absmax(GridDev g, Vel V, int64_t np, const float* __restrict__ pos,
+                                                          float* __restrict__ out, uint32_t* vmax_bits) {
+    uint32_t m = 0;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x; p < np; p += stride) {
+        float a, b, c;
+        sample_vel(V, g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], a, b, c);
+        out[3 * p] = a; out[3 * p + 1] = b; out[3 * p + 2] = c;
+        m = max(m, max(__float_as_uint(fabsf(a)), max(__float_as_uint(fabsf(b)), __float_as_uint(fabsf(c)))));
+    }
+    block_max_atomic(m, vmax_bits);
 }
 
 // statisticalFluidDensity (EulerParticlesTool.cu:27-46): +1 per particle.
@@ -355,9 +564,38 @@ hipError_t launch_p2g(hipStream_t s, const GridDev& g, int kind, int64_t np, con
     const size_t n = size_t(p.nx) * p.ny * p.nz;
     hipError_t e = launch_fill(s, field, n, 0.0f);
     if (e == hipSuccess) e = launch_fill(s, weight, n, 0.0f);
-    if (e == hipSuccess && np > 0)
-        e = LAUNCH(k_p2g, grid_for(size_t(np)), p, np, pos, val, span, off, alg, field, weight);
+    if (e == hipSuccess && np > 0) {
+        const P2GOut out{{field, field, field}, {weight, weight, weight}};
+        // table of 1024 slots for the 8-node supports, 2048 for the 64-node ones
+        if (alg == MGF_PG_LINEAR || alg == MGF_PG_P2GSUM)
+            e = LAUNCH((k_p2g_lds<1, 10>), grid_for(size_t(np)), p, p, p, np, pos, val, span, off, alg, out);
+        else
+            e = LAUNCH((k_p2g_lds<1, 11>), grid_for(size_t(np)), p, p, p, np, pos, val, span, off, alg, out);
+    }
     if (e == hipSuccess && normalize) e = LAUNCH(k_p2g_normalize, grid_for(n), n, field, weight);
+    return e;
+}
+
+hipError_t launch_p2g_velocity(hipStream_t s, const GridDev& g, int64_t np, const float* pos, const float* vel,
+                               int alg, float* const field[3], float* const weight[3]) {
+    PGrid p[3];
+    size_t n[3];
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < 3; ++c) {
+        p[c] = pgrid_of(g, c + 1);
+        n[c] = size_t(p[c].nx) * p[c].ny * p[c].nz;
+        if (e == hipSuccess) e = launch_fill(s, field[c], n[c], 0.0f);
+        if (e == hipSuccess) e = launch_fill(s, weight[c], n[c], 0.0f);
+    }
+    if (e == hipSuccess && np > 0) {
+        const P2GOut out{{field[0], field[1], field[2]}, {weight[0], weight[1], weight[2]}};
+        if (alg == MGF_PG_LINEAR || alg == MGF_PG_P2GSUM)
+            e = LAUNCH((k_p2g_lds<3, 11>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
+        else
+            e = LAUNCH((k_p2g_lds<3, 12>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
+    }
+    for (int c = 0; c < 3; ++c)
+        if (e == hipSuccess) e = LAUNCH(k_p2g_normalize, grid_for(n[c]), n[c], field[c], weight[c]);
     return e;
 }
 
@@ -438,6 +676,42 @@ hipError_t launch_particle_markers_u8(hipStream_t s, const GridDev& g, const uin
     if (e == hipSuccess && np > 0)
         e = LAUNCH(k_markers_particles, grid_for(size_t(np)), pgrid_of(g, 0), np, pos, markers);
     return e;
+}
+
+}  // namespace mgf
+
+namespace mgf {
+
+// Work-groups of the kernels that reduce a maximum: a multiple of the XCD
+// count, a few per CU, each looping over its share.
+constexpr uint32_t kMaxReduceBlocks = 4096;
+
+hipError_t launch_transfer_vel(hipStream_t s, const GridDev& g, int64_t np, const float* pos,
+                               const float* const in[3], const float* const delta[3], int kind, int alg, float keep,
+                               float mixing, float* vel) {
+    if (np <= 0) return hipSuccess;
+    const Faces3 fi{{in[0], in[1], in[2]}};
+    const Faces3 fd{{delta ? delta[0] : nullptr, delta ? delta[1] : nullptr, delta ? delta[2] : nullptr}};
+    return LAUNCH(k_transfer_vel, grid_for(size_t(np)), pgrid_of(g, 1), pgrid_of(g, 2), pgrid_of(g, 3), np, pos, fi,
+                  fd, kind, alg, keep, mixing, vel);
+}
+
+hipError_t launch_sample_absmax(hipStream_t s, const GridDev& g, Vel V, int64_t np, const float* pos, float* out,
+                                uint32_t* vmax_bits) {
+    hipError_t e = hipMemsetAsync(vmax_bits, 0, sizeof(uint32_t), s);
+    if (e != hipSuccess || np <= 0) return e;
+    return LAUNCH(k_sample_absmax, std::min<uint32_t>(grid_for(size_t(np)), kMaxReduceBlocks), g, V, np, pos, out,
+                  vmax_bits);
+}
+
+hipError_t launch_rk_substep(hipStream_t s, const GridDev& g, Vel V, int64_t np, float* pos, float* v1, int order,
+                             float step, float half, float three_q, float c1, float c2, float c3, int resample,
+                             uint32_t* vmax_bits) {
+    hipError_t e = hipSuccess;
+    if (resample) e = hipMemsetAsync(vmax_bits, 0, sizeof(uint32_t), s);
+    if (e != hipSuccess || np <= 0) return e;
+    return LAUNCH(k_rk_substep, std::min<uint32_t>(grid_for(size_t(np)), kMaxReduceBlocks), g, V, np, pos, v1, order,
+                  step, half, three_q, c1, c2, c3, resample, vmax_bits);
 }
 
 }  // namespace mgf

@@ -68,7 +68,7 @@ mgf_status particles_make(mgf_context ctx, int64_t n, const float* pos, const fl
         TRY(owned_alloc(p->allocations, &p->color, m));
         TRY(owned_alloc(p->allocations, &p->scalar, size_t(n)));
         TRY(owned_alloc(p->allocations, &p->vector, m));
-        TRY(owned_alloc(p->allocations, &p->scratch, 5 * m));
+        TRY(owned_alloc(p->allocations, &p->scratch, m));
         TRY(owned_alloc(p->allocations, &p->vmax, 64));
         hipStream_t st = ctx->stream;
         float* dst[3] = {p->pos, p->vel, p->color};
@@ -112,19 +112,18 @@ namespace mgf {
 // CEulerParticles::advectParticlesInVelField (EulerParticles.cpp:231-325):
 // substeps of cfl * min(h) / max|v| (the last two shortened to end at dt),
 // RK1/2/3 with the axpys of the positions, then the clamp to the grid box.
-// The substep length needs max|v| on the host, one read-back per substep.
+// The substep length needs max|v| on the host, one read-back per substep; a
+// substep is one fused kernel that also samples the next substep's velocity.
 mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t n, float* pos, float dt, float cfl,
                                 int order, int fix, float* scratch, uint32_t* vmax_bits) {
     if (n == 0) return MGF_OK;
-    const size_t m = 3 * size_t(n);
-    float *v1 = scratch, *v2 = scratch + m, *v3 = scratch + 2 * m, *mid = scratch + 3 * m, *tq = scratch + 4 * m;
+    float* v1 = scratch;
     const float minh = std::min(std::min(g.hx, g.hy), g.hz);
     float t = 0.0f;
     bool finished = false;
+    MGF_HIP(launch_sample_absmax(st, g, V, n, pos, v1, vmax_bits));
     for (int64_t sub = 0; !finished; ++sub) {
         if (sub > 10000000) return fail(MGF_ERR_NONFINITE, "particle advection did not reach dt");
-        MGF_HIP(launch_sample_particles(st, g, V, n, pos, v1));
-        MGF_HIP(launch_absmax(st, m, v1, vmax_bits));
         uint32_t bits = 0;
         MGF_HIP(hipMemcpyAsync(&bits, vmax_bits, sizeof bits, hipMemcpyDeviceToHost, st));
         MGF_HIP(hipStreamSynchronize(st));
@@ -139,28 +138,11 @@ mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t
         }
         if (!finished && !(step > 0.0f && std::isfinite(step)))
             return fail(MGF_ERR_NONFINITE, "particle velocity gives no finite substep");
-        if (order == 1) {
-            MGF_HIP(launch_axpy_n(st, m, step, v1, pos));
-        } else if (order == 2) {
-            const float half = float(0.5 * double(step));
-            MGF_HIP(hipMemcpyAsync(mid, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
-            MGF_HIP(launch_axpy_n(st, m, half, v1, mid));
-            MGF_HIP(launch_sample_particles(st, g, V, n, mid, v2));
-            MGF_HIP(launch_axpy_n(st, m, step, v2, pos));
-        } else if (order == 3) {
-            const float half = float(0.5 * double(step)), three_q = float(0.75 * double(step));
-            const float c1 = float(2.0 / 9.0 * double(step)), c2 = float(3.0 / 9.0 * double(step));
-            const float c3 = float(4.0 / 9.0 * double(step));
-            MGF_HIP(hipMemcpyAsync(mid, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
-            MGF_HIP(hipMemcpyAsync(tq, pos, m * sizeof(float), hipMemcpyDeviceToDevice, st));
-            MGF_HIP(launch_axpy_n(st, m, half, v1, mid));
-            MGF_HIP(launch_sample_particles(st, g, V, n, mid, v2));
-            MGF_HIP(launch_axpy_n(st, m, three_q, v2, tq));
-            MGF_HIP(launch_sample_particles(st, g, V, n, tq, v3));
-            MGF_HIP(launch_axpy_n(st, m, c1, v1, pos));
-            MGF_HIP(launch_axpy_n(st, m, c2, v2, pos));
-            MGF_HIP(launch_axpy_n(st, m, c3, v3, pos));
-        }
+        const float half = float(0.5 * double(step)), three_q = float(0.75 * double(step));
+        const float c1 = float(2.0 / 9.0 * double(step)), c2 = float(3.0 / 9.0 * double(step));
+        const float c3 = float(4.0 / 9.0 * double(step));
+        MGF_HIP(launch_rk_substep(st, g, V, n, pos, v1, order, step, half, three_q, c1, c2, c3, finished ? 0 : 1,
+                                  vmax_bits));
         t += step;
     }
     if (fix) MGF_HIP(launch_clamp_particles(st, g, n, pos));
@@ -173,34 +155,24 @@ mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in
     mgf_particles p = s->p;
     const GridDev g = make_grid(&s->grid);
     const int64_t n = p->n;
-    const size_t m = 3 * size_t(n);
-    if (s->kind == MGF_PSOLVER_PIC) {
-        // transferVelField2Particles (ParticleInCell.cpp:101)
-        for (int k = 0; k < 3; ++k) MGF_HIP(launch_g2p(st, g, k + 1, n, p->pos, in[k], s->alg, p->vel, 3, k));
-    } else {
-        // MIX: the PIC velocity first (MixPICAndFLIP.cpp:114)
-        if (s->kind == MGF_PSOLVER_MIX)
-            for (int k = 0; k < 3; ++k) MGF_HIP(launch_g2p(st, g, k + 1, n, p->pos, in[k], s->alg, s->picvel, 3, k));
-        // the grid change since the last output (FluidImplicitParticle.cpp:98-107)
+    // the grid change since the last output (FluidImplicitParticle.cpp:98-107)
+    if (s->kind != MGF_PSOLVER_PIC) {
         for (int k = 0; k < 3; ++k) {
             const size_t nf = pg_field_count(g, k + 1);
             if (s->vel_seeded) MGF_HIP(launch_flip_delta(st, nf, in[k], s->delta[k]));
             else MGF_HIP(hipMemcpyAsync(s->delta[k], in[k], nf * sizeof(float), hipMemcpyDeviceToDevice, st));
         }
         s->vel_seeded = true;
-        for (int k = 0; k < 3; ++k)
-            MGF_HIP(launch_g2p(st, g, k + 1, n, p->pos, s->delta[k], s->alg, s->dvel, 3, k));
-        MGF_HIP(launch_axpy_n(st, m, 1.0f, s->dvel, p->vel));
-        if (s->kind == MGF_PSOLVER_MIX) {
-            // scaleReal(vel, 1 - c); axpyReal(picvel, vel, c) (MixPICAndFLIP.cpp:128-129)
-            MGF_HIP(launch_scale_n(st, m, float(1.0 - double(s->mixing)), p->vel));
-            MGF_HIP(launch_axpy_n(st, m, s->mixing, s->picvel, p->vel));
-        }
     }
+    // the particle velocities: PIC transferVelField2Particles (ParticleInCell.cpp:101);
+    // FLIP + the grid change (FluidImplicitParticle.cpp:109-110); MIX blends the
+    // two (MixPICAndFLIP.cpp:114-129)
+    const float* const* delta = s->kind == MGF_PSOLVER_PIC ? nullptr : s->delta;
+    MGF_HIP(launch_transfer_vel(st, g, n, p->pos, in, delta, s->kind, s->alg, float(1.0 - double(s->mixing)),
+                                s->mixing, p->vel));
     TRY(particles_advect_run(st, g, vel3(vel), n, p->pos, dt, s->cfl, order, fix, p->scratch, p->vmax));
     // transferParticlesVel2Field, normalised by the weights
-    for (int k = 0; k < 3; ++k)
-        MGF_HIP(launch_p2g(st, g, k + 1, n, p->pos, p->vel, 3, k, s->alg, out[k], s->weight[k], 1));
+    MGF_HIP(launch_p2g_velocity(st, g, n, p->pos, p->vel, s->alg, out, s->weight));
     if (s->kind != MGF_PSOLVER_PIC)   // m_DeltaGridVelField = voOutputField
         for (int k = 0; k < 3; ++k)
             MGF_HIP(hipMemcpyAsync(s->delta[k], out[k], pg_field_count(g, k + 1) * sizeof(float),
@@ -358,14 +330,6 @@ mgf_status mgf_psolver_create(mgf_context ctx, const mgf_grid* grid, int32_t kin
         for (int k = 0; k < 3; ++k) {
             TRY(owned_alloc(s->allocations, &s->weight[k], pg_field_count(g, k + 1)));
             if (kind != MGF_PSOLVER_PIC) TRY(owned_alloc(s->allocations, &s->delta[k], pg_field_count(g, k + 1)));
-        }
-        if (kind != MGF_PSOLVER_PIC) {
-            TRY(owned_alloc(s->allocations, &s->dvel, 3 * size_t(particles->n)));
-            MGF_HIP(launch_fill(ctx->stream, s->dvel, 3 * size_t(particles->n), 0.0f));
-        }
-        if (kind == MGF_PSOLVER_MIX) {
-            TRY(owned_alloc(s->allocations, &s->picvel, 3 * size_t(particles->n)));
-            MGF_HIP(launch_fill(ctx->stream, s->picvel, 3 * size_t(particles->n), 0.0f));
         }
         MGF_HIP(hipStreamSynchronize(ctx->stream));
         return MGF_OK;

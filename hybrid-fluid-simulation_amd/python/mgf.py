@@ -153,6 +153,7 @@ _SIGS = {
     "mgf_psolver_advect_vector": [_VP, _VP, _VP, _VP, _VP, _VP, _VP],
     "mgf_scene_pressure_solve_particles": [_VP, _VP, C.POINTER(StepStats)],
     "mgf_step_particles": [_VP, _VP, C.c_int32, C.POINTER(StepStats)],
+    "mgf_scene_generate_particles": [_VP, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)],
     "mgf_op_scale": [_VP, C.c_int64, C.c_float, _VP],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
@@ -363,6 +364,12 @@ class Scene:
         if not (allow_unconverged and rc == ERR_NOT_CONVERGED):
             check(rc, "mgf_scene_pressure_solve_particles")
         return st.as_dict()
+
+    def generate_particles(self, per_cell, seed):
+        """Particles in the scene's fluid cells (outside its solids)."""
+        h = C.c_void_p()
+        check(lib().mgf_scene_generate_particles(self.h, per_cell, seed, C.byref(h)), "mgf_scene_generate_particles")
+        return Particles(self.ctx, handle=h)
 
     def step_particles(self, solver, n=1, allow_unconverged=False):
         st = StepStats()

@@ -509,6 +509,12 @@ MGF_API mgf_status mgf_psolver_advect_vector(mgf_psolver s, const float* in_x, c
  * instead of the fluid domain; the rest of the projection is the same. */
 MGF_API mgf_status mgf_scene_pressure_solve_particles(mgf_scene scene, mgf_particles particles,
         mgf_step_stats* stats);
+/* CGridFluidSolver::generateFluid's particle branch (GridFluidSolver.cpp:
+ * 93-158): mgf_particles_generate on the scene's fluid domain and solid marker
+ * (the marker of solids added since the last step is built first, at their
+ * current translations). */
+MGF_API mgf_status mgf_scene_generate_particles(mgf_scene scene, int32_t per_cell, uint64_t seed,
+        mgf_particles* out);
 /* CGridFluidSolver::update() with a particle advection solver
  * (GridFluidSolver.cpp:336-370, 399-416, 434-525): boundaries, then the
  * velocity, the colour (when present) and the fluid domain through the

@@ -266,3 +266,22 @@ def test_step_particles_matches_oracle(ctx, orc, kind):
         vo = np.concatenate([o.download(f) for f in (0, 1, 2)])
         assert rel_l2(vg, vo) <= 1e-4, (kind, step, rel_l2(vg, vo))
     g.close(); P.close(); ops.close(); s.close(); o.close()
+
+
+def test_scene_generate_particles_matches_oracle(ctx, orc):
+    """generateFluid's particle branch: the scene's fluid domain and the marker
+    of its solids (built before the first step) give the oracle's particles."""
+    spec = scenes.obstacles(24, seed=3, nspheres=2)
+    s = scenes.build_gpu_scene(mgf, ctx, spec, **plain_desc())
+    o = scenes.build_oracle_scene(orc, spec)
+    o.update_boundaries(orc.params(spec.dt))       # static solids: the marker only
+    P = s.generate_particles(3, seed=41)
+    exp = orc.generate_particles(spec.res, spec.origin, spec.spacing, o.download(3), o.download(4), 3, seed=41)
+    assert len(P) == len(exp) > 0
+    assert_bitexact(P.get("pos"), exp, "scene particles")
+    # the scene's own step still starts from the unmoved solids
+    g = mgf.ParticleSolver(ctx, spec.res, spec.origin, spec.spacing, "pic", P)
+    st = s.step_particles(g, 1)
+    assert st["converged"] == 1
+    assert_bitexact(s.download(4), o.download(4), "solid marker after the step")
+    g.close(); P.close(); s.close(); o.close()
