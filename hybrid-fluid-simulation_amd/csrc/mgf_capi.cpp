@@ -768,9 +768,9 @@ mgf_status step_particles_once(mgf_scene s, mgf_psolver ps, mgf_step_stats* acc)
     if (s->color[0][0]) {
         const float* ic[3] = {s->color[c][0], s->color[c][1], s->color[c][2]};
         float* oc[3] = {s->color[n][0], s->color[n][1], s->color[n][2]};
-        TRY(psolver_vector(ps, st, ic, oc));
+        TRY(psolver_vector(ps, st, ic, oc, true));
     }
-    TRY(psolver_scalar(ps, st, s->phi[c], s->phi[n]));
+    TRY(psolver_scalar(ps, st, s->phi[c], s->phi[n], true));
     s->cur = n;
     TRY(apply_forces(s));
     if (t) { TRY(sync(s)); t1 = now_ms(); }

@@ -147,6 +147,7 @@ struct mgf_particles_s {
     float* vector = nullptr;   // 3n
     float* scratch = nullptr;  // 3n: the velocities sampled at the particles
     uint32_t* vmax = nullptr;  // max |v| bits
+    uint64_t pos_version = 0;  // bumped whenever the positions change
     std::vector<void*> allocations;
 };
 
@@ -159,6 +160,11 @@ struct mgf_psolver_s {
     float cfl = 1.0f, mixing = 0.1f;
     float* delta[3] = {nullptr, nullptr, nullptr};   // previous output velocity / grid change (FLIP, MIX)
     float* weight[3] = {nullptr, nullptr, nullptr};  // weight fields (face extents; cells fit in them)
+    // the particles binned by support corner for the splat (valid for bins_version)
+    uint32_t *bin_counts = nullptr, *bin_tiles = nullptr, *bin_start = nullptr, *bin_cursor = nullptr;
+    uint32_t* order = nullptr;
+    float* acc = nullptr;                            // the splat's per-bin slots (LINEAR)
+    uint64_t bins_version = ~uint64_t(0);
     bool scalar_seeded = false, vector_seeded = false, vel_seeded = false;
     std::vector<void*> allocations;
 };
@@ -338,6 +344,19 @@ hipError_t launch_gen_count(hipStream_t s, size_t ncell, const float* fluid, con
 hipError_t launch_gen_emit(hipStream_t s, const GridDev& g, const float* fluid, const float* solid,
                            const uint32_t* tiles, int per_cell, uint64_t seed, float* pos);
 hipError_t launch_flip_delta(hipStream_t s, size_t n, const float* in, float* delta);
+// Particles counting-sorted by the lower corner of their support on the cell
+// grid (bin_count(g) bins; counts, cursor: that many u32, start one more,
+// tiles bin_tiles(g) + 1, order np), then the LINEAR splat onto grid `kind`
+// in two atomic-free passes through acc (p2g_acc_floats(g) floats); every
+// node of the field and weight is written.
+size_t bin_count(const GridDev& g);
+uint32_t bin_tiles(const GridDev& g);
+size_t p2g_acc_floats(const GridDev& g);
+hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, const float* pos, uint32_t* counts,
+                                uint32_t* tiles, uint32_t* start, uint32_t* cursor, uint32_t* order);
+hipError_t launch_p2g_binned(hipStream_t s, const GridDev& g, int kind, const uint32_t* start, const uint32_t* order,
+                             const float* pos, const float* val, int span, int off, float* acc, float* field,
+                             float* weight, int normalize);
 // Fused particle kernels: the velocity splat of the three face grids; the
 // particle-velocity update of PIC / FLIP / MIX (delta = NULL for PIC); the
 // first velocity sample of a substep loop with its max |v|; one RK substep
@@ -360,8 +379,11 @@ mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t
                                 int order, int fix, float* scratch, uint32_t* vmax);
 mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in[3], const float* const vel[3],
                             float dt, int order, int fix, float* const out[3]);
-mgf_status psolver_scalar(mgf_psolver s, hipStream_t st, const float* in, float* out);
-mgf_status psolver_vector(mgf_psolver s, hipStream_t st, const float* const in[3], float* const out[3]);
+// reuse_bins: the particles have not moved since the solver's last splat
+// (the fields of one library step).
+mgf_status psolver_scalar(mgf_psolver s, hipStream_t st, const float* in, float* out, bool reuse_bins);
+mgf_status psolver_vector(mgf_psolver s, hipStream_t st, const float* const in[3], float* const out[3],
+                          bool reuse_bins);
 // Boundaries.
 hipError_t launch_resample_solid(hipStream_t s, const GridDev& g, const float* sdf0,
                                  const float t[3], float* out);

@@ -20,6 +20,8 @@ namespace mgf {
 struct PGrid {
     int nx, ny, nz;
     float ox, oy, oz, hx, hy, hz;
+    float rhx, rhy, rhz;   // 1/h, used when every spacing is a power of two (pow2)
+    int pow2;
 };
 
 // Transfer kernels (MPhysics/EulerMathTool.cu:268-294): written with double
@@ -44,9 +46,9 @@ __device__ __forceinline__ float pg_cubic(float v) {
 
 // transPos2Index (EulerMathTool.cu:29-40): truncation of (p - o) / h, clamped.
 __device__ __forceinline__ uint32_t pos_cell(const PGrid& g, float px, float py, float pz) {
-    const int ix = clampi(int((px - g.ox) / g.hx), g.nx);
-    const int iy = clampi(int((py - g.oy) / g.hy), g.ny);
-    const int iz = clampi(int((pz - g.oz) / g.hz), g.nz);
+    const int ix = clampi(int(div_h(px - g.ox, g.hx, g.rhx, g.pow2)), g.nx);
+    const int iy = clampi(int(div_h(py - g.oy, g.hy, g.rhy, g.pow2)), g.ny);
+    const int iz = clampi(int(div_h(pz - g.oz, g.hz, g.rhz, g.pow2)), g.nz);
     return (uint32_t(iz) * uint32_t(g.ny) + uint32_t(iy)) * uint32_t(g.nx) + uint32_t(ix);
 }
 
@@ -56,54 +58,52 @@ struct PStencil {
     int i0[3];
     float w[3][4];
 };
-__device__ __forceinline__ void pg_stencil(const PGrid& g, float px, float py, float pz, int alg, PStencil& st) {
-    const float rel[3] = {px - g.ox - g.hx * 0.5f, py - g.oy - g.hy * 0.5f, pz - g.oz - g.hz * 0.5f};
-    const float h[3] = {g.hx, g.hy, g.hz};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        st.i0[k] = floor_ref(rel[k] / h[k]);
-        const float f = (rel[k] - float(st.i0[k]) * h[k]) / h[k];
-        if (alg == MGF_PG_LINEAR) {
-            st.w[k][0] = pg_linear(f);
-            st.w[k][1] = pg_linear(float(double(f) - 1.0));
-            st.w[k][2] = st.w[k][3] = 0.0f;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float a = float(double(f) + 1.0 - double(q));
-                st.w[k][q] = alg == MGF_PG_QUADRATIC ? pg_quadratic(a) : pg_cubic(a);
-            }
-        }
+// One axis of the support: lower corner and the kernel's weights.
+__device__ __forceinline__ void pg_axis(float rel, float h, float rh, int pow2, int alg, int& i0, float (&w)[4]) {
+    i0 = floor_ref(div_h(rel, h, rh, pow2));
+    const float f = div_h(rel - float(i0) * h, h, rh, pow2);
+    if (alg == MGF_PG_LINEAR) {
+        w[0] = pg_linear(f);
+        w[1] = pg_linear(float(double(f) - 1.0));
+        w[2] = w[3] = 0.0f;
+    } else if (alg == MGF_PG_QUADRATIC) {
+        w[0] = pg_quadratic(float(double(f) + 1.0));
+        w[1] = pg_quadratic(float(double(f) + 1.0 - 1.0));
+        w[2] = pg_quadratic(float(double(f) + 1.0 - 2.0));
+        w[3] = pg_quadratic(float(double(f) + 1.0 - 3.0));
+    } else {
+        w[0] = pg_cubic(float(double(f) + 1.0));
+        w[1] = pg_cubic(float(double(f) + 1.0 - 1.0));
+        w[2] = pg_cubic(float(double(f) + 1.0 - 2.0));
+        w[3] = pg_cubic(float(double(f) + 1.0 - 3.0));
     }
+}
+
+__device__ __forceinline__ void pg_stencil(const PGrid& g, float px, float py, float pz, int alg, PStencil& st) {
+    pg_axis(px - g.ox - g.hx * 0.5f, g.hx, g.rhx, g.pow2, alg, st.i0[0], st.w[0]);
+    pg_axis(py - g.oy - g.hy * 0.5f, g.hy, g.rhy, g.pow2, alg, st.i0[1], st.w[1]);
+    pg_axis(pz - g.oz - g.hz * 0.5f, g.hz, g.rhz, g.pow2, alg, st.i0[2], st.w[2]);
 }
 
 __device__ __forceinline__ uint32_t pg_index(const PGrid& g, int x, int y, int z) {
     return lin3(x, y, z, g.nx, g.ny, g.nz);
 }
 
-// accumulateParticles2CCSField (EulerParticlesTool.cu:79-196).
-__global__ void __launch_bounds__(kBlock) k_p2g(PGrid g, int64_t np, const float* __restrict__ pos,
-                                                const float* __restrict__ val, int span, int off, int alg,
-                                                float* field, float* weight) {
-    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
-    if (p >= np) return;
-    const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
-    const float v = val[int64_t(span) * p + off];
-    if (alg == MGF_PG_P2GSUM) {
-        atomicAdd(&field[pos_cell(g, px, py, pz)], v);
-        return;
-    }
-    PStencil st;
-    pg_stencil(g, px, py, pz, alg, st);
-    const int n = alg == MGF_PG_LINEAR ? 2 : 4, lo = alg == MGF_PG_LINEAR ? 0 : -1;
-    for (int z = 0; z < n; ++z)
-        for (int y = 0; y < n; ++y)
-            for (int x = 0; x < n; ++x) {
-                const float wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
-                const uint32_t c = pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo);
-                atomicAdd(&field[c], wt * v);
-                atomicAdd(&weight[c], wt);
-            }
+// The corners of a support in the reference's order (z, y, x), with their
+// weights w_x * w_y * w_z; N = 2 (LINEAR, corners i0 .. i0+1) or 4
+// (QUADRATIC / CUBIC, corners i0-1 .. i0+2), unrolled so the weights stay in
+// registers.
+template <int N, class F>
+__device__ __forceinline__ void for_corners(const PGrid& g, const PStencil& st, F&& f) {
+    constexpr int lo = N == 2 ? 0 : -1;
+#pragma unroll
+    for (int z = 0; z < N; ++z)
+#pragma unroll
+        for (int y = 0; y < N; ++y)
+#pragma unroll
+            for (int x = 0; x < N; ++x)
+                f(pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo),
+                  st.w[0][x] * st.w[1][y] * st.w[2][z]);
 }
 
 // The same sums with a work-group's particles first merged in an LDS hash
@@ -143,7 +143,26 @@ __device__ __forceinline__ void lds_accumulate(uint32_t* keys, float* sv, float*
     if (with_w) atomicAdd(gw, b);
 }
 
-template <int NC, int HB>
+// One particle's value on one grid (component c) into the table.
+template <int HB, int N>
+__device__ __forceinline__ void p2g_one(uint32_t* keys, float* sv, float* sw, const PGrid& g, int c, float px,
+                                        float py, float pz, float v, int alg, const P2GOut& out) {
+    const uint32_t tag = uint32_t(c) << 30;
+    if (alg == MGF_PG_P2GSUM) {
+        const uint32_t node = pos_cell(g, px, py, pz);
+        lds_accumulate<HB>(keys, sv, sw, tag | node, v, 0.0f, false, &out.field[c][node], nullptr);
+        return;
+    }
+    PStencil st;
+    pg_stencil(g, px, py, pz, alg, st);
+    auto add = [&](uint32_t node, float wt) {
+        lds_accumulate<HB>(keys, sv, sw, tag | node, wt * v, wt, true, &out.field[c][node], &out.weight[c][node]);
+    };
+    for_corners<N>(g, st, add);
+}
+
+// N = 2: the P2GSUM and LINEAR kernels; N = 4: QUADRATIC and CUBIC.
+template <int NC, int HB, int N>
 __global__ void __launch_bounds__(kBlock) k_p2g_lds(PGrid g0, PGrid g1, PGrid g2, int64_t np,
                                                     const float* __restrict__ pos, const float* __restrict__ val,
                                                     int span, int off, int alg, P2GOut out) {
@@ -160,27 +179,10 @@ __global__ void __launch_bounds__(kBlock) k_p2g_lds(PGrid g0, PGrid g1, PGrid g2
     const bool with_w = alg != MGF_PG_P2GSUM;
     if (p < np) {
         const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const PGrid& g = c == 0 ? g0 : (c == 1 ? g1 : g2);
-            const float v = val[int64_t(span) * p + (NC == 1 ? off : c)];
-            const uint32_t tag = uint32_t(c) << 30;
-            if (alg == MGF_PG_P2GSUM) {
-                const uint32_t node = pos_cell(g, px, py, pz);
-                lds_accumulate<HB>(keys, sv, sw, tag | node, v, 0.0f, false, &out.field[c][node], nullptr);
-                continue;
-            }
-            PStencil st;
-            pg_stencil(g, px, py, pz, alg, st);
-            const int n = alg == MGF_PG_LINEAR ? 2 : 4, lo = alg == MGF_PG_LINEAR ? 0 : -1;
-            for (int z = 0; z < n; ++z)
-                for (int y = 0; y < n; ++y)
-                    for (int x = 0; x < n; ++x) {
-                        const float wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
-                        const uint32_t node = pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo);
-                        lds_accumulate<HB>(keys, sv, sw, tag | node, wt * v, wt, true, &out.field[c][node],
-                                           &out.weight[c][node]);
-                    }
+        p2g_one<HB, N>(keys, sv, sw, g0, 0, px, py, pz, val[int64_t(span) * p + (NC == 1 ? off : 0)], alg, out);
+        if (NC == 3) {
+            p2g_one<HB, N>(keys, sv, sw, g1, 1, px, py, pz, val[int64_t(span) * p + 1], alg, out);
+            p2g_one<HB, N>(keys, sv, sw, g2, 2, px, py, pz, val[int64_t(span) * p + 2], alg, out);
         }
     }
     __syncthreads();
@@ -200,47 +202,33 @@ __global__ void __launch_bounds__(kBlock) k_p2g_normalize(size_t n, float* __res
     if (i < n && weight[i] > 0.0f) field[i] /= weight[i];
 }
 
+// The value of one field at a particle: summed from 0 in corner order
+// z, y, x (G2PNEAREST: the value of its cell).
+template <int N>
+__device__ __forceinline__ float g2p_one(const PGrid& g, float px, float py, float pz, const float* __restrict__ f,
+                                         int alg) {
+    if (alg == MGF_PG_G2PNEAREST) return f[pos_cell(g, px, py, pz)];
+    PStencil st;
+    pg_stencil(g, px, py, pz, alg, st);
+    float acc = 0.0f;
+    for_corners<N>(g, st, [&](uint32_t node, float wt) { acc += wt * f[node]; });
+    return acc;
+}
+
 // accumulateCCSField2Particles (EulerParticlesTool.cu:198-288): summed from
 // 0 in corner order z, y, x.
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_g2p(PGrid g, int64_t np, const float* __restrict__ pos,
                                                 const float* __restrict__ field, int alg, float* val, int span,
                                                 int off) {
     const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
     if (p >= np) return;
     const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
-    float acc = 0.0f;
-    if (alg == MGF_PG_G2PNEAREST) {
-        acc = field[pos_cell(g, px, py, pz)];
-    } else if (alg == MGF_PG_LINEAR || alg == MGF_PG_QUADRATIC || alg == MGF_PG_CUBIC) {
-        PStencil st;
-        pg_stencil(g, px, py, pz, alg, st);
-        const int n = alg == MGF_PG_LINEAR ? 2 : 4, lo = alg == MGF_PG_LINEAR ? 0 : -1;
-        for (int z = 0; z < n; ++z)
-            for (int y = 0; y < n; ++y)
-                for (int x = 0; x < n; ++x) {
-                    const float wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
-                    acc += wt * field[pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo)];
-                }
-    }
-    val[int64_t(span) * p + off] = acc;
+    val[int64_t(span) * p + off] =
+        (alg == MGF_PG_G2PNEAREST || alg == MGF_PG_LINEAR || alg == MGF_PG_QUADRATIC || alg == MGF_PG_CUBIC)
+            ? g2p_one<N>(g, px, py, pz, field, alg) : 0.0f;
 }
 
-// The value of one field at a particle (the body of k_g2p).
-__device__ __forceinline__ float g2p_one(const PGrid& g, float px, float py, float pz, const float* __restrict__ f,
-                                         int alg) {
-    if (alg == MGF_PG_G2PNEAREST) return f[pos_cell(g, px, py, pz)];
-    PStencil st;
-    pg_stencil(g, px, py, pz, alg, st);
-    const int n = alg == MGF_PG_LINEAR ? 2 : 4, lo = alg == MGF_PG_LINEAR ? 0 : -1;
-    float acc = 0.0f;
-    for (int z = 0; z < n; ++z)
-        for (int y = 0; y < n; ++y)
-            for (int x = 0; x < n; ++x) {
-                const float wt = st.w[0][x] * st.w[1][y] * st.w[2][z];
-                acc += wt * f[pg_index(g, st.i0[0] + x + lo, st.i0[1] + y + lo, st.i0[2] + z + lo)];
-            }
-    return acc;
-}
 
 // The particle-velocity update of the three solvers in one pass, with the
 // operations and rounding of the reference's separate calls:
@@ -251,6 +239,21 @@ __device__ __forceinline__ float g2p_one(const PGrid& g, float px, float py, flo
 struct Faces3 {
     const float* f[3];
 };
+template <int N>
+__device__ __forceinline__ void transfer_one(const PGrid& g, float px, float py, float pz, const float* fin,
+                                             const float* fdelta, int kind, int alg, float keep, float mixing,
+                                             float* v) {
+    if (kind == MGF_PSOLVER_PIC) {
+        *v = g2p_one<N>(g, px, py, pz, fin, alg);
+        return;
+    }
+    const float pic = kind == MGF_PSOLVER_MIX ? g2p_one<N>(g, px, py, pz, fin, alg) : 0.0f;
+    float u = __builtin_fmaf(1.0f, g2p_one<N>(g, px, py, pz, fdelta, alg), *v);
+    if (kind == MGF_PSOLVER_MIX) u = __builtin_fmaf(mixing, pic, keep * u);
+    *v = u;
+}
+
+template <int N>
 __global__ void __launch_bounds__(kBlock) k_transfer_vel(PGrid g0, PGrid g1, PGrid g2, int64_t np,
                                                          const float* __restrict__ pos, Faces3 in, Faces3 delta,
                                                          int kind, int alg, float keep, float mixing,
@@ -258,18 +261,9 @@ __global__ void __launch_bounds__(kBlock) k_transfer_vel(PGrid g0, PGrid g1, PGr
     const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
     if (p >= np) return;
     const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const PGrid& g = c == 0 ? g0 : (c == 1 ? g1 : g2);
-        if (kind == MGF_PSOLVER_PIC) {
-            vel[3 * p + c] = g2p_one(g, px, py, pz, in.f[c], alg);
-            continue;
-        }
-        const float pic = kind == MGF_PSOLVER_MIX ? g2p_one(g, px, py, pz, in.f[c], alg) : 0.0f;
-        float v = __builtin_fmaf(1.0f, g2p_one(g, px, py, pz, delta.f[c], alg), vel[3 * p + c]);
-        if (kind == MGF_PSOLVER_MIX) v = __builtin_fmaf(mixing, pic, keep * v);
-        vel[3 * p + c] = v;
-    }
+    transfer_one<N>(g0, px, py, pz, in.f[0], delta.f[0], kind, alg, keep, mixing, &vel[3 * p]);
+    transfer_one<N>(g1, px, py, pz, in.f[1], delta.f[1], kind, alg, keep, mixing, &vel[3 * p + 1]);
+    transfer_one<N>(g2, px, py, pz, in.f[2], delta.f[2], kind, alg, keep, mixing, &vel[3 * p + 2]);
 }
 
 // max over the work-group, then one atomic per work-group.
@@ -544,9 +538,226 @@ __global__ void __launch_bounds__(kBlock) k_markers_particles(PGrid g, int64_t n
     if (markers[c] != 2) markers[c] = 1;
 }
 
+// ---- the particle-to-grid splat as a gather over binned particles ---------
+// Particles are counting-sorted by the lower corner s of their support on the
+// cell grid (s = floor((p - o)/h - 1/2) per axis, clamped to [-1, n-1]: bins
+// (nx+1)(ny+1)(nz+1)).  A node of a target grid (the cells, or the faces of
+// one axis) then sums, with no atomics, over the particles of the bins whose
+// supports can reach it: on the target grid a node i receives from supports
+// with lower corner in [i - lo - n + 1, i - lo] (lo, n: the kernel's corner
+// offset and width), and on the axis the target grid is shifted along, that
+// corner is the cell grid's or the next one (the same floor taken half a cell
+// further), so one more bin is visited there.  Each visited particle's
+// support on the target grid is evaluated as in the per-particle splat; the
+// corners that clamp onto the node add their weights.
+struct BinGrid {
+    int nx, ny, nz;   // cells of the grid the bins follow
+};
+
+__device__ __forceinline__ uint32_t bin_of(const PGrid& c, float px, float py, float pz) {
+    const float rel[3] = {px - c.ox - c.hx * 0.5f, py - c.oy - c.hy * 0.5f, pz - c.oz - c.hz * 0.5f};
+    const float h[3] = {c.hx, c.hy, c.hz};
+    const float rh[3] = {c.rhx, c.rhy, c.rhz};
+    const int n[3] = {c.nx, c.ny, c.nz};
+    uint32_t b[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int s = floor_ref(div_h(rel[k], h[k], rh[k], c.pow2));
+        s = s < -1 ? -1 : (s > n[k] - 1 ? n[k] - 1 : s);
+        b[k] = uint32_t(s + 1);
+    }
+    return (b[2] * uint32_t(c.ny + 1) + b[1]) * uint32_t(c.nx + 1) + b[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_count(PGrid c, int64_t np, const float* __restrict__ pos,
+                                                      uint32_t* counts) {
+    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    if (p >= np) return;
+    atomicAdd(&counts[bin_of(c, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2])], 1u);
+}
+
+// Tile sums of a u32 array (kGenTile entries per work-group).
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(size_t n, const uint32_t* __restrict__ a,
+                                                       uint32_t* __restrict__ tiles) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const size_t base = size_t(blockIdx.x) * kGenTile + size_t(threadIdx.x) * kGenRows;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kGenRows; ++j)
+        if (base + j < n) sum += a[base + j];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += uint32_t(__shfl_xor(int(sum), o, 64));
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int q = 0; q < kBlock / 64; ++q) t += wsum[q];
+        tiles[blockIdx.x] = t;
+    }
+}
+
+// start[i] = exclusive prefix of a (tile offsets already scanned), and a copy
+// in cursor; start[n] = the total.
+__global__ void __launch_bounds__(kBlock) k_scan_apply(size_t n, const uint32_t* __restrict__ a,
+                                                       const uint32_t* __restrict__ tiles, uint32_t nt,
+                                                       uint32_t* __restrict__ start, uint32_t* __restrict__ cursor) {
+    __shared__ uint32_t part[kBlock];
+    const size_t base = size_t(blockIdx.x) * kGenTile + size_t(threadIdx.x) * kGenRows;
+    uint32_t v[kGenRows], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kGenRows; ++j) {
+        v[j] = base + j < n ? a[base + j] : 0u;
+        sum += v[j];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {   // inclusive scan of the work-items' sums
+        const uint32_t t = threadIdx.x >= uint32_t(o) ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = tiles[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0u);
+#pragma unroll
+    for (int j = 0; j < kGenRows; ++j) {
+        if (base + j < n) {
+            start[base + j] = run;
+            cursor[base + j] = run;
+        }
+        run += v[j];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = tiles[nt];
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin_scatter(PGrid c, int64_t np, const float* __restrict__ pos,
+                                                        uint32_t* cursor, uint32_t* __restrict__ order) {
+    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const uint32_t slot = atomicAdd(&cursor[bin_of(c, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2])], 1u);
+    order[slot] = uint32_t(p);
+}
+
+// Two passes per target grid (the cells, or the faces of one axis; LINEAR
+// kernel).  Pass 1, one work-item per bin: every particle of the bin adds
+// its weights, and weight times value, to slots indexed by the target node
+// relative to the bin's first node clamp(s) (two slots per axis, three on the
+// axis the target grid is shifted along: its support starts at the cell
+// grid's corner or the next).  A support's corners that clamp onto one node
+// share a slot, so an edge bin collects far-outside particles too.  Pass 2,
+// one work-item per node: the sum, in a fixed bin order, of the slots of the
+// neighbouring bins that point at it; then the normalisation.
+constexpr int kSlotsMax = 12;
+
+__device__ __forceinline__ int clamp_bin(int s, int cells) { return s < -1 ? -1 : (s > cells - 1 ? cells - 1 : s); }
+
+__global__ void __launch_bounds__(kBlock) k_p2g_bins(PGrid g, int shift, BinGrid bg, const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ order, const float* __restrict__ pos,
+                                                     const float* __restrict__ val, int span, int off,
+                                                     float* __restrict__ acc) {
+    const size_t nb = size_t(bg.nx + 1) * (bg.ny + 1) * (bg.nz + 1);
+    const size_t b = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (b >= nb) return;
+    const int s[3] = {int(b % uint32_t(bg.nx + 1)) - 1, int((b / uint32_t(bg.nx + 1)) % uint32_t(bg.ny + 1)) - 1,
+                      int(b / (size_t(bg.nx + 1) * uint32_t(bg.ny + 1))) - 1};
+    const int len[3] = {g.nx, g.ny, g.nz};
+    int base[3], ns[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        base[a] = clampi(s[a], len[a]);
+        ns[a] = a == shift ? 3 : 2;
+    }
+    float sv[kSlotsMax], sw[kSlotsMax];
+#pragma unroll
+    for (int q = 0; q < kSlotsMax; ++q) sv[q] = sw[q] = 0.0f;
+    const uint32_t q0 = start[b], q1 = start[b + 1];
+    for (uint32_t q = q0; q < q1; ++q) {
+        const uint32_t p = order[q];
+        PStencil st;
+        pg_stencil(g, pos[3 * size_t(p)], pos[3 * size_t(p) + 1], pos[3 * size_t(p) + 2], MGF_PG_LINEAR, st);
+        const float v = val[size_t(span) * p + off];
+        // per-axis slot weights: the corners that clamp onto slot t
+        float W[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) W[a][t] = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int t = clampi(st.i0[a] + c, len[a]) - base[a];
+#pragma unroll
+                for (int u = 0; u < 3; ++u)
+                    if (t == u) W[a][u] += st.w[a][c];
+            }
+        }
+#pragma unroll
+        for (int tz = 0; tz < 3; ++tz)
+#pragma unroll
+            for (int ty = 0; ty < 3; ++ty)
+#pragma unroll
+                for (int tx = 0; tx < 3; ++tx) {
+                    if (tx >= ns[0] || ty >= ns[1] || tz >= ns[2]) continue;
+                    const int slot = (tz * ns[1] + ty) * ns[0] + tx;
+                    const float wt = W[0][tx] * W[1][ty] * W[2][tz];
+#pragma unroll
+                    for (int u = 0; u < kSlotsMax; ++u)
+                        if (u == slot) {
+                            sv[u] += wt * v;
+                            sw[u] += wt;
+                        }
+                }
+    }
+    const int nslot = ns[0] * ns[1] * ns[2];
+#pragma unroll
+    for (int u = 0; u < kSlotsMax; ++u)
+        if (u < nslot) {
+            acc[(2 * size_t(u)) * nb + b] = sv[u];
+            acc[(2 * size_t(u) + 1) * nb + b] = sw[u];
+        }
+}
+
+__global__ void __launch_bounds__(kBlock) k_p2g_nodes(PGrid g, int shift, BinGrid bg, const float* __restrict__ acc,
+                                                      float* __restrict__ field, float* __restrict__ weight,
+                                                      int normalize) {
+    const size_t total = size_t(g.nx) * g.ny * g.nz;
+    const size_t node = size_t(xcd_block()) * kBlock + threadIdx.x;
+    if (node >= total) return;
+    const size_t nb = size_t(bg.nx + 1) * (bg.ny + 1) * (bg.nz + 1);
+    const int idx[3] = {int(node % uint32_t(g.nx)), int((node / uint32_t(g.nx)) % uint32_t(g.ny)),
+                        int(node / (size_t(g.nx) * uint32_t(g.ny)))};
+    const int len[3] = {g.nx, g.ny, g.nz}, cells[3] = {bg.nx, bg.ny, bg.nz};
+    int ns[3], s0[3], s1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        ns[a] = a == shift ? 3 : 2;
+        s0[a] = clamp_bin(idx[a] - ns[a], cells[a]);
+        s1[a] = clamp_bin(idx[a], cells[a]);
+    }
+    float sv = 0.0f, sw = 0.0f;
+    for (int bz = s0[2]; bz <= s1[2]; ++bz) {
+        const int tz = idx[2] - clampi(bz, len[2]);
+        if (tz < 0 || tz >= ns[2]) continue;
+        for (int by = s0[1]; by <= s1[1]; ++by) {
+            const int ty = idx[1] - clampi(by, len[1]);
+            if (ty < 0 || ty >= ns[1]) continue;
+            for (int bx = s0[0]; bx <= s1[0]; ++bx) {
+                const int tx = idx[0] - clampi(bx, len[0]);
+                if (tx < 0 || tx >= ns[0]) continue;
+                const size_t b = (size_t(bz + 1) * uint32_t(bg.ny + 1) + uint32_t(by + 1)) * uint32_t(bg.nx + 1) +
+                                 uint32_t(bx + 1);
+                const size_t slot = size_t((tz * ns[1] + ty) * ns[0] + tx);
+                sv += acc[2 * slot * nb + b];
+                sw += acc[(2 * slot + 1) * nb + b];
+            }
+        }
+    }
+    field[node] = normalize && sw > 0.0f ? sv / sw : sv;
+    weight[node] = sw;
+}
+
 // ===========================================================================
 static PGrid pgrid_of(const GridDev& g, int kind) {
-    PGrid p{g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz};
+    PGrid p{g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g.hx, g.hy, g.hz, g.rhx, g.rhy, g.rhz, g.pow2};
     if (kind == 1) { p.nx += 1; p.ox = g.ox - g.hx / 2; }
     if (kind == 2) { p.ny += 1; p.oy = g.oy - g.hy / 2; }
     if (kind == 3) { p.nz += 1; p.oz = g.oz - g.hz / 2; }
@@ -568,9 +779,9 @@ hipError_t launch_p2g(hipStream_t s, const GridDev& g, int kind, int64_t np, con
         const P2GOut out{{field, field, field}, {weight, weight, weight}};
         // table of 1024 slots for the 8-node supports, 2048 for the 64-node ones
         if (alg == MGF_PG_LINEAR || alg == MGF_PG_P2GSUM)
-            e = LAUNCH((k_p2g_lds<1, 10>), grid_for(size_t(np)), p, p, p, np, pos, val, span, off, alg, out);
+            e = LAUNCH((k_p2g_lds<1, 10, 2>), grid_for(size_t(np)), p, p, p, np, pos, val, span, off, alg, out);
         else
-            e = LAUNCH((k_p2g_lds<1, 11>), grid_for(size_t(np)), p, p, p, np, pos, val, span, off, alg, out);
+            e = LAUNCH((k_p2g_lds<1, 11, 4>), grid_for(size_t(np)), p, p, p, np, pos, val, span, off, alg, out);
     }
     if (e == hipSuccess && normalize) e = LAUNCH(k_p2g_normalize, grid_for(n), n, field, weight);
     return e;
@@ -590,9 +801,9 @@ hipError_t launch_p2g_velocity(hipStream_t s, const GridDev& g, int64_t np, cons
     if (e == hipSuccess && np > 0) {
         const P2GOut out{{field[0], field[1], field[2]}, {weight[0], weight[1], weight[2]}};
         if (alg == MGF_PG_LINEAR || alg == MGF_PG_P2GSUM)
-            e = LAUNCH((k_p2g_lds<3, 11>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
+            e = LAUNCH((k_p2g_lds<3, 11, 2>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
         else
-            e = LAUNCH((k_p2g_lds<3, 12>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
+            e = LAUNCH((k_p2g_lds<3, 12, 4>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
     }
     for (int c = 0; c < 3; ++c)
         if (e == hipSuccess) e = LAUNCH(k_p2g_normalize, grid_for(n[c]), n[c], field[c], weight[c]);
@@ -602,7 +813,9 @@ hipError_t launch_p2g_velocity(hipStream_t s, const GridDev& g, int64_t np, cons
 hipError_t launch_g2p(hipStream_t s, const GridDev& g, int kind, int64_t np, const float* pos, const float* field,
                       int alg, float* val, int span, int off) {
     if (np <= 0) return hipSuccess;
-    return LAUNCH(k_g2p, grid_for(size_t(np)), pgrid_of(g, kind), np, pos, field, alg, val, span, off);
+    if (alg == MGF_PG_QUADRATIC || alg == MGF_PG_CUBIC)
+        return LAUNCH(k_g2p<4>, grid_for(size_t(np)), pgrid_of(g, kind), np, pos, field, alg, val, span, off);
+    return LAUNCH(k_g2p<2>, grid_for(size_t(np)), pgrid_of(g, kind), np, pos, field, alg, val, span, off);
 }
 
 hipError_t launch_particle_markers(hipStream_t s, const GridDev& g, const float* solid, int64_t np, const float* pos,
@@ -692,8 +905,11 @@ hipError_t launch_transfer_vel(hipStream_t s, const GridDev& g, int64_t np, cons
     if (np <= 0) return hipSuccess;
     const Faces3 fi{{in[0], in[1], in[2]}};
     const Faces3 fd{{delta ? delta[0] : nullptr, delta ? delta[1] : nullptr, delta ? delta[2] : nullptr}};
-    return LAUNCH(k_transfer_vel, grid_for(size_t(np)), pgrid_of(g, 1), pgrid_of(g, 2), pgrid_of(g, 3), np, pos, fi,
-                  fd, kind, alg, keep, mixing, vel);
+    if (alg == MGF_PG_QUADRATIC || alg == MGF_PG_CUBIC)
+        return LAUNCH(k_transfer_vel<4>, grid_for(size_t(np)), pgrid_of(g, 1), pgrid_of(g, 2), pgrid_of(g, 3), np,
+                      pos, fi, fd, kind, alg, keep, mixing, vel);
+    return LAUNCH(k_transfer_vel<2>, grid_for(size_t(np)), pgrid_of(g, 1), pgrid_of(g, 2), pgrid_of(g, 3), np, pos,
+                  fi, fd, kind, alg, keep, mixing, vel);
 }
 
 hipError_t launch_sample_absmax(hipStream_t s, const GridDev& g, Vel V, int64_t np, const float* pos, float* out,
@@ -712,6 +928,42 @@ hipError_t launch_rk_substep(hipStream_t s, const GridDev& g, Vel V, int64_t np,
     if (e != hipSuccess || np <= 0) return e;
     return LAUNCH(k_rk_substep, std::min<uint32_t>(grid_for(size_t(np)), kMaxReduceBlocks), g, V, np, pos, v1, order,
                   step, half, three_q, c1, c2, c3, resample, vmax_bits);
+}
+
+}  // namespace mgf
+
+namespace mgf {
+
+size_t bin_count(const GridDev& g) { return size_t(g.nx + 1) * size_t(g.ny + 1) * size_t(g.nz + 1); }
+uint32_t bin_tiles(const GridDev& g) { return gen_tiles(bin_count(g)); }
+
+hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, const float* pos, uint32_t* counts,
+                                uint32_t* tiles, uint32_t* start, uint32_t* cursor, uint32_t* order) {
+    const size_t nb = bin_count(g);
+    const uint32_t nt = gen_tiles(nb);
+    const PGrid c = pgrid_of(g, 0);
+    hipError_t e = hipMemsetAsync(counts, 0, nb * sizeof(uint32_t), s);
+    if (e == hipSuccess && np > 0) e = LAUNCH(k_bin_count, grid_for(size_t(np)), c, np, pos, counts);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles);
+    hipLaunchKernelGGL(k_gen_scan, dim3(1), dim3(1024), 0, s, nt, tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles, nt, start, cursor);
+    e = hipGetLastError();
+    if (e == hipSuccess && np > 0) e = LAUNCH(k_bin_scatter, grid_for(size_t(np)), c, np, pos, cursor, order);
+    return e;
+}
+
+size_t p2g_acc_floats(const GridDev& g) { return 2 * size_t(kSlotsMax) * bin_count(g); }
+
+hipError_t launch_p2g_binned(hipStream_t s, const GridDev& g, int kind, const uint32_t* start, const uint32_t* order,
+                             const float* pos, const float* val, int span, int off, float* acc, float* field,
+                             float* weight, int normalize) {
+    const PGrid p = pgrid_of(g, kind);
+    const size_t n = size_t(p.nx) * p.ny * p.nz;
+    const BinGrid bg{g.nx, g.ny, g.nz};
+    hipError_t e = LAUNCH(k_p2g_bins, grid_for(bin_count(g)), p, kind - 1, bg, start, order, pos, val, span, off, acc);
+    if (e == hipSuccess) e = LAUNCH(k_p2g_nodes, grid_for(n), p, kind - 1, bg, acc, field, weight, normalize);
+    return e;
 }
 
 }  // namespace mgf

@@ -149,6 +149,32 @@ mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t
     return MGF_OK;
 }
 
+// One field splatted from the particles' values (span, off) onto grid `kind`,
+// normalised: LINEAR through the bins, the wider kernels through the
+// per-particle splat.
+mgf_status splat(mgf_psolver s, hipStream_t st, const GridDev& g, int kind, const float* val, int span, int off,
+                 float* out, float* weight) {
+    mgf_particles p = s->p;
+    if (s->alg == MGF_PG_LINEAR)
+        MGF_HIP(launch_p2g_binned(st, g, kind, s->bin_start, s->order, p->pos, val, span, off, s->acc, out, weight, 1));
+    else
+        MGF_HIP(launch_p2g(st, g, kind, p->n, p->pos, val, span, off, s->alg, out, weight, 1));
+    return MGF_OK;
+}
+
+// The solver's particles binned at their current positions.  The bins are
+// reused only when `reuse` says that nothing outside the library can have
+// moved the particles since they were made (within one library step); a
+// caller may write positions through the borrowed device pointer.
+mgf_status ensure_bins(mgf_psolver s, hipStream_t st, const GridDev& g, bool reuse) {
+    mgf_particles p = s->p;
+    if (s->alg != MGF_PG_LINEAR || (reuse && s->bins_version == p->pos_version)) return MGF_OK;
+    MGF_HIP(launch_bin_particles(st, g, p->n, p->pos, s->bin_counts, s->bin_tiles, s->bin_start, s->bin_cursor,
+                                 s->order));
+    s->bins_version = p->pos_version;
+    return MGF_OK;
+}
+
 // advect(CFaceCenteredVectorField) of the three solvers.
 mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in[3], const float* const vel[3],
                             float dt, int order, int fix, float* const out[3]) {
@@ -170,9 +196,14 @@ mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in
     const float* const* delta = s->kind == MGF_PSOLVER_PIC ? nullptr : s->delta;
     MGF_HIP(launch_transfer_vel(st, g, n, p->pos, in, delta, s->kind, s->alg, float(1.0 - double(s->mixing)),
                                 s->mixing, p->vel));
-    TRY(particles_advect_run(st, g, vel3(vel), n, p->pos, dt, s->cfl, order, fix, p->scratch, p->vmax));
-    // transferParticlesVel2Field, normalised by the weights
-    MGF_HIP(launch_p2g_velocity(st, g, n, p->pos, p->vel, s->alg, out, s->weight));
+    const mgf_status adv = particles_advect_run(st, g, vel3(vel), n, p->pos, dt, s->cfl, order, fix, p->scratch,
+                                                p->vmax);
+    ++p->pos_version;
+    TRY(adv);
+    // transferParticlesVel2Field, normalised by the weights: a gather over the
+    // particles binned at their new positions
+    TRY(ensure_bins(s, st, g, false));
+    for (int k = 0; k < 3; ++k) TRY(splat(s, st, g, k + 1, p->vel, 3, k, out[k], s->weight[k]));
     if (s->kind != MGF_PSOLVER_PIC)   // m_DeltaGridVelField = voOutputField
         for (int k = 0; k < 3; ++k)
             MGF_HIP(hipMemcpyAsync(s->delta[k], out[k], pg_field_count(g, k + 1) * sizeof(float),
@@ -181,24 +212,26 @@ mgf_status psolver_velocity(mgf_psolver s, hipStream_t st, const float* const in
 }
 
 // advect(CCellCenteredScalarField) (ParticleInCell.cpp:58-75).
-mgf_status psolver_scalar(mgf_psolver s, hipStream_t st, const float* in, float* out) {
+mgf_status psolver_scalar(mgf_psolver s, hipStream_t st, const float* in, float* out, bool reuse_bins) {
     mgf_particles p = s->p;
     const GridDev g = make_grid(&s->grid);
     if (!s->scalar_seeded) MGF_HIP(launch_g2p(st, g, 0, p->n, p->pos, in, s->alg, p->scalar, 1, 0));
     s->scalar_seeded = true;
-    MGF_HIP(launch_p2g(st, g, 0, p->n, p->pos, p->scalar, 1, 0, s->alg, out, s->weight[0], 1));
+    TRY(ensure_bins(s, st, g, reuse_bins));
+    TRY(splat(s, st, g, 0, p->scalar, 1, 0, out, s->weight[0]));
     return MGF_OK;
 }
 
 // advect(CCellCenteredVectorField) (ParticleInCell.cpp:77-94).
-mgf_status psolver_vector(mgf_psolver s, hipStream_t st, const float* const in[3], float* const out[3]) {
+mgf_status psolver_vector(mgf_psolver s, hipStream_t st, const float* const in[3], float* const out[3],
+                          bool reuse_bins) {
     mgf_particles p = s->p;
     const GridDev g = make_grid(&s->grid);
     if (!s->vector_seeded)
         for (int k = 0; k < 3; ++k) MGF_HIP(launch_g2p(st, g, 0, p->n, p->pos, in[k], s->alg, p->vector, 3, k));
     s->vector_seeded = true;
-    for (int k = 0; k < 3; ++k)
-        MGF_HIP(launch_p2g(st, g, 0, p->n, p->pos, p->vector, 3, k, s->alg, out[k], s->weight[k], 1));
+    TRY(ensure_bins(s, st, g, reuse_bins));
+    for (int k = 0; k < 3; ++k) TRY(splat(s, st, g, 0, p->vector, 3, k, out[k], s->weight[k]));
     return MGF_OK;
 }
 
@@ -277,6 +310,7 @@ mgf_status mgf_particles_upload(mgf_particles p, int32_t which, const float* hos
     if (!a) return fail(MGF_ERR_INVALID, "unknown particle array");
     if (count != want) return fail(MGF_ERR_INVALID, "count does not match the particle array");
     MGF_HIP(hipSetDevice(p->ctx->device));
+    if (which == MGF_PARTICLE_POS) ++p->pos_version;
     if (count) MGF_HIP(hipMemcpyAsync(a, host, size_t(count) * sizeof(float), hipMemcpyHostToDevice, p->ctx->stream));
     MGF_HIP(hipStreamSynchronize(p->ctx->stream));
     return MGF_OK;
@@ -301,6 +335,7 @@ mgf_status mgf_particles_advect(mgf_particles p, const mgf_grid* grid, const flo
     if (!std::isfinite(dt) || !std::isfinite(cfl)) return fail(MGF_ERR_INVALID, "dt and cfl must be finite");
     MGF_HIP(hipSetDevice(p->ctx->device));
     hipStream_t st = p->ctx->stream;
+    ++p->pos_version;
     const mgf_status r = particles_advect_run(st, make_grid(grid), Vel{u, v, w}, p->n, p->pos, dt, cfl, order,
                                               fix_to_grid, p->scratch, p->vmax);
     (void)hipStreamSynchronize(st);
@@ -330,6 +365,15 @@ mgf_status mgf_psolver_create(mgf_context ctx, const mgf_grid* grid, int32_t kin
         for (int k = 0; k < 3; ++k) {
             TRY(owned_alloc(s->allocations, &s->weight[k], pg_field_count(g, k + 1)));
             if (kind != MGF_PSOLVER_PIC) TRY(owned_alloc(s->allocations, &s->delta[k], pg_field_count(g, k + 1)));
+        }
+        if (alg == MGF_PG_LINEAR) {   // the binned two-pass splat
+            const size_t nb = bin_count(g);
+            TRY(owned_alloc(s->allocations, &s->bin_counts, nb));
+            TRY(owned_alloc(s->allocations, &s->bin_cursor, nb));
+            TRY(owned_alloc(s->allocations, &s->bin_start, nb + 1));
+            TRY(owned_alloc(s->allocations, &s->bin_tiles, size_t(bin_tiles(g)) + 1));
+            TRY(owned_alloc(s->allocations, &s->order, size_t(particles->n)));
+            TRY(owned_alloc(s->allocations, &s->acc, p2g_acc_floats(g)));
         }
         MGF_HIP(hipStreamSynchronize(ctx->stream));
         return MGF_OK;
@@ -376,7 +420,7 @@ mgf_status mgf_psolver_advect_scalar(mgf_psolver s, const float* in, float* out)
     if (!s || !in || !out) return fail(MGF_ERR_INVALID, "null argument");
     MGF_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    const mgf_status r = psolver_scalar(s, st, in, out);
+    const mgf_status r = psolver_scalar(s, st, in, out, false);
     (void)hipStreamSynchronize(st);
     return r;
 }
@@ -388,7 +432,7 @@ mgf_status mgf_psolver_advect_vector(mgf_psolver s, const float* in_x, const flo
     float* out[3] = {out_x, out_y, out_z};
     MGF_HIP(hipSetDevice(s->ctx->device));
     hipStream_t st = s->ctx->stream;
-    const mgf_status r = psolver_vector(s, st, in, out);
+    const mgf_status r = psolver_vector(s, st, in, out, false);
     (void)hipStreamSynchronize(st);
     return r;
 }
