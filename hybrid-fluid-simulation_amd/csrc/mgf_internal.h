@@ -332,8 +332,6 @@ hipError_t launch_g2p(hipStream_t s, const GridDev& g, int kind, int64_t np, con
                       int alg, float* val, int span, int off);
 hipError_t launch_particle_markers(hipStream_t s, const GridDev& g, const float* solid, int64_t np, const float* pos,
                                    float* out);
-hipError_t launch_sample_particles(hipStream_t s, const GridDev& g, Vel V, int64_t np, const float* pos, float* out);
-hipError_t launch_absmax(hipStream_t s, size_t n, const float* v, uint32_t* out);
 hipError_t launch_axpy_n(hipStream_t s, size_t n, float a, const float* x, float* y);
 hipError_t launch_scale_n(hipStream_t s, size_t n, float a, float* x);
 hipError_t launch_clamp_particles(hipStream_t s, const GridDev& g, int64_t np, float* pos);
@@ -357,12 +355,10 @@ hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, con
 hipError_t launch_p2g_binned(hipStream_t s, const GridDev& g, int kind, const uint32_t* start, const uint32_t* order,
                              const float* pos, const float* val, int span, int off, float* acc, float* field,
                              float* weight, int normalize);
-// Fused particle kernels: the velocity splat of the three face grids; the
-// particle-velocity update of PIC / FLIP / MIX (delta = NULL for PIC); the
-// first velocity sample of a substep loop with its max |v|; one RK substep
-// (resample: the next substep's velocities and max |v|).
-hipError_t launch_p2g_velocity(hipStream_t s, const GridDev& g, int64_t np, const float* pos, const float* vel,
-                               int alg, float* const field[3], float* const weight[3]);
+// Fused particle kernels: the particle-velocity update of PIC / FLIP / MIX
+// (delta = NULL for PIC); the first velocity sample of a substep loop with
+// its max |v|; one RK substep (resample: the next substep's velocities and
+// max |v|).
 hipError_t launch_transfer_vel(hipStream_t s, const GridDev& g, int64_t np, const float* pos,
                                const float* const in[3], const float* const delta[3], int kind, int alg, float keep,
                                float mixing, float* vel);

@@ -358,31 +358,6 @@ __global__ void __launch_bounds__(kBlock) k_particle_markers(size_t n, const flo
     io[i] = solid[i] < 0.0f ? 2.0f : (io[i] > 0.0f ? 1.0f : 0.0f);
 }
 
-// The face-centred velocity at each particle (sampleFaceCenteredVectorField
-// Invoker on a position array: the trilinear sampler per component).
-__global__ void __launch_bounds__(kBlock) k_sample_particles(GridDev g, Vel V, int64_t np,
-                                                             const float* __restrict__ pos, float* __restrict__ out) {
-    const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
-    if (p >= np) return;
-    float a, b, c;
-    sample_vel(V, g, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2], a, b, c);
-    out[3 * p] = a;
-    out[3 * p + 1] = b;
-    out[3 * p + 2] = c;
-}
-
-// max |v| over the sampled components (getMaxValue / getMinValue through
-// cublasI[sa]max/min, EulerParticles.cpp:251): the bit pattern of a
-// non-negative float orders like its value.
-__global__ void __launch_bounds__(kBlock) k_absmax(size_t n, const float* __restrict__ v, uint32_t* out) {
-    const size_t i0 = size_t(blockIdx.x) * kBlock + threadIdx.x;
-    uint32_t m = 0;
-    for (size_t i = i0; i < n; i += size_t(gridDim.x) * kBlock) m = max(m, __float_as_uint(fabsf(v[i])));
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, uint32_t(__shfl_xor(int(m), o, 64)));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
-}
-
 // axpyReal (ThrustWapper.cu:281-292, cuBLAS axpy): y = a x + y as one fused
 // multiply-add; scaleReal (:294-304): x = a x.
 __global__ void __launch_bounds__(kBlock) k_axpy_n(size_t n, float a, const float* __restrict__ x, float* y) {
@@ -787,29 +762,6 @@ hipError_t launch_p2g(hipStream_t s, const GridDev& g, int kind, int64_t np, con
     return e;
 }
 
-hipError_t launch_p2g_velocity(hipStream_t s, const GridDev& g, int64_t np, const float* pos, const float* vel,
-                               int alg, float* const field[3], float* const weight[3]) {
-    PGrid p[3];
-    size_t n[3];
-    hipError_t e = hipSuccess;
-    for (int c = 0; c < 3; ++c) {
-        p[c] = pgrid_of(g, c + 1);
-        n[c] = size_t(p[c].nx) * p[c].ny * p[c].nz;
-        if (e == hipSuccess) e = launch_fill(s, field[c], n[c], 0.0f);
-        if (e == hipSuccess) e = launch_fill(s, weight[c], n[c], 0.0f);
-    }
-    if (e == hipSuccess && np > 0) {
-        const P2GOut out{{field[0], field[1], field[2]}, {weight[0], weight[1], weight[2]}};
-        if (alg == MGF_PG_LINEAR || alg == MGF_PG_P2GSUM)
-            e = LAUNCH((k_p2g_lds<3, 11, 2>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
-        else
-            e = LAUNCH((k_p2g_lds<3, 12, 4>), grid_for(size_t(np)), p[0], p[1], p[2], np, pos, vel, 3, 0, alg, out);
-    }
-    for (int c = 0; c < 3; ++c)
-        if (e == hipSuccess) e = LAUNCH(k_p2g_normalize, grid_for(n[c]), n[c], field[c], weight[c]);
-    return e;
-}
-
 hipError_t launch_g2p(hipStream_t s, const GridDev& g, int kind, int64_t np, const float* pos, const float* field,
                       int alg, float* val, int span, int off) {
     if (np <= 0) return hipSuccess;
@@ -825,18 +777,6 @@ hipError_t launch_particle_markers(hipStream_t s, const GridDev& g, const float*
     if (e == hipSuccess && np > 0) e = LAUNCH(k_particle_counts, grid_for(size_t(np)), pgrid_of(g, 0), np, pos, out);
     if (e == hipSuccess && solid) e = LAUNCH(k_particle_markers, grid_for(n), n, solid, out);
     return e;
-}
-
-hipError_t launch_sample_particles(hipStream_t s, const GridDev& g, Vel V, int64_t np, const float* pos, float* out) {
-    if (np <= 0) return hipSuccess;
-    return LAUNCH(k_sample_particles, grid_for(size_t(np)), g, V, np, pos, out);
-}
-
-hipError_t launch_absmax(hipStream_t s, size_t n, const float* v, uint32_t* out) {
-    hipError_t e = hipMemsetAsync(out, 0, sizeof(uint32_t), s);
-    if (e != hipSuccess || n == 0) return e;
-    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 2048);
-    return LAUNCH(k_absmax, blocks, n, v, out);
 }
 
 hipError_t launch_axpy_n(hipStream_t s, size_t n, float a, const float* x, float* y) {
