@@ -540,8 +540,12 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         TRY(ensure_precond(s));
         TRY(mg_setup(s, sa));
     }
-    // x64 = 0, ||b||^2 and the overall target
-    MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, nullptr, s->x64, s->b, nullptr, 0, s->red, s->red.out2));
+    // x64 = 0 (or, warm, the previous pressure on the fluid cells and its
+    // residual), ||b||^2 and the overall target
+    const bool warm = d.warm_start && s->x64_valid;
+    s->x64_valid = false;
+    MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, nullptr, s->x64, s->b, warm ? s->r : nullptr, warm ? 3 : 0,
+                               s->red, s->red.out2));
     MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.out2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
     MGF_HIP(hipStreamSynchronize(st));
     const double b2 = s->host_scratch[1];
@@ -555,9 +559,9 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     s->x_is_64 = true;
     int64_t it_total = 0;
     int sweeps = 0;
-    bool conv = b2 <= target2 || b2 == 0.0;
-    double rr_true = b2;
-    const float* rsrc = s->b;
+    double rr_true = warm ? s->host_scratch[0] : b2;
+    bool conv = rr_true <= target2 || b2 == 0.0;
+    const float* rsrc = warm ? s->r : s->b;
     CgScalars hs{};
     while (!conv && sweeps < max_sweeps && it_total < max_it) {
         MGF_HIP(launch_cg_start(st, s->g, rsrc, s->r, s->x, s->scal, s->red, inner, target2));
@@ -592,8 +596,8 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         it_total += hs.iters;
         if (hs.done == 2) { rr_true = std::nan(""); break; }
         // x64 (+)= x; r = b - A x64 in FP64
-        MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, s->x, s->x64, s->b, s->r, sweeps > 0 ? 2 : 1, s->red,
-                                   s->red.out2));
+        MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, s->x, s->x64, s->b, s->r, (sweeps > 0 || warm) ? 2 : 1,
+                                   s->red, s->red.out2));
         MGF_HIP(hipMemcpyAsync(s->host_scratch, s->red.out2, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
         MGF_HIP(hipStreamSynchronize(st));
         rr_true = s->host_scratch[0];
@@ -602,6 +606,7 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         if (rr_true <= target2) conv = true;
         if (!std::isfinite(rr_true)) break;
     }
+    s->x64_valid = conv && std::isfinite(rr_true);
     acc->cg_iterations += it_total;
     acc->refinement_sweeps += sweeps;
     acc->converged = acc->converged && conv;

@@ -106,6 +106,7 @@ struct mgf_scene_s {
     int mg_nx = -1, mg_ny = -1, mg_nz = -1;
     int mg_l0 = 0;                 // levels [mg_l0, L) run in one work-group
     bool x_is_64 = false;          // which pressure the last solve produced
+    bool x64_valid = false;        // x64 holds a finished mixed solve (a warm start's guess)
 
     // extrapolation scratch
     int32_t* layer = nullptr;      // distance of every face (nfx+nfy+nfz)
@@ -270,8 +271,9 @@ hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float* p, co
 // max(rel_inner^2 r.r, abs_target2)).
 hipError_t launch_cg_start(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,
                            ReduceBuf red, double rel_inner, double abs_target2);
-// Refinement: mode 0: x64 = 0; mode 1: x64 = xcorr; mode 2: x64 += xcorr; then
-// (modes 1, 2) r = b - A x64 in FP64, stored FP32 when r != null.  out2 (device)
+// Refinement: mode 0: x64 = 0; mode 1: x64 = xcorr; mode 2: x64 += xcorr;
+// mode 3: x64 kept on fluid cells, zeroed elsewhere (a warm start); then
+// (modes 1-3) r = b - A x64 in FP64, stored FP32 when r != null.  out2 (device)
 // receives ||b - A x64||^2 (0 in mode 0) and ||b||^2.
 hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
                               const float* xcorr, double* x64, const float* b, float* r, int mode, ReduceBuf red,

@@ -334,6 +334,12 @@ __device__ __forceinline__ void refine_body(const Flat& F, uint32_t base, const 
             vc = xo[k];
 #pragma unroll
             for (int m = 0; m < 6; ++m) v[m] = xon[k][m];
+            // a warm start keeps the fluid cells' pressure and zeroes the
+            // rest (fluid rows never read a non-fluid neighbour's value)
+            if (MODE == 3 && !(cd[k] & kCodeFluid)) {
+                vc = 0.0;
+                x64[c] = 0.0;
+            }
         }
         const double res = double(bc[k]) - arow64(cd[k], vc, v, sx, sy, sz, aw);
         if (r) r[c] = float(res);
@@ -1144,6 +1150,8 @@ hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale
     hipError_t e;
     if (mode == 0)
         e = LAUNCH(k_refine<0>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
+    else if (mode == 3)
+        e = LAUNCH(k_refine<3>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
     else if (mode == 1)
         e = LAUNCH(k_refine<1>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
     else {

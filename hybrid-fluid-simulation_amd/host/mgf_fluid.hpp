@@ -714,6 +714,7 @@ struct PressureOptions {
     int32_t tol_mode = MGF_TOL_ABS_SQRT_NORM;
     double tolerance = 1e-3;
     int64_t max_iterations = 0;
+    bool warm_start = false;   // MIXED: start from the previous pressure (not the reference's x = 0)
 };
 
 inline void apply_options(mgf_scene_desc& d, const PressureOptions& p) {
@@ -722,6 +723,7 @@ inline void apply_options(mgf_scene_desc& d, const PressureOptions& p) {
     d.tol_mode = p.tol_mode;
     d.tolerance = p.tolerance;
     d.max_iterations = p.max_iterations;
+    d.warm_start = p.warm_start ? 1 : 0;
 }
 
 // A scene of the library, owned.

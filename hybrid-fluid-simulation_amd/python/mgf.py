@@ -37,7 +37,7 @@ class SceneDesc(C.Structure):
                 ("max_iterations", C.c_int64), ("precision", C.c_int32),
                 ("preconditioner", C.c_int32), ("extrapolation_layers", C.c_int32),
                 ("max_refinements", C.c_int32), ("inner_tolerance", C.c_double),
-                ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float)]
+                ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float), ("warm_start", C.c_int32)]
 
 
 class StepStats(C.Structure):
@@ -254,7 +254,7 @@ def desc_defaults(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0):
 def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode=TOL_ABS_SQRT_NORM,
               tol=1e-3, max_it=0, precision=PRECISION_PLAIN, layers=1000, gravity=(0, -9.8, 0),
               force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3,
-              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9):
+              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9, warm_start=False):
     d = desc_defaults(res, origin, spacing, dt)
     d.rk_order = rk
     d.tol_mode = tol_mode
@@ -270,6 +270,7 @@ def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode
     d.preconditioner = precond
     d.mg_sweeps = mg_sweeps
     d.mg_omega = mg_omega
+    d.warm_start = 1 if warm_start else 0
     return d
 
 

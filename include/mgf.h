@@ -145,6 +145,11 @@ typedef struct mgf_scene_desc {
     int32_t mg_sweeps;         /* MULTIGRID: Jacobi sweeps before and after the
                                   coarse correction (<= 0: 3; at most 8)       */
     float   mg_omega;          /* MULTIGRID: Jacobi damping (<= 0: 0.9; < 2)  */
+    int32_t warm_start;        /* MIXED: 1 = start the CG from the previous
+                                  step's pressure (fluid cells; 0 elsewhere)
+                                  instead of x = 0 (PressureSolver.cpp:145).
+                                  Same stop rule, different iterates; off by
+                                  default                                    */
 } mgf_scene_desc;
 
 /* Per-call statistics (summed over the steps of one mgf_step call). */
