@@ -361,9 +361,9 @@ mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
 // ---------------------------------------------------------------------------
 // Preconditioners of the MIXED solver.
 constexpr int kCoarsestSweeps = 8;
-constexpr size_t kCoarseCells = 1000;    // levels this small run in one work-group
+constexpr size_t kCoarseCells = 8192;    // levels this small run in one work-group
 constexpr int kCoarseLevelsMax = 8;
-constexpr size_t kCoarseLdsMax = 65536;
+constexpr size_t kCoarseLdsMax = 131072;
 
 mgf_status ensure_precond(mgf_scene s) {
     if (!s->z) {
