@@ -165,6 +165,8 @@ struct mgf_psolver_s {
     uint32_t *bin_counts = nullptr, *bin_tiles = nullptr, *bin_start = nullptr, *bin_cursor = nullptr;
     uint32_t* order = nullptr;
     float* acc = nullptr;                            // the splat's per-bin slots (LINEAR)
+    float* pos_s = nullptr;                          // positions in bin order (3n)
+    float* vel_s = nullptr;                          // velocities in bin order (3n)
     uint64_t bins_version = ~uint64_t(0);
     bool scalar_seeded = false, vector_seeded = false, vel_seeded = false;
     std::vector<void*> allocations;
@@ -346,17 +348,20 @@ hipError_t launch_gen_emit(hipStream_t s, const GridDev& g, const float* fluid, 
 hipError_t launch_flip_delta(hipStream_t s, size_t n, const float* in, float* delta);
 // Particles counting-sorted by the lower corner of their support on the cell
 // grid (bin_count(g) bins; counts, cursor: that many u32, start one more,
-// tiles bin_tiles(g) + 1, order np), then the LINEAR splat onto grid `kind`
-// in two atomic-free passes through acc (p2g_acc_floats(g) floats); every
-// node of the field and weight is written.
+// tiles bin_tiles(g) + 1, order np; pos_s 3 np and, when vel is given, vel_s
+// 3 np: sorted copies), then the LINEAR splat onto grid `kind` in two
+// atomic-free passes through acc (p2g_acc_floats(g) floats); val is in bin
+// order when val_sorted, else per particle; every node of the field and
+// weight is written.
 size_t bin_count(const GridDev& g);
 uint32_t bin_tiles(const GridDev& g);
 size_t p2g_acc_floats(const GridDev& g);
 hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, const float* pos, uint32_t* counts,
-                                uint32_t* tiles, uint32_t* start, uint32_t* cursor, uint32_t* order);
+                                uint32_t* tiles, uint32_t* start, uint32_t* cursor, uint32_t* order, float* pos_s,
+                                const float* vel, float* vel_s);
 hipError_t launch_p2g_binned(hipStream_t s, const GridDev& g, int kind, const uint32_t* start, const uint32_t* order,
-                             const float* pos, const float* val, int span, int off, float* acc, float* field,
-                             float* weight, int normalize);
+                             const float* pos_s, const float* val, int span, int off, int val_sorted, float* acc,
+                             float* field, float* weight, int normalize);
 // Fused particle kernels: the particle-velocity update of PIC / FLIP / MIX
 // (delta = NULL for PIC); the first velocity sample of a substep loop with
 // its max |v|; one RK substep (resample: the next substep's velocities and

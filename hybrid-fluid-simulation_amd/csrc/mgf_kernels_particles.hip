@@ -605,12 +605,25 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(size_t n, const uint32_t*
     if (blockIdx.x == 0 && threadIdx.x == 0) start[n] = tiles[nt];
 }
 
+// Each particle's slot in bin order, with sorted copies of its position and
+// (when vel is given) its velocity, so the splat reads them contiguously.
 __global__ void __launch_bounds__(kBlock) k_bin_scatter(PGrid c, int64_t np, const float* __restrict__ pos,
-                                                        uint32_t* cursor, uint32_t* __restrict__ order) {
+                                                        uint32_t* cursor, uint32_t* __restrict__ order,
+                                                        float* __restrict__ pos_s, const float* __restrict__ vel,
+                                                        float* __restrict__ vel_s) {
     const int64_t p = int64_t(xcd_block()) * kBlock + threadIdx.x;
     if (p >= np) return;
-    const uint32_t slot = atomicAdd(&cursor[bin_of(c, pos[3 * p], pos[3 * p + 1], pos[3 * p + 2])], 1u);
+    const float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+    const uint32_t slot = atomicAdd(&cursor[bin_of(c, px, py, pz)], 1u);
     order[slot] = uint32_t(p);
+    pos_s[3 * size_t(slot)] = px;
+    pos_s[3 * size_t(slot) + 1] = py;
+    pos_s[3 * size_t(slot) + 2] = pz;
+    if (vel) {
+        vel_s[3 * size_t(slot)] = vel[3 * p];
+        vel_s[3 * size_t(slot) + 1] = vel[3 * p + 1];
+        vel_s[3 * size_t(slot) + 2] = vel[3 * p + 2];
+    }
 }
 
 // Two passes per target grid (the cells, or the faces of one axis; LINEAR
@@ -619,16 +632,20 @@ __global__ void __launch_bounds__(kBlock) k_bin_scatter(PGrid c, int64_t np, con
 // relative to the bin's first node clamp(s) (two slots per axis, three on the
 // axis the target grid is shifted along: its support starts at the cell
 // grid's corner or the next).  A support's corners that clamp onto one node
-// share a slot, so an edge bin collects far-outside particles too.  Pass 2,
-// one work-item per node: the sum, in a fixed bin order, of the slots of the
-// neighbouring bins that point at it; then the normalisation.
+// share a slot, so an edge bin collects far-outside particles too.  Empty
+// bins write nothing.  Pass 2, one work-item per node: the sum, in a fixed bin
+// order, of the slots of the non-empty neighbouring bins that point at it;
+// then the normalisation.
 constexpr int kSlotsMax = 12;
 
 __device__ __forceinline__ int clamp_bin(int s, int cells) { return s < -1 ? -1 : (s > cells - 1 ? cells - 1 : s); }
 
-__global__ void __launch_bounds__(kBlock) k_p2g_bins(PGrid g, int shift, BinGrid bg, const uint32_t* __restrict__ start,
-                                                     const uint32_t* __restrict__ order, const float* __restrict__ pos,
-                                                     const float* __restrict__ val, int span, int off,
+// pos_s: positions in bin order; val is in bin order too when val_sorted,
+// else indexed by particle through order.
+template <int SHIFT>
+__global__ void __launch_bounds__(kBlock) k_p2g_bins(PGrid g, BinGrid bg, const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ order, const float* __restrict__ pos_s,
+                                                     const float* __restrict__ val, int span, int off, int val_sorted,
                                                      float* __restrict__ acc) {
     const size_t nb = size_t(bg.nx + 1) * (bg.ny + 1) * (bg.nz + 1);
     const size_t b = size_t(xcd_block()) * kBlock + threadIdx.x;
@@ -636,21 +653,19 @@ __global__ void __launch_bounds__(kBlock) k_p2g_bins(PGrid g, int shift, BinGrid
     const int s[3] = {int(b % uint32_t(bg.nx + 1)) - 1, int((b / uint32_t(bg.nx + 1)) % uint32_t(bg.ny + 1)) - 1,
                       int(b / (size_t(bg.nx + 1) * uint32_t(bg.ny + 1))) - 1};
     const int len[3] = {g.nx, g.ny, g.nz};
-    int base[3], ns[3];
+    constexpr int ns[3] = {SHIFT == 0 ? 3 : 2, SHIFT == 1 ? 3 : 2, SHIFT == 2 ? 3 : 2};
+    int base[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        base[a] = clampi(s[a], len[a]);
-        ns[a] = a == shift ? 3 : 2;
-    }
+    for (int a = 0; a < 3; ++a) base[a] = clampi(s[a], len[a]);
+    const uint32_t q0 = start[b], q1 = start[b + 1];
+    if (q0 == q1) return;   // an empty bin's slots are never read
     float sv[kSlotsMax], sw[kSlotsMax];
 #pragma unroll
     for (int q = 0; q < kSlotsMax; ++q) sv[q] = sw[q] = 0.0f;
-    const uint32_t q0 = start[b], q1 = start[b + 1];
     for (uint32_t q = q0; q < q1; ++q) {
-        const uint32_t p = order[q];
         PStencil st;
-        pg_stencil(g, pos[3 * size_t(p)], pos[3 * size_t(p) + 1], pos[3 * size_t(p) + 2], MGF_PG_LINEAR, st);
-        const float v = val[size_t(span) * p + off];
+        pg_stencil(g, pos_s[3 * size_t(q)], pos_s[3 * size_t(q) + 1], pos_s[3 * size_t(q) + 2], MGF_PG_LINEAR, st);
+        const float v = val[size_t(span) * (val_sorted ? q : order[q]) + off];
         // per-axis slot weights: the corners that clamp onto slot t
         float W[3][3];
 #pragma unroll
@@ -666,23 +681,18 @@ __global__ void __launch_bounds__(kBlock) k_p2g_bins(PGrid g, int shift, BinGrid
             }
         }
 #pragma unroll
-        for (int tz = 0; tz < 3; ++tz)
+        for (int tz = 0; tz < ns[2]; ++tz)
 #pragma unroll
-            for (int ty = 0; ty < 3; ++ty)
+            for (int ty = 0; ty < ns[1]; ++ty)
 #pragma unroll
-                for (int tx = 0; tx < 3; ++tx) {
-                    if (tx >= ns[0] || ty >= ns[1] || tz >= ns[2]) continue;
+                for (int tx = 0; tx < ns[0]; ++tx) {
                     const int slot = (tz * ns[1] + ty) * ns[0] + tx;
                     const float wt = W[0][tx] * W[1][ty] * W[2][tz];
-#pragma unroll
-                    for (int u = 0; u < kSlotsMax; ++u)
-                        if (u == slot) {
-                            sv[u] += wt * v;
-                            sw[u] += wt;
-                        }
+                    sv[slot] += wt * v;
+                    sw[slot] += wt;
                 }
     }
-    const int nslot = ns[0] * ns[1] * ns[2];
+    constexpr int nslot = ns[0] * ns[1] * ns[2];
 #pragma unroll
     for (int u = 0; u < kSlotsMax; ++u)
         if (u < nslot) {
@@ -691,9 +701,9 @@ __global__ void __launch_bounds__(kBlock) k_p2g_bins(PGrid g, int shift, BinGrid
         }
 }
 
-__global__ void __launch_bounds__(kBlock) k_p2g_nodes(PGrid g, int shift, BinGrid bg, const float* __restrict__ acc,
-                                                      float* __restrict__ field, float* __restrict__ weight,
-                                                      int normalize) {
+__global__ void __launch_bounds__(kBlock) k_p2g_nodes(PGrid g, int shift, BinGrid bg, const uint32_t* __restrict__ start,
+                                                      const float* __restrict__ acc, float* __restrict__ field,
+                                                      float* __restrict__ weight, int normalize) {
     const size_t total = size_t(g.nx) * g.ny * g.nz;
     const size_t node = size_t(xcd_block()) * kBlock + threadIdx.x;
     if (node >= total) return;
@@ -720,6 +730,7 @@ __global__ void __launch_bounds__(kBlock) k_p2g_nodes(PGrid g, int shift, BinGri
                 if (tx < 0 || tx >= ns[0]) continue;
                 const size_t b = (size_t(bz + 1) * uint32_t(bg.ny + 1) + uint32_t(by + 1)) * uint32_t(bg.nx + 1) +
                                  uint32_t(bx + 1);
+                if (start[b] == start[b + 1]) continue;   // empty bin: no slots written
                 const size_t slot = size_t((tz * ns[1] + ty) * ns[0] + tx);
                 sv += acc[2 * slot * nb + b];
                 sw += acc[(2 * slot + 1) * nb + b];
@@ -878,7 +889,8 @@ size_t bin_count(const GridDev& g) { return size_t(g.nx + 1) * size_t(g.ny + 1) 
 uint32_t bin_tiles(const GridDev& g) { return gen_tiles(bin_count(g)); }
 
 hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, const float* pos, uint32_t* counts,
-                                uint32_t* tiles, uint32_t* start, uint32_t* cursor, uint32_t* order) {
+                                uint32_t* tiles, uint32_t* start, uint32_t* cursor, uint32_t* order, float* pos_s,
+                                const float* vel, float* vel_s) {
     const size_t nb = bin_count(g);
     const uint32_t nt = gen_tiles(nb);
     const PGrid c = pgrid_of(g, 0);
@@ -889,20 +901,28 @@ hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, con
     hipLaunchKernelGGL(k_gen_scan, dim3(1), dim3(1024), 0, s, nt, tiles);
     hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles, nt, start, cursor);
     e = hipGetLastError();
-    if (e == hipSuccess && np > 0) e = LAUNCH(k_bin_scatter, grid_for(size_t(np)), c, np, pos, cursor, order);
+    if (e == hipSuccess && np > 0)
+        e = LAUNCH(k_bin_scatter, grid_for(size_t(np)), c, np, pos, cursor, order, pos_s, vel, vel_s);
     return e;
 }
 
 size_t p2g_acc_floats(const GridDev& g) { return 2 * size_t(kSlotsMax) * bin_count(g); }
 
 hipError_t launch_p2g_binned(hipStream_t s, const GridDev& g, int kind, const uint32_t* start, const uint32_t* order,
-                             const float* pos, const float* val, int span, int off, float* acc, float* field,
-                             float* weight, int normalize) {
+                             const float* pos_s, const float* val, int span, int off, int val_sorted, float* acc,
+                             float* field, float* weight, int normalize) {
     const PGrid p = pgrid_of(g, kind);
     const size_t n = size_t(p.nx) * p.ny * p.nz;
     const BinGrid bg{g.nx, g.ny, g.nz};
-    hipError_t e = LAUNCH(k_p2g_bins, grid_for(bin_count(g)), p, kind - 1, bg, start, order, pos, val, span, off, acc);
-    if (e == hipSuccess) e = LAUNCH(k_p2g_nodes, grid_for(n), p, kind - 1, bg, acc, field, weight, normalize);
+    const uint32_t gb = grid_for(bin_count(g));
+    hipError_t e;
+    switch (kind) {
+        case 1: e = LAUNCH(k_p2g_bins<0>, gb, p, bg, start, order, pos_s, val, span, off, val_sorted, acc); break;
+        case 2: e = LAUNCH(k_p2g_bins<1>, gb, p, bg, start, order, pos_s, val, span, off, val_sorted, acc); break;
+        case 3: e = LAUNCH(k_p2g_bins<2>, gb, p, bg, start, order, pos_s, val, span, off, val_sorted, acc); break;
+        default: e = LAUNCH(k_p2g_bins<-1>, gb, p, bg, start, order, pos_s, val, span, off, val_sorted, acc); break;
+    }
+    if (e == hipSuccess) e = LAUNCH(k_p2g_nodes, grid_for(n), p, kind - 1, bg, start, acc, field, weight, normalize);
     return e;
 }
 

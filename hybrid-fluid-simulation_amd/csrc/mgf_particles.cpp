@@ -155,8 +155,11 @@ mgf_status particles_advect_run(hipStream_t st, const GridDev& g, Vel V, int64_t
 mgf_status splat(mgf_psolver s, hipStream_t st, const GridDev& g, int kind, const float* val, int span, int off,
                  float* out, float* weight) {
     mgf_particles p = s->p;
+    // the velocities have a copy in bin order, made with the bins
+    const bool sorted = val == p->vel;
     if (s->alg == MGF_PG_LINEAR)
-        MGF_HIP(launch_p2g_binned(st, g, kind, s->bin_start, s->order, p->pos, val, span, off, s->acc, out, weight, 1));
+        MGF_HIP(launch_p2g_binned(st, g, kind, s->bin_start, s->order, s->pos_s, sorted ? s->vel_s : val, span, off,
+                                  sorted ? 1 : 0, s->acc, out, weight, 1));
     else
         MGF_HIP(launch_p2g(st, g, kind, p->n, p->pos, val, span, off, s->alg, out, weight, 1));
     return MGF_OK;
@@ -170,7 +173,7 @@ mgf_status ensure_bins(mgf_psolver s, hipStream_t st, const GridDev& g, bool reu
     mgf_particles p = s->p;
     if (s->alg != MGF_PG_LINEAR || (reuse && s->bins_version == p->pos_version)) return MGF_OK;
     MGF_HIP(launch_bin_particles(st, g, p->n, p->pos, s->bin_counts, s->bin_tiles, s->bin_start, s->bin_cursor,
-                                 s->order));
+                                 s->order, s->pos_s, p->vel, s->vel_s));
     s->bins_version = p->pos_version;
     return MGF_OK;
 }
@@ -374,6 +377,8 @@ mgf_status mgf_psolver_create(mgf_context ctx, const mgf_grid* grid, int32_t kin
             TRY(owned_alloc(s->allocations, &s->bin_tiles, size_t(bin_tiles(g)) + 1));
             TRY(owned_alloc(s->allocations, &s->order, size_t(particles->n)));
             TRY(owned_alloc(s->allocations, &s->acc, p2g_acc_floats(g)));
+            TRY(owned_alloc(s->allocations, &s->pos_s, 3 * size_t(particles->n)));
+            TRY(owned_alloc(s->allocations, &s->vel_s, 3 * size_t(particles->n)));
         }
         MGF_HIP(hipStreamSynchronize(ctx->stream));
         return MGF_OK;
