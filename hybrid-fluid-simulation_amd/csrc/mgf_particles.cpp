@@ -276,9 +276,15 @@ mgf_status mgf_particles_generate(mgf_context ctx, const mgf_grid* grid, const f
         }
         return MGF_OK;
     };
+    *out = nullptr;
     const mgf_status r = run();
     (void)hipStreamSynchronize(ctx->stream);
     free_all(tmp);
+    if (r != MGF_OK && *out) {   // no half-made set reaches the caller
+        free_all((*out)->allocations);
+        delete *out;
+        *out = nullptr;
+    }
     return r;
 }
 
