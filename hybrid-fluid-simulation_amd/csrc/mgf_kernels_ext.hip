@@ -366,9 +366,8 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const int32_t* 
 
 // ---------------------------------------------------------------------------
 // 3. Layer L: each listed face takes the mean of its neighbours in the
-// previous layer (lower-neighbour mask from the sort); the others are read at
-// the face itself and left out by the mask, so every load of a work-item is
-// issued at once and no distance is read.
+// previous layer (lower-neighbour mask from the sort); only those are loaded,
+// all at once, and no distance is read.
 //
 // The mean of face i (component-local) over the neighbours flagged in m, in
 // the reference's order (-x, +x, -y, +y, -z, +z) with one divide; the
@@ -378,8 +377,9 @@ __device__ __forceinline__ float layer_mean(const float* f, uint32_t i, uint32_t
     const uint32_t j[6] = {(m & 1u) ? i - 1 : i, (m & 2u) ? i + 1 : i, (m & 4u) ? i - ex : i,
                            (m & 8u) ? i + ex : i, (m & 16u) ? i - sz : i, (m & 32u) ? i + sz : i};
     float fv[6];
+    // only the flagged neighbours are loaded (the others are never summed)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) fv[q] = f[j[q]];
+    for (int q = 0; q < 6; ++q) fv[q] = ((m >> q) & 1u) ? f[j[q]] : 0.0f;
     float acc = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
