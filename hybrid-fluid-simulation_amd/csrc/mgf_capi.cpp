@@ -95,6 +95,7 @@ mgf_status validate_desc(const mgf_scene_desc* d) {
         return fail(MGF_ERR_UNSUPPORTED, "PLAIN precision runs the reference's identity-preconditioned CG only");
     if (d->mg_sweeps > 8) return fail(MGF_ERR_INVALID, "mg_sweeps must be <= 8");
     if (!(d->mg_omega < 2.0f) || !std::isfinite(d->mg_omega)) return fail(MGF_ERR_INVALID, "mg_omega must be < 2");
+    if (d->mg_per_sweep != 0 && d->mg_per_sweep != 1) return fail(MGF_ERR_INVALID, "mg_per_sweep must be 0 or 1");
     if (d->extrapolation_layers < 0 || d->extrapolation_layers > 1000000)
         return fail(MGF_ERR_INVALID, "extrapolation_layers out of range");
     return MGF_OK;
@@ -451,9 +452,20 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
     // descent: pre-smoothing from zero (the first two sweeps in one pass),
     // restriction of the residual
+    const bool fuse = nu == 3 && !d.mg_per_sweep;
+    auto fused = [&](int l) {
+        return fuse && l + 1 < L && mg_march_ok(lv[l].d) && mg_march_grid(lv[l].d, lv[l + 1].d) <= kMaxBlocks;
+    };
     for (int l = 0; l < l0; ++l) {
         Lv& v = lv[l];
         const bool coarsest = l == L - 1;
+        if (fused(l)) {
+            const int pk = l == 0 ? probe_begin(s) : -1;
+            MGF_HIP(launch_mg_descend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
+                                      const_cast<float*>(lv[l + 1].b), omega, s->scal));
+            probe_end(s, pk, MGF_PROBE_MG_DESCEND, 10.5 * double(v.d.nx) * v.d.ny * v.d.nz);
+            continue;
+        }
         const int sweeps = coarsest ? kCoarsestSweeps : nu;
         auto rz_at = [&](int k) { return (l == 0 && coarsest && k == sweeps - 1) ? 1 : 0; };
         int k;
@@ -496,6 +508,14 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         Lv& v = lv[l];
         auto rz_at = [&](int k) { return (l == 0 && k == nu - 1) ? 1 : 0; };
         const double ncell = double(v.d.nx) * v.d.ny * v.d.nz;
+        if (fused(l)) {
+            const int pf = l == 0 ? probe_begin(s) : -1;
+            MGF_HIP(launch_mg_ascend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
+                                     s->red, l == 0 ? 1 : 0));
+            probe_end(s, pf, MGF_PROBE_MG_ASCEND, 14.5 * ncell);
+            std::swap(v.x, v.t);
+            continue;
+        }
         int pk = (l == 0 && !rz_at(0)) ? probe_begin(s) : -1;
         MGF_HIP(launch_mg_smooth_prolong(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
                                          s->red, rz_at(0)));
@@ -1217,6 +1237,32 @@ static bool same_grid(const mgf_scene_desc& d, const mgf_grid& g) {
     return d.nx == g.nx && d.ny == g.ny && d.nz == g.nz && d.origin[0] == g.origin[0] &&
            d.origin[1] == g.origin[1] && d.origin[2] == g.origin[2] && d.spacing[0] == g.spacing[0] &&
            d.spacing[1] == g.spacing[1] && d.spacing[2] == g.spacing[2];
+}
+
+mgf_status mgf_scene_apply_preconditioner(mgf_scene s, const float* r, float* z, double* rz) {
+    if (!s || !r || !z || r == z) return fail(MGF_ERR_INVALID, "null or aliased buffers");
+    const auto& d = s->desc;
+    if (d.precision != MGF_PRECISION_MIXED || d.preconditioner == MGF_PRECOND_IDENTITY)
+        return fail(MGF_ERR_UNSUPPORTED, "only the MIXED solver's JACOBI or MULTIGRID preconditioner");
+    MGF_HIP(hipSetDevice(s->ctx->device));
+    hipStream_t st = s->stream;
+    float sa[3], sg[3];
+    scales(s, sa, sg);
+    // the markers and codes of a pressure solve of the current state
+    MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[s->cur], s->markers));
+    MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
+    TRY(ensure_precond(s));
+    TRY(mg_setup(s, sa));
+    CgScalars hs{};
+    MGF_HIP(hipMemcpyAsync(s->scal, &hs, sizeof(CgScalars), hipMemcpyHostToDevice, st));
+    MGF_HIP(hipMemcpyAsync(s->r, r, sizeof(float) * s->ncell, hipMemcpyDeviceToDevice, st));
+    const float* zv = nullptr;
+    TRY(apply_precond(s, sa, &zv));
+    MGF_HIP(hipMemcpyAsync(z, zv, sizeof(float) * s->ncell, hipMemcpyDeviceToDevice, st));
+    MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+    TRY(sync(s));
+    if (rz) *rz = hs.rz;
+    return MGF_OK;
 }
 
 mgf_status mgf_scene_generate_particles(mgf_scene s, int32_t per_cell, uint64_t seed, mgf_particles* out) {

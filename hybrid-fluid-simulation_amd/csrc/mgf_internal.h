@@ -293,6 +293,17 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
                                     CgScalars* sc, ReduceBuf red, int with_rz);
 // The V-cycle of levels[0..nlev) in one work-group with the iterates in LDS:
 // b0 = right-hand side of levels[0] (global), x0 = its result (global).
+// Fused descent (the first three sweeps from zero and the restriction of the
+// residual) and ascent (prolongation and three sweeps; with_rz: r . z) of a
+// level whose every axis is halved (mg_march_ok).  mg_march_grid = blocks of
+// one launch (the partials the ascent writes).
+bool mg_march_ok(const MgDims& f);
+uint32_t mg_march_grid(const MgDims& f, const MgDims& c);
+hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                             float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc);
+hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                            const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
+                            int with_rz);
 size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev);
 hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
                                   const float* b0, float* x0, int nu, int last_sweeps, float omega,

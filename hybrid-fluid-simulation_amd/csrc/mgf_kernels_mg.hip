@@ -19,6 +19,8 @@
 // one-block kernel sums the partials in a fixed order and updates the CG
 // scalars.  The kernel boundary makes the partials visible: no device-scope
 // fence (which writes back and invalidates the XCD's L2) is executed per block.
+#include <algorithm>
+
 #include "mgf_internal.h"
 #include "mgf_kcommon.h"
 
@@ -1110,6 +1112,302 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
 }
 
 // ===========================================================================
+// Fused descent and ascent of one level (every axis halved, three sweeps),
+// marching along z.
+//
+// A block owns a tile of kMTX x kMTY columns and the planes [za, za + chunk)
+// of the level.  Its 64 x 16 work-items are the tile's columns and a ring of
+// kMH = 3 columns around it (lane = x, wave = y).  The block walks up the
+// planes, one per step: the step that loads plane q forms stage 0 (pointwise)
+// at q, stage 1 at q - 1, stage 2 at q - 2 and stage 3 at q - 3.  A stage reads
+// the previous stage at the work-item's own column from registers (the
+// planes below, and the plane above, formed earlier in the same step) and at
+// the four xy neighbours from LDS, where it was written one step earlier.
+// Each stage has two LDS slots that alternate with the step's parity, so one
+// barrier per step orders every write before its reads and every read before
+// the slot is written again.  Stage s is exact on the columns at least s
+// cells inside the ring's edge, so stage 3 is exact on the tile; what the
+// ring's outer columns hold is never stored.  Cells outside the level read as
+// code 0 and b = x = 0, so every stage is 0 there.
+//
+// Descent (PRE): stage 0 = x1 = omega D^-1 b, stages 1 and 2 = the sweeps x2
+// and x3 (x3 is stored), stage 3 = the residual b - A x3 of the fluid cells,
+// summed over each 2 x 2 x 2 block of children into the coarse right-hand
+// side (pairs along x, then y, then z; weight 1/8).  Every value is formed
+// by the same operations in the same order as in k_mg_smooth2_first,
+// k_mg_smooth and the restriction kernels, so the results are bitwise
+// theirs.
+// Ascent (POST): stage 0 = x3 + xc[parent] (the prolongation by injection),
+// stages 1-3 = three sweeps, stage 3 stored; with RZ (level 0) also the block
+// partials of b . x.  Bitwise k_mg_smooth_prolong's and k_mg_smooth's values.
+constexpr int kMX = 64, kMY = 16, kMH = 3;
+constexpr int kMTX = kMX - 2 * kMH, kMTY = kMY - 2 * kMH;   // 58 x 10 tile
+constexpr int kMThreads = kMX * kMY;
+
+struct March {
+    int nx, ny, nz;          // level extents
+    uint32_t sy, sz;         // level strides
+    int cnz;                 // coarse extent along z
+    uint32_t csy, csz;       // coarse strides
+    uint32_t tx, ty;         // tiles along x and y
+    uint32_t blocks;         // tiles x chunks; blocks past it idle
+    int chunk;               // planes per block (even)
+};
+
+// Per-work-item constants of a march.
+struct MarchCol {
+    bool in;                 // the column lies in the level
+    bool tile;               // ... and in the tile (stage 3 exact)
+    bool owner;              // tile column at an even x and y of the tile (PRE)
+    uint32_t col, pcol;      // column offset in the level and of its parent
+    int za, zs, zr;          // first plane, end of the stored planes, end of the pairs
+    int me, xm, xp, ym, yp;  // LDS index of the column and of its xy neighbours
+    int wy;                  // the wave's row (wave-uniform)
+};
+
+// The rolling state of a work-item: codes and b of the planes q-1 .. q-3 and
+// each stage at the two planes below the one it forms next.
+struct MarchRing {
+    uint16_t c1, c2, c3;
+    float b1, b2, b3;
+    float v0a, v0b;          // stage 0 at q-1, q-2
+    float v1a, v1b;          // stage 1 at q-2, q-3
+    float v2a, v2b;          // stage 2 at q-3, q-4
+    float pe;                // PRE: pair sum of the last even plane
+    double acc;              // POST: sum of b . x over the stored cells
+};
+
+// One damped Jacobi sweep of a cell (the flat kernels' formula).
+__device__ __forceinline__ float jacobi(uint16_t cd, float bc, float vc, const float (&v)[6], const MgLevel& L,
+                                        const float* aw, const float* w) {
+    const float wd = wdiag(cd, w);
+    const float ax = arow(cd, vc, v, L.sx, L.sy, L.sz, aw);
+    return (cd & kCodeFluid) ? vc + wd * (bc - ax) : 0.0f;
+}
+
+// A fluid cell whose six neighbours are all fluid: arow with every mask all
+// ones and no air (aw[0]), the same operations in the same order.
+constexpr uint16_t kCodeFull = uint16_t(kCodeFluid | (63u << kCplShift));
+__device__ __forceinline__ float arow_full(float vc, const float (&v)[6], const MgLevel& L, float aw0) {
+    const float ex = (vc - v[0]) + (vc - v[1]);
+    const float ey = (vc - v[2]) + (vc - v[3]);
+    const float ez = (vc - v[4]) + (vc - v[5]);
+    return aw0 * vc + L.sx * ex + L.sy * ey + L.sz * ez;
+}
+
+// Wave-uniform shortcuts of a stage: a wave of non-fluid cells gives 0
+// without reading its neighbours; a wave of full cells skips the masks and
+// the table lookups (w[63] = omega / d of six fluid neighbours).  Bitwise
+// jacobi() in both cases.
+enum { kWaveMixed = 0, kWaveDry = 1, kWaveFull = 2 };
+__device__ __forceinline__ int wave_kind(uint16_t cd) {
+    if (__all((cd & kCodeFluid) == 0)) return kWaveDry;
+    if (__all(cd == kCodeFull)) return kWaveFull;
+    return kWaveMixed;
+}
+
+__device__ __forceinline__ float march_jacobi(uint16_t cd, float bc, float vc, const float* S, const MarchCol& K,
+                                              float vzm, float vzp, const MgLevel& L, const float* aw,
+                                              const float* w) {
+    const int kind = wave_kind(cd);
+    if (kind == kWaveDry) return 0.0f;
+    const float n[6] = {S[K.xm], S[K.xp], S[K.ym], S[K.yp], vzm, vzp};
+    if (kind == kWaveFull) return vc + w[63] * (bc - arow_full(vc, n, L, aw[0]));
+    return jacobi(cd, bc, vc, n, L, aw, w);
+}
+
+// b - A x on a fluid cell, 0 elsewhere (the restriction kernels' residual).
+__device__ __forceinline__ float march_residual(uint16_t cd, float bc, float vc, const float* S, const MarchCol& K,
+                                                float vzm, float vzp, const MgLevel& L, const float* aw) {
+    const int kind = wave_kind(cd);
+    if (kind == kWaveDry) return 0.0f;
+    const float n[6] = {S[K.xm], S[K.xp], S[K.ym], S[K.yp], vzm, vzp};
+    if (kind == kWaveFull) return bc - arow_full(vc, n, L, aw[0]);
+    return (cd & kCodeFluid) ? bc - arow(cd, vc, n, L.sx, L.sy, L.sz, aw) : 0.0f;
+}
+
+// Loads of plane q of the work-item's column, at the plane clamped into the
+// level (the values of a plane or column outside the level are replaced
+// where they are used).
+template <bool POST>
+__device__ __forceinline__ void march_load(const March& M, const MarchCol& K, int q,
+                                           const uint16_t* __restrict__ code, const float* __restrict__ b,
+                                           const float* __restrict__ xi, const float* __restrict__ xc,
+                                           uint16_t& cd, float& bv, float& xv, float& ev) {
+    const int qc = q < 0 ? 0 : (q >= M.nz ? M.nz - 1 : q);
+    const uint32_t i = uint32_t(qc) * M.sz + K.col;
+    cd = code[i];
+    bv = b[i];
+    if constexpr (POST) {
+        xv = xi[i];
+        ev = xc[uint32_t(qc >> 1) * M.csz + K.pcol];
+    }
+}
+
+// PRE: pair sum of the residuals of plane p (in slot RS); an odd plane
+// completes its coarse cell.
+__device__ __forceinline__ void march_pairs(const March& M, const MarchCol& K, int p, const float* RS,
+                                            const uint16_t* __restrict__ codec, float* __restrict__ bco,
+                                            MarchRing& R) {
+    if (!K.owner || p < K.za || p >= K.zr) return;
+    const int e = K.me;
+    const float P = (RS[e] + RS[e + 1]) + (RS[e + kMX] + RS[e + kMX + 1]);
+    if (((p - K.za) & 1) == 0) {
+        R.pe = P;
+        return;
+    }
+    const int Z = p >> 1;
+    if (Z >= M.cnz) return;
+    const uint32_t c = uint32_t(Z) * M.csz + K.pcol;
+    const float s = R.pe + P;
+    bco[c] = (codec[c] & kCodeFluid) ? s * 0.125f : 0.0f;
+}
+
+// LDS of a march (floats): the tables aw and w, then two slots of kMThreads
+// per stage 0, 1, 2 and (PRE) for the residuals.  One array, so that every
+// access is a constant offset from the work-item's index.
+constexpr int kLdsAw = 0, kLdsW = 64, kLdsSlots = 128;
+__device__ __forceinline__ constexpr int slot(int stage, int h) { return kLdsSlots + (2 * stage + h) * kMThreads; }
+
+// Planes loaded ahead: the loads of plane q are issued kMPre steps before
+// its step (kMPre register sets in rotation), so that about kMPre planes of
+// every block are in flight.
+constexpr int kMPre = 4;
+
+// One step (the plane q = za - 3 + t; H = t & 1 selects the LDS slots
+// written now).  cdN .. eN hold plane q on entry and plane q + kMPre on exit.
+template <bool POST, bool RZ, int H>
+__device__ __forceinline__ void march_step(const March& M, const MgLevel& L, const MarchCol& K, int t,
+                                           uint16_t& cdN, float& bN, float& xN, float& eN, MarchRing& R,
+                                           const uint16_t* __restrict__ code, const float* __restrict__ b,
+                                           const float* __restrict__ xi, const float* __restrict__ xc,
+                                           float* __restrict__ xo, const uint16_t* __restrict__ codec,
+                                           float* __restrict__ bco, float* lds) {
+    const float* aw = lds + kLdsAw;
+    const float* w = lds + kLdsW;
+    const int q = K.za - 3 + t;
+    const bool in0 = K.in && unsigned(q) < unsigned(M.nz);
+    const uint16_t cd0 = in0 ? cdN : uint16_t(0);
+    const float b0 = in0 ? bN : 0.0f;
+    // stage 0 at q
+    const float v0 = POST ? (in0 ? xN + eN : 0.0f) : wdiag(cd0, w) * b0;
+    march_load<POST>(M, K, q + kMPre, code, b, xi, xc, cdN, bN, xN, eN);
+    lds[slot(0, H) + K.me] = v0;
+    // Stage s is needed on the rows at least s from the block's edge only:
+    // waves outside skip it (what their registers and slots then hold is
+    // never read by a needed value).
+    float v1 = 0.0f, v2 = 0.0f;
+    // stage 1 at q - 1
+    if (K.wy >= 1 && K.wy < kMY - 1) {
+        v1 = march_jacobi(R.c1, R.b1, R.v0a, lds + slot(0, H ^ 1), K, R.v0b, v0, L, aw, w);
+        lds[slot(1, H) + K.me] = v1;
+    }
+    // stage 2 at q - 2
+    if (K.wy >= 2 && K.wy < kMY - 2) {
+        v2 = march_jacobi(R.c2, R.b2, R.v1a, lds + slot(1, H ^ 1), K, R.v1b, v1, L, aw, w);
+        lds[slot(2, H) + K.me] = v2;
+    }
+    // stage 3 at q - 3
+    if (K.wy >= kMH && K.wy < kMY - kMH) {
+        const float* S = lds + slot(2, H ^ 1);
+        if constexpr (POST) {
+            const float v3 = march_jacobi(R.c3, R.b3, R.v2a, S, K, R.v2b, v2, L, aw, w);
+            const int p = q - 3;
+            if (K.tile && p >= K.za && p < K.zs) {
+                xo[uint32_t(p) * M.sz + K.col] = v3;
+                if (RZ) R.acc += double(R.b3) * double(v3);
+            }
+        } else {
+            const int p = q - 2;
+            if (K.tile && p >= K.za && p < K.zs) xo[uint32_t(p) * M.sz + K.col] = v2;
+            lds[slot(3, H) + K.me] = march_residual(R.c3, R.b3, R.v2a, S, K, R.v2b, v2, L, aw);
+            march_pairs(M, K, q - 4, lds + slot(3, H ^ 1), codec, bco, R);
+        }
+    }
+    R.c3 = R.c2; R.c2 = R.c1; R.c1 = cd0;
+    R.b3 = R.b2; R.b2 = R.b1; R.b1 = b0;
+    R.v0b = R.v0a; R.v0a = v0;
+    R.v1b = R.v1a; R.v1a = v1;
+    R.v2b = R.v2a; R.v2a = v2;
+    __syncthreads();
+}
+
+template <bool POST, bool RZ>
+__global__ void __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* __restrict__ b,
+           const float* __restrict__ xi, const float* __restrict__ xc, float* __restrict__ xo,
+           const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
+           const CgScalars* __restrict__ sc, double* __restrict__ part) {
+    __shared__ float lds[kLdsSlots + (POST ? 6 : 8) * kMThreads];
+    __shared__ double red[kMThreads / 64];
+    if (sc->done) return;
+    const uint32_t blk = xcd_block();
+    if (blk >= M.blocks) {
+        if (RZ && threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
+    build_tables<true>(lds + kLdsAw, lds + kLdsW, L.sx, L.sy, L.sz, omega);
+    MarchCol K;
+    {
+        const uint32_t tx = blk % M.tx, rest = blk / M.tx;
+        const uint32_t ty = rest % M.ty, tc = rest / M.ty;
+        const int cx = int(threadIdx.x) & (kMX - 1), cy = int(threadIdx.x) / kMX;
+        const int gx = int(tx) * kMTX - kMH + cx, gy = int(ty) * kMTY - kMH + cy;
+        K.in = unsigned(gx) < unsigned(M.nx) && unsigned(gy) < unsigned(M.ny);
+        K.tile = K.in && cx >= kMH && cx < kMH + kMTX && cy >= kMH && cy < kMH + kMTY;
+        K.owner = !POST && K.tile && ((cx - kMH) & 1) == 0 && ((cy - kMH) & 1) == 0;
+        K.col = K.in ? uint32_t(gy) * M.sy + uint32_t(gx) : 0u;
+        K.pcol = K.in ? uint32_t(gy >> 1) * M.csy + uint32_t(gx >> 1) : 0u;
+        K.za = int(tc) * M.chunk;
+        K.zr = K.za + M.chunk;
+        K.zs = K.zr < M.nz ? K.zr : M.nz;
+        K.me = int(threadIdx.x);
+        K.xm = cx > 0 ? K.me - 1 : K.me;
+        K.xp = cx < kMX - 1 ? K.me + 1 : K.me;
+        K.ym = cy > 0 ? K.me - kMX : K.me;
+        K.yp = cy < kMY - 1 ? K.me + kMX : K.me;
+        K.wy = __builtin_amdgcn_readfirstlane(cy);
+    }
+    MarchRing R;
+    R.c1 = R.c2 = R.c3 = 0;
+    R.b1 = R.b2 = R.b3 = 0.0f;
+    R.v0a = R.v0b = R.v1a = R.v1b = R.v2a = R.v2b = 0.0f;
+    R.pe = 0.0f;
+    R.acc = 0.0;
+    uint16_t cd[kMPre];
+    float bv[kMPre], xv[kMPre], ev[kMPre];
+#pragma unroll
+    for (int k = 0; k < kMPre; ++k) {
+        xv[k] = ev[k] = 0.0f;
+        march_load<POST>(M, K, K.za - 3 + k, code, b, xi, xc, cd[k], bv[k], xv[k], ev[k]);
+    }
+    // Stage 3 reaches the last plane za + chunk - 1 at step chunk + 5, and
+    // (PRE) its pair sums one step later; the steps are rounded up to whole
+    // rotations (the extra planes store nothing).
+    const int steps = (M.chunk + (POST ? 6 : 7) + kMPre - 1) / kMPre * kMPre;
+    for (int t = 0; t < steps; t += kMPre) {
+#pragma unroll
+        for (int k = 0; k < kMPre; k += 2) {
+            march_step<POST, RZ, 0>(M, L, K, t + k, cd[k], bv[k], xv[k], ev[k], R, code, b, xi, xc, xo, codec,
+                                    bco, lds);
+            march_step<POST, RZ, 1>(M, L, K, t + k + 1, cd[k + 1], bv[k + 1], xv[k + 1], ev[k + 1], R, code, b, xi,
+                                    xc, xo, codec, bco, lds);
+        }
+    }
+    if constexpr (RZ) {
+        const double a = wave_sum(R.acc);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < kMThreads / 64; ++q) t += red[q];
+            part[blockIdx.x] = t;
+        }
+    }
+}
+
+// ===========================================================================
 // Launchers.
 static uint32_t flat_grid(uint32_t n) {
     size_t g = (size_t(n) + size_t(kBlock) * kCpt - 1) / (size_t(kBlock) * kCpt);
@@ -1238,6 +1536,70 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
         e = LAUNCH(k_mg_smooth_prolong<true>, G, F, f, P, code, b, xi, xc, xo, omega, sc, red.partials);
     else
         e = LAUNCH(k_mg_smooth_prolong<false>, G, F, f, P, code, b, xi, xc, xo, omega, sc, red.partials);
+    if (e != hipSuccess || !with_rz) return e;
+    return finish(s, red.partials, G, sc, kFinishRz);
+}
+
+// Fused descent / ascent.  Chunks of about kMarchChunk planes; more chunks
+// (down to 8 planes) until the grid holds about kMarchBlocks blocks, two per
+// CU.
+constexpr int kMarchChunk = 32;
+constexpr uint32_t kMarchBlocks = 512;
+
+bool mg_march_ok(const MgDims& f) { return f.nx > 1 && f.ny > 1 && f.nz > 1; }
+
+static March make_march(const MgDims& f, const MgDims& c) {
+    March M;
+    M.nx = f.nx; M.ny = f.ny; M.nz = f.nz;
+    M.sy = uint32_t(f.nx);
+    M.sz = uint32_t(f.nx) * uint32_t(f.ny);
+    M.cnz = c.nz;
+    M.csy = uint32_t(c.nx);
+    M.csz = uint32_t(c.nx) * uint32_t(c.ny);
+    M.tx = uint32_t((f.nx + kMTX - 1) / kMTX);
+    M.ty = uint32_t((f.ny + kMTY - 1) / kMTY);
+    const uint32_t xy = M.tx * M.ty;
+    int chunks = (f.nz + kMarchChunk - 1) / kMarchChunk;
+    const int want = int((kMarchBlocks + xy - 1) / xy);
+    chunks = std::max(chunks, std::min(want, (f.nz + 7) / 8));
+    int C = (f.nz + chunks - 1) / chunks;
+    C += C & 1;
+    M.chunk = C;
+    M.blocks = xy * uint32_t((f.nz + C - 1) / C);
+    return M;
+}
+
+uint32_t mg_march_grid(const MgDims& f, const MgDims& c) {
+    const March M = make_march(f, c);
+    return (M.blocks + kXcds - 1) / kXcds * kXcds;
+}
+
+hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                             float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc) {
+    if (!mg_march_ok(f)) return hipErrorInvalidValue;
+    const March M = make_march(f, c);
+    const uint32_t G = mg_march_grid(f, c);
+    return launch_dims(k_mg_march<false, false>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b,
+                       static_cast<const float*>(nullptr), static_cast<const float*>(nullptr), x, codec, bc, omega,
+                       sc, static_cast<double*>(nullptr));
+}
+
+hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                            const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
+                            int with_rz) {
+    if (!mg_march_ok(f)) return hipErrorInvalidValue;
+    const March M = make_march(f, c);
+    const uint32_t G = mg_march_grid(f, c);
+    const CgScalars* csc = sc;
+    hipError_t e;
+    if (with_rz)
+        e = launch_dims(k_mg_march<true, true>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
+                        static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
+                        red.partials);
+    else
+        e = launch_dims(k_mg_march<true, false>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
+                        static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
+                        static_cast<double*>(nullptr));
     if (e != hipSuccess || !with_rz) return e;
     return finish(s, red.partials, G, sc, kFinishRz);
 }

@@ -19,10 +19,10 @@ RK1, RK2, RK3 = 1, 2, 3
 TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
-PROBE_COUNT = 8  # MGF_PROBE_COUNT
+PROBE_COUNT = 10  # MGF_PROBE_COUNT
 PG = dict(p2gsum=0, g2pnearest=1, linear=2, quadratic=3, cubic=4)   # MGF_PG_*
 PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "k_mg_smooth_prolong",
-               "k_mg_restrict", "k_ext_layer", "k_advect_fc")
+               "k_mg_restrict", "k_ext_layer", "k_advect_fc", "k_mg_descend", "k_mg_ascend")
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
              solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
              divergence=12, color_x=13, color_y=14, color_z=15, fluid_sdf=16, fluid_density=17)
@@ -37,7 +37,8 @@ class SceneDesc(C.Structure):
                 ("max_iterations", C.c_int64), ("precision", C.c_int32),
                 ("preconditioner", C.c_int32), ("extrapolation_layers", C.c_int32),
                 ("max_refinements", C.c_int32), ("inner_tolerance", C.c_double),
-                ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float), ("warm_start", C.c_int32)]
+                ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float), ("warm_start", C.c_int32),
+                ("mg_per_sweep", C.c_int32)]
 
 
 class StepStats(C.Structure):
@@ -154,6 +155,7 @@ _SIGS = {
     "mgf_scene_pressure_solve_particles": [_VP, _VP, C.POINTER(StepStats)],
     "mgf_step_particles": [_VP, _VP, C.c_int32, C.POINTER(StepStats)],
     "mgf_scene_generate_particles": [_VP, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)],
+    "mgf_scene_apply_preconditioner": [_VP, _VP, _VP, C.POINTER(C.c_double)],
     "mgf_op_scale": [_VP, C.c_int64, C.c_float, _VP],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
@@ -254,7 +256,7 @@ def desc_defaults(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0):
 def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode=TOL_ABS_SQRT_NORM,
               tol=1e-3, max_it=0, precision=PRECISION_PLAIN, layers=1000, gravity=(0, -9.8, 0),
               force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3,
-              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9, warm_start=False):
+              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9, warm_start=False, mg_per_sweep=False):
     d = desc_defaults(res, origin, spacing, dt)
     d.rk_order = rk
     d.tol_mode = tol_mode
@@ -271,6 +273,7 @@ def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode
     d.mg_sweeps = mg_sweeps
     d.mg_omega = mg_omega
     d.warm_start = 1 if warm_start else 0
+    d.mg_per_sweep = 1 if mg_per_sweep else 0
     return d
 
 
@@ -346,6 +349,20 @@ class Scene:
         if not (allow_unconverged and rc == ERR_NOT_CONVERGED):
             check(rc, "mgf_scene_pressure_solve")
         return st.as_dict()
+
+    def apply_preconditioner(self, r):
+        """z = M^-1 r once (mgf_scene_apply_preconditioner) for a host array r
+        of nx*ny*nz floats; returns (z, r . z)."""
+        r = f32(r).ravel()
+        R = self.ctx.to_device(r)
+        Z = self.ctx.alloc(4 * r.size)
+        rz = C.c_double()
+        try:
+            check(lib().mgf_scene_apply_preconditioner(self.h, R, Z, C.byref(rz)), "mgf_scene_apply_preconditioner")
+            return self.ctx.to_host(Z, r.size), rz.value
+        finally:
+            self.ctx.free(R)
+            self.ctx.free(Z)
 
     def extrapolate(self):
         n = C.c_int32()

@@ -150,6 +150,10 @@ typedef struct mgf_scene_desc {
                                   instead of x = 0 (PressureSolver.cpp:145).
                                   Same stop rule, different iterates; off by
                                   default                                    */
+    int32_t mg_per_sweep;      /* MULTIGRID: 1 = one kernel per sweep on every
+                                  level; 0 (default) = with 3 sweeps, each
+                                  3-D level's descent and ascent in one fused
+                                  kernel each.  Bitwise the same z = M^-1 r  */
 } mgf_scene_desc;
 
 /* Per-call statistics (summed over the steps of one mgf_step call). */
@@ -267,10 +271,21 @@ MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
 #define MGF_PROBE_MG_RESTRICT 5  /* level-0 restriction                        */
 #define MGF_PROBE_EXT_LAYER   6  /* one extrapolation layer                    */
 #define MGF_PROBE_ADVECT_FC   7  /* face-velocity advection (+ forces)         */
-#define MGF_PROBE_COUNT       8
+#define MGF_PROBE_MG_DESCEND  8  /* level-0 fused sweeps + restriction        */
+#define MGF_PROBE_MG_ASCEND   9  /* level-0 fused prolongation + sweeps       */
+#define MGF_PROBE_COUNT       10
 MGF_API mgf_status mgf_scene_set_kernel_probes(mgf_scene scene, int32_t enabled);
 MGF_API mgf_status mgf_scene_kernel_probes(mgf_scene scene, double* ms, int64_t* launches, double* bytes,
                                            int32_t reset);
+
+/* The MIXED solver's preconditioner applied once (diagnostic; the reference's
+ * CG has none, CuMatrixFreePCG.h:27-28): z = M^-1 r for the markers of the
+ * scene's current state (as mgf_scene_pressure_solve forms them), with the
+ * scene's preconditioner settings.  r and z are device buffers of nx*ny*nz
+ * floats (distinct); *rz (may be null) receives r . z in FP64 as the solver
+ * forms it.  MGF_ERR_UNSUPPORTED unless precision is MIXED and the
+ * preconditioner is not IDENTITY. */
+MGF_API mgf_status mgf_scene_apply_preconditioner(mgf_scene scene, const float* r, float* z, double* rz);
 
 /* ---- batched scenes ------------------------------------------------------ */
 /* No counterpart in the reference, which steps one CGridFluidSolver at a time
