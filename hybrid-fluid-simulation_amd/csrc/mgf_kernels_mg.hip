@@ -1140,9 +1140,13 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
 // Ascent (POST): stage 0 = x3 + xc[parent] (the prolongation by injection),
 // stages 1-3 = three sweeps, stage 3 stored; with RZ (level 0) also the block
 // partials of b . x.  Bitwise k_mg_smooth_prolong's and k_mg_smooth's values.
-constexpr int kMX = 64, kMY = 16, kMH = 3;
-constexpr int kMTX = kMX - 2 * kMH, kMTY = kMY - 2 * kMH;   // 58 x 10 tile
-constexpr int kMThreads = kMX * kMY;
+#ifndef MGF_MARCH_X
+#define MGF_MARCH_X 32
+#endif
+constexpr int kMThreads = 1024;
+constexpr int kMX = MGF_MARCH_X, kMY = kMThreads / kMX, kMH = 3;
+constexpr int kMRows = 64 / kMX;                             // rows per wave
+constexpr int kMTX = kMX - 2 * kMH, kMTY = kMY - 2 * kMH;   // the tile
 
 struct March {
     int nx, ny, nz;          // level extents
@@ -1162,14 +1166,16 @@ struct MarchCol {
     uint32_t col, pcol;      // column offset in the level and of its parent
     int za, zs, zr;          // first plane, end of the stored planes, end of the pairs
     int me, xm, xp, ym, yp;  // LDS index of the column and of its xy neighbours
-    int wy;                  // the wave's row (wave-uniform)
+    int wy;                  // the wave (wave-uniform); it holds rows kMRows wy ..
 };
 
 // The rolling state of a work-item: codes and b of the planes q-1 .. q-3 and
 // each stage at the two planes below the one it forms next.
 struct MarchRing {
-    uint16_t c1, c2, c3;
+    uint32_t c1, c2, c3;
     float b1, b2, b3;
+    float w1, w2, w3;        // omega / d of the planes q-1 .. q-3
+    float a1, a2, a3;        // air weight aw[air mask] of the planes q-1 .. q-3
     float v0a, v0b;          // stage 0 at q-1, q-2
     float v1a, v1b;          // stage 1 at q-2, q-3
     float v2a, v2b;          // stage 2 at q-3, q-4
@@ -1195,35 +1201,62 @@ __device__ __forceinline__ float arow_full(float vc, const float (&v)[6], const 
     return aw0 * vc + L.sx * ex + L.sy * ey + L.sz * ez;
 }
 
+// keep_if with the mask formed by one bit-field extract (the compiler would
+// otherwise test the bit and select, three instructions instead of two).
+__device__ __forceinline__ float keep_bit(float v, uint32_t cd, int b) {
+    int32_t m;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(m) : "v"(cd), "i"(b));
+    return __int_as_float(__float_as_int(v) & m);
+}
+
+// arow with the air weight awc = aw[air_mask(cd)] of the cell given.
+__device__ __forceinline__ float arow_w(uint32_t cd, float awc, float vc, const float (&v)[6], const MgLevel& L) {
+    const float ex = keep_bit(vc - v[0], cd, kCplShift + 0) + keep_bit(vc - v[1], cd, kCplShift + 1);
+    const float ey = keep_bit(vc - v[2], cd, kCplShift + 2) + keep_bit(vc - v[3], cd, kCplShift + 3);
+    const float ez = keep_bit(vc - v[4], cd, kCplShift + 4) + keep_bit(vc - v[5], cd, kCplShift + 5);
+    const float a = awc * vc + L.sx * ex + L.sy * ey + L.sz * ez;
+    return (cd & kCodeFluid) ? a : vc;
+}
+
 // Wave-uniform shortcuts of a stage: a wave of non-fluid cells gives 0
-// without reading its neighbours; a wave of full cells skips the masks and
-// the table lookups (w[63] = omega / d of six fluid neighbours).  Bitwise
-// jacobi() in both cases.
+// without reading its neighbours; a wave of full cells skips the masks.
+// Otherwise jacobi() / the restriction's residual with the cell's omega / d
+// (wd) and air weight (awc) formed once per plane.  Bitwise the same values.
 enum { kWaveMixed = 0, kWaveDry = 1, kWaveFull = 2 };
-__device__ __forceinline__ int wave_kind(uint16_t cd) {
+__device__ __forceinline__ int wave_kind(uint32_t cd) {
     if (__all((cd & kCodeFluid) == 0)) return kWaveDry;
     if (__all(cd == kCodeFull)) return kWaveFull;
     return kWaveMixed;
 }
 
-__device__ __forceinline__ float march_jacobi(uint16_t cd, float bc, float vc, const float* S, const MarchCol& K,
-                                              float vzm, float vzp, const MgLevel& L, const float* aw,
-                                              const float* w) {
-    const int kind = wave_kind(cd);
-    if (kind == kWaveDry) return 0.0f;
-    const float n[6] = {S[K.xm], S[K.xp], S[K.ym], S[K.yp], vzm, vzp};
-    if (kind == kWaveFull) return vc + w[63] * (bc - arow_full(vc, n, L, aw[0]));
-    return jacobi(cd, bc, vc, n, L, aw, w);
+// The four xy neighbours of the work-item in an LDS slot.
+__device__ __forceinline__ void xy_nbrs(const float* S, const MarchCol& K, float (&n)[4]) {
+    n[0] = S[K.xm];
+    n[1] = S[K.xp];
+    n[2] = S[K.ym];
+    n[3] = S[K.yp];
 }
 
-// b - A x on a fluid cell, 0 elsewhere (the restriction kernels' residual).
-__device__ __forceinline__ float march_residual(uint16_t cd, float bc, float vc, const float* S, const MarchCol& K,
-                                                float vzm, float vzp, const MgLevel& L, const float* aw) {
+__device__ __forceinline__ float march_jacobi(uint32_t cd, float wd, float awc, float bc, float vc, const float* S,
+                                              const MarchCol& K, float vzm, float vzp, const MgLevel& L) {
     const int kind = wave_kind(cd);
     if (kind == kWaveDry) return 0.0f;
-    const float n[6] = {S[K.xm], S[K.xp], S[K.ym], S[K.yp], vzm, vzp};
-    if (kind == kWaveFull) return bc - arow_full(vc, n, L, aw[0]);
-    return (cd & kCodeFluid) ? bc - arow(cd, vc, n, L.sx, L.sy, L.sz, aw) : 0.0f;
+    float xy[4];
+    xy_nbrs(S, K, xy);
+    const float n[6] = {xy[0], xy[1], xy[2], xy[3], vzm, vzp};
+    if (kind == kWaveFull) return vc + wd * (bc - arow_full(vc, n, L, awc));
+    return (cd & kCodeFluid) ? vc + wd * (bc - arow_w(cd, awc, vc, n, L)) : 0.0f;
+}
+
+__device__ __forceinline__ float march_residual(uint32_t cd, float awc, float bc, float vc, const float* S,
+                                                const MarchCol& K, float vzm, float vzp, const MgLevel& L) {
+    const int kind = wave_kind(cd);
+    if (kind == kWaveDry) return 0.0f;
+    float xy[4];
+    xy_nbrs(S, K, xy);
+    const float n[6] = {xy[0], xy[1], xy[2], xy[3], vzm, vzp};
+    if (kind == kWaveFull) return bc - arow_full(vc, n, L, awc);
+    return (cd & kCodeFluid) ? bc - arow_w(cd, awc, vc, n, L) : 0.0f;
 }
 
 // Loads of plane q of the work-item's column, at the plane clamped into the
@@ -1263,6 +1296,11 @@ __device__ __forceinline__ void march_pairs(const March& M, const MarchCol& K, i
     bco[c] = (codec[c] & kCodeFluid) ? s * 0.125f : 0.0f;
 }
 
+// Whether the wave wy holds a row at least s rows from the block's edge.
+__device__ __forceinline__ bool rows_need(int wy, int s) {
+    return wy * kMRows + kMRows - 1 >= s && wy * kMRows < kMY - s;
+}
+
 // LDS of a march (floats): the tables aw and w, then two slots of kMThreads
 // per stage 0, 1, 2 and (PRE) for the residuals.  One array, so that every
 // access is a constant offset from the work-item's index.
@@ -1272,7 +1310,7 @@ __device__ __forceinline__ constexpr int slot(int stage, int h) { return kLdsSlo
 // Planes loaded ahead: the loads of plane q are issued kMPre steps before
 // its step (kMPre register sets in rotation), so that about kMPre planes of
 // every block are in flight.
-constexpr int kMPre = 4;
+constexpr int kMPre = 2;
 
 // One step (the plane q = za - 3 + t; H = t & 1 selects the LDS slots
 // written now).  cdN .. eN hold plane q on entry and plane q + kMPre on exit.
@@ -1287,10 +1325,12 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     const float* w = lds + kLdsW;
     const int q = K.za - 3 + t;
     const bool in0 = K.in && unsigned(q) < unsigned(M.nz);
-    const uint16_t cd0 = in0 ? cdN : uint16_t(0);
+    const uint32_t cd0 = in0 ? uint32_t(cdN) : 0u;
     const float b0 = in0 ? bN : 0.0f;
+    const float wd0 = wdiag(uint16_t(cd0), w);
+    const float aw0 = aw[air_mask(uint16_t(cd0))];
     // stage 0 at q
-    const float v0 = POST ? (in0 ? xN + eN : 0.0f) : wdiag(cd0, w) * b0;
+    const float v0 = POST ? (in0 ? xN + eN : 0.0f) : wd0 * b0;
     march_load<POST>(M, K, q + kMPre, code, b, xi, xc, cdN, bN, xN, eN);
     lds[slot(0, H) + K.me] = v0;
     // Stage s is needed on the rows at least s from the block's edge only:
@@ -1298,20 +1338,20 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     // never read by a needed value).
     float v1 = 0.0f, v2 = 0.0f;
     // stage 1 at q - 1
-    if (K.wy >= 1 && K.wy < kMY - 1) {
-        v1 = march_jacobi(R.c1, R.b1, R.v0a, lds + slot(0, H ^ 1), K, R.v0b, v0, L, aw, w);
+    if (rows_need(K.wy, 1)) {
+        v1 = march_jacobi(R.c1, R.w1, R.a1, R.b1, R.v0a, lds + slot(0, H ^ 1), K, R.v0b, v0, L);
         lds[slot(1, H) + K.me] = v1;
     }
     // stage 2 at q - 2
-    if (K.wy >= 2 && K.wy < kMY - 2) {
-        v2 = march_jacobi(R.c2, R.b2, R.v1a, lds + slot(1, H ^ 1), K, R.v1b, v1, L, aw, w);
+    if (rows_need(K.wy, 2)) {
+        v2 = march_jacobi(R.c2, R.w2, R.a2, R.b2, R.v1a, lds + slot(1, H ^ 1), K, R.v1b, v1, L);
         lds[slot(2, H) + K.me] = v2;
     }
     // stage 3 at q - 3
-    if (K.wy >= kMH && K.wy < kMY - kMH) {
+    if (rows_need(K.wy, 3)) {
         const float* S = lds + slot(2, H ^ 1);
         if constexpr (POST) {
-            const float v3 = march_jacobi(R.c3, R.b3, R.v2a, S, K, R.v2b, v2, L, aw, w);
+            const float v3 = march_jacobi(R.c3, R.w3, R.a3, R.b3, R.v2a, S, K, R.v2b, v2, L);
             const int p = q - 3;
             if (K.tile && p >= K.za && p < K.zs) {
                 xo[uint32_t(p) * M.sz + K.col] = v3;
@@ -1320,12 +1360,14 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
         } else {
             const int p = q - 2;
             if (K.tile && p >= K.za && p < K.zs) xo[uint32_t(p) * M.sz + K.col] = v2;
-            lds[slot(3, H) + K.me] = march_residual(R.c3, R.b3, R.v2a, S, K, R.v2b, v2, L, aw);
+            lds[slot(3, H) + K.me] = march_residual(R.c3, R.a3, R.b3, R.v2a, S, K, R.v2b, v2, L);
             march_pairs(M, K, q - 4, lds + slot(3, H ^ 1), codec, bco, R);
         }
     }
     R.c3 = R.c2; R.c2 = R.c1; R.c1 = cd0;
     R.b3 = R.b2; R.b2 = R.b1; R.b1 = b0;
+    R.w3 = R.w2; R.w2 = R.w1; R.w1 = wd0;
+    R.a3 = R.a2; R.a2 = R.a1; R.a1 = aw0;
     R.v0b = R.v0a; R.v0a = v0;
     R.v1b = R.v1a; R.v1a = v1;
     R.v2b = R.v2a; R.v2a = v2;
@@ -1366,11 +1408,13 @@ k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* _
         K.xp = cx < kMX - 1 ? K.me + 1 : K.me;
         K.ym = cy > 0 ? K.me - kMX : K.me;
         K.yp = cy < kMY - 1 ? K.me + kMX : K.me;
-        K.wy = __builtin_amdgcn_readfirstlane(cy);
+        K.wy = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
     }
     MarchRing R;
-    R.c1 = R.c2 = R.c3 = 0;
+    R.c1 = R.c2 = R.c3 = 0u;
     R.b1 = R.b2 = R.b3 = 0.0f;
+    R.w1 = R.w2 = R.w3 = 0.0f;
+    R.a1 = R.a2 = R.a3 = 0.0f;
     R.v0a = R.v0b = R.v1a = R.v1b = R.v2a = R.v2b = 0.0f;
     R.pe = 0.0f;
     R.acc = 0.0;
@@ -1543,7 +1587,10 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
 // Fused descent / ascent.  Chunks of about kMarchChunk planes; more chunks
 // (down to 8 planes) until the grid holds about kMarchBlocks blocks, two per
 // CU.
-constexpr int kMarchChunk = 32;
+#ifndef MGF_MARCH_CHUNK
+#define MGF_MARCH_CHUNK 32
+#endif
+constexpr int kMarchChunk = MGF_MARCH_CHUNK;
 constexpr uint32_t kMarchBlocks = 512;
 
 bool mg_march_ok(const MgDims& f) { return f.nx > 1 && f.ny > 1 && f.nz > 1; }
