@@ -452,9 +452,13 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
     // descent: pre-smoothing from zero (the first two sweeps in one pass),
     // restriction of the residual
+    // fused descent / ascent on the 3-D levels; the level-0 ascent writes one
+    // partial of r . z per block into the scene's partials buffer
     const bool fuse = nu == 3 && !d.mg_per_sweep;
+    const size_t part_cap = size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2;
     auto fused = [&](int l) {
-        return fuse && l + 1 < L && mg_march_ok(lv[l].d) && mg_march_grid(lv[l].d, lv[l + 1].d) <= kMaxBlocks;
+        return fuse && l + 1 < L && mg_march_ok(lv[l].d) &&
+               (l > 0 || mg_march_grid(lv[l].d, lv[l + 1].d) <= part_cap);
     };
     for (int l = 0; l < l0; ++l) {
         Lv& v = lv[l];
