@@ -95,7 +95,8 @@ mgf_status validate_desc(const mgf_scene_desc* d) {
         return fail(MGF_ERR_UNSUPPORTED, "PLAIN precision runs the reference's identity-preconditioned CG only");
     if (d->mg_sweeps > 8) return fail(MGF_ERR_INVALID, "mg_sweeps must be <= 8");
     if (!(d->mg_omega < 2.0f) || !std::isfinite(d->mg_omega)) return fail(MGF_ERR_INVALID, "mg_omega must be < 2");
-    if (d->mg_per_sweep != 0 && d->mg_per_sweep != 1) return fail(MGF_ERR_INVALID, "mg_per_sweep must be 0 or 1");
+    if (d->mg_kernels != MGF_MG_AUTO && d->mg_kernels != MGF_MG_PER_SWEEP && d->mg_kernels != MGF_MG_FUSED)
+        return fail(MGF_ERR_INVALID, "unknown mg_kernels");
     if (d->extrapolation_layers < 0 || d->extrapolation_layers > 1000000)
         return fail(MGF_ERR_INVALID, "extrapolation_layers out of range");
     return MGF_OK;
@@ -365,6 +366,7 @@ constexpr int kCoarsestSweeps = 8;
 constexpr size_t kCoarseCells = 8192;    // levels this small run in one work-group
 constexpr int kCoarseLevelsMax = 8;
 constexpr size_t kCoarseLdsMax = 131072;
+constexpr size_t kMarchMinCells = size_t(1) << 22;   // MGF_MG_AUTO fuses levels at least this large
 
 mgf_status ensure_precond(mgf_scene s) {
     if (!s->z) {
@@ -452,13 +454,17 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
     // descent: pre-smoothing from zero (the first two sweeps in one pass),
     // restriction of the residual
-    // fused descent / ascent on the 3-D levels; the level-0 ascent writes one
-    // partial of r . z per block into the scene's partials buffer
-    const bool fuse = nu == 3 && !d.mg_per_sweep;
+    // fused descent / ascent on the 3-D levels (AUTO: the large ones, where
+    // they beat one kernel per sweep; on small levels the march's serial
+    // steps cost more than the launches they save); the level-0 ascent
+    // writes one partial of r . z per block into the scene's partials buffer
+    const bool fuse = nu == 3 && d.mg_kernels != MGF_MG_PER_SWEEP;
     const size_t part_cap = size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2;
     auto fused = [&](int l) {
-        return fuse && l + 1 < L && mg_march_ok(lv[l].d) &&
-               (l > 0 || mg_march_grid(lv[l].d, lv[l + 1].d) <= part_cap);
+        if (!fuse || l + 1 >= L || !mg_march_ok(lv[l].d)) return false;
+        const size_t n = size_t(lv[l].d.nx) * lv[l].d.ny * lv[l].d.nz;
+        if (d.mg_kernels == MGF_MG_AUTO && n < kMarchMinCells) return false;
+        return l > 0 || mg_march_grid(lv[l].d, lv[l + 1].d) <= part_cap;
     };
     for (int l = 0; l < l0; ++l) {
         Lv& v = lv[l];

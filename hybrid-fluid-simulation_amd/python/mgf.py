@@ -20,6 +20,7 @@ TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
 PROBE_COUNT = 10  # MGF_PROBE_COUNT
+MG_AUTO, MG_PER_SWEEP, MG_FUSED = 0, 1, 2   # MGF_MG_*
 PG = dict(p2gsum=0, g2pnearest=1, linear=2, quadratic=3, cubic=4)   # MGF_PG_*
 PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "k_mg_smooth_prolong",
                "k_mg_restrict", "k_ext_layer", "k_advect_fc", "k_mg_descend", "k_mg_ascend")
@@ -38,7 +39,7 @@ class SceneDesc(C.Structure):
                 ("preconditioner", C.c_int32), ("extrapolation_layers", C.c_int32),
                 ("max_refinements", C.c_int32), ("inner_tolerance", C.c_double),
                 ("mg_sweeps", C.c_int32), ("mg_omega", C.c_float), ("warm_start", C.c_int32),
-                ("mg_per_sweep", C.c_int32)]
+                ("mg_kernels", C.c_int32)]
 
 
 class StepStats(C.Structure):
@@ -256,7 +257,7 @@ def desc_defaults(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0):
 def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode=TOL_ABS_SQRT_NORM,
               tol=1e-3, max_it=0, precision=PRECISION_PLAIN, layers=1000, gravity=(0, -9.8, 0),
               force=(0, 0, 0), density=1.0, max_refinements=8, inner_tol=1e-3,
-              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9, warm_start=False, mg_per_sweep=False):
+              precond=PRECOND_IDENTITY, mg_sweeps=3, mg_omega=0.9, warm_start=False, mg_kernels=0):
     d = desc_defaults(res, origin, spacing, dt)
     d.rk_order = rk
     d.tol_mode = tol_mode
@@ -273,7 +274,7 @@ def make_desc(res, origin=(0, 0, 0), spacing=(1, 1, 1), dt=1.0, rk=RK3, tol_mode
     d.mg_sweeps = mg_sweeps
     d.mg_omega = mg_omega
     d.warm_start = 1 if warm_start else 0
-    d.mg_per_sweep = 1 if mg_per_sweep else 0
+    d.mg_kernels = mg_kernels
     return d
 
 

@@ -150,11 +150,16 @@ typedef struct mgf_scene_desc {
                                   instead of x = 0 (PressureSolver.cpp:145).
                                   Same stop rule, different iterates; off by
                                   default                                    */
-    int32_t mg_per_sweep;      /* MULTIGRID: 1 = one kernel per sweep on every
-                                  level; 0 (default) = with 3 sweeps, each
-                                  3-D level's descent and ascent in one fused
-                                  kernel each.  Bitwise the same z = M^-1 r  */
+    int32_t mg_kernels;        /* MULTIGRID: MGF_MG_* (below); the same
+                                  z = M^-1 r bit for bit either way          */
 } mgf_scene_desc;
+
+/* Kernels of the multigrid V-cycle (mgf_scene_desc.mg_kernels).  With three
+ * sweeps, a 3-D level's descent (three sweeps and the restriction) and ascent
+ * (prolongation and three sweeps) can each run as one fused kernel. */
+#define MGF_MG_AUTO      0  /* fused on levels of at least 4M cells (default) */
+#define MGF_MG_PER_SWEEP 1  /* one kernel per sweep on every level            */
+#define MGF_MG_FUSED     2  /* fused on every 3-D level                       */
 
 /* Per-call statistics (summed over the steps of one mgf_step call). */
 typedef struct mgf_step_stats {

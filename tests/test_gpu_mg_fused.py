@@ -1,4 +1,5 @@
-"""Fused multigrid descent / ascent (k_mg_march) against one kernel per sweep.
+"""Fused multigrid descent / ascent (k_mg_march, forced on every 3-D level
+with MGF_MG_FUSED) against one kernel per sweep (MGF_MG_PER_SWEEP).
 
 The fused kernels form every value of the V-cycle with the same operations in
 the same order as the per-sweep kernels (k_mg_smooth2_first, k_mg_smooth,
@@ -28,13 +29,13 @@ def bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-def random_scene(ctx, res, seed, per_sweep, psolid=0.1, pfluid=0.7):
+def random_scene(ctx, res, seed, kernels, psolid=0.1, pfluid=0.7):
     rng = np.random.default_rng(seed)
     n = int(np.prod(res))
     solid = np.where(rng.random(n) < psolid, -1.0, 1.0).astype(np.float32)
     fluid = np.where(rng.random(n) < pfluid, -1.0, 1.0).astype(np.float32)
     d = mgf.make_desc(res, (0, 0, 0), (0.1, 0.2, 0.3), 0.01, tol_mode=mgf.TOL_RELATIVE, tol=1e-6,
-                      precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID, mg_per_sweep=per_sweep)
+                      precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID, mg_kernels=kernels)
     s = mgf.Scene(ctx, d)
     s.add_solid(solid)
     s.upload(3, fluid)
@@ -42,7 +43,8 @@ def random_scene(ctx, res, seed, per_sweep, psolid=0.1, pfluid=0.7):
 
 
 def both(ctx, res, seed, **kw):
-    return random_scene(ctx, res, seed, False, **kw), random_scene(ctx, res, seed, True, **kw)
+    return (random_scene(ctx, res, seed, mgf.MG_FUSED, **kw),
+            random_scene(ctx, res, seed, mgf.MG_PER_SWEEP, **kw))
 
 
 def check_same(fused, ref, r, what):
@@ -97,8 +99,8 @@ def test_fused_vcycle_obstacles_130(ctx):
     spec = scenes.obstacles(128, seed=1)
     kw = dict(rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=mgf.PRECISION_MIXED,
               precond=mgf.PRECOND_MULTIGRID)
-    f = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
-    p = scenes.build_gpu_scene(mgf, ctx, spec, mg_per_sweep=True, **kw)
+    f = scenes.build_gpu_scene(mgf, ctx, spec, mg_kernels=mgf.MG_FUSED, **kw)
+    p = scenes.build_gpu_scene(mgf, ctx, spec, mg_kernels=mgf.MG_PER_SWEEP, **kw)
     n = int(np.prod(spec.res))
     r = np.random.default_rng(3).standard_normal(n).astype(np.float32)
     check_same(f, p, r, "obstacles 130^3")
@@ -112,8 +114,8 @@ def test_fused_solve_matches_per_sweep(ctx):
     spec = scenes.obstacles(96, seed=4)
     kw = dict(rk=3, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=mgf.PRECISION_MIXED,
               precond=mgf.PRECOND_MULTIGRID)
-    f = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
-    p = scenes.build_gpu_scene(mgf, ctx, spec, mg_per_sweep=True, **kw)
+    f = scenes.build_gpu_scene(mgf, ctx, spec, mg_kernels=mgf.MG_FUSED, **kw)
+    p = scenes.build_gpu_scene(mgf, ctx, spec, mg_kernels=mgf.MG_PER_SWEEP, **kw)
     for step in range(3):
         sf, sp = f.step(1), p.step(1)
         print("step", step, sf["cg_iterations"], sp["cg_iterations"], sf["residual_rel"], sp["residual_rel"],
@@ -130,6 +132,26 @@ def test_fused_solve_matches_per_sweep(ctx):
     f.close(); p.close()
 
 
+def test_auto_fuses_large_levels_only(ctx):
+    """MGF_MG_AUTO: the 256^3 obstacle scene (17M cells) runs the fused level-0
+    kernels (the probes see their launches), a 64^3 scene none; z is bitwise
+    the per-sweep V-cycle's in both."""
+    for n, fused in ((256, True), (64, False)):
+        spec = scenes.obstacles(n, seed=1)
+        kw = dict(rk=1, tol_mode=mgf.TOL_RELATIVE, tol=1e-6, precision=mgf.PRECISION_MIXED,
+                  precond=mgf.PRECOND_MULTIGRID)
+        a = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
+        p = scenes.build_gpu_scene(mgf, ctx, spec, mg_kernels=mgf.MG_PER_SWEEP, **kw)
+        a.set_kernel_probes(True)
+        a.kernel_probes(reset=True)
+        r = np.random.default_rng(n).standard_normal(int(np.prod(spec.res))).astype(np.float32)
+        check_same(a, p, r, "auto %d^3" % n)
+        pr = a.kernel_probes(reset=True)
+        assert (pr["k_mg_descend"]["launches"] > 0) == fused, pr["k_mg_descend"]
+        assert (pr["k_mg_ascend"]["launches"] > 0) == fused, pr["k_mg_ascend"]
+        a.close(); p.close()
+
+
 def test_apply_preconditioner_arguments(ctx):
     d = mgf.make_desc((8, 8, 8), precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_IDENTITY)
     s = mgf.Scene(ctx, d)
@@ -138,6 +160,6 @@ def test_apply_preconditioner_arguments(ctx):
     assert ei.value.status == mgf.ERR_UNSUPPORTED
     s.close()
     d = mgf.make_desc((8, 8, 8), precision=mgf.PRECISION_MIXED, precond=mgf.PRECOND_MULTIGRID)
-    d.mg_per_sweep = 2
+    d.mg_kernels = 3
     with pytest.raises(mgf.MgfError):
         mgf.Scene(ctx, d)
