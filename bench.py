@@ -30,21 +30,30 @@ BYTES_UPDATE = 24
 PRECONDS = {"identity": 0, "jacobi": 1, "multigrid": 2}
 
 
-def precond_bytes(name, sweeps=3):
+def precond_bytes(name, sweeps=3, ncell=None):
     """Algorithmic bytes per padded cell of one preconditioner application.
-    Jacobi: read r, code; write z.  V-cycle per level: the first two sweeps
-    from zero in one pass (b, code -> x: 10), each further sweep (b, x, code
-    -> x': 14), restriction (b, x, code of the fine level + 1/8 of a coarse b
-    and code: 10.75), the first post-sweep with the prolongation folded in
-    (b, x, code + 1/8 of a coarse x -> x': 14.5); the levels shrink by 8, so
-    the hierarchy costs 8/7 of level 0."""
+    Jacobi: read r, code; write z.  V-cycle: the levels shrink by 8 (the
+    coarse cycle's levels of <= 8192 cells are left out).  A level of at least
+    4M cells with three sweeps runs the fused descent (code, b -> x, 1/8 of a
+    coarse b: 10.5) and ascent (code, b, x, 1/8 of a coarse x -> x': 14.5);
+    other levels one kernel per sweep: the first two sweeps from zero in one
+    pass (b, code -> x: 10), each further sweep (b, x, code -> x': 14),
+    restriction (b, x, code of the fine level + 1/8 of a coarse b and code:
+    10.75), the first post-sweep with the prolongation folded in (b, x, code
+    + 1/8 of a coarse x -> x': 14.5).  Without ncell, every level per sweep."""
     if name == "identity":
         return 0.0
     if name == "jacobi":
         return 10.0
     pre = 10 + 14 * (sweeps - 2) if sweeps >= 2 else 10
-    lvl0 = pre + 10.75 + 14.5 + 14 * (sweeps - 1)
-    return lvl0 * 8.0 / 7.0
+    per_sweep = pre + 10.75 + 14.5 + 14 * (sweeps - 1)
+    if ncell is None:
+        return per_sweep * 8.0 / 7.0
+    total, n = 0.0, float(ncell)
+    while n > 8192:
+        total += n * (25.0 if (sweeps == 3 and n >= 4 * 1024 * 1024) else per_sweep)
+        n /= 8.0
+    return total / float(ncell)
 
 
 def traffic_for(kernel):
@@ -168,10 +177,10 @@ def batched_leg(mgf, scenes, ctx, args):
     value = float((cells * conv).sum() * args.batch_steps / wall)
     iters = float(np.mean([st.cg_iterations for st in stats]) / max(args.batch_steps, 1))
     # the step-byte model per scene with its own iteration and sweep counts
-    pcb = precond_bytes("multigrid", 3)
     nst = max(args.batch_steps, 1)
-    moved = sum(c * nst * (109 + (BYTES_P_AP + BYTES_UPDATE + pcb) * st.cg_iterations / nst
-                           + 25 * st.refinement_sweeps / nst) for c, st in zip(cells, stats))
+    moved = sum(c * nst * (109 + (BYTES_P_AP + BYTES_UPDATE + precond_bytes("multigrid", 3, int(np.prod(p.res))))
+                           * st.cg_iterations / nst + 25 * st.refinement_sweeps / nst)
+                for c, st, p in zip(cells, stats, specs))
     frac = moved / (wall * HBM_PEAK_GBPS * 1e9)
     res_max = float(max(st.residual_rel for st in stats))
     for s in sc:
@@ -403,7 +412,7 @@ def main():
             log("  %-22s %8.4f ms/launch %6.1f launches/step %7.3f ms/step %6.0f GB/s"
                 % (name, v["ms_per_launch"], v["launches_per_step"], v["ms_per_step"], v["GBps"]))
         dominant = max(ktab.items(), key=lambda kv: kv[1]["ms_per_step"])
-        pcb = precond_bytes(args.precond, args.mg_sweeps)
+        pcb = precond_bytes(args.precond, args.mg_sweeps, ncell_padded)
         s.close()
 
         plain = None
@@ -495,7 +504,10 @@ def main():
             "ms_per_launch": dominant[1]["ms_per_launch"],
             "note": ("HIP events recorded at each launch's own start and end (hipExtLaunchKernelGGL) during %d "
                      "ordinary steps after the timed ones; CG/multigrid launches that returned at once after "
-                     "convergence are left out" % probe_steps),
+                     "convergence are left out.  The fused multigrid descent / ascent (k_mg_descend, "
+                     "k_mg_ascend) are bound by instruction issue and LDS/barrier latency, not HBM "
+                     "(rocprofv3 PMC at 256^3: 43%% VALU issue); their algorithmic bytes are 10.5 and 14.5 B "
+                     "per cell, against 77 B for the six per-sweep launches they replace" % probe_steps),
             "kernels": ktab,
         },
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),

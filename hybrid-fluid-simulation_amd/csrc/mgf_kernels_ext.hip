@@ -23,6 +23,8 @@
 //
 // The mean is summed in the reference's neighbour order (-x, +x, -y, +y, -z,
 // +z) in FP32 and divided once, so the values are bitwise the reference's.
+#include <algorithm>
+
 #include "mgf_internal.h"
 #include "mgf_kcommon.h"
 
@@ -157,6 +159,64 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
             }
         }
         k1 = k0;
+    }
+}
+
+// 1a'. The same row transform with the row held in registers: NCH chunks of
+// 64 faces per lane (rows of at most 64 NCH faces), one read of the faces and
+// one write of the distances (the two-pass kernel above re-reads and
+// re-writes them).  The same operations, so the same distances.
+template <int NCH>
+__global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, int32_t* __restrict__ dist) {
+    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (wave >= F.rows[3]) return;
+    const int c = comp_of(F.rows, wave);
+    const uint32_t ex = pick(F.ex, c);
+    const uint32_t local = (wave - pick4(F.rows, c)) * ex;
+    const float* f = pick(F.f, c);
+    int32_t* d = dist + pick4(F.off, c) + local;
+    const uint32_t nch = (ex + 63) / 64;
+    int32_t v[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const uint32_t x = uint32_t(ch) * 64 + uint32_t(lane);
+        v[ch] = (x < ex && f[local + x] != MGF_UNKNOWN) ? 0 : kDistInf;
+    }
+    // forward: v = min(x + min_{k<=x} (d(k) - k), INF)
+    int32_t carry = kDistInf;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        if (uint32_t(ch) < nch) {
+            const int32_t x = ch * 64 + lane;
+            int32_t g = v[ch] - x;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t t = __shfl_up(g, o, 64);
+                if (lane >= o) g = min(g, t);
+            }
+            g = min(g, carry);
+            carry = __shfl(g, 63, 64);
+            v[ch] = min(g + x, kDistInf);
+        }
+    }
+    // backward: d = min(min_{k>=x} (v(k) + k) - x, INF)
+    carry = 2 * kDistInf;
+#pragma unroll
+    for (int ch = NCH - 1; ch >= 0; --ch) {
+        if (uint32_t(ch) < nch) {
+            const int32_t x = ch * 64 + lane;
+            const bool ok = uint32_t(x) < ex;
+            int32_t h = ok ? v[ch] + x : 2 * kDistInf;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t t = __shfl_down(h, o, 64);
+                if (lane + o < 64) h = min(h, t);
+            }
+            h = min(h, carry);
+            carry = __shfl(h, 0, 64);
+            if (ok) d[x] = min(h - x, kDistInf);
+        }
     }
 }
 
@@ -649,7 +709,13 @@ int32_t ext_max_distance(const GridDev& g) { return g.nx + g.ny + g.nz + 3; }
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
                                int32_t* dist) {
     const Faces F = faces_of(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
-    hipLaunchKernelGGL(k_ext_rows, dim3((F.rows[3] + 3) / 4), dim3(kBlock), 0, s, F, dist);
+    // rows: held in registers when they fit 16 chunks of 64 faces
+    const uint32_t exm = F.ex[0];   // the x-faces have the longest rows
+    const dim3 rg((F.rows[3] + 3) / 4);
+    if (exm <= 256) hipLaunchKernelGGL(k_ext_rows_reg<4>, rg, dim3(kBlock), 0, s, F, dist);
+    else if (exm <= 512) hipLaunchKernelGGL(k_ext_rows_reg<8>, rg, dim3(kBlock), 0, s, F, dist);
+    else if (exm <= 1024) hipLaunchKernelGGL(k_ext_rows_reg<16>, rg, dim3(kBlock), 0, s, F, dist);
+    else hipLaunchKernelGGL(k_ext_rows, rg, dim3(kBlock), 0, s, F, dist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ext_lines<1>, dim3((F.ylines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);

@@ -157,6 +157,27 @@ def test_extrapolate_bitexact(ctx, orc, layers):
         ctx.free(p)
 
 
+@pytest.mark.parametrize("res", [(300, 20, 18), (600, 9, 7), (1100, 4, 3), (8, 530, 6), (7, 6, 520)])
+def test_extrapolate_bitexact_long_lines(ctx, orc, res):
+    """Grids whose rows select the other distance-transform kernels: rows of
+    301 and 601 faces (8 and 16 register chunks per lane), of 1101 (the
+    two-pass row kernel); y and z lines of 531 and 521 faces."""
+    rng = np.random.default_rng(sum(res))
+    u, v, w = rand_vel(rng, res, 1.0)
+    for a in (u, v, w):
+        a[rng.random(a.size) < 0.97] = FLT_MAX
+    eu, ev, ew, used = orc.extrapolate_fc(res, u, v, w, 1000)
+    D = [ctx.to_device(a) for a in (u, v, w)]
+    got_used = mgf.C.c_int32()
+    mgf.check(mgf.lib().mgf_op_extrapolate_fc(ctx.h, mgf.grid(res), D[0], D[1], D[2], 1000,
+                                              mgf.C.byref(got_used)), "extrapolate")
+    for k, e in enumerate((eu, ev, ew)):
+        assert_bitexact(ctx.to_host(D[k], e.size), e, "extrapolate c%d" % k)
+    assert got_used.value == used
+    for p in D:
+        ctx.free(p)
+
+
 # ---------------------------------------------------------------------------
 # Reference goldens through the GPU.
 def pressure_scene_from_case(ctx, c, precision=mgf.PRECISION_PLAIN):
