@@ -372,6 +372,24 @@ constexpr int kCoarseLevelsMax = 8;
 constexpr size_t kCoarseLdsMax = 131072;
 constexpr size_t kMarchMinCells = size_t(1) << 22;   // MGF_MG_AUTO fuses levels at least this large
 
+// Whether level l (dims f; the next coarser level c) runs the fused descent
+// and ascent: three sweeps, a 3-D level above the coarse cycle, under
+// MGF_MG_AUTO at least kMarchMinCells (on smaller levels the march's serial
+// steps cost more than the launches they save); on level 0 the ascent's
+// partials of r . z must fit the scene's buffer.
+bool march_on(mgf_scene s, int l, const MgDims& f, const MgDims& c) {
+    const auto& d = s->desc;
+    const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
+    const int L = 1 + int(s->mg.size());
+    const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
+    if (d.preconditioner != MGF_PRECOND_MULTIGRID || nu != 3 || d.mg_kernels == MGF_MG_PER_SWEEP) return false;
+    if (l >= l0 || l + 1 >= L || !mg_march_ok(f)) return false;
+    const size_t n = size_t(f.nx) * f.ny * f.nz;
+    if (d.mg_kernels == MGF_MG_AUTO && n < kMarchMinCells) return false;
+    const size_t part_cap = size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2;
+    return l > 0 || mg_march_grid(f, c) <= part_cap;
+}
+
 mgf_status ensure_precond(mgf_scene s) {
     if (!s->z) {
         TRY(dalloc(s, &s->z, s->ncell));
@@ -411,6 +429,18 @@ mgf_status ensure_precond(mgf_scene s) {
             break;
         }
     }
+    // one byte per block of each fused level (which blocks hold fluid)
+    MgDims f{s->g.nx, s->g.ny, s->g.nz, 1.0f, 1.0f, 1.0f};
+    for (size_t l = 0; l < s->mg.size(); ++l) {
+        const MgDims& c = s->mg[l].d;
+        if (march_on(s, int(l), f, c)) {
+            uint8_t** wet = l == 0 ? &s->wet0 : &s->mg[l - 1].wet;
+            size_t* wet_n = l == 0 ? &s->wet0_n : &s->mg[l - 1].wet_n;
+            *wet_n = mg_march_grid(f, c);
+            TRY(dalloc(s, wet, *wet_n));
+        }
+        f = c;
+    }
     return MGF_OK;
 }
 
@@ -431,6 +461,17 @@ mgf_status mg_setup(mgf_scene s, const float sa[3]) {
         f = L.d;
         mf = L.markers;
     }
+    // the fused levels' blocks that hold fluid, for these codes
+    f = MgDims{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    const uint16_t* cf = s->codes;
+    for (size_t l = 0; l < s->mg.size(); ++l) {
+        const MgDims& c = s->mg[l].d;
+        uint8_t** wet = l == 0 ? &s->wet0 : &s->mg[l - 1].wet;
+        size_t* wet_n = l == 0 ? &s->wet0_n : &s->mg[l - 1].wet_n;
+        if (*wet && *wet_n >= mg_march_grid(f, c)) MGF_HIP(launch_mg_march_wet(st, f, c, cf, *wet));
+        f = c;
+        cf = s->mg[l].codes;
+    }
     return MGF_OK;
 }
 
@@ -448,12 +489,12 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
     const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.9f;
     const int L = 1 + int(s->mg.size());
-    struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; };
+    struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint8_t* wet; };
     Lv lv[17];
-    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt};
+    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->wet0};
     for (int l = 1; l < L; ++l) {
         auto& m = s->mg[size_t(l - 1)];
-        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t};
+        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, m.wet};
     }
     // levels [l0, L) run in one work-group
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
@@ -463,22 +504,18 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     // they beat one kernel per sweep; on small levels the march's serial
     // steps cost more than the launches they save); the level-0 ascent
     // writes one partial of r . z per block into the scene's partials buffer
-    const bool fuse = nu == 3 && d.mg_kernels != MGF_MG_PER_SWEEP;
-    const size_t part_cap = size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2;
-    auto fused = [&](int l) {
-        if (!fuse || l + 1 >= L || !mg_march_ok(lv[l].d)) return false;
-        const size_t n = size_t(lv[l].d.nx) * lv[l].d.ny * lv[l].d.nz;
-        if (d.mg_kernels == MGF_MG_AUTO && n < kMarchMinCells) return false;
-        return l > 0 || mg_march_grid(lv[l].d, lv[l + 1].d) <= part_cap;
-    };
+    auto fused = [&](int l) { return l + 1 < L && march_on(s, l, lv[l].d, lv[l + 1].d); };
     for (int l = 0; l < l0; ++l) {
         Lv& v = lv[l];
         const bool coarsest = l == L - 1;
         if (fused(l)) {
             const int pk = l == 0 ? probe_begin(s) : -1;
             MGF_HIP(launch_mg_descend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
-                                      const_cast<float*>(lv[l + 1].b), omega, s->scal));
-            probe_end(s, pk, MGF_PROBE_MG_DESCEND, 10.5 * double(v.d.nx) * v.d.ny * v.d.nz);
+                                      const_cast<float*>(lv[l + 1].b), omega, s->scal, v.wet));
+            // blocks without fluid store x = 0 and the coarse b = 0 only
+            const double nl = double(v.d.nx) * v.d.ny * v.d.nz;
+            const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : nl;
+            probe_end(s, pk, MGF_PROBE_MG_DESCEND, 10.5 * wet + 4.5 * (nl - wet));
             continue;
         }
         const int sweeps = coarsest ? kCoarsestSweeps : nu;
@@ -526,8 +563,9 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         if (fused(l)) {
             const int pf = l == 0 ? probe_begin(s) : -1;
             MGF_HIP(launch_mg_ascend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
-                                     s->red, l == 0 ? 1 : 0));
-            probe_end(s, pf, MGF_PROBE_MG_ASCEND, 14.5 * ncell);
+                                     s->red, l == 0 ? 1 : 0, v.wet));
+            const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : ncell;
+            probe_end(s, pf, MGF_PROBE_MG_ASCEND, 14.5 * wet + 4.0 * (ncell - wet));
             std::swap(v.x, v.t);
             continue;
         }
@@ -571,10 +609,32 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         return int(std::max(1.0, std::min(double(chunk), n)));
     };
     MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
+    // allocated with the scene when it was created MIXED; a scene switched to
+    // MIXED by mgf_scene_set_params gets it here
+    if (!s->wet_flat) TRY(dalloc(s, &s->wet_flat, col_blocks(s->g)));
+    MGF_HIP(launch_flat_wet(st, s->g, s->codes, s->wet_flat));
     if (!ident) {
         TRY(ensure_precond(s));
         TRY(mg_setup(s, sa));
     }
+    s->wet_flat_cells = s->wet0_cells = -1.0;
+    if (s->probes) {
+        // the bytes probed launches move: blocks without fluid move (almost) nothing
+        std::vector<uint8_t> h(col_blocks(s->g));
+        MGF_HIP(hipMemcpyAsync(h.data(), s->wet_flat, h.size(), hipMemcpyDeviceToHost, st));
+        MGF_HIP(hipStreamSynchronize(st));
+        s->wet_flat_cells = flat_wet_cells(s->g, h.data());
+        if (s->wet0 && !s->mg.empty()) {
+            h.resize(s->wet0_n);
+            MGF_HIP(hipMemcpyAsync(h.data(), s->wet0, h.size(), hipMemcpyDeviceToHost, st));
+            MGF_HIP(hipStreamSynchronize(st));
+            s->wet0_cells = mg_march_wet_cells(MgDims{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]},
+                                               s->mg[0].d, h.data());
+        }
+    }
+    const double ncell_all = double(s->ncell);
+    const double flat_cells = s->wet_flat_cells >= 0.0 ? s->wet_flat_cells : ncell_all;
+    bool first_pap = true;
     // x64 = 0 (or, warm, the previous pressure on the fluid cells and its
     // residual), ||b||^2 and the overall target
     const bool warm = d.warm_start && s->x64_valid;
@@ -600,6 +660,7 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     CgScalars hs{};
     while (!conv && sweeps < max_sweeps && it_total < max_it) {
         MGF_HIP(launch_cg_start(st, s->g, rsrc, s->r, s->x, s->scal, s->red, inner, target2));
+        first_pap = true;
         const float* zv = s->r;
         if (!ident) TRY(apply_precond(s, sa, &zv));
         // r.r at the sweep start and the sweep's target, as the device sets them
@@ -613,11 +674,13 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
             const int k = int(std::min<int64_t>(predict(rr_now, sweep_target), std::max<int64_t>(left, 1)));
             for (int q = 0; q < k; ++q) {
                 int pk = probe_begin(s);
-                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red));
-                probe_end(s, pk, MGF_PROBE_CG_PAP, 22.0 * double(s->ncell));
+                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red, s->wet_flat));
+                probe_end(s, pk, MGF_PROBE_CG_PAP, 22.0 * (first_pap ? ncell_all : flat_cells));
+                first_pap = false;
                 pk = probe_begin(s);
-                MGF_HIP(launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0));
-                probe_end(s, pk, MGF_PROBE_CG_UPDATE, 24.0 * double(s->ncell));
+                MGF_HIP(launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0,
+                                          s->wet_flat));
+                probe_end(s, pk, MGF_PROBE_CG_UPDATE, 24.0 * flat_cells);
                 if (!ident) TRY(apply_precond(s, sa, &zv));
             }
             MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
@@ -703,6 +766,7 @@ mgf_status alloc_scene(mgf_scene s) {
     A(&s->b, s->ncell); A(&s->x, s->ncell); A(&s->x64, s->ncell);
     A(&s->r, s->ncell); A(&s->p0, s->ncell); A(&s->p1, s->ncell); A(&s->ap, s->ncell);
     A(&s->codes, s->ncell);
+    if (s->desc.precision == MGF_PRECISION_MIXED) A(&s->wet_flat, col_blocks(s->g));
     A(&s->scal, 1);
     A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2);
     A(&s->red.out2, 2);

@@ -47,6 +47,8 @@ struct MgLevelBuf {
     float* b = nullptr;
     float* x = nullptr;
     float* t = nullptr;     // Jacobi ping-pong partner of x
+    uint8_t* wet = nullptr; // fused-march blocks with fluid (this level as the fine one)
+    size_t wet_n = 0;
 };
 
 struct Solid {
@@ -102,6 +104,12 @@ struct mgf_scene_s {
     // preconditioner (allocated on first use)
     float* z = nullptr;            // M^-1 r
     float* zt = nullptr;           // Jacobi ping-pong partner of z
+    uint8_t* wet0 = nullptr;       // fused-march blocks of level 0 with fluid
+    size_t wet0_n = 0;
+    uint8_t* wet_flat = nullptr;   // blocks of the flat CG kernels with fluid
+    // with probes on: cells in blocks with fluid (flat kernels, level-0
+    // march) for the bytes a probed launch moved; < 0 = every cell
+    double wet_flat_cells = -1.0, wet0_cells = -1.0;
     std::vector<mgf::MgLevelBuf> mg;   // coarse levels 1..L-1
     int mg_nx = -1, mg_ny = -1, mg_nz = -1;
     int mg_l0 = 0;                 // levels [mg_l0, L) run in one work-group
@@ -262,13 +270,19 @@ hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* ma
 // Blocks of the column kernels on grid g (the partial-sum buffer must hold
 // four doubles per block).
 uint32_t col_blocks(const GridDev& g);
+// wet: per block of the flat kernels, whether it holds a fluid cell
+// (launch_flat_wet, once per set of codes); null: no block skips.
+hipError_t launch_flat_wet(hipStream_t s, const GridDev& g, const uint16_t* codes, uint8_t* wet);
+// Cells in the blocks a host copy of the wet bytes marks (measurement).
+double flat_wet_cells(const GridDev& g, const uint8_t* wet_host);
+double mg_march_wet_cells(const MgDims& f, const MgDims& c, const uint8_t* wet_host);
 // K_I: p = z + beta p, Ap = A z + beta Ap in place (p = z, Ap = A z on the
 // first iteration), p . Ap, alpha.
 hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
-                           const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red);
+                           const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red, const uint8_t* wet);
 // K_II: x += alpha p, r -= alpha Ap, r . r; identity != 0: beta = rr / rz, rz = rr.
 hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float* p, const float* ap, float* x, float* r,
-                             CgScalars* sc, ReduceBuf red, int identity);
+                             CgScalars* sc, ReduceBuf red, int identity, const uint8_t* wet);
 // Start of an inner sweep: r = rsrc, x = 0, the scalars from r.r (target =
 // max(rel_inner^2 r.r, abs_target2)).
 hipError_t launch_cg_start(hipStream_t s, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,
@@ -299,11 +313,16 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
 // one launch (the partials the ascent writes).
 bool mg_march_ok(const MgDims& f);
 uint32_t mg_march_grid(const MgDims& f, const MgDims& c);
+// wet: one byte per block (launch_mg_march_wet, once per set of codes):
+// blocks whose tile holds no fluid store zeros and skip the march (null: none
+// skips).
+hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet);
 hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
-                             float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc);
+                             float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
+                             const uint8_t* wet);
 hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                             const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
-                            int with_rz);
+                            int with_rz, const uint8_t* wet);
 size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev);
 hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
                                   const float* b0, float* x0, int nu, int last_sweeps, float omega,

@@ -212,15 +212,39 @@ __device__ __forceinline__ double cg_pap_body(const Flat& F, uint32_t base, cons
     return acc;
 }
 
+// Blocks of the flat CG kernels (kBlock * kCpt consecutive cells each) that
+// hold no fluid cell (wet[block] == 0).  On non-fluid cells b = 0, so r and
+// x stay +0, z = M^-1 r is 0, and from the first K_I of a sweep on p and Ap
+// are +0 too: such a block would store the values its cells hold and add +0
+// to the dot products, so K_I (after the first iteration) and K_II return at
+// once with a zero partial.
+__global__ void __launch_bounds__(kBlock) k_flat_wet(Flat F, const uint16_t* __restrict__ code,
+                                                     uint8_t* __restrict__ wet) {
+    const uint32_t blk = xcd_block();
+    const uint32_t b0 = blk * (kBlock * kCpt);
+    int any = 0;
+#pragma unroll
+    for (int k = 0; k < kCpt; ++k) {
+        const uint32_t c = b0 + threadIdx.x + uint32_t(k) * kBlock;
+        if (c < F.n) any |= code[c] & kCodeFluid;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) wet[blk] = uint8_t(any != 0);
+}
+
 __global__ void __launch_bounds__(kBlock) k_cg_pap(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
                                                    float sz, const float* __restrict__ zv, float* __restrict__ p,
                                                    float* __restrict__ ap, const CgScalars* __restrict__ sc,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part, const uint8_t* __restrict__ wet) {
     __shared__ float aw[64];
     if (sc->done) return;
+    const bool first = sc->iters == 0;
+    if (!first && wet && !wet[xcd_block()]) {
+        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
     build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
     const float beta = float(sc->beta);
-    const bool first = sc->iters == 0;
     const uint32_t b0 = xcd_block() * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
     const double acc = (b0 + kBlock * kCpt <= F.n)
@@ -262,8 +286,12 @@ __device__ __forceinline__ double cg_update_body(const Flat& F, uint32_t base, c
 __global__ void __launch_bounds__(kBlock) k_cg_update(Flat F, const float* __restrict__ p,
                                                       const float* __restrict__ ap, float* __restrict__ x,
                                                       float* __restrict__ r, const CgScalars* __restrict__ sc,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, const uint8_t* __restrict__ wet) {
     if (sc->done) return;
+    if (wet && !wet[xcd_block()]) {
+        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
     const double alpha = sc->alpha;
     const uint32_t b0 = xcd_block() * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
@@ -1374,17 +1402,81 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     __syncthreads();
 }
 
+// The tile of block blk: columns [x0, x1) x [y0, y1), planes [za, zs) and the
+// coarse planes [za / 2, zc) its pairs complete.
+struct MarchTile {
+    int x0, x1, y0, y1, za, zs, zc;
+};
+__device__ __forceinline__ MarchTile march_tile(const March& M, uint32_t blk) {
+    const uint32_t tx = blk % M.tx, rest = blk / M.tx;
+    const uint32_t ty = rest % M.ty, tc = rest / M.ty;
+    MarchTile T;
+    T.x0 = int(tx) * kMTX;
+    T.x1 = min(T.x0 + kMTX, M.nx);
+    T.y0 = int(ty) * kMTY;
+    T.y1 = min(T.y0 + kMTY, M.ny);
+    T.za = int(tc) * M.chunk;
+    T.zs = min(T.za + M.chunk, M.nz);
+    T.zc = min((T.za + M.chunk) / 2, M.cnz);
+    return T;
+}
+
+// A block whose tile holds no fluid cell in its planes: every value it
+// stores is +0 (the sweeps give 0 on non-fluid cells, and so does the
+// residual, hence each restricted pair sum), so it stores the zeros and
+// skips the march.
+template <bool POST>
+__device__ __forceinline__ void march_dry(const March& M, uint32_t blk, float* __restrict__ xo,
+                                          float* __restrict__ bco) {
+    const MarchTile T = march_tile(M, blk);
+    const int w = T.x1 - T.x0, h = T.y1 - T.y0;
+    for (int e = int(threadIdx.x); e < w * h; e += kMThreads) {
+        const int y = T.y0 + e / w, x = T.x0 + e % w;
+        for (int z = T.za; z < T.zs; ++z) xo[uint32_t(z) * M.sz + uint32_t(y) * M.sy + uint32_t(x)] = 0.0f;
+    }
+    if constexpr (!POST) {
+        const int cw = (w + 1) / 2, ch = (h + 1) / 2;
+        for (int e = int(threadIdx.x); e < cw * ch; e += kMThreads) {
+            const int Y = T.y0 / 2 + e / cw, X = T.x0 / 2 + e % cw;
+            for (int Z = T.za / 2; Z < T.zc; ++Z)
+                bco[uint32_t(Z) * M.csz + uint32_t(Y) * M.csy + uint32_t(X)] = 0.0f;
+        }
+    }
+}
+
+// wet[blk] = 1 when block blk's tile holds a fluid cell in its planes (the
+// codes do not change during a pressure solve: one pass per solve).
+__global__ void __launch_bounds__(kMThreads) k_mg_march_wet(March M, const uint16_t* __restrict__ code,
+                                                            uint8_t* __restrict__ wet) {
+    const uint32_t blk = xcd_block();
+    if (blk >= M.blocks) return;
+    const MarchTile T = march_tile(M, blk);
+    const int w = T.x1 - T.x0, h = T.y1 - T.y0;
+    int any = 0;
+    for (int e = int(threadIdx.x); e < w * h; e += kMThreads) {
+        const uint32_t col = uint32_t(T.y0 + e / w) * M.sy + uint32_t(T.x0 + e % w);
+        for (int z = T.za; z < T.zs; ++z) any |= code[uint32_t(z) * M.sz + col] & kCodeFluid;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) wet[blk] = uint8_t(any != 0);
+}
+
 template <bool POST, bool RZ>
 __global__ void __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* __restrict__ b,
            const float* __restrict__ xi, const float* __restrict__ xc, float* __restrict__ xo,
            const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
-           const CgScalars* __restrict__ sc, double* __restrict__ part) {
+           const CgScalars* __restrict__ sc, double* __restrict__ part, const uint8_t* __restrict__ wet) {
     __shared__ float lds[kLdsSlots + (POST ? 6 : 8) * kMThreads];
     __shared__ double red[kMThreads / 64];
     if (sc->done) return;
     const uint32_t blk = xcd_block();
     if (blk >= M.blocks) {
+        if (RZ && threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
+    if (wet && !wet[blk]) {
+        march_dry<POST>(M, blk, xo, bco);
         if (RZ && threadIdx.x == 0) part[blockIdx.x] = 0.0;
         return;
     }
@@ -1506,20 +1598,35 @@ hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale
     return hipGetLastError();
 }
 
+double flat_wet_cells(const GridDev& g, const uint8_t* wet_host) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t per = uint32_t(kBlock * kCpt), nb = (F.n + per - 1) / per;
+    double cells = 0.0;
+    for (uint32_t b = 0; b < nb; ++b)
+        if (wet_host[b]) cells += double(std::min(per, F.n - b * per));
+    return cells;
+}
+
+hipError_t launch_flat_wet(hipStream_t s, const GridDev& g, const uint16_t* codes, uint8_t* wet) {
+    if (!wet || !codes) return hipErrorInvalidValue;
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    return LAUNCH(k_flat_wet, flat_grid(F.n), F, codes, wet);
+}
+
 hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
-                           const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red) {
+                           const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red, const uint8_t* wet) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
     const uint32_t G = flat_grid(F.n);
-    hipError_t e = LAUNCH(k_cg_pap, G, F, codes, scale[0], scale[1], scale[2], zv, p, ap, sc, red.partials);
+    hipError_t e = LAUNCH(k_cg_pap, G, F, codes, scale[0], scale[1], scale[2], zv, p, ap, sc, red.partials, wet);
     if (e != hipSuccess) return e;
     return finish(s, red.partials, G, sc, kFinishAlpha);
 }
 
 hipError_t launch_col_update(hipStream_t s, const GridDev& g, const float* p, const float* ap, float* x, float* r,
-                             CgScalars* sc, ReduceBuf red, int identity) {
+                             CgScalars* sc, ReduceBuf red, int identity, const uint8_t* wet) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
     const uint32_t G = flat_grid(F.n);
-    hipError_t e = LAUNCH(k_cg_update, G, F, p, ap, x, r, sc, red.partials);
+    hipError_t e = LAUNCH(k_cg_update, G, F, p, ap, x, r, sc, red.partials, wet);
     if (e != hipSuccess) return e;
     return finish(s, red.partials, G, sc, identity ? kFinishRrIdent : kFinishRr);
 }
@@ -1621,19 +1728,40 @@ uint32_t mg_march_grid(const MgDims& f, const MgDims& c) {
     return (M.blocks + kXcds - 1) / kXcds * kXcds;
 }
 
+double mg_march_wet_cells(const MgDims& f, const MgDims& c, const uint8_t* wet_host) {
+    const March M = make_march(f, c);
+    double cells = 0.0;
+    for (uint32_t blk = 0; blk < M.blocks; ++blk) {
+        if (!wet_host[blk]) continue;
+        const uint32_t tx = blk % M.tx, rest = blk / M.tx, ty = rest % M.ty, tc = rest / M.ty;
+        const int w = std::min(kMTX, M.nx - int(tx) * kMTX), h = std::min(kMTY, M.ny - int(ty) * kMTY);
+        const int d = std::min(M.chunk, M.nz - int(tc) * M.chunk);
+        cells += double(w) * h * d;
+    }
+    return cells;
+}
+
+hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet) {
+    if (!mg_march_ok(f) || !wet || !code) return hipErrorInvalidValue;
+    const March M = make_march(f, c);
+    hipLaunchKernelGGL(k_mg_march_wet, dim3(mg_march_grid(f, c)), dim3(kMThreads), 0, s, M, code, wet);
+    return hipGetLastError();
+}
+
 hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
-                             float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc) {
+                             float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
+                             const uint8_t* wet) {
     if (!mg_march_ok(f)) return hipErrorInvalidValue;
     const March M = make_march(f, c);
     const uint32_t G = mg_march_grid(f, c);
     return launch_dims(k_mg_march<false, false>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b,
                        static_cast<const float*>(nullptr), static_cast<const float*>(nullptr), x, codec, bc, omega,
-                       sc, static_cast<double*>(nullptr));
+                       sc, static_cast<double*>(nullptr), wet);
 }
 
 hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                             const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
-                            int with_rz) {
+                            int with_rz, const uint8_t* wet) {
     if (!mg_march_ok(f)) return hipErrorInvalidValue;
     const March M = make_march(f, c);
     const uint32_t G = mg_march_grid(f, c);
@@ -1642,11 +1770,11 @@ hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, con
     if (with_rz)
         e = launch_dims(k_mg_march<true, true>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
                         static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
-                        red.partials);
+                        red.partials, wet);
     else
         e = launch_dims(k_mg_march<true, false>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
                         static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
-                        static_cast<double*>(nullptr));
+                        static_cast<double*>(nullptr), wet);
     if (e != hipSuccess || !with_rz) return e;
     return finish(s, red.partials, G, sc, kFinishRz);
 }

@@ -654,6 +654,25 @@ def test_multigrid_cuts_iterations(ctx):
     assert its[mgf.PRECOND_MULTIGRID] * 8 <= its[mgf.PRECOND_IDENTITY], its
 
 
+@pytest.mark.parametrize("precond", PRECONDS)
+def test_switch_to_mixed_after_creation(ctx, precond):
+    """A scene created with the plain FP32 path and switched to the mixed solver
+    by mgf_scene_set_params (as CGridFluidSolver::setPressureOptions does)
+    takes the buffers the mixed solver needs on its first solve and gives the
+    same step as a scene created mixed."""
+    spec = scenes.obstacles(64, seed=5)
+    kw = dam_desc_kw(mgf.PRECISION_MIXED, precond)
+    made = scenes.build_gpu_scene(mgf, ctx, spec, **kw)
+    switched = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_PLAIN))
+    switched.set_params(mgf.make_desc(spec.res, spec.origin, spec.spacing, spec.dt, **kw))
+    a, b = made.step(1), switched.step(1)
+    assert a["converged"] == 1 and b["converged"] == 1
+    assert a["cg_iterations"] == b["cg_iterations"]
+    for fid in (0, 1, 2):
+        assert_bitexact(switched.download(fid), made.download(fid), "field %d" % fid)
+    made.close(); switched.close()
+
+
 def test_preconditioner_options_are_checked(ctx):
     d = mgf.make_desc((8, 8, 8), precision=mgf.PRECISION_PLAIN, precond=mgf.PRECOND_MULTIGRID)
     with pytest.raises(mgf.MgfError):
