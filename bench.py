@@ -34,7 +34,7 @@ def precond_bytes(name, sweeps=3, ncell=None):
     """Algorithmic bytes per padded cell of one preconditioner application.
     Jacobi: read r, code; write z.  V-cycle: the levels shrink by 8 (the
     coarse cycle's levels of <= 8192 cells are left out).  A level of at least
-    4M cells with three sweeps runs the fused descent (code, b -> x, 1/8 of a
+    1M cells with three sweeps runs the fused descent (code, b -> x, 1/8 of a
     coarse b: 10.5) and ascent (code, b, x, 1/8 of a coarse x -> x': 14.5);
     other levels one kernel per sweep: the first two sweeps from zero in one
     pass (b, code -> x: 10), each further sweep (b, x, code -> x': 14),
@@ -51,7 +51,7 @@ def precond_bytes(name, sweeps=3, ncell=None):
         return per_sweep * 8.0 / 7.0
     total, n = 0.0, float(ncell)
     while n > 8192:
-        total += n * (25.0 if (sweeps == 3 and n >= 4 * 1024 * 1024) else per_sweep)
+        total += n * (25.0 if (sweeps == 3 and n >= 1024 * 1024) else per_sweep)
         n /= 8.0
     return total / float(ncell)
 

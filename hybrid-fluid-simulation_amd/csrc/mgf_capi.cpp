@@ -370,7 +370,9 @@ constexpr int kMgStopExtent = 8;
 constexpr size_t kCoarseCells = 8192;    // levels this small run in one work-group
 constexpr int kCoarseLevelsMax = 8;
 constexpr size_t kCoarseLdsMax = 131072;
-constexpr size_t kMarchMinCells = size_t(1) << 22;   // MGF_MG_AUTO fuses levels at least this large
+// MGF_MG_AUTO fuses levels at least this large (measured at 256^3: fusing the
+// 129^3 level too is 0.8% faster per step, the 65^3 one slower)
+constexpr size_t kMarchMinCells = size_t(1) << 20;
 
 // Whether level l (dims f; the next coarser level c) runs the fused descent
 // and ascent: three sweeps, a 3-D level above the coarse cycle, under

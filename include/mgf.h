@@ -157,7 +157,7 @@ typedef struct mgf_scene_desc {
 /* Kernels of the multigrid V-cycle (mgf_scene_desc.mg_kernels).  With three
  * sweeps, a 3-D level's descent (three sweeps and the restriction) and ascent
  * (prolongation and three sweeps) can each run as one fused kernel. */
-#define MGF_MG_AUTO      0  /* fused on levels of at least 4M cells (default) */
+#define MGF_MG_AUTO      0  /* fused on levels of at least 1M cells (default) */
 #define MGF_MG_PER_SWEEP 1  /* one kernel per sweep on every level            */
 #define MGF_MG_FUSED     2  /* fused on every 3-D level                       */
 
