@@ -781,6 +781,7 @@ mgf_status alloc_scene(mgf_scene s) {
 // layer; returns the largest non-empty layer (< layers).
 mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
     const int32_t K = ext_max_distance(s->g);
+    if (K >= 65535) return fail(MGF_ERR_UNSUPPORTED, "nx + ny + nz must be below 65532 (16-bit face distances)");
     TRY(dalloc(s, &s->layer, nface));
     TRY(dalloc(s, &s->list, nface));
     TRY(dalloc(s, &s->lmask, nface));
@@ -801,10 +802,11 @@ mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, in
     const int32_t K = ext_max_distance(s->g);
     const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
     if (cap <= 0) return MGF_OK;   // layer 0 only: the reference's fill loop is empty
-    MGF_HIP(launch_ext_distance(st, s->g, u, v, w, s->layer));
+    uint16_t* dist = reinterpret_cast<uint16_t*>(s->layer);
+    MGF_HIP(launch_ext_distance(st, s->g, u, v, w, dist));
     const uint32_t* offs = nullptr;
     const uint32_t* maxd = nullptr;
-    MGF_HIP(launch_ext_sort(st, s->g, s->layer, cap, K, s->counts, s->list, s->lmask, s->fmask, &offs, &maxd));
+    MGF_HIP(launch_ext_sort(st, s->g, dist, cap, K, s->counts, s->list, s->lmask, s->fmask, &offs, &maxd));
     // offsets of layers 0..cap+1, then the largest distance present
     MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));

@@ -117,7 +117,7 @@ struct mgf_scene_s {
     bool x64_valid = false;        // x64 holds a finished mixed solve (a warm start's guess)
 
     // extrapolation scratch
-    int32_t* layer = nullptr;      // distance of every face (nfx+nfy+nfz)
+    int32_t* layer = nullptr;      // nfx+nfy+nfz words: the faces' uint16 distances, or the SDF's scratch
     uint32_t* list = nullptr;      // faces sorted by distance (nfx+nfy+nfz)
     uint8_t* lmask = nullptr;      // previous-layer neighbour mask of each list entry
     uint8_t* fmask = nullptr;      // the same mask per face (read for neighbours of the next layer)
@@ -336,13 +336,13 @@ int32_t ext_max_distance(const GridDev& g);
 // dist = L1 distance of every face to the nearest known face (1 << 29 where
 // a component has none); faces indexed u, then v, then w.
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
-                               int32_t* dist);
+                               uint16_t* dist);
 // Counting sort of the faces with 1 <= dist <= cap (cap <= K) into list, by
 // distance; lmask[k] = the mask of list[k]'s neighbours one layer below.  counts: ext_counts_size(K) entries of scratch; *offs receives
 // the device address of offs[0..cap+1] (offs[L] = first slot of distance L,
 // offs[cap+1] = total) and *maxd that of the largest distance present.
 size_t ext_counts_size(int32_t K);
-hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
+hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const uint16_t* dist, int32_t cap, int32_t K,
                            uint32_t* counts, uint32_t* list, uint8_t* lmask, uint8_t* fmask, const uint32_t** offs,
                            const uint32_t** maxd);
 // One layer: list[begin, begin+count) take the mean of their neighbours in

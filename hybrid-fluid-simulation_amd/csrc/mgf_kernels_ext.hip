@@ -30,7 +30,11 @@
 
 namespace mgf {
 
-constexpr int32_t kDistInf = 1 << 29;   // unreached
+constexpr int32_t kDistInf = 1 << 29;   // unreached (cell distances of the fluid SDF)
+// Face distances are stored as uint16 (65535 = unreached): every reachable
+// distance is at most nx + ny + nz + 3 < 65535 (the extrapolation refuses
+// larger grids), so the transforms compute in int32 and clamp on store.
+constexpr int32_t kFaceInf = 0xFFFF;
 
 struct Faces {
     float* f[3];
@@ -69,7 +73,7 @@ __device__ __forceinline__ uint32_t udiv_r(uint32_t a, uint32_t d, double rd) {
 // 1a. Rows along x: d = 0 on known faces, INF elsewhere; then the 1-D L1
 // distance: forward d(x) = x + min_{k<=x} (d(k) - k), backward
 // d(x) = min_{k>=x} (d(k) + k) - x.  One wave per row.
-__global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restrict__ dist) {
+__global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, uint16_t* __restrict__ dist) {
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= F.rows[3]) return;
@@ -77,13 +81,13 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restric
     const uint32_t ex = pick(F.ex, c);
     const uint32_t local = (wave - pick4(F.rows, c)) * ex;     // first face of the row in component c
     const float* f = pick(F.f, c);
-    int32_t* d = dist + pick4(F.off, c) + local;
+    uint16_t* d = dist + pick4(F.off, c) + local;
     // forward
-    int32_t carry = kDistInf;
+    int32_t carry = kFaceInf;
     for (uint32_t x0 = 0; x0 < ex; x0 += 64) {
         const uint32_t x = x0 + uint32_t(lane);
         const bool ok = x < ex;
-        const int32_t d0 = ok ? (f[local + x] != MGF_UNKNOWN ? 0 : kDistInf) : kDistInf;
+        const int32_t d0 = ok ? (f[local + x] != MGF_UNKNOWN ? 0 : kFaceInf) : kFaceInf;
         int32_t g = d0 - int32_t(x);
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -92,15 +96,15 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restric
         }
         g = min(g, carry);
         carry = __shfl(g, 63, 64);
-        if (ok) d[x] = min(g + int32_t(x), kDistInf);
+        if (ok) d[x] = uint16_t(min(g + int32_t(x), kFaceInf));
     }
     // backward (each lane re-reads what it wrote)
-    carry = 2 * kDistInf;
+    carry = 2 * kFaceInf;
     const uint32_t nch = (ex + 63) / 64;
     for (uint32_t ch = nch; ch-- > 0;) {
         const uint32_t x = ch * 64 + uint32_t(lane);
         const bool ok = x < ex;
-        int32_t h = ok ? d[x] + int32_t(x) : 2 * kDistInf;
+        int32_t h = ok ? int32_t(d[x]) + int32_t(x) : 2 * kFaceInf;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const int32_t t = __shfl_down(h, o, 64);
@@ -108,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restric
         }
         h = min(h, carry);
         carry = __shfl(h, 0, 64);
-        if (ok) d[x] = min(h - int32_t(x), kDistInf);
+        if (ok) d[x] = uint16_t(min(h - int32_t(x), kFaceInf));
     }
 }
 
@@ -117,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, int32_t* __restric
 // loads in flight per chunk (a line is a chain of dependent minima, so the
 // loads of a chunk are issued before its first minimum).
 template <int AXIS>
-__global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restrict__ dist) {
+__global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, uint16_t* __restrict__ dist) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
     if (t >= pre[3]) return;
@@ -129,33 +133,33 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
     uint32_t start, stride, len;
     if (AXIS == 1) { start = w * ex * ey + x; stride = ex; len = ey; }
     else { start = w * ex + x; stride = ex * ey; len = pick(F.ez, c); }
-    int32_t* d = dist + pick4(F.off, c) + start;
+    uint16_t* d = dist + pick4(F.off, c) + start;
     constexpr int K = 32;
-    int32_t prev = kDistInf;
+    int32_t prev = kFaceInf;
     for (uint32_t k0 = 0; k0 < len; k0 += K) {
         int32_t v[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = (k0 + q < len) ? d[size_t(k0 + q) * stride] : 0;
+        for (int q = 0; q < K; ++q) v[q] = (k0 + q < len) ? int32_t(d[size_t(k0 + q) * stride]) : 0;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             if (k0 + q < len) {
                 prev = min(v[q], prev + 1);
-                d[size_t(k0 + q) * stride] = prev;
+                d[size_t(k0 + q) * stride] = uint16_t(prev);
             }
         }
     }
-    prev = kDistInf;
+    prev = kFaceInf;
     for (uint32_t k1 = len; k1 > 0;) {
         const uint32_t k0 = k1 >= K ? k1 - K : 0;
         const uint32_t m = k1 - k0;
         int32_t v[K];
 #pragma unroll
-        for (int q = 0; q < K; ++q) v[q] = (uint32_t(q) < m) ? d[size_t(k1 - 1 - q) * stride] : 0;
+        for (int q = 0; q < K; ++q) v[q] = (uint32_t(q) < m) ? int32_t(d[size_t(k1 - 1 - q) * stride]) : 0;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             if (uint32_t(q) < m) {
                 prev = min(v[q], prev + 1);
-                d[size_t(k1 - 1 - q) * stride] = min(prev, kDistInf);
+                d[size_t(k1 - 1 - q) * stride] = uint16_t(min(prev, kFaceInf));
             }
         }
         k1 = k0;
@@ -167,7 +171,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, int32_t* __restri
 // one write of the distances (the two-pass kernel above re-reads and
 // re-writes them).  The same operations, so the same distances.
 template <int NCH>
-__global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, int32_t* __restrict__ dist) {
+__global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, uint16_t* __restrict__ dist) {
     const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= F.rows[3]) return;
@@ -175,16 +179,16 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, int32_t* __res
     const uint32_t ex = pick(F.ex, c);
     const uint32_t local = (wave - pick4(F.rows, c)) * ex;
     const float* f = pick(F.f, c);
-    int32_t* d = dist + pick4(F.off, c) + local;
+    uint16_t* d = dist + pick4(F.off, c) + local;
     const uint32_t nch = (ex + 63) / 64;
     int32_t v[NCH];
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
         const uint32_t x = uint32_t(ch) * 64 + uint32_t(lane);
-        v[ch] = (x < ex && f[local + x] != MGF_UNKNOWN) ? 0 : kDistInf;
+        v[ch] = (x < ex && f[local + x] != MGF_UNKNOWN) ? 0 : kFaceInf;
     }
     // forward: v = min(x + min_{k<=x} (d(k) - k), INF)
-    int32_t carry = kDistInf;
+    int32_t carry = kFaceInf;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
         if (uint32_t(ch) < nch) {
@@ -197,17 +201,17 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, int32_t* __res
             }
             g = min(g, carry);
             carry = __shfl(g, 63, 64);
-            v[ch] = min(g + x, kDistInf);
+            v[ch] = min(g + x, kFaceInf);
         }
     }
     // backward: d = min(min_{k>=x} (v(k) + k) - x, INF)
-    carry = 2 * kDistInf;
+    carry = 2 * kFaceInf;
 #pragma unroll
     for (int ch = NCH - 1; ch >= 0; --ch) {
         if (uint32_t(ch) < nch) {
             const int32_t x = ch * 64 + lane;
             const bool ok = uint32_t(x) < ex;
-            int32_t h = ok ? v[ch] + x : 2 * kDistInf;
+            int32_t h = ok ? v[ch] + x : 2 * kFaceInf;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const int32_t t = __shfl_down(h, o, 64);
@@ -215,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, int32_t* __res
             }
             h = min(h, carry);
             carry = __shfl(h, 0, 64);
-            if (ok) d[x] = min(h - x, kDistInf);
+            if (ok) d[x] = uint16_t(min(h - x, kFaceInf));
         }
     }
 }
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_ext_zero(uint32_t* __restrict__ a, u
     if (i < n) a[i] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_ext_count(const int32_t* __restrict__ dist, uint32_t total, int32_t cap,
+__global__ void __launch_bounds__(kBlock) k_ext_count(const uint16_t* __restrict__ dist, uint32_t total, int32_t cap,
                                                       uint32_t* __restrict__ hist, uint32_t K2) {
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
@@ -353,7 +357,7 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
 // Bit q of the lower-neighbour mask of a face: its in-grid neighbour in
 // direction q (-x, +x, -y, +y, -z, +z) has a smaller distance, i.e. lies in
 // the previous layer.
-__device__ __forceinline__ uint32_t lower_mask(const Faces& F, const int32_t* __restrict__ dist, uint32_t gi,
+__device__ __forceinline__ uint32_t lower_mask(const Faces& F, const uint16_t* __restrict__ dist, uint32_t gi,
                                                int32_t dv) {
     const int c = comp_of(F.off, gi);
     const uint32_t o = pick4(F.off, c);
@@ -375,7 +379,7 @@ __device__ __forceinline__ uint32_t lower_mask(const Faces& F, const int32_t* __
     return m;
 }
 
-__global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const int32_t* __restrict__ dist, uint32_t total,
+__global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const uint16_t* __restrict__ dist, uint32_t total,
                                                         int32_t cap, uint32_t K2, const uint32_t* __restrict__ poff,
                                                         uint32_t* cursor, uint32_t* __restrict__ list,
                                                         uint8_t* __restrict__ lmask, uint8_t* __restrict__ fmask) {
@@ -707,7 +711,7 @@ static Faces faces_of(const GridDev& g, float* u, float* v, float* w) {
 int32_t ext_max_distance(const GridDev& g) { return g.nx + g.ny + g.nz + 3; }
 
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
-                               int32_t* dist) {
+                               uint16_t* dist) {
     const Faces F = faces_of(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
     // rows: held in registers when they fit 16 chunks of 64 faces
     const uint32_t exm = F.ex[0];   // the x-faces have the longest rows
@@ -727,7 +731,7 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
 
 size_t ext_counts_size(int32_t K) { return (3 * size_t(kCopies) + 1) * (size_t(K) + 2) + 1; }
 
-hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const int32_t* dist, int32_t cap, int32_t K,
+hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const uint16_t* dist, int32_t cap, int32_t K,
                            uint32_t* counts, uint32_t* list, uint8_t* lmask, uint8_t* fmask, const uint32_t** offs,
                            const uint32_t** maxd) {
     const Faces F = faces_of(g, nullptr, nullptr, nullptr);
