@@ -781,7 +781,7 @@ mgf_status alloc_scene(mgf_scene s) {
 // layer; returns the largest non-empty layer (< layers).
 mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
     const int32_t K = ext_max_distance(s->g);
-    if (K >= 65535) return fail(MGF_ERR_UNSUPPORTED, "nx + ny + nz must be below 65532 (16-bit face distances)");
+    if (K >= 65535) return fail(MGF_ERR_UNSUPPORTED, "grid too long for 16-bit face distances (nx + ny + nz + 3 must be below 65535)");
     TRY(dalloc(s, &s->layer, nface));
     TRY(dalloc(s, &s->list, nface));
     TRY(dalloc(s, &s->lmask, nface));

@@ -333,8 +333,9 @@ hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, c
 // ---- extrapolation (mgf_kernels_ext.hip) ------------------------------------
 // Largest L1 distance between two faces of one component grid (+ margin).
 int32_t ext_max_distance(const GridDev& g);
-// dist = L1 distance of every face to the nearest known face (1 << 29 where
-// a component has none); faces indexed u, then v, then w.
+// dist = L1 distance of every face to the nearest known face (65535 where
+// a component has none; nx + ny + nz + 3 must be below 65535); faces indexed
+// u, then v, then w.
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
                                uint16_t* dist);
 // Counting sort of the faces with 1 <= dist <= cap (cap <= K) into list, by

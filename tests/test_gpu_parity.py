@@ -178,6 +178,36 @@ def test_extrapolate_bitexact_long_lines(ctx, orc, res):
         ctx.free(p)
 
 
+def test_extrapolate_bitexact_16bit_distances(ctx, orc):
+    """Face distances are stored in 16 bits: a row of 65001 u faces with one
+    known face at its start has distances up to 65000 (the sorted layers stop
+    at 1000); a grid with nx + ny + nz + 3 >= 65535 is refused before any
+    launch."""
+    res = (65000, 2, 1)
+    rng = np.random.default_rng(7)
+    u, v, w = rand_vel(rng, res, 1.0)
+    u[:] = FLT_MAX
+    u[0] = 0.5
+    for a in (v, w):
+        a[rng.random(a.size) < 0.97] = FLT_MAX
+    eu, ev, ew, used = orc.extrapolate_fc(res, u, v, w, 1000)
+    D = [ctx.to_device(a) for a in (u, v, w)]
+    got_used = mgf.C.c_int32()
+    mgf.check(mgf.lib().mgf_op_extrapolate_fc(ctx.h, mgf.grid(res), D[0], D[1], D[2], 1000,
+                                              mgf.C.byref(got_used)), "extrapolate")
+    for k, e in enumerate((eu, ev, ew)):
+        assert_bitexact(ctx.to_host(D[k], e.size), e, "extrapolate c%d" % k)
+    assert got_used.value == used
+    for p in D:
+        ctx.free(p)
+    big = (65533, 1, 1)
+    B = [ctx.to_device(np.full(n, FLT_MAX, np.float32)) for n in (65534, 2 * 65533, 2 * 65533)]
+    st = mgf.lib().mgf_op_extrapolate_fc(ctx.h, mgf.grid(big), B[0], B[1], B[2], 1000, mgf.C.byref(got_used))
+    assert st == mgf.ERR_UNSUPPORTED
+    for p in B:
+        ctx.free(p)
+
+
 # ---------------------------------------------------------------------------
 # Reference goldens through the GPU.
 def pressure_scene_from_case(ctx, c, precision=mgf.PRECISION_PLAIN):
