@@ -410,11 +410,13 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const uint16_t*
     for (int b = 1 + threadIdx.x; b < nb; b += kBlock)
         if (h[b]) h[b] = atomicAdd(&cu[b], h[b]);
     __syncthreads();
+    // Only faces that enter the list need their mask: known faces (0) and
+    // those past the cap skip the six neighbour loads.
     uint32_t msk[kSortCpt];
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
         const uint32_t i = base + uint32_t(k) * kBlock;
-        msk[k] = lower_mask(F, dist, i < total ? i : total - 1, dv[k]);
+        msk[k] = (dv[k] >= 1 && dv[k] <= cap) ? lower_mask(F, dist, i < total ? i : total - 1, dv[k]) : 0u;
     }
 #pragma unroll
     for (int k = 0; k < kSortCpt; ++k) {
