@@ -79,7 +79,10 @@ def log(*a):
 
 
 def run_steps(scene, steps):
-    times, iters, sweeps, res, conv = [], [], [], [], 0
+    """Steps one at a time; a step counts as converged when its solve met the
+    tolerance by its own stop rule (converged) and by the FP64 residual of
+    the pressure it produced (true_converged)."""
+    times, iters, sweeps, res, div, conv, conv_true = [], [], [], [], [], 0, 0
     for _ in range(steps):
         t0 = time.perf_counter()
         st = scene.step(1, allow_unconverged=True)
@@ -87,55 +90,52 @@ def run_steps(scene, steps):
         iters.append(st["cg_iterations"])
         sweeps.append(st["refinement_sweeps"])
         res.append(st["residual_rel"])
+        div.append(st["div_residual"])
         if st["converged"] == 1:
             conv += 1
+        if st["true_converged"] == 1:
+            conv_true += 1
     return dict(times=np.array(times), iters=np.array(iters), sweeps=np.array(sweeps),
-                res=np.array(res), converged=conv)
+                res=np.array(res), div=np.array(div), converged=conv, converged_true=conv_true)
 
 
-def cpu_baseline(spec, plain_iters_per_step, seconds_hint):
-    """The FP32 oracle (C++ CPU restatement of the reference step) on the host
-    cores: one 256^3 step's stages are timed, the CG at two iteration caps to
-    separate its per-iteration cost; the step time is assembled with the
-    plain GPU path's iteration count for the same scene."""
+def cpu_baseline(spec, steps):
+    """The FP32 oracle (C++ CPU restatement of the reference step, plain FP32
+    CG to the 1e-6 relative stop rule) on the host cores, timed: config 1
+    (64^3 dam break) for all of its 10 steps, then `steps` full steps of the
+    headline scene.  The value is the headline scene's interior cell-steps/s
+    at the median step time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import mac_oracle as orc
     import scenes
-    s = scenes.build_oracle_scene(orc, spec)
-    p = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-6)
-    t0 = time.perf_counter()
-    s.update_boundaries(p)
-    s.advect(p)
-    s.apply_forces(p)
-    t_adv = time.perf_counter() - t0
-    res, o, h = spec.res, spec.origin, spec.spacing
-    fields = [s.download(f) for f in range(8)]
-    u, v, w, phi, solid, us, vs, ws = fields
-    k_lo, k_hi = 5, 25
-    t0 = time.perf_counter()
-    orc.pressure_solve(res, o, h, spec.dt, u, v, w, us, vs, ws, solid, phi, tol_mode=1, tol=1e-30,
-                       max_it=k_lo)
-    t_lo = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    out = orc.pressure_solve(res, o, h, spec.dt, u, v, w, us, vs, ws, solid, phi, tol_mode=1, tol=1e-30,
-                             max_it=k_hi)
-    t_hi = time.perf_counter() - t0
-    per_iter = (t_hi - t_lo) / (k_hi - k_lo)
-    fixed = max(t_lo - k_lo * per_iter, 0.0)
-    s.upload(0, out["u"]); s.upload(1, out["v"]); s.upload(2, out["w"])
-    t0 = time.perf_counter()
-    s.extrapolate(p)
-    t_ext = time.perf_counter() - t0
-    s.close()
-    step_s = t_adv + fixed + plain_iters_per_step * per_iter + t_ext
-    return dict(value=spec.interior_cells / step_s, unit="cell-steps/s", cores=orc.num_threads(),
-                kind="port",
-                sample=("FP32 oracle (oracle/mac_oracle.hpp, OpenMP) on one %d^3 step of the same "
-                        "scene: boundaries+advection+forces %.2fs, CG %.3fs/iteration (timed at %d "
-                        "and %d iterations) x %.0f iterations/step (the plain GPU path's mean), "
-                        "fixed solve cost %.2fs, extrapolation %.2fs; step %.1fs")
-                % (spec.n, t_adv, per_iter, k_lo, k_hi, plain_iters_per_step, fixed, t_ext, step_s),
-                step_seconds=step_s)
+
+    def timed(sp, n):
+        o = scenes.build_oracle_scene(orc, sp)
+        p = orc.params(sp.dt, rk=1, tol_mode=1, tol=1e-6)
+        ts, its = [], []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            st = o.step(p)
+            ts.append(time.perf_counter() - t0)
+            its.append(st["cg_iterations"])
+        o.close()
+        return np.array(ts), np.array(its)
+
+    d64 = scenes.dam_break(64)
+    t64, i64 = timed(d64, 10)
+    t, its = timed(spec, steps)
+    step_s = float(np.median(t))
+    cores = orc.num_threads()
+    return dict(value=spec.interior_cells / step_s, unit="cell-steps/s", cores=cores, kind="port",
+                sample=("FP32 oracle (oracle/mac_oracle.hpp, OpenMP, %d threads, OMP_NUM_THREADS=%s) timed on "
+                        "%d full steps of the %d^3 scene (median %.2f s/step, %.0f plain CG iterations/step) "
+                        "and on all 10 steps of the 64^3 dam break (median %.3f s/step, %.3g cell-steps/s)")
+                % (cores, os.environ.get("OMP_NUM_THREADS", "unset"), steps, spec.n, step_s, its.mean(),
+                   float(np.median(t64)), d64.interior_cells / float(np.median(t64))),
+                step_seconds=step_s, steps=int(steps), step_seconds_all=[float(x) for x in t],
+                dam_break_64={"steps": 10, "median_step_s": float(np.median(t64)),
+                              "cell_steps_per_s": d64.interior_cells / float(np.median(t64)),
+                              "cg_iterations_per_step": float(i64.mean())})
 
 
 MIXED_KW = dict(rk=1, tol_mode=1, tol=1e-6, precision=1, precond=2)   # RK1, relative 1e-6, MIXED, MULTIGRID
@@ -156,6 +156,7 @@ def plain_one_at_a_time(mgf, scenes, ctx, specs):
         secs += time.perf_counter() - t0
         cells += p.interior_cells if st["converged"] == 1 else 0
         s.close()
+    # the reference's stop rule (FP32 recurrence residual); see true_converged
     return cells / secs if secs > 0 else 0.0
 
 
@@ -173,7 +174,7 @@ def batched_leg(mgf, scenes, ctx, args):
     t0 = time.perf_counter()
     status, stats, ms = mgf.batch_step(ctx, sc, args.batch_steps, workers=args.batch_workers, step_latency=True)
     wall = time.perf_counter() - t0
-    conv = np.array([st.converged == 1 and st.steps == args.batch_steps for st in stats])
+    conv = np.array([st.true_converged == 1 and st.steps == args.batch_steps for st in stats])
     value = float((cells * conv).sum() * args.batch_steps / wall)
     iters = float(np.mean([st.cg_iterations for st in stats]) / max(args.batch_steps, 1))
     # the step-byte model per scene with its own iteration and sweep counts
@@ -265,7 +266,7 @@ def served_leg(mgf, scenes, ctx, args):
             got += drain(n_req - len(got))
             span = (max(r.t_end_ms for r in got) - min(r.t_submit_ms for r in got)) * 1e-3
             lat = np.array([r.t_end_ms - r.t_submit_ms for r in got])
-            ok = [r for r in got if r.status == 0 and r.stats.converged == 1]
+            ok = [r for r in got if r.status == 0 and r.stats.true_converged == 1]
             cs = sum(cells[idx_of[r.scene]] for r in ok)
             row = {"load": frac, "lambda_per_s": lam, "requests": n_req, "converged": len(ok),
                    "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
@@ -343,6 +344,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--precond", choices=sorted(PRECONDS), default="multigrid")
     ap.add_argument("--inner-tol", type=float, default=1e-3)
     ap.add_argument("--mg-sweeps", type=int, default=3)
@@ -378,39 +380,49 @@ def main():
         s = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=tol,
                                    precision=mgf.PRECISION_MIXED, precond=PRECONDS[args.precond],
                                    inner_tol=args.inner_tol, mg_sweeps=args.mg_sweeps, mg_omega=args.mg_omega)
+        # the divergence check (||D u|| / ||b|| of the stored FP32 velocity
+        # after each projection, FP64 on the device) runs inside the timed steps
+        s.set_divergence_check(True)
         run_steps(s, args.warmup)
         s.sync()
         t0 = time.perf_counter()
         r = run_steps(s, args.steps)
         s.sync()
         wall = time.perf_counter() - t0
-        log("mixed: %d steps in %.3fs, %d converged, CG iters/step %.1f, sweeps/step %.2f, max resid %.2e"
-            % (args.steps, wall, r["converged"], r["iters"].mean(), r["sweeps"].mean(), r["res"].max()))
-        value = spec.interior_cells * r["converged"] / wall
+        log("mixed: %d steps in %.3fs, %d converged, CG iters/step %.1f, sweeps/step %.2f, max resid %.2e, "
+            "max div %.2e" % (args.steps, wall, r["converged_true"], r["iters"].mean(), r["sweeps"].mean(),
+                              r["res"].max(), r["div"].max()))
+        value = spec.interior_cells * r["converged_true"] / wall
         # per-stage breakdown and kernel probes: a few more steps with the stage
         # clock on (a synchronisation between stages) and HIP events around
-        # the probed kernels' launches, outside the timed region
+        # every launch, outside the timed region
         probe_steps = 3
         s.set_timing(True)
+        stg = s.step(probe_steps, allow_unconverged=True)
+        s.set_timing(False)
         s.set_kernel_probes(True)
         s.kernel_probes(reset=True)
-        stg = s.step(probe_steps, allow_unconverged=True)
+        s.step(probe_steps, allow_unconverged=True)
         probes = s.kernel_probes(reset=True)
         s.set_kernel_probes(False)
-        s.set_timing(False)
         stages = {k: stg[k] / probe_steps for k in ("ms_advect", "ms_pressure", "ms_extrapolate", "ms_total")}
         log("stages (ms/step): advect+forces %.2f, pressure %.2f, extrapolation %.2f, total %.2f"
             % (stages["ms_advect"], stages["ms_pressure"], stages["ms_extrapolate"], stages["ms_total"]))
         ktab = {}
         for name, p in probes.items():
-            if p["launches"] > 0 and p["ms"] > 0:
-                ktab[name] = {"ms_per_launch": p["ms"] / p["launches"], "launches_per_step": p["launches"] / probe_steps,
-                              "ms_per_step": p["ms"] / probe_steps,
-                              "bytes_per_launch": p["bytes"] / p["launches"],
-                              "GBps": p["bytes"] / (p["ms"] * 1e-3) / 1e9}
-        for name, v in sorted(ktab.items(), key=lambda kv: -kv[1]["ms_per_step"]):
-            log("  %-22s %8.4f ms/launch %6.1f launches/step %7.3f ms/step %6.0f GB/s"
-                % (name, v["ms_per_launch"], v["launches_per_step"], v["ms_per_step"], v["GBps"]))
+            ktab[name] = {"launches_per_step": p["launches"] / probe_steps, "ms_per_step": p["ms"] / probe_steps,
+                          "ms_per_launch": p["ms"] / max(p["launches"], 1),
+                          "busy_launches_per_step": p["busy_launches"] / probe_steps,
+                          "busy_ms_per_launch": p["busy_ms"] / max(p["busy_launches"], 1),
+                          "bytes_per_busy_launch": p["busy_bytes"] / max(p["busy_launches"], 1),
+                          "GBps": (p["busy_bytes"] / (p["busy_ms"] * 1e-3) / 1e9) if p["busy_ms"] > 0 else None}
+        gpu_ms = sum(v["ms_per_step"] for v in ktab.values())
+        for name, v in sorted(ktab.items(), key=lambda kv: -kv[1]["ms_per_step"])[:16]:
+            log("  %-22s %7.3f ms/step (%4.1f%%) %6.1f launches/step, busy %7.4f ms/launch %8s GB/s"
+                % (name, v["ms_per_step"], 100 * v["ms_per_step"] / gpu_ms, v["launches_per_step"],
+                   v["busy_ms_per_launch"], "%.0f" % v["GBps"] if v["GBps"] else "-"))
+        # the dominant kernel: the most GPU time per step over all its
+        # launches, as a kernel trace ranks them
         dominant = max(ktab.items(), key=lambda kv: kv[1]["ms_per_step"])
         pcb = precond_bytes(args.precond, args.mg_sweeps, ncell_padded)
         s.close()
@@ -419,6 +431,7 @@ def main():
         if not args.no_plain and args.plain_steps > 0:
             sp = scenes.build_gpu_scene(mgf, ctx, spec, rk=1, tol_mode=mgf.TOL_RELATIVE, tol=tol,
                                         precision=mgf.PRECISION_PLAIN)
+            sp.set_divergence_check(True)
             run_steps(sp, 1)
             sp.sync()
             t0 = time.perf_counter()
@@ -427,11 +440,17 @@ def main():
             wp = time.perf_counter() - t0
             sp.close()
             plain = dict(value=spec.interior_cells * rp["converged"] / wp, unit="cell-steps/s",
+                         stop_rule="the reference's: FP32 recurrence residual <= 1e-6 relative "
+                                   "(CuMatrixFreePCG.cpp:51-52)",
+                         value_fp64_rule=spec.interior_cells * rp["converged_true"] / wp,
                          steps=args.plain_steps, ms_per_step=1e3 * wp / args.plain_steps,
                          cg_iterations_per_step=float(rp["iters"].mean()),
-                         residual_rel_max=float(rp["res"].max()), converged_steps=rp["converged"],
+                         residual_rel_max=float(rp["res"].max()), div_resid_max=float(rp["div"].max()),
+                         converged_steps=rp["converged"], true_converged_steps=rp["converged_true"],
                          path="plain FP32 CG: one kernel per BLAS-1 op, scalars read by the host")
-            log("plain: %.1f ms/step, CG iters/step %.1f" % (plain["ms_per_step"], plain["cg_iterations_per_step"]))
+            log("plain: %.1f ms/step, CG iters/step %.1f, FP64 residual up to %.2e (%d of %d steps meet 1e-6)"
+                % (plain["ms_per_step"], plain["cg_iterations_per_step"], plain["residual_rel_max"],
+                   rp["converged_true"], args.plain_steps))
 
         # the opt-in warm start (previous pressure as the first iterate) on the same scene
         warm = None
@@ -447,9 +466,9 @@ def main():
             sw.sync()
             ww = time.perf_counter() - t0
             sw.close()
-            warm = dict(value=spec.interior_cells * rw["converged"] / ww, unit="cell-steps/s", steps=args.steps,
+            warm = dict(value=spec.interior_cells * rw["converged_true"] / ww, unit="cell-steps/s", steps=args.steps,
                         ms_per_step=1e3 * ww / args.steps, cg_iterations_per_step=float(rw["iters"].mean()),
-                        residual_rel_max=float(rw["res"].max()), converged_steps=rw["converged"],
+                        residual_rel_max=float(rw["res"].max()), converged_steps=rw["converged_true"],
                         path="mixed precision, CG started from the previous step's pressure (opt-in; the "
                              "headline starts from x = 0 as the reference does)")
             log("warm start: %.2f ms/step, CG iters/step %.1f, max resid %.2e"
@@ -461,18 +480,18 @@ def main():
                      else particles_leg(mgf, scenes, ctx, args, spec))
 
     cpu = None
-    if not args.no_cpu:
-        k_plain = plain["cg_iterations_per_step"] if plain else float(r["iters"].mean())
-        cpu = cpu_baseline(spec, k_plain, 20)
+    if not args.no_cpu and args.cpu_steps > 0:
+        cpu = cpu_baseline(spec, args.cpu_steps)
         log("cpu: " + cpu["sample"])
 
     ms = np.array(r["times"]) * 1e3
     k = float(r["iters"].mean())
     rs = float(r["sweeps"].mean())
-    # per interior cell: fixed stages 109 B, per CG iteration 36 B + the
+    # per interior cell: fixed stages 109 B, per CG iteration 46 B + the
     # preconditioner, per refinement sweep 25 B (cell counts of the padded and
     # interior grids differ by 2%; the interior count is used throughout)
     step_bytes = spec.interior_cells * (109 + (BYTES_P_AP + BYTES_UPDATE + pcb) * k + 25 * rs)
+    dk = dominant[1]
     out = {
         "metric": METRIC,
         "value": value,
@@ -495,19 +514,21 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dominant[0],
-            "achieved": dominant[1]["GBps"],
+            "achieved": dk["GBps"],
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
-            "frac": dominant[1]["GBps"] / HBM_PEAK_GBPS,
+            "frac": (dk["GBps"] / HBM_PEAK_GBPS) if dk["GBps"] else None,
             "traffic": traffic_for(dominant[0]),
-            "bytes_per_launch": dominant[1]["bytes_per_launch"],
-            "ms_per_launch": dominant[1]["ms_per_launch"],
-            "note": ("HIP events recorded at each launch's own start and end (hipExtLaunchKernelGGL) during %d "
-                     "ordinary steps after the timed ones; CG/multigrid launches that returned at once after "
-                     "convergence are left out.  The fused multigrid descent / ascent (k_mg_descend, "
-                     "k_mg_ascend) are bound by instruction issue and LDS/barrier latency, not HBM "
-                     "(rocprofv3 PMC at 256^3: 43%% VALU issue); their algorithmic bytes are 10.5 and 14.5 B "
-                     "per cell, against 77 B for the six per-sweep launches they replace" % probe_steps),
+            "bytes_per_launch": dk["bytes_per_busy_launch"],
+            "ms_per_launch": dk["busy_ms_per_launch"],
+            "ms_per_step": dk["ms_per_step"],
+            "share_of_gpu_time": dk["ms_per_step"] / gpu_ms,
+            "note": ("HIP events recorded at every launch's own start and end (hipExtLaunchKernelGGL) during %d "
+                     "ordinary steps after the timed ones, aggregated per kernel (template instantiations and "
+                     "multigrid levels together); the dominant kernel has the most GPU time per step over all "
+                     "its launches, as the rocprofv3 kernel trace ranks them; achieved = algorithmic bytes of "
+                     "its busy launches (those that did not return at once after convergence) / their summed "
+                     "duration" % probe_steps),
             "kernels": ktab,
         },
         "step_roofline_frac": step_bytes / (1e-3 * ms.mean() * HBM_PEAK_GBPS * 1e9),
@@ -520,10 +541,17 @@ def main():
         "cg_iterations_per_step": k,
         "refinement_sweeps_per_step": rs,
         "residual_rel_max": float(r["res"].max()),
-        "converged_steps": r["converged"],
+        "div_resid_max": float(r["div"].max()),
+        "div_resid_note": ("||D u||_2 / ||b||_2 over the fluid cells from the stored FP32 velocity after each "
+                           "projection, in FP64: the FP64 pressure residual (residual_rel_max) plus the rounding "
+                           "of the projected velocity to FP32, which grows with |u| / ||b||"),
+        "converged_steps": r["converged_true"],
         "plain_fp32_gpu": plain,
         "warm_start": warm,
         "speedup_vs_plain": (value / plain["value"]) if plain and plain["value"] > 0 else None,
+        "speedup_vs_plain_note": ("plain counted by the reference's stop rule (FP32 recurrence residual); "
+                                  "by the FP64 residual of its pressure none of its steps meet 1e-6 at this size")
+                                 if plain and plain["true_converged_steps"] == 0 else None,
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,

@@ -64,6 +64,8 @@ void merge_stats(mgf_step_stats* acc, const mgf_step_stats& one, bool first) {
     acc->ms_extrapolate += one.ms_extrapolate;
     acc->ms_total += one.ms_total;
     acc->fluid_cells = one.fluid_cells;
+    acc->true_converged = acc->true_converged && one.true_converged;
+    if (!(one.div_residual <= acc->div_residual)) acc->div_residual = one.div_residual;
 }
 
 // Runs `nsteps` steps of one scene; with step_ms, one synchronised step at a
@@ -322,16 +324,19 @@ mgf_status mgf_server_close_scene(mgf_server srv, mgf_scene scene) {
 mgf_status mgf_server_submit(mgf_server srv, mgf_scene scene, int32_t nsteps, int64_t* ticket) {
     if (!srv || !scene) return fail(MGF_ERR_INVALID, "null server or scene");
     if (nsteps < 0) return fail(MGF_ERR_INVALID, "nsteps < 0");
-    // solids added since admission count against the budget from now on
-    int64_t bytes = 0;
-    const mgf_status st = mgf_scene_device_bytes(scene, &bytes);
-    if (st != MGF_OK) return st;
     std::lock_guard<std::mutex> lk(srv->mu);
+    // the handle is only dereferenced once it is known to be open here
     auto it = srv->scenes.find(scene);
     if (it == srv->scenes.end() || it->second.closing) return fail(MGF_ERR_INVALID, "scene is not open on this server");
     auto& r = it->second;
-    srv->resident += bytes - r.bytes;
-    r.bytes = bytes;
+    // solids added since admission count against the budget from now on;
+    // while a worker steps the scene, the worker refreshes the count after
+    // the request instead
+    if (!r.running) {
+        const int64_t bytes = scene->device_bytes;
+        srv->resident += bytes - r.bytes;
+        r.bytes = bytes;
+    }
     const int64_t t = srv->next_ticket;
     try {
         r.pending.push_back({t, nsteps, now_ms()});

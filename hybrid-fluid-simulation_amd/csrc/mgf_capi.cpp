@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -16,14 +17,66 @@ using namespace mgf;
 namespace mgf {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
-static thread_local LaunchProbe g_launch_probe;
-LaunchProbe& launch_probe() { return g_launch_probe; }
+static thread_local ProbeSink* g_probe_sink = nullptr;
+static thread_local double g_probe_bytes = 0.0;
+ProbeSink*& probe_sink() { return g_probe_sink; }
+void probe_next_bytes(double bytes) { g_probe_bytes = bytes; }
+double probe_take_bytes() {
+    const double b = g_probe_bytes;
+    g_probe_bytes = 0.0;
+    return b;
+}
+
+bool ProbeSink::next_pair(hipEvent_t& a, hipEvent_t& b) {
+    if (ev.size() < 2 * (used + 1)) {
+        hipEvent_t x = nullptr, y = nullptr;
+        if (hipEventCreate(&x) != hipSuccess || hipEventCreate(&y) != hipSuccess) {
+            (void)hipGetLastError();
+            if (x) (void)hipEventDestroy(x);
+            return false;
+        }
+        ev.push_back(x);
+        ev.push_back(y);
+    }
+    a = ev[2 * used];
+    b = ev[2 * used + 1];
+    return true;
+}
+
+// Template instantiations are reported under the kernel's base name.
+void ProbeSink::collect() {
+    for (size_t k = 0; k < used; ++k) {
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        std::string nm(name[k]);
+        const size_t lt = nm.find('<');
+        if (lt != std::string::npos) nm.resize(lt);
+        done_name.push_back(nm);
+        done_ms.push_back(double(t));
+        done_bytes.push_back(bytes[k]);
+    }
+    used = 0;
+    name.clear();
+    bytes.clear();
+}
+
+void ProbeSink::release() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    name.clear();
+    bytes.clear();
+    used = 0;
+}
 }  // namespace mgf
 
 namespace {
 
 constexpr int kPlainDotBlocks = 1024;
 constexpr int kResidualBlocks = 1024;
+constexpr int kDivBlocks = 1024;          // divergence check (grid-stride)
 
 double now_ms() {
     using namespace std::chrono;
@@ -123,67 +176,17 @@ void scales(const mgf_scene_s* s, float sa[3], float sg[3]) {
     }
 }
 
-// ---- kernel probes: HIP event pairs around selected launches, read at the
-// next synchronisation of a stage entry point.
-int probe_begin(mgf_scene s) {
-    if (!s->probes) return -1;
-    const size_t k = s->probe_used;
-    if (s->probe_ev.size() < 2 * (k + 1)) {
-        hipEvent_t a = nullptr, b = nullptr;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-            (void)hipGetLastError();
-            if (a) (void)hipEventDestroy(a);
-            return -1;
-        }
-        s->probe_ev.push_back(a);
-        s->probe_ev.push_back(b);
-        s->probe_kind.push_back(0);
-        s->probe_bytes.push_back(0.0);
-    }
-    // armed for the next launch of this thread: the probed kernel
-    LaunchProbe& p = launch_probe();
-    p.start = s->probe_ev[2 * k];
-    p.stop = s->probe_ev[2 * k + 1];
-    p.used = false;
-    return int(k);
-}
-
-void probe_end(mgf_scene s, int k, int kind, double bytes) {
-    if (k < 0) return;
-    LaunchProbe& p = launch_probe();
-    const bool launched = p.used;
-    p.start = p.stop = nullptr;
-    p.used = false;
-    if (!launched) return;   // nothing was launched (an empty layer)
-    s->probe_kind[size_t(k)] = kind;
-    s->probe_bytes[size_t(k)] = bytes;
-    s->probe_used = size_t(k) + 1;
-}
-
-// The stream is synchronised.  A CG or multigrid launch that returned at once
-// because the solve had converged (shorter than a quarter of the longest
-// launch of its kind seen so far) moved no data and is left out.
-void probe_collect(mgf_scene s) {
-    for (size_t k = 0; k < s->probe_used; ++k) {
-        float t = 0.0f;
-        if (hipEventElapsedTime(&t, s->probe_ev[2 * k], s->probe_ev[2 * k + 1]) != hipSuccess) {
-            (void)hipGetLastError();
-            continue;
-        }
-        const int kind = s->probe_kind[k];
-        s->probe_max[kind] = std::max(s->probe_max[kind], double(t));
-        const bool may_exit_early = kind != MGF_PROBE_EXT_LAYER && kind != MGF_PROBE_ADVECT_FC;
-        if (may_exit_early && double(t) < 0.25 * s->probe_max[kind]) continue;
-        s->probe_ms[kind] += t;
-        s->probe_b[kind] += s->probe_bytes[k];
-        s->probe_n[kind] += 1;
-    }
-    s->probe_used = 0;
-}
+// ---- kernel probes: the scene's sink is installed for the host thread while
+// one of its entry points runs (ProbeSink, mgf_internal.h).
+struct ProbeScope {
+    ProbeSink* prev;
+    explicit ProbeScope(mgf_scene s) : prev(probe_sink()) { probe_sink() = s->probes ? &s->sink : nullptr; }
+    ~ProbeScope() { probe_sink() = prev; }
+};
 
 mgf_status sync(mgf_scene s) {
     MGF_HIP(hipStreamSynchronize(s->stream));
-    probe_collect(s);
+    if (s->sink.used) s->sink.collect();
     return MGF_OK;
 }
 
@@ -261,15 +264,16 @@ mgf_status advect(mgf_scene s, bool with_forces) {
         addf = on(f);
     }
     Vel V = vel_of(s->u[c], s->v[c], s->w[c]);
-    const int pk = probe_begin(s);
+    probe_next_bytes(12.0 * double(s->nfx + s->nfy + s->nfz));
     MGF_HIP(launch_advect_fc(st, s->g, V, V, dt, s->desc.rk_order, s->u[n], s->v[n], s->w[n], addg,
                              g, addf, f));
-    probe_end(s, pk, MGF_PROBE_ADVECT_FC, 12.0 * double(s->nfx + s->nfy + s->nfz));
     if (s->color[0][0]) {
         const float* ic[3] = {s->color[c][0], s->color[c][1], s->color[c][2]};
         float* oc[3] = {s->color[n][0], s->color[n][1], s->color[n][2]};
+        probe_next_bytes(32.0 * double(s->ncell));
         MGF_HIP(launch_advect_cc4(st, s->g, s->phi[c], ic, V, dt, s->desc.rk_order, s->phi[n], oc));
     } else {
+        probe_next_bytes(8.0 * double(s->ncell));
         MGF_HIP(launch_advect_cc(st, s->g, s->phi[c], V, dt, s->desc.rk_order, s->phi[n]));
     }
     s->cur = n;
@@ -353,6 +357,11 @@ mgf_status cg_plain(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     const double b2 = host_sum(s->host_scratch, kResidualBlocks, 2, 1);
     acc->cg_iterations += it;
     acc->converged = acc->converged && conv;
+    // the same stop rule on the FP64 residual of the FP32 pressure
+    const double rt = std::sqrt(rr), bt = std::sqrt(b2);
+    const bool true_ok = d.tol_mode == MGF_TOL_ABS_SQRT_NORM ? std::sqrt(rt) < d.tolerance
+                                                             : (bt == 0.0 ? rt == 0.0 : rt <= d.tolerance * bt);
+    acc->true_converged = acc->true_converged && true_ok;
     acc->residual_rel = b2 > 0 ? std::sqrt(rr / b2) : std::sqrt(rr);
     acc->residual_recurrence = rn_last;
     acc->b_norm = std::sqrt(b2);
@@ -493,10 +502,10 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     const int L = 1 + int(s->mg.size());
     struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint8_t* wet; };
     Lv lv[17];
-    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->wet0};
+    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->skip_dry ? s->wet0 : nullptr};
     for (int l = 1; l < L; ++l) {
         auto& m = s->mg[size_t(l - 1)];
-        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, m.wet};
+        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, s->skip_dry ? m.wet : nullptr};
     }
     // levels [l0, L) run in one work-group
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
@@ -511,13 +520,12 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         Lv& v = lv[l];
         const bool coarsest = l == L - 1;
         if (fused(l)) {
-            const int pk = l == 0 ? probe_begin(s) : -1;
-            MGF_HIP(launch_mg_descend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
-                                      const_cast<float*>(lv[l + 1].b), omega, s->scal, v.wet));
             // blocks without fluid store x = 0 and the coarse b = 0 only
             const double nl = double(v.d.nx) * v.d.ny * v.d.nz;
             const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : nl;
-            probe_end(s, pk, MGF_PROBE_MG_DESCEND, 10.5 * wet + 4.5 * (nl - wet));
+            probe_next_bytes(10.5 * wet + 4.5 * (nl - wet));
+            MGF_HIP(launch_mg_descend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
+                                      const_cast<float*>(lv[l + 1].b), omega, s->scal, v.wet));
             continue;
         }
         const int sweeps = coarsest ? kCoarsestSweeps : nu;
@@ -525,25 +533,22 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         int k;
         const double ncell = double(v.d.nx) * v.d.ny * v.d.nz;
         if (sweeps >= 2) {
-            const int pk = l == 0 ? probe_begin(s) : -1;
+            probe_next_bytes(10.0 * ncell);
             MGF_HIP(launch_mg_smooth2_first(st, v.d, v.codes, v.b, v.x, omega, s->scal, s->red, rz_at(1)));
-            probe_end(s, pk, MGF_PROBE_MG_SMOOTH2, 10.0 * ncell);
             k = 2;
         } else {
             MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.b, v.x, omega, s->scal, s->red, 1, rz_at(0)));
             k = 1;
         }
         for (; k < sweeps; ++k) {
-            const int pk = (l == 0 && !rz_at(k)) ? probe_begin(s) : -1;
+            probe_next_bytes(14.0 * ncell);
             MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k)));
-            probe_end(s, pk, MGF_PROBE_MG_SMOOTH, 14.0 * ncell);
             std::swap(v.x, v.t);
         }
         if (!coarsest) {
-            const int pk = l == 0 ? probe_begin(s) : -1;
+            probe_next_bytes(10.5 * ncell);
             MGF_HIP(launch_mg_restrict(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
                                        const_cast<float*>(lv[l + 1].b), s->scal));
-            probe_end(s, pk, MGF_PROBE_MG_RESTRICT, 10.5 * ncell);
         }
     }
     if (l0 < L) {
@@ -563,23 +568,20 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         auto rz_at = [&](int k) { return (l == 0 && k == nu - 1) ? 1 : 0; };
         const double ncell = double(v.d.nx) * v.d.ny * v.d.nz;
         if (fused(l)) {
-            const int pf = l == 0 ? probe_begin(s) : -1;
+            const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : ncell;
+            probe_next_bytes(14.5 * wet + 4.0 * (ncell - wet));
             MGF_HIP(launch_mg_ascend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
                                      s->red, l == 0 ? 1 : 0, v.wet));
-            const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : ncell;
-            probe_end(s, pf, MGF_PROBE_MG_ASCEND, 14.5 * wet + 4.0 * (ncell - wet));
             std::swap(v.x, v.t);
             continue;
         }
-        int pk = (l == 0 && !rz_at(0)) ? probe_begin(s) : -1;
+        probe_next_bytes(14.5 * ncell);
         MGF_HIP(launch_mg_smooth_prolong(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
                                          s->red, rz_at(0)));
-        probe_end(s, pk, MGF_PROBE_MG_PROLONG, 14.5 * ncell);
         std::swap(v.x, v.t);
         for (int k = 1; k < nu; ++k) {
-            pk = (l == 0 && !rz_at(k)) ? probe_begin(s) : -1;
+            probe_next_bytes(14.0 * ncell);
             MGF_HIP(launch_mg_smooth(st, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k)));
-            probe_end(s, pk, MGF_PROBE_MG_SMOOTH, 14.0 * ncell);
             std::swap(v.x, v.t);
         }
     }
@@ -615,12 +617,13 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     // MIXED by mgf_scene_set_params gets it here
     if (!s->wet_flat) TRY(dalloc(s, &s->wet_flat, col_blocks(s->g)));
     MGF_HIP(launch_flat_wet(st, s->g, s->codes, s->wet_flat));
+    const uint8_t* wet_flat = s->skip_dry ? s->wet_flat : nullptr;
     if (!ident) {
         TRY(ensure_precond(s));
         TRY(mg_setup(s, sa));
     }
     s->wet_flat_cells = s->wet0_cells = -1.0;
-    if (s->probes) {
+    if (s->probes && s->skip_dry) {
         // the bytes probed launches move: blocks without fluid move (almost) nothing
         std::vector<uint8_t> h(col_blocks(s->g));
         MGF_HIP(hipMemcpyAsync(h.data(), s->wet_flat, h.size(), hipMemcpyDeviceToHost, st));
@@ -675,14 +678,12 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
             const int64_t left = max_it - it_total - done_before;
             const int k = int(std::min<int64_t>(predict(rr_now, sweep_target), std::max<int64_t>(left, 1)));
             for (int q = 0; q < k; ++q) {
-                int pk = probe_begin(s);
-                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red, s->wet_flat));
-                probe_end(s, pk, MGF_PROBE_CG_PAP, 22.0 * (first_pap ? ncell_all : flat_cells));
+                probe_next_bytes(22.0 * (first_pap ? ncell_all : flat_cells));
+                MGF_HIP(launch_col_p_ap(st, s->g, sa, s->codes, zv, s->p0, s->p1, s->scal, s->red, wet_flat));
                 first_pap = false;
-                pk = probe_begin(s);
+                probe_next_bytes(24.0 * flat_cells);
                 MGF_HIP(launch_col_update(st, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, ident ? 1 : 0,
-                                          s->wet_flat));
-                probe_end(s, pk, MGF_PROBE_CG_UPDATE, 24.0 * flat_cells);
+                                          wet_flat));
                 if (!ident) TRY(apply_precond(s, sa, &zv));
             }
             MGF_HIP(hipMemcpyAsync(&hs, s->scal, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
@@ -710,6 +711,7 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
     acc->cg_iterations += it_total;
     acc->refinement_sweeps += sweeps;
     acc->converged = acc->converged && conv;
+    acc->true_converged = acc->true_converged && conv;
     acc->residual_rel = b2 > 0 ? std::sqrt(rr_true / b2) : std::sqrt(rr_true);
     acc->residual_recurrence = std::sqrt(hs.rr);
     acc->b_norm = std::sqrt(b2);
@@ -724,15 +726,23 @@ mgf_status pressure(mgf_scene s, mgf_step_stats* acc, mgf_particles parts = null
     float sa[3], sg[3];
     scales(s, sa, sg);
     if (parts) MGF_HIP(launch_particle_markers_u8(st, s->g, s->solid_mask, parts->n, parts->pos, s->markers));
-    else MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
+    else {
+        probe_next_bytes(6.0 * double(s->ncell));
+        MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
+    }
+    probe_next_bytes(41.0 * double(s->ncell));
     MGF_HIP(launch_rhs(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), vel_of(s->us, s->vs, s->ws),
                        s->markers, s->b));
     const int64_t max_it = s->desc.max_iterations > 0 ? s->desc.max_iterations : int64_t(s->ncell);
     if (s->desc.precision == MGF_PRECISION_PLAIN) TRY(cg_plain(s, sa, max_it, acc));
     else TRY(cg_mixed(s, sa, max_it, acc));
+    probe_next_bytes(33.0 * double(s->ncell));   // markers 1, FP64 pressure 8, faces read 12 and written 12
     MGF_HIP(launch_project(st, s->g, sg, s->markers, s->x_is_64 ? nullptr : s->x,
                            s->x_is_64 ? s->x64 : nullptr, s->u[c], s->v[c], s->w[c], s->us, s->vs,
                            s->ws));
+    if (s->div_check)
+        MGF_HIP(launch_div_check(st, s->g, s->markers, s->u[c], s->v[c], s->w[c], s->b, s->red.partials,
+                                 kDivBlocks, s->div_dev));
     if (!std::isfinite(acc->residual_rel)) return fail(MGF_ERR_NONFINITE, "pressure solve produced NaN/Inf");
     return MGF_OK;
 }
@@ -772,6 +782,7 @@ mgf_status alloc_scene(mgf_scene s) {
     A(&s->scal, 1);
     A(&s->red.partials, size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2);
     A(&s->red.out2, 2);
+    A(&s->div_dev, 3);
     if (st == MGF_OK) st = alloc_extrapolation(s, s->nfx + s->nfy + s->nfz);
     return st;
 }
@@ -780,8 +791,7 @@ mgf_status alloc_scene(mgf_scene s) {
 // Extrapolation: distances, counting sort by distance, one gather kernel per
 // layer; returns the largest non-empty layer (< layers).
 mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
-    const int32_t K = ext_max_distance(s->g);
-    if (K >= 65535) return fail(MGF_ERR_UNSUPPORTED, "grid too long for 16-bit face distances (nx + ny + nz + 3 must be below 65535)");
+    const int32_t K = std::min<int32_t>(ext_max_distance(s->g), 65535);
     TRY(dalloc(s, &s->layer, nface));
     TRY(dalloc(s, &s->list, nface));
     TRY(dalloc(s, &s->lmask, nface));
@@ -797,41 +807,8 @@ mgf_status alloc_extrapolation(mgf_scene s, size_t nface) {
 }
 
 mgf_status extrapolate(mgf_scene s, float* u, float* v, float* w, int layers, int* used) {
-    hipStream_t st = s->stream;
-    *used = 0;
-    const int32_t K = ext_max_distance(s->g);
-    const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
-    if (cap <= 0) return MGF_OK;   // layer 0 only: the reference's fill loop is empty
-    uint16_t* dist = reinterpret_cast<uint16_t*>(s->layer);
-    MGF_HIP(launch_ext_distance(st, s->g, u, v, w, dist));
-    const uint32_t* offs = nullptr;
-    const uint32_t* maxd = nullptr;
-    MGF_HIP(launch_ext_sort(st, s->g, dist, cap, K, s->counts, s->list, s->lmask, s->fmask, &offs, &maxd));
-    // offsets of layers 0..cap+1, then the largest distance present
-    MGF_HIP(hipMemcpyAsync(s->host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
-    MGF_HIP(hipMemcpyAsync(s->host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    MGF_HIP(hipStreamSynchronize(st));
-    const int32_t top = int32_t(s->host_offs[cap + 2]);
-    // Small layers are launch-bound: two of them share a launch (the second
-    // recomputes its previous-layer neighbours, several loads per face).
-    // Large layers are bound by their gathers, which the pairing would
-    // multiply, so they run one per launch.
-    constexpr uint32_t kPairMax = 65536;   // faces of two layers launched together
-    for (int32_t L = 1; L <= top;) {
-        const uint32_t b = s->host_offs[L], m = s->host_offs[L + 1];
-        const uint32_t e = L + 1 <= top ? s->host_offs[L + 2] : m;
-        const bool pair = L + 1 <= top && e - b <= kPairMax;
-        const int pk = probe_begin(s);
-        if (pair)
-            MGF_HIP(launch_ext_layer2(st, s->g, u, v, w, s->list, s->lmask, s->fmask, b, m - b, e - m));
-        else
-            MGF_HIP(launch_ext_layer(st, s->g, u, v, w, s->list, s->lmask, b, m - b));
-        // per entry: list 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
-        probe_end(s, pk, MGF_PROBE_EXT_LAYER, 13.0 * double((pair ? e : m) - b));
-        L += pair ? 2 : 1;
-    }
-    *used = top;
-    return MGF_OK;
+    ExtBufs B{reinterpret_cast<uint16_t*>(s->layer), s->list, s->lmask, s->fmask, s->counts, s->host_offs};
+    return extrapolate_run(s->stream, s->g, u, v, w, layers, B, used);
 }
 
 mgf_status step_once(mgf_scene s, mgf_step_stats* acc) {
@@ -902,6 +879,25 @@ mgf_status step_particles_once(mgf_scene s, mgf_psolver ps, mgf_step_stats* acc)
 void stats_reset(mgf_step_stats* st) {
     std::memset(st, 0, sizeof(*st));
     st->converged = 1;
+    st->true_converged = 1;
+    st->div_residual = -1.0;
+}
+
+// The divergence check's running maximum: cleared before the solves of one
+// entry-point call, read after its final synchronisation.
+mgf_status div_begin(mgf_scene s) {
+    if (!s->div_check) return MGF_OK;
+    MGF_HIP(launch_fill64(s->stream, s->div_dev, 1, -1.0));
+    return MGF_OK;
+}
+
+mgf_status div_end(mgf_scene s, mgf_step_stats* acc) {
+    if (!s->div_check) return MGF_OK;
+    double h = -1.0;
+    MGF_HIP(hipMemcpyAsync(&h, s->div_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    MGF_HIP(hipStreamSynchronize(s->stream));
+    acc->div_residual = h;
+    return MGF_OK;
 }
 
 }  // namespace
@@ -919,6 +915,46 @@ mgf_status scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t*
     if (desc->precision == MGF_PRECISION_MIXED && desc->preconditioner != MGF_PRECOND_IDENTITY)
         TRY(ensure_precond(&dry));
     *out = dry.device_bytes;
+    return MGF_OK;
+}
+
+mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v, float* w, int layers,
+                           const ExtBufs& B, int* used) {
+    *used = 0;
+    const int32_t K = ext_max_distance(g);
+    const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
+    if (cap <= 0) return MGF_OK;   // layer 0 only: the reference's fill loop is empty
+    if (K >= 65535)
+        return fail(MGF_ERR_UNSUPPORTED, "extrapolation over more than one layer needs nx + ny + nz + 3 < 65535 "
+                                         "(16-bit face distances)");
+    uint16_t* dist = B.dist;
+    MGF_HIP(launch_ext_distance(st, g, u, v, w, dist));
+    const uint32_t* offs = nullptr;
+    const uint32_t* maxd = nullptr;
+    MGF_HIP(launch_ext_sort(st, g, dist, cap, K, B.counts, B.list, B.lmask, B.fmask, &offs, &maxd));
+    // offsets of layers 0..cap+1, then the largest distance present
+    MGF_HIP(hipMemcpyAsync(B.host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipMemcpyAsync(B.host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    MGF_HIP(hipStreamSynchronize(st));
+    const int32_t top = int32_t(B.host_offs[cap + 2]);
+    // Small layers are launch-bound: two of them share a launch (the second
+    // recomputes its previous-layer neighbours, several loads per face).
+    // Large layers are bound by their gathers, which the pairing would
+    // multiply, so they run one per launch.
+    constexpr uint32_t kPairMax = 65536;   // faces of two layers launched together
+    for (int32_t L = 1; L <= top;) {
+        const uint32_t b = B.host_offs[L], m = B.host_offs[L + 1];
+        const uint32_t e = L + 1 <= top ? B.host_offs[L + 2] : m;
+        const bool pair = L + 1 <= top && e - b <= kPairMax;
+        // per entry: list 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
+        probe_next_bytes(13.0 * double((pair ? e : m) - b));
+        if (pair)
+            MGF_HIP(launch_ext_layer2(st, g, u, v, w, B.list, B.lmask, B.fmask, b, m - b, e - m));
+        else
+            MGF_HIP(launch_ext_layer(st, g, u, v, w, B.list, B.lmask, b, m - b));
+        L += pair ? 2 : 1;
+    }
+    *used = top;
     return MGF_OK;
 }
 
@@ -1067,7 +1103,7 @@ mgf_status mgf_scene_destroy(mgf_scene s) {
     for (auto& so : s->solids) (void)hipFree(so.sdf0);
     if (s->host_scratch) (void)hipHostFree(s->host_scratch);
     if (s->host_offs) (void)hipHostFree(s->host_offs);
-    for (hipEvent_t e : s->probe_ev) (void)hipEventDestroy(e);
+    s->sink.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return MGF_OK;
@@ -1276,28 +1312,34 @@ mgf_status mgf_scene_download_device(mgf_scene s, int32_t field, float* dev, int
 
 mgf_status mgf_scene_update_boundaries(mgf_scene s) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    ProbeScope probe(s);
     TRY(refresh_boundaries(s));
     return sync(s);
 }
 
 mgf_status mgf_scene_advect(mgf_scene s) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    ProbeScope probe(s);
     TRY(advect(s, false));
     return sync(s);
 }
 
 mgf_status mgf_scene_apply_forces(mgf_scene s) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    ProbeScope probe(s);
     TRY(apply_forces(s));
     return sync(s);
 }
 
 mgf_status mgf_scene_pressure_solve(mgf_scene s, mgf_step_stats* stats) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    ProbeScope probe(s);
     mgf_step_stats acc;
     stats_reset(&acc);
+    TRY(div_begin(s));
     TRY(pressure(s, &acc));
     TRY(sync(s));
+    TRY(div_end(s, &acc));
     if (stats) *stats = acc;
     if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "pressure solve stopped before its tolerance");
     return MGF_OK;
@@ -1307,10 +1349,13 @@ mgf_status mgf_scene_pressure_solve_particles(mgf_scene s, mgf_particles parts, 
     if (!s || !parts) return fail(MGF_ERR_INVALID, "null argument");
     if (parts->ctx->device != s->ctx->device) return fail(MGF_ERR_INVALID, "particles on another device");
     MGF_HIP(hipSetDevice(s->ctx->device));
+    ProbeScope probe(s);
     mgf_step_stats acc;
     stats_reset(&acc);
+    TRY(div_begin(s));
     TRY(pressure(s, &acc, parts));
     TRY(sync(s));
+    TRY(div_end(s, &acc));
     if (stats) *stats = acc;
     if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "pressure solve stopped before its tolerance");
     return MGF_OK;
@@ -1328,6 +1373,7 @@ mgf_status mgf_scene_apply_preconditioner(mgf_scene s, const float* r, float* z,
     if (d.precision != MGF_PRECISION_MIXED || d.preconditioner == MGF_PRECOND_IDENTITY)
         return fail(MGF_ERR_UNSUPPORTED, "only the MIXED solver's JACOBI or MULTIGRID preconditioner");
     MGF_HIP(hipSetDevice(s->ctx->device));
+    ProbeScope probe(s);
     hipStream_t st = s->stream;
     float sa[3], sg[3];
     scales(s, sa, sg);
@@ -1371,8 +1417,10 @@ mgf_status mgf_step_particles(mgf_scene s, mgf_psolver ps, int32_t nsteps, mgf_s
     if (!same_grid(s->desc, ps->grid)) return fail(MGF_ERR_INVALID, "the solver's grid is not the scene's");
     if (ps->ctx->device != s->ctx->device) return fail(MGF_ERR_INVALID, "solver on another device");
     MGF_HIP(hipSetDevice(s->ctx->device));
+    ProbeScope probe(s);
     mgf_step_stats acc;
     stats_reset(&acc);
+    TRY(div_begin(s));
     for (int i = 0; i < nsteps; ++i) {
         const mgf_status r = step_particles_once(s, ps, &acc);
         if (r != MGF_OK) {
@@ -1381,6 +1429,7 @@ mgf_status mgf_step_particles(mgf_scene s, mgf_psolver ps, int32_t nsteps, mgf_s
         }
     }
     TRY(sync(s));
+    TRY(div_end(s, &acc));
     if (stats) *stats = acc;
     if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "a pressure solve stopped before its tolerance");
     return MGF_OK;
@@ -1388,6 +1437,7 @@ mgf_status mgf_step_particles(mgf_scene s, mgf_psolver ps, int32_t nsteps, mgf_s
 
 mgf_status mgf_scene_extrapolate_velocity(mgf_scene s, int32_t* layers_used) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    ProbeScope probe(s);
     int used = 0;
     TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
     if (layers_used) *layers_used = used;
@@ -1398,10 +1448,13 @@ mgf_status mgf_step(mgf_scene s, int32_t nsteps, mgf_step_stats* stats) {
     if (!s) return fail(MGF_ERR_INVALID, "null scene");
     if (nsteps < 0) return fail(MGF_ERR_INVALID, "nsteps < 0");
     MGF_HIP(hipSetDevice(s->ctx->device));
+    ProbeScope probe(s);
     mgf_step_stats acc;
     stats_reset(&acc);
+    TRY(div_begin(s));
     for (int i = 0; i < nsteps; ++i) TRY(step_once(s, &acc));
     TRY(sync(s));
+    TRY(div_end(s, &acc));
     if (stats) *stats = acc;
     if (!acc.converged) return fail(MGF_ERR_NOT_CONVERGED, "a pressure solve stopped before its tolerance");
     return MGF_OK;
@@ -1424,20 +1477,61 @@ mgf_status mgf_scene_set_kernel_probes(mgf_scene s, int32_t enabled) {
     return MGF_OK;
 }
 
-mgf_status mgf_scene_kernel_probes(mgf_scene s, double* ms, int64_t* launches, double* bytes, int32_t reset) {
-    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+mgf_status mgf_scene_kernel_probes(mgf_scene s, mgf_kernel_probe* out, int32_t cap, int32_t* count,
+                                   int32_t reset) {
+    if (!s || !count || cap < 0 || (cap > 0 && !out)) return fail(MGF_ERR_INVALID, "null argument");
     MGF_HIP(hipSetDevice(s->ctx->device));
     TRY(sync(s));
-    for (int k = 0; k < MGF_PROBE_COUNT; ++k) {
-        if (ms) ms[k] = s->probe_ms[k];
-        if (launches) launches[k] = s->probe_n[k];
-        if (bytes) bytes[k] = s->probe_b[k];
-        if (reset) {
-            s->probe_ms[k] = 0.0;
-            s->probe_b[k] = 0.0;
-            s->probe_n[k] = 0;
+    ProbeSink& k = s->sink;
+    // per name, in order of first appearance
+    std::vector<std::string> names;
+    std::vector<double> longest;
+    auto index_of = [&](const std::string& nm) {
+        for (size_t q = 0; q < names.size(); ++q)
+            if (names[q] == nm) return q;
+        names.push_back(nm);
+        longest.push_back(0.0);
+        return names.size() - 1;
+    };
+    std::vector<size_t> idx(k.done_name.size());
+    for (size_t r = 0; r < k.done_name.size(); ++r) {
+        idx[r] = index_of(k.done_name[r]);
+        longest[idx[r]] = std::max(longest[idx[r]], k.done_ms[r]);
+    }
+    std::vector<mgf_kernel_probe> agg(names.size());
+    for (size_t q = 0; q < names.size(); ++q) {
+        std::memset(&agg[q], 0, sizeof(mgf_kernel_probe));
+        std::snprintf(agg[q].name, sizeof(agg[q].name), "%s", names[q].c_str());
+    }
+    for (size_t r = 0; r < k.done_name.size(); ++r) {
+        mgf_kernel_probe& a = agg[idx[r]];
+        a.launches += 1;
+        a.ms += k.done_ms[r];
+        if (k.done_ms[r] >= 0.25 * longest[idx[r]]) {
+            a.busy_launches += 1;
+            a.busy_ms += k.done_ms[r];
+            a.busy_bytes += k.done_bytes[r];
         }
     }
+    *count = int32_t(agg.size());
+    for (int32_t q = 0; q < cap && size_t(q) < agg.size(); ++q) out[q] = agg[size_t(q)];
+    if (reset) {
+        k.done_name.clear();
+        k.done_ms.clear();
+        k.done_bytes.clear();
+    }
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_set_divergence_check(mgf_scene s, int32_t enabled) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    s->div_check = enabled != 0;
+    return MGF_OK;
+}
+
+mgf_status mgf_scene_set_block_skip(mgf_scene s, int32_t enabled) {
+    if (!s) return fail(MGF_ERR_INVALID, "null scene");
+    s->skip_dry = enabled != 0;
     return MGF_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+
 #include "../../include/mgf.h"
 #include "mgf_device.h"
 
@@ -49,6 +50,25 @@ struct MgLevelBuf {
     float* t = nullptr;     // Jacobi ping-pong partner of x
     uint8_t* wet = nullptr; // fused-march blocks with fluid (this level as the fine one)
     size_t wet_n = 0;
+};
+
+// Kernel probes (measurement).  While a probed scene's entry point runs, its
+// sink is installed for the host thread and every kernel launch records a
+// pair of HIP events at the kernel's own start and end
+// (hipExtLaunchKernelGGL), with the kernel's name and the algorithmic bytes
+// the launch site gave for it (probe_next_bytes; 0 when none was given).
+struct ProbeSink {
+    std::vector<hipEvent_t> ev;          // event pool, two per record
+    std::vector<const char*> name;       // per record since the last collection
+    std::vector<double> bytes;
+    size_t used = 0;
+    // collected records (durations read after a synchronisation)
+    std::vector<std::string> done_name;
+    std::vector<double> done_ms, done_bytes;
+    // the next unused event pair; false when none can be created
+    bool next_pair(hipEvent_t& a, hipEvent_t& b);
+    void collect();                      // the stream has been synchronised
+    void release();
 };
 
 struct Solid {
@@ -129,16 +149,12 @@ struct mgf_scene_s {
     bool solids_unbuilt = false;   // solids added since the marker was last built
     bool boundary_user = false;    // SOLID_SDF / SOLID_VEL uploaded directly
     bool timing = false;
-    // kernel probes: event pairs recorded around selected launches
+    // kernel probes: every launch of the scene's entry points while enabled
     bool probes = false;
-    std::vector<hipEvent_t> probe_ev;   // 2 per pair
-    std::vector<int> probe_kind;
-    std::vector<double> probe_bytes;
-    size_t probe_used = 0;              // pairs recorded since the last collection
-    double probe_ms[MGF_PROBE_COUNT] = {};
-    double probe_b[MGF_PROBE_COUNT] = {};
-    int64_t probe_n[MGF_PROBE_COUNT] = {};
-    double probe_max[MGF_PROBE_COUNT] = {};   // longest launch of each kind seen
+    mgf::ProbeSink sink;
+    bool skip_dry = true;          // CG / fused multigrid blocks without fluid return at once
+    bool div_check = false;        // divergence check after each projection
+    double* div_dev = nullptr;     // [0] largest ratio of the call, [1..2] the last sums
 
     int64_t device_bytes = 0;
     std::vector<void*> allocations;
@@ -188,15 +204,10 @@ constexpr int kMaxBlocks = 4096;   // fused reductions use at most this many blo
 // Error helpers.
 void set_error(const std::string& msg);
 
-// Kernel probes: start/stop events armed for the next kernel launch of this
-// host thread.  The launch records them at the kernel's own start and end,
-// so their interval is the kernel's execution alone, without the queue's
-// dispatch gaps; `used` tells that a launch consumed them.
-struct LaunchProbe {
-    hipEvent_t start = nullptr, stop = nullptr;
-    bool used = false;
-};
-LaunchProbe& launch_probe();
+ProbeSink*& probe_sink();
+// Algorithmic bytes of the next kernel launched by this host thread.
+void probe_next_bytes(double bytes);
+double probe_take_bytes();
 
 // Device bytes a scene with this descriptor holds once it has stepped (the
 // preconditioner's levels included, solids excluded); nothing is allocated.
@@ -204,6 +215,21 @@ mgf_status scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t*
 // Makes the allocations a scene's first step would make, so that stepping
 // allocates nothing.
 mgf_status scene_prepare(mgf_scene s);
+// Extrapolation buffers (face count F, K = ext_max_distance, below 65535):
+// dist F, list F, lmask F, fmask F, counts ext_counts_size(K); host_offs is
+// host memory of K + 3 words the layer offsets are read into.
+struct ExtBufs {
+    uint16_t* dist;
+    uint32_t* list;
+    uint8_t* lmask;
+    uint8_t* fmask;
+    uint32_t* counts;
+    uint32_t* host_offs;
+};
+// extrapolatingDataInvoker(FCV) on (u, v, w) in place on stream st; *used
+// receives the largest non-empty layer (< layers).
+mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v, float* w, int layers,
+                           const ExtBufs& B, int* used);
 #define MGF_HIP(expr)                                                           \
     do {                                                                        \
         hipError_t e_ = (expr);                                                 \
@@ -265,6 +291,12 @@ hipError_t launch_project(hipStream_t s, const GridDev& g, const float sg[3],
                           const uint8_t* markers, const float* p32, const double* p64,
                           float* u, float* v, float* w, const float* us, const float* vs,
                           const float* ws);
+// Divergence check of the projected velocity: out[0] = max(out[0], ||D u|| /
+// ||b||) over the fluid cells (FP64; out[1], out[2] receive the two sums);
+// partials holds 2 * blocks doubles.
+hipError_t launch_div_check(hipStream_t s, const GridDev& g, const uint8_t* markers, const float* u,
+                            const float* v, const float* w, const float* b, double* partials, int blocks,
+                            double* out);
 // ---- column-marching CG and multigrid (mgf_kernels_mg.hip) -----------------
 hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* markers, uint16_t* codes);
 // Blocks of the column kernels on grid g (the partial-sum buffer must hold

@@ -56,25 +56,33 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
     __syncthreads();
 }
 
-// Kernel probes (LaunchProbe, mgf_internal.h): the armed events are recorded
-// at the kernel's own start and end by hipExtLaunchKernelGGL.
+// Every launch goes through launch_dims: with a probe sink installed for this
+// host thread (ProbeSink, mgf_internal.h) the kernel records its start/stop
+// events under `name`.
 template <class K, class... A>
-inline hipError_t launch_dims(K kern, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, A... args) {
-    LaunchProbe& p = launch_probe();
-    if (p.start) {
-        hipExtLaunchKernelGGL(kern, grid, block, shmem, s, p.start, p.stop, 0u, args...);
-        p.start = p.stop = nullptr;
-        p.used = true;
-    } else {
-        hipLaunchKernelGGL(kern, grid, block, shmem, s, args...);
+inline hipError_t launch_dims(K kern, const char* name, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s,
+                              A... args) {
+    const double bytes = probe_take_bytes();
+    if (ProbeSink* ps = probe_sink()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (ps->next_pair(a, b)) {
+            hipExtLaunchKernelGGL(kern, grid, block, shmem, s, a, b, 0u, args...);
+            ps->name.push_back(name);
+            ps->bytes.push_back(bytes);
+            ps->used += 1;
+            return hipGetLastError();
+        }
     }
+    hipLaunchKernelGGL(kern, grid, block, shmem, s, args...);
     return hipGetLastError();
 }
 
 template <class K, class... A>
-inline hipError_t launch_k(K kern, uint32_t grid, hipStream_t s, A... args) {
-    return launch_dims(kern, dim3(grid), dim3(kBlock), 0u, s, args...);
+inline hipError_t launch_k(K kern, const char* name, uint32_t grid, hipStream_t s, A... args) {
+    return launch_dims(kern, name, dim3(grid), dim3(kBlock), 0u, s, args...);
 }
-#define LAUNCH(kern, grid, ...) launch_k(kern, uint32_t(grid), s, __VA_ARGS__)
+#define LAUNCH(kern, grid, ...) launch_k(kern, #kern, uint32_t(grid), s, __VA_ARGS__)
+// with an explicit kernel name (templates: one name for every instantiation)
+#define LAUNCHN(kern, name, grid, ...) launch_k(kern, name, uint32_t(grid), s, __VA_ARGS__)
 
 }  // namespace mgf

@@ -670,7 +670,7 @@ __global__ void __launch_bounds__(kBlock) k_sdf_value(Cells C, const float* __re
 
 hipError_t launch_fluid_density(hipStream_t s, const GridDev& g, const float* phi, const float* solid, float* out) {
     const Cells C{uint32_t(g.nx), uint32_t(g.ny), uint32_t(g.nz), uint32_t(g.nx) * g.ny * g.nz};
-    hipLaunchKernelGGL(k_fluid_density, dim3((C.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, phi, solid, out);
+    (void)launch_dims(k_fluid_density, "k_fluid_density", dim3((C.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, phi, solid, out);
     return hipGetLastError();
 }
 
@@ -678,16 +678,16 @@ hipError_t launch_fluid_sdf(hipStream_t s, const GridDev& g, const float* phi, i
                             float* out) {
     const Cells C{uint32_t(g.nx), uint32_t(g.ny), uint32_t(g.nz), uint32_t(g.nx) * g.ny * g.nz};
     const uint32_t G = (C.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_sdf_seed, dim3(G), dim3(kBlock), 0, s, C, phi, scratch);
+    (void)launch_dims(k_sdf_seed, "k_sdf_seed", dim3(G), dim3(kBlock), 0, s, C, phi, scratch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cdt_rows, dim3((C.ny * C.nz + 3) / 4), dim3(kBlock), 0, s, C, scratch);
+    (void)launch_dims(k_cdt_rows, "k_cdt_rows", dim3((C.ny * C.nz + 3) / 4), dim3(kBlock), 0, s, C, scratch);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cdt_lines<1>, dim3((C.nx * C.nz + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, scratch);
+    (void)launch_dims(k_cdt_lines<1>, "k_cdt_lines", dim3((C.nx * C.nz + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, scratch);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cdt_lines<2>, dim3((C.nx * C.ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, scratch);
+    (void)launch_dims(k_cdt_lines<2>, "k_cdt_lines", dim3((C.nx * C.ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s, C, scratch);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sdf_value, dim3(G), dim3(kBlock), 0, s, C, phi, scratch, passes, out);
+    (void)launch_dims(k_sdf_value, "k_sdf_value", dim3(G), dim3(kBlock), 0, s, C, phi, scratch, passes, out);
     return hipGetLastError();
 }
 
@@ -718,16 +718,20 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
     // rows: held in registers when they fit 16 chunks of 64 faces
     const uint32_t exm = F.ex[0];   // the x-faces have the longest rows
     const dim3 rg((F.rows[3] + 3) / 4);
-    if (exm <= 256) hipLaunchKernelGGL(k_ext_rows_reg<4>, rg, dim3(kBlock), 0, s, F, dist);
-    else if (exm <= 512) hipLaunchKernelGGL(k_ext_rows_reg<8>, rg, dim3(kBlock), 0, s, F, dist);
-    else if (exm <= 1024) hipLaunchKernelGGL(k_ext_rows_reg<16>, rg, dim3(kBlock), 0, s, F, dist);
-    else hipLaunchKernelGGL(k_ext_rows, rg, dim3(kBlock), 0, s, F, dist);
+    const double nf = double(F.off[3]);
+    probe_next_bytes(6.0 * nf);   // face value 4 read, distance 2 written
+    if (exm <= 256) (void)launch_dims(k_ext_rows_reg<4>, "k_ext_rows_reg", rg, dim3(kBlock), 0, s, F, dist);
+    else if (exm <= 512) (void)launch_dims(k_ext_rows_reg<8>, "k_ext_rows_reg", rg, dim3(kBlock), 0, s, F, dist);
+    else if (exm <= 1024) (void)launch_dims(k_ext_rows_reg<16>, "k_ext_rows_reg", rg, dim3(kBlock), 0, s, F, dist);
+    else (void)launch_dims(k_ext_rows, "k_ext_rows", rg, dim3(kBlock), 0, s, F, dist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_lines<1>, dim3((F.ylines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
+    probe_next_bytes(4.0 * nf);   // distance 2 read and written (the backward sweep re-reads it from cache)
+    (void)launch_dims(k_ext_lines<1>, "k_ext_lines", dim3((F.ylines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_lines<2>, dim3((F.zlines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
+    probe_next_bytes(4.0 * nf);
+    (void)launch_dims(k_ext_lines<2>, "k_ext_lines", dim3((F.zlines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
     return hipGetLastError();
 }
 
@@ -743,18 +747,20 @@ hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const uint16_t* dist
     *offs = B.offs;
     *maxd = B.maxd;
     const uint32_t nh = kCopies * K2;
-    hipLaunchKernelGGL(k_ext_zero, dim3((nh + kBlock - 1) / kBlock), dim3(kBlock), 0, s, B.hist, nh);
+    (void)launch_dims(k_ext_zero, "k_ext_zero", dim3((nh + kBlock - 1) / kBlock), dim3(kBlock), 0, s, B.hist, nh);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t G = (size_t(total) + size_t(kBlock) * kSortCpt - 1) / (size_t(kBlock) * kSortCpt);
     G = (G + kXcds - 1) / kXcds * kXcds;
-    hipLaunchKernelGGL(k_ext_count, dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, B.hist, K2);
+    probe_next_bytes(2.0 * double(total));
+    (void)launch_dims(k_ext_count, "k_ext_count", dim3(uint32_t(G)), dim3(kBlock), 0, s, dist, total, cap, B.hist, K2);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_scan, dim3(1), dim3(1024), 0, s, B.hist, cap, K2, B.offs, B.poff, B.cursor, B.maxd);
+    (void)launch_dims(k_ext_scan, "k_ext_scan", dim3(1), dim3(1024), 0, s, B.hist, cap, K2, B.offs, B.poff, B.cursor, B.maxd);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_ext_scatter, dim3(uint32_t(G)), dim3(kBlock), 0, s, F, dist, total, cap, K2, B.poff,
+    probe_next_bytes(4.0 * double(total));   // distance 2 read, fmask 1 and (listed faces) list 4 + lmask 1 written
+    (void)launch_dims(k_ext_scatter, "k_ext_scatter", dim3(uint32_t(G)), dim3(kBlock), 0, s, F, dist, total, cap, K2, B.poff,
                        B.cursor, list, lmask, fmask);
     return hipGetLastError();
 }
@@ -763,7 +769,7 @@ hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v,
                             const uint8_t* lmask, uint32_t begin, uint32_t count) {
     if (count == 0) return hipSuccess;
     const Faces F = faces_of(g, u, v, w);
-    return launch_dims(k_ext_layer, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask, begin,
+    return launch_dims(k_ext_layer, "k_ext_layer", dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask, begin,
                        count);
 }
 
@@ -771,7 +777,7 @@ hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v
                              const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1) {
     if (n0 + n1 == 0) return hipSuccess;
     const Faces F = faces_of(g, u, v, w);
-    return launch_dims(k_ext_layer2, dim3((n0 + n1 + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask,
+    return launch_dims(k_ext_layer2, "k_ext_layer2", dim3((n0 + n1 + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask,
                        fmask, begin, n0, n1);
 }
 

@@ -529,6 +529,49 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Divergence check of the projected velocity: per block the FP64 sums of
+// d^2 and b^2 over the fluid cells, d = (Right - Left)/hx + (Up - Down)/hy +
+// (Front - Back)/hz from the stored FP32 faces (CUDADivergence,
+// MPhysics/FieldMathTool.cu:370-395, evaluated in FP64).  Cells on the last
+// layer of an axis are left out: the projection sets their upper face to
+// UNKNOWN (MPhysics/EulerSolverTool.cu:318-321).  Grid-stride loop; the pair
+// of partials of block k goes to part[2k], part[2k+1].
+__global__ void __launch_bounds__(kBlock) k_div_check(GridDev g, const uint8_t* __restrict__ m,
+                                                      const float* __restrict__ u, const float* __restrict__ v,
+                                                      const float* __restrict__ w, const float* __restrict__ b,
+                                                      double* __restrict__ part, uint32_t n) {
+    const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny), nz = uint32_t(g.nz);
+    const double hx = g.hx, hy = g.hy, hz = g.hz;
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        if (m[i] != kFluid) continue;
+        const uint32_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+        if (x + 1 >= nx || y + 1 >= ny || z + 1 >= nz) continue;
+        const uint32_t fu = (z * ny + y) * (nx + 1) + x;
+        const uint32_t fv = (z * (ny + 1) + y) * nx + x;
+        const uint32_t fw = (z * ny + y) * nx + x;
+        const double d = (double(u[fu + 1]) - double(u[fu])) / hx + (double(v[fv + nx]) - double(v[fv])) / hy +
+                         (double(w[fw + nx * ny]) - double(w[fw])) / hz;
+        const double bc = double(b[i]);
+        acc[0] += d * d;
+        acc[1] += bc * bc;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        part[2 * size_t(blockIdx.x)] = acc[0];
+        part[2 * size_t(blockIdx.x) + 1] = acc[1];
+    }
+}
+
+// out[0] = max(out[0], sqrt(sums[0] / sums[1])) (sqrt(sums[0]) when ||b|| = 0);
+// a NaN ratio is kept.
+__global__ void k_div_fold(const double* __restrict__ sums, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const double r = sums[1] > 0.0 ? sqrt(sums[0] / sums[1]) : sqrt(sums[0]);
+    if (!(r <= out[0])) out[0] = r;
+}
+
+// ---------------------------------------------------------------------------
 // Multigrid (preconditioner).  Level l: cells with code_l, right-hand side
 // b_l, iterate x_l; a coarse cell is a 2x2x2 block of fine cells (one layer
 // on an axis whose extent is 1).  Damped Jacobi smoothing with the diagonal
@@ -1563,7 +1606,7 @@ hipError_t launch_codes(hipStream_t s, int nx, int ny, int nz, const uint8_t* ma
 
 static hipError_t finish(hipStream_t s, const double* part, uint32_t count, CgScalars* sc, int kind,
                          double rel_inner = 0.0, double abs_target2 = 0.0) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kFinishBlock), 0, s, part, count, sc, kind, rel_inner, abs_target2);
+    (void)launch_dims(k_finish, "k_finish", dim3(1), dim3(kFinishBlock), 0, s, part, count, sc, kind, rel_inner, abs_target2);
     return hipGetLastError();
 }
 
@@ -1594,8 +1637,20 @@ hipError_t launch_refine_true(hipStream_t s, const GridDev& g, const float scale
         e = LAUNCH(k_refine<2>, G, F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials);
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(1024), 0, s, red.partials, G, out2);
+    (void)launch_dims(k_sum2, "k_sum2", dim3(1), dim3(1024), 0, s, red.partials, G, out2);
     return hipGetLastError();
+}
+
+hipError_t launch_div_check(hipStream_t s, const GridDev& g, const uint8_t* markers, const float* u,
+                            const float* v, const float* w, const float* b, double* partials, int blocks,
+                            double* out) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    probe_next_bytes(15.0 * double(n));   // markers 1, three faces 12 (each shared by two cells), b 4 on fluid cells
+    hipError_t e = LAUNCH(k_div_check, uint32_t(blocks), g, markers, u, v, w, b, partials, n);
+    if (e != hipSuccess) return e;
+    e = launch_dims(k_sum2, "k_sum2", dim3(1), dim3(1024), 0u, s, (const double*)partials, uint32_t(blocks), out + 1);
+    if (e != hipSuccess) return e;
+    return launch_dims(k_div_fold, "k_div_fold", dim3(1), dim3(64), 0u, s, (const double*)(out + 1), out);
 }
 
 double flat_wet_cells(const GridDev& g, const uint8_t* wet_host) {
@@ -1741,7 +1796,7 @@ double mg_march_wet_cells(const MgDims& f, const MgDims& c, const uint8_t* wet_h
 hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet) {
     if (!mg_march_ok(f) || !wet || !code) return hipErrorInvalidValue;
     const March M = make_march(f, c);
-    hipLaunchKernelGGL(k_mg_march_wet, dim3(mg_march_grid(f, c)), dim3(kMThreads), 0, s, M, code, wet);
+    (void)launch_dims(k_mg_march_wet, "k_mg_march_wet", dim3(mg_march_grid(f, c)), dim3(kMThreads), 0, s, M, code, wet);
     return hipGetLastError();
 }
 
@@ -1751,7 +1806,7 @@ hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, co
     if (!mg_march_ok(f)) return hipErrorInvalidValue;
     const March M = make_march(f, c);
     const uint32_t G = mg_march_grid(f, c);
-    return launch_dims(k_mg_march<false, false>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b,
+    return launch_dims(k_mg_march<false, false>, "k_mg_march", dim3(G), dim3(kMThreads), 0u, s, M, f, code, b,
                        static_cast<const float*>(nullptr), static_cast<const float*>(nullptr), x, codec, bc, omega,
                        sc, static_cast<double*>(nullptr), wet);
 }
@@ -1765,11 +1820,11 @@ hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, con
     const CgScalars* csc = sc;
     hipError_t e;
     if (with_rz)
-        e = launch_dims(k_mg_march<true, true>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
+        e = launch_dims(k_mg_march<true, true>, "k_mg_march", dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
                         static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
                         red.partials, wet);
     else
-        e = launch_dims(k_mg_march<true, false>, dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
+        e = launch_dims(k_mg_march<true, false>, "k_mg_march", dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
                         static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
                         static_cast<double*>(nullptr), wet);
     if (e != hipSuccess || !with_rz) return e;
@@ -1801,7 +1856,7 @@ hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uin
         A.lv[l].sx = levels[l].sx; A.lv[l].sy = levels[l].sy; A.lv[l].sz = levels[l].sz;
         A.lv[l].code = codes[l];
     }
-    hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(kCoarseThreads), mg_coarse_lds_bytes(levels, nlev), s, A,
+    (void)launch_dims(k_mg_coarse_cycle, "k_mg_coarse_cycle", dim3(1), dim3(kCoarseThreads), mg_coarse_lds_bytes(levels, nlev), s, A,
                        sc);
     return hipGetLastError();
 }

@@ -815,15 +815,15 @@ uint32_t gen_tiles(size_t ncell) { return uint32_t((ncell + kGenTile - 1) / kGen
 
 hipError_t launch_gen_count(hipStream_t s, size_t ncell, const float* fluid, const float* solid, uint32_t* tiles) {
     const uint32_t nt = gen_tiles(ncell);
-    hipLaunchKernelGGL(k_gen_count, dim3(nt), dim3(kBlock), 0, s, ncell, fluid, solid, tiles);
-    hipLaunchKernelGGL(k_gen_scan, dim3(1), dim3(1024), 0, s, nt, tiles);
+    (void)launch_dims(k_gen_count, "k_gen_count", dim3(nt), dim3(kBlock), 0, s, ncell, fluid, solid, tiles);
+    (void)launch_dims(k_gen_scan, "k_gen_scan", dim3(1), dim3(1024), 0, s, nt, tiles);
     return hipGetLastError();
 }
 
 hipError_t launch_gen_emit(hipStream_t s, const GridDev& g, const float* fluid, const float* solid,
                            const uint32_t* tiles, int per_cell, uint64_t seed, float* pos) {
     const size_t ncell = size_t(g.nx) * g.ny * g.nz;
-    hipLaunchKernelGGL(k_gen_emit, dim3(gen_tiles(ncell)), dim3(kBlock), 0, s, g, ncell, fluid, solid, tiles,
+    (void)launch_dims(k_gen_emit, "k_gen_emit", dim3(gen_tiles(ncell)), dim3(kBlock), 0, s, g, ncell, fluid, solid, tiles,
                        per_cell, seed, pos);
     return hipGetLastError();
 }
@@ -897,9 +897,9 @@ hipError_t launch_bin_particles(hipStream_t s, const GridDev& g, int64_t np, con
     hipError_t e = hipMemsetAsync(counts, 0, nb * sizeof(uint32_t), s);
     if (e == hipSuccess && np > 0) e = LAUNCH(k_bin_count, grid_for(size_t(np)), c, np, pos, counts);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles);
-    hipLaunchKernelGGL(k_gen_scan, dim3(1), dim3(1024), 0, s, nt, tiles);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles, nt, start, cursor);
+    (void)launch_dims(k_scan_tiles, "k_scan_tiles", dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles);
+    (void)launch_dims(k_gen_scan, "k_gen_scan", dim3(1), dim3(1024), 0, s, nt, tiles);
+    (void)launch_dims(k_scan_apply, "k_scan_apply", dim3(nt), dim3(kBlock), 0, s, nb, counts, tiles, nt, start, cursor);
     e = hipGetLastError();
     if (e == hipSuccess && np > 0)
         e = LAUNCH(k_bin_scatter, grid_for(size_t(np)), c, np, pos, cursor, order, pos_s, vel, vel_s);

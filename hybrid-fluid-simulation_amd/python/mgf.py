@@ -19,11 +19,8 @@ RK1, RK2, RK3 = 1, 2, 3
 TOL_ABS_SQRT_NORM, TOL_RELATIVE = 0, 1
 PRECISION_PLAIN, PRECISION_MIXED = 0, 1
 PRECOND_IDENTITY, PRECOND_JACOBI, PRECOND_MULTIGRID = 0, 1, 2
-PROBE_COUNT = 10  # MGF_PROBE_COUNT
 MG_AUTO, MG_PER_SWEEP, MG_FUSED = 0, 1, 2   # MGF_MG_*
 PG = dict(p2gsum=0, g2pnearest=1, linear=2, quadratic=3, cubic=4)   # MGF_PG_*
-PROBE_NAMES = ("k_cg_pap", "k_cg_update", "k_mg_smooth", "k_mg_smooth2_first", "k_mg_smooth_prolong",
-               "k_mg_restrict", "k_ext_layer", "k_advect_fc", "k_mg_descend", "k_mg_ascend")
 FIELD = dict(vel_x=0, vel_y=1, vel_z=2, fluid_domain=3, solid_sdf=4, solid_vel_x=5,
              solid_vel_y=6, solid_vel_z=7, markers=8, pressure=9, rhs=10, matrix_a=11,
              divergence=12, color_x=13, color_y=14, color_z=15, fluid_sdf=16, fluid_density=17)
@@ -49,10 +46,16 @@ class StepStats(C.Structure):
                 ("extrapolation_layers", C.c_int32), ("steps", C.c_int32),
                 ("ms_advect", C.c_double), ("ms_pressure", C.c_double),
                 ("ms_extrapolate", C.c_double), ("ms_total", C.c_double),
-                ("fluid_cells", C.c_int64)]
+                ("fluid_cells", C.c_int64), ("div_residual", C.c_double),
+                ("true_converged", C.c_int32), ("reserved0", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class KernelProbe(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_int64), ("ms", C.c_double),
+                ("busy_launches", C.c_int64), ("busy_ms", C.c_double), ("busy_bytes", C.c_double)]
 
 
 class RequestResult(C.Structure):
@@ -105,8 +108,9 @@ _SIGS = {
     "mgf_scene_sync": [_VP],
     "mgf_scene_set_timing": [_VP, C.c_int32],
     "mgf_scene_set_kernel_probes": [_VP, C.c_int32],
-    "mgf_scene_kernel_probes": [_VP, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double),
-                                C.c_int32],
+    "mgf_scene_kernel_probes": [_VP, C.POINTER(KernelProbe), C.c_int32, C.POINTER(C.c_int32), C.c_int32],
+    "mgf_scene_set_divergence_check": [_VP, C.c_int32],
+    "mgf_scene_set_block_skip": [_VP, C.c_int32],
     "mgf_batch_step": [_VP, C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32, C.POINTER(StepStats),
                        C.POINTER(C.c_int32), C.POINTER(C.c_double)],
     "mgf_scene_bytes_for": [_VP, C.POINTER(SceneDesc), C.POINTER(C.c_int64)],
@@ -404,14 +408,28 @@ class Scene:
         check(lib().mgf_scene_set_kernel_probes(self.h, 1 if on else 0), "mgf_scene_set_kernel_probes")
 
     def kernel_probes(self, reset=True):
-        """Per probed kernel kind: summed ms, launches and algorithmic bytes
-        (HIP events on the scene's stream around the launches of ordinary
-        steps while probes are on)."""
-        ms = (C.c_double * PROBE_COUNT)()
-        n = (C.c_int64 * PROBE_COUNT)()
-        by = (C.c_double * PROBE_COUNT)()
-        check(lib().mgf_scene_kernel_probes(self.h, ms, n, by, 1 if reset else 0), "mgf_scene_kernel_probes")
-        return {name: {"ms": ms[i], "launches": n[i], "bytes": by[i]} for i, name in enumerate(PROBE_NAMES)}
+        """Per kernel launched while probes were on (template instantiations
+        under one name, in order of first appearance): every launch and its
+        summed ms, as a kernel trace counts them, and the busy launches (those
+        that ran at least a quarter of the kernel's longest launch) with their
+        ms and algorithmic bytes."""
+        cap = 256
+        arr = (KernelProbe * cap)()
+        n = C.c_int32()
+        check(lib().mgf_scene_kernel_probes(self.h, arr, cap, C.byref(n), 1 if reset else 0),
+              "mgf_scene_kernel_probes")
+        out = {}
+        for i in range(min(n.value, cap)):
+            e = arr[i]
+            out[e.name.decode()] = {"launches": e.launches, "ms": e.ms, "busy_launches": e.busy_launches,
+                                    "busy_ms": e.busy_ms, "busy_bytes": e.busy_bytes}
+        return out
+
+    def set_divergence_check(self, on):
+        check(lib().mgf_scene_set_divergence_check(self.h, 1 if on else 0), "mgf_scene_set_divergence_check")
+
+    def set_block_skip(self, on):
+        check(lib().mgf_scene_set_block_skip(self.h, 1 if on else 0), "mgf_scene_set_block_skip")
 
     def set_timing(self, on):
         check(lib().mgf_scene_set_timing(self.h, 1 if on else 0), "mgf_scene_set_timing")

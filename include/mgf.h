@@ -138,7 +138,11 @@ typedef struct mgf_scene_desc {
     int64_t max_iterations;    /* <= 0: number of cells (PressureSolver.cpp:82) */
     int32_t precision;         /* MGF_PRECISION_*                             */
     int32_t preconditioner;    /* MGF_PRECOND_*                               */
-    int32_t extrapolation_layers; /* 1000 in the reference (GridFluidSolver.cpp:176) */
+    int32_t extrapolation_layers; /* 1000 in the reference (GridFluidSolver.cpp:176);
+                                  above 1, the extrapolation needs
+                                  nx + ny + nz + 3 < 65535 (16-bit face
+                                  distances) and returns MGF_ERR_UNSUPPORTED
+                                  on longer grids                            */
     int32_t max_refinements;   /* MIXED: outer sweeps cap (<= 0: 8)          */
     double  inner_tolerance;   /* MIXED: relative tolerance of one inner CG
                                   (<= 0: 1e-3)                                 */
@@ -176,6 +180,25 @@ typedef struct mgf_step_stats {
     double  ms_extrapolate;
     double  ms_total;
     int64_t fluid_cells;         /* last step: cells with marker 1            */
+    double  div_residual;        /* with the divergence check on
+                                    (mgf_scene_set_divergence_check): the
+                                    largest over the call's solves of
+                                    ||D u||_2 / ||b||_2 over the fluid cells,
+                                    formed in FP64 from the stored FP32
+                                    velocity right after the projection
+                                    (D = the face divergence of
+                                    FieldMathTool.cu:370-395; b = -D of the
+                                    advected velocity with solid faces at the
+                                    solid velocity, EulerSolverTool.cu:
+                                    128-197); -1 when the check is off      */
+    int32_t true_converged;      /* 1 when every solve's FP64 residual
+                                    ||b - A x||_2 met the tolerance.  MIXED
+                                    stops on it, so this equals converged;
+                                    PLAIN stops on the FP32 recurrence
+                                    residual as the reference does
+                                    (CuMatrixFreePCG.cpp:51-52), and its true
+                                    residual can stay above the tolerance    */
+    int32_t reserved0;
 } mgf_step_stats;
 
 typedef struct mgf_context_s* mgf_context;
@@ -257,31 +280,41 @@ MGF_API mgf_status mgf_step(mgf_scene scene, int32_t nsteps, mgf_step_stats* sta
 MGF_API mgf_status mgf_scene_sync(mgf_scene scene);
 /* Host wall-clock timing of stages in mgf_step (adds synchronisation). */
 MGF_API mgf_status mgf_scene_set_timing(mgf_scene scene, int32_t enabled);
+/* The divergence check (no counterpart in the reference): when enabled, every
+ * pressure solve of the scene also forms ||D u||_2 / ||b||_2 over the fluid
+ * cells from the projected FP32 velocity, in FP64 (cells on the grid's last
+ * layer along an axis are left out: their upper face is UNKNOWN after the
+ * projection, EulerSolverTool.cu:318-321), and the stats report the largest
+ * value of the call in div_residual.  One extra pass over the velocity per
+ * solve, no host synchronisation.  Off by default. */
+MGF_API mgf_status mgf_scene_set_divergence_check(mgf_scene scene, int32_t enabled);
+/* Diagnostic: whether the mixed solver's CG and fused multigrid kernels skip
+ * blocks that hold no fluid cell (1, the default) or run every block (0).
+ * The results are the same bit for bit either way. */
+MGF_API mgf_status mgf_scene_set_block_skip(mgf_scene scene, int32_t enabled);
 
 /* Kernel probes (measurement; no counterpart in the reference).  While
- * enabled, the launches of the kinds below record HIP events at the kernel's
- * own start and end (hipExtLaunchKernelGGL on the scene's stream) during
- * ordinary steps and stage calls.  mgf_scene_kernel_probes returns, per kind,
- * the summed duration in ms, the number of launches and the algorithmic bytes
- * of those launches (each value read or written once; padded-grid cells),
- * accumulated since the last reset.  CG and multigrid launches that returned
- * at once after convergence (under a quarter of the kind's longest launch)
- * are not counted.  Arrays have MGF_PROBE_COUNT entries; any pointer may be
- * null. */
-#define MGF_PROBE_CG_PAP      0  /* p / A p kernel with its scalar finish      */
-#define MGF_PROBE_CG_UPDATE   1  /* x / r update kernel with its scalar finish */
-#define MGF_PROBE_MG_SMOOTH   2  /* level-0 Jacobi sweep (not the last one)    */
-#define MGF_PROBE_MG_SMOOTH2  3  /* level-0 first two sweeps from zero          */
-#define MGF_PROBE_MG_PROLONG  4  /* level-0 sweep with the prolongation        */
-#define MGF_PROBE_MG_RESTRICT 5  /* level-0 restriction                        */
-#define MGF_PROBE_EXT_LAYER   6  /* one extrapolation layer                    */
-#define MGF_PROBE_ADVECT_FC   7  /* face-velocity advection (+ forces)         */
-#define MGF_PROBE_MG_DESCEND  8  /* level-0 fused sweeps + restriction        */
-#define MGF_PROBE_MG_ASCEND   9  /* level-0 fused prolongation + sweeps       */
-#define MGF_PROBE_COUNT       10
+ * enabled, every kernel the scene's entry points launch records HIP events at
+ * the kernel's own start and end (hipExtLaunchKernelGGL on the scene's
+ * stream).  mgf_scene_kernel_probes aggregates the launches since the last
+ * reset per kernel (template instantiations under one name), in order of
+ * first appearance: every launch, as a kernel trace counts them, and the
+ * "busy" launches, those that ran at least a quarter of the kernel's longest
+ * launch (the others are CG and multigrid launches that returned at once once
+ * their solve had converged), with the algorithmic bytes their launch sites
+ * gave (each value read or written once; 0 where a site gives none).
+ * *count receives the number of kernels (at most cap are written). */
+typedef struct mgf_kernel_probe {
+    char    name[48];
+    int64_t launches;
+    double  ms;               /* summed duration of every launch            */
+    int64_t busy_launches;
+    double  busy_ms;
+    double  busy_bytes;
+} mgf_kernel_probe;
 MGF_API mgf_status mgf_scene_set_kernel_probes(mgf_scene scene, int32_t enabled);
-MGF_API mgf_status mgf_scene_kernel_probes(mgf_scene scene, double* ms, int64_t* launches, double* bytes,
-                                           int32_t reset);
+MGF_API mgf_status mgf_scene_kernel_probes(mgf_scene scene, mgf_kernel_probe* out, int32_t cap,
+                                           int32_t* count, int32_t reset);
 
 /* The MIXED solver's preconditioner applied once (diagnostic; the reference's
  * CG has none, CuMatrixFreePCG.h:27-28): z = M^-1 r for the markers of the

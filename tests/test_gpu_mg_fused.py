@@ -147,8 +147,7 @@ def test_auto_fuses_large_levels_only(ctx):
         r = np.random.default_rng(n).standard_normal(int(np.prod(spec.res))).astype(np.float32)
         check_same(a, p, r, "auto %d^3" % n)
         pr = a.kernel_probes(reset=True)
-        assert (pr["k_mg_descend"]["launches"] > 0) == fused, pr["k_mg_descend"]
-        assert (pr["k_mg_ascend"]["launches"] > 0) == fused, pr["k_mg_ascend"]
+        assert ("k_mg_march" in pr and pr["k_mg_march"]["launches"] > 0) == fused, pr.get("k_mg_march")
         a.close(); p.close()
 
 

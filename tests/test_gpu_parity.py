@@ -518,39 +518,98 @@ def compare_trajectory(s, o, step, tol_v, what):
     assert rel_l2(g_, e_) <= tol_v, (what, step, rel_l2(g_, e_))
 
 
+def host_div_rel(s):
+    """||D u|| / ||b|| over the fluid cells (last layer of each axis left out)
+    from the downloaded FP32 velocity, markers and right-hand side, in FP64:
+    the host restatement of the library's divergence check."""
+    nx, ny, nz = [int(v) for v in (s.desc.nx, s.desc.ny, s.desc.nz)]
+    h = [float(v) for v in s.desc.spacing]
+    u = s.download(0).astype(np.float64).reshape(nz, ny, nx + 1)
+    v = s.download(1).astype(np.float64).reshape(nz, ny + 1, nx)
+    w = s.download(2).astype(np.float64).reshape(nz + 1, ny, nx)
+    m = s.download(8).reshape(nz, ny, nx) == 1
+    m[-1, :, :] = False; m[:, -1, :] = False; m[:, :, -1] = False
+    with np.errstate(over="ignore", invalid="ignore"):
+        d = (u[:, :, 1:] - u[:, :, :-1]) / h[0] + (v[:, 1:, :] - v[:, :-1, :]) / h[1] + (w[1:] - w[:-1]) / h[2]
+    b = s.download(10).astype(np.float64).reshape(nz, ny, nx)
+    return float(np.linalg.norm(d[m]) / np.linalg.norm(b[m]))
+
+
+def fp32_rounding_floor(o64, spec):
+    """||D round32(u)|| / ||b|| of the FP64 oracle's projected velocity: the
+    divergence left by storing an (almost) exactly projected velocity in FP32."""
+    nx, ny, nz = spec.res
+    h = spec.spacing
+    with np.errstate(over="ignore", invalid="ignore"):
+        u = o64.download(0, f64=True).astype(np.float32).astype(np.float64).reshape(nz, ny, nx + 1)
+        v = o64.download(1, f64=True).astype(np.float32).astype(np.float64).reshape(nz, ny + 1, nx)
+        w = o64.download(2, f64=True).astype(np.float32).astype(np.float64).reshape(nz + 1, ny, nx)
+        d = (u[:, :, 1:] - u[:, :, :-1]) / h[0] + (v[:, 1:, :] - v[:, :-1, :]) / h[1] + (w[1:] - w[:-1]) / h[2]
+    m = o64.download(8).reshape(nz, ny, nx) == 1
+    m[-1, :, :] = False; m[:, -1, :] = False; m[:, :, -1] = False
+    b = o64.download(10, f64=True).astype(np.float64).reshape(nz, ny, nx)
+    return float(np.linalg.norm(d[m]) / np.linalg.norm(b[m]))
+
+
 def test_dam_break_64_plain_matches_fp32_oracle(ctx, orc):
+    """Config 1 with the plain FP32 CG, all 10 steps against the FP32 oracle.
+    The reference's stop rule is on the FP32 recurrence residual; the FP64
+    residual of the FP32 pressure stagnates above 1e-6 (true_converged 0)."""
     spec = scenes.dam_break(64)
     s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_PLAIN))
+    s.set_divergence_check(True)
     o = scenes.build_oracle_scene(orc, spec)
     p = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-6)
-    for step in range(3):
+    for step in range(10):
         st = s.step(1)
         ost = o.step(p)
         assert st["converged"] == 1
         assert abs(st["cg_iterations"] - ost["cg_iterations"]) <= 2, (step, st, ost)
         compare_trajectory(s, o, step, 1e-5, "plain")
+        g_ = np.concatenate([s.download(f) for f in (0, 1, 2)])
+        e_ = np.concatenate([o.download(f) for f in (0, 1, 2)])
+        # the divergence of the stored velocity is the FP64 residual of the
+        # FP32 pressure (the projection invariant D u = A p - b), to rounding
+        assert st["div_residual"] > 0 and abs(st["div_residual"] - st["residual_rel"]) <= 0.05 * st["residual_rel"] + 2e-6, st
+        assert abs(host_div_rel(s) - st["div_residual"]) <= 1e-6 * st["div_residual"] + 1e-12
+        print("plain step %d: vel drift vs FP32 oracle %.2e, iterations %d/%d, FP64 residual %.2e, "
+              "div %.2e, true_converged %d" % (step, rel_l2(g_, e_), st["cg_iterations"], ost["cg_iterations"],
+                                               st["residual_rel"], st["div_residual"], st["true_converged"]))
     s.close(); o.close()
 
 
 @pytest.mark.parametrize("precond", PRECONDS)
 def test_dam_break_64_mixed(ctx, orc, precond):
-    """Mixed precision: true FP64 residual <= 1e-6; velocity within 1e-4
-    relative L2 of both the FP32 and the FP64 oracle trajectories."""
+    """Config 1, all 10 steps, mixed precision: true FP64 residual <= 1e-6;
+    velocity within 1e-4 relative L2 of both the FP32 and the FP64 oracle
+    trajectories; the divergence of the stored FP32 velocity is the residual
+    plus the FP32 rounding floor of the FP64 oracle's projected velocity."""
     spec = scenes.dam_break(64)
     s = scenes.build_gpu_scene(mgf, ctx, spec, **dam_desc_kw(mgf.PRECISION_MIXED, precond))
+    s.set_divergence_check(True)
     o32 = scenes.build_oracle_scene(orc, spec)
     o64 = scenes.build_oracle_scene(orc, spec, f64=True)
     p = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-6)
     p64 = orc.params(spec.dt, rk=1, tol_mode=1, tol=1e-12)
-    for step in range(3):
+    for step in range(10):
         st = s.step(1)
         o32.step(p)
-        o64.step(p64)
-        assert st["converged"] == 1 and st["residual_rel"] <= 1e-6, st
+        # the FP64 oracle by stages, to read its projected velocity
+        o64.update_boundaries(p64); o64.advect(p64); o64.apply_forces(p64); o64.pressure_solve(p64)
+        floor = fp32_rounding_floor(o64, spec)
+        o64.extrapolate(p64)
+        assert st["converged"] == 1 and st["true_converged"] == 1 and st["residual_rel"] <= 1e-6, st
         compare_trajectory(s, o32, step, 1e-4, "mixed vs fp32")
         g_ = np.concatenate([s.download(f) for f in (0, 1, 2)])
         e_ = np.concatenate([o64.download(f, f64=True) for f in (0, 1, 2)])
-        assert rel_l2(g_, e_) <= 1e-4, (step, rel_l2(g_, e_))
+        e32 = np.concatenate([o32.download(f) for f in (0, 1, 2)])
+        d64 = rel_l2(g_, e_)
+        assert d64 <= 1e-4, (step, d64)
+        div = st["div_residual"]
+        assert abs(host_div_rel(s) - div) <= 1e-6 * div + 1e-12
+        assert 0 < div <= st["residual_rel"] + 2.0 * floor + 1e-7, (step, div, st["residual_rel"], floor)
+        print("mixed step %d: vel drift vs FP32 oracle %.2e, vs FP64 oracle %.2e; FP64 residual %.2e, "
+              "div %.2e (FP32 rounding floor %.2e)" % (step, rel_l2(g_, e32), d64, st["residual_rel"], div, floor))
     s.close(); o32.close(); o64.close()
 
 
