@@ -186,22 +186,51 @@ mgf_status mgf_batch_step(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t
         const mgf_status st = scene_prepare(scenes[i]);
         if (st != MGF_OK) return st;
     }
-    // largest scenes first: the last scenes to start are the short ones
-    std::vector<int32_t> order(static_cast<size_t>(n));
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return scenes[a]->ncell > scenes[b]->ncell; });
-    const int W = std::min<int>(n, workers <= 0 ? kDefaultWorkers : std::min(workers, kMaxWorkers));
     const size_t un = static_cast<size_t>(n);
     std::vector<mgf_status> st(un, MGF_OK);
     std::vector<std::string> err(un);
     std::vector<mgf_step_stats> local(un);
+    // Scenes that can run in lockstep do (their pressure solves as
+    // multi-scene launches); the others run on their own streams from the
+    // worker threads.
+    std::vector<int32_t> lock_idx, solo_idx;
+    for (int32_t i = 0; i < n; ++i)
+        ((ctx->lockstep && lockstep_eligible(scenes[i])) ? lock_idx : solo_idx).push_back(i);
+    if (lock_idx.size() < 2) {
+        solo_idx.insert(solo_idx.end(), lock_idx.begin(), lock_idx.end());
+        lock_idx.clear();
+    }
+    if (!lock_idx.empty()) {
+        std::vector<mgf_scene> ls;
+        for (int32_t i : lock_idx) ls.push_back(scenes[i]);
+        std::vector<mgf_step_stats> ls_stats;
+        std::vector<mgf_status> ls_status;
+        std::vector<std::string> ls_err;
+        std::vector<double> ls_ms(step_ms ? ls.size() * size_t(nsteps) : 0);
+        const mgf_status r = batch_lockstep(ctx, ls, nsteps, workers, ls_stats, ls_status, ls_err,
+                                            step_ms ? ls_ms.data() : nullptr);
+        const std::string batch_err = r != MGF_OK ? std::string(mgf_last_error()) : std::string();
+        for (size_t k = 0; k < lock_idx.size(); ++k) {
+            const size_t i = size_t(lock_idx[k]);
+            local[i] = ls_stats[k];
+            st[i] = ls_status[k] != MGF_OK ? ls_status[k] : r;
+            err[i] = !ls_err[k].empty() ? ls_err[k] : batch_err;
+            if (step_ms)
+                for (int32_t q = 0; q < nsteps; ++q) step_ms[i * size_t(nsteps) + size_t(q)] = ls_ms[k * size_t(nsteps) + size_t(q)];
+        }
+    }
+    // largest scenes first: the last scenes to start are the short ones
+    std::vector<int32_t> order = solo_idx;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return scenes[a]->ncell > scenes[b]->ncell; });
+    const int32_t nsolo = int32_t(order.size());
+    const int W = std::max(1, std::min<int>(nsolo, workers <= 0 ? kDefaultWorkers : std::min(workers, kMaxWorkers)));
     std::atomic<int32_t> next{0};
     auto body = [&] {
         (void)hipSetDevice(ctx->device);
         for (;;) {
             const int32_t k = next.fetch_add(1);
-            if (k >= n) return;
+            if (k >= nsolo) return;
             const int32_t i = order[size_t(k)];
             st[size_t(i)] = run_scene(scenes[i], nsteps, &local[size_t(i)],
                                       step_ms ? step_ms + size_t(i) * size_t(nsteps) : nullptr, &err[size_t(i)]);

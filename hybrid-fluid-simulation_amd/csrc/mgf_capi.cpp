@@ -7,6 +7,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <limits>
+#include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -876,6 +880,550 @@ mgf_status step_particles_once(mgf_scene s, mgf_psolver ps, mgf_step_stats* acc)
     return MGF_OK;
 }
 
+// ===========================================================================
+// Lockstep batched step.  Scenes stepped together run their pressure solves
+// as multi-scene launches (k_multi): one launch per kernel of the mixed CG
+// and of the V-cycle covers every scene, a block finding its scene and its
+// block of that scene's own launch through a map, so a batch of small scenes
+// costs the launches of one scene instead of one set per scene.  Every scene
+// receives the arguments of its own launches (the rec_* records of the
+// launch_* calls cg_mixed and apply_precond make), in its own order, so its
+// results are bitwise those of mgf_step.  The stages before and after the
+// solve (advection, markers, right-hand side, projection, extrapolation) run
+// on the scenes' own streams.
+
+// Lockstep applies to mixed-precision multigrid scenes starting from x = 0.
+bool lock_eligible(mgf_scene s) {
+    const auto& d = s->desc;
+    return d.precision == MGF_PRECISION_MIXED && d.preconditioner == MGF_PRECOND_MULTIGRID && !d.warm_start;
+}
+
+// One record of a scene's sequence, keyed by its place: sequences of
+// different scenes merge by key (same key, same kernel = one launch).
+struct KRec {
+    int a, b, c, d;
+    MRec r;
+};
+
+void keyed(std::vector<KRec>& out, const std::vector<MRec>& recs, int a, int b, int c) {
+    for (size_t k = 0; k < recs.size(); ++k) out.push_back(KRec{a, b, c, int(k), recs[k]});
+}
+
+// apply_precond's launches for scene s as keyed records; *zout = the buffer
+// that ends up holding z.  Keys: (0, level, step) on the way down, (1, 0, 0)
+// the coarse cycle, (2, -level, step) on the way up.
+mgf_status vcycle_records(mgf_scene s, const float sa[3], std::vector<KRec>& out, const float** zout) {
+    const auto& d = s->desc;
+    const MgDims d0{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
+    const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.9f;
+    const int L = 1 + int(s->mg.size());
+    struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint8_t* wet; };
+    Lv lv[17];
+    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->skip_dry ? s->wet0 : nullptr};
+    for (int l = 1; l < L; ++l) {
+        auto& m = s->mg[size_t(l - 1)];
+        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, s->skip_dry ? m.wet : nullptr};
+    }
+    const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
+    auto fused = [&](int l) { return l + 1 < L && march_on(s, l, lv[l].d, lv[l + 1].d); };
+    std::vector<MRec> r;
+    for (int l = 0; l < l0; ++l) {
+        Lv& v = lv[l];
+        const bool coarsest = l == L - 1;
+        if (fused(l)) {
+            r.clear();
+            rec_mg_descend(r, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes, const_cast<float*>(lv[l + 1].b),
+                           omega, s->scal, v.wet);
+            keyed(out, r, 0, l, 0);
+            continue;
+        }
+        const int sweeps = coarsest ? kCoarsestSweeps : nu;
+        auto rz_at = [&](int k) { return (l == 0 && coarsest && k == sweeps - 1) ? 1 : 0; };
+        int k;
+        r.clear();
+        if (sweeps >= 2) {
+            rec_mg_smooth2_first(r, v.d, v.codes, v.b, v.x, omega, s->scal, s->red, rz_at(1));
+            k = 2;
+        } else {
+            rec_mg_smooth(r, v.d, v.codes, v.b, v.b, v.x, omega, s->scal, s->red, 1, rz_at(0));
+            k = 1;
+        }
+        keyed(out, r, 0, l, 0);
+        for (; k < sweeps; ++k) {
+            r.clear();
+            rec_mg_smooth(r, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k));
+            keyed(out, r, 0, l, k);
+            std::swap(v.x, v.t);
+        }
+        if (!coarsest) {
+            r.clear();
+            rec_mg_restrict(r, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes, const_cast<float*>(lv[l + 1].b),
+                            s->scal);
+            keyed(out, r, 0, l, 1000);
+        }
+    }
+    if (l0 < L) {
+        MgDims dims[kCoarseLevelsMax];
+        const uint16_t* codes[kCoarseLevelsMax];
+        for (int l = l0; l < L; ++l) {
+            dims[l - l0] = lv[l].d;
+            codes[l - l0] = lv[l].codes;
+        }
+        r.clear();
+        if (!rec_mg_coarse_cycle(r, dims, codes, L - l0, lv[l0].b, lv[l0].x, nu, kCoarsestSweeps, omega, s->scal))
+            return fail(MGF_ERR_INVALID, "coarse cycle levels");
+        keyed(out, r, 1, 0, 0);
+    }
+    for (int l = std::min(L - 2, l0 - 1); l >= 0; --l) {
+        Lv& v = lv[l];
+        auto rz_at = [&](int k) { return (l == 0 && k == nu - 1) ? 1 : 0; };
+        r.clear();
+        if (fused(l)) {
+            rec_mg_ascend(r, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal, s->red,
+                          l == 0 ? 1 : 0, v.wet);
+            keyed(out, r, 2, -l, 0);
+            std::swap(v.x, v.t);
+            continue;
+        }
+        rec_mg_smooth_prolong(r, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal, s->red,
+                              rz_at(0));
+        keyed(out, r, 2, -l, 0);
+        std::swap(v.x, v.t);
+        for (int k = 1; k < nu; ++k) {
+            r.clear();
+            rec_mg_smooth(r, v.d, v.codes, v.b, v.x, v.t, omega, s->scal, s->red, 0, rz_at(k));
+            keyed(out, r, 2, -l, k);
+            std::swap(v.x, v.t);
+        }
+    }
+    *zout = lv[0].x;
+    return MGF_OK;
+}
+
+// Everything of a step before the CG sweeps, enqueued on the scene's stream:
+// boundaries, advection with forces, markers, right-hand side, stencil codes,
+// the multigrid hierarchy, x64 = 0 and ||b||^2 into red.out2 (cg_mixed's
+// prologue, which the sweeps follow).
+mgf_status lock_pre(mgf_scene s) {
+    TRY(refresh_boundaries(s));
+    TRY(advect(s, true));
+    hipStream_t st = s->stream;
+    const int c = s->cur;
+    float sa[3], sg[3];
+    scales(s, sa, sg);
+    MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
+    MGF_HIP(launch_rhs(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), vel_of(s->us, s->vs, s->ws), s->markers, s->b));
+    MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
+    if (!s->wet_flat) TRY(dalloc(s, &s->wet_flat, col_blocks(s->g)));
+    MGF_HIP(launch_flat_wet(st, s->g, s->codes, s->wet_flat));
+    TRY(ensure_precond(s));
+    TRY(mg_setup(s, sa));
+    s->wet_flat_cells = s->wet0_cells = -1.0;
+    s->x64_valid = false;
+    MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, nullptr, s->x64, s->b, nullptr, 0, s->red, s->red.out2));
+    return MGF_OK;
+}
+
+// The rest of the step after the solve: projection with the FP64 pressure,
+// the divergence check, extrapolation (pressure() and step_once's tail).
+mgf_status lock_post(mgf_scene s, mgf_step_stats* acc) {
+    hipStream_t st = s->stream;
+    const int c = s->cur;
+    float sa[3], sg[3];
+    scales(s, sa, sg);
+    s->x_is_64 = true;
+    MGF_HIP(launch_project(st, s->g, sg, s->markers, nullptr, s->x64, s->u[c], s->v[c], s->w[c], s->us, s->vs,
+                           s->ws));
+    if (s->div_check)
+        MGF_HIP(launch_div_check(st, s->g, s->markers, s->u[c], s->v[c], s->w[c], s->b, s->red.partials, kDivBlocks,
+                                 s->div_dev));
+    if (!std::isfinite(acc->residual_rel)) return fail(MGF_ERR_NONFINITE, "pressure solve produced NaN/Inf");
+    int used = 0;
+    TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
+    acc->extrapolation_layers = used;
+    acc->steps += 1;
+    return MGF_OK;
+}
+
+// A sequence of multi-scene launches merged from the scenes' keyed records.
+struct LSlot {
+    int32_t kind;
+    uint32_t nblocks, shmem;
+    size_t tab, map;   // byte offsets into the plan's device buffer
+};
+using LSeq = std::vector<LSlot>;
+
+constexpr int kActiveRing = 8;
+
+struct LockPlan {
+    std::vector<mgf_scene> sc;
+    hipStream_t stream = nullptr;
+    unsigned char* dev = nullptr;
+    uint8_t* active_dev = nullptr;
+    CgScalars* gsc_dev = nullptr;
+    double* gout_dev = nullptr;
+    unsigned char* host = nullptr;   // pinned: gathered scalars and sums, then kActiveRing active-flag slots
+    uint8_t* active_host = nullptr;
+    size_t active_stride = 0;
+    int ring = 0;                    // active-flag slots used since the last synchronisation
+    CgScalars* gsc_host = nullptr;
+    double* gout_host = nullptr;
+    LSeq start_b, start_r, cg, vcycle, refine1, refine2, gather;
+    std::vector<const float*> zv;
+    std::vector<unsigned char> blob;   // host copy of the tables and maps
+    ~LockPlan() {
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
+    }
+};
+
+// Merges per-scene keyed records into launches and lays out their tables and
+// maps (appended to `blob` at 256-byte boundaries).
+LSeq merge_seq(const std::vector<std::vector<KRec>>& per, std::vector<unsigned char>& blob) {
+    struct Key {
+        int a, b, c, d, kind;
+        bool operator<(const Key& o) const {
+            if (a != o.a) return a < o.a;
+            if (b != o.b) return b < o.b;
+            if (c != o.c) return c < o.c;
+            if (d != o.d) return d < o.d;
+            return kind < o.kind;
+        }
+    };
+    std::vector<std::pair<Key, std::pair<int, const MRec*>>> all;
+    for (size_t i = 0; i < per.size(); ++i)
+        for (const KRec& k : per[i]) all.push_back({Key{k.a, k.b, k.c, k.d, k.r.kind}, {int(i), &k.r}});
+    std::stable_sort(all.begin(), all.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    auto align = [&]() { blob.resize((blob.size() + 255) / 256 * 256); };
+    LSeq seq;
+    for (size_t i = 0; i < all.size();) {
+        size_t j = i;
+        while (j < all.size() && !(all[i].first < all[j].first) && !(all[j].first < all[i].first)) ++j;
+        LSlot sl{all[i].first.kind, 0, 0, 0, 0};
+        align();
+        // one entry per scene of the batch, at the scene's index (the map
+        // names the scene); scenes without this launch leave theirs zero
+        sl.tab = blob.size();
+        const size_t bytes = all[i].second.second->bytes;
+        blob.resize(blob.size() + bytes * per.size(), 0);
+        std::vector<uint8_t> seen(per.size(), 0);
+        for (size_t q = i; q < j; ++q) {
+            const MRec* r = all[q].second.second;
+            const size_t scene = size_t(all[q].second.first);
+            // a scene has at most one record per key (its entry would be overwritten)
+            if (seen[scene]++ || r->bytes != bytes) {
+                seq.clear();
+                return seq;
+            }
+            std::memcpy(blob.data() + sl.tab + scene * bytes, r->args, bytes);
+            sl.shmem = std::max(sl.shmem, r->shmem);
+        }
+        align();
+        sl.map = blob.size();
+        for (size_t q = i; q < j; ++q) {
+            const MRec* r = all[q].second.second;
+            const uint32_t scene = uint32_t(all[q].second.first);
+            for (uint32_t b = 0; b < r->blocks; ++b) {
+                const uint32_t e = (scene << kMapShift) | b;
+                const unsigned char* pe = reinterpret_cast<const unsigned char*>(&e);
+                blob.insert(blob.end(), pe, pe + 4);
+            }
+            sl.nblocks += r->blocks;
+        }
+        seq.push_back(sl);
+        i = j;
+    }
+    return seq;
+}
+
+mgf_status run_seq(LockPlan& P, const LSeq& seq) {
+    for (const LSlot& sl : seq)
+        MGF_HIP(launch_multi(P.stream, sl.kind, P.dev + sl.tab, reinterpret_cast<const uint32_t*>(P.dev + sl.map),
+                             sl.nblocks, sl.shmem, P.active_dev));
+    return MGF_OK;
+}
+
+// Builds the plan for scenes whose hierarchies exist (after lock_pre).
+mgf_status build_plan(LockPlan& P) {
+    const size_t n = P.sc.size();
+    if (n >= (size_t(1) << (32 - kMapShift))) return fail(MGF_ERR_INVALID, "too many scenes for one lockstep batch");
+    std::vector<std::vector<KRec>> start_b(n), start_r(n), cg(n), vc(n), ref1(n), ref2(n), gat(n);
+    P.zv.assign(n, nullptr);
+    // the gather outputs live in the plan's buffer: their offsets are known
+    // before the device buffer exists, so records hold device addresses
+    // computed from a base fixed below
+    const size_t gsc_off = 0, gout_off = (n * sizeof(CgScalars) + 255) / 256 * 256,
+                 act_off = gout_off + (n * 2 * sizeof(double) + 255) / 256 * 256,
+                 act_stride = (n + 255) / 256 * 256, head = act_off + act_stride;
+    // size the tables first with null addresses, then allocate and fill
+    for (int pass = 0; pass < 2; ++pass) {
+        unsigned char* base = pass == 0 ? nullptr : P.dev;
+        CgScalars* gsc = reinterpret_cast<CgScalars*>(base + gsc_off);
+        double* gout = reinterpret_cast<double*>(base + gout_off);
+        for (size_t i = 0; i < n; ++i) {
+            mgf_scene s = P.sc[i];
+            float sa[3], sg[3];
+            scales(s, sa, sg);
+            const double inner = s->desc.inner_tolerance > 0 ? s->desc.inner_tolerance : 1e-3;
+            const uint8_t* wet_flat = s->skip_dry ? s->wet_flat : nullptr;
+            std::vector<MRec> r;
+            start_b[i].clear(); start_r[i].clear(); cg[i].clear(); vc[i].clear();
+            ref1[i].clear(); ref2[i].clear(); gat[i].clear();
+            // the overall target of each sweep's start is set per step
+            // (set_start_targets)
+            rec_cg_start(r, s->g, s->b, s->r, s->x, s->scal, s->red, inner, 0.0);
+            keyed(start_b[i], r, 0, 0, 0);
+            r.clear();
+            rec_cg_start(r, s->g, s->r, s->r, s->x, s->scal, s->red, inner, 0.0);
+            keyed(start_r[i], r, 0, 0, 0);
+            TRY(vcycle_records(s, sa, vc[i], &P.zv[i]));
+            r.clear();
+            rec_col_p_ap(r, s->g, sa, s->codes, P.zv[i], s->p0, s->p1, s->scal, s->red, wet_flat);
+            keyed(cg[i], r, 0, 0, 0);
+            r.clear();
+            rec_col_update(r, s->g, s->p0, s->p1, s->x, s->r, s->scal, s->red, 0, wet_flat);
+            keyed(cg[i], r, 0, 0, 1);
+            r.clear();
+            rec_refine_true(r, s->g, sa, s->codes, s->x, s->x64, s->b, s->r, 1, s->red, s->red.out2);
+            keyed(ref1[i], r, 0, 0, 0);
+            r.clear();
+            rec_refine_true(r, s->g, sa, s->codes, s->x, s->x64, s->b, s->r, 2, s->red, s->red.out2);
+            keyed(ref2[i], r, 0, 0, 0);
+            r.clear();
+            rec_gather(r, s->scal, s->red.out2, gsc + i, gout + 2 * i);
+            keyed(gat[i], r, 0, 0, 0);
+        }
+        std::vector<unsigned char>& blob = P.blob;
+        blob.assign(head, 0);
+        P.start_b = merge_seq(start_b, blob);
+        P.start_r = merge_seq(start_r, blob);
+        P.cg = merge_seq(cg, blob);
+        P.vcycle = merge_seq(vc, blob);
+        P.refine1 = merge_seq(ref1, blob);
+        P.refine2 = merge_seq(ref2, blob);
+        P.gather = merge_seq(gat, blob);
+        for (const LSeq* q : {&P.start_b, &P.start_r, &P.cg, &P.vcycle, &P.refine1, &P.refine2, &P.gather})
+            if (q->empty()) return fail(MGF_ERR_INVALID, "lockstep plan: a scene has two records under one key");
+        if (pass == 0) {
+            if (hipMalloc(reinterpret_cast<void**>(&P.dev), blob.size()) != hipSuccess) {
+                (void)hipGetLastError();
+                P.dev = nullptr;
+                return fail(MGF_ERR_NOMEM, "lockstep plan: hipMalloc failed");
+            }
+            const size_t hb = act_off + kActiveRing * act_stride;
+            if (hipHostMalloc(reinterpret_cast<void**>(&P.host), hb) != hipSuccess) {
+                (void)hipGetLastError();
+                P.host = nullptr;
+                return fail(MGF_ERR_NOMEM, "lockstep plan: hipHostMalloc failed");
+            }
+            continue;
+        }
+        MGF_HIP(hipMemcpyAsync(P.dev, blob.data(), blob.size(), hipMemcpyHostToDevice, P.stream));
+        MGF_HIP(hipStreamSynchronize(P.stream));
+    }
+    P.gsc_dev = reinterpret_cast<CgScalars*>(P.dev + gsc_off);
+    P.gout_dev = reinterpret_cast<double*>(P.dev + gout_off);
+    P.active_dev = P.dev + act_off;
+    P.gsc_host = reinterpret_cast<CgScalars*>(P.host + gsc_off);
+    P.gout_host = reinterpret_cast<double*>(P.host + gout_off);
+    P.active_host = P.host + act_off;
+    P.active_stride = act_stride;
+    P.ring = 0;
+    return MGF_OK;
+}
+
+// Gathers every scene's scalars and refinement sums to the host.
+mgf_status gather(LockPlan& P) {
+    const size_t n = P.sc.size();
+    // the gather runs for every scene, active or not
+    MGF_HIP(hipMemsetAsync(P.active_dev, 1, n, P.stream));
+    TRY(run_seq(P, P.gather));
+    MGF_HIP(hipMemcpyAsync(P.host, P.dev, reinterpret_cast<unsigned char*>(P.active_dev) - P.dev, hipMemcpyDeviceToHost,
+                           P.stream));
+    MGF_HIP(hipStreamSynchronize(P.stream));
+    P.ring = 0;
+    return MGF_OK;
+}
+
+// Copies the active flags in through a pinned slot that no copy still in
+// flight reads (the slots are reused after the next synchronisation).
+mgf_status set_active(LockPlan& P, const std::vector<uint8_t>& act) {
+    if (P.ring >= kActiveRing) {
+        MGF_HIP(hipStreamSynchronize(P.stream));
+        P.ring = 0;
+    }
+    uint8_t* slot = P.active_host + size_t(P.ring++) * P.active_stride;
+    std::memcpy(slot, act.data(), act.size());
+    MGF_HIP(hipMemcpyAsync(P.active_dev, slot, act.size(), hipMemcpyHostToDevice, P.stream));
+    return MGF_OK;
+}
+
+// Sets each scene's overall stop target in the finishing records of the
+// sweep starts (the only argument that changes between steps).
+mgf_status set_start_targets(LockPlan& P, const std::vector<double>& target2) {
+    for (LSeq* q : {&P.start_b, &P.start_r})
+        for (const LSlot& sl : *q) {
+            if (sl.kind != kMkFinish) continue;
+            const size_t bytes = finish_arg_bytes();
+            for (size_t i = 0; i < target2.size(); ++i) finish_set_target(P.blob.data() + sl.tab + i * bytes, target2[i]);
+            MGF_HIP(hipMemcpyAsync(P.dev + sl.tab, P.blob.data() + sl.tab, bytes * target2.size(), hipMemcpyHostToDevice,
+                                   P.stream));
+        }
+    return MGF_OK;
+}
+
+// The sweeps of cg_mixed for every scene at once (after lock_pre on each
+// scene's stream and a wait for them on the plan's stream).
+mgf_status lock_solve(LockPlan& P, std::vector<mgf_step_stats>& acc) {
+    const size_t n = P.sc.size();
+    TRY(gather(P));
+    struct St {
+        double b2 = 0, target2 = 0, rr_true = 0, inner = 1e-3, log_rho = 0, rr0 = 0, sweep_target = 0, rr_now = 0;
+        int64_t it_total = 0, max_it = 0;
+        int sweeps = 0, max_sweeps = 8, done_before = 0, chunk = 16, first_chunk = 4;
+        bool conv = false, stop = false, running = false, nan = false;
+        CgScalars hs{};
+    };
+    std::vector<St> S(n);
+    for (size_t i = 0; i < n; ++i) {
+        mgf_scene s = P.sc[i];
+        const auto& d = s->desc;
+        St& t = S[i];
+        t.inner = d.inner_tolerance > 0 ? d.inner_tolerance : 1e-3;
+        t.max_sweeps = d.max_refinements > 0 ? d.max_refinements : 8;
+        t.max_it = d.max_iterations > 0 ? d.max_iterations : int64_t(s->ncell);
+        t.b2 = P.gout_host[2 * i + 1];
+        if (d.tol_mode == MGF_TOL_ABS_SQRT_NORM) {
+            const double q = d.tolerance * d.tolerance;
+            t.target2 = q * q;
+        } else {
+            t.target2 = d.tolerance * d.tolerance * t.b2;
+        }
+        t.rr_true = t.b2;
+        t.conv = t.rr_true <= t.target2 || t.b2 == 0.0;
+        s->x_is_64 = true;
+    }
+    {
+        std::vector<double> tg(n);
+        for (size_t i = 0; i < n; ++i) tg[i] = S[i].target2;
+        TRY(set_start_targets(P, tg));
+    }
+    auto predict = [](const St& t) {
+        if (!(t.log_rho < 0.0) || !(t.rr_now > t.sweep_target) || !(t.sweep_target > 0.0)) return t.first_chunk;
+        const double k = std::floor(std::log(t.sweep_target / t.rr_now) / t.log_rho + 0.5);
+        return int(std::max(1.0, std::min(double(t.chunk), k)));
+    };
+    std::vector<uint8_t> act(n, 0);
+    for (;;) {
+        bool any = false;
+        for (size_t i = 0; i < n; ++i) {
+            St& t = S[i];
+            act[i] = (!t.conv && !t.stop && t.sweeps < t.max_sweeps && t.it_total < t.max_it) ? 1 : 0;
+            any = any || act[i];
+        }
+        if (!any) break;
+        TRY(set_active(P, act));
+        // each scene's first sweep starts from b, later ones from the refined residual
+        std::vector<uint8_t> from_b(n, 0), from_r(n, 0);
+        for (size_t i = 0; i < n; ++i) {
+            from_b[i] = act[i] && S[i].sweeps == 0;
+            from_r[i] = act[i] && S[i].sweeps > 0;
+        }
+        if (std::count(from_b.begin(), from_b.end(), 1)) {
+            TRY(set_active(P, from_b));
+            TRY(run_seq(P, P.start_b));
+        }
+        if (std::count(from_r.begin(), from_r.end(), 1)) {
+            TRY(set_active(P, from_r));
+            TRY(run_seq(P, P.start_r));
+        }
+        TRY(set_active(P, act));
+        TRY(run_seq(P, P.vcycle));
+        for (size_t i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            St& t = S[i];
+            t.rr0 = t.rr_true;
+            t.sweep_target = std::max(t.inner * t.inner * t.rr0, t.target2);
+            t.rr_now = t.rr0;
+            t.done_before = 0;
+            t.running = true;
+        }
+        for (;;) {
+            int k = 0;
+            int64_t left = std::numeric_limits<int64_t>::max();
+            for (size_t i = 0; i < n; ++i) {
+                if (!act[i] || !S[i].running) continue;
+                k = std::max(k, predict(S[i]));
+                left = std::min(left, std::max<int64_t>(S[i].max_it - S[i].it_total - S[i].done_before, 1));
+            }
+            if (k == 0) break;
+            k = int(std::min<int64_t>(k, left));
+            for (int q = 0; q < k; ++q) {
+                TRY(run_seq(P, P.cg));
+                TRY(run_seq(P, P.vcycle));
+            }
+            TRY(gather(P));
+            TRY(set_active(P, act));
+            for (size_t i = 0; i < n; ++i) {
+                if (!act[i] || !S[i].running) continue;
+                St& t = S[i];
+                t.hs = P.gsc_host[i];
+                if (t.hs.iters > 0 && t.hs.rr > 0.0 && t.rr0 > 0.0 && t.hs.rr < t.rr0)
+                    t.log_rho = std::log(t.hs.rr / t.rr0) / double(t.hs.iters);
+                t.rr_now = t.hs.rr;
+                t.done_before = t.hs.iters;
+                if (t.hs.done || t.it_total + t.hs.iters >= t.max_it) t.running = false;
+            }
+        }
+        // the refinement of every scene that swept
+        std::vector<uint8_t> r1(n, 0), r2(n, 0);
+        for (size_t i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            St& t = S[i];
+            t.it_total += t.hs.iters;
+            if (t.hs.done == 2) {
+                t.rr_true = std::nan("");
+                t.nan = true;
+                t.stop = true;
+                continue;
+            }
+            (t.sweeps > 0 ? r2 : r1)[i] = 1;
+        }
+        if (std::count(r1.begin(), r1.end(), 1)) {
+            TRY(set_active(P, r1));
+            TRY(run_seq(P, P.refine1));
+        }
+        if (std::count(r2.begin(), r2.end(), 1)) {
+            TRY(set_active(P, r2));
+            TRY(run_seq(P, P.refine2));
+        }
+        TRY(gather(P));
+        for (size_t i = 0; i < n; ++i) {
+            if (!(r1[i] || r2[i])) continue;
+            St& t = S[i];
+            t.rr_true = P.gout_host[2 * i];
+            ++t.sweeps;
+            if (t.rr_true <= t.target2) t.conv = true;
+            if (!std::isfinite(t.rr_true)) t.stop = true;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        mgf_scene s = P.sc[i];
+        St& t = S[i];
+        s->x64_valid = t.conv && std::isfinite(t.rr_true);
+        mgf_step_stats& a = acc[i];
+        a.cg_iterations += t.it_total;
+        a.refinement_sweeps += t.sweeps;
+        a.converged = a.converged && t.conv;
+        a.true_converged = a.true_converged && t.conv;
+        a.residual_rel = t.b2 > 0 ? std::sqrt(t.rr_true / t.b2) : std::sqrt(t.rr_true);
+        a.residual_recurrence = std::sqrt(t.hs.rr);
+        a.b_norm = std::sqrt(t.b2);
+    }
+    return MGF_OK;
+}
+
 void stats_reset(mgf_step_stats* st) {
     std::memset(st, 0, sizeof(*st));
     st->converged = 1;
@@ -958,6 +1506,122 @@ mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v,
     return MGF_OK;
 }
 
+bool lockstep_eligible(mgf_scene s) { return lock_eligible(s); }
+
+// Runs fn(i) for i in [0, n) on up to W host threads (the caller included).
+template <class F>
+void parallel_for(int n, int W, int device, F fn) {
+    std::atomic<int> next{0};
+    auto body = [&] {
+        (void)hipSetDevice(device);
+        for (int i; (i = next.fetch_add(1)) < n;) fn(i);
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (int w = 1; w < std::min(W, n); ++w) pool.emplace_back(body);
+    } catch (...) {
+    }
+    body();
+    for (auto& t : pool) t.join();
+}
+
+mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int32_t nsteps, int32_t workers,
+                          std::vector<mgf_step_stats>& stats, std::vector<mgf_status>& status,
+                          std::vector<std::string>& errs, double* step_ms) {
+    const int n = int(sc.size());
+    stats.assign(size_t(n), mgf_step_stats{});
+    status.assign(size_t(n), MGF_OK);
+    errs.assign(size_t(n), std::string());
+    if (n == 0) return MGF_OK;
+    MGF_HIP(hipSetDevice(ctx->device));
+    LockPlan P;
+    P.sc = sc;
+    if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    std::vector<hipEvent_t> ev(size_t(n) + 1, nullptr);
+    mgf_status result = MGF_OK;
+    auto cleanup = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamSynchronize(P.stream);
+        (void)hipStreamDestroy(P.stream);
+        P.stream = nullptr;
+    };
+    for (auto& e : ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            cleanup();
+            return fail(MGF_ERR_DEVICE, "hipEventCreate failed");
+        }
+    const int W = workers <= 0 ? 8 : std::min(workers, 64);
+    auto note = [&](int i, mgf_status st) {
+        if (st != MGF_OK && status[size_t(i)] == MGF_OK) {
+            status[size_t(i)] = st;
+            errs[size_t(i)] = mgf_last_error();
+        }
+    };
+    for (int i = 0; i < n; ++i) {
+        stats_reset(&stats[size_t(i)]);
+        note(i, div_begin(sc[size_t(i)]));
+    }
+    for (int32_t step = 0; step < nsteps && result == MGF_OK; ++step) {
+        const double t0 = now_ms();
+        // before the solve, on the scenes' streams
+        parallel_for(n, W, ctx->device, [&](int i) {
+            mgf_scene s = sc[size_t(i)];
+            mgf_status st = lock_pre(s);
+            if (st == MGF_OK && hipEventRecord(ev[size_t(i)], s->stream) != hipSuccess) {
+                (void)hipGetLastError();
+                st = fail(MGF_ERR_DEVICE, "hipEventRecord failed");
+            }
+            note(i, st);
+        });
+        for (int i = 0; i < n; ++i)
+            if (status[size_t(i)] != MGF_OK) result = status[size_t(i)];
+        if (result != MGF_OK) break;
+        if (step == 0) {
+            // the hierarchies exist now: the plan's records hold their buffers
+            for (int i = 0; i < n; ++i) MGF_HIP(hipStreamSynchronize(sc[size_t(i)]->stream));
+            result = build_plan(P);
+            if (result != MGF_OK) break;
+        }
+        for (int i = 0; i < n; ++i) MGF_HIP(hipStreamWaitEvent(P.stream, ev[size_t(i)], 0));
+        result = lock_solve(P, stats);
+        if (result != MGF_OK) break;
+        MGF_HIP(hipEventRecord(ev[size_t(n)], P.stream));
+        // after the solve, on the scenes' streams
+        parallel_for(n, W, ctx->device, [&](int i) {
+            mgf_scene s = sc[size_t(i)];
+            mgf_status st = MGF_OK;
+            if (hipStreamWaitEvent(s->stream, ev[size_t(n)], 0) != hipSuccess) {
+                (void)hipGetLastError();
+                st = fail(MGF_ERR_DEVICE, "hipStreamWaitEvent failed");
+            }
+            if (st == MGF_OK) st = lock_post(s, &stats[size_t(i)]);
+            if (st == MGF_OK) st = sync(s);
+            note(i, st);
+        });
+        for (int i = 0; i < n; ++i)
+            if (status[size_t(i)] != MGF_OK) result = status[size_t(i)];
+        const double dt_ms = now_ms() - t0;
+        if (step_ms)
+            for (int i = 0; i < n; ++i) step_ms[size_t(i) * size_t(nsteps) + size_t(step)] = dt_ms;
+    }
+    for (int i = 0; i < n; ++i) {
+        mgf_scene s = sc[size_t(i)];
+        (void)hipStreamSynchronize(s->stream);
+        note(i, div_end(s, &stats[size_t(i)]));
+        if (status[size_t(i)] == MGF_OK && !stats[size_t(i)].converged) {
+            status[size_t(i)] = MGF_ERR_NOT_CONVERGED;
+            errs[size_t(i)] = "a pressure solve stopped before its tolerance";
+        }
+    }
+    cleanup();
+    return result;
+}
+
 mgf_status scene_prepare(mgf_scene s) {
     if (s->desc.precision == MGF_PRECISION_MIXED && s->desc.preconditioner != MGF_PRECOND_IDENTITY) {
         MGF_HIP(hipSetDevice(s->ctx->device));
@@ -1009,6 +1673,12 @@ mgf_status mgf_init(int32_t device, mgf_context* out) {
         c->lds_per_block = prop.sharedMemPerBlock;
     }
     *out = c;
+    return MGF_OK;
+}
+
+mgf_status mgf_batch_set_lockstep(mgf_context ctx, int32_t enabled) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    ctx->lockstep = enabled != 0;
     return MGF_OK;
 }
 

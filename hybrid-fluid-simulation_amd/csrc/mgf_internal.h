@@ -84,6 +84,7 @@ struct mgf_context_s {
     hipStream_t stream = nullptr;
     int cu_count = 0;
     size_t lds_per_block = 65536;
+    bool lockstep = true;   // mgf_batch_step runs eligible scenes in lockstep
 };
 
 struct mgf_scene_s {
@@ -215,6 +216,17 @@ mgf_status scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t*
 // Makes the allocations a scene's first step would make, so that stepping
 // allocates nothing.
 mgf_status scene_prepare(mgf_scene s);
+// Whether a scene can be stepped in lockstep with others (mixed precision,
+// multigrid preconditioner, cold start).
+bool lockstep_eligible(mgf_scene s);
+// nsteps steps of every scene in sc (all eligible, distinct, of ctx): the
+// pressure solves as multi-scene launches on one stream, the other stages on
+// the scenes' streams from up to `workers` host threads.  stats, status and
+// errs receive one entry per scene; step_ms (n * nsteps, scene-major, may be
+// null) the wall time of each batch step.
+mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int32_t nsteps, int32_t workers,
+                          std::vector<mgf_step_stats>& stats, std::vector<mgf_status>& status,
+                          std::vector<std::string>& errs, double* step_ms);
 // Extrapolation buffers (face count F, K = ext_max_distance, below 65535):
 // dist F, list F, lmask F, fmask F, counts ext_counts_size(K); host_offs is
 // host memory of K + 3 words the layer offsets are read into.
@@ -362,6 +374,62 @@ hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uin
 hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* codef,
                               const float* bf, const float* xf, const uint16_t* codec, float* bc,
                               const CgScalars* sc);
+// ---- multi-scene launches (mgf_kernels_mg.hip) -----------------------------
+// One scene's part of a multi-scene launch (k_multi): the kernel, its
+// argument struct as bytes, and the block count and dynamic LDS of the
+// scene's own launch.  The rec_* functions append the records of the launches
+// the launch_* function of the same name makes, in its order.
+enum MultiKind : int32_t {
+    kMkStart, kMkFinish, kMkPap, kMkUpdate, kMkRefine1, kMkRefine2, kMkAcc64, kMkSum2,
+    kMkSmoothTT, kMkSmoothTF, kMkSmoothFT, kMkSmoothFF, kMkSmooth2T, kMkSmooth2F, kMkProlongT, kMkProlongF,
+    kMkRestrict, kMkRestrict2, kMkRestrictTile, kMkCoarse, kMkMarchDesc, kMkMarchAsc, kMkMarchAscRz, kMkGather,
+    kMkCount
+};
+constexpr size_t kMRecBytes = 512;
+// A multi-scene block map entry: (scene << kMapShift) | block of the scene's launch.
+constexpr uint32_t kMapShift = 20;
+constexpr uint32_t kMapLocal = (1u << kMapShift) - 1u;
+struct MRec {
+    int32_t kind = -1;
+    uint32_t blocks = 0, shmem = 0, bytes = 0;
+    alignas(16) unsigned char args[kMRecBytes];
+};
+void rec_cg_start(std::vector<MRec>& out, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,
+                  ReduceBuf red, double rel_inner, double abs_target2);
+void rec_col_p_ap(std::vector<MRec>& out, const GridDev& g, const float scale[3], const uint16_t* codes,
+                  const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red, const uint8_t* wet);
+void rec_col_update(std::vector<MRec>& out, const GridDev& g, const float* p, const float* ap, float* x, float* r,
+                    CgScalars* sc, ReduceBuf red, int identity, const uint8_t* wet);
+void rec_refine_true(std::vector<MRec>& out, const GridDev& g, const float scale[3], const uint16_t* codes,
+                     const float* xcorr, double* x64, const float* b, float* r, int mode, ReduceBuf red,
+                     double* out2);
+void rec_mg_smooth(std::vector<MRec>& out, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
+                   float* xo, float omega, CgScalars* sc, ReduceBuf red, int first, int with_rz);
+void rec_mg_smooth2_first(std::vector<MRec>& out, const MgDims& l, const uint16_t* code, const float* b, float* xo,
+                          float omega, CgScalars* sc, ReduceBuf red, int with_rz);
+void rec_mg_smooth_prolong(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code,
+                           const float* b, const float* xi, const float* xc, float* xo, float omega, CgScalars* sc,
+                           ReduceBuf red, int with_rz);
+void rec_mg_restrict(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* codef, const float* bf,
+                     const float* xf, const uint16_t* codec, float* bc, const CgScalars* sc);
+bool rec_mg_coarse_cycle(std::vector<MRec>& out, const MgDims* levels, const uint16_t* const* codes, int nlev,
+                         const float* b0, float* x0, int nu, int last_sweeps, float omega, const CgScalars* sc);
+void rec_mg_descend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                    float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
+                    const uint8_t* wet);
+void rec_mg_ascend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                   const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
+                   int with_rz, const uint8_t* wet);
+// The finishing record's size, and its abs_target2 set in place.
+size_t finish_arg_bytes();
+void finish_set_target(unsigned char* args, double abs_target2);
+void rec_gather(std::vector<MRec>& out, const CgScalars* sc, const double* out2, CgScalars* dst_sc, double* dst2);
+// One multi-scene launch: tab = the participating scenes' argument structs
+// (one kind), map = per block (scene << 20) | block of the scene's launch;
+// scenes whose active byte is 0 return at once (active may be null).
+hipError_t launch_multi(hipStream_t s, int32_t kind, const void* tab, const uint32_t* map, uint32_t nblocks,
+                        uint32_t shmem, const uint8_t* active);
+
 // ---- extrapolation (mgf_kernels_ext.hip) ------------------------------------
 // Largest L1 distance between two faces of one component grid (+ margin).
 int32_t ext_max_distance(const GridDev& g);

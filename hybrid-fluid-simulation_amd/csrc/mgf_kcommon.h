@@ -18,6 +18,12 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return (b % kXcds) * (G / kXcds) + b / kXcds;
 }
 
+// The same remapping for block b of a launch of G blocks (multi-scene
+// launches apply it to a block's index within its scene's own launch).
+__device__ __forceinline__ uint32_t xcd_of(uint32_t b, uint32_t G) {
+    return (b % kXcds) * (G / kXcds) + b / kXcds;
+}
+
 inline uint32_t grid_for(size_t n) {
     size_t g = (n + kBlock - 1) / kBlock;
     g = (g + kXcds - 1) / kXcds * kXcds;
@@ -75,6 +81,25 @@ inline hipError_t launch_dims(K kern, const char* name, dim3 grid, dim3 block, u
     }
     hipLaunchKernelGGL(kern, grid, block, shmem, s, args...);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Multi-scene launches.  One launch covers the blocks of many scenes' own
+// launches of one kernel: block b reads map[b] = (scene << kMapShift) | its
+// index within that scene's launch, skips scenes whose active byte is 0, and
+// runs the kernel body with that scene's arguments from a device table
+// (A::run).  The body sees the block index, and applies the XCD remapping,
+// of the scene's own launch, so each scene computes bitwise what it computes
+// alone, partial sums included.
+
+template <class A>
+__global__ void __launch_bounds__(A::kThreads) k_multi(const A* __restrict__ tab, const uint32_t* __restrict__ map,
+                                                       const uint8_t* __restrict__ active) {
+    const uint32_t e = map[blockIdx.x];
+    const uint32_t s = e >> kMapShift;
+    if (active && !active[s]) return;
+    const A a = tab[s];
+    a.run(e & kMapLocal);
 }
 
 template <class K, class... A>

@@ -20,6 +20,8 @@
 // scalars.  The kernel boundary makes the partials visible: no device-scope
 // fence (which writes back and invalidates the XCD's L2) is executed per block.
 #include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "mgf_internal.h"
 #include "mgf_kcommon.h"
@@ -138,11 +140,11 @@ __device__ __forceinline__ void nbrs(uint32_t c, const Flat& F, uint32_t (&j)[6]
     j[5] = c + F.sz <= last ? c + F.sz : last;
 }
 
-// Block sum of one double, stored to part[blockIdx.x] by thread 0.
-__device__ __forceinline__ void block_partial(double v, double* part) {
+// Block sum of one double, stored to part[slot] by thread 0.
+__device__ __forceinline__ void block_partial(double v, double* part, uint32_t slot) {
     double a[1] = {v};
     block_sum<1>(a);
-    if (threadIdx.x == 0) part[blockIdx.x] = a[0];
+    if (threadIdx.x == 0) part[slot] = a[0];
 }
 
 // a / d for a < 2^31, d >= 1 via the FP64 reciprocal rd = 1/d: the estimate
@@ -232,26 +234,50 @@ __global__ void __launch_bounds__(kBlock) k_flat_wet(Flat F, const uint16_t* __r
     if (threadIdx.x == 0) wet[blk] = uint8_t(any != 0);
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_pap(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
-                                                   float sz, const float* __restrict__ zv, float* __restrict__ p,
-                                                   float* __restrict__ ap, const CgScalars* __restrict__ sc,
-                                                   double* __restrict__ part, const uint8_t* __restrict__ wet) {
+// A kernel body runs block `blk` of its launch (after the XCD remapping) and
+// writes its partial sums to slot `slot` (the launch's own block index).
+__device__ __forceinline__ void cg_pap_run(Flat F, const uint16_t* __restrict__ code, float sx, float sy, float sz,
+                                           const float* __restrict__ zv, float* __restrict__ p,
+                                           float* __restrict__ ap, const CgScalars* __restrict__ sc,
+                                           double* __restrict__ part, const uint8_t* __restrict__ wet, uint32_t blk,
+                                           uint32_t slot) {
     __shared__ float aw[64];
     if (sc->done) return;
     const bool first = sc->iters == 0;
-    if (!first && wet && !wet[xcd_block()]) {
-        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+    if (!first && wet && !wet[blk]) {
+        if (threadIdx.x == 0) part[slot] = 0.0;
         return;
     }
     build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
     const float beta = float(sc->beta);
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t b0 = blk * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
     const double acc = (b0 + kBlock * kCpt <= F.n)
                            ? cg_pap_body<false>(F, base, code, sx, sy, sz, zv, p, ap, beta, first, aw)
                            : cg_pap_body<true>(F, base, code, sx, sy, sz, zv, p, ap, beta, first, aw);
-    block_partial(acc, part);
+    block_partial(acc, part, slot);
 }
+
+__global__ void __launch_bounds__(kBlock) k_cg_pap(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
+                                                   float sz, const float* __restrict__ zv, float* __restrict__ p,
+                                                   float* __restrict__ ap, const CgScalars* __restrict__ sc,
+                                                   double* __restrict__ part, const uint8_t* __restrict__ wet) {
+    cg_pap_run(F, code, sx, sy, sz, zv, p, ap, sc, part, wet, xcd_block(), blockIdx.x);
+}
+
+struct MPap {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    const uint16_t* code;
+    float sx, sy, sz;
+    const float* zv;
+    float *p, *ap;
+    const CgScalars* sc;
+    double* part;
+    const uint8_t* wet;
+    uint32_t G;
+    __device__ void run(uint32_t b) const { cg_pap_run(F, code, sx, sy, sz, zv, p, ap, sc, part, wet, xcd_of(b, G), b); }
+};
 
 // K_II (pointwise): x += alpha p; r -= alpha Ap (each formed in FP64, stored
 // FP32); block partials of r . r.
@@ -283,22 +309,41 @@ __device__ __forceinline__ double cg_update_body(const Flat& F, uint32_t base, c
     return acc;
 }
 
+__device__ __forceinline__ void cg_update_run(Flat F, const float* __restrict__ p, const float* __restrict__ ap,
+                                              float* __restrict__ x, float* __restrict__ r,
+                                              const CgScalars* __restrict__ sc, double* __restrict__ part,
+                                              const uint8_t* __restrict__ wet, uint32_t blk, uint32_t slot) {
+    if (sc->done) return;
+    if (wet && !wet[blk]) {
+        if (threadIdx.x == 0) part[slot] = 0.0;
+        return;
+    }
+    const double alpha = sc->alpha;
+    const uint32_t b0 = blk * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n) ? cg_update_body<false>(F, base, p, ap, x, r, alpha)
+                                                   : cg_update_body<true>(F, base, p, ap, x, r, alpha);
+    block_partial(acc, part, slot);
+}
+
 __global__ void __launch_bounds__(kBlock) k_cg_update(Flat F, const float* __restrict__ p,
                                                       const float* __restrict__ ap, float* __restrict__ x,
                                                       float* __restrict__ r, const CgScalars* __restrict__ sc,
                                                       double* __restrict__ part, const uint8_t* __restrict__ wet) {
-    if (sc->done) return;
-    if (wet && !wet[xcd_block()]) {
-        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
-        return;
-    }
-    const double alpha = sc->alpha;
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
-    const uint32_t base = b0 + threadIdx.x;
-    const double acc = (b0 + kBlock * kCpt <= F.n) ? cg_update_body<false>(F, base, p, ap, x, r, alpha)
-                                                   : cg_update_body<true>(F, base, p, ap, x, r, alpha);
-    block_partial(acc, part);
+    cg_update_run(F, p, ap, x, r, sc, part, wet, xcd_block(), blockIdx.x);
 }
+
+struct MUpdate {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    const float *p, *ap;
+    float *x, *r;
+    const CgScalars* sc;
+    double* part;
+    const uint8_t* wet;
+    uint32_t G;
+    __device__ void run(uint32_t b) const { cg_update_run(F, p, ap, x, r, sc, part, wet, xcd_of(b, G), b); }
+};
 
 // Refinement (outer sweeps of the mixed solver).
 // MODE 0: x64 = 0; partials of ||b||^2.
@@ -378,13 +423,13 @@ __device__ __forceinline__ void refine_body(const Flat& F, uint32_t base, const 
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_refine(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
-                                                   float sz, const float* __restrict__ xc, double* __restrict__ x64,
-                                                   const float* __restrict__ b, float* __restrict__ r,
-                                                   double* __restrict__ part) {
+__device__ __forceinline__ void refine_run(Flat F, const uint16_t* __restrict__ code, float sx, float sy, float sz,
+                                           const float* __restrict__ xc, double* __restrict__ x64,
+                                           const float* __restrict__ b, float* __restrict__ r,
+                                           double* __restrict__ part, uint32_t blk, uint32_t slot) {
     __shared__ float aw[64];
     build_tables<false>(aw, nullptr, sx, sy, sz, 0.0f);
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t b0 = blk * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (b0 + kBlock * kCpt <= F.n)
@@ -393,16 +438,55 @@ __global__ void __launch_bounds__(kBlock) k_refine(Flat F, const uint16_t* __res
         refine_body<MODE, true>(F, base, code, sx, sy, sz, xc, x64, b, r, aw, v[0], v[1]);
     block_sum<2>(v);
     if (threadIdx.x == 0) {
-        part[2 * size_t(blockIdx.x)] = v[0];
-        part[2 * size_t(blockIdx.x) + 1] = v[1];
+        part[2 * size_t(slot)] = v[0];
+        part[2 * size_t(slot) + 1] = v[1];
     }
 }
 
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_refine(Flat F, const uint16_t* __restrict__ code, float sx, float sy,
+                                                   float sz, const float* __restrict__ xc, double* __restrict__ x64,
+                                                   const float* __restrict__ b, float* __restrict__ r,
+                                                   double* __restrict__ part) {
+    refine_run<MODE>(F, code, sx, sy, sz, xc, x64, b, r, part, xcd_block(), blockIdx.x);
+}
+
+template <int MODE>
+struct MRefine {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    const uint16_t* code;
+    float sx, sy, sz;
+    const float* xc;
+    double* x64;
+    const float* b;
+    float* r;
+    double* part;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        refine_run<MODE>(F, code, sx, sy, sz, xc, x64, b, r, part, xcd_of(blk, G), blk);
+    }
+};
+
 // x64 += xc, pointwise.
-__global__ void __launch_bounds__(kBlock) k_acc64(uint32_t n, const float* __restrict__ xc, double* __restrict__ x64) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+__device__ __forceinline__ void acc64_run(uint32_t n, const float* __restrict__ xc, double* __restrict__ x64,
+                                          uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i < n) x64[i] = x64[i] + double(xc[i]);
 }
+
+__global__ void __launch_bounds__(kBlock) k_acc64(uint32_t n, const float* __restrict__ xc, double* __restrict__ x64) {
+    acc64_run(n, xc, x64, xcd_block());
+}
+
+struct MAcc64 {
+    static constexpr int kThreads = kBlock;
+    uint32_t n;
+    const float* xc;
+    double* x64;
+    uint32_t G;
+    __device__ void run(uint32_t b) const { acc64_run(n, xc, x64, xcd_of(b, G)); }
+};
 
 // Start of an inner sweep: r = rsrc (when distinct), x = 0; partials of r.r.
 template <bool CHECK>
@@ -424,18 +508,32 @@ __device__ __forceinline__ double start_body(const Flat& F, uint32_t base, const
     return acc;
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_start(Flat F, const float* __restrict__ rsrc, float* r,
-                                                     float* __restrict__ x, double* __restrict__ part) {
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+__device__ __forceinline__ void cg_start_run(Flat F, const float* __restrict__ rsrc, float* r, float* __restrict__ x,
+                                             double* __restrict__ part, uint32_t blk, uint32_t slot) {
+    const uint32_t b0 = blk * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
     const double acc = (b0 + kBlock * kCpt <= F.n) ? start_body<false>(F, base, rsrc, r, x)
                                                    : start_body<true>(F, base, rsrc, r, x);
-    block_partial(acc, part);
+    block_partial(acc, part, slot);
 }
 
+__global__ void __launch_bounds__(kBlock) k_cg_start(Flat F, const float* __restrict__ rsrc, float* r,
+                                                     float* __restrict__ x, double* __restrict__ part) {
+    cg_start_run(F, rsrc, r, x, part, xcd_block(), blockIdx.x);
+}
+
+struct MStart {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    const float* rsrc;
+    float *r, *x;
+    double* part;
+    uint32_t G;
+    __device__ void run(uint32_t b) const { cg_start_run(F, rsrc, r, x, part, xcd_of(b, G), b); }
+};
+
 // Sums `count` pairs of partials (fixed order) into out[0], out[1].
-__global__ void __launch_bounds__(1024) k_sum2(const double* __restrict__ part, uint32_t count,
-                                               double* __restrict__ out) {
+__device__ __forceinline__ void sum2_run(const double* __restrict__ part, uint32_t count, double* __restrict__ out) {
     double s[2] = {0.0, 0.0};
     for (uint32_t i = threadIdx.x; i < count; i += 1024) {
         s[0] += part[2 * size_t(i)];
@@ -458,6 +556,19 @@ __global__ void __launch_bounds__(1024) k_sum2(const double* __restrict__ part, 
     }
 }
 
+__global__ void __launch_bounds__(1024) k_sum2(const double* __restrict__ part, uint32_t count,
+                                               double* __restrict__ out) {
+    sum2_run(part, count, out);
+}
+
+struct MSum2 {
+    static constexpr int kThreads = 1024;
+    const double* part;
+    uint32_t count;
+    double* out;
+    __device__ void run(uint32_t) const { sum2_run(part, count, out); }
+};
+
 // The one-block finishing kernel: sums `count` partials in a fixed order and
 // updates the scalars.
 enum FinishKind : int {
@@ -469,9 +580,8 @@ enum FinishKind : int {
 };
 constexpr int kFinishBlock = 1024;
 
-__global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restrict__ part, uint32_t count,
-                                                         CgScalars* sc, int kind, double rel_inner,
-                                                         double abs_target2) {
+__device__ __forceinline__ void finish_run(const double* __restrict__ part, uint32_t count, CgScalars* sc, int kind,
+                                           double rel_inner, double abs_target2) {
     if (kind != kFinishInit && sc->done) return;
     // four independent chains: the loads of a thread are in flight together
     double s4[4] = {0.0, 0.0, 0.0, 0.0};
@@ -527,6 +637,22 @@ __global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restric
             break;
     }
 }
+
+__global__ void __launch_bounds__(kFinishBlock) k_finish(const double* __restrict__ part, uint32_t count,
+                                                         CgScalars* sc, int kind, double rel_inner,
+                                                         double abs_target2) {
+    finish_run(part, count, sc, kind, rel_inner, abs_target2);
+}
+
+struct MFinish {
+    static constexpr int kThreads = kFinishBlock;
+    const double* part;
+    uint32_t count;
+    CgScalars* sc;
+    int kind;
+    double rel_inner, abs_target2;
+    __device__ void run(uint32_t) const { finish_run(part, count, sc, kind, rel_inner, abs_target2); }
+};
 
 // ---------------------------------------------------------------------------
 // Divergence check of the projected velocity: per block the FP64 sums of
@@ -651,20 +777,45 @@ __device__ __forceinline__ double smooth_body(const Flat& F, const MgLevel& L, u
 }
 
 template <bool FIRST, bool RZ>
-__global__ void __launch_bounds__(kBlock) k_mg_smooth(Flat F, MgLevel L, const uint16_t* __restrict__ code,
-                                                      const float* __restrict__ b, const float* __restrict__ xi,
-                                                      float* __restrict__ xo, float omega,
-                                                      const CgScalars* __restrict__ sc, double* __restrict__ part) {
+__device__ __forceinline__ void smooth_run(Flat F, MgLevel L, const uint16_t* __restrict__ code,
+                                           const float* __restrict__ b, const float* __restrict__ xi,
+                                           float* __restrict__ xo, float omega, const CgScalars* __restrict__ sc,
+                                           double* __restrict__ part, uint32_t blk, uint32_t slot) {
     __shared__ float aw[64], w[64];
     if (sc->done) return;
     build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
+    const uint32_t b0 = blk * (kBlock * kCpt);
     const uint32_t base = b0 + threadIdx.x;
     const double acc = (b0 + kBlock * kCpt <= F.n)
                            ? smooth_body<FIRST, false>(F, L, base, code, b, xi, xo, aw, w)
                            : smooth_body<FIRST, true>(F, L, base, code, b, xi, xo, aw, w);
-    if (RZ) block_partial(acc, part);
+    if (RZ) block_partial(acc, part, slot);
 }
+
+template <bool FIRST, bool RZ>
+__global__ void __launch_bounds__(kBlock) k_mg_smooth(Flat F, MgLevel L, const uint16_t* __restrict__ code,
+                                                      const float* __restrict__ b, const float* __restrict__ xi,
+                                                      float* __restrict__ xo, float omega,
+                                                      const CgScalars* __restrict__ sc, double* __restrict__ part) {
+    smooth_run<FIRST, RZ>(F, L, code, b, xi, xo, omega, sc, part, xcd_block(), blockIdx.x);
+}
+
+template <bool FIRST, bool RZ>
+struct MSmooth {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    MgLevel L;
+    const uint16_t* code;
+    const float *b, *xi;
+    float* xo;
+    float omega;
+    const CgScalars* sc;
+    double* part;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        smooth_run<FIRST, RZ>(F, L, code, b, xi, xo, omega, sc, part, xcd_of(blk, G), blk);
+    }
+};
 
 // The first two sweeps from x = 0 in one pass: x1 = omega D^-1 b is
 // pointwise, so x2 = x1 + omega D^-1 (b - A x1) needs only b and the codes at
@@ -708,19 +859,44 @@ __device__ __forceinline__ double smooth2_body(const Flat& F, const MgLevel& L, 
 }
 
 template <bool RZ>
+__device__ __forceinline__ void smooth2_run(Flat F, MgLevel L, const uint16_t* __restrict__ code,
+                                            const float* __restrict__ b, float* __restrict__ xo, float omega,
+                                            const CgScalars* __restrict__ sc, double* __restrict__ part,
+                                            uint32_t blk, uint32_t slot) {
+    __shared__ float aw[64], w[64];
+    if (sc->done) return;
+    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
+    const uint32_t b0 = blk * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n) ? smooth2_body<false>(F, L, base, code, b, xo, aw, w)
+                                                   : smooth2_body<true>(F, L, base, code, b, xo, aw, w);
+    if (RZ) block_partial(acc, part, slot);
+}
+
+template <bool RZ>
 __global__ void __launch_bounds__(kBlock) k_mg_smooth2_first(Flat F, MgLevel L, const uint16_t* __restrict__ code,
                                                              const float* __restrict__ b, float* __restrict__ xo,
                                                              float omega, const CgScalars* __restrict__ sc,
                                                              double* __restrict__ part) {
-    __shared__ float aw[64], w[64];
-    if (sc->done) return;
-    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
-    const uint32_t base = b0 + threadIdx.x;
-    const double acc = (b0 + kBlock * kCpt <= F.n) ? smooth2_body<false>(F, L, base, code, b, xo, aw, w)
-                                                   : smooth2_body<true>(F, L, base, code, b, xo, aw, w);
-    if (RZ) block_partial(acc, part);
+    smooth2_run<RZ>(F, L, code, b, xo, omega, sc, part, xcd_block(), blockIdx.x);
 }
+
+template <bool RZ>
+struct MSmooth2 {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    MgLevel L;
+    const uint16_t* code;
+    const float* b;
+    float* xo;
+    float omega;
+    const CgScalars* sc;
+    double* part;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        smooth2_run<RZ>(F, L, code, b, xo, omega, sc, part, xcd_of(blk, G), blk);
+    }
+};
 
 // The first sweep after the coarse correction, with the prolongation by
 // injection applied on the fly: x_in = x + xc[parent] at the seven points
@@ -780,6 +956,23 @@ __device__ __forceinline__ double smooth_prolong_body(const Flat& F, const MgLev
 }
 
 template <bool RZ>
+__device__ __forceinline__ void smooth_prolong_run(Flat F, MgLevel L, Prolong P, const uint16_t* __restrict__ code,
+                                                   const float* __restrict__ b, const float* __restrict__ xi,
+                                                   const float* __restrict__ xc, float* __restrict__ xo, float omega,
+                                                   const CgScalars* __restrict__ sc, double* __restrict__ part,
+                                                   uint32_t blk, uint32_t slot) {
+    __shared__ float aw[64], w[64];
+    if (sc->done) return;
+    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
+    const uint32_t b0 = blk * (kBlock * kCpt);
+    const uint32_t base = b0 + threadIdx.x;
+    const double acc = (b0 + kBlock * kCpt <= F.n)
+                           ? smooth_prolong_body<false>(F, L, P, base, code, b, xi, xc, xo, aw, w)
+                           : smooth_prolong_body<true>(F, L, P, base, code, b, xi, xc, xo, aw, w);
+    if (RZ) block_partial(acc, part, slot);
+}
+
+template <bool RZ>
 __global__ void __launch_bounds__(kBlock) k_mg_smooth_prolong(Flat F, MgLevel L, Prolong P,
                                                               const uint16_t* __restrict__ code,
                                                               const float* __restrict__ b,
@@ -787,29 +980,38 @@ __global__ void __launch_bounds__(kBlock) k_mg_smooth_prolong(Flat F, MgLevel L,
                                                               const float* __restrict__ xc, float* __restrict__ xo,
                                                               float omega, const CgScalars* __restrict__ sc,
                                                               double* __restrict__ part) {
-    __shared__ float aw[64], w[64];
-    if (sc->done) return;
-    build_tables<true>(aw, w, L.sx, L.sy, L.sz, omega);
-    const uint32_t b0 = xcd_block() * (kBlock * kCpt);
-    const uint32_t base = b0 + threadIdx.x;
-    const double acc = (b0 + kBlock * kCpt <= F.n)
-                           ? smooth_prolong_body<false>(F, L, P, base, code, b, xi, xc, xo, aw, w)
-                           : smooth_prolong_body<true>(F, L, P, base, code, b, xi, xc, xo, aw, w);
-    if (RZ) block_partial(acc, part);
+    smooth_prolong_run<RZ>(F, L, P, code, b, xi, xc, xo, omega, sc, part, xcd_block(), blockIdx.x);
 }
+
+template <bool RZ>
+struct MProlong {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    MgLevel L;
+    Prolong P;
+    const uint16_t* code;
+    const float *b, *xi, *xc;
+    float* xo;
+    float omega;
+    const CgScalars* sc;
+    double* part;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        smooth_prolong_run<RZ>(F, L, P, code, b, xi, xc, xo, omega, sc, part, xcd_of(blk, G), blk);
+    }
+};
 
 // Restriction of the residual, one work-item per coarse cell:
 // bc = weight * sum over fluid children of (b - A x).
-__global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Flat FF, MgLevel Fd,
-                                                        const uint16_t* __restrict__ codef,
-                                                        const float* __restrict__ bf, const float* __restrict__ xf,
-                                                        const uint16_t* __restrict__ codec, float* __restrict__ bc,
-                                                        float weight, double rnx, double rplane,
-                                                        const CgScalars* __restrict__ sc) {
+__device__ __forceinline__ void restrict_run(Flat FC, MgLevel Cd, Flat FF, MgLevel Fd,
+                                             const uint16_t* __restrict__ codef, const float* __restrict__ bf,
+                                             const float* __restrict__ xf, const uint16_t* __restrict__ codec,
+                                             float* __restrict__ bc, float weight, double rnx, double rplane,
+                                             const CgScalars* __restrict__ sc, uint32_t blk) {
     __shared__ float aw[64];
     if (sc->done) return;
     build_tables<false>(aw, nullptr, Fd.sx, Fd.sy, Fd.sz, 0.0f);
-    const uint32_t c = xcd_block() * kBlock + threadIdx.x;
+    const uint32_t c = blk * kBlock + threadIdx.x;
     const bool ok = c < FC.n;
     const uint32_t cc = ok ? c : FC.n - 1;
     const uint32_t Z = udiv(cc, FC.sz, rplane);
@@ -843,21 +1045,48 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Fla
     if (ok) bc[c] = (codec[c] & kCodeFluid) ? sum * weight : 0.0f;
 }
 
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(Flat FC, MgLevel Cd, Flat FF, MgLevel Fd,
+                                                        const uint16_t* __restrict__ codef,
+                                                        const float* __restrict__ bf, const float* __restrict__ xf,
+                                                        const uint16_t* __restrict__ codec, float* __restrict__ bc,
+                                                        float weight, double rnx, double rplane,
+                                                        const CgScalars* __restrict__ sc) {
+    restrict_run(FC, Cd, FF, Fd, codef, bf, xf, codec, bc, weight, rnx, rplane, sc, xcd_block());
+}
+
+struct MRestrict {
+    static constexpr int kThreads = kBlock;
+    Flat FC;
+    MgLevel Cd;
+    Flat FF;
+    MgLevel Fd;
+    const uint16_t* codef;
+    const float *bf, *xf;
+    const uint16_t* codec;
+    float* bc;
+    float weight;
+    double rnx, rplane;
+    const CgScalars* sc;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        restrict_run(FC, Cd, FF, Fd, codef, bf, xf, codec, bc, weight, rnx, rplane, sc, xcd_of(blk, G));
+    }
+};
+
 // Restriction when every axis is halved: one work-item per fine cell.  A
 // block covers 64 fine cells along x in each of the 2 x 2 rows (y, z) of one
 // coarse row: each wave computes its row's residuals, sums x-pairs with a
 // lane shuffle, and 32 work-items add the four rows' pair sums (a fixed
 // order) into the 32 coarse cells.
-__global__ void __launch_bounds__(kBlock) k_mg_restrict2(Flat FF, MgLevel Fd, Flat FC, MgLevel Cd, uint32_t gx,
-                                                         const uint16_t* __restrict__ codef,
-                                                         const float* __restrict__ bf, const float* __restrict__ xf,
-                                                         const uint16_t* __restrict__ codec, float* __restrict__ bc,
-                                                         float weight, const CgScalars* __restrict__ sc) {
+__device__ __forceinline__ void restrict2_run(Flat FF, MgLevel Fd, Flat FC, MgLevel Cd, uint32_t gx,
+                                              const uint16_t* __restrict__ codef, const float* __restrict__ bf,
+                                              const float* __restrict__ xf, const uint16_t* __restrict__ codec,
+                                              float* __restrict__ bc, float weight, const CgScalars* __restrict__ sc,
+                                              uint32_t blk) {
     __shared__ float aw[64];
     __shared__ float sh[4][32];
     if (sc->done) return;
     build_tables<false>(aw, nullptr, Fd.sx, Fd.sy, Fd.sz, 0.0f);
-    const uint32_t blk = xcd_block();
     const uint32_t xcb = blk % gx;                  // 64-cell chunk along x
     const uint32_t YZ = blk / gx;
     const uint32_t Y = YZ % uint32_t(Cd.ny), Z = YZ / uint32_t(Cd.ny);
@@ -887,6 +1116,33 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict2(Flat FF, MgLevel Fd, Fl
         }
     }
 }
+
+__global__ void __launch_bounds__(kBlock) k_mg_restrict2(Flat FF, MgLevel Fd, Flat FC, MgLevel Cd, uint32_t gx,
+                                                         const uint16_t* __restrict__ codef,
+                                                         const float* __restrict__ bf, const float* __restrict__ xf,
+                                                         const uint16_t* __restrict__ codec, float* __restrict__ bc,
+                                                         float weight, const CgScalars* __restrict__ sc) {
+    restrict2_run(FF, Fd, FC, Cd, gx, codef, bf, xf, codec, bc, weight, sc, xcd_block());
+}
+
+struct MRestrict2 {
+    static constexpr int kThreads = kBlock;
+    Flat FF;
+    MgLevel Fd;
+    Flat FC;
+    MgLevel Cd;
+    uint32_t gx;
+    const uint16_t* codef;
+    const float *bf, *xf;
+    const uint16_t* codec;
+    float* bc;
+    float weight;
+    const CgScalars* sc;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        restrict2_run(FF, Fd, FC, Cd, gx, codef, bf, xf, codec, bc, weight, sc, xcd_of(blk, G));
+    }
+};
 
 // ---------------------------------------------------------------------------
 // Tiled restriction.  A block owns a 64 x 4 x 4 tile of fine cells (work-item
@@ -924,8 +1180,7 @@ inline uint32_t tiles_grid(const Tiles& T) {
 
 // Tile origin of this block (the extra blocks of the rounded grid get a z
 // beyond the grid: every cell of theirs is outside).
-__device__ __forceinline__ void tile_origin(const Tiles& T, int& x0, int& y0, int& z0) {
-    const uint32_t blk = xcd_block();
+__device__ __forceinline__ void tile_origin(const Tiles& T, uint32_t blk, int& x0, int& y0, int& z0) {
     const uint32_t tx = blk % T.tx, r = blk / T.tx;
     const uint32_t ty = r % T.ty, tz = r / T.ty;
     x0 = int(tx) * kTX;
@@ -976,20 +1231,17 @@ __device__ __forceinline__ void box_nbrs(const float* box, int e, float (&v)[6])
 // Restriction on a tile (every axis halved): the residuals of the tile's
 // cells go to LDS, then 128 work-items sum each 2 x 2 x 2 block in a fixed
 // order into the 32 x 2 x 2 coarse cells of the tile.
-__global__ void __launch_bounds__(kBlock) k_mg_restrict_tile(Tiles TL, MgLevel Fd, Tiles TC,
-                                                             const uint16_t* __restrict__ codef,
-                                                             const float* __restrict__ bf,
-                                                             const float* __restrict__ xf,
-                                                             const uint16_t* __restrict__ codec,
-                                                             float* __restrict__ bc,
-                                                             const CgScalars* __restrict__ sc) {
+__device__ __forceinline__ void restrict_tile_run(Tiles TL, MgLevel Fd, Tiles TC, const uint16_t* __restrict__ codef,
+                                                  const float* __restrict__ bf, const float* __restrict__ xf,
+                                                  const uint16_t* __restrict__ codec, float* __restrict__ bc,
+                                                  const CgScalars* __restrict__ sc, uint32_t blk) {
     __shared__ float aw[64];
     __shared__ float xb[kBox];
     __shared__ float res[kTZ][kTY][kTX];
     if (sc->done) return;
     build_tables<false>(aw, nullptr, Fd.sx, Fd.sy, Fd.sz, 0.0f);
     int x0, y0, z0;
-    tile_origin(TL, x0, y0, z0);
+    tile_origin(TL, blk, x0, y0, z0);
     const int lx = int(threadIdx.x) & 63, ly = int(threadIdx.x) >> 6;
     const int gx = x0 + lx, gy = y0 + ly;
     uint16_t cd[kTZ];
@@ -1027,6 +1279,32 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict_tile(Tiles TL, MgLevel F
         }
     }
 }
+
+__global__ void __launch_bounds__(kBlock) k_mg_restrict_tile(Tiles TL, MgLevel Fd, Tiles TC,
+                                                             const uint16_t* __restrict__ codef,
+                                                             const float* __restrict__ bf,
+                                                             const float* __restrict__ xf,
+                                                             const uint16_t* __restrict__ codec,
+                                                             float* __restrict__ bc,
+                                                             const CgScalars* __restrict__ sc) {
+    restrict_tile_run(TL, Fd, TC, codef, bf, xf, codec, bc, sc, xcd_block());
+}
+
+struct MRestrictTile {
+    static constexpr int kThreads = kBlock;
+    Tiles TL;
+    MgLevel Fd;
+    Tiles TC;
+    const uint16_t* codef;
+    const float *bf, *xf;
+    const uint16_t* codec;
+    float* bc;
+    const CgScalars* sc;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        restrict_tile_run(TL, Fd, TC, codef, bf, xf, codec, bc, sc, xcd_of(blk, G));
+    }
+};
 
 // ---------------------------------------------------------------------------
 // The V-cycle on the smallest levels in one work-group, iterates in LDS.
@@ -1079,7 +1357,7 @@ __device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const uint16_t* 
 // LDS layout (floats): per level the tables aw[64], w[64]; then per level the
 // codes (uint16, each level padded to an even count); then level 0: x, t;
 // deeper levels: b, x, t.
-__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A, const CgScalars* __restrict__ sc) {
+__device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars* __restrict__ sc) {
     if (sc->done) return;
     extern __shared__ float lds[];
     float* tab = lds;
@@ -1181,6 +1459,17 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
     }
     for (uint32_t c = threadIdx.x; c < ns[0]; c += kCoarseThreads) A.x0[c] = xs[0][c];
 }
+
+__global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A, const CgScalars* __restrict__ sc) {
+    coarse_run(A, sc);
+}
+
+struct MCoarse {
+    static constexpr int kThreads = kCoarseThreads;
+    CoarseArgs A;
+    const CgScalars* sc;
+    __device__ void run(uint32_t) const { coarse_run(A, sc); }
+};
 
 // ===========================================================================
 // Fused descent and ascent of one level (every axis halved, three sweeps),
@@ -1505,22 +1794,22 @@ __global__ void __launch_bounds__(kMThreads) k_mg_march_wet(March M, const uint1
 }
 
 template <bool POST, bool RZ>
-__global__ void __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(8)))
-k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* __restrict__ b,
-           const float* __restrict__ xi, const float* __restrict__ xc, float* __restrict__ xo,
-           const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
-           const CgScalars* __restrict__ sc, double* __restrict__ part, const uint8_t* __restrict__ wet) {
+__device__ __forceinline__ void march_run(const March& M, const MgLevel& L, const uint16_t* __restrict__ code,
+                                          const float* __restrict__ b, const float* __restrict__ xi,
+                                          const float* __restrict__ xc, float* __restrict__ xo,
+                                          const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
+                                          const CgScalars* __restrict__ sc, double* __restrict__ part,
+                                          const uint8_t* __restrict__ wet, uint32_t blk, uint32_t slot) {
     __shared__ float lds[kLdsSlots + (POST ? 6 : 8) * kMThreads];
     __shared__ double red[kMThreads / 64];
     if (sc->done) return;
-    const uint32_t blk = xcd_block();
     if (blk >= M.blocks) {
-        if (RZ && threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        if (RZ && threadIdx.x == 0) part[slot] = 0.0;
         return;
     }
     if (wet && !wet[blk]) {
         march_dry<POST>(M, blk, xo, bco);
-        if (RZ && threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        if (RZ && threadIdx.x == 0) part[slot] = 0.0;
         return;
     }
     build_tables<true>(lds + kLdsAw, lds + kLdsW, L.sx, L.sy, L.sz, omega);
@@ -1581,9 +1870,49 @@ k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* _
             double t = 0.0;
 #pragma unroll
             for (int q = 0; q < kMThreads / 64; ++q) t += red[q];
-            part[blockIdx.x] = t;
+            part[slot] = t;
         }
     }
+}
+
+template <bool POST, bool RZ>
+__global__ void __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* __restrict__ b,
+           const float* __restrict__ xi, const float* __restrict__ xc, float* __restrict__ xo,
+           const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
+           const CgScalars* __restrict__ sc, double* __restrict__ part, const uint8_t* __restrict__ wet) {
+    march_run<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, sc, part, wet, xcd_block(), blockIdx.x);
+}
+
+template <bool POST, bool RZ>
+struct MMarch {
+    static constexpr int kThreads = kMThreads;
+    March M;
+    MgLevel L;
+    const uint16_t* code;
+    const float *b, *xi, *xc;
+    float* xo;
+    const uint16_t* codec;
+    float* bco;
+    float omega;
+    const CgScalars* sc;
+    double* part;
+    const uint8_t* wet;
+    uint32_t G;
+    __device__ void run(uint32_t blk) const {
+        march_run<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, sc, part, wet, xcd_of(blk, G), blk);
+    }
+};
+
+// The multi-scene wrapper of the fused march, with its occupancy attribute.
+template <class A>
+__global__ void __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_multi_march(const A* __restrict__ tab, const uint32_t* __restrict__ map, const uint8_t* __restrict__ active) {
+    const uint32_t e = map[blockIdx.x];
+    const uint32_t s = e >> kMapShift;
+    if (active && !active[s]) return;
+    const A a = tab[s];
+    a.run(e & kMapLocal);
 }
 
 // ===========================================================================
@@ -1877,6 +2206,249 @@ hipError_t launch_mg_restrict(hipStream_t s, const MgDims& f, const MgDims& c, c
     }
     return LAUNCH(k_mg_restrict, grid_for(FC.n), FC, c, FF, f, codef, bf, xf, codec, bc, 1.0f / float(slots),
                   1.0 / double(FC.sy), 1.0 / double(FC.sz), sc);
+}
+
+// ===========================================================================
+// Multi-scene records: the launches launch_cg_start, launch_col_p_ap,
+// launch_col_update, launch_refine_true, launch_mg_smooth,
+// launch_mg_smooth2_first, launch_mg_smooth_prolong, launch_mg_restrict,
+// launch_mg_coarse_cycle, launch_mg_descend and launch_mg_ascend make, as
+// argument structs for k_multi (the same arguments, grids and order).
+template <class A>
+static void push_rec(std::vector<MRec>& out, int32_t kind, const A& a, uint32_t blocks, uint32_t shmem = 0) {
+    static_assert(sizeof(A) <= kMRecBytes, "argument struct fits a record");
+    MRec r;
+    r.kind = kind;
+    r.blocks = blocks;
+    r.shmem = shmem;
+    r.bytes = uint32_t(sizeof(A));
+    std::memcpy(r.args, &a, sizeof(A));
+    out.push_back(r);
+}
+
+static void rec_finish(std::vector<MRec>& out, const double* part, uint32_t count, CgScalars* sc, int kind,
+                       double rel_inner = 0.0, double abs_target2 = 0.0) {
+    push_rec(out, kMkFinish, MFinish{part, count, sc, kind, rel_inner, abs_target2}, 1u);
+}
+
+size_t finish_arg_bytes() { return sizeof(MFinish); }
+void finish_set_target(unsigned char* args, double abs_target2) {
+    MFinish f;
+    std::memcpy(&f, args, sizeof(MFinish));
+    f.abs_target2 = abs_target2;
+    std::memcpy(args, &f, sizeof(MFinish));
+}
+
+void rec_cg_start(std::vector<MRec>& out, const GridDev& g, const float* rsrc, float* r, float* x, CgScalars* sc,
+                  ReduceBuf red, double rel_inner, double abs_target2) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    push_rec(out, kMkStart, MStart{F, rsrc, r, x, red.partials, G}, G);
+    rec_finish(out, red.partials, G, sc, kFinishInit, rel_inner, abs_target2);
+}
+
+void rec_col_p_ap(std::vector<MRec>& out, const GridDev& g, const float scale[3], const uint16_t* codes,
+                  const float* zv, float* p, float* ap, CgScalars* sc, ReduceBuf red, const uint8_t* wet) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    push_rec(out, kMkPap, MPap{F, codes, scale[0], scale[1], scale[2], zv, p, ap, sc, red.partials, wet, G}, G);
+    rec_finish(out, red.partials, G, sc, kFinishAlpha);
+}
+
+void rec_col_update(std::vector<MRec>& out, const GridDev& g, const float* p, const float* ap, float* x, float* r,
+                    CgScalars* sc, ReduceBuf red, int identity, const uint8_t* wet) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    push_rec(out, kMkUpdate, MUpdate{F, p, ap, x, r, sc, red.partials, wet, G}, G);
+    rec_finish(out, red.partials, G, sc, identity ? kFinishRrIdent : kFinishRr);
+}
+
+void rec_refine_true(std::vector<MRec>& out, const GridDev& g, const float scale[3], const uint16_t* codes,
+                     const float* xcorr, double* x64, const float* b, float* r, int mode, ReduceBuf red,
+                     double* out2) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    if (mode == 1) {
+        push_rec(out, kMkRefine1, MRefine<1>{F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials, G},
+                 G);
+    } else {
+        const uint32_t ga = grid_for(F.n);
+        push_rec(out, kMkAcc64, MAcc64{F.n, xcorr, x64, ga}, ga);
+        push_rec(out, kMkRefine2, MRefine<2>{F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials, G},
+                 G);
+    }
+    push_rec(out, kMkSum2, MSum2{red.partials, G, out2}, 1u);
+}
+
+void rec_mg_smooth(std::vector<MRec>& out, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
+                   float* xo, float omega, CgScalars* sc, ReduceBuf red, int first, int with_rz) {
+    const Flat F = make_flat(l.nx, l.ny, l.nz);
+    const uint32_t G = flat_grid(F.n);
+    if (first && with_rz) push_rec(out, kMkSmoothTT, MSmooth<true, true>{F, l, code, b, xi, xo, omega, sc, red.partials, G}, G);
+    else if (first) push_rec(out, kMkSmoothTF, MSmooth<true, false>{F, l, code, b, xi, xo, omega, sc, red.partials, G}, G);
+    else if (with_rz) push_rec(out, kMkSmoothFT, MSmooth<false, true>{F, l, code, b, xi, xo, omega, sc, red.partials, G}, G);
+    else push_rec(out, kMkSmoothFF, MSmooth<false, false>{F, l, code, b, xi, xo, omega, sc, red.partials, G}, G);
+    if (with_rz) rec_finish(out, red.partials, G, sc, kFinishRz);
+}
+
+void rec_mg_smooth2_first(std::vector<MRec>& out, const MgDims& l, const uint16_t* code, const float* b, float* xo,
+                          float omega, CgScalars* sc, ReduceBuf red, int with_rz) {
+    const Flat F = make_flat(l.nx, l.ny, l.nz);
+    const uint32_t G = flat_grid(F.n);
+    if (with_rz) push_rec(out, kMkSmooth2T, MSmooth2<true>{F, l, code, b, xo, omega, sc, red.partials, G}, G);
+    else push_rec(out, kMkSmooth2F, MSmooth2<false>{F, l, code, b, xo, omega, sc, red.partials, G}, G);
+    if (with_rz) rec_finish(out, red.partials, G, sc, kFinishRz);
+}
+
+void rec_mg_smooth_prolong(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code,
+                           const float* b, const float* xi, const float* xc, float* xo, float omega, CgScalars* sc,
+                           ReduceBuf red, int with_rz) {
+    const Flat F = make_flat(f.nx, f.ny, f.nz);
+    const uint32_t G = flat_grid(F.n);
+    Prolong P;
+    P.FC = make_flat(c.nx, c.ny, c.nz);
+    P.hx = f.nx > 1 ? 1 : 0;
+    P.hy = f.ny > 1 ? 1 : 0;
+    P.hz = f.nz > 1 ? 1 : 0;
+    P.rnx = 1.0 / double(F.sy);
+    P.rplane = 1.0 / double(F.sz);
+    if (with_rz) push_rec(out, kMkProlongT, MProlong<true>{F, f, P, code, b, xi, xc, xo, omega, sc, red.partials, G}, G);
+    else push_rec(out, kMkProlongF, MProlong<false>{F, f, P, code, b, xi, xc, xo, omega, sc, red.partials, G}, G);
+    if (with_rz) rec_finish(out, red.partials, G, sc, kFinishRz);
+}
+
+void rec_mg_restrict(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* codef, const float* bf,
+                     const float* xf, const uint16_t* codec, float* bc, const CgScalars* sc) {
+    const Flat FC = make_flat(c.nx, c.ny, c.nz), FF = make_flat(f.nx, f.ny, f.nz);
+    const int slots = (f.nx > 1 ? 2 : 1) * (f.ny > 1 ? 2 : 1) * (f.nz > 1 ? 2 : 1);
+    if (slots == 8 && use_tiles(f)) {
+        const Tiles T = make_tiles(f), TC = make_tiles(c);
+        const uint32_t G = tiles_grid(T);
+        push_rec(out, kMkRestrictTile, MRestrictTile{T, f, TC, codef, bf, xf, codec, bc, sc, G}, G);
+        return;
+    }
+    if (slots == 8) {
+        const uint32_t gx = (uint32_t(f.nx) + 63) / 64;
+        const uint32_t G = uint32_t(size_t(gx) * c.ny * c.nz + (8 - (size_t(gx) * c.ny * c.nz) % 8) % 8);
+        push_rec(out, kMkRestrict2, MRestrict2{FF, f, FC, c, gx, codef, bf, xf, codec, bc, 0.125f, sc, G}, G);
+        return;
+    }
+    const uint32_t G = grid_for(FC.n);
+    push_rec(out, kMkRestrict,
+             MRestrict{FC, c, FF, f, codef, bf, xf, codec, bc, 1.0f / float(slots), 1.0 / double(FC.sy),
+                       1.0 / double(FC.sz), sc, G},
+             G);
+}
+
+bool rec_mg_coarse_cycle(std::vector<MRec>& out, const MgDims* levels, const uint16_t* const* codes, int nlev,
+                         const float* b0, float* x0, int nu, int last_sweeps, float omega, const CgScalars* sc) {
+    if (nlev < 1 || nlev > kCoarseMaxLevels) return false;
+    MCoarse m;
+    m.A.nlev = nlev;
+    m.A.nu = nu;
+    m.A.last_sweeps = last_sweeps;
+    m.A.omega = omega;
+    m.A.b0 = b0;
+    m.A.x0 = x0;
+    for (int l = 0; l < nlev; ++l) {
+        m.A.lv[l].nx = levels[l].nx; m.A.lv[l].ny = levels[l].ny; m.A.lv[l].nz = levels[l].nz;
+        m.A.lv[l].sx = levels[l].sx; m.A.lv[l].sy = levels[l].sy; m.A.lv[l].sz = levels[l].sz;
+        m.A.lv[l].code = codes[l];
+    }
+    m.sc = sc;
+    push_rec(out, kMkCoarse, m, 1u, uint32_t(mg_coarse_lds_bytes(levels, nlev)));
+    return true;
+}
+
+void rec_mg_descend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                    float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
+                    const uint8_t* wet) {
+    const March M = make_march(f, c);
+    const uint32_t G = mg_march_grid(f, c);
+    push_rec(out, kMkMarchDesc,
+             MMarch<false, false>{M, f, code, b, nullptr, nullptr, x, codec, bc, omega, sc, nullptr, wet, G}, G);
+}
+
+void rec_mg_ascend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
+                   const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
+                   int with_rz, const uint8_t* wet) {
+    const March M = make_march(f, c);
+    const uint32_t G = mg_march_grid(f, c);
+    if (with_rz) {
+        push_rec(out, kMkMarchAscRz,
+                 MMarch<true, true>{M, f, code, b, xi, xc, xo, nullptr, nullptr, omega, sc, red.partials, wet, G}, G);
+        rec_finish(out, red.partials, G, sc, kFinishRz);
+    } else {
+        push_rec(out, kMkMarchAsc,
+                 MMarch<true, false>{M, f, code, b, xi, xc, xo, nullptr, nullptr, omega, sc, nullptr, wet, G}, G);
+    }
+}
+
+// Copies each scene's CG scalars and two refinement sums into packed arrays
+// (one work-item per scene).
+struct MGather {
+    static constexpr int kThreads = 64;
+    const CgScalars* sc;
+    const double* out2;
+    CgScalars* dst_sc;
+    double* dst2;
+    __device__ void run(uint32_t) const {
+        if (threadIdx.x == 0) {
+            *dst_sc = *sc;
+            dst2[0] = out2[0];
+            dst2[1] = out2[1];
+        }
+    }
+};
+
+void rec_gather(std::vector<MRec>& out, const CgScalars* sc, const double* out2, CgScalars* dst_sc, double* dst2) {
+    push_rec(out, kMkGather, MGather{sc, out2, dst_sc, dst2}, 1u);
+}
+
+template <class A>
+static hipError_t launch_m(hipStream_t s, const char* name, const void* tab, const uint32_t* map, uint32_t nblocks,
+                           uint32_t shmem, const uint8_t* active) {
+    return launch_dims(k_multi<A>, name, dim3(nblocks), dim3(A::kThreads), shmem, s, static_cast<const A*>(tab), map,
+                       active);
+}
+
+template <class A>
+static hipError_t launch_m_march(hipStream_t s, const void* tab, const uint32_t* map, uint32_t nblocks,
+                                 const uint8_t* active) {
+    return launch_dims(k_multi_march<A>, "k_multi_march", dim3(nblocks), dim3(A::kThreads), 0u, s,
+                       static_cast<const A*>(tab), map, active);
+}
+
+hipError_t launch_multi(hipStream_t s, int32_t kind, const void* tab, const uint32_t* map, uint32_t nblocks,
+                        uint32_t shmem, const uint8_t* active) {
+    if (nblocks == 0) return hipSuccess;
+    switch (kind) {
+        case kMkStart: return launch_m<MStart>(s, "k_multi_cg_start", tab, map, nblocks, 0, active);
+        case kMkFinish: return launch_m<MFinish>(s, "k_multi_finish", tab, map, nblocks, 0, active);
+        case kMkPap: return launch_m<MPap>(s, "k_multi_cg_pap", tab, map, nblocks, 0, active);
+        case kMkUpdate: return launch_m<MUpdate>(s, "k_multi_cg_update", tab, map, nblocks, 0, active);
+        case kMkRefine1: return launch_m<MRefine<1>>(s, "k_multi_refine", tab, map, nblocks, 0, active);
+        case kMkRefine2: return launch_m<MRefine<2>>(s, "k_multi_refine", tab, map, nblocks, 0, active);
+        case kMkAcc64: return launch_m<MAcc64>(s, "k_multi_acc64", tab, map, nblocks, 0, active);
+        case kMkSum2: return launch_m<MSum2>(s, "k_multi_sum2", tab, map, nblocks, 0, active);
+        case kMkSmoothTT: return launch_m<MSmooth<true, true>>(s, "k_multi_mg_smooth", tab, map, nblocks, 0, active);
+        case kMkSmoothTF: return launch_m<MSmooth<true, false>>(s, "k_multi_mg_smooth", tab, map, nblocks, 0, active);
+        case kMkSmoothFT: return launch_m<MSmooth<false, true>>(s, "k_multi_mg_smooth", tab, map, nblocks, 0, active);
+        case kMkSmoothFF: return launch_m<MSmooth<false, false>>(s, "k_multi_mg_smooth", tab, map, nblocks, 0, active);
+        case kMkSmooth2T: return launch_m<MSmooth2<true>>(s, "k_multi_mg_smooth2_first", tab, map, nblocks, 0, active);
+        case kMkSmooth2F: return launch_m<MSmooth2<false>>(s, "k_multi_mg_smooth2_first", tab, map, nblocks, 0, active);
+        case kMkProlongT: return launch_m<MProlong<true>>(s, "k_multi_mg_smooth_prolong", tab, map, nblocks, 0, active);
+        case kMkProlongF: return launch_m<MProlong<false>>(s, "k_multi_mg_smooth_prolong", tab, map, nblocks, 0, active);
+        case kMkRestrict: return launch_m<MRestrict>(s, "k_multi_mg_restrict", tab, map, nblocks, 0, active);
+        case kMkRestrict2: return launch_m<MRestrict2>(s, "k_multi_mg_restrict", tab, map, nblocks, 0, active);
+        case kMkRestrictTile: return launch_m<MRestrictTile>(s, "k_multi_mg_restrict_tile", tab, map, nblocks, 0, active);
+        case kMkCoarse: return launch_m<MCoarse>(s, "k_multi_mg_coarse_cycle", tab, map, nblocks, shmem, active);
+        case kMkMarchDesc: return launch_m_march<MMarch<false, false>>(s, tab, map, nblocks, active);
+        case kMkMarchAsc: return launch_m_march<MMarch<true, false>>(s, tab, map, nblocks, active);
+        case kMkMarchAscRz: return launch_m_march<MMarch<true, true>>(s, tab, map, nblocks, active);
+        case kMkGather: return launch_m<MGather>(s, "k_multi_gather", tab, map, nblocks, 0, active);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace mgf
