@@ -76,6 +76,13 @@ void ProbeSink::release() {
 }
 }  // namespace mgf
 
+// Scratch of mgf_op_extrapolate_fc_noalloc (layout and limits with the entry
+// point below): 35.1 MB held by every device the library runs on until the
+// process ends.  Host code never passes this name to a kernel: it uses the
+// device address looked up per device (na_setup).
+static constexpr size_t kNoAllocScratchBytes = 35143424;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
 
 constexpr int kPlainDotBlocks = 1024;
@@ -1448,6 +1455,56 @@ mgf_status div_end(mgf_scene s, mgf_step_stats* acc) {
     return MGF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of mgf_op_extrapolate_fc_noalloc: one file-scope device array, so
+// that the entry point allocates nothing.  Every device the library runs on
+// holds its own copy until the process ends; its address is looked up once
+// per device at mgf_init (hipGetSymbolAddress with that device current).
+// Layout (each part 256-byte aligned): face distances uint16[kNaFaces],
+// sorted list uint32[kNaFaces], list masks uint8[kNaFaces], face masks
+// uint8[kNaFaces], sort counters uint32[kNaCountWords].
+constexpr size_t kNaFaces = size_t(1) << 22;   // faces of all three components
+constexpr int32_t kNaMaxK = 4093;              // nx + ny + nz + 3 with more than one layer
+constexpr size_t kNaCountWords = (3 * 32 + 1) * (size_t(kNaMaxK) + 2) + 1;
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaDistOff = 0;
+constexpr size_t kNaListOff = kNaDistOff + na_align(2 * kNaFaces);
+constexpr size_t kNaLmaskOff = kNaListOff + na_align(4 * kNaFaces);
+constexpr size_t kNaFmaskOff = kNaLmaskOff + na_align(kNaFaces);
+constexpr size_t kNaCountOff = kNaFmaskOff + na_align(kNaFaces);
+constexpr size_t kNaBytes = kNaCountOff + na_align(4 * kNaCountWords);
+static_assert(kNaBytes <= kNoAllocScratchBytes, "the scratch array holds the layout");
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};
+std::string g_na_err[kNaMaxDevices];
+
+// With `device` current: the device address of this device's scratch.
+void na_setup(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (g_na_ptr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        g_na_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_na_ptr[device] = p;
+}
+
+void* na_ptr(int device, std::string* err) {
+    if (device < 0 || device >= kNaMaxDevices) {
+        *err = "device index beyond the scratch table";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) *err = g_na_err[device].empty() ? "scratch not set up" : g_na_err[device];
+    return g_na_ptr[device];
+}
+
 }  // namespace
 
 namespace mgf {
@@ -1672,6 +1729,7 @@ mgf_status mgf_init(int32_t device, mgf_context* out) {
         c->cu_count = prop.multiProcessorCount;
         c->lds_per_block = prop.sharedMemPerBlock;
     }
+    na_setup(device);   // the no-allocation entry point's scratch on this device
     *out = c;
     return MGF_OK;
 }
@@ -2437,6 +2495,37 @@ mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid, float* u
     for (void* p : tmp.allocations) (void)hipFree(p);
     if (tmp.host_offs) (void)hipHostFree(tmp.host_offs);
     if (layers_used) *layers_used = used;
+    return st;
+}
+
+mgf_status mgf_op_extrapolate_fc_noalloc(mgf_context ctx, const mgf_grid* grid, float* u, float* v, float* w,
+                                         int32_t layers, int32_t* layers_used) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    TRY(check_grid(grid));
+    if (layers < 0) return fail(MGF_ERR_INVALID, "layers < 0");
+    if (!u || !v || !w) return fail(MGF_ERR_INVALID, "null face array");
+    const size_t nface = size_t(grid->nx + 1) * grid->ny * grid->nz + size_t(grid->nx) * (grid->ny + 1) * grid->nz +
+                         size_t(grid->nx) * grid->ny * (grid->nz + 1);
+    if (nface > kNaFaces)
+        return fail(MGF_ERR_INVALID, "mgf_op_extrapolate_fc_noalloc: " + std::to_string(nface) +
+                                         " faces exceed its scratch (" + std::to_string(kNaFaces) + " faces)");
+    const GridDev g = make_grid(grid);
+    const int32_t K = ext_max_distance(g);
+    if (layers > 1 && (K > kNaMaxK || ext_counts_size(K) > kNaCountWords))
+        return fail(MGF_ERR_INVALID, "mgf_op_extrapolate_fc_noalloc: nx + ny + nz + 3 = " + std::to_string(K) +
+                                         " exceeds its scratch (" + std::to_string(kNaMaxK) + ")");
+    std::string err;
+    MGF_HIP(hipSetDevice(ctx->device));
+    unsigned char* base = static_cast<unsigned char*>(na_ptr(ctx->device, &err));
+    if (!base) return fail(MGF_ERR_DEVICE, "mgf_op_extrapolate_fc_noalloc: " + err);
+    // the layer offsets come back into this stack buffer (no host allocation)
+    uint32_t offs[kNaMaxK + 3];
+    ExtBufs B{reinterpret_cast<uint16_t*>(base + kNaDistOff), reinterpret_cast<uint32_t*>(base + kNaListOff),
+              base + kNaLmaskOff, base + kNaFmaskOff, reinterpret_cast<uint32_t*>(base + kNaCountOff), offs};
+    int used = 0;
+    const mgf_status st = extrapolate_run(ctx->stream, g, u, v, w, layers, B, &used);
+    MGF_HIP(hipStreamSynchronize(ctx->stream));
+    if (layers_used) *layers_used = st == MGF_OK ? used : 0;
     return st;
 }
 

@@ -164,6 +164,7 @@ _SIGS = {
     "mgf_op_scale": [_VP, C.c_int64, C.c_float, _VP],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
     "mgf_batch_set_lockstep": [_VP, C.c_int32],
+    "mgf_op_extrapolate_fc_noalloc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
 _RESTYPE = {"mgf_status_string": C.c_char_p, "mgf_last_error": C.c_char_p}
 

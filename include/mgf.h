@@ -596,6 +596,22 @@ MGF_API mgf_status mgf_step_particles(mgf_scene scene, mgf_psolver solver, int32
 MGF_API mgf_status mgf_op_extrapolate_fc(mgf_context ctx, const mgf_grid* grid,
         float* u, float* v, float* w, int32_t layers, int32_t* layers_used);
 
+/* mgf_op_extrapolate_fc without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device or pinned host memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).  Its
+ * scratch (face distances, sorted list, masks, sort counters) lives in one
+ * static device array of 35.1 MB per device, set up by mgf_init; the layer
+ * offsets come back through a stack buffer.  Limits: at most 4194304 faces
+ * over the three components ((nx+1)ny nz + nx(ny+1)nz + nx ny(nz+1)), and
+ * with more than one layer nx + ny + nz + 3 <= 4093; larger grids return
+ * MGF_ERR_INVALID before any launch.  Results are bitwise those of
+ * mgf_op_extrapolate_fc.  Every call on one device shares that device's
+ * array: calls that overlap on one device (several host threads, or several
+ * contexts of the same device) race, so serialise them.  Returns after the
+ * context's stream has finished the work. */
+MGF_API mgf_status mgf_op_extrapolate_fc_noalloc(mgf_context ctx, const mgf_grid* grid,
+        float* u, float* v, float* w, int32_t layers, int32_t* layers_used);
+
 #ifdef __cplusplus
 }
 #endif
