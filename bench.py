@@ -167,9 +167,18 @@ def batched_leg(mgf, scenes, ctx, args):
     and per-step latency percentiles."""
     specs = [scenes.ragged_scene(i) for i in range(args.batch_scenes)]
     t0 = time.perf_counter()
-    sc = [scenes.build_gpu_scene(mgf, ctx, p, **MIXED_KW) for p in specs]
+    kw = dict(MIXED_KW, mg_kernels={"auto": 0, "fused": 2, "tune": 0}[args.batch_mg])
+    sc = [scenes.build_gpu_scene(mgf, ctx, p, **kw) for p in specs]
     t_setup = time.perf_counter() - t0
     cells = np.array([p.interior_cells for p in specs], dtype=np.float64)
+    for s in sc:
+        s.set_timing(True)      # lockstep: wall time of each batch step's phases
+    tuned = None
+    if args.batch_mg == "tune":
+        # the V-cycle kernels chosen by timing both on this batch (mgf_batch_autotune)
+        chosen, tms = mgf.batch_autotune(ctx, sc, trial_steps=3)
+        tuned = {"mg_kernels": {0: "auto", 2: "fused"}[chosen],
+                 "trial_ms_per_step": {"auto": tms[0], "fused": tms[1]}}
     mgf.batch_step(ctx, sc, 1, workers=args.batch_workers)      # warm: first-step allocations, code paths
     t0 = time.perf_counter()
     status, stats, ms = mgf.batch_step(ctx, sc, args.batch_steps, workers=args.batch_workers, step_latency=True)
@@ -196,6 +205,14 @@ def batched_leg(mgf, scenes, ctx, args):
            "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99)),
                           "max": float(ms.max())},
            "cg_iterations_per_step": iters, "residual_rel_max": res_max, "setup_s": t_setup,
+           "autotune": tuned,
+           "phases_ms_per_step": {"before_solve": float(stats[0].ms_advect / nst),
+                                  "lockstep_solve": float(stats[0].ms_pressure / nst),
+                                  "after_solve": float(stats[0].ms_extrapolate / nst),
+                                  "note": "wall time of the batch step's phases, each a sequence of "
+                                          "multi-scene launches: advection, markers, right-hand side and "
+                                          "hierarchy setup; the pressure solves in lockstep; projection and "
+                                          "extrapolation"},
            "step_roofline_frac": frac,
            "plain_one_at_a_time": plain, "plain_sample_scenes": nplain,
            "speedup_vs_plain": (value / plain) if plain else None}
@@ -353,6 +370,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-workers", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-mg", choices=["tune", "auto", "fused"], default="tune")
     ap.add_argument("--served-scenes", type=int, default=32)
     ap.add_argument("--served-seconds", type=float, default=3.0)
     ap.add_argument("--served-load", type=float, nargs="+", default=[0.5, 0.8, 1.0])

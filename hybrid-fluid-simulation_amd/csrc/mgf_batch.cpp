@@ -259,6 +259,50 @@ mgf_status mgf_batch_step(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t
     return first;
 }
 
+mgf_status mgf_batch_autotune(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t trial_steps, int32_t* chosen,
+                              double* ms) {
+    if (!ctx) return fail(MGF_ERR_INVALID, "null context");
+    if (n < 0 || (n > 0 && !scenes)) return fail(MGF_ERR_INVALID, "bad scene array");
+    if (trial_steps < 2) return fail(MGF_ERR_INVALID, "trial_steps must be >= 2");
+    std::vector<mgf_scene> ls;
+    std::unordered_set<mgf_scene> seen;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!scenes[i] || scenes[i]->ctx != ctx) return fail(MGF_ERR_INVALID, "bad scene " + std::to_string(i));
+        if (!seen.insert(scenes[i]).second) return fail(MGF_ERR_INVALID, "scene " + std::to_string(i) + " appears twice");
+        if (lockstep_eligible(scenes[i])) ls.push_back(scenes[i]);
+    }
+    if (ls.size() < 2) return fail(MGF_ERR_INVALID, "fewer than two scenes can run in lockstep");
+    for (mgf_scene s : ls) {
+        const mgf_status st = scene_prepare(s);
+        if (st != MGF_OK) return st;
+    }
+    // the V-cycle's kernels per level: fused descent / ascent on the large
+    // levels only (AUTO) or on every 3-D level above the coarse cycle (FUSED)
+    const int32_t cand[2] = {MGF_MG_AUTO, MGF_MG_FUSED};
+    double best_ms = 0.0;
+    int32_t best = MGF_MG_AUTO;
+    for (int k = 0; k < 2; ++k) {
+        for (mgf_scene s : ls) s->desc.mg_kernels = cand[k];
+        std::vector<mgf_step_stats> st;
+        std::vector<mgf_status> status;
+        std::vector<std::string> err;
+        std::vector<double> step_ms(ls.size() * size_t(trial_steps));
+        const mgf_status r = batch_lockstep(ctx, ls, trial_steps, 0, st, status, err, step_ms.data());
+        if (r != MGF_OK && r != MGF_ERR_NOT_CONVERGED) return r;
+        // the fastest step after the first (which builds the plans)
+        double t = step_ms[1];
+        for (int32_t q = 2; q < trial_steps; ++q) t = std::min(t, step_ms[size_t(q)]);
+        if (ms) ms[k] = t;
+        if (k == 0 || t < best_ms) {
+            best_ms = t;
+            best = cand[k];
+        }
+    }
+    for (mgf_scene s : ls) s->desc.mg_kernels = best;
+    if (chosen) *chosen = best;
+    return MGF_OK;
+}
+
 mgf_status mgf_server_create(mgf_context ctx, int64_t hbm_budget_bytes, int32_t slots, mgf_server* out) {
     if (!ctx || !out) return fail(MGF_ERR_INVALID, "null context or output");
     *out = nullptr;

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <atomic>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -1008,48 +1009,118 @@ mgf_status vcycle_records(mgf_scene s, const float sa[3], std::vector<KRec>& out
     return MGF_OK;
 }
 
-// Everything of a step before the CG sweeps, enqueued on the scene's stream:
-// boundaries, advection with forces, markers, right-hand side, stencil codes,
-// the multigrid hierarchy, x64 = 0 and ||b||^2 into red.out2 (cg_mixed's
-// prologue, which the sweeps follow).
-mgf_status lock_pre(mgf_scene s) {
-    TRY(refresh_boundaries(s));
-    TRY(advect(s, true));
-    hipStream_t st = s->stream;
-    const int c = s->cur;
+// Allocations and host state the phase records rely on: the flat kernels'
+// block flags, the hierarchy and its levels' operator scales (mg_setup sets
+// the same scales each solve).
+mgf_status lock_prepare(mgf_scene s) {
+    if (!s->wet_flat) TRY(dalloc(s, &s->wet_flat, col_blocks(s->g)));
+    TRY(ensure_precond(s));
     float sa[3], sg[3];
     scales(s, sa, sg);
-    MGF_HIP(launch_markers(st, s->g, s->solid_mask, s->phi[c], s->markers));
-    MGF_HIP(launch_rhs(st, s->g, vel_of(s->u[c], s->v[c], s->w[c]), vel_of(s->us, s->vs, s->ws), s->markers, s->b));
-    MGF_HIP(launch_codes(st, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes));
-    if (!s->wet_flat) TRY(dalloc(s, &s->wet_flat, col_blocks(s->g)));
-    MGF_HIP(launch_flat_wet(st, s->g, s->codes, s->wet_flat));
-    TRY(ensure_precond(s));
-    TRY(mg_setup(s, sa));
-    s->wet_flat_cells = s->wet0_cells = -1.0;
-    s->x64_valid = false;
-    MGF_HIP(launch_refine_true(st, s->g, sa, s->codes, nullptr, s->x64, s->b, nullptr, 0, s->red, s->red.out2));
+    MgDims f{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    for (auto& L : s->mg) {
+        L.d.sx = f.nx > 1 ? f.sx * 0.25f : f.sx;
+        L.d.sy = f.ny > 1 ? f.sy * 0.25f : f.sy;
+        L.d.sz = f.nz > 1 ? f.sz * 0.25f : f.sz;
+        f = L.d;
+    }
     return MGF_OK;
 }
 
-// The rest of the step after the solve: projection with the FP64 pressure,
-// the divergence check, extrapolation (pressure() and step_once's tail).
-mgf_status lock_post(mgf_scene s, mgf_step_stats* acc) {
-    hipStream_t st = s->stream;
-    const int c = s->cur;
+// The launches of a step before the CG sweeps for a scene whose current
+// buffers are `c` (advection with forces, markers, right-hand side, stencil
+// codes, the hierarchy's markers and codes, x64 = 0 and ||b||^2: advect,
+// pressure() and cg_mixed's prologue), and after them (projection with the
+// FP64 pressure, the extrapolation's distance transform and sort, its layer
+// offsets and largest layer copied to `offs_out`), as keyed records; the
+// layers themselves are launched from those offsets.
+mgf_status phase_records(mgf_scene s, int c, std::vector<KRec>& pre, std::vector<KRec>& post, uint32_t* offs_out) {
+    const int n = 1 - c;
+    const float dt = s->desc.dt;
     float sa[3], sg[3];
     scales(s, sa, sg);
-    s->x_is_64 = true;
-    MGF_HIP(launch_project(st, s->g, sg, s->markers, nullptr, s->x64, s->u[c], s->v[c], s->w[c], s->us, s->vs,
-                           s->ws));
-    if (s->div_check)
-        MGF_HIP(launch_div_check(st, s->g, s->markers, s->u[c], s->v[c], s->w[c], s->b, s->red.partials, kDivBlocks,
-                                 s->div_dev));
-    if (!std::isfinite(acc->residual_rel)) return fail(MGF_ERR_NONFINITE, "pressure solve produced NaN/Inf");
-    int used = 0;
-    TRY(extrapolate(s, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->desc.extrapolation_layers, &used));
-    acc->extrapolation_layers = used;
-    acc->steps += 1;
+    float g[3], f[3];
+    for (int k = 0; k < 3; ++k) {
+        g[k] = s->desc.gravity[k] * dt;
+        f[k] = s->desc.external_force[k] * dt;
+    }
+    auto on = [](const float* a) {
+        return !(std::fabs(a[0]) < MGF_FLT_EPSILON && std::fabs(a[1]) < MGF_FLT_EPSILON &&
+                 std::fabs(a[2]) < MGF_FLT_EPSILON);
+    };
+    std::vector<MRec> r;
+    const Vel V = vel_of(s->u[c], s->v[c], s->w[c]);
+    rec_advect_fc(r, s->g, V, V, dt, s->desc.rk_order, s->u[n], s->v[n], s->w[n], on(g), g, on(f), f);
+    keyed(pre, r, 0, 0, 0);
+    r.clear();
+    if (s->color[0][0]) {
+        const float* ic[3] = {s->color[c][0], s->color[c][1], s->color[c][2]};
+        float* oc[3] = {s->color[n][0], s->color[n][1], s->color[n][2]};
+        rec_advect_cc4(r, s->g, s->phi[c], ic, V, dt, s->desc.rk_order, s->phi[n], oc);
+    } else {
+        rec_advect_cc(r, s->g, s->phi[c], V, dt, s->desc.rk_order, s->phi[n]);
+    }
+    keyed(pre, r, 0, 1, 0);
+    r.clear();
+    rec_markers(r, s->g, s->solid_mask, s->phi[n], s->markers);
+    keyed(pre, r, 0, 2, 0);
+    r.clear();
+    rec_rhs(r, s->g, vel_of(s->u[n], s->v[n], s->w[n]), vel_of(s->us, s->vs, s->ws), s->markers, s->b);
+    keyed(pre, r, 0, 3, 0);
+    r.clear();
+    rec_codes(r, s->g.nx, s->g.ny, s->g.nz, s->markers, s->codes);
+    keyed(pre, r, 0, 4, 0);
+    r.clear();
+    rec_flat_wet(r, s->g, s->codes, s->wet_flat);
+    keyed(pre, r, 0, 5, 0);
+    // mg_setup
+    MgDims fd{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    const uint8_t* mf = s->markers;
+    for (size_t l = 0; l < s->mg.size(); ++l) {
+        auto& L = s->mg[l];
+        r.clear();
+        rec_mg_coarsen(r, fd, L.d, mf, L.markers, 0);
+        rec_codes(r, L.d.nx, L.d.ny, L.d.nz, L.markers, L.codes);
+        keyed(pre, r, 1, int(l), 0);
+        fd = L.d;
+        mf = L.markers;
+    }
+    fd = MgDims{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+    const uint16_t* cf = s->codes;
+    for (size_t l = 0; l < s->mg.size(); ++l) {
+        const MgDims& cd = s->mg[l].d;
+        uint8_t* wet = l == 0 ? s->wet0 : s->mg[l - 1].wet;
+        const size_t wet_n = l == 0 ? s->wet0_n : s->mg[l - 1].wet_n;
+        if (wet && wet_n >= mg_march_grid(fd, cd)) {
+            r.clear();
+            rec_mg_march_wet(r, fd, cd, cf, wet);
+            keyed(pre, r, 2, int(l), 0);
+        }
+        fd = cd;
+        cf = s->mg[l].codes;
+    }
+    r.clear();
+    rec_refine_true(r, s->g, sa, s->codes, nullptr, s->x64, s->b, nullptr, 0, s->red, s->red.out2);
+    keyed(pre, r, 3, 0, 0);
+    // after the solve
+    r.clear();
+    rec_project(r, s->g, sg, s->markers, nullptr, s->x64, s->u[n], s->v[n], s->w[n], s->us, s->vs, s->ws);
+    keyed(post, r, 0, 0, 0);
+    const int32_t K = ext_max_distance(s->g);
+    const int32_t cap = int32_t(std::min<int64_t>(int64_t(s->desc.extrapolation_layers) - 1, K));
+    if (cap > 0) {
+        if (K >= 65535)
+            return fail(MGF_ERR_UNSUPPORTED, "extrapolation over more than one layer needs nx + ny + nz + 3 < 65535 "
+                                             "(16-bit face distances)");
+        r.clear();
+        const uint32_t* maxd = nullptr;
+        rec_ext_device(r, s->g, s->u[n], s->v[n], s->w[n], reinterpret_cast<uint16_t*>(s->layer), cap, K, s->counts,
+                       s->list, s->lmask, s->fmask, &maxd, false);
+        keyed(post, r, 1, 0, 0);
+        r.clear();
+        rec_copy_offs(r, s->counts, K, cap, offs_out);
+        keyed(post, r, 2, 0, 0);
+    }
     return MGF_OK;
 }
 
@@ -1079,7 +1150,9 @@ struct LockPlan {
     LSeq start_b, start_r, cg, vcycle, refine1, refine2, gather;
     std::vector<const float*> zv;
     std::vector<unsigned char> blob;   // host copy of the tables and maps
+    hipGraphExec_t iter = nullptr;     // one CG iteration (cg + vcycle), captured
     ~LockPlan() {
+        if (iter) (void)hipGraphExecDestroy(iter);
         if (dev) (void)hipFree(dev);
         if (host) (void)hipHostFree(host);
     }
@@ -1144,14 +1217,88 @@ LSeq merge_seq(const std::vector<std::vector<KRec>>& per, std::vector<unsigned c
     return seq;
 }
 
-mgf_status run_seq(LockPlan& P, const LSeq& seq) {
+mgf_status run_seq_at(hipStream_t st, unsigned char* base, const LSeq& seq, const uint8_t* active) {
     for (const LSlot& sl : seq)
-        MGF_HIP(launch_multi(P.stream, sl.kind, P.dev + sl.tab, reinterpret_cast<const uint32_t*>(P.dev + sl.map),
-                             sl.nblocks, sl.shmem, P.active_dev));
+        MGF_HIP(launch_multi(st, sl.kind, base + sl.tab, reinterpret_cast<const uint32_t*>(base + sl.map), sl.nblocks,
+                             sl.shmem, active));
     return MGF_OK;
 }
 
-// Builds the plan for scenes whose hierarchies exist (after lock_pre).
+mgf_status run_seq(LockPlan& P, const LSeq& seq) { return run_seq_at(P.stream, P.dev, seq, P.active_dev); }
+
+// The launches before and after the solves for one assignment of current
+// buffers (each scene's `cur`): built once per assignment and kept (a batch
+// alternates between two).
+struct PhasePlan {
+    LSeq pre, post;
+    unsigned char* dev = nullptr;   // the packed layer offsets, the layer table, then tables and maps
+    std::vector<size_t> offs_at;    // per scene: its offsets' first word in the packed area (cap + 3 words)
+    std::vector<int32_t> cap;       // per scene: the layer cap (0: no extrapolation)
+    size_t offs_words = 0;
+    size_t layer_tab = 0;           // byte offset of the per-scene k_multi_layer arguments
+    uint32_t* offs_host = nullptr;  // pinned copy of the packed offsets
+    PhasePlan() = default;
+    PhasePlan(const PhasePlan&) = delete;
+    PhasePlan& operator=(const PhasePlan&) = delete;
+    ~PhasePlan() {
+        if (dev) (void)hipFree(dev);
+        if (offs_host) (void)hipHostFree(offs_host);
+    }
+};
+
+mgf_status build_phase(const std::vector<mgf_scene>& sc, const std::vector<uint8_t>& cur, hipStream_t st,
+                       PhasePlan& Q) {
+    const size_t n = sc.size();
+    Q.offs_at.assign(n, 0);
+    Q.cap.assign(n, 0);
+    Q.offs_words = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t K = ext_max_distance(sc[i]->g);
+        Q.cap[i] = int32_t(std::max<int64_t>(0, std::min<int64_t>(int64_t(sc[i]->desc.extrapolation_layers) - 1, K)));
+        Q.offs_at[i] = Q.offs_words;
+        if (Q.cap[i] > 0) Q.offs_words += size_t(Q.cap[i]) + 3;
+    }
+    Q.layer_tab = (Q.offs_words * sizeof(uint32_t) + 255) / 256 * 256;
+    const size_t head = Q.layer_tab + (n * layer_arg_bytes() + 255) / 256 * 256;
+    std::vector<std::vector<KRec>> pre(n), post(n);
+    for (int pass = 0; pass < 2; ++pass) {
+        uint32_t* packed = reinterpret_cast<uint32_t*>(pass == 0 ? nullptr : Q.dev);
+        for (size_t i = 0; i < n; ++i) {
+            pre[i].clear();
+            post[i].clear();
+            TRY(phase_records(sc[i], cur[i], pre[i], post[i], packed + Q.offs_at[i]));
+        }
+        std::vector<unsigned char> blob(head, 0);
+        for (size_t i = 0; i < n; ++i) {
+            mgf_scene s = sc[i];
+            const int c1 = 1 - cur[i];
+            layer_args(blob.data() + Q.layer_tab + i * layer_arg_bytes(), s->g, s->u[c1], s->v[c1], s->w[c1], s->list,
+                       s->lmask, s->counts, ext_max_distance(s->g));
+        }
+        Q.pre = merge_seq(pre, blob);
+        Q.post = merge_seq(post, blob);
+        if (Q.pre.empty() || Q.post.empty()) return fail(MGF_ERR_INVALID, "lockstep phases: a scene has two records under one key");
+        if (pass == 0) {
+            if (hipMalloc(reinterpret_cast<void**>(&Q.dev), blob.size()) != hipSuccess) {
+                (void)hipGetLastError();
+                Q.dev = nullptr;
+                return fail(MGF_ERR_NOMEM, "lockstep phases: hipMalloc failed");
+            }
+            if (hipHostMalloc(reinterpret_cast<void**>(&Q.offs_host), std::max<size_t>(Q.offs_words, 1) * 4) !=
+                hipSuccess) {
+                (void)hipGetLastError();
+                Q.offs_host = nullptr;
+                return fail(MGF_ERR_NOMEM, "lockstep phases: hipHostMalloc failed");
+            }
+            continue;
+        }
+        MGF_HIP(hipMemcpyAsync(Q.dev, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+        MGF_HIP(hipStreamSynchronize(st));
+    }
+    return MGF_OK;
+}
+
+// Builds the plan for scenes whose hierarchies exist (after lock_prepare).
 mgf_status build_plan(LockPlan& P) {
     const size_t n = P.sc.size();
     if (n >= (size_t(1) << (32 - kMapShift))) return fail(MGF_ERR_INVALID, "too many scenes for one lockstep batch");
@@ -1240,6 +1387,33 @@ mgf_status build_plan(LockPlan& P) {
     return MGF_OK;
 }
 
+// Captures one CG iteration of every scene (K_I, K_II with their finishing
+// kernels, the V-cycle) into a graph: its launches and tables do not change
+// within a batch call, so a chunk of k iterations is k launches of the graph.
+mgf_status capture_iteration(LockPlan& P) {
+    hipGraph_t g = nullptr;
+    MGF_HIP(hipStreamBeginCapture(P.stream, hipStreamCaptureModeRelaxed));
+    mgf_status st = run_seq(P, P.cg);
+    if (st == MGF_OK) st = run_seq(P, P.vcycle);
+    const hipError_t e = hipStreamEndCapture(P.stream, &g);
+    if (st != MGF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MGF_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&P.iter, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        (void)hipGetLastError();
+        P.iter = nullptr;
+        return fail(MGF_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    return MGF_OK;
+}
+
 // Gathers every scene's scalars and refinement sums to the host.
 mgf_status gather(LockPlan& P) {
     const size_t n = P.sc.size();
@@ -1280,7 +1454,7 @@ mgf_status set_start_targets(LockPlan& P, const std::vector<double>& target2) {
     return MGF_OK;
 }
 
-// The sweeps of cg_mixed for every scene at once (after lock_pre on each
+// The sweeps of cg_mixed for every scene at once (after the pre-solve phase
 // scene's stream and a wait for them on the plan's stream).
 mgf_status lock_solve(LockPlan& P, std::vector<mgf_step_stats>& acc) {
     const size_t n = P.sc.size();
@@ -1366,10 +1540,7 @@ mgf_status lock_solve(LockPlan& P, std::vector<mgf_step_stats>& acc) {
             }
             if (k == 0) break;
             k = int(std::min<int64_t>(k, left));
-            for (int q = 0; q < k; ++q) {
-                TRY(run_seq(P, P.cg));
-                TRY(run_seq(P, P.vcycle));
-            }
+            for (int q = 0; q < k; ++q) MGF_HIP(hipGraphLaunch(P.iter, P.stream));
             TRY(gather(P));
             TRY(set_active(P, act));
             for (size_t i = 0; i < n; ++i) {
@@ -1524,7 +1695,7 @@ mgf_status scene_bytes_for(mgf_context ctx, const mgf_scene_desc* desc, int64_t*
 }
 
 mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v, float* w, int layers,
-                           const ExtBufs& B, int* used) {
+                           const ExtBufs& B, int* used, bool device_only) {
     *used = 0;
     const int32_t K = ext_max_distance(g);
     const int32_t cap = int32_t(std::min<int64_t>(int64_t(layers) - 1, K));
@@ -1537,6 +1708,13 @@ mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v,
     const uint32_t* offs = nullptr;
     const uint32_t* maxd = nullptr;
     MGF_HIP(launch_ext_sort(st, g, dist, cap, K, B.counts, B.list, B.lmask, B.fmask, &offs, &maxd));
+    if (device_only) {
+        probe_next_bytes(0.0);
+        MGF_HIP(launch_ext_tail(st, g, u, v, w, B.list, B.lmask, offs, 1, 0, maxd));
+        MGF_HIP(hipMemcpyAsync(B.host_offs, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        *used = -1;
+        return MGF_OK;
+    }
     // offsets of layers 0..cap+1, then the largest distance present
     MGF_HIP(hipMemcpyAsync(B.host_offs, offs, sizeof(uint32_t) * (size_t(cap) + 2), hipMemcpyDeviceToHost, st));
     MGF_HIP(hipMemcpyAsync(B.host_offs + cap + 2, maxd, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -1547,10 +1725,16 @@ mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v,
     // Large layers are bound by their gathers, which the pairing would
     // multiply, so they run one per launch.
     constexpr uint32_t kPairMax = 65536;   // faces of two layers launched together
-    for (int32_t L = 1; L <= top;) {
+    // From the first layer after which no layer exceeds kTailMax entries on,
+    // the layers run in one work-group (k_ext_tail): a launch per layer
+    // costs ~8 us, a small layer ~2-4 us there.
+    constexpr uint32_t kTailMax = 1024;
+    int32_t tail = top + 1;
+    while (tail > 1 && B.host_offs[tail] - B.host_offs[tail - 1] <= kTailMax) --tail;
+    for (int32_t L = 1; L < tail;) {
         const uint32_t b = B.host_offs[L], m = B.host_offs[L + 1];
-        const uint32_t e = L + 1 <= top ? B.host_offs[L + 2] : m;
-        const bool pair = L + 1 <= top && e - b <= kPairMax;
+        const uint32_t e = L + 1 < tail ? B.host_offs[L + 2] : m;
+        const bool pair = L + 1 < tail && e - b <= kPairMax;
         // per entry: list 4 + mask 1 + write 4 + the values of the previous-layer neighbours (one on average)
         probe_next_bytes(13.0 * double((pair ? e : m) - b));
         if (pair)
@@ -1559,32 +1743,85 @@ mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v,
             MGF_HIP(launch_ext_layer(st, g, u, v, w, B.list, B.lmask, b, m - b));
         L += pair ? 2 : 1;
     }
+    if (tail <= top) {
+        probe_next_bytes(13.0 * double(B.host_offs[top + 1] - B.host_offs[tail]));
+        MGF_HIP(launch_ext_tail(st, g, u, v, w, B.list, B.lmask, offs, tail, top));
+    }
     *used = top;
     return MGF_OK;
 }
 
 bool lockstep_eligible(mgf_scene s) { return lock_eligible(s); }
 
-// Runs fn(i) for i in [0, n) on up to W host threads (the caller included).
-template <class F>
-void parallel_for(int n, int W, int device, F fn) {
-    std::atomic<int> next{0};
-    auto body = [&] {
-        (void)hipSetDevice(device);
-        for (int i; (i = next.fetch_add(1)) < n;) fn(i);
-    };
-    std::vector<std::thread> pool;
-    try {
-        for (int w = 1; w < std::min(W, n); ++w) pool.emplace_back(body);
-    } catch (...) {
+// The extrapolation layers of every scene from its offsets (in Q.offs_host):
+// for each layer index, one k_multi_layer launch over the scenes whose layer
+// is larger than kBatchTailMax entries, then one launch of every scene's
+// remaining layers (from the first layer after which none is large) in one
+// work-group each.  A scene's layers run in order: its large layers are a
+// prefix, and launches on one stream run in order.
+mgf_status launch_layers(hipStream_t st, const std::vector<mgf_scene>& sc, const PhasePlan& Q,
+                         std::vector<unsigned char>& host, unsigned char*& dev, size_t& dev_cap) {
+    constexpr uint32_t kBatchTailMax = 4096;
+    const size_t n = sc.size();
+    const uint32_t epb = layer_entries_per_block();
+    const size_t tb = tail_arg_bytes();
+    std::vector<std::vector<uint32_t>> maps;
+    std::vector<uint32_t> tail_map;
+    host.assign((n * tb + 255) / 256 * 256, 0);
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t cap = Q.cap[i];
+        if (cap <= 0) continue;
+        const uint32_t* o = Q.offs_host + Q.offs_at[i];
+        const int32_t top = int32_t(o[cap + 2]);
+        int32_t first = top + 1;
+        while (first > 1 && o[first] - o[first - 1] <= kBatchTailMax) --first;
+        if (maps.size() < size_t(first)) maps.resize(size_t(first));
+        for (int32_t L = 1; L < first; ++L) {
+            const uint32_t cnt = o[L + 1] - o[L];
+            for (uint32_t b = 0; b * epb < cnt; ++b) maps[size_t(L)].push_back((uint32_t(i) << kMapShift) | b);
+        }
+        if (first <= top) {
+            mgf_scene s = sc[i];
+            tail_args(host.data() + i * tb, s->g, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->list, s->lmask,
+                      s->counts, ext_max_distance(s->g), first, top);
+            tail_map.push_back(uint32_t(i) << kMapShift);
+        }
     }
-    body();
-    for (auto& t : pool) t.join();
+    std::vector<size_t> map_at(maps.size(), 0);
+    for (size_t L = 0; L < maps.size(); ++L) {
+        map_at[L] = host.size();
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(maps[L].data());
+        host.insert(host.end(), p, p + 4 * maps[L].size());
+    }
+    const size_t tail_map_at = host.size();
+    const unsigned char* tp = reinterpret_cast<const unsigned char*>(tail_map.data());
+    host.insert(host.end(), tp, tp + 4 * tail_map.size());
+    if (host.size() > dev_cap) {
+        MGF_HIP(hipStreamSynchronize(st));
+        if (dev) (void)hipFree(dev);
+        dev = nullptr;
+        dev_cap = 0;
+        const size_t want = host.size() + host.size() / 2 + 4096;
+        if (hipMalloc(reinterpret_cast<void**>(&dev), want) != hipSuccess) {
+            (void)hipGetLastError();
+            dev = nullptr;
+            return fail(MGF_ERR_NOMEM, "lockstep layers: hipMalloc failed");
+        }
+        dev_cap = want;
+    }
+    MGF_HIP(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, st));
+    for (size_t L = 1; L < maps.size(); ++L)
+        MGF_HIP(launch_multi_layer(st, Q.dev + Q.layer_tab, reinterpret_cast<const uint32_t*>(dev + map_at[L]),
+                                   uint32_t(maps[L].size()), int32_t(L)));
+    MGF_HIP(launch_multi(st, kMkExtTail, dev, reinterpret_cast<const uint32_t*>(dev + tail_map_at),
+                         uint32_t(tail_map.size()), 0, nullptr));
+    return MGF_OK;
 }
 
 mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int32_t nsteps, int32_t workers,
                           std::vector<mgf_step_stats>& stats, std::vector<mgf_status>& status,
                           std::vector<std::string>& errs, double* step_ms) {
+    (void)workers;
     const int n = int(sc.size());
     stats.assign(size_t(n), mgf_step_stats{});
     status.assign(size_t(n), MGF_OK);
@@ -1597,86 +1834,153 @@ mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int
         (void)hipGetLastError();
         return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
     }
-    std::vector<hipEvent_t> ev(size_t(n) + 1, nullptr);
-    mgf_status result = MGF_OK;
+    hipEvent_t ev = nullptr;
+    std::vector<std::pair<std::vector<uint8_t>, std::unique_ptr<PhasePlan>>> phases;
+    std::vector<unsigned char> step_host;   // this step's layer maps and tail table
+    unsigned char* step_dev = nullptr;
+    size_t step_cap = 0;
     auto cleanup = [&]() {
-        for (hipEvent_t e : ev)
-            if (e) (void)hipEventDestroy(e);
         (void)hipStreamSynchronize(P.stream);
+        if (ev) (void)hipEventDestroy(ev);
+        if (step_dev) (void)hipFree(step_dev);
+        phases.clear();
         (void)hipStreamDestroy(P.stream);
         P.stream = nullptr;
     };
-    for (auto& e : ev)
-        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+    auto fin = [&](mgf_status st) {
+        cleanup();
+        return st;
+    };
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return fin(fail(MGF_ERR_DEVICE, "lockstep batch: hipEventCreate failed"));
+    }
+    // with timing on (any scene), the batch step's phases are timed: before
+    // the solve, the lockstep solve, after the solve
+    bool timing = false;
+    for (mgf_scene s : sc) timing = timing || s->timing;
+    for (int i = 0; i < n; ++i) {
+        stats_reset(&stats[size_t(i)]);
+        mgf_scene s = sc[size_t(i)];
+        mgf_status st = lock_prepare(s);
+        if (st == MGF_OK) st = div_begin(s);
+        // what the caller queued on the scene's stream comes first
+        if (st == MGF_OK && (hipEventRecord(ev, s->stream) != hipSuccess ||
+                             hipStreamWaitEvent(P.stream, ev, 0) != hipSuccess)) {
             (void)hipGetLastError();
-            cleanup();
-            return fail(MGF_ERR_DEVICE, "hipEventCreate failed");
+            st = fail(MGF_ERR_DEVICE, "hipEventRecord failed");
         }
-    const int W = workers <= 0 ? 8 : std::min(workers, 64);
-    auto note = [&](int i, mgf_status st) {
+        if (st != MGF_OK) return fin(st);
+    }
+    mgf_status result = build_plan(P);
+    if (result == MGF_OK) result = capture_iteration(P);
+    if (result != MGF_OK) return fin(result);
+    std::vector<uint8_t> cur(static_cast<size_t>(n), 0);
+    for (int32_t step = 0; step < nsteps && result == MGF_OK; ++step) {
+        const double t0 = now_ms();
+        // boundaries that changed (added or moving solids) on the scenes'
+        // streams, then everything else on the batch stream
+        for (int i = 0; i < n; ++i) {
+            mgf_scene s = sc[size_t(i)];
+            bool moving = false;
+            for (auto& so : s->solids)
+                if (so.vel[0] != 0.0f || so.vel[1] != 0.0f || so.vel[2] != 0.0f) moving = true;
+            if (!s->boundary_dirty && !moving) {
+                result = refresh_boundaries(s);   // static solids: the translations only
+                if (result != MGF_OK) return fin(result);
+                continue;
+            }
+            // after the batch stream's work so far, on the scene's stream
+            if (hipEventRecord(ev, P.stream) != hipSuccess || hipStreamWaitEvent(s->stream, ev, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                return fin(fail(MGF_ERR_DEVICE, "hipEventRecord failed"));
+            }
+            result = refresh_boundaries(s);
+            if (result != MGF_OK) return fin(result);
+            if (hipEventRecord(ev, s->stream) != hipSuccess || hipStreamWaitEvent(P.stream, ev, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                return fin(fail(MGF_ERR_DEVICE, "hipEventRecord failed"));
+            }
+        }
+        for (int i = 0; i < n; ++i) cur[size_t(i)] = uint8_t(sc[size_t(i)]->cur);
+        PhasePlan* Q = nullptr;
+        for (auto& q : phases)
+            if (q.first == cur) Q = q.second.get();
+        if (!Q) {
+            phases.emplace_back(cur, std::make_unique<PhasePlan>());
+            Q = phases.back().second.get();
+            result = build_phase(sc, cur, P.stream, *Q);
+            if (result != MGF_OK) return fin(result);
+        }
+        result = run_seq_at(P.stream, Q->dev, Q->pre, nullptr);
+        if (result != MGF_OK) return fin(result);
+        for (int i = 0; i < n; ++i) {
+            mgf_scene s = sc[size_t(i)];
+            s->cur = 1 - s->cur;
+            s->wet_flat_cells = s->wet0_cells = -1.0;
+            s->x64_valid = false;
+        }
+        if (timing) MGF_HIP(hipStreamSynchronize(P.stream));
+        const double t1 = now_ms();
+        result = lock_solve(P, stats);
+        if (result != MGF_OK) return fin(result);
+        const double t2 = now_ms();
+        result = run_seq_at(P.stream, Q->dev, Q->post, nullptr);
+        if (result != MGF_OK) return fin(result);
+        for (int i = 0; i < n; ++i) {
+            mgf_scene s = sc[size_t(i)];
+            if (s->div_check)
+                MGF_HIP(launch_div_check(P.stream, s->g, s->markers, s->u[s->cur], s->v[s->cur], s->w[s->cur], s->b,
+                                         s->red.partials, kDivBlocks, s->div_dev));
+        }
+        // the layers: each scene's offsets to the host, then per layer index
+        // one launch over every scene whose layer is large, then each
+        // scene's tail of small layers in one work-group (all scenes in one
+        // launch)
+        if (Q->offs_words > 0) {
+            MGF_HIP(hipMemcpyAsync(Q->offs_host, Q->dev, sizeof(uint32_t) * Q->offs_words, hipMemcpyDeviceToHost,
+                                   P.stream));
+            MGF_HIP(hipStreamSynchronize(P.stream));
+            result = launch_layers(P.stream, sc, *Q, step_host, step_dev, step_cap);
+            if (result != MGF_OK) return fin(result);
+        }
+        MGF_HIP(hipStreamSynchronize(P.stream));
+        for (int i = 0; i < n; ++i) {
+            mgf_step_stats& a = stats[size_t(i)];
+            a.extrapolation_layers = Q->cap[size_t(i)] > 0 ? int32_t(Q->offs_host[Q->offs_at[size_t(i)] + size_t(Q->cap[size_t(i)]) + 2]) : 0;
+            a.steps += 1;
+            if (!std::isfinite(a.residual_rel) && status[size_t(i)] == MGF_OK) {
+                status[size_t(i)] = MGF_ERR_NONFINITE;
+                errs[size_t(i)] = "pressure solve produced NaN/Inf";
+            }
+        }
+        const double dt_ms = now_ms() - t0;
+        if (timing)
+            for (int i = 0; i < n; ++i) {
+                mgf_step_stats& a = stats[size_t(i)];
+                a.ms_advect += t1 - t0;
+                a.ms_pressure += t2 - t1;
+                a.ms_extrapolate += dt_ms - (t2 - t0);
+                a.ms_total += dt_ms;
+            }
+        if (step_ms)
+            for (int i = 0; i < n; ++i) step_ms[size_t(i) * size_t(nsteps) + size_t(step)] = dt_ms;
+        for (int i = 0; i < n; ++i)
+            if (status[size_t(i)] != MGF_OK) result = status[size_t(i)];
+    }
+    for (int i = 0; i < n; ++i) {
+        mgf_scene s = sc[size_t(i)];
+        mgf_status st = div_end(s, &stats[size_t(i)]);
         if (st != MGF_OK && status[size_t(i)] == MGF_OK) {
             status[size_t(i)] = st;
             errs[size_t(i)] = mgf_last_error();
         }
-    };
-    for (int i = 0; i < n; ++i) {
-        stats_reset(&stats[size_t(i)]);
-        note(i, div_begin(sc[size_t(i)]));
-    }
-    for (int32_t step = 0; step < nsteps && result == MGF_OK; ++step) {
-        const double t0 = now_ms();
-        // before the solve, on the scenes' streams
-        parallel_for(n, W, ctx->device, [&](int i) {
-            mgf_scene s = sc[size_t(i)];
-            mgf_status st = lock_pre(s);
-            if (st == MGF_OK && hipEventRecord(ev[size_t(i)], s->stream) != hipSuccess) {
-                (void)hipGetLastError();
-                st = fail(MGF_ERR_DEVICE, "hipEventRecord failed");
-            }
-            note(i, st);
-        });
-        for (int i = 0; i < n; ++i)
-            if (status[size_t(i)] != MGF_OK) result = status[size_t(i)];
-        if (result != MGF_OK) break;
-        if (step == 0) {
-            // the hierarchies exist now: the plan's records hold their buffers
-            for (int i = 0; i < n; ++i) MGF_HIP(hipStreamSynchronize(sc[size_t(i)]->stream));
-            result = build_plan(P);
-            if (result != MGF_OK) break;
-        }
-        for (int i = 0; i < n; ++i) MGF_HIP(hipStreamWaitEvent(P.stream, ev[size_t(i)], 0));
-        result = lock_solve(P, stats);
-        if (result != MGF_OK) break;
-        MGF_HIP(hipEventRecord(ev[size_t(n)], P.stream));
-        // after the solve, on the scenes' streams
-        parallel_for(n, W, ctx->device, [&](int i) {
-            mgf_scene s = sc[size_t(i)];
-            mgf_status st = MGF_OK;
-            if (hipStreamWaitEvent(s->stream, ev[size_t(n)], 0) != hipSuccess) {
-                (void)hipGetLastError();
-                st = fail(MGF_ERR_DEVICE, "hipStreamWaitEvent failed");
-            }
-            if (st == MGF_OK) st = lock_post(s, &stats[size_t(i)]);
-            if (st == MGF_OK) st = sync(s);
-            note(i, st);
-        });
-        for (int i = 0; i < n; ++i)
-            if (status[size_t(i)] != MGF_OK) result = status[size_t(i)];
-        const double dt_ms = now_ms() - t0;
-        if (step_ms)
-            for (int i = 0; i < n; ++i) step_ms[size_t(i) * size_t(nsteps) + size_t(step)] = dt_ms;
-    }
-    for (int i = 0; i < n; ++i) {
-        mgf_scene s = sc[size_t(i)];
-        (void)hipStreamSynchronize(s->stream);
-        note(i, div_end(s, &stats[size_t(i)]));
         if (status[size_t(i)] == MGF_OK && !stats[size_t(i)].converged) {
             status[size_t(i)] = MGF_ERR_NOT_CONVERGED;
             errs[size_t(i)] = "a pressure solve stopped before its tolerance";
         }
     }
-    cleanup();
-    return result;
+    return fin(result);
 }
 
 mgf_status scene_prepare(mgf_scene s) {

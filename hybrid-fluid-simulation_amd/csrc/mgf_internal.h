@@ -239,9 +239,12 @@ struct ExtBufs {
     uint32_t* host_offs;
 };
 // extrapolatingDataInvoker(FCV) on (u, v, w) in place on stream st; *used
-// receives the largest non-empty layer (< layers).
+// receives the largest non-empty layer (< layers).  device_only: no host
+// synchronisation (every layer in one work-group, the largest layer copied
+// to host_offs[0] when the stream gets there; *used = -1 when kernels were
+// enqueued).
 mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v, float* w, int layers,
-                           const ExtBufs& B, int* used);
+                           const ExtBufs& B, int* used, bool device_only = false);
 #define MGF_HIP(expr)                                                           \
     do {                                                                        \
         hipError_t e_ = (expr);                                                 \
@@ -383,6 +386,10 @@ enum MultiKind : int32_t {
     kMkStart, kMkFinish, kMkPap, kMkUpdate, kMkRefine1, kMkRefine2, kMkAcc64, kMkSum2,
     kMkSmoothTT, kMkSmoothTF, kMkSmoothFT, kMkSmoothFF, kMkSmooth2T, kMkSmooth2F, kMkProlongT, kMkProlongF,
     kMkRestrict, kMkRestrict2, kMkRestrictTile, kMkCoarse, kMkMarchDesc, kMkMarchAsc, kMkMarchAscRz, kMkGather,
+    kMkRefine0, kMkCodes, kMkFlatWet, kMkCoarsen, kMkMarchWet,
+    kMkAdvectFc, kMkAdvectCc, kMkAdvectCc4, kMkMarkers, kMkRhs, kMkProject, kMkCopyWord,
+    kMkExtRows, kMkExtRowsReg4, kMkExtRowsReg8, kMkExtRowsReg16, kMkExtLines1, kMkExtLines2, kMkExtZero, kMkExtCount,
+    kMkExtScan, kMkExtScatter, kMkExtTail, kMkCopyOffs,
     kMkCount
 };
 constexpr size_t kMRecBytes = 512;
@@ -424,6 +431,46 @@ void rec_mg_ascend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, con
 size_t finish_arg_bytes();
 void finish_set_target(unsigned char* args, double abs_target2);
 void rec_gather(std::vector<MRec>& out, const CgScalars* sc, const double* out2, CgScalars* dst_sc, double* dst2);
+void rec_codes(std::vector<MRec>& out, int nx, int ny, int nz, const uint8_t* markers, uint16_t* codes);
+void rec_flat_wet(std::vector<MRec>& out, const GridDev& g, const uint16_t* codes, uint8_t* wet);
+void rec_mg_coarsen(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint8_t* mf, uint8_t* mc,
+                    int fluid_first);
+void rec_mg_march_wet(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet);
+void rec_advect_fc(std::vector<MRec>& out, const GridDev& g, Vel in, Vel V, float dt, int order, float* ou, float* ov,
+                   float* ow, int add0, const float inc0[3], int add1, const float inc1[3]);
+void rec_advect_cc(std::vector<MRec>& out, const GridDev& g, const float* in, Vel V, float dt, int order, float* o);
+void rec_advect_cc4(std::vector<MRec>& out, const GridDev& g, const float* phi, const float* const c[3], Vel V,
+                    float dt, int order, float* ophi, float* const oc[3]);
+void rec_markers(std::vector<MRec>& out, const GridDev& g, const uint8_t* solid_mask, const float* fluid,
+                 uint8_t* markers);
+void rec_rhs(std::vector<MRec>& out, const GridDev& g, Vel U, Vel S, const uint8_t* markers, float* b);
+void rec_project(std::vector<MRec>& out, const GridDev& g, const float sg[3], const uint8_t* markers, const float* p32,
+                 const double* p64, float* u, float* v, float* w, const float* us, const float* vs, const float* ws);
+void rec_copy_word(std::vector<MRec>& out, const uint32_t* src, uint32_t* dst);
+// The extrapolation with no host synchronisation (distance transform, sort,
+// every layer in one work-group; *maxd = the device address of the largest
+// layer); K = ext_max_distance(g) < 65535, 1 <= cap <= K.
+void rec_ext_device(std::vector<MRec>& out, const GridDev& g, float* u, float* v, float* w, uint16_t* dist,
+                    int32_t cap, int32_t K, uint32_t* counts, uint32_t* list, uint8_t* lmask, uint8_t* fmask,
+                    const uint32_t** maxd, bool with_layers = true);
+// The sort's offsets 0..cap+1 and largest layer packed into dst (cap + 3 words).
+void rec_copy_offs(std::vector<MRec>& out, uint32_t* counts, int32_t K, int32_t cap, uint32_t* dst);
+// Layer L of many scenes in one launch (k_multi_layer): a table of
+// layer_arg_bytes() per scene (layer_args), map entries (scene, block) with
+// layer_entries_per_block() entries a block.
+uint32_t layer_entries_per_block();
+size_t layer_arg_bytes();
+void layer_args(unsigned char* dst, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                const uint8_t* lmask, uint32_t* counts, int32_t K);
+hipError_t launch_multi_layer(hipStream_t s, const void* tab, const uint32_t* map, uint32_t nblocks, int32_t L);
+// The one-work-group layers first..last as a kMkExtTail record's arguments.
+size_t tail_arg_bytes();
+void tail_args(unsigned char* dst, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+               const uint8_t* lmask, uint32_t* counts, int32_t K, int32_t first, int32_t last);
+hipError_t launch_multi_k(hipStream_t s, int32_t kind, const void* tab, const uint32_t* map, uint32_t nblocks,
+                          uint32_t shmem, const uint8_t* active);
+hipError_t launch_multi_ext(hipStream_t s, int32_t kind, const void* tab, const uint32_t* map, uint32_t nblocks,
+                            uint32_t shmem, const uint8_t* active);
 // One multi-scene launch: tab = the participating scenes' argument structs
 // (one kind), map = per block (scene << 20) | block of the scene's launch;
 // scenes whose active byte is 0 return at once (active may be null).
@@ -450,6 +497,11 @@ hipError_t launch_ext_sort(hipStream_t s, const GridDev& g, const uint16_t* dist
 // the previous layer (lmask).
 hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
                             const uint8_t* lmask, uint32_t begin, uint32_t count);
+// Layers first..last in one work-group (offs: the device offsets of the
+// sort; with last_dev, last is read from there on the device).
+hipError_t launch_ext_tail(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                           const uint8_t* lmask, const uint32_t* offs, int32_t first, int32_t last,
+                           const uint32_t* last_dev = nullptr);
 // Layers L and L+1 (n0 then n1 list entries from `begin`) in one launch;
 // fmask = every face's lower-neighbour mask.
 hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,

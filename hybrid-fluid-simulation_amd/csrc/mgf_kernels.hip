@@ -6,6 +6,9 @@
 // are 256 work-items and block IDs are remapped so that each of the 8 XCDs
 // walks one contiguous slab of the grid (neighbouring planes then meet in the
 // same L2).
+#include <cstring>
+#include <vector>
+
 #include "mgf_internal.h"
 #include "mgf_kcommon.h"
 
@@ -41,11 +44,11 @@ struct Inc {
     int add_a, add_b;
 };
 
-__global__ void __launch_bounds__(kBlock) k_advect_fc(GridDev g, Vel in, Vel V, RkCoef k, int order,
+__device__ __forceinline__ void k_advect_fc_run(GridDev g, Vel in, Vel V, RkCoef k, int order,
                                                       float* __restrict__ ou, float* __restrict__ ov,
                                                       float* __restrict__ ow, uint32_t nfx,
-                                                      uint32_t nfy, uint32_t nfz, Inc inc) {
-    uint32_t i = xcd_block() * kBlock + threadIdx.x;
+                                                      uint32_t nfy, uint32_t nfz, Inc inc, uint32_t blk) {
+    uint32_t i = blk * kBlock + threadIdx.x;
     int c;
     if (i < nfx) c = 0;
     else if (i < nfx + nfy) { c = 1; i -= nfx; }
@@ -65,11 +68,36 @@ __global__ void __launch_bounds__(kBlock) k_advect_fc(GridDev g, Vel in, Vel V, 
     dst[i] = val;
 }
 
+__global__ void __launch_bounds__(kBlock) k_advect_fc(GridDev g, Vel in, Vel V, RkCoef k, int order,
+                                                      float* __restrict__ ou, float* __restrict__ ov,
+                                                      float* __restrict__ ow, uint32_t nfx,
+                                                      uint32_t nfy, uint32_t nfz, Inc inc) {
+    k_advect_fc_run(g, in, V, k, order, ou, ov, ow, nfx, nfy, nfz, inc, xcd_block());
+}
+
+struct MAdvectFc {
+    static constexpr int kThreads = kBlock;
+    GridDev g;
+    Vel in;
+    Vel V;
+    RkCoef k;
+    int order;
+    float* ou;
+    float* ov;
+    float* ow;
+    uint32_t nfx;
+    uint32_t nfy;
+    uint32_t nfz;
+    Inc inc;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_advect_fc_run(g, in, V, k, order, ou, ov, ow, nfx, nfy, nfz, inc, xcd_of(vb, G)); }
+};
+
 // CSemiLagrangian::advect(CCS) (MPhysics/SemiLagrangian.cpp:135-150).
-__global__ void __launch_bounds__(kBlock) k_advect_cc(GridDev g, const float* __restrict__ in, Vel V,
+__device__ __forceinline__ void k_advect_cc_run(GridDev g, const float* __restrict__ in, Vel V,
                                                       RkCoef k, int order, float* __restrict__ out,
-                                                      uint32_t n) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+                                                      uint32_t n, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
     const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
@@ -77,6 +105,25 @@ __global__ void __launch_bounds__(kBlock) k_advect_cc(GridDev g, const float* __
     backtrace(V, g, order, k, px, py, pz);
     out[i] = trilerp(in, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, px, py, pz);
 }
+
+__global__ void __launch_bounds__(kBlock) k_advect_cc(GridDev g, const float* __restrict__ in, Vel V,
+                                                      RkCoef k, int order, float* __restrict__ out,
+                                                      uint32_t n) {
+    k_advect_cc_run(g, in, V, k, order, out, n, xcd_block());
+}
+
+struct MAdvectCc {
+    static constexpr int kThreads = kBlock;
+    GridDev g;
+    const float* in;
+    Vel V;
+    RkCoef k;
+    int order;
+    float* out;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_advect_cc_run(g, in, V, k, order, out, n, xcd_of(vb, G)); }
+};
 
 // CSemiLagrangian::advect(CCS) of the fluid domain and advect(CCV) of the
 // colour (MPhysics/SemiLagrangian.cpp:135-167): the four advects of one step
@@ -87,8 +134,8 @@ struct Cc4 {
     const float* in[4];
     float* out[4];
 };
-__global__ void __launch_bounds__(kBlock) k_advect_cc4(GridDev g, Cc4 F, Vel V, RkCoef k, int order, uint32_t n) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_advect_cc4_run(GridDev g, Cc4 F, Vel V, RkCoef k, int order, uint32_t n, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
     const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
@@ -99,6 +146,22 @@ __global__ void __launch_bounds__(kBlock) k_advect_cc4(GridDev g, Cc4 F, Vel V, 
         F.out[q][i] = trilerp(F.in[q], g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, px, py, pz);
 }
 
+__global__ void __launch_bounds__(kBlock) k_advect_cc4(GridDev g, Cc4 F, Vel V, RkCoef k, int order, uint32_t n) {
+    k_advect_cc4_run(g, F, V, k, order, n, xcd_block());
+}
+
+struct MAdvectCc4 {
+    static constexpr int kThreads = kBlock;
+    GridDev g;
+    Cc4 F;
+    Vel V;
+    RkCoef k;
+    int order;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_advect_cc4_run(g, F, V, k, order, n, xcd_of(vb, G)); }
+};
+
 // plusAlphaX with a unit field: a[i] = inc*1 + a[i].
 __global__ void __launch_bounds__(kBlock) k_add_uniform(float* a, size_t n, float inc) {
     const size_t i = size_t(xcd_block()) * kBlock + threadIdx.x;
@@ -107,13 +170,29 @@ __global__ void __launch_bounds__(kBlock) k_add_uniform(float* a, size_t n, floa
 
 // ---------------------------------------------------------------------------
 // buildFluidMarkers (MPhysics/EulerSolverTool.cu:30-56).
-__global__ void __launch_bounds__(kBlock) k_markers(uint32_t n, const uint8_t* __restrict__ solid,
+__device__ __forceinline__ void k_markers_run(uint32_t n, const uint8_t* __restrict__ solid,
                                                     const float* __restrict__ fluid,
-                                                    uint8_t* __restrict__ m) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+                                                    uint8_t* __restrict__ m, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     m[i] = solid[i] ? kSolid : (fluid[i] < 0.0f ? kFluid : kAir);
 }
+
+__global__ void __launch_bounds__(kBlock) k_markers(uint32_t n, const uint8_t* __restrict__ solid,
+                                                    const float* __restrict__ fluid,
+                                                    uint8_t* __restrict__ m) {
+    k_markers_run(n, solid, fluid, m, xcd_block());
+}
+
+struct MMarkers {
+    static constexpr int kThreads = kBlock;
+    uint32_t n;
+    const uint8_t* solid;
+    const float* fluid;
+    uint8_t* m;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_markers_run(n, solid, fluid, m, xcd_of(vb, G)); }
+};
 
 struct MarkerView {
     const uint8_t* __restrict__ m;
@@ -126,10 +205,10 @@ struct MarkerView {
 // Divergence + buildPressureVectorb (MPhysics/FieldMathTool.cu:370-395,
 // MPhysics/EulerSolverTool.cu:128-197): b = -div on fluid cells, corrected by
 // (u - u_solid)/h on each face shared with a solid (clamped) neighbour.
-__global__ void __launch_bounds__(kBlock) k_rhs(GridDev g, Vel U, Vel S, const uint8_t* __restrict__ m,
+__device__ __forceinline__ void k_rhs_run(GridDev g, Vel U, Vel S, const uint8_t* __restrict__ m,
                                                 float sx, float sy, float sz, float* __restrict__ b,
-                                                uint32_t n) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+                                                uint32_t n, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
@@ -149,6 +228,27 @@ __global__ void __launch_bounds__(kBlock) k_rhs(GridDev g, Vel U, Vel S, const u
     if (m[lin3(x, y, z + 1, nx, ny, nz)] == kSolid) val += sz * (wf - S.w[ff]);
     b[i] = val;
 }
+
+__global__ void __launch_bounds__(kBlock) k_rhs(GridDev g, Vel U, Vel S, const uint8_t* __restrict__ m,
+                                                float sx, float sy, float sz, float* __restrict__ b,
+                                                uint32_t n) {
+    k_rhs_run(g, U, S, m, sx, sy, sz, b, n, xcd_block());
+}
+
+struct MRhs {
+    static constexpr int kThreads = kBlock;
+    GridDev g;
+    Vel U;
+    Vel S;
+    const uint8_t* m;
+    float sx;
+    float sy;
+    float sz;
+    float* b;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_rhs_run(g, U, S, m, sx, sy, sz, b, n, xcd_of(vb, G)); }
+};
 
 // CUDADivergence alone (download of MGF_FIELD_DIVERGENCE).
 __global__ void __launch_bounds__(kBlock) k_divergence(GridDev g, Vel U, float* __restrict__ d,
@@ -331,7 +431,7 @@ __global__ void __launch_bounds__(kBlock) k_residual32(GridDev g, const uint8_t*
 // ---------------------------------------------------------------------------
 // applyPressureGradient (MPhysics/EulerSolverTool.cu:229-322).  With an FP64
 // pressure the update is formed in FP64 and rounded once.
-__global__ void __launch_bounds__(kBlock) k_project(GridDev g, const uint8_t* __restrict__ m,
+__device__ __forceinline__ void k_project_run(GridDev g, const uint8_t* __restrict__ m,
                                                     float sx, float sy, float sz,
                                                     const float* __restrict__ p32,
                                                     const double* __restrict__ p64,
@@ -339,8 +439,8 @@ __global__ void __launch_bounds__(kBlock) k_project(GridDev g, const uint8_t* __
                                                     float* __restrict__ w,
                                                     const float* __restrict__ us,
                                                     const float* __restrict__ vs,
-                                                    const float* __restrict__ ws, uint32_t n) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+                                                    const float* __restrict__ ws, uint32_t n, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
@@ -377,6 +477,38 @@ __global__ void __launch_bounds__(kBlock) k_project(GridDev g, const uint8_t* __
     if (y == ny - 1) v[lin3(x, y + 1, z, nx, ny + 1, nz)] = MGF_UNKNOWN;
     if (z == nz - 1) w[lin3(x, y, z + 1, nx, ny, nz + 1)] = MGF_UNKNOWN;
 }
+
+__global__ void __launch_bounds__(kBlock) k_project(GridDev g, const uint8_t* __restrict__ m,
+                                                    float sx, float sy, float sz,
+                                                    const float* __restrict__ p32,
+                                                    const double* __restrict__ p64,
+                                                    float* __restrict__ u, float* __restrict__ v,
+                                                    float* __restrict__ w,
+                                                    const float* __restrict__ us,
+                                                    const float* __restrict__ vs,
+                                                    const float* __restrict__ ws, uint32_t n) {
+    k_project_run(g, m, sx, sy, sz, p32, p64, u, v, w, us, vs, ws, n, xcd_block());
+}
+
+struct MProject {
+    static constexpr int kThreads = kBlock;
+    GridDev g;
+    const uint8_t* m;
+    float sx;
+    float sy;
+    float sz;
+    const float* p32;
+    const double* p64;
+    float* u;
+    float* v;
+    float* w;
+    const float* us;
+    const float* vs;
+    const float* ws;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_project_run(g, m, sx, sy, sz, p32, p64, u, v, w, us, vs, ws, n, xcd_of(vb, G)); }
+};
 
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
@@ -637,6 +769,111 @@ hipError_t launch_fluid_box(hipStream_t s, const GridDev& g, const float mn[3], 
                             float* phi) {
     const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
     return LAUNCH(k_fluid_box, grid_for(n), g, mn[0], mn[1], mn[2], mx[0], mx[1], mx[2], phi, n);
+}
+
+// ===========================================================================
+// Multi-scene records of the launches launch_advect_fc, launch_advect_cc,
+// launch_advect_cc4, launch_markers, launch_rhs and launch_project make (the
+// same arguments and grids).
+template <class A>
+static void push_rec_k(std::vector<MRec>& out, int32_t kind, const A& a, uint32_t blocks) {
+    static_assert(sizeof(A) <= kMRecBytes, "argument struct fits a record");
+    MRec r;
+    r.kind = kind;
+    r.blocks = blocks;
+    r.bytes = uint32_t(sizeof(A));
+    std::memcpy(r.args, &a, sizeof(A));
+    out.push_back(r);
+}
+
+void rec_advect_fc(std::vector<MRec>& out, const GridDev& g, Vel in, Vel V, float dt, int order, float* ou, float* ov,
+                   float* ow, int add0, const float inc0[3], int add1, const float inc1[3]) {
+    const uint32_t nfx = uint32_t(g.nx + 1) * g.ny * g.nz, nfy = uint32_t(g.nx) * (g.ny + 1) * g.nz,
+                   nfz = uint32_t(g.nx) * g.ny * (g.nz + 1);
+    Inc inc{};
+    for (int q = 0; q < 3; ++q) {
+        inc.a[q] = inc0 ? inc0[q] : 0.0f;
+        inc.b[q] = inc1 ? inc1[q] : 0.0f;
+    }
+    inc.add_a = add0;
+    inc.add_b = add1;
+    const uint32_t G = grid_for(size_t(nfx) + nfy + nfz);
+    push_rec_k(out, kMkAdvectFc, MAdvectFc{g, in, V, rk_coef(dt), order, ou, ov, ow, nfx, nfy, nfz, inc, G}, G);
+}
+
+void rec_advect_cc(std::vector<MRec>& out, const GridDev& g, const float* in, Vel V, float dt, int order, float* o) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    const uint32_t G = grid_for(n);
+    push_rec_k(out, kMkAdvectCc, MAdvectCc{g, in, V, rk_coef(dt), order, o, n, G}, G);
+}
+
+void rec_advect_cc4(std::vector<MRec>& out, const GridDev& g, const float* phi, const float* const c[3], Vel V,
+                    float dt, int order, float* ophi, float* const oc[3]) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    Cc4 F;
+    F.in[0] = phi;
+    F.out[0] = ophi;
+    for (int q = 0; q < 3; ++q) {
+        F.in[q + 1] = c[q];
+        F.out[q + 1] = oc[q];
+    }
+    const uint32_t G = grid_for(n);
+    push_rec_k(out, kMkAdvectCc4, MAdvectCc4{g, F, V, rk_coef(dt), order, n, G}, G);
+}
+
+void rec_markers(std::vector<MRec>& out, const GridDev& g, const uint8_t* solid_mask, const float* fluid,
+                 uint8_t* markers) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    const uint32_t G = grid_for(n);
+    push_rec_k(out, kMkMarkers, MMarkers{n, solid_mask, fluid, markers, G}, G);
+}
+
+void rec_rhs(std::vector<MRec>& out, const GridDev& g, Vel U, Vel S, const uint8_t* markers, float* b) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    const uint32_t G = grid_for(n);
+    push_rec_k(out, kMkRhs, MRhs{g, U, S, markers, 1.0f / g.hx, 1.0f / g.hy, 1.0f / g.hz, b, n, G}, G);
+}
+
+void rec_project(std::vector<MRec>& out, const GridDev& g, const float sg[3], const uint8_t* markers, const float* p32,
+                 const double* p64, float* u, float* v, float* w, const float* us, const float* vs, const float* ws) {
+    const uint32_t n = uint32_t(g.nx) * g.ny * g.nz;
+    const uint32_t G = grid_for(n);
+    push_rec_k(out, kMkProject, MProject{g, markers, sg[0], sg[1], sg[2], p32, p64, u, v, w, us, vs, ws, n, G}, G);
+}
+
+// Copies one 32-bit word per scene into a packed array (one work-item).
+struct MCopyWord {
+    static constexpr int kThreads = 64;
+    const uint32_t* src;
+    uint32_t* dst;
+    __device__ void run(uint32_t) const {
+        if (threadIdx.x == 0) *dst = *src;
+    }
+};
+
+void rec_copy_word(std::vector<MRec>& out, const uint32_t* src, uint32_t* dst) {
+    push_rec_k(out, kMkCopyWord, MCopyWord{src, dst}, 1u);
+}
+
+template <class A>
+static hipError_t launch_mk(hipStream_t s, const char* name, const void* tab, const uint32_t* map, uint32_t nblocks,
+                            const uint8_t* active) {
+    return launch_dims(k_multi<A>, name, dim3(nblocks), dim3(A::kThreads), 0u, s, static_cast<const A*>(tab), map,
+                       active);
+}
+
+hipError_t launch_multi_k(hipStream_t s, int32_t kind, const void* tab, const uint32_t* map, uint32_t nblocks,
+                          uint32_t shmem, const uint8_t* active) {
+    switch (kind) {
+        case kMkAdvectFc: return launch_mk<MAdvectFc>(s, "k_multi_advect_fc", tab, map, nblocks, active);
+        case kMkAdvectCc: return launch_mk<MAdvectCc>(s, "k_multi_advect_cc", tab, map, nblocks, active);
+        case kMkAdvectCc4: return launch_mk<MAdvectCc4>(s, "k_multi_advect_cc4", tab, map, nblocks, active);
+        case kMkMarkers: return launch_mk<MMarkers>(s, "k_multi_markers", tab, map, nblocks, active);
+        case kMkRhs: return launch_mk<MRhs>(s, "k_multi_rhs", tab, map, nblocks, active);
+        case kMkProject: return launch_mk<MProject>(s, "k_multi_project", tab, map, nblocks, active);
+        case kMkCopyWord: return launch_mk<MCopyWord>(s, "k_multi_copy_word", tab, map, nblocks, active);
+        default: return launch_multi_ext(s, kind, tab, map, nblocks, shmem, active);
+    }
 }
 
 }  // namespace mgf

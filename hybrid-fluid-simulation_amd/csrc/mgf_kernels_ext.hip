@@ -24,6 +24,8 @@
 // The mean is summed in the reference's neighbour order (-x, +x, -y, +y, -z,
 // +z) in FP32 and divided once, so the values are bitwise the reference's.
 #include <algorithm>
+#include <cstring>
+#include <vector>
 
 #include "mgf_internal.h"
 #include "mgf_kcommon.h"
@@ -73,8 +75,10 @@ __device__ __forceinline__ uint32_t udiv_r(uint32_t a, uint32_t d, double rd) {
 // 1a. Rows along x: d = 0 on known faces, INF elsewhere; then the 1-D L1
 // distance: forward d(x) = x + min_{k<=x} (d(k) - k), backward
 // d(x) = min_{k>=x} (d(k) + k) - x.  One wave per row.
-__global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, uint16_t* __restrict__ dist) {
-    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+__device__ __forceinline__ void k_ext_rows_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
+    const uint32_t wave = slot * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= F.rows[3]) return;
     const int c = comp_of(F.rows, wave);
@@ -116,13 +120,27 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, uint16_t* __restri
     }
 }
 
+__global__ void __launch_bounds__(kBlock) k_ext_rows(Faces F, uint16_t* __restrict__ dist) {
+    k_ext_rows_run(F, dist, xcd_block(), blockIdx.x);
+}
+
+struct MExtRows {
+    static constexpr int kThreads = kBlock;
+    Faces F;
+    uint16_t* dist;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_rows_run(F, dist, xcd_of(vb, G), vb); }
+};
+
 // 1b. Lines along y (axis 1) or z (axis 2): d = min(d, d(prev) + 1) forward
 // then backward; one work-item per line, lines x-fastest (coalesced), 32
 // loads in flight per chunk (a line is a chain of dependent minima, so the
 // loads of a chunk are issued before its first minimum).
 template <int AXIS>
-__global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, uint16_t* __restrict__ dist) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_ext_lines_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
+    const uint32_t t = slot * kBlock + threadIdx.x;
     const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
     if (t >= pre[3]) return;
     const int c = comp_of(pre, t);
@@ -166,13 +184,29 @@ __global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, uint16_t* __restr
     }
 }
 
+template <int AXIS>
+__global__ void __launch_bounds__(kBlock) k_ext_lines(Faces F, uint16_t* __restrict__ dist) {
+    k_ext_lines_run<AXIS>(F, dist, xcd_block(), blockIdx.x);
+}
+
+template <int AXIS>
+struct MExtLines {
+    static constexpr int kThreads = kBlock;
+    Faces F;
+    uint16_t* dist;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_lines_run<AXIS>(F, dist, xcd_of(vb, G), vb); }
+};
+
 // 1a'. The same row transform with the row held in registers: NCH chunks of
 // 64 faces per lane (rows of at most 64 NCH faces), one read of the faces and
 // one write of the distances (the two-pass kernel above re-reads and
 // re-writes them).  The same operations, so the same distances.
 template <int NCH>
-__global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, uint16_t* __restrict__ dist) {
-    const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+__device__ __forceinline__ void k_ext_rows_reg_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
+    const uint32_t wave = slot * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (wave >= F.rows[3]) return;
     const int c = comp_of(F.rows, wave);
@@ -224,6 +258,20 @@ __global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, uint16_t* __re
     }
 }
 
+template <int NCH>
+__global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, uint16_t* __restrict__ dist) {
+    k_ext_rows_reg_run<NCH>(F, dist, xcd_block(), blockIdx.x);
+}
+
+template <int NCH>
+struct MExtRowsReg {
+    static constexpr int kThreads = kBlock;
+    Faces F;
+    uint16_t* dist;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_rows_reg_run<NCH>(F, dist, xcd_of(vb, G), vb); }
+};
+
 // ---------------------------------------------------------------------------
 // 2. Counting sort by distance.  Bins 1..cap.  A block counts its faces in
 // LDS (kLdsBins bins; larger distances go straight to the global counters),
@@ -271,19 +319,35 @@ __device__ __forceinline__ uint32_t lds_count(uint32_t* ctr_of_key, int32_t key,
     return active ? atomicAdd(&ctr_of_key[key], 1u) : 0u;
 }
 
-__global__ void __launch_bounds__(kBlock) k_ext_zero(uint32_t* __restrict__ a, uint32_t n) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_ext_zero_run(uint32_t* __restrict__ a, uint32_t n, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
+    const uint32_t i = slot * kBlock + threadIdx.x;
     if (i < n) a[i] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_ext_count(const uint16_t* __restrict__ dist, uint32_t total, int32_t cap,
-                                                      uint32_t* __restrict__ hist, uint32_t K2) {
+__global__ void __launch_bounds__(kBlock) k_ext_zero(uint32_t* __restrict__ a, uint32_t n) {
+    k_ext_zero_run(a, n, xcd_block(), blockIdx.x);
+}
+
+struct MExtZero {
+    static constexpr int kThreads = kBlock;
+    uint32_t* a;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_zero_run(a, n, xcd_of(vb, G), vb); }
+};
+
+__device__ __forceinline__ void k_ext_count_run(const uint16_t* __restrict__ dist, uint32_t total, int32_t cap,
+                                                      uint32_t* __restrict__ hist, uint32_t K2, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
     __syncthreads();
-    uint32_t* hp = hist + size_t(blockIdx.x % kCopies) * K2;
-    const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
+    uint32_t* hp = hist + size_t(slot % kCopies) * K2;
+    const uint32_t base = blk * (kBlock * kSortCpt) + threadIdx.x;
     // all loads first (clamped index, no branch), then the counting
     int32_t dv[kSortCpt];
 #pragma unroll
@@ -303,13 +367,31 @@ __global__ void __launch_bounds__(kBlock) k_ext_count(const uint16_t* __restrict
         if (h[b]) atomicAdd(&hp[b], h[b]);
 }
 
+__global__ void __launch_bounds__(kBlock) k_ext_count(const uint16_t* __restrict__ dist, uint32_t total, int32_t cap,
+                                                      uint32_t* __restrict__ hist, uint32_t K2) {
+    k_ext_count_run(dist, total, cap, hist, K2, xcd_block(), blockIdx.x);
+}
+
+struct MExtCount {
+    static constexpr int kThreads = kBlock;
+    const uint16_t* dist;
+    uint32_t total;
+    int32_t cap;
+    uint32_t* hist;
+    uint32_t K2;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_count_run(dist, total, cap, hist, K2, xcd_of(vb, G), vb); }
+};
+
 // Per distance b: T = sum over copies; offs[b] = exclusive scan of T over
 // b = 1..cap (offs[cap+1] = total); poff[p][b] = offs[b] + counts of the
 // copies before p; cursors cleared; *maxd = the largest non-empty distance.
 // One block.
-__global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ hist, int32_t cap, uint32_t K2,
+__device__ __forceinline__ void k_ext_scan_run(const uint32_t* __restrict__ hist, int32_t cap, uint32_t K2,
                                                    uint32_t* __restrict__ offs, uint32_t* __restrict__ poff,
-                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ maxd) {
+                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ maxd, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
     __shared__ int32_t top;
@@ -354,6 +436,25 @@ __global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ 
     }
 }
 
+__global__ void __launch_bounds__(1024) k_ext_scan(const uint32_t* __restrict__ hist, int32_t cap, uint32_t K2,
+                                                   uint32_t* __restrict__ offs, uint32_t* __restrict__ poff,
+                                                   uint32_t* __restrict__ cursor, uint32_t* __restrict__ maxd) {
+    k_ext_scan_run(hist, cap, K2, offs, poff, cursor, maxd, xcd_block(), blockIdx.x);
+}
+
+struct MExtScan {
+    static constexpr int kThreads = 1024;
+    const uint32_t* hist;
+    int32_t cap;
+    uint32_t K2;
+    uint32_t* offs;
+    uint32_t* poff;
+    uint32_t* cursor;
+    uint32_t* maxd;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_scan_run(hist, cap, K2, offs, poff, cursor, maxd, xcd_of(vb, G), vb); }
+};
+
 // Bit q of the lower-neighbour mask of a face: its in-grid neighbour in
 // direction q (-x, +x, -y, +y, -z, +z) has a smaller distance, i.e. lies in
 // the previous layer.
@@ -379,18 +480,20 @@ __device__ __forceinline__ uint32_t lower_mask(const Faces& F, const uint16_t* _
     return m;
 }
 
-__global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const uint16_t* __restrict__ dist, uint32_t total,
+__device__ __forceinline__ void k_ext_scatter_run(Faces F, const uint16_t* __restrict__ dist, uint32_t total,
                                                         int32_t cap, uint32_t K2, const uint32_t* __restrict__ poff,
                                                         uint32_t* cursor, uint32_t* __restrict__ list,
-                                                        uint8_t* __restrict__ lmask, uint8_t* __restrict__ fmask) {
+                                                        uint8_t* __restrict__ lmask, uint8_t* __restrict__ fmask, uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
     __shared__ uint32_t h[kLdsBins];
     const int32_t nb = min(cap + 1, kLdsBins);
     for (int b = threadIdx.x; b < nb; b += kBlock) h[b] = 0;
     __syncthreads();
-    const size_t copy = size_t(blockIdx.x % kCopies) * K2;
+    const size_t copy = size_t(slot % kCopies) * K2;
     const uint32_t* po = poff + copy;
     uint32_t* cu = cursor + copy;
-    const uint32_t base = xcd_block() * (kBlock * kSortCpt) + threadIdx.x;
+    const uint32_t base = blk * (kBlock * kSortCpt) + threadIdx.x;
     int32_t dv[kSortCpt];
     uint32_t rank[kSortCpt];
 #pragma unroll
@@ -429,6 +532,29 @@ __global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const uint16_t*
         }
     }
 }
+
+__global__ void __launch_bounds__(kBlock) k_ext_scatter(Faces F, const uint16_t* __restrict__ dist, uint32_t total,
+                                                        int32_t cap, uint32_t K2, const uint32_t* __restrict__ poff,
+                                                        uint32_t* cursor, uint32_t* __restrict__ list,
+                                                        uint8_t* __restrict__ lmask, uint8_t* __restrict__ fmask) {
+    k_ext_scatter_run(F, dist, total, cap, K2, poff, cursor, list, lmask, fmask, xcd_block(), blockIdx.x);
+}
+
+struct MExtScatter {
+    static constexpr int kThreads = kBlock;
+    Faces F;
+    const uint16_t* dist;
+    uint32_t total;
+    int32_t cap;
+    uint32_t K2;
+    const uint32_t* poff;
+    uint32_t* cursor;
+    uint32_t* list;
+    uint8_t* lmask;
+    uint8_t* fmask;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_scatter_run(F, dist, total, cap, K2, poff, cursor, list, lmask, fmask, xcd_of(vb, G), vb); }
+};
 
 // ---------------------------------------------------------------------------
 // 3. Layer L: each listed face takes the mean of its neighbours in the
@@ -513,6 +639,90 @@ __global__ void __launch_bounds__(kBlock) k_ext_layer2(Faces F, const uint32_t* 
     if (cnt != 0.0f) acc /= cnt;
     f[i] = acc;
 }
+
+// Layers first..last in one work-group: layer by layer, a barrier between
+// layers ordering each layer's stores before the next layer's loads (the
+// work-group's work-items share one CU), kTailCpt entries per work-item per
+// round.  A round is latency-bound (list entry, then the neighbours, then
+// the store), so every load of a round is issued before any is used: the
+// six neighbours are loaded unconditionally (an unflagged one at the face's
+// own index) and only the flagged ones are summed.  For the tail of small
+// layers, where a launch per layer costs more than the layer's work, and
+// for small scenes' whole extrapolation.
+constexpr int kTailThreads = 1024;
+constexpr int kTailCpt = 8;
+
+__device__ __forceinline__ void k_ext_tail_run(Faces F, const uint32_t* __restrict__ list,
+                                               const uint8_t* __restrict__ lmask, const uint32_t* __restrict__ offs,
+                                               int32_t first, int32_t last, const uint32_t* __restrict__ last_dev,
+                                               uint32_t blk, uint32_t slot) {
+    (void)blk;
+    (void)slot;
+    if (last_dev) last = int32_t(*last_dev);
+    for (int32_t L = first; L <= last; ++L) {
+        const uint32_t b = offs[L], e = offs[L + 1];
+        for (uint32_t r0 = b; r0 < e; r0 += uint32_t(kTailThreads * kTailCpt)) {
+            uint32_t gi[kTailCpt], m[kTailCpt];
+#pragma unroll
+            for (int k = 0; k < kTailCpt; ++k) {
+                const uint32_t t = r0 + uint32_t(k * kTailThreads) + threadIdx.x;
+                const uint32_t tc = t < e ? t : e - 1;
+                gi[k] = list[tc];
+                m[k] = t < e ? uint32_t(lmask[tc]) : 0u;
+            }
+            float fv[kTailCpt][6];
+#pragma unroll
+            for (int k = 0; k < kTailCpt; ++k) {
+                const int c = comp_of(F.off, gi[k]);
+                const uint32_t i = gi[k] - pick4(F.off, c);
+                const uint32_t ex = pick(F.ex, c), sz = ex * pick(F.ey, c);
+                const float* f = pick(F.f, c);
+                const uint32_t mk = m[k];
+                const uint32_t j[6] = {(mk & 1u) ? i - 1 : i, (mk & 2u) ? i + 1 : i, (mk & 4u) ? i - ex : i,
+                                       (mk & 8u) ? i + ex : i, (mk & 16u) ? i - sz : i, (mk & 32u) ? i + sz : i};
+#pragma unroll
+                for (int q = 0; q < 6; ++q) fv[k][q] = f[j[q]];
+            }
+#pragma unroll
+            for (int k = 0; k < kTailCpt; ++k) {
+                const uint32_t t = r0 + uint32_t(k * kTailThreads) + threadIdx.x;
+                float acc = 0.0f, cnt = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const bool use = (m[k] >> q) & 1u;
+                    acc = use ? acc + fv[k][q] : acc;
+                    cnt = use ? cnt + 1.0f : cnt;
+                }
+                if (cnt != 0.0f) acc /= cnt;
+                if (t < e) {
+                    const int c = comp_of(F.off, gi[k]);
+                    pick(F.f, c)[gi[k] - pick4(F.off, c)] = acc;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kTailThreads) k_ext_tail(Faces F, const uint32_t* __restrict__ list,
+                                                           const uint8_t* __restrict__ lmask,
+                                                           const uint32_t* __restrict__ offs, int32_t first,
+                                                           int32_t last, const uint32_t* __restrict__ last_dev) {
+    k_ext_tail_run(F, list, lmask, offs, first, last, last_dev, xcd_block(), blockIdx.x);
+}
+
+struct MExtTail {
+    static constexpr int kThreads = kTailThreads;
+    Faces F;
+    const uint32_t* list;
+    const uint8_t* lmask;
+    const uint32_t* offs;
+    int32_t first;
+    int32_t last;
+    const uint32_t* last_dev;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_tail_run(F, list, lmask, offs, first, last, last_dev, xcd_of(vb, G), vb); }
+};
 
 // ===========================================================================
 // Fields the hybrid coupling reads (GridFluidSolver.cpp:571-613) on the
@@ -773,12 +983,178 @@ hipError_t launch_ext_layer(hipStream_t s, const GridDev& g, float* u, float* v,
                        count);
 }
 
+hipError_t launch_ext_tail(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                           const uint8_t* lmask, const uint32_t* offs, int32_t first, int32_t last,
+                           const uint32_t* last_dev) {
+    if (!last_dev && last < first) return hipSuccess;
+    const Faces F = faces_of(g, u, v, w);
+    return launch_dims(k_ext_tail, "k_ext_tail", dim3(1), dim3(kTailThreads), 0u, s, F, list, lmask, offs, first, last,
+                       last_dev);
+}
+
 hipError_t launch_ext_layer2(hipStream_t s, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
                              const uint8_t* lmask, const uint8_t* fmask, uint32_t begin, uint32_t n0, uint32_t n1) {
     if (n0 + n1 == 0) return hipSuccess;
     const Faces F = faces_of(g, u, v, w);
     return launch_dims(k_ext_layer2, "k_ext_layer2", dim3((n0 + n1 + kBlock - 1) / kBlock), dim3(kBlock), 0u, s, F, list, lmask,
                        fmask, begin, n0, n1);
+}
+
+// ===========================================================================
+// Multi-scene records of the distance transform, the counting sort and the
+// one-work-group layers (launch_ext_distance, launch_ext_sort and
+// launch_ext_tail with the last layer read on the device): the same
+// arguments and grids.
+template <class A>
+static void push_rec_e(std::vector<MRec>& out, int32_t kind, const A& a, uint32_t blocks) {
+    static_assert(sizeof(A) <= kMRecBytes, "argument struct fits a record");
+    MRec r;
+    r.kind = kind;
+    r.blocks = blocks;
+    r.bytes = uint32_t(sizeof(A));
+    std::memcpy(r.args, &a, sizeof(A));
+    out.push_back(r);
+}
+
+void rec_ext_device(std::vector<MRec>& out, const GridDev& g, float* u, float* v, float* w, uint16_t* dist,
+                    int32_t cap, int32_t K, uint32_t* counts, uint32_t* list, uint8_t* lmask, uint8_t* fmask,
+                    const uint32_t** maxd, bool with_layers) {
+    const Faces F = faces_of(g, u, v, w);
+    const uint32_t exm = F.ex[0];
+    const uint32_t rg = (F.rows[3] + 3) / 4;
+    if (exm <= 256) push_rec_e(out, kMkExtRowsReg4, MExtRowsReg<4>{F, dist, rg}, rg);
+    else if (exm <= 512) push_rec_e(out, kMkExtRowsReg8, MExtRowsReg<8>{F, dist, rg}, rg);
+    else if (exm <= 1024) push_rec_e(out, kMkExtRowsReg16, MExtRowsReg<16>{F, dist, rg}, rg);
+    else push_rec_e(out, kMkExtRows, MExtRows{F, dist, rg}, rg);
+    const uint32_t g1 = (F.ylines[3] + kBlock - 1) / kBlock, g2 = (F.zlines[3] + kBlock - 1) / kBlock;
+    push_rec_e(out, kMkExtLines1, MExtLines<1>{F, dist, g1}, g1);
+    push_rec_e(out, kMkExtLines2, MExtLines<2>{F, dist, g2}, g2);
+    const uint32_t total = F.off[3];
+    const uint32_t K2 = uint32_t(K) + 2;
+    const SortBufs B = sort_bufs(counts, K);
+    *maxd = B.maxd;
+    const uint32_t nh = kCopies * K2;
+    const uint32_t gz = (nh + kBlock - 1) / kBlock;
+    push_rec_e(out, kMkExtZero, MExtZero{B.hist, nh, gz}, gz);
+    size_t G = (size_t(total) + size_t(kBlock) * kSortCpt - 1) / (size_t(kBlock) * kSortCpt);
+    G = (G + kXcds - 1) / kXcds * kXcds;
+    push_rec_e(out, kMkExtCount, MExtCount{dist, total, cap, B.hist, K2, uint32_t(G)}, uint32_t(G));
+    push_rec_e(out, kMkExtScan, MExtScan{B.hist, cap, K2, B.offs, B.poff, B.cursor, B.maxd, 1u}, 1u);
+    push_rec_e(out, kMkExtScatter,
+               MExtScatter{F, dist, total, cap, K2, B.poff, B.cursor, list, lmask, fmask, uint32_t(G)},
+               uint32_t(G));
+    if (with_layers) push_rec_e(out, kMkExtTail, MExtTail{F, list, lmask, B.offs, 1, 0, B.maxd, 1u}, 1u);
+}
+
+// Layer L of many scenes in one launch: block b of a scene takes entries
+// offs[L] + b * kBlock * kLayerCpt ... of its list (the bounds read on the
+// device), each the mean of its flagged previous-layer neighbours (as
+// k_ext_layer).  map entries: (scene << kMapShift) | block.
+constexpr int kLayerCpt = 4;
+
+struct MLayer {
+    Faces F;
+    const uint32_t* list;
+    const uint8_t* lmask;
+    const uint32_t* offs;
+};
+
+__global__ void __launch_bounds__(kBlock) k_multi_layer(const MLayer* __restrict__ tab,
+                                                        const uint32_t* __restrict__ map, int32_t L) {
+    const uint32_t e = map[blockIdx.x];
+    const MLayer a = tab[e >> kMapShift];
+    const uint32_t b = a.offs[L], end = a.offs[L + 1];
+    const uint32_t base = b + (e & kMapLocal) * uint32_t(kBlock * kLayerCpt) + threadIdx.x;
+    uint32_t gi[kLayerCpt], m[kLayerCpt];
+#pragma unroll
+    for (int k = 0; k < kLayerCpt; ++k) {
+        const uint32_t t = base + uint32_t(k * kBlock);
+        const uint32_t tc = t < end ? t : end - 1;
+        gi[k] = a.list[tc];
+        m[k] = t < end ? uint32_t(a.lmask[tc]) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kLayerCpt; ++k) {
+        const uint32_t t = base + uint32_t(k * kBlock);
+        if (t >= end) continue;
+        const int c = comp_of(a.F.off, gi[k]);
+        const uint32_t i = gi[k] - pick4(a.F.off, c);
+        const uint32_t ex = pick(a.F.ex, c), sz = ex * pick(a.F.ey, c);
+        float* f = pick(a.F.f, c);
+        f[i] = layer_mean(f, i, m[k], ex, sz);
+    }
+}
+
+uint32_t layer_entries_per_block() { return uint32_t(kBlock * kLayerCpt); }
+
+size_t layer_arg_bytes() { return sizeof(MLayer); }
+
+void layer_args(unsigned char* dst, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+                const uint8_t* lmask, uint32_t* counts, int32_t K) {
+    const MLayer a{faces_of(g, u, v, w), list, lmask, sort_bufs(counts, K).offs};
+    std::memcpy(dst, &a, sizeof(MLayer));
+}
+
+hipError_t launch_multi_layer(hipStream_t s, const void* tab, const uint32_t* map, uint32_t nblocks, int32_t L) {
+    if (nblocks == 0) return hipSuccess;
+    return launch_dims(k_multi_layer, "k_multi_layer", dim3(nblocks), dim3(kBlock), 0u, s,
+                       static_cast<const MLayer*>(tab), map, L);
+}
+
+// Each scene's layer offsets 0..cap+1 and its largest layer, packed for one
+// copy to the host (one block per scene).
+struct MCopyOffs {
+    static constexpr int kThreads = kBlock;
+    const uint32_t* offs;
+    const uint32_t* maxd;
+    uint32_t n;
+    uint32_t* dst;
+    __device__ void run(uint32_t) const {
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) dst[i] = offs[i];
+        if (threadIdx.x == 0) dst[n] = *maxd;
+    }
+};
+
+void rec_copy_offs(std::vector<MRec>& out, uint32_t* counts, int32_t K, int32_t cap, uint32_t* dst) {
+    const SortBufs B = sort_bufs(counts, K);
+    push_rec_e(out, kMkCopyOffs, MCopyOffs{B.offs, B.maxd, uint32_t(cap) + 2u, dst}, 1u);
+}
+
+// The sort's layers from `first` to the largest present in one work-group
+// (the layers a scene's own launches do not take).
+size_t tail_arg_bytes() { return sizeof(MExtTail); }
+
+void tail_args(unsigned char* dst, const GridDev& g, float* u, float* v, float* w, const uint32_t* list,
+               const uint8_t* lmask, uint32_t* counts, int32_t K, int32_t first, int32_t last) {
+    const SortBufs B = sort_bufs(counts, K);
+    const MExtTail a{faces_of(g, u, v, w), list, lmask, B.offs, first, last, nullptr, 1u};
+    std::memcpy(dst, &a, sizeof(MExtTail));
+}
+
+template <class A>
+static hipError_t launch_me(hipStream_t s, const char* name, const void* tab, const uint32_t* map, uint32_t nblocks,
+                            const uint8_t* active) {
+    return launch_dims(k_multi<A>, name, dim3(nblocks), dim3(A::kThreads), 0u, s, static_cast<const A*>(tab), map,
+                       active);
+}
+
+hipError_t launch_multi_ext(hipStream_t s, int32_t kind, const void* tab, const uint32_t* map, uint32_t nblocks,
+                            uint32_t, const uint8_t* active) {
+    switch (kind) {
+        case kMkExtRows: return launch_me<MExtRows>(s, "k_multi_ext_rows", tab, map, nblocks, active);
+        case kMkExtRowsReg4: return launch_me<MExtRowsReg<4>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
+        case kMkExtRowsReg8: return launch_me<MExtRowsReg<8>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
+        case kMkExtRowsReg16: return launch_me<MExtRowsReg<16>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
+        case kMkExtLines1: return launch_me<MExtLines<1>>(s, "k_multi_ext_lines", tab, map, nblocks, active);
+        case kMkExtLines2: return launch_me<MExtLines<2>>(s, "k_multi_ext_lines", tab, map, nblocks, active);
+        case kMkExtZero: return launch_me<MExtZero>(s, "k_multi_ext_zero", tab, map, nblocks, active);
+        case kMkExtCount: return launch_me<MExtCount>(s, "k_multi_ext_count", tab, map, nblocks, active);
+        case kMkExtScan: return launch_me<MExtScan>(s, "k_multi_ext_scan", tab, map, nblocks, active);
+        case kMkExtScatter: return launch_me<MExtScatter>(s, "k_multi_ext_scatter", tab, map, nblocks, active);
+        case kMkExtTail: return launch_me<MExtTail>(s, "k_multi_ext_tail", tab, map, nblocks, active);
+        case kMkCopyOffs: return launch_me<MCopyOffs>(s, "k_multi_copy_offs", tab, map, nblocks, active);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace mgf

@@ -42,9 +42,9 @@ enum : uint16_t {
     kCplShift = 7,
 };
 
-__global__ void __launch_bounds__(kBlock) k_codes(int nx, int ny, int nz, const uint8_t* __restrict__ m,
-                                                  uint16_t* __restrict__ code, uint32_t n) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_codes_run(int nx, int ny, int nz, const uint8_t* __restrict__ m,
+                                                  uint16_t* __restrict__ code, uint32_t n, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
               z = int(i / (uint32_t(nx) * uint32_t(ny)));
@@ -61,6 +61,23 @@ __global__ void __launch_bounds__(kBlock) k_codes(int nx, int ny, int nz, const 
     }
     code[i] = uint16_t(c);
 }
+
+__global__ void __launch_bounds__(kBlock) k_codes(int nx, int ny, int nz, const uint8_t* __restrict__ m,
+                                                  uint16_t* __restrict__ code, uint32_t n) {
+    k_codes_run(nx, ny, nz, m, code, n, xcd_block());
+}
+
+struct MCodes {
+    static constexpr int kThreads = kBlock;
+    int nx;
+    int ny;
+    int nz;
+    const uint8_t* m;
+    uint16_t* code;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_codes_run(nx, ny, nz, m, code, n, xcd_of(vb, G)); }
+};
 
 // Per-level tables indexed by a 6-bit neighbour mask (-x,+x,-y,+y,-z,+z):
 // sum of the scales of the masked directions, added in the reference's order
@@ -220,10 +237,9 @@ __device__ __forceinline__ double cg_pap_body(const Flat& F, uint32_t base, cons
 // are +0 too: such a block would store the values its cells hold and add +0
 // to the dot products, so K_I (after the first iteration) and K_II return at
 // once with a zero partial.
-__global__ void __launch_bounds__(kBlock) k_flat_wet(Flat F, const uint16_t* __restrict__ code,
-                                                     uint8_t* __restrict__ wet) {
-    const uint32_t blk = xcd_block();
-    const uint32_t b0 = blk * (kBlock * kCpt);
+__device__ __forceinline__ void k_flat_wet_run(Flat F, const uint16_t* __restrict__ code,
+                                                     uint8_t* __restrict__ wet, uint32_t blk) {
+        const uint32_t b0 = blk * (kBlock * kCpt);
     int any = 0;
 #pragma unroll
     for (int k = 0; k < kCpt; ++k) {
@@ -233,6 +249,20 @@ __global__ void __launch_bounds__(kBlock) k_flat_wet(Flat F, const uint16_t* __r
     any = __syncthreads_or(any);
     if (threadIdx.x == 0) wet[blk] = uint8_t(any != 0);
 }
+
+__global__ void __launch_bounds__(kBlock) k_flat_wet(Flat F, const uint16_t* __restrict__ code,
+                                                     uint8_t* __restrict__ wet) {
+    k_flat_wet_run(F, code, wet, xcd_block());
+}
+
+struct MFlatWet {
+    static constexpr int kThreads = kBlock;
+    Flat F;
+    const uint16_t* code;
+    uint8_t* wet;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_flat_wet_run(F, code, wet, xcd_of(vb, G)); }
+};
 
 // A kernel body runs block `blk` of its launch (after the XCD remapping) and
 // writes its partial sums to slot `slot` (the launch's own block index).
@@ -709,9 +739,9 @@ using MgLevel = MgDims;
 
 // Coarse markers: with fluid_first, fluid if any child is fluid, else air if
 // any child is air, else solid; otherwise air wins over fluid.
-__global__ void __launch_bounds__(kBlock) k_mg_coarsen(MgLevel F, MgLevel Cc, const uint8_t* __restrict__ mf,
-                                                       uint8_t* __restrict__ mc, int fluid_first, uint32_t n) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_mg_coarsen_run(MgLevel F, MgLevel Cc, const uint8_t* __restrict__ mf,
+                                                       uint8_t* __restrict__ mc, int fluid_first, uint32_t n, uint32_t blk) {
+    const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const int X = int(i % uint32_t(Cc.nx)), Y = int((i / uint32_t(Cc.nx)) % uint32_t(Cc.ny)),
               Z = int(i / (uint32_t(Cc.nx) * uint32_t(Cc.ny)));
@@ -731,6 +761,23 @@ __global__ void __launch_bounds__(kBlock) k_mg_coarsen(MgLevel F, MgLevel Cc, co
     else out = any_a ? kAir : (any_f ? kFluid : kSolid);
     mc[i] = out;
 }
+
+__global__ void __launch_bounds__(kBlock) k_mg_coarsen(MgLevel F, MgLevel Cc, const uint8_t* __restrict__ mf,
+                                                       uint8_t* __restrict__ mc, int fluid_first, uint32_t n) {
+    k_mg_coarsen_run(F, Cc, mf, mc, fluid_first, n, xcd_block());
+}
+
+struct MCoarsen {
+    static constexpr int kThreads = kBlock;
+    MgLevel F;
+    MgLevel Cc;
+    const uint8_t* mf;
+    uint8_t* mc;
+    int fluid_first;
+    uint32_t n;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_mg_coarsen_run(F, Cc, mf, mc, fluid_first, n, xcd_of(vb, G)); }
+};
 
 // One damped Jacobi sweep xo = xi + omega D^-1 (b - A xi) on fluid cells (0
 // elsewhere, and on zero rows: a fluid cell whose neighbours are all solid
@@ -1778,10 +1825,9 @@ __device__ __forceinline__ void march_dry(const March& M, uint32_t blk, float* _
 
 // wet[blk] = 1 when block blk's tile holds a fluid cell in its planes (the
 // codes do not change during a pressure solve: one pass per solve).
-__global__ void __launch_bounds__(kMThreads) k_mg_march_wet(March M, const uint16_t* __restrict__ code,
-                                                            uint8_t* __restrict__ wet) {
-    const uint32_t blk = xcd_block();
-    if (blk >= M.blocks) return;
+__device__ __forceinline__ void k_mg_march_wet_run(March M, const uint16_t* __restrict__ code,
+                                                            uint8_t* __restrict__ wet, uint32_t blk) {
+        if (blk >= M.blocks) return;
     const MarchTile T = march_tile(M, blk);
     const int w = T.x1 - T.x0, h = T.y1 - T.y0;
     int any = 0;
@@ -1792,6 +1838,20 @@ __global__ void __launch_bounds__(kMThreads) k_mg_march_wet(March M, const uint1
     any = __syncthreads_or(any);
     if (threadIdx.x == 0) wet[blk] = uint8_t(any != 0);
 }
+
+__global__ void __launch_bounds__(kMThreads) k_mg_march_wet(March M, const uint16_t* __restrict__ code,
+                                                            uint8_t* __restrict__ wet) {
+    k_mg_march_wet_run(M, code, wet, xcd_block());
+}
+
+struct MMarchWet {
+    static constexpr int kThreads = kMThreads;
+    March M;
+    const uint16_t* code;
+    uint8_t* wet;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_mg_march_wet_run(M, code, wet, xcd_of(vb, G)); }
+};
 
 template <bool POST, bool RZ>
 __device__ __forceinline__ void march_run(const March& M, const MgLevel& L, const uint16_t* __restrict__ code,
@@ -2268,7 +2328,10 @@ void rec_refine_true(std::vector<MRec>& out, const GridDev& g, const float scale
                      double* out2) {
     const Flat F = make_flat(g.nx, g.ny, g.nz);
     const uint32_t G = flat_grid(F.n);
-    if (mode == 1) {
+    if (mode == 0) {
+        push_rec(out, kMkRefine0, MRefine<0>{F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials, G},
+                 G);
+    } else if (mode == 1) {
         push_rec(out, kMkRefine1, MRefine<1>{F, codes, scale[0], scale[1], scale[2], xcorr, x64, b, r, red.partials, G},
                  G);
     } else {
@@ -2278,6 +2341,31 @@ void rec_refine_true(std::vector<MRec>& out, const GridDev& g, const float scale
                  G);
     }
     push_rec(out, kMkSum2, MSum2{red.partials, G, out2}, 1u);
+}
+
+void rec_codes(std::vector<MRec>& out, int nx, int ny, int nz, const uint8_t* markers, uint16_t* codes) {
+    const uint32_t n = uint32_t(nx) * ny * nz;
+    const uint32_t G = grid_for(n);
+    push_rec(out, kMkCodes, MCodes{nx, ny, nz, markers, codes, n, G}, G);
+}
+
+void rec_flat_wet(std::vector<MRec>& out, const GridDev& g, const uint16_t* codes, uint8_t* wet) {
+    const Flat F = make_flat(g.nx, g.ny, g.nz);
+    const uint32_t G = flat_grid(F.n);
+    push_rec(out, kMkFlatWet, MFlatWet{F, codes, wet, G}, G);
+}
+
+void rec_mg_coarsen(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint8_t* mf, uint8_t* mc,
+                    int fluid_first) {
+    const uint32_t n = uint32_t(c.nx) * c.ny * c.nz;
+    const uint32_t G = grid_for(n);
+    push_rec(out, kMkCoarsen, MCoarsen{f, c, mf, mc, fluid_first, n, G}, G);
+}
+
+void rec_mg_march_wet(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet) {
+    const March M = make_march(f, c);
+    const uint32_t G = mg_march_grid(f, c);
+    push_rec(out, kMkMarchWet, MMarchWet{M, code, wet, G}, G);
 }
 
 void rec_mg_smooth(std::vector<MRec>& out, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
@@ -2447,7 +2535,12 @@ hipError_t launch_multi(hipStream_t s, int32_t kind, const void* tab, const uint
         case kMkMarchAsc: return launch_m_march<MMarch<true, false>>(s, tab, map, nblocks, active);
         case kMkMarchAscRz: return launch_m_march<MMarch<true, true>>(s, tab, map, nblocks, active);
         case kMkGather: return launch_m<MGather>(s, "k_multi_gather", tab, map, nblocks, 0, active);
-        default: return hipErrorInvalidValue;
+        case kMkRefine0: return launch_m<MRefine<0>>(s, "k_multi_refine", tab, map, nblocks, 0, active);
+        case kMkCodes: return launch_m<MCodes>(s, "k_multi_codes", tab, map, nblocks, 0, active);
+        case kMkFlatWet: return launch_m<MFlatWet>(s, "k_multi_flat_wet", tab, map, nblocks, 0, active);
+        case kMkCoarsen: return launch_m<MCoarsen>(s, "k_multi_mg_coarsen", tab, map, nblocks, 0, active);
+        case kMkMarchWet: return launch_m<MMarchWet>(s, "k_multi_mg_march_wet", tab, map, nblocks, 0, active);
+        default: return launch_multi_k(s, kind, tab, map, nblocks, shmem, active);
     }
 }
 

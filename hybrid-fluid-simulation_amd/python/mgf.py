@@ -164,6 +164,7 @@ _SIGS = {
     "mgf_op_scale": [_VP, C.c_int64, C.c_float, _VP],
     "mgf_op_extrapolate_fc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
     "mgf_batch_set_lockstep": [_VP, C.c_int32],
+    "mgf_batch_autotune": [_VP, _VP, C.c_int32, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)],
     "mgf_op_extrapolate_fc_noalloc": [_VP, C.POINTER(Grid), _VP, _VP, _VP, C.c_int32, C.POINTER(C.c_int32)],
 }
 _RESTYPE = {"mgf_status_string": C.c_char_p, "mgf_last_error": C.c_char_p}
@@ -443,6 +444,16 @@ def grid(res, origin=(0, 0, 0), spacing=(1, 1, 1)):
     g.origin = farr3(origin)
     g.spacing = farr3(spacing)
     return g
+
+
+def batch_autotune(ctx, scenes, trial_steps=3):
+    """mgf_batch_autotune over Scene objects: (chosen mg_kernels, [ms AUTO, ms FUSED])."""
+    n = len(scenes)
+    arr = (C.c_void_p * n)(*[sc.h.value if isinstance(sc.h, C.c_void_p) else sc.h for sc in scenes])
+    chosen = C.c_int32()
+    ms = (C.c_double * 2)()
+    check(lib().mgf_batch_autotune(ctx.h, arr, n, trial_steps, C.byref(chosen), ms), "mgf_batch_autotune")
+    return chosen.value, [ms[0], ms[1]]
 
 
 def batch_step(ctx, scenes, nsteps, workers=0, step_latency=False):

@@ -354,6 +354,20 @@ MGF_API mgf_status mgf_batch_step(mgf_context ctx, mgf_scene* scenes, int32_t n,
                                   double* step_ms);
 /* Enables (default) or disables lockstep in mgf_batch_step for this context. */
 MGF_API mgf_status mgf_batch_set_lockstep(mgf_context ctx, int32_t enabled);
+/* Autotuner of the lockstep batch (no counterpart in the reference).  Times
+ * the V-cycle kernel choices on these scenes, each over one lockstep batch
+ * call of trial_steps (>= 2) steps, and keeps the faster on every scene that
+ * runs in lockstep (their mg_kernels field): MGF_MG_AUTO (fused descent and
+ * ascent on levels of at least 1M cells, one kernel per sweep below) and
+ * MGF_MG_FUSED (fused on every 3-D level above the coarse cycle; small
+ * levels of many scenes fill the device together).  Every scene advances by
+ * 2 * trial_steps steps.  The choice changes only the summation order of
+ * r.z (z = M^-1 r is bitwise the same), so the iterates differ within FP64
+ * rounding; a scene then computes bitwise what it computes alone with the
+ * same mg_kernels.  *chosen (may be null) receives the choice, ms (2
+ * entries, may be null) each candidate's fastest step after its first. */
+MGF_API mgf_status mgf_batch_autotune(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t trial_steps,
+                                      int32_t* chosen, double* ms);
 
 /* Device bytes a scene with this descriptor would hold once it has stepped
  * (solids excluded); allocates nothing. */
