@@ -400,6 +400,9 @@ constexpr size_t kMarchMinCells = size_t(1) << 20;
 // MGF_MG_AUTO at least kMarchMinCells (on smaller levels the march's serial
 // steps cost more than the launches they save); on level 0 the ascent's
 // partials of r . z must fit the scene's buffer.
+// Blocks of a fused march resident at once: two per CU.
+uint32_t march_slots(mgf_scene s) { return 2u * uint32_t(std::max(1, s->ctx->cu_count)); }
+
 bool march_on(mgf_scene s, int l, const MgDims& f, const MgDims& c) {
     const auto& d = s->desc;
     const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
@@ -410,15 +413,17 @@ bool march_on(mgf_scene s, int l, const MgDims& f, const MgDims& c) {
     const size_t n = size_t(f.nx) * f.ny * f.nz;
     if (d.mg_kernels == MGF_MG_AUTO && n < kMarchMinCells) return false;
     const size_t part_cap = size_t(std::max<uint32_t>(kMaxBlocks, col_blocks(s->g))) * 2;
-    return l > 0 || mg_march_grid(f, c) <= part_cap;
+    return l > 0 || mg_march_parts(f, c) <= part_cap;
 }
+
+mgf_status ensure_sched(mgf_scene s);
 
 mgf_status ensure_precond(mgf_scene s) {
     if (!s->z) {
         TRY(dalloc(s, &s->z, s->ncell));
         TRY(dalloc(s, &s->zt, s->ncell));
     }
-    if (s->desc.preconditioner != MGF_PRECOND_MULTIGRID || s->mg_nx >= 0) return MGF_OK;
+    if (s->desc.preconditioner != MGF_PRECOND_MULTIGRID || s->mg_nx >= 0) return ensure_sched(s);
     // hierarchy: halve every axis longer than one cell until none exceeds
     // kMgStopExtent
     int nx = s->g.nx, ny = s->g.ny, nz = s->g.nz;
@@ -452,15 +457,20 @@ mgf_status ensure_precond(mgf_scene s) {
             break;
         }
     }
-    // one byte per block of each fused level (which blocks hold fluid)
+    return ensure_sched(s);
+}
+
+// The schedule of each level fused now (mg_kernels may change between
+// solves), allocated once.
+mgf_status ensure_sched(mgf_scene s) {
     MgDims f{s->g.nx, s->g.ny, s->g.nz, 1.0f, 1.0f, 1.0f};
     for (size_t l = 0; l < s->mg.size(); ++l) {
         const MgDims& c = s->mg[l].d;
-        if (march_on(s, int(l), f, c)) {
-            uint8_t** wet = l == 0 ? &s->wet0 : &s->mg[l - 1].wet;
-            size_t* wet_n = l == 0 ? &s->wet0_n : &s->mg[l - 1].wet_n;
-            *wet_n = mg_march_grid(f, c);
-            TRY(dalloc(s, wet, *wet_n));
+        uint32_t** sched = l == 0 ? &s->sched0 : &s->mg[l - 1].sched;
+        size_t* sched_n = l == 0 ? &s->sched0_n : &s->mg[l - 1].sched_n;
+        if (!*sched && march_on(s, int(l), f, c)) {
+            *sched_n = mg_march_sched_words(f, c);
+            TRY(dalloc(s, sched, *sched_n));
         }
         f = c;
     }
@@ -484,14 +494,17 @@ mgf_status mg_setup(mgf_scene s, const float sa[3]) {
         f = L.d;
         mf = L.markers;
     }
-    // the fused levels' blocks that hold fluid, for these codes
+    // the fused levels' schedules, for these codes
     f = MgDims{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
     const uint16_t* cf = s->codes;
     for (size_t l = 0; l < s->mg.size(); ++l) {
         const MgDims& c = s->mg[l].d;
-        uint8_t** wet = l == 0 ? &s->wet0 : &s->mg[l - 1].wet;
-        size_t* wet_n = l == 0 ? &s->wet0_n : &s->mg[l - 1].wet_n;
-        if (*wet && *wet_n >= mg_march_grid(f, c)) MGF_HIP(launch_mg_march_wet(st, f, c, cf, *wet));
+        uint32_t* sched = l == 0 ? s->sched0 : s->mg[l - 1].sched;
+        const size_t sched_n = l == 0 ? s->sched0_n : s->mg[l - 1].sched_n;
+        if (sched && sched_n >= mg_march_sched_words(f, c)) {
+            MGF_HIP(launch_mg_march_wet(st, f, c, cf, sched));
+            MGF_HIP(launch_mg_march_plan(st, f, c, sched, march_slots(s)));
+        }
         f = c;
         cf = s->mg[l].codes;
     }
@@ -512,13 +525,16 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
     const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
     const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.9f;
     const int L = 1 + int(s->mg.size());
-    struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint8_t* wet; };
+    struct Lv {
+        MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint32_t* sched; double cells;
+    };
     Lv lv[17];
-    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->skip_dry ? s->wet0 : nullptr};
+    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->sched0, s->wet0_cells};
     for (int l = 1; l < L; ++l) {
         auto& m = s->mg[size_t(l - 1)];
-        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, s->skip_dry ? m.wet : nullptr};
+        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, m.sched, m.march_cells};
     }
+    const int skip = s->skip_dry ? 1 : 0;
     // levels [l0, L) run in one work-group
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
     // descent: pre-smoothing from zero (the first two sweeps in one pass),
@@ -534,10 +550,10 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         if (fused(l)) {
             // blocks without fluid store x = 0 and the coarse b = 0 only
             const double nl = double(v.d.nx) * v.d.ny * v.d.nz;
-            const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : nl;
+            const double wet = (skip && v.cells >= 0.0) ? v.cells : nl;
             probe_next_bytes(10.5 * wet + 4.5 * (nl - wet));
             MGF_HIP(launch_mg_descend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes,
-                                      const_cast<float*>(lv[l + 1].b), omega, s->scal, v.wet));
+                                      const_cast<float*>(lv[l + 1].b), omega, s->scal, v.sched, skip));
             continue;
         }
         const int sweeps = coarsest ? kCoarsestSweeps : nu;
@@ -580,10 +596,10 @@ mgf_status apply_precond(mgf_scene s, const float sa[3], const float** zout) {
         auto rz_at = [&](int k) { return (l == 0 && k == nu - 1) ? 1 : 0; };
         const double ncell = double(v.d.nx) * v.d.ny * v.d.nz;
         if (fused(l)) {
-            const double wet = (l == 0 && s->wet0_cells >= 0.0) ? s->wet0_cells : ncell;
+            const double wet = (skip && v.cells >= 0.0) ? v.cells : ncell;
             probe_next_bytes(14.5 * wet + 4.0 * (ncell - wet));
             MGF_HIP(launch_mg_ascend(st, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal,
-                                     s->red, l == 0 ? 1 : 0, v.wet));
+                                     s->red, l == 0 ? 1 : 0, v.sched, skip));
             std::swap(v.x, v.t);
             continue;
         }
@@ -641,12 +657,20 @@ mgf_status cg_mixed(mgf_scene s, const float sa[3], int64_t max_it, mgf_step_sta
         MGF_HIP(hipMemcpyAsync(h.data(), s->wet_flat, h.size(), hipMemcpyDeviceToHost, st));
         MGF_HIP(hipStreamSynchronize(st));
         s->wet_flat_cells = flat_wet_cells(s->g, h.data());
-        if (s->wet0 && !s->mg.empty()) {
-            h.resize(s->wet0_n);
-            MGF_HIP(hipMemcpyAsync(h.data(), s->wet0, h.size(), hipMemcpyDeviceToHost, st));
-            MGF_HIP(hipStreamSynchronize(st));
-            s->wet0_cells = mg_march_wet_cells(MgDims{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]},
-                                               s->mg[0].d, h.data());
+        // the cells each fused level's schedule marches
+        MgDims f{s->g.nx, s->g.ny, s->g.nz, sa[0], sa[1], sa[2]};
+        for (size_t l = 0; l < s->mg.size(); ++l) {
+            const uint32_t* sched = l == 0 ? s->sched0 : s->mg[l - 1].sched;
+            double* cells = l == 0 ? &s->wet0_cells : &s->mg[l - 1].march_cells;
+            const MgDims& c = s->mg[l].d;
+            *cells = -1.0;
+            if (sched) {
+                std::vector<uint32_t> hs(l == 0 ? s->sched0_n : s->mg[l - 1].sched_n);
+                MGF_HIP(hipMemcpyAsync(hs.data(), sched, hs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                MGF_HIP(hipStreamSynchronize(st));
+                *cells = mg_march_sched_cells(f, c, hs.data());
+            }
+            f = c;
         }
     }
     const double ncell_all = double(s->ncell);
@@ -926,13 +950,16 @@ mgf_status vcycle_records(mgf_scene s, const float sa[3], std::vector<KRec>& out
     const int nu = d.mg_sweeps > 0 ? d.mg_sweeps : 3;
     const float omega = d.mg_omega > 0.0f ? d.mg_omega : 0.9f;
     const int L = 1 + int(s->mg.size());
-    struct Lv { MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint8_t* wet; };
+    struct Lv {
+        MgDims d; const uint16_t* codes; const float* b; float* x; float* t; const uint32_t* sched; double cells;
+    };
     Lv lv[17];
-    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->skip_dry ? s->wet0 : nullptr};
+    lv[0] = Lv{d0, s->codes, s->r, s->z, s->zt, s->sched0, s->wet0_cells};
     for (int l = 1; l < L; ++l) {
         auto& m = s->mg[size_t(l - 1)];
-        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, s->skip_dry ? m.wet : nullptr};
+        lv[l] = Lv{m.d, m.codes, m.b, m.x, m.t, m.sched, m.march_cells};
     }
+    const int skip = s->skip_dry ? 1 : 0;
     const int l0 = s->mg_l0 < L ? s->mg_l0 : L;
     auto fused = [&](int l) { return l + 1 < L && march_on(s, l, lv[l].d, lv[l + 1].d); };
     std::vector<MRec> r;
@@ -942,7 +969,7 @@ mgf_status vcycle_records(mgf_scene s, const float sa[3], std::vector<KRec>& out
         if (fused(l)) {
             r.clear();
             rec_mg_descend(r, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].codes, const_cast<float*>(lv[l + 1].b),
-                           omega, s->scal, v.wet);
+                           omega, s->scal, v.sched, skip);
             keyed(out, r, 0, l, 0);
             continue;
         }
@@ -989,7 +1016,7 @@ mgf_status vcycle_records(mgf_scene s, const float sa[3], std::vector<KRec>& out
         r.clear();
         if (fused(l)) {
             rec_mg_ascend(r, v.d, lv[l + 1].d, v.codes, v.b, v.x, lv[l + 1].x, v.t, omega, s->scal, s->red,
-                          l == 0 ? 1 : 0, v.wet);
+                          l == 0 ? 1 : 0, v.sched, skip);
             keyed(out, r, 2, -l, 0);
             std::swap(v.x, v.t);
             continue;
@@ -1089,11 +1116,12 @@ mgf_status phase_records(mgf_scene s, int c, std::vector<KRec>& pre, std::vector
     const uint16_t* cf = s->codes;
     for (size_t l = 0; l < s->mg.size(); ++l) {
         const MgDims& cd = s->mg[l].d;
-        uint8_t* wet = l == 0 ? s->wet0 : s->mg[l - 1].wet;
-        const size_t wet_n = l == 0 ? s->wet0_n : s->mg[l - 1].wet_n;
-        if (wet && wet_n >= mg_march_grid(fd, cd)) {
+        uint32_t* sched = l == 0 ? s->sched0 : s->mg[l - 1].sched;
+        const size_t sched_n = l == 0 ? s->sched0_n : s->mg[l - 1].sched_n;
+        if (sched && sched_n >= mg_march_sched_words(fd, cd)) {
             r.clear();
-            rec_mg_march_wet(r, fd, cd, cf, wet);
+            rec_mg_march_wet(r, fd, cd, cf, sched);
+            rec_mg_march_plan(r, fd, cd, sched);
             keyed(pre, r, 2, int(l), 0);
         }
         fd = cd;

@@ -48,8 +48,9 @@ struct MgLevelBuf {
     float* b = nullptr;
     float* x = nullptr;
     float* t = nullptr;     // Jacobi ping-pong partner of x
-    uint8_t* wet = nullptr; // fused-march blocks with fluid (this level as the fine one)
-    size_t wet_n = 0;
+    uint32_t* sched = nullptr; // fused-march schedule (this level as the fine one)
+    size_t sched_n = 0;
+    double march_cells = -1.0; // cells its schedule marches (measurement; < 0: unknown)
 };
 
 // Kernel probes (measurement).  While a probed scene's entry point runs, its
@@ -125,8 +126,8 @@ struct mgf_scene_s {
     // preconditioner (allocated on first use)
     float* z = nullptr;            // M^-1 r
     float* zt = nullptr;           // Jacobi ping-pong partner of z
-    uint8_t* wet0 = nullptr;       // fused-march blocks of level 0 with fluid
-    size_t wet0_n = 0;
+    uint32_t* sched0 = nullptr;    // fused-march schedule of level 0
+    size_t sched0_n = 0;
     uint8_t* wet_flat = nullptr;   // blocks of the flat CG kernels with fluid
     // with probes on: cells in blocks with fluid (flat kernels, level-0
     // march) for the bytes a probed launch moved; < 0 = every cell
@@ -322,7 +323,6 @@ uint32_t col_blocks(const GridDev& g);
 hipError_t launch_flat_wet(hipStream_t s, const GridDev& g, const uint16_t* codes, uint8_t* wet);
 // Cells in the blocks a host copy of the wet bytes marks (measurement).
 double flat_wet_cells(const GridDev& g, const uint8_t* wet_host);
-double mg_march_wet_cells(const MgDims& f, const MgDims& c, const uint8_t* wet_host);
 // K_I: p = z + beta p, Ap = A z + beta Ap in place (p = z, Ap = A z on the
 // first iteration), p . Ap, alpha.
 hipError_t launch_col_p_ap(hipStream_t s, const GridDev& g, const float scale[3], const uint16_t* codes,
@@ -357,19 +357,28 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
 // Fused descent (the first three sweeps from zero and the restriction of the
 // residual) and ascent (prolongation and three sweeps; with_rz: r . z) of a
 // level whose every axis is halved (mg_march_ok).  mg_march_grid = blocks of
-// one launch (the partials the ascent writes).
+// one launch; mg_march_parts = the partials of r . z the ascent writes.
 bool mg_march_ok(const MgDims& f);
 uint32_t mg_march_grid(const MgDims& f, const MgDims& c);
-// wet: one byte per block (launch_mg_march_wet, once per set of codes):
-// blocks whose tile holds no fluid store zeros and skip the march (null: none
-// skips).
-hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet);
+uint32_t mg_march_parts(const MgDims& f, const MgDims& c);
+// The schedule of a fused level (32-bit words, mg_march_sched_words): made by
+// launch_mg_march_wet and then launch_mg_march_plan once per set of codes,
+// for `slots` resident blocks (two per CU).  It picks the planes per block
+// (the record, for many scenes at once: the static chunk) and puts the
+// blocks whose tile holds fluid first, evenly over the XCDs;
+// with skip, the others store zeros and skip the march.  Null: the static
+// split, every block marched.  Host copy: the marched cells (measurement).
+size_t mg_march_sched_words(const MgDims& f, const MgDims& c);
+hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code,
+                               uint32_t* sched);
+hipError_t launch_mg_march_plan(hipStream_t s, const MgDims& f, const MgDims& c, uint32_t* sched, uint32_t slots);
+double mg_march_sched_cells(const MgDims& f, const MgDims& c, const uint32_t* sched_host);
 hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                              float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
-                             const uint8_t* wet);
+                             const uint32_t* sched, int skip);
 hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                             const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
-                            int with_rz, const uint8_t* wet);
+                            int with_rz, const uint32_t* sched, int skip);
 size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev);
 hipError_t launch_mg_coarse_cycle(hipStream_t s, const MgDims* levels, const uint16_t* const* codes, int nlev,
                                   const float* b0, float* x0, int nu, int last_sweeps, float omega,
@@ -386,7 +395,7 @@ enum MultiKind : int32_t {
     kMkStart, kMkFinish, kMkPap, kMkUpdate, kMkRefine1, kMkRefine2, kMkAcc64, kMkSum2,
     kMkSmoothTT, kMkSmoothTF, kMkSmoothFT, kMkSmoothFF, kMkSmooth2T, kMkSmooth2F, kMkProlongT, kMkProlongF,
     kMkRestrict, kMkRestrict2, kMkRestrictTile, kMkCoarse, kMkMarchDesc, kMkMarchAsc, kMkMarchAscRz, kMkGather,
-    kMkRefine0, kMkCodes, kMkFlatWet, kMkCoarsen, kMkMarchWet,
+    kMkRefine0, kMkCodes, kMkFlatWet, kMkCoarsen, kMkMarchWet, kMkMarchPlan,
     kMkAdvectFc, kMkAdvectCc, kMkAdvectCc4, kMkMarkers, kMkRhs, kMkProject, kMkCopyWord,
     kMkExtRows, kMkExtRowsReg4, kMkExtRowsReg8, kMkExtRowsReg16, kMkExtLines1, kMkExtLines2, kMkExtZero, kMkExtCount,
     kMkExtScan, kMkExtScatter, kMkExtTail, kMkCopyOffs,
@@ -423,10 +432,10 @@ bool rec_mg_coarse_cycle(std::vector<MRec>& out, const MgDims* levels, const uin
                          const float* b0, float* x0, int nu, int last_sweeps, float omega, const CgScalars* sc);
 void rec_mg_descend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                     float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
-                    const uint8_t* wet);
+                    const uint32_t* sched, int skip);
 void rec_mg_ascend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                    const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
-                   int with_rz, const uint8_t* wet);
+                   int with_rz, const uint32_t* sched, int skip);
 // The finishing record's size, and its abs_target2 set in place.
 size_t finish_arg_bytes();
 void finish_set_target(unsigned char* args, double abs_target2);
@@ -435,7 +444,9 @@ void rec_codes(std::vector<MRec>& out, int nx, int ny, int nz, const uint8_t* ma
 void rec_flat_wet(std::vector<MRec>& out, const GridDev& g, const uint16_t* codes, uint8_t* wet);
 void rec_mg_coarsen(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint8_t* mf, uint8_t* mc,
                     int fluid_first);
-void rec_mg_march_wet(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet);
+void rec_mg_march_wet(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code,
+                      uint32_t* sched);
+void rec_mg_march_plan(std::vector<MRec>& out, const MgDims& f, const MgDims& c, uint32_t* sched);
 void rec_advect_fc(std::vector<MRec>& out, const GridDev& g, Vel in, Vel V, float dt, int order, float* ou, float* ov,
                    float* ow, int add0, const float inc0[3], int add1, const float inc1[3]);
 void rec_advect_cc(std::vector<MRec>& out, const GridDev& g, const float* in, Vel V, float dt, int order, float* o);

@@ -1563,7 +1563,38 @@ struct March {
     uint32_t tx, ty;         // tiles along x and y
     uint32_t blocks;         // tiles x chunks; blocks past it idle
     int chunk;               // planes per block (even)
+    uint32_t tiles;          // tx * ty
+    uint32_t gmax;           // blocks of a launch (any schedule's units fit)
+    uint32_t wunits;         // units of the fluid-range pass (tile x kWetChunk planes)
+    int cmin;                // shortest chunk a schedule may choose (a multiple of kRzSlab)
+    int skip;                // units without fluid store their zeros and skip the march
+    uint32_t parts;          // partials of r . z: tiles x slabs of kRzSlab planes
 };
+
+// The schedule of a fused level, made once per set of codes on the device
+// (k_mg_march_wet, then k_mg_march_plan), in one array of 32-bit words:
+//   [0] C, the planes per unit (even); [1] U, the units (tiles x chunks of C
+//   planes); [2] H, the units whose tile holds fluid in their planes;
+//   [kSchedHead + b], b < U: the unit of block b, bit 31 set when it holds
+//   fluid;
+//   [kSchedHead + gmax + w], w < wunits: planes lo | hi << 16 holding fluid
+//   in the tile's columns of the pass unit w (lo > hi: none); the plan
+//   overwrites the first `tiles` entries with each tile's range.
+// Unit u is tile u % tiles (x fastest) and chunk u / tiles, as the static
+// numbering of blocks.  The units with fluid come first in launch order and
+// in equal runs per XCD (block b runs on XCD b % 8), so that the heavy
+// blocks start at once and no XCD holds more of them than another.  Which
+// block runs a unit changes no stored value; the partial sums of r . z are
+// kept per unit, so the reduction's order does not depend on it either.
+constexpr int kSchedHead = 4;
+constexpr int kWetChunk = 32;
+constexpr uint32_t kSchedHeavy = 0x80000000u;
+// Every chunk is a multiple of kRzSlab planes, and the ascent of level 0
+// writes one partial of r . z per tile and slab of kRzSlab planes (the
+// planes summed in order, then the waves in order): the sum, and so the CG
+// iterates, do not depend on the schedule.
+constexpr int kRzSlab = 4;
+constexpr int kRzLanes = 32;   // lanes of wave 0 that add a slab's sums
 
 // Per-work-item constants of a march.
 struct MarchCol {
@@ -1572,8 +1603,9 @@ struct MarchCol {
     bool owner;              // tile column at an even x and y of the tile (PRE)
     uint32_t col, pcol;      // column offset in the level and of its parent
     int za, zs, zr;          // first plane, end of the stored planes, end of the pairs
-    int me, xm, xp, ym, yp;  // LDS index of the column and of its xy neighbours
+    int me;                  // LDS index of the column
     int wy;                  // the wave (wave-uniform); it holds rows kMRows wy ..
+    uint32_t tix;            // the tile's index (x fastest)
 };
 
 // The rolling state of a work-item: codes and b of the planes q-1 .. q-3 and
@@ -1638,10 +1670,10 @@ __device__ __forceinline__ int wave_kind(uint32_t cd) {
 
 // The four xy neighbours of the work-item in an LDS slot.
 __device__ __forceinline__ void xy_nbrs(const float* S, const MarchCol& K, float (&n)[4]) {
-    n[0] = S[K.xm];
-    n[1] = S[K.xp];
-    n[2] = S[K.ym];
-    n[3] = S[K.yp];
+    n[0] = S[K.me - 1];
+    n[1] = S[K.me + 1];
+    n[2] = S[K.me - kMX];
+    n[3] = S[K.me + kMX];
 }
 
 __device__ __forceinline__ float march_jacobi(uint32_t cd, float wd, float awc, float bc, float vc, const float* S,
@@ -1711,8 +1743,17 @@ __device__ __forceinline__ bool rows_need(int wy, int s) {
 // LDS of a march (floats): the tables aw and w, then two slots of kMThreads
 // per stage 0, 1, 2 and (PRE) for the residuals.  One array, so that every
 // access is a constant offset from the work-item's index.
+// Each slot has kSlotPad floats before and after it (shared with the next
+// slot), so that the four xy neighbours of every work-item are constant
+// offsets of its own index: the first and last row read the pads and the
+// first and last column the row beside, neither ever written in the step
+// that reads it.  Those are the outermost ring columns, whose values are
+// never needed (stage s is exact s columns inside the ring).
 constexpr int kLdsAw = 0, kLdsW = 64, kLdsSlots = 128;
-__device__ __forceinline__ constexpr int slot(int stage, int h) { return kLdsSlots + (2 * stage + h) * kMThreads; }
+constexpr int kSlotPad = kMX, kSlotStride = kMThreads + kSlotPad;
+__device__ __forceinline__ constexpr int slot(int stage, int h) {
+    return kLdsSlots + kSlotPad + (2 * stage + h) * kSlotStride;
+}
 
 // Planes loaded ahead: the loads of plane q are issued kMPre steps before
 // its step (kMPre register sets in rotation), so that about kMPre planes of
@@ -1727,7 +1768,8 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
                                            const uint16_t* __restrict__ code, const float* __restrict__ b,
                                            const float* __restrict__ xi, const float* __restrict__ xc,
                                            float* __restrict__ xo, const uint16_t* __restrict__ codec,
-                                           float* __restrict__ bco, float* lds) {
+                                           float* __restrict__ bco, float* lds, double* red,
+                                           double* __restrict__ part) {
     const float* aw = lds + kLdsAw;
     const float* w = lds + kLdsW;
     const int q = K.za - 3 + t;
@@ -1778,7 +1820,38 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     R.v0b = R.v0a; R.v0a = v0;
     R.v1b = R.v1a; R.v1a = v1;
     R.v2b = R.v2a; R.v2a = v2;
+    // RZ: the last stored plane of a slab (the same for the whole block)
+    // closes the slab's partial.  Every work-item leaves its sum in red;
+    // after the barrier, lane l < 32 of wave 0 adds entries l, l + 32, ..
+    // in order, and lane 0 adds those 32 sums in order.  Only LDS and
+    // additions: no cross-lane operation holds registers in the march.
+    int slab = -1;
+    if constexpr (RZ) {
+        const int p = q - 3;
+        if (p >= K.za && p < K.zs && ((p + 1) % kRzSlab == 0 || p + 1 == K.zs)) {
+            red[threadIdx.x] = R.acc;
+            R.acc = 0.0;
+            slab = p / kRzSlab;
+        }
+    }
     __syncthreads();
+    if constexpr (RZ) {
+        if (slab >= 0 && threadIdx.x < kRzLanes) {
+            double t = 0.0;
+#pragma unroll 4
+            for (int k = 0; k < kMThreads / kRzLanes; ++k) t += red[threadIdx.x + kRzLanes * k];
+            red[kMThreads + threadIdx.x] = t;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (threadIdx.x == 0) {
+                double u = 0.0;
+#pragma unroll 4
+                for (int k = 0; k < kRzLanes; ++k) u += red[kMThreads + k];
+                part[K.tix + M.tiles * uint32_t(slab)] = u;
+            }
+        }
+    }
 }
 
 // The tile of block blk: columns [x0, x1) x [y0, y1), planes [za, zs) and the
@@ -1800,7 +1873,7 @@ __device__ __forceinline__ MarchTile march_tile(const March& M, uint32_t blk) {
     return T;
 }
 
-// A block whose tile holds no fluid cell in its planes: every value it
+// A unit whose tile holds no fluid cell in its planes: every value it
 // stores is +0 (the sweeps give 0 on non-fluid cells, and so does the
 // residual, hence each restricted pair sum), so it stores the zeros and
 // skips the march.
@@ -1823,53 +1896,178 @@ __device__ __forceinline__ void march_dry(const March& M, uint32_t blk, float* _
     }
 }
 
-// wet[blk] = 1 when block blk's tile holds a fluid cell in its planes (the
-// codes do not change during a pressure solve: one pass per solve).
+// Pass unit w (tile w % tiles, planes [kWetChunk k, kWetChunk (k + 1)) with
+// k = w / tiles): the first and last plane holding a fluid cell in the
+// tile's columns (the codes do not change during a pressure solve: one pass
+// per solve).
 __device__ __forceinline__ void k_mg_march_wet_run(March M, const uint16_t* __restrict__ code,
-                                                            uint8_t* __restrict__ wet, uint32_t blk) {
-        if (blk >= M.blocks) return;
-    const MarchTile T = march_tile(M, blk);
-    const int w = T.x1 - T.x0, h = T.y1 - T.y0;
-    int any = 0;
-    for (int e = int(threadIdx.x); e < w * h; e += kMThreads) {
-        const uint32_t col = uint32_t(T.y0 + e / w) * M.sy + uint32_t(T.x0 + e % w);
-        for (int z = T.za; z < T.zs; ++z) any |= code[uint32_t(z) * M.sz + col] & kCodeFluid;
+                                                   uint32_t* __restrict__ sched, uint32_t blk) {
+    __shared__ int lo_s, hi_s;
+    if (blk >= M.wunits) return;
+    if (threadIdx.x == 0) {
+        lo_s = 0xFFFF;
+        hi_s = 0;
     }
-    any = __syncthreads_or(any);
-    if (threadIdx.x == 0) wet[blk] = uint8_t(any != 0);
+    __syncthreads();
+    const uint32_t t = blk % M.tiles, k = blk / M.tiles;
+    const int x0 = int(t % M.tx) * kMTX, y0 = int(t / M.tx) * kMTY;
+    const int w = min(kMTX, M.nx - x0), h = min(kMTY, M.ny - y0);
+    const int z0 = int(k) * kWetChunk, z1 = min(z0 + kWetChunk, M.nz);
+    int lo = 0xFFFF, hi = 0;
+    for (int e = int(threadIdx.x); e < w * h; e += kMThreads) {
+        const uint32_t col = uint32_t(y0 + e / w) * M.sy + uint32_t(x0 + e % w);
+        for (int z = z0; z < z1; ++z)
+            if (code[uint32_t(z) * M.sz + col] & kCodeFluid) {
+                lo = min(lo, z);
+                hi = max(hi, z);
+            }
+    }
+    if (lo <= hi) {
+        atomicMin(&lo_s, lo);
+        atomicMax(&hi_s, hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) sched[kSchedHead + M.gmax + blk] = uint32_t(lo_s) | (uint32_t(hi_s) << 16);
 }
 
 __global__ void __launch_bounds__(kMThreads) k_mg_march_wet(March M, const uint16_t* __restrict__ code,
-                                                            uint8_t* __restrict__ wet) {
-    k_mg_march_wet_run(M, code, wet, xcd_block());
+                                                            uint32_t* __restrict__ sched) {
+    k_mg_march_wet_run(M, code, sched, blockIdx.x);
 }
 
 struct MMarchWet {
     static constexpr int kThreads = kMThreads;
     March M;
     const uint16_t* code;
-    uint8_t* wet;
+    uint32_t* sched;
     uint32_t G;
-    __device__ void run(uint32_t vb) const { k_mg_march_wet_run(M, code, wet, xcd_of(vb, G)); }
+    __device__ void run(uint32_t vb) const { k_mg_march_wet_run(M, code, sched, vb); }
+};
+
+// Units of tile range r (lo | hi << 16) at chunk length C: the chunks its
+// fluid planes meet.
+__device__ __forceinline__ uint32_t range_units(uint32_t r, uint32_t C) {
+    const uint32_t lo = r & 0xFFFFu, hi = r >> 16;
+    return lo > hi ? 0u : hi / C - lo / C + 1u;
+}
+
+// The schedule (one work-group): each tile's fluid range from its pass
+// units; the chunk length C (a multiple of kRzSlab, at least M.cmin) with the least
+// ceil(H / slots) x (C + 7), the units with fluid H filling `slots` resident
+// blocks (two per CU) per round and a unit taking about C + 7 plane steps
+// (ties: the longer chunk); then the launch order.
+constexpr int kPlanThreads = 1024;
+
+__device__ __forceinline__ void k_mg_march_plan_run(const March& M, uint32_t* __restrict__ sched,
+                                                    uint32_t slots, uint32_t force_c) {
+    __shared__ unsigned long long best;
+    __shared__ uint32_t part[kPlanThreads];
+    uint32_t* rng = sched + kSchedHead + M.gmax;
+    const uint32_t wch = M.wunits / M.tiles;
+    if (threadIdx.x == 0) best = ~0ull;
+    for (uint32_t t = threadIdx.x; t < M.tiles; t += kPlanThreads) {
+        uint32_t lo = 0xFFFFu, hi = 0u;
+        for (uint32_t k = 0; k < wch; ++k) {
+            const uint32_t r = rng[t + k * M.tiles];
+            lo = min(lo, r & 0xFFFFu);
+            hi = max(hi, r >> 16);
+        }
+        rng[t] = lo <= hi ? (lo | (hi << 16)) : 0xFFFFu;
+    }
+    __syncthreads();
+    const uint32_t cmax = (uint32_t(M.nz) + kRzSlab - 1) / kRzSlab * kRzSlab;
+    for (uint32_t C = uint32_t(M.cmin) + kRzSlab * threadIdx.x; C <= max(cmax, uint32_t(M.cmin)) && !force_c;
+         C += kRzSlab * kPlanThreads) {
+        uint32_t H = 0;
+        for (uint32_t t = 0; t < M.tiles; ++t) H += range_units(rng[t], C);
+        const unsigned long long cost = (unsigned long long)((H + slots - 1) / slots) * (C + 7u);
+        atomicMin(&best, (cost << 20) | (0xFFFFFu - C));
+    }
+    __syncthreads();
+    const uint32_t C = force_c ? force_c : 0xFFFFFu - uint32_t(best & 0xFFFFFull);
+    const uint32_t nch = (uint32_t(M.nz) + C - 1) / C, U = M.tiles * nch;
+    // heavy units counted per contiguous segment of units, then scanned
+    const uint32_t seg = (U + kPlanThreads - 1) / kPlanThreads;
+    const uint32_t u0 = min(U, threadIdx.x * seg), u1 = min(U, u0 + seg);
+    auto heavy = [&](uint32_t u) {
+        const uint32_t r = rng[u % M.tiles], k = u / M.tiles;
+        const uint32_t lo = r & 0xFFFFu, hi = r >> 16;
+        return lo <= hi && k >= lo / C && k <= hi / C;
+    };
+    uint32_t n = 0;
+    for (uint32_t u = u0; u < u1; ++u) n += heavy(u) ? 1u : 0u;
+    part[threadIdx.x] = n;
+    __syncthreads();
+    for (int off = 1; off < kPlanThreads; off <<= 1) {
+        const uint32_t v = threadIdx.x >= uint32_t(off) ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t H = part[kPlanThreads - 1];
+    uint32_t hk = part[threadIdx.x] - n;     // heavy units before u0
+    uint32_t dk = u0 - hk;                   // units without fluid before u0
+    // Blocks [0, H) run the heavy units, [H, U) the others; XCD j runs the
+    // blocks b = j (mod 8), each a contiguous run of units.
+    auto place = [&](uint32_t k, uint32_t first, uint32_t count) {
+        // the k-th of `count` units given to blocks [first, first + count)
+        uint32_t before = 0;
+        for (uint32_t j = 0; j < kXcds; ++j) {
+            const uint32_t a = (first + kXcds - 1 - j) / kXcds;            // first m with j + 8 m >= first
+            const uint32_t e = (first + count + kXcds - 1 - j) / kXcds;    // end m
+            const uint32_t cj = e > a ? e - a : 0u;
+            if (k < before + cj) return j + kXcds * (a + (k - before));
+            before += cj;
+        }
+        return 0u;
+    };
+    for (uint32_t u = u0; u < u1; ++u) {
+        if (heavy(u)) {
+            sched[kSchedHead + place(hk, 0u, H)] = u | kSchedHeavy;
+            ++hk;
+        } else {
+            sched[kSchedHead + place(dk, H, U - H)] = u;
+            ++dk;
+        }
+    }
+    if (threadIdx.x == 0) {
+        sched[0] = C;
+        sched[1] = U;
+        sched[2] = H;
+        sched[3] = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(kPlanThreads) k_mg_march_plan(March M, uint32_t* __restrict__ sched,
+                                                               uint32_t slots, uint32_t force_c) {
+    k_mg_march_plan_run(M, sched, slots, force_c);
+}
+
+struct MMarchPlan {
+    static constexpr int kThreads = kPlanThreads;
+    March M;
+    uint32_t* sched;
+    uint32_t slots;
+    uint32_t force_c;
+    __device__ void run(uint32_t) const { k_mg_march_plan_run(M, sched, slots, force_c); }
 };
 
 template <bool POST, bool RZ>
-__device__ __forceinline__ void march_run(const March& M, const MgLevel& L, const uint16_t* __restrict__ code,
-                                          const float* __restrict__ b, const float* __restrict__ xi,
-                                          const float* __restrict__ xc, float* __restrict__ xo,
-                                          const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
-                                          const CgScalars* __restrict__ sc, double* __restrict__ part,
-                                          const uint8_t* __restrict__ wet, uint32_t blk, uint32_t slot) {
-    __shared__ float lds[kLdsSlots + (POST ? 6 : 8) * kMThreads];
-    __shared__ double red[kMThreads / 64];
-    if (sc->done) return;
-    if (blk >= M.blocks) {
-        if (RZ && threadIdx.x == 0) part[slot] = 0.0;
-        return;
-    }
-    if (wet && !wet[blk]) {
+__device__ __forceinline__ void march_unit(const March& M, const MgLevel& L, const uint16_t* __restrict__ code,
+                                           const float* __restrict__ b, const float* __restrict__ xi,
+                                           const float* __restrict__ xc, float* __restrict__ xo,
+                                           const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
+                                           double* __restrict__ part, uint32_t blk, bool heavy, float* lds,
+                                           double* red) {
+    if (M.skip && !heavy) {
         march_dry<POST>(M, blk, xo, bco);
-        if (RZ && threadIdx.x == 0) part[slot] = 0.0;
+        if (RZ) {
+            // the unit's slabs sum to +0
+            const MarchTile T = march_tile(M, blk);
+            const uint32_t t = blk % M.tiles;
+            for (int k = T.za / kRzSlab + int(threadIdx.x); k * kRzSlab < T.zs; k += kMThreads)
+                part[t + M.tiles * uint32_t(k)] = 0.0;
+        }
         return;
     }
     build_tables<true>(lds + kLdsAw, lds + kLdsW, L.sx, L.sy, L.sz, omega);
@@ -1888,11 +2086,8 @@ __device__ __forceinline__ void march_run(const March& M, const MgLevel& L, cons
         K.zr = K.za + M.chunk;
         K.zs = K.zr < M.nz ? K.zr : M.nz;
         K.me = int(threadIdx.x);
-        K.xm = cx > 0 ? K.me - 1 : K.me;
-        K.xp = cx < kMX - 1 ? K.me + 1 : K.me;
-        K.ym = cy > 0 ? K.me - kMX : K.me;
-        K.yp = cy < kMY - 1 ? K.me + kMX : K.me;
         K.wy = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / 64);
+        K.tix = blk % M.tiles;
     }
     MarchRing R;
     R.c1 = R.c2 = R.c3 = 0u;
@@ -1917,22 +2112,41 @@ __device__ __forceinline__ void march_run(const March& M, const MgLevel& L, cons
 #pragma unroll
         for (int k = 0; k < kMPre; k += 2) {
             march_step<POST, RZ, 0>(M, L, K, t + k, cd[k], bv[k], xv[k], ev[k], R, code, b, xi, xc, xo, codec,
-                                    bco, lds);
+                                    bco, lds, red, part);
             march_step<POST, RZ, 1>(M, L, K, t + k + 1, cd[k + 1], bv[k + 1], xv[k + 1], ev[k + 1], R, code, b, xi,
-                                    xc, xo, codec, bco, lds);
+                                    xc, xo, codec, bco, lds, red, part);
         }
     }
-    if constexpr (RZ) {
-        const double a = wave_sum(R.acc);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-#pragma unroll
-            for (int q = 0; q < kMThreads / 64; ++q) t += red[q];
-            part[slot] = t;
-        }
+}
+
+
+// The unit of block `slot`: from the schedule (in launch order), else the
+// static unit vblk (XCD-remapped), every unit marched.  A launch has a block
+// for every unit of the schedule it follows: each solve makes its levels'
+// schedules before its first V-cycle (a scene alone: any chunk, gmax
+// blocks; many scenes' records: the static chunk, the static blocks).
+template <bool POST, bool RZ>
+__device__ __forceinline__ void march_run(const March& M0, const MgLevel& L, const uint16_t* __restrict__ code,
+                                          const float* __restrict__ b, const float* __restrict__ xi,
+                                          const float* __restrict__ xc, float* __restrict__ xo,
+                                          const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
+                                          const CgScalars* __restrict__ sc, double* __restrict__ part,
+                                          const uint32_t* __restrict__ sched, uint32_t vblk, uint32_t slot) {
+    __shared__ float lds[kLdsSlots + kSlotPad + (POST ? 6 : 8) * kSlotStride];
+    __shared__ double red[RZ ? kMThreads + kRzLanes : 1];
+    if (sc->done) return;
+    if (!sched) {
+        if (vblk < M0.blocks) march_unit<POST, RZ>(M0, L, code, b, xi, xc, xo, codec, bco, omega, part, vblk, true, lds, red);
+        return;
     }
+    const uint32_t U = sched[1];
+    if (slot >= U) return;
+    March M = M0;
+    M.chunk = int(sched[0]);
+    M.blocks = U;
+    const uint32_t e = sched[kSchedHead + slot];
+    march_unit<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, part, e & ~kSchedHeavy,
+                         (e & kSchedHeavy) != 0u, lds, red);
 }
 
 template <bool POST, bool RZ>
@@ -1940,8 +2154,8 @@ __global__ void __launch_bounds__(kMThreads) __attribute__((amdgpu_waves_per_eu(
 k_mg_march(March M, MgLevel L, const uint16_t* __restrict__ code, const float* __restrict__ b,
            const float* __restrict__ xi, const float* __restrict__ xc, float* __restrict__ xo,
            const uint16_t* __restrict__ codec, float* __restrict__ bco, float omega,
-           const CgScalars* __restrict__ sc, double* __restrict__ part, const uint8_t* __restrict__ wet) {
-    march_run<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, sc, part, wet, xcd_block(), blockIdx.x);
+           const CgScalars* __restrict__ sc, double* __restrict__ part, const uint32_t* __restrict__ sched) {
+    march_run<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, sc, part, sched, xcd_block(), blockIdx.x);
 }
 
 template <bool POST, bool RZ>
@@ -1957,10 +2171,10 @@ struct MMarch {
     float omega;
     const CgScalars* sc;
     double* part;
-    const uint8_t* wet;
+    const uint32_t* sched;
     uint32_t G;
     __device__ void run(uint32_t blk) const {
-        march_run<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, sc, part, wet, xcd_of(blk, G), blk);
+        march_run<POST, RZ>(M, L, code, b, xi, xc, xo, codec, bco, omega, sc, part, sched, xcd_of(blk, G), blk);
     }
 };
 
@@ -2135,15 +2349,17 @@ hipError_t launch_mg_smooth_prolong(hipStream_t s, const MgDims& f, const MgDims
     return finish(s, red.partials, G, sc, kFinishRz);
 }
 
-// Fused descent / ascent.  Chunks of about kMarchChunk planes; more chunks
-// (down to 8 planes) until the grid holds about kMarchBlocks blocks, two per
-// CU.
-constexpr int kMarchChunk = 32;   // 16 .. 66 measured within a few percent
+// Fused descent / ascent.  The static split (no schedule): chunks of about
+// kMarchChunk planes; more chunks (down to 8 planes) until the grid holds
+// about kMarchBlocks blocks, two per CU.  A schedule's chunk is at least
+// cmin, the shortest that keeps its units within kMarchMaxUnits.
+constexpr int kMarchChunk = 32;
 constexpr uint32_t kMarchBlocks = 512;
+constexpr size_t kMarchMaxUnits = 2048;
 
-bool mg_march_ok(const MgDims& f) { return f.nx > 1 && f.ny > 1 && f.nz > 1; }
+bool mg_march_ok(const MgDims& f) { return f.nx > 1 && f.ny > 1 && f.nz > 1 && f.nz < 0xFFFF; }
 
-static March make_march(const MgDims& f, const MgDims& c) {
+static March make_march(const MgDims& f, const MgDims& c, int skip = 1) {
     March M;
     M.nx = f.nx; M.ny = f.ny; M.nz = f.nz;
     M.sy = uint32_t(f.nx);
@@ -2158,66 +2374,94 @@ static March make_march(const MgDims& f, const MgDims& c) {
     const int want = int((kMarchBlocks + xy - 1) / xy);
     chunks = std::max(chunks, std::min(want, (f.nz + 7) / 8));
     int C = (f.nz + chunks - 1) / chunks;
-    C += C & 1;
+    C = (C + kRzSlab - 1) / kRzSlab * kRzSlab;
     M.chunk = C;
     M.blocks = xy * uint32_t((f.nz + C - 1) / C);
+    M.tiles = xy;
+    M.wunits = xy * uint32_t((f.nz + kWetChunk - 1) / kWetChunk);
+    int cm = kRzSlab;
+    while (cm < f.nz && size_t(xy) * size_t((f.nz + cm - 1) / cm) > std::max<size_t>(kMarchMaxUnits, xy))
+        cm += kRzSlab;
+    M.cmin = cm;
+    M.parts = xy * uint32_t((f.nz + kRzSlab - 1) / kRzSlab);
+    const uint32_t units = std::max(xy * uint32_t((f.nz + cm - 1) / cm), M.blocks);
+    M.gmax = (units + kXcds - 1) / kXcds * kXcds;
+    M.skip = skip;
     return M;
 }
 
-uint32_t mg_march_grid(const MgDims& f, const MgDims& c) {
+uint32_t mg_march_grid(const MgDims& f, const MgDims& c) { return make_march(f, c).gmax; }
+
+uint32_t mg_march_parts(const MgDims& f, const MgDims& c) { return make_march(f, c).parts; }
+
+size_t mg_march_sched_words(const MgDims& f, const MgDims& c) {
     const March M = make_march(f, c);
-    return (M.blocks + kXcds - 1) / kXcds * kXcds;
+    return size_t(kSchedHead) + M.gmax + M.wunits;
 }
 
-double mg_march_wet_cells(const MgDims& f, const MgDims& c, const uint8_t* wet_host) {
+static uint32_t march_wet_grid(const March& M) { return (M.wunits + kXcds - 1) / kXcds * kXcds; }
+
+double mg_march_sched_cells(const MgDims& f, const MgDims& c, const uint32_t* sched_host) {
     const March M = make_march(f, c);
+    const int C = int(sched_host[0]);
+    const uint32_t U = sched_host[1];
+    if (C <= 0 || U > M.gmax) return -1.0;
     double cells = 0.0;
-    for (uint32_t blk = 0; blk < M.blocks; ++blk) {
-        if (!wet_host[blk]) continue;
-        const uint32_t tx = blk % M.tx, rest = blk / M.tx, ty = rest % M.ty, tc = rest / M.ty;
+    for (uint32_t b = 0; b < U; ++b) {
+        const uint32_t e = sched_host[kSchedHead + b];
+        if (!(e & kSchedHeavy)) continue;
+        const uint32_t u = e & ~kSchedHeavy;
+        const uint32_t tx = u % M.tx, rest = u / M.tx, ty = rest % M.ty, tc = rest / M.ty;
         const int w = std::min(kMTX, M.nx - int(tx) * kMTX), h = std::min(kMTY, M.ny - int(ty) * kMTY);
-        const int d = std::min(M.chunk, M.nz - int(tc) * M.chunk);
+        const int d = std::min(C, M.nz - int(tc) * C);
         cells += double(w) * h * d;
     }
     return cells;
 }
 
-hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet) {
-    if (!mg_march_ok(f) || !wet || !code) return hipErrorInvalidValue;
+hipError_t launch_mg_march_wet(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code,
+                               uint32_t* sched) {
+    if (!mg_march_ok(f) || !sched || !code) return hipErrorInvalidValue;
     const March M = make_march(f, c);
-    (void)launch_dims(k_mg_march_wet, "k_mg_march_wet", dim3(mg_march_grid(f, c)), dim3(kMThreads), 0, s, M, code, wet);
-    return hipGetLastError();
+    return launch_dims(k_mg_march_wet, "k_mg_march_wet", dim3(march_wet_grid(M)), dim3(kMThreads), 0u, s, M, code,
+                       sched);
+}
+
+hipError_t launch_mg_march_plan(hipStream_t s, const MgDims& f, const MgDims& c, uint32_t* sched, uint32_t slots) {
+    if (!mg_march_ok(f) || !sched || slots == 0) return hipErrorInvalidValue;
+    const March M = make_march(f, c);
+    return launch_dims(k_mg_march_plan, "k_mg_march_plan", dim3(1), dim3(kPlanThreads), 0u, s, M, sched, slots, 0u);
 }
 
 hipError_t launch_mg_descend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                              float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
-                             const uint8_t* wet) {
+                             const uint32_t* sched, int skip) {
     if (!mg_march_ok(f)) return hipErrorInvalidValue;
-    const March M = make_march(f, c);
-    const uint32_t G = mg_march_grid(f, c);
+    const March M = make_march(f, c, skip);
+    const uint32_t G = sched ? M.gmax : (M.blocks + kXcds - 1) / kXcds * kXcds;
     return launch_dims(k_mg_march<false, false>, "k_mg_march", dim3(G), dim3(kMThreads), 0u, s, M, f, code, b,
                        static_cast<const float*>(nullptr), static_cast<const float*>(nullptr), x, codec, bc, omega,
-                       sc, static_cast<double*>(nullptr), wet);
+                       sc, static_cast<double*>(nullptr), sched);
 }
 
 hipError_t launch_mg_ascend(hipStream_t s, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                             const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
-                            int with_rz, const uint8_t* wet) {
+                            int with_rz, const uint32_t* sched, int skip) {
     if (!mg_march_ok(f)) return hipErrorInvalidValue;
-    const March M = make_march(f, c);
-    const uint32_t G = mg_march_grid(f, c);
+    const March M = make_march(f, c, skip);
+    const uint32_t G = sched ? M.gmax : (M.blocks + kXcds - 1) / kXcds * kXcds;
     const CgScalars* csc = sc;
     hipError_t e;
     if (with_rz)
         e = launch_dims(k_mg_march<true, true>, "k_mg_march", dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
                         static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
-                        red.partials, wet);
+                        red.partials, sched);
     else
         e = launch_dims(k_mg_march<true, false>, "k_mg_march", dim3(G), dim3(kMThreads), 0u, s, M, f, code, b, xi, xc, xo,
                         static_cast<const uint16_t*>(nullptr), static_cast<float*>(nullptr), omega, csc,
-                        static_cast<double*>(nullptr), wet);
+                        static_cast<double*>(nullptr), sched);
     if (e != hipSuccess || !with_rz) return e;
-    return finish(s, red.partials, G, sc, kFinishRz);
+    return finish(s, red.partials, M.parts, sc, kFinishRz);
 }
 
 size_t mg_coarse_lds_bytes(const MgDims* levels, int nlev) {
@@ -2362,10 +2606,22 @@ void rec_mg_coarsen(std::vector<MRec>& out, const MgDims& f, const MgDims& c, co
     push_rec(out, kMkCoarsen, MCoarsen{f, c, mf, mc, fluid_first, n, G}, G);
 }
 
-void rec_mg_march_wet(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, uint8_t* wet) {
+void rec_mg_march_wet(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code,
+                      uint32_t* sched) {
     const March M = make_march(f, c);
-    const uint32_t G = mg_march_grid(f, c);
-    push_rec(out, kMkMarchWet, MMarchWet{M, code, wet, G}, G);
+    const uint32_t G = march_wet_grid(M);
+    push_rec(out, kMkMarchWet, MMarchWet{M, code, sched, G}, G);
+}
+
+// A record's schedule has the static chunk, and without one the units are
+// the static blocks: its launches need only those.
+static uint32_t rec_march_grid(const March& M, const uint32_t*) { return (M.blocks + kXcds - 1) / kXcds * kXcds; }
+
+void rec_mg_march_plan(std::vector<MRec>& out, const MgDims& f, const MgDims& c, uint32_t* sched) {
+    // many scenes share the device: the static chunk (short blocks that pack
+    // the launch), ordered like a scene's own schedule
+    const March M = make_march(f, c);
+    push_rec(out, kMkMarchPlan, MMarchPlan{M, sched, 8u, uint32_t(M.chunk)}, 1u);
 }
 
 void rec_mg_smooth(std::vector<MRec>& out, const MgDims& l, const uint16_t* code, const float* b, const float* xi,
@@ -2450,25 +2706,25 @@ bool rec_mg_coarse_cycle(std::vector<MRec>& out, const MgDims* levels, const uin
 
 void rec_mg_descend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                     float* x, const uint16_t* codec, float* bc, float omega, const CgScalars* sc,
-                    const uint8_t* wet) {
-    const March M = make_march(f, c);
-    const uint32_t G = mg_march_grid(f, c);
+                    const uint32_t* sched, int skip) {
+    const March M = make_march(f, c, skip);
+    const uint32_t G = rec_march_grid(M, sched);
     push_rec(out, kMkMarchDesc,
-             MMarch<false, false>{M, f, code, b, nullptr, nullptr, x, codec, bc, omega, sc, nullptr, wet, G}, G);
+             MMarch<false, false>{M, f, code, b, nullptr, nullptr, x, codec, bc, omega, sc, nullptr, sched, G}, G);
 }
 
 void rec_mg_ascend(std::vector<MRec>& out, const MgDims& f, const MgDims& c, const uint16_t* code, const float* b,
                    const float* xi, const float* xc, float* xo, float omega, CgScalars* sc, ReduceBuf red,
-                   int with_rz, const uint8_t* wet) {
-    const March M = make_march(f, c);
-    const uint32_t G = mg_march_grid(f, c);
+                   int with_rz, const uint32_t* sched, int skip) {
+    const March M = make_march(f, c, skip);
+    const uint32_t G = rec_march_grid(M, sched);
     if (with_rz) {
         push_rec(out, kMkMarchAscRz,
-                 MMarch<true, true>{M, f, code, b, xi, xc, xo, nullptr, nullptr, omega, sc, red.partials, wet, G}, G);
-        rec_finish(out, red.partials, G, sc, kFinishRz);
+                 MMarch<true, true>{M, f, code, b, xi, xc, xo, nullptr, nullptr, omega, sc, red.partials, sched, G}, G);
+        rec_finish(out, red.partials, M.parts, sc, kFinishRz);
     } else {
         push_rec(out, kMkMarchAsc,
-                 MMarch<true, false>{M, f, code, b, xi, xc, xo, nullptr, nullptr, omega, sc, nullptr, wet, G}, G);
+                 MMarch<true, false>{M, f, code, b, xi, xc, xo, nullptr, nullptr, omega, sc, nullptr, sched, G}, G);
     }
 }
 
@@ -2540,6 +2796,7 @@ hipError_t launch_multi(hipStream_t s, int32_t kind, const void* tab, const uint
         case kMkFlatWet: return launch_m<MFlatWet>(s, "k_multi_flat_wet", tab, map, nblocks, 0, active);
         case kMkCoarsen: return launch_m<MCoarsen>(s, "k_multi_mg_coarsen", tab, map, nblocks, 0, active);
         case kMkMarchWet: return launch_m<MMarchWet>(s, "k_multi_mg_march_wet", tab, map, nblocks, 0, active);
+        case kMkMarchPlan: return launch_m<MMarchPlan>(s, "k_multi_mg_march_plan", tab, map, nblocks, 0, active);
         default: return launch_multi_k(s, kind, tab, map, nblocks, shmem, active);
     }
 }
