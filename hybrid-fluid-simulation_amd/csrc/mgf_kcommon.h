@@ -92,14 +92,26 @@ inline hipError_t launch_dims(K kern, const char* name, dim3 grid, dim3 block, u
 // of the scene's own launch, so each scene computes bitwise what it computes
 // alone, partial sums included.
 
+// An argument struct that indexes its own arrays at run time declares
+// kByRef: k_multi then runs it in place in the table (a copy would sit in
+// scratch memory).
+template <class A>
+constexpr auto multi_by_ref(int) -> decltype(A::kByRef, bool()) { return A::kByRef; }
+template <class A>
+constexpr bool multi_by_ref(long) { return false; }
+
 template <class A>
 __global__ void __launch_bounds__(A::kThreads) k_multi(const A* __restrict__ tab, const uint32_t* __restrict__ map,
                                                        const uint8_t* __restrict__ active) {
     const uint32_t e = map[blockIdx.x];
     const uint32_t s = e >> kMapShift;
     if (active && !active[s]) return;
-    const A a = tab[s];
-    a.run(e & kMapLocal);
+    if constexpr (multi_by_ref<A>(0)) {
+        tab[s].run(e & kMapLocal);   // arrays indexed at run time stay in the table
+    } else {
+        const A a = tab[s];
+        a.run(e & kMapLocal);
+    }
 }
 
 template <class K, class... A>

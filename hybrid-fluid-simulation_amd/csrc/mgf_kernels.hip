@@ -62,8 +62,12 @@ __device__ __forceinline__ void k_advect_fc_run(GridDev g, Vel in, Vel V, RkCoef
     backtrace(V, g, order, k, px, py, pz);
     const float* src = c == 0 ? in.u : (c == 1 ? in.v : in.w);
     float val = sample_comp(src, c, g, px, py, pz);
-    if (inc.add_a) val = val + inc.a[c];
-    if (inc.add_b) val = val + inc.b[c];
+    // the component's increments by selection (an index at run time would
+    // copy the struct to scratch memory)
+    const float ia = c == 0 ? inc.a[0] : (c == 1 ? inc.a[1] : inc.a[2]);
+    const float ib = c == 0 ? inc.b[0] : (c == 1 ? inc.b[1] : inc.b[2]);
+    if (inc.add_a) val = val + ia;
+    if (inc.add_b) val = val + ib;
     float* dst = c == 0 ? ou : (c == 1 ? ov : ow);
     dst[i] = val;
 }

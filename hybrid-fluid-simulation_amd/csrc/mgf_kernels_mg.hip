@@ -1407,28 +1407,34 @@ __device__ __forceinline__ void lds_sweep(const CoarseLevel& L, const uint16_t* 
 __device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars* __restrict__ sc) {
     if (sc->done) return;
     extern __shared__ float lds[];
+    // Offsets (floats from lds) of each level's codes, right-hand side,
+    // iterate and scratch iterate, in LDS rather than in per-work-item arrays
+    // (which a runtime level index would put in scratch memory); par bit l
+    // swaps level l's iterate and scratch iterate.
+    __shared__ uint32_t s_code[kCoarseMaxLevels], s_b[kCoarseMaxLevels], s_x[kCoarseMaxLevels],
+        s_t[kCoarseMaxLevels], s_n[kCoarseMaxLevels];
     float* tab = lds;
-    uint16_t* codes[kCoarseMaxLevels];
-    float* xs[kCoarseMaxLevels];
-    float* ts[kCoarseMaxLevels];
-    const float* bs[kCoarseMaxLevels];
-    uint32_t ns[kCoarseMaxLevels];
-    float* q = lds + 128 * A.nlev;
-    for (int l = 0; l < A.nlev; ++l) {
-        ns[l] = uint32_t(A.lv[l].nx) * uint32_t(A.lv[l].ny) * uint32_t(A.lv[l].nz);
-        codes[l] = reinterpret_cast<uint16_t*>(q);
-        q += (ns[l] + 1) / 2;
-    }
-    for (int l = 0; l < A.nlev; ++l) {
-        if (l == 0) {
-            bs[0] = A.b0;
-        } else {
-            bs[l] = q;
-            q += ns[l];
+    if (threadIdx.x == 0) {
+        uint32_t q = 128u * uint32_t(A.nlev);
+        for (int l = 0; l < A.nlev; ++l) {
+            s_n[l] = uint32_t(A.lv[l].nx) * uint32_t(A.lv[l].ny) * uint32_t(A.lv[l].nz);
+            s_code[l] = q;
+            q += (s_n[l] + 1) / 2;
         }
-        xs[l] = q; q += ns[l];
-        ts[l] = q; q += ns[l];
+        for (int l = 0; l < A.nlev; ++l) {
+            s_b[l] = q;
+            if (l > 0) q += s_n[l];
+            s_x[l] = q; q += s_n[l];
+            s_t[l] = q; q += s_n[l];
+        }
     }
+    __syncthreads();
+    uint32_t par = 0;
+    auto codes = [&](int l) { return reinterpret_cast<uint16_t*>(lds + s_code[l]); };
+    auto bs = [&](int l) { return l == 0 ? A.b0 : static_cast<const float*>(lds + s_b[l]); };
+    auto xs = [&](int l) { return lds + (((par >> l) & 1u) ? s_t[l] : s_x[l]); };
+    auto ts = [&](int l) { return lds + (((par >> l) & 1u) ? s_x[l] : s_t[l]); };
+    auto ns = [&](int l) { return s_n[l]; };
     // tables and codes of every level
     for (int l = 0; l < A.nlev; ++l) {
         if (threadIdx.x < 64) {
@@ -1436,7 +1442,7 @@ __device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars*
             tab[128 * l + threadIdx.x] = d;
             tab[128 * l + 64 + threadIdx.x] = d != 0.0f ? A.omega / d : 0.0f;
         }
-        for (uint32_t c = threadIdx.x; c < ns[l]; c += kCoarseThreads) codes[l][c] = A.lv[l].code[c];
+        for (uint32_t c = threadIdx.x; c < ns(l); c += kCoarseThreads) codes(l)[c] = A.lv[l].code[c];
     }
     __syncthreads();
     // descent
@@ -1445,21 +1451,21 @@ __device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars*
         const float* aw = tab + 128 * l;
         const float* w = aw + 64;
         const int sweeps = l == A.nlev - 1 ? A.last_sweeps : A.nu;
-        lds_sweep(L, codes[l], aw, w, bs[l], xs[l], xs[l], true);
+        lds_sweep(L, codes(l), aw, w, bs(l), xs(l), xs(l), true);
         for (int k = 1; k < sweeps; ++k) {
-            lds_sweep(L, codes[l], aw, w, bs[l], xs[l], ts[l], false);
-            float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
+            lds_sweep(L, codes(l), aw, w, bs(l), xs(l), ts(l), false);
+            par ^= 1u << l;
         }
         if (l == A.nlev - 1) break;
         // restriction of the residual into level l+1
         const CoarseLevel& C = A.lv[l + 1];
-        const uint32_t nc = ns[l + 1];
+        const uint32_t nc = ns(l + 1);
         const int fx = L.nx > 1 ? 2 : 1, fy = L.ny > 1 ? 2 : 1, fz = L.nz > 1 ? 2 : 1;
         const float weight = 1.0f / float(fx * fy * fz);
-        const Flat FF{ns[l], uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
-        float* bc = const_cast<float*>(bs[l + 1]);
-        const float* xf = xs[l];
-        const float* bf = bs[l];
+        const Flat FF{ns(l), uint32_t(L.nx), uint32_t(L.nx) * uint32_t(L.ny)};
+        float* bc = const_cast<float*>(bs(l + 1));
+        const float* xf = xs(l);
+        const float* bf = bs(l);
         for (uint32_t c = threadIdx.x; c < nc; c += kCoarseThreads) {
             const uint32_t X = c % uint32_t(C.nx), Y = (c / uint32_t(C.nx)) % uint32_t(C.ny),
                            Z = c / (uint32_t(C.nx) * uint32_t(C.ny));
@@ -1470,7 +1476,7 @@ __device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars*
                         const uint32_t x = X * fx + dx, y = Y * fy + dy, z = Z * fz + dz;
                         if (x >= uint32_t(L.nx) || y >= uint32_t(L.ny) || z >= uint32_t(L.nz)) continue;
                         const uint32_t f = z * FF.sz + y * FF.sy + x;
-                        const uint16_t cd = codes[l][f];
+                        const uint16_t cd = codes(l)[f];
                         if (!(cd & kCodeFluid)) continue;
                         uint32_t j[6];
                         nbrs(f, FF, j);
@@ -1479,7 +1485,7 @@ __device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars*
                         for (int m = 0; m < 6; ++m) v[m] = xf[j[m]];
                         sum += bf[f] - arow(cd, xf[f], v, L.sx, L.sy, L.sz, aw);
                     }
-            bc[c] = (codes[l + 1][c] & kCodeFluid) ? sum * weight : 0.0f;
+            bc[c] = (codes(l + 1)[c] & kCodeFluid) ? sum * weight : 0.0f;
         }
         __syncthreads();
     }
@@ -1490,21 +1496,21 @@ __device__ __forceinline__ void coarse_run(const CoarseArgs& A, const CgScalars*
         const float* aw = tab + 128 * l;
         const float* w = aw + 64;
         const int hx = L.nx > 1 ? 1 : 0, hy = L.ny > 1 ? 1 : 0, hz = L.nz > 1 ? 1 : 0;
-        const float* xc = xs[l + 1];
-        float* xf = xs[l];
-        for (uint32_t c = threadIdx.x; c < ns[l]; c += kCoarseThreads) {
+        const float* xc = xs(l + 1);
+        float* xf = xs(l);
+        for (uint32_t c = threadIdx.x; c < ns(l); c += kCoarseThreads) {
             const uint32_t x = c % uint32_t(L.nx), y = (c / uint32_t(L.nx)) % uint32_t(L.ny),
                            z = c / (uint32_t(L.nx) * uint32_t(L.ny));
-            if (codes[l][c] & kCodeFluid)
+            if (codes(l)[c] & kCodeFluid)
                 xf[c] += xc[((z >> hz) * uint32_t(C.ny) + (y >> hy)) * uint32_t(C.nx) + (x >> hx)];
         }
         __syncthreads();
         for (int k = 0; k < A.nu; ++k) {
-            lds_sweep(L, codes[l], aw, w, bs[l], xs[l], ts[l], false);
-            float* t = xs[l]; xs[l] = ts[l]; ts[l] = t;
+            lds_sweep(L, codes(l), aw, w, bs(l), xs(l), ts(l), false);
+            par ^= 1u << l;
         }
     }
-    for (uint32_t c = threadIdx.x; c < ns[0]; c += kCoarseThreads) A.x0[c] = xs[0][c];
+    for (uint32_t c = threadIdx.x; c < ns(0); c += kCoarseThreads) A.x0[c] = xs(0)[c];
 }
 
 __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A, const CgScalars* __restrict__ sc) {
@@ -1513,6 +1519,7 @@ __global__ void __launch_bounds__(kCoarseThreads) k_mg_coarse_cycle(CoarseArgs A
 
 struct MCoarse {
     static constexpr int kThreads = kCoarseThreads;
+    static constexpr bool kByRef = true;
     CoarseArgs A;
     const CgScalars* sc;
     __device__ void run(uint32_t) const { coarse_run(A, sc); }
