@@ -343,12 +343,14 @@ MGF_API mgf_status mgf_scene_apply_preconditioner(mgf_scene scene, const float* 
  *
  * Lockstep (on by default, mgf_batch_set_lockstep): when two or more of the
  * scenes use MGF_PRECISION_MIXED with MGF_PRECOND_MULTIGRID and no warm
- * start, their pressure solves run together: every kernel of the CG and of
- * the V-cycle is one launch over all of them (per-scene arguments from a
- * device table), sweeps and CG chunks in lockstep with per-scene convergence
- * flags on the device; the stages before and after the solve run on the
- * scenes' own streams.  A scene's results are bitwise the same either way.
- * step_ms then holds each batch step's wall time for each of these scenes. */
+ * start, they step together: every kernel of the step (advection, right-hand
+ * side, the CG and the V-cycle, projection, extrapolation) is one launch over
+ * all of them (per-scene arguments from a device table), with CG chunks and
+ * sweeps in lockstep, per-scene convergence flags on the device and one
+ * launch per extrapolation layer depth; only the boundary refresh runs on
+ * the scenes' own streams.  A scene's results are bitwise the same either
+ * way.  step_ms then holds each batch step's wall time for each of these
+ * scenes. */
 MGF_API mgf_status mgf_batch_step(mgf_context ctx, mgf_scene* scenes, int32_t n, int32_t nsteps,
                                   int32_t workers, mgf_step_stats* stats, mgf_status* status,
                                   double* step_ms);
