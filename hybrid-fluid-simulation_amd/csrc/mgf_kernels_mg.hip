@@ -1597,11 +1597,33 @@ constexpr int kSchedHead = 4;
 constexpr int kWetChunk = 32;
 constexpr uint32_t kSchedHeavy = 0x80000000u;
 // Every chunk is a multiple of kRzSlab planes, and the ascent of level 0
-// writes one partial of r . z per tile and slab of kRzSlab planes (the
-// planes summed in order, then the waves in order): the sum, and so the CG
-// iterates, do not depend on the schedule.
-constexpr int kRzSlab = 4;
-constexpr int kRzLanes = 32;   // lanes of wave 0 that add a slab's sums
+// writes one partial of r . z per tile and slab of kRzSlab planes (each
+// work-item's planes in order, then a fixed order over the work-items): the
+// sum, and so the CG iterates, do not depend on the schedule.
+constexpr int kRzSlab = 2;
+
+// Sum over a wave: xor butterflies within each half of 32 lanes by
+// ds_swizzle (its lane pattern is an immediate, so no address registers),
+// then the two halves through readlane.  Every lane of a half holds the same
+// value after each step (a + b = b + a), so the result is one fixed sum.
+template <int M>
+__device__ __forceinline__ double swz_xor_add(double v) {
+    const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), 0x1F | (M << 10));
+    const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x1F | (M << 10));
+    return v + __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_swz(double v) {
+    v = swz_xor_add<1>(v);
+    v = swz_xor_add<2>(v);
+    v = swz_xor_add<4>(v);
+    v = swz_xor_add<8>(v);
+    v = swz_xor_add<16>(v);
+    const double a = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                      __builtin_amdgcn_readlane(__double2loint(v), 0));
+    const double b = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 32),
+                                      __builtin_amdgcn_readlane(__double2loint(v), 32));
+    return a + b;
+}
 
 // Per-work-item constants of a march.
 struct MarchCol {
@@ -1829,9 +1851,9 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     R.v2b = R.v2a; R.v2a = v2;
     // RZ: the last stored plane of a slab (the same for the whole block)
     // closes the slab's partial.  Every work-item leaves its sum in red;
-    // after the barrier, lane l < 32 of wave 0 adds entries l, l + 32, ..
-    // in order, and lane 0 adds those 32 sums in order.  Only LDS and
-    // additions: no cross-lane operation holds registers in the march.
+    // after the barrier, lane l of wave 0 adds entries l, l + 64, .. in
+    // order and wave_sum_swz adds the lanes.  The cross-lane work is in wave
+    // 0's branch, where the stages' values are dead.
     int slab = -1;
     if constexpr (RZ) {
         const int p = q - 3;
@@ -1843,20 +1865,12 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     }
     __syncthreads();
     if constexpr (RZ) {
-        if (slab >= 0 && threadIdx.x < kRzLanes) {
+        if (slab >= 0 && threadIdx.x < 64) {
             double t = 0.0;
 #pragma unroll 4
-            for (int k = 0; k < kMThreads / kRzLanes; ++k) t += red[threadIdx.x + kRzLanes * k];
-            red[kMThreads + threadIdx.x] = t;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (threadIdx.x == 0) {
-                double u = 0.0;
-#pragma unroll 4
-                for (int k = 0; k < kRzLanes; ++k) u += red[kMThreads + k];
-                part[K.tix + M.tiles * uint32_t(slab)] = u;
-            }
+            for (int k = 0; k < kMThreads / 64; ++k) t += red[threadIdx.x + 64 * k];
+            t = wave_sum_swz(t);
+            if (threadIdx.x == 0) part[K.tix + M.tiles * uint32_t(slab)] = t;
         }
     }
 }
@@ -2140,7 +2154,7 @@ __device__ __forceinline__ void march_run(const March& M0, const MgLevel& L, con
                                           const CgScalars* __restrict__ sc, double* __restrict__ part,
                                           const uint32_t* __restrict__ sched, uint32_t vblk, uint32_t slot) {
     __shared__ float lds[kLdsSlots + kSlotPad + (POST ? 6 : 8) * kSlotStride];
-    __shared__ double red[RZ ? kMThreads + kRzLanes : 1];
+    __shared__ double red[RZ ? kMThreads : 1];
     if (sc->done) return;
     if (!sched) {
         if (vblk < M0.blocks) march_unit<POST, RZ>(M0, L, code, b, xi, xc, xo, codec, bco, omega, part, vblk, true, lds, red);
@@ -2386,7 +2400,7 @@ static March make_march(const MgDims& f, const MgDims& c, int skip = 1) {
     M.blocks = xy * uint32_t((f.nz + C - 1) / C);
     M.tiles = xy;
     M.wunits = xy * uint32_t((f.nz + kWetChunk - 1) / kWetChunk);
-    int cm = kRzSlab;
+    int cm = 4;
     while (cm < f.nz && size_t(xy) * size_t((f.nz + cm - 1) / cm) > std::max<size_t>(kMarchMaxUnits, xy))
         cm += kRzSlab;
     M.cmin = cm;
