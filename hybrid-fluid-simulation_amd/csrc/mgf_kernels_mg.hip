@@ -1729,12 +1729,15 @@ __device__ __forceinline__ float march_residual(uint32_t cd, float awc, float bc
 
 // Loads of plane q of the work-item's column, at the plane clamped into the
 // level (the values of a plane or column outside the level are replaced
-// where they are used).
+// where they are used).  POST: also x and the parent's coarse x.  PRE: the
+// code of the coarse cell that the pairs of plane q - 4 complete (in ev's
+// bits), so that the step issuing the restriction waits on no load.
 template <bool POST>
 __device__ __forceinline__ void march_load(const March& M, const MarchCol& K, int q,
                                            const uint16_t* __restrict__ code, const float* __restrict__ b,
                                            const float* __restrict__ xi, const float* __restrict__ xc,
-                                           uint16_t& cd, float& bv, float& xv, float& ev) {
+                                           const uint16_t* __restrict__ codec, uint16_t& cd, float& bv,
+                                           float& xv, float& ev) {
     const int qc = q < 0 ? 0 : (q >= M.nz ? M.nz - 1 : q);
     const uint32_t i = uint32_t(qc) * M.sz + K.col;
     cd = code[i];
@@ -1742,14 +1745,17 @@ __device__ __forceinline__ void march_load(const March& M, const MarchCol& K, in
     if constexpr (POST) {
         xv = xi[i];
         ev = xc[uint32_t(qc >> 1) * M.csz + K.pcol];
+    } else {
+        const int Z = (q - 4) >> 1;
+        const int Zc = Z < 0 ? 0 : (Z >= M.cnz ? M.cnz - 1 : Z);
+        ev = __uint_as_float(uint32_t(codec[uint32_t(Zc) * M.csz + K.pcol]));
     }
 }
 
 // PRE: pair sum of the residuals of plane p (in slot RS); an odd plane
-// completes its coarse cell.
+// completes its coarse cell, whose code (prefetched with plane p + 4) is cc.
 __device__ __forceinline__ void march_pairs(const March& M, const MarchCol& K, int p, const float* RS,
-                                            const uint16_t* __restrict__ codec, float* __restrict__ bco,
-                                            MarchRing& R) {
+                                            uint32_t cc, float* __restrict__ bco, MarchRing& R) {
     if (!K.owner || p < K.za || p >= K.zr) return;
     const int e = K.me;
     const float P = (RS[e] + RS[e + 1]) + (RS[e + kMX] + RS[e + kMX + 1]);
@@ -1761,7 +1767,7 @@ __device__ __forceinline__ void march_pairs(const March& M, const MarchCol& K, i
     if (Z >= M.cnz) return;
     const uint32_t c = uint32_t(Z) * M.csz + K.pcol;
     const float s = R.pe + P;
-    bco[c] = (codec[c] & kCodeFluid) ? s * 0.125f : 0.0f;
+    bco[c] = (cc & kCodeFluid) ? s * 0.125f : 0.0f;
 }
 
 // Whether the wave wy holds a row at least s rows from the block's edge.
@@ -1809,7 +1815,8 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
     const float aw0 = aw[air_mask(uint16_t(cd0))];
     // stage 0 at q
     const float v0 = POST ? (in0 ? xN + eN : 0.0f) : wd0 * b0;
-    march_load<POST>(M, K, q + kMPre, code, b, xi, xc, cdN, bN, xN, eN);
+    const uint32_t ccode = POST ? 0u : __float_as_uint(eN);   // PRE: the coarse code for the pairs of q - 4
+    march_load<POST>(M, K, q + kMPre, code, b, xi, xc, codec, cdN, bN, xN, eN);
     lds[slot(0, H) + K.me] = v0;
     // Stage s is needed on the rows at least s from the block's edge only:
     // waves outside skip it (what their registers and slots then hold is
@@ -1839,7 +1846,7 @@ __device__ __forceinline__ void march_step(const March& M, const MgLevel& L, con
             const int p = q - 2;
             if (K.tile && p >= K.za && p < K.zs) xo[uint32_t(p) * M.sz + K.col] = v2;
             lds[slot(3, H) + K.me] = march_residual(R.c3, R.a3, R.b3, R.v2a, S, K, R.v2b, v2, L);
-            march_pairs(M, K, q - 4, lds + slot(3, H ^ 1), codec, bco, R);
+            march_pairs(M, K, q - 4, lds + slot(3, H ^ 1), ccode, bco, R);
         }
     }
     R.c3 = R.c2; R.c2 = R.c1; R.c1 = cd0;
@@ -2123,7 +2130,7 @@ __device__ __forceinline__ void march_unit(const March& M, const MgLevel& L, con
 #pragma unroll
     for (int k = 0; k < kMPre; ++k) {
         xv[k] = ev[k] = 0.0f;
-        march_load<POST>(M, K, K.za - 3 + k, code, b, xi, xc, cd[k], bv[k], xv[k], ev[k]);
+        march_load<POST>(M, K, K.za - 3 + k, code, b, xi, xc, codec, cd[k], bv[k], xv[k], ev[k]);
     }
     // Stage 3 reaches the last plane za + chunk - 1 at step chunk + 5, and
     // (PRE) its pair sums one step later; the steps are rounded up to whole
