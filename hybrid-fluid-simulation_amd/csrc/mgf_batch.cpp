@@ -95,11 +95,19 @@ mgf_status run_scene(mgf_scene s, int32_t nsteps, mgf_step_stats* out, double* s
 
 // ---------------------------------------------------------------------------
 // Served loop state.
+// A round of the served loop steps at least this many ready scenes in
+// lockstep (one step each); below it, a request runs on its scene's own
+// stream, which keeps the latency of a lightly loaded server low.
+constexpr size_t kServeBatchMin = 8;
+
 struct mgf_server_s {
     struct Request {
         int64_t ticket;
         int32_t nsteps;
         double t_submit;
+        int32_t done = 0;          // steps already run (in lockstep rounds)
+        double t_start = 0.0;
+        mgf_step_stats acc{};
     };
     struct Resident {
         std::deque<Request> pending;
@@ -120,7 +128,64 @@ struct mgf_server_s {
     int64_t resident = 0;
     int32_t queued = 0, running = 0;
     bool stop = false;
+    bool in_round = false;             // a lockstep round is running
     std::vector<std::thread> workers;
+
+    // Finishes a request (under the lock): its result, the scene back in the
+    // ready queue when more requests wait.
+    void finish_request(mgf_scene s, Resident& r, mgf_status st, double t_end) {
+        const Request& q = r.pending.front();
+        mgf_request_result res{};
+        res.ticket = q.ticket;
+        res.scene = s;
+        res.steps = q.nsteps;
+        res.t_submit_ms = q.t_submit;
+        res.t_start_ms = q.t_start;
+        res.t_end_ms = t_end;
+        res.status = st;
+        res.stats = q.acc;
+        r.pending.pop_front();
+        done.push_back(res);
+    }
+
+    // One lockstep round over the scenes in bs (taken from the ready queue,
+    // marked running): one step each, then each request that has all its
+    // steps, or failed, is finished.
+    void round(std::vector<mgf_scene>& bs, std::unique_lock<std::mutex>& lk) {
+        const double t0 = now_ms();
+        for (mgf_scene s : bs) {
+            Request& q = scenes.at(s).pending.front();
+            if (q.done == 0) q.t_start = t0;
+        }
+        lk.unlock();
+        std::vector<mgf_step_stats> st;
+        std::vector<mgf_status> status;
+        std::vector<std::string> errs;
+        mgf_status r = MGF_OK;
+        for (mgf_scene s : bs)
+            if (r == MGF_OK) r = scene_prepare(s);
+        if (r == MGF_OK) r = batch_lockstep(ctx, bs, 1, 0, st, status, errs, nullptr);
+        std::vector<int64_t> bytes(bs.size(), 0);
+        for (size_t i = 0; i < bs.size(); ++i) (void)mgf_scene_device_bytes(bs[i], &bytes[i]);
+        const double t1 = now_ms();
+        lk.lock();
+        for (size_t i = 0; i < bs.size(); ++i) {
+            mgf_scene s = bs[i];
+            Resident& rr = scenes.at(s);
+            Request& q = rr.pending.front();
+            const mgf_status si = r != MGF_OK ? r : (i < status.size() ? status[i] : MGF_ERR_DEVICE);
+            if (i < st.size()) merge_stats(&q.acc, st[i], q.done == 0);
+            ++q.done;
+            if (si != MGF_OK || q.done >= q.nsteps) finish_request(s, rr, si, t1);
+            resident += bytes[i] - rr.bytes;
+            rr.bytes = bytes[i];
+            rr.running = false;
+            --running;
+            if (!rr.pending.empty()) ready.push_back(s);
+        }
+        cv_done.notify_all();
+        if (!ready.empty()) cv_work.notify_all();
+    }
 
     void work() {
         (void)hipSetDevice(ctx->device);
@@ -128,35 +193,60 @@ struct mgf_server_s {
         for (;;) {
             cv_work.wait(lk, [&] { return stop || !ready.empty(); });
             if (ready.empty()) return;   // stop with nothing left
+            // enough ready scenes that can step in lockstep, and no round in
+            // flight: one round over all of them
+            if (!in_round && ctx->lockstep) {
+                std::vector<mgf_scene> bs;
+                for (mgf_scene s : ready)
+                    if (lockstep_eligible(s)) bs.push_back(s);
+                if (bs.size() >= kServeBatchMin) {
+                    std::unordered_set<mgf_scene> take(bs.begin(), bs.end());
+                    std::deque<mgf_scene> rest;
+                    for (mgf_scene s : ready)
+                        if (!take.count(s)) rest.push_back(s);
+                    ready.swap(rest);
+                    for (mgf_scene s : bs) {
+                        Resident& r = scenes.at(s);
+                        if (r.pending.front().done == 0) --queued;
+                        r.running = true;
+                        ++running;
+                    }
+                    in_round = true;
+                    round(bs, lk);
+                    in_round = false;
+                    continue;
+                }
+            }
             mgf_scene s = ready.front();
             ready.pop_front();
             Resident& r = scenes.at(s);
-            const Request q = r.pending.front();
-            r.pending.pop_front();
+            Request& q = r.pending.front();
+            if (q.done == 0) {
+                --queued;
+                q.t_start = now_ms();
+            }
+            const int32_t left = q.nsteps - q.done;
             r.running = true;
-            --queued;
             ++running;
             lk.unlock();
 
-            mgf_request_result res{};
-            res.ticket = q.ticket;
-            res.scene = s;
-            res.steps = q.nsteps;
-            res.t_submit_ms = q.t_submit;
-            res.t_start_ms = now_ms();
-            res.status = run_scene(s, q.nsteps, &res.stats, nullptr, nullptr);
-            res.t_end_ms = now_ms();
+            mgf_step_stats one{};
+            const mgf_status st = run_scene(s, left, &one, nullptr, nullptr);
+            const double t_end = now_ms();
             int64_t bytes = 0;
             (void)mgf_scene_device_bytes(s, &bytes);
 
             lk.lock();
             Resident& r2 = scenes.at(s);
+            Request& q2 = r2.pending.front();
+            merge_stats(&q2.acc, one, q2.done == 0);
+            q2.done = q2.nsteps;
+            finish_request(s, r2, st, t_end);
             resident += bytes - r2.bytes;
             r2.bytes = bytes;
             r2.running = false;
             --running;
             if (!r2.pending.empty()) ready.push_back(s);
-            done.push_back(res);
             cv_done.notify_all();
             if (!ready.empty()) cv_work.notify_one();
         }

@@ -177,6 +177,50 @@ def test_server_results_match_scenes_stepped_alone(ctx):
         s.close()
 
 
+def test_server_lockstep_rounds_match_scenes_stepped_alone(ctx):
+    """With at least 8 ready scenes that can step in lockstep, a worker steps
+    them together, one step per round (requests of two steps span two
+    rounds, or finish on their scene's stream); every scene's result equals
+    the same steps run alone, and the rounds show as groups of at least 8
+    results ending at the same instant."""
+    sp = specs(12, seed0=700)
+    kw = desc_kw()
+    solo = [scenes.build_gpu_scene(mgf, ctx, p, **kw) for p in sp]
+    with mgf.Server(ctx, budget_bytes=8 << 30, slots=2) as srv:
+        served = []
+        for p in sp:
+            s = srv.open_scene(mgf.make_desc(p.res, p.origin, p.spacing, p.dt, **kw))
+            assert s is not None
+            scenes.setup_gpu_scene(s, p)
+            served.append(s)
+        tickets = {}
+        steps = [0] * len(sp)
+        for k in range(3):
+            for i in range(len(sp)):
+                n = 1 + (i + k) % 2
+                tickets[srv.submit(served[i], n)] = (i, n)
+                steps[i] += n
+        got = []
+        while len(got) < len(tickets):
+            got += srv.poll(cap=64, wait_ms=10000)
+        assert sorted(r.ticket for r in got) == sorted(tickets)
+        ends = {}
+        for r in got:
+            i, n = tickets[r.ticket]
+            assert r.status == mgf.OK and r.steps == n and r.stats.steps == n, (i, r.status, r.stats.steps)
+            assert r.stats.true_converged == 1
+            assert r.t_submit_ms <= r.t_start_ms <= r.t_end_ms
+            ends[r.t_end_ms] = ends.get(r.t_end_ms, 0) + 1
+        assert max(ends.values()) >= 8, sorted(ends.values())
+        for i, s in enumerate(solo):
+            s.step(steps[i])
+            assert_same(snapshot(served[i]), snapshot(s), sp[i].name)
+        b, q, run = srv.state()
+        assert q == 0 and run == 0 and b > 0
+    for s in solo:
+        s.close()
+
+
 def test_server_admission_by_budget(ctx):
     sp = specs(2, seed0=700)
     kw = desc_kw()
