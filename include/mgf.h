@@ -382,7 +382,12 @@ MGF_API mgf_status mgf_scene_bytes_for(mgf_context ctx, const mgf_scene_desc* de
  * A server owns the scenes it admits and runs step requests on `slots` host
  * worker threads, each driving its scene's own stream, so requests that
  * arrive while others run start as soon as a slot is free and overlap with
- * them on the device.  Requests for one scene run in submission order, one at
+ * them on the device.  When at least 8 scenes with a pending request could
+ * step in lockstep (as in mgf_batch_step; mgf_batch_set_lockstep) and no
+ * round is running, a free worker steps all of them together, one step per
+ * round; a request of several steps stays at the head of its scene's queue
+ * until it has them all.  Results are bitwise those of the scenes stepped
+ * alone either way.  Requests for one scene run in submission order, one at
  * a time.  Admission is by HBM budget: mgf_server_open_scene refuses with
  * MGF_ERR_BUDGET when the resident bytes plus the new scene's
  * (mgf_scene_bytes_for) would exceed hbm_budget_bytes; solids added to an
