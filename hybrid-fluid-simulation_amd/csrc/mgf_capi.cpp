@@ -1163,6 +1163,7 @@ using LSeq = std::vector<LSlot>;
 constexpr int kActiveRing = 8;
 
 struct LockPlan {
+    mgf_context ctx = nullptr;
     std::vector<mgf_scene> sc;
     hipStream_t stream = nullptr;
     unsigned char* dev = nullptr;
@@ -1181,8 +1182,10 @@ struct LockPlan {
     hipGraphExec_t iter = nullptr;     // one CG iteration (cg + vcycle), captured
     ~LockPlan() {
         if (iter) (void)hipGraphExecDestroy(iter);
-        if (dev) (void)hipFree(dev);
-        if (host) (void)hipHostFree(host);
+        if (ctx) {
+            pool_put(ctx, dev);
+            pool_put(ctx, host);
+        }
     }
 };
 
@@ -1258,6 +1261,7 @@ mgf_status run_seq(LockPlan& P, const LSeq& seq) { return run_seq_at(P.stream, P
 // buffers (each scene's `cur`): built once per assignment and kept (a batch
 // alternates between two).
 struct PhasePlan {
+    mgf_context ctx = nullptr;
     LSeq pre, post;
     unsigned char* dev = nullptr;   // the packed layer offsets, the layer table, then tables and maps
     std::vector<size_t> offs_at;    // per scene: its offsets' first word in the packed area (cap + 3 words)
@@ -1269,8 +1273,10 @@ struct PhasePlan {
     PhasePlan(const PhasePlan&) = delete;
     PhasePlan& operator=(const PhasePlan&) = delete;
     ~PhasePlan() {
-        if (dev) (void)hipFree(dev);
-        if (offs_host) (void)hipHostFree(offs_host);
+        if (ctx) {
+            pool_put(ctx, dev);
+            pool_put(ctx, offs_host);
+        }
     }
 };
 
@@ -1307,17 +1313,11 @@ mgf_status build_phase(const std::vector<mgf_scene>& sc, const std::vector<uint8
         Q.post = merge_seq(post, blob);
         if (Q.pre.empty() || Q.post.empty()) return fail(MGF_ERR_INVALID, "lockstep phases: a scene has two records under one key");
         if (pass == 0) {
-            if (hipMalloc(reinterpret_cast<void**>(&Q.dev), blob.size()) != hipSuccess) {
-                (void)hipGetLastError();
-                Q.dev = nullptr;
-                return fail(MGF_ERR_NOMEM, "lockstep phases: hipMalloc failed");
-            }
-            if (hipHostMalloc(reinterpret_cast<void**>(&Q.offs_host), std::max<size_t>(Q.offs_words, 1) * 4) !=
-                hipSuccess) {
-                (void)hipGetLastError();
-                Q.offs_host = nullptr;
-                return fail(MGF_ERR_NOMEM, "lockstep phases: hipHostMalloc failed");
-            }
+            Q.ctx = sc[0]->ctx;
+            Q.dev = static_cast<unsigned char*>(pool_get(Q.ctx, blob.size(), false));
+            if (!Q.dev) return fail(MGF_ERR_NOMEM, "lockstep phases: hipMalloc failed");
+            Q.offs_host = static_cast<uint32_t*>(pool_get(Q.ctx, std::max<size_t>(Q.offs_words, 1) * 4, true));
+            if (!Q.offs_host) return fail(MGF_ERR_NOMEM, "lockstep phases: hipHostMalloc failed");
             continue;
         }
         MGF_HIP(hipMemcpyAsync(Q.dev, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
@@ -1388,17 +1388,11 @@ mgf_status build_plan(LockPlan& P) {
         for (const LSeq* q : {&P.start_b, &P.start_r, &P.cg, &P.vcycle, &P.refine1, &P.refine2, &P.gather})
             if (q->empty()) return fail(MGF_ERR_INVALID, "lockstep plan: a scene has two records under one key");
         if (pass == 0) {
-            if (hipMalloc(reinterpret_cast<void**>(&P.dev), blob.size()) != hipSuccess) {
-                (void)hipGetLastError();
-                P.dev = nullptr;
-                return fail(MGF_ERR_NOMEM, "lockstep plan: hipMalloc failed");
-            }
+            P.dev = static_cast<unsigned char*>(pool_get(P.ctx, blob.size(), false));
+            if (!P.dev) return fail(MGF_ERR_NOMEM, "lockstep plan: hipMalloc failed");
             const size_t hb = act_off + kActiveRing * act_stride;
-            if (hipHostMalloc(reinterpret_cast<void**>(&P.host), hb) != hipSuccess) {
-                (void)hipGetLastError();
-                P.host = nullptr;
-                return fail(MGF_ERR_NOMEM, "lockstep plan: hipHostMalloc failed");
-            }
+            P.host = static_cast<unsigned char*>(pool_get(P.ctx, hb, true));
+            if (!P.host) return fail(MGF_ERR_NOMEM, "lockstep plan: hipHostMalloc failed");
             continue;
         }
         MGF_HIP(hipMemcpyAsync(P.dev, blob.data(), blob.size(), hipMemcpyHostToDevice, P.stream));
@@ -1779,6 +1773,67 @@ mgf_status extrapolate_run(hipStream_t st, const GridDev& g, float* u, float* v,
     return MGF_OK;
 }
 
+void* pool_get(mgf_context ctx, size_t bytes, bool pinned) {
+    std::lock_guard<std::mutex> lk(ctx->pool.mu);
+    CtxPool::Buf* best = nullptr;
+    for (auto& b : ctx->pool.bufs)
+        if (!b.used && b.pinned == pinned && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+    if (best) {
+        best->used = true;
+        return best->p;
+    }
+    void* p = nullptr;
+    const size_t n = std::max<size_t>(bytes, 256);
+    const hipError_t e = pinned ? hipHostMalloc(&p, n) : hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    ctx->pool.bufs.push_back(CtxPool::Buf{p, n, pinned, true});
+    return p;
+}
+
+void pool_put(mgf_context ctx, void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(ctx->pool.mu);
+    for (auto& b : ctx->pool.bufs)
+        if (b.p == p) b.used = false;
+}
+
+hipStream_t pool_stream(mgf_context ctx) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->pool.mu);
+        if (!ctx->pool.streams.empty()) {
+            hipStream_t s = ctx->pool.streams.back();
+            ctx->pool.streams.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return s;
+}
+
+void pool_stream_put(mgf_context ctx, hipStream_t s) {
+    if (!s) return;
+    std::lock_guard<std::mutex> lk(ctx->pool.mu);
+    ctx->pool.streams.push_back(s);
+}
+
+void pool_release(mgf_context ctx) {
+    std::lock_guard<std::mutex> lk(ctx->pool.mu);
+    for (auto& b : ctx->pool.bufs) {
+        if (b.pinned) (void)hipHostFree(b.p);
+        else (void)hipFree(b.p);
+    }
+    ctx->pool.bufs.clear();
+    for (hipStream_t st : ctx->pool.streams) (void)hipStreamDestroy(st);
+    ctx->pool.streams.clear();
+}
+
 bool lockstep_eligible(mgf_scene s) { return lock_eligible(s); }
 
 // The extrapolation layers of every scene from its offsets (in Q.offs_host):
@@ -1826,13 +1881,12 @@ mgf_status launch_layers(hipStream_t st, const std::vector<mgf_scene>& sc, const
     host.insert(host.end(), tp, tp + 4 * tail_map.size());
     if (host.size() > dev_cap) {
         MGF_HIP(hipStreamSynchronize(st));
-        if (dev) (void)hipFree(dev);
+        pool_put(sc[0]->ctx, dev);
         dev = nullptr;
         dev_cap = 0;
         const size_t want = host.size() + host.size() / 2 + 4096;
-        if (hipMalloc(reinterpret_cast<void**>(&dev), want) != hipSuccess) {
-            (void)hipGetLastError();
-            dev = nullptr;
+        dev = static_cast<unsigned char*>(pool_get(sc[0]->ctx, want, false));
+        if (!dev) {
             return fail(MGF_ERR_NOMEM, "lockstep layers: hipMalloc failed");
         }
         dev_cap = want;
@@ -1857,11 +1911,10 @@ mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int
     if (n == 0) return MGF_OK;
     MGF_HIP(hipSetDevice(ctx->device));
     LockPlan P;
+    P.ctx = ctx;
     P.sc = sc;
-    if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
-    }
+    P.stream = pool_stream(ctx);
+    if (!P.stream) return fail(MGF_ERR_DEVICE, "hipStreamCreate failed");
     hipEvent_t ev = nullptr;
     std::vector<std::pair<std::vector<uint8_t>, std::unique_ptr<PhasePlan>>> phases;
     std::vector<unsigned char> step_host;   // this step's layer maps and tail table
@@ -1870,9 +1923,9 @@ mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(P.stream);
         if (ev) (void)hipEventDestroy(ev);
-        if (step_dev) (void)hipFree(step_dev);
+        pool_put(ctx, step_dev);
         phases.clear();
-        (void)hipStreamDestroy(P.stream);
+        pool_stream_put(ctx, P.stream);
         P.stream = nullptr;
     };
     auto fin = [&](mgf_status st) {
@@ -2077,6 +2130,7 @@ mgf_status mgf_shutdown(mgf_context ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
+    pool_release(ctx);
     delete ctx;
     return MGF_OK;
 }

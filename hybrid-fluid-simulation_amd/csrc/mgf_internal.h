@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <mutex>
 #include <vector>
 
 
@@ -80,12 +81,31 @@ struct Solid {
 
 }  // namespace mgf
 
+namespace mgf {
+// Device buffers, pinned host buffers and streams of the lockstep batch
+// plans, kept by the context and reused from call to call (a served loop
+// builds plans every round): no allocation, free or stream creation after
+// the first rounds, and no free that waits on the device.
+struct CtxPool {
+    struct Buf {
+        void* p;
+        size_t bytes;
+        bool pinned;
+        bool used;
+    };
+    std::mutex mu;
+    std::vector<Buf> bufs;
+    std::vector<hipStream_t> streams;   // free streams (default priority)
+};
+}  // namespace mgf
+
 struct mgf_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
     size_t lds_per_block = 65536;
     bool lockstep = true;   // mgf_batch_step runs eligible scenes in lockstep
+    mgf::CtxPool pool;
 };
 
 struct mgf_scene_s {
@@ -220,6 +240,14 @@ mgf_status scene_prepare(mgf_scene s);
 // Whether a scene can be stepped in lockstep with others (mixed precision,
 // multigrid preconditioner, cold start).
 bool lockstep_eligible(mgf_scene s);
+// The context's pool: a buffer of at least `bytes` (device or pinned host),
+// back with pool_put; a stream, back with pool_stream_put; pool_release
+// frees everything (at mgf_shutdown).
+void* pool_get(mgf_context ctx, size_t bytes, bool pinned);
+void pool_put(mgf_context ctx, void* p);
+hipStream_t pool_stream(mgf_context ctx);
+void pool_stream_put(mgf_context ctx, hipStream_t s);
+void pool_release(mgf_context ctx);
 // nsteps steps of every scene in sc (all eligible, distinct, of ctx): the
 // pressure solves as multi-scene launches on one stream, the other stages on
 // the scenes' streams from up to `workers` host threads.  stats, status and
