@@ -427,6 +427,8 @@ enum MultiKind : int32_t {
     kMkAdvectFc, kMkAdvectCc, kMkAdvectCc4, kMkMarkers, kMkRhs, kMkProject, kMkCopyWord,
     kMkExtRows, kMkExtRowsReg4, kMkExtRowsReg8, kMkExtRowsReg16, kMkExtLines1, kMkExtLines2, kMkExtZero, kMkExtCount,
     kMkExtScan, kMkExtScatter, kMkExtTail, kMkCopyOffs,
+    kMkExtLinesSeg,                          // 8 kinds: (axis - 1) * 4 + segment size class
+    kMkExtLinesSegLast = kMkExtLinesSeg + 7,
     kMkCount
 };
 constexpr size_t kMRecBytes = 512;

@@ -71,6 +71,35 @@ __device__ __forceinline__ uint32_t udiv_r(uint32_t a, uint32_t d, double rd) {
     return q;
 }
 
+// Min-scans over the 64 lanes of a wave: inclusive prefix (lane l gets the
+// minimum over lanes 0..l) and suffix (over lanes l..63).  Within each row of
+// 16 lanes four DPP row shifts (a lane whose source lies outside its row keeps
+// the identity); across the four rows, each row's end lane read with
+// v_readlane.  Integer minima, so any order gives the same result.
+constexpr int32_t kMinId = 0x7FFFFFFF;
+__device__ __forceinline__ int32_t wave_min_prefix(int32_t g) {
+    g = min(g, __builtin_amdgcn_update_dpp(kMinId, g, 0x111, 0xF, 0xF, false));   // row_shr:1
+    g = min(g, __builtin_amdgcn_update_dpp(kMinId, g, 0x112, 0xF, 0xF, false));   // row_shr:2
+    g = min(g, __builtin_amdgcn_update_dpp(kMinId, g, 0x114, 0xF, 0xF, false));   // row_shr:4
+    g = min(g, __builtin_amdgcn_update_dpp(kMinId, g, 0x118, 0xF, 0xF, false));   // row_shr:8
+    const int32_t r0 = __builtin_amdgcn_readlane(g, 15);
+    const int32_t r1 = min(r0, __builtin_amdgcn_readlane(g, 31));
+    const int32_t r2 = min(r1, __builtin_amdgcn_readlane(g, 47));
+    const uint32_t row = (threadIdx.x & 63) >> 4;
+    return min(g, row == 0 ? kMinId : (row == 1 ? r0 : (row == 2 ? r1 : r2)));
+}
+__device__ __forceinline__ int32_t wave_min_suffix(int32_t h) {
+    h = min(h, __builtin_amdgcn_update_dpp(kMinId, h, 0x101, 0xF, 0xF, false));   // row_shl:1
+    h = min(h, __builtin_amdgcn_update_dpp(kMinId, h, 0x102, 0xF, 0xF, false));   // row_shl:2
+    h = min(h, __builtin_amdgcn_update_dpp(kMinId, h, 0x104, 0xF, 0xF, false));   // row_shl:4
+    h = min(h, __builtin_amdgcn_update_dpp(kMinId, h, 0x108, 0xF, 0xF, false));   // row_shl:8
+    const int32_t r3 = __builtin_amdgcn_readlane(h, 48);
+    const int32_t r2 = min(r3, __builtin_amdgcn_readlane(h, 32));
+    const int32_t r1 = min(r2, __builtin_amdgcn_readlane(h, 16));
+    const uint32_t row = (threadIdx.x & 63) >> 4;
+    return min(h, row == 3 ? kMinId : (row == 2 ? r3 : (row == 1 ? r2 : r1)));
+}
+
 // ---------------------------------------------------------------------------
 // 1a. Rows along x: d = 0 on known faces, INF elsewhere; then the 1-D L1
 // distance: forward d(x) = x + min_{k<=x} (d(k) - k), backward
@@ -92,14 +121,8 @@ __device__ __forceinline__ void k_ext_rows_run(Faces F, uint16_t* __restrict__ d
         const uint32_t x = x0 + uint32_t(lane);
         const bool ok = x < ex;
         const int32_t d0 = ok ? (f[local + x] != MGF_UNKNOWN ? 0 : kFaceInf) : kFaceInf;
-        int32_t g = d0 - int32_t(x);
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t t = __shfl_up(g, o, 64);
-            if (lane >= o) g = min(g, t);
-        }
-        g = min(g, carry);
-        carry = __shfl(g, 63, 64);
+        const int32_t g = min(wave_min_prefix(d0 - int32_t(x)), carry);
+        carry = __builtin_amdgcn_readlane(g, 63);
         if (ok) d[x] = uint16_t(min(g + int32_t(x), kFaceInf));
     }
     // backward (each lane re-reads what it wrote)
@@ -108,14 +131,8 @@ __device__ __forceinline__ void k_ext_rows_run(Faces F, uint16_t* __restrict__ d
     for (uint32_t ch = nch; ch-- > 0;) {
         const uint32_t x = ch * 64 + uint32_t(lane);
         const bool ok = x < ex;
-        int32_t h = ok ? int32_t(d[x]) + int32_t(x) : 2 * kFaceInf;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t t = __shfl_down(h, o, 64);
-            if (lane + o < 64) h = min(h, t);
-        }
-        h = min(h, carry);
-        carry = __shfl(h, 0, 64);
+        const int32_t h = min(wave_min_suffix(ok ? int32_t(d[x]) + int32_t(x) : 2 * kFaceInf), carry);
+        carry = __builtin_amdgcn_readlane(h, 0);
         if (ok) d[x] = uint16_t(min(h - int32_t(x), kFaceInf));
     }
 }
@@ -198,6 +215,110 @@ struct MExtLines {
     __device__ void run(uint32_t vb) const { k_ext_lines_run<AXIS>(F, dist, xcd_of(vb, G), vb); }
 };
 
+// 1b'. The same line transform in one read and one write of the distances
+// (k_ext_lines re-reads and re-writes them in its backward sweep).  A block of
+// kLineSegWaves waves takes 64 consecutive lines (one per lane); wave k holds
+// the k-th segment of every line in registers (at most SEG faces).  With
+// P_k = min (d(j) - j) and S_k = min (d(j) + j) over segment k, kept in LDS:
+//   forward   F(i) = i + min(min_{m<k} P_m, min_{a_k<=j<=i} (d(j) - j))
+//   backward  d'(i) = min(min_{m>k} S_m, min_{i<=j<b_k} (F(j) + j)) - i,
+// which is min_j (d(j) + |i - j|), the result of the two full-line sweeps
+// (integer minima: the same distances).
+constexpr int kLineSegWaves = 16;
+constexpr int kLineSegThreads = 64 * kLineSegWaves;
+constexpr int kLineSegSizes[4] = {12, 20, 36, 64};   // SEG classes: lines of at most 16 SEG faces
+constexpr int32_t kSegNone = 1 << 20;                 // past the line's end: above any d(j) +- j
+// Word 3 of a raw buffer resource on gfx9 (32-bit data format, no swizzle).
+constexpr int32_t kBufWord3 = 0x00020000;
+
+// Stripes of 64 lines in component c (the last one partial).
+template <int AXIS>
+__device__ __forceinline__ uint32_t seg_stripes(const Faces& F, int c) {
+    const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
+    const uint32_t n = c == 0 ? pre[1] - pre[0] : (c == 1 ? pre[2] - pre[1] : pre[3] - pre[2]);
+    return (n + 63) >> 6;
+}
+
+template <int AXIS, int SEG>
+__device__ __forceinline__ void k_ext_lines_seg_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk, uint32_t slot) {
+    (void)slot;
+    __shared__ int32_t sp[kLineSegWaves][64], ss[kLineSegWaves][64];
+    const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
+    const uint32_t n0 = seg_stripes<AXIS>(F, 0), n1 = seg_stripes<AXIS>(F, 1), n2 = seg_stripes<AXIS>(F, 2);
+    if (blk >= n0 + n1 + n2) return;
+    const int c = blk < n0 ? 0 : (blk < n0 + n1 ? 1 : 2);
+    const uint32_t sb = blk - (c == 0 ? 0u : (c == 1 ? n0 : n0 + n1));
+    const uint32_t lines = c == 0 ? pre[1] - pre[0] : (c == 1 ? pre[2] - pre[1] : pre[3] - pre[2]);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t l = sb * 64 + lane;
+    const bool ok = l < lines;
+    const uint32_t ex = pick(F.ex, c), ey = pick(F.ey, c);
+    const uint32_t w = udiv_r(ok ? l : 0u, ex, pick(F.rex, c));
+    const uint32_t x = (ok ? l : 0u) - w * ex;
+    uint32_t start, stride, len;
+    if (AXIS == 1) { start = w * ex * ey + x; stride = ex; len = ey; }
+    else { start = w * ex + x; stride = ex * ey; len = pick(F.ez, c); }
+    const uint32_t lb = 2u * (pick4(F.off, c) + start);   // this lane's line start, in bytes (< 2^32)
+    const uint32_t seg = (len + kLineSegWaves - 1) / kLineSegWaves;
+    const uint32_t a = wv * seg;
+    const uint32_t n = a < len ? min(seg, len - a) : 0u;   // faces of this wave's segment
+    // Buffer accesses: the lane's line start in the VGPR offset, the row's
+    // wave-uniform byte offset in the SGPR offset (no 64-bit address per row).
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(dist, int16_t(0), int32_t(2u * F.off[3]), kBufWord3);
+    const uint32_t rb = 2u * stride;
+    int32_t v[SEG];
+#pragma unroll
+    for (int q = 0; q < SEG; ++q) {
+        v[q] = (ok && uint32_t(q) < n) ? int32_t(__builtin_amdgcn_raw_buffer_load_b16(rs, lb, (a + q) * rb, 0))
+                                       : kSegNone;
+    }
+    int32_t pm = kSegNone, sm = kSegNone;
+#pragma unroll
+    for (int q = 0; q < SEG; ++q) {
+        pm = min(pm, v[q] - int32_t(a + q));
+        sm = min(sm, v[q] + int32_t(a + q));
+    }
+    sp[wv][lane] = pm;
+    ss[wv][lane] = sm;
+    __syncthreads();
+    int32_t f = kSegNone, g = kSegNone;
+#pragma unroll
+    for (int m = 0; m < kLineSegWaves; ++m) {
+        if (uint32_t(m) < wv) f = min(f, sp[m][lane]);
+        if (uint32_t(m) > wv) g = min(g, ss[m][lane]);
+    }
+#pragma unroll
+    for (int q = 0; q < SEG; ++q) {
+        f = min(f, v[q] - int32_t(a + q));
+        v[q] = f + int32_t(a + q);
+    }
+#pragma unroll
+    for (int q = SEG - 1; q >= 0; --q) {
+        g = min(g, v[q] + int32_t(a + q));
+        v[q] = g - int32_t(a + q);
+    }
+#pragma unroll
+    for (int q = 0; q < SEG; ++q) {
+        if (ok && uint32_t(q) < n)
+            __builtin_amdgcn_raw_buffer_store_b16(uint16_t(min(v[q], kFaceInf)), rs, lb, (a + q) * rb, 0);
+    }
+}
+
+template <int AXIS, int SEG>
+__global__ void __launch_bounds__(kLineSegThreads) k_ext_lines_seg(Faces F, uint16_t* __restrict__ dist) {
+    k_ext_lines_seg_run<AXIS, SEG>(F, dist, xcd_block(), blockIdx.x);
+}
+
+template <int AXIS, int SEG>
+struct MExtLinesSeg {
+    static constexpr int kThreads = kLineSegThreads;
+    Faces F;
+    uint16_t* dist;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_lines_seg_run<AXIS, SEG>(F, dist, xcd_of(vb, G), vb); }
+};
+
 // 1a'. The same row transform with the row held in registers: NCH chunks of
 // 64 faces per lane (rows of at most 64 NCH faces), one read of the faces and
 // one write of the distances (the two-pass kernel above re-reads and
@@ -227,14 +348,8 @@ __device__ __forceinline__ void k_ext_rows_reg_run(Faces F, uint16_t* __restrict
     for (int ch = 0; ch < NCH; ++ch) {
         if (uint32_t(ch) < nch) {
             const int32_t x = ch * 64 + lane;
-            int32_t g = v[ch] - x;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int32_t t = __shfl_up(g, o, 64);
-                if (lane >= o) g = min(g, t);
-            }
-            g = min(g, carry);
-            carry = __shfl(g, 63, 64);
+            const int32_t g = min(wave_min_prefix(v[ch] - x), carry);
+            carry = __builtin_amdgcn_readlane(g, 63);
             v[ch] = min(g + x, kFaceInf);
         }
     }
@@ -245,14 +360,8 @@ __device__ __forceinline__ void k_ext_rows_reg_run(Faces F, uint16_t* __restrict
         if (uint32_t(ch) < nch) {
             const int32_t x = ch * 64 + lane;
             const bool ok = uint32_t(x) < ex;
-            int32_t h = ok ? v[ch] + x : 2 * kFaceInf;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int32_t t = __shfl_down(h, o, 64);
-                if (lane + o < 64) h = min(h, t);
-            }
-            h = min(h, carry);
-            carry = __shfl(h, 0, 64);
+            const int32_t h = min(wave_min_suffix(ok ? v[ch] + x : 2 * kFaceInf), carry);
+            carry = __builtin_amdgcn_readlane(h, 0);
             if (ok) d[x] = uint16_t(min(h - x, kFaceInf));
         }
     }
@@ -308,12 +417,12 @@ __device__ __forceinline__ uint32_t lds_count(uint32_t* ctr_of_key, int32_t key,
     if (act == 0) return 0;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((unsigned long long)act) - 1;
-    const int32_t k0 = __shfl(key, leader, 64);
+    const int32_t k0 = __builtin_amdgcn_readlane(key, leader);
     const uint64_t same = __ballot(active && key == k0);
     if (same == act) {
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(&ctr_of_key[k0], uint32_t(__popcll(act)));
-        base = __shfl(base, leader, 64);
+        base = __builtin_amdgcn_readlane(base, leader);
         return base + uint32_t(__popcll(act & ((1ull << lane) - 1ull)));
     }
     return active ? atomicAdd(&ctr_of_key[key], 1u) : 0u;
@@ -791,28 +900,16 @@ __global__ void __launch_bounds__(kBlock) k_cdt_rows(Cells C, int32_t* __restric
     for (uint32_t x0 = 0; x0 < ex; x0 += 64) {
         const uint32_t x = x0 + uint32_t(lane);
         const bool ok = x < ex;
-        int32_t g = (ok ? d[x] : kDistInf) - int32_t(x);
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t t = __shfl_up(g, o, 64);
-            if (lane >= o) g = min(g, t);
-        }
-        g = min(g, carry);
-        carry = __shfl(g, 63, 64);
+        const int32_t g = min(wave_min_prefix((ok ? d[x] : kDistInf) - int32_t(x)), carry);
+        carry = __builtin_amdgcn_readlane(g, 63);
         if (ok) d[x] = min(g + int32_t(x), kDistInf);
     }
     carry = 2 * kDistInf;
     for (uint32_t ch = (ex + 63) / 64; ch-- > 0;) {
         const uint32_t x = ch * 64 + uint32_t(lane);
         const bool ok = x < ex;
-        int32_t h = ok ? d[x] + int32_t(x) : 2 * kDistInf;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t t = __shfl_down(h, o, 64);
-            if (lane + o < 64) h = min(h, t);
-        }
-        h = min(h, carry);
-        carry = __shfl(h, 0, 64);
+        const int32_t h = min(wave_min_suffix(ok ? d[x] + int32_t(x) : 2 * kDistInf), carry);
+        carry = __builtin_amdgcn_readlane(h, 0);
         if (ok) d[x] = min(h - int32_t(x), kDistInf);
     }
 }
@@ -922,6 +1019,25 @@ static Faces faces_of(const GridDev& g, float* u, float* v, float* w) {
 
 int32_t ext_max_distance(const GridDev& g) { return g.nx + g.ny + g.nz + 3; }
 
+// The smallest SEG class of k_ext_lines_seg that holds the longest line along
+// `axis` (1 = y, 2 = z) in kLineSegWaves segments, or -1 (k_ext_lines then;
+// also for 2^31 faces or more, whose byte offsets exceed 32 bits).
+static int line_seg_class(const Faces& F, int axis) {
+    if (F.off[3] >= (1u << 31)) return -1;
+    uint32_t len = 0;
+    for (int c = 0; c < 3; ++c) len = std::max(len, axis == 1 ? F.ey[c] : F.ez[c]);
+    for (int k = 0; k < 4; ++k)
+        if (len <= uint32_t(kLineSegWaves * kLineSegSizes[k])) return k;
+    return -1;
+}
+
+static uint32_t line_seg_grid(const Faces& F, int axis) {
+    const uint32_t* pre = axis == 1 ? F.ylines : F.zlines;
+    uint32_t n = 0;
+    for (int c = 0; c < 3; ++c) n += (pre[c + 1] - pre[c] + 63) / 64;
+    return (n + kXcds - 1) / kXcds * kXcds;
+}
+
 hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, const float* v, const float* w,
                                uint16_t* dist) {
     const Faces F = faces_of(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
@@ -936,13 +1052,30 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
     else (void)launch_dims(k_ext_rows, "k_ext_rows", rg, dim3(kBlock), 0, s, F, dist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    probe_next_bytes(4.0 * nf);   // distance 2 read and written (the backward sweep re-reads it from cache)
-    (void)launch_dims(k_ext_lines<1>, "k_ext_lines", dim3((F.ylines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    probe_next_bytes(4.0 * nf);
-    (void)launch_dims(k_ext_lines<2>, "k_ext_lines", dim3((F.zlines[3] + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
-    return hipGetLastError();
+    for (int axis = 1; axis <= 2; ++axis) {
+        probe_next_bytes(4.0 * nf);   // distance 2 read and written
+        const int cls = line_seg_class(F, axis);
+        if (cls < 0) {
+            const uint32_t n = axis == 1 ? F.ylines[3] : F.zlines[3];
+            if (axis == 1) (void)launch_dims(k_ext_lines<1>, "k_ext_lines", dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
+            else (void)launch_dims(k_ext_lines<2>, "k_ext_lines", dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, F, dist);
+        } else {
+            const dim3 gr(line_seg_grid(F, axis)), bl(kLineSegThreads);
+            switch ((axis - 1) * 4 + cls) {
+                case 0: (void)launch_dims(k_ext_lines_seg<1, kLineSegSizes[0]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 1: (void)launch_dims(k_ext_lines_seg<1, kLineSegSizes[1]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 2: (void)launch_dims(k_ext_lines_seg<1, kLineSegSizes[2]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 3: (void)launch_dims(k_ext_lines_seg<1, kLineSegSizes[3]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 4: (void)launch_dims(k_ext_lines_seg<2, kLineSegSizes[0]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 5: (void)launch_dims(k_ext_lines_seg<2, kLineSegSizes[1]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 6: (void)launch_dims(k_ext_lines_seg<2, kLineSegSizes[2]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                default: (void)launch_dims(k_ext_lines_seg<2, kLineSegSizes[3]>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+            }
+        }
+        const hipError_t e2 = hipGetLastError();
+        if (e2 != hipSuccess) return e2;
+    }
+    return hipSuccess;
 }
 
 size_t ext_counts_size(int32_t K) { return (3 * size_t(kCopies) + 1) * (size_t(K) + 2) + 1; }
@@ -1026,9 +1159,27 @@ void rec_ext_device(std::vector<MRec>& out, const GridDev& g, float* u, float* v
     else if (exm <= 512) push_rec_e(out, kMkExtRowsReg8, MExtRowsReg<8>{F, dist, rg}, rg);
     else if (exm <= 1024) push_rec_e(out, kMkExtRowsReg16, MExtRowsReg<16>{F, dist, rg}, rg);
     else push_rec_e(out, kMkExtRows, MExtRows{F, dist, rg}, rg);
-    const uint32_t g1 = (F.ylines[3] + kBlock - 1) / kBlock, g2 = (F.zlines[3] + kBlock - 1) / kBlock;
-    push_rec_e(out, kMkExtLines1, MExtLines<1>{F, dist, g1}, g1);
-    push_rec_e(out, kMkExtLines2, MExtLines<2>{F, dist, g2}, g2);
+    for (int axis = 1; axis <= 2; ++axis) {
+        const int cls = line_seg_class(F, axis);
+        if (cls < 0) {
+            const uint32_t gl = ((axis == 1 ? F.ylines[3] : F.zlines[3]) + kBlock - 1) / kBlock;
+            if (axis == 1) push_rec_e(out, kMkExtLines1, MExtLines<1>{F, dist, gl}, gl);
+            else push_rec_e(out, kMkExtLines2, MExtLines<2>{F, dist, gl}, gl);
+            continue;
+        }
+        const uint32_t gl = line_seg_grid(F, axis);
+        const int32_t kind = kMkExtLinesSeg + (axis - 1) * 4 + cls;
+        switch ((axis - 1) * 4 + cls) {
+            case 0: push_rec_e(out, kind, MExtLinesSeg<1, kLineSegSizes[0]>{F, dist, gl}, gl); break;
+            case 1: push_rec_e(out, kind, MExtLinesSeg<1, kLineSegSizes[1]>{F, dist, gl}, gl); break;
+            case 2: push_rec_e(out, kind, MExtLinesSeg<1, kLineSegSizes[2]>{F, dist, gl}, gl); break;
+            case 3: push_rec_e(out, kind, MExtLinesSeg<1, kLineSegSizes[3]>{F, dist, gl}, gl); break;
+            case 4: push_rec_e(out, kind, MExtLinesSeg<2, kLineSegSizes[0]>{F, dist, gl}, gl); break;
+            case 5: push_rec_e(out, kind, MExtLinesSeg<2, kLineSegSizes[1]>{F, dist, gl}, gl); break;
+            case 6: push_rec_e(out, kind, MExtLinesSeg<2, kLineSegSizes[2]>{F, dist, gl}, gl); break;
+            default: push_rec_e(out, kind, MExtLinesSeg<2, kLineSegSizes[3]>{F, dist, gl}, gl); break;
+        }
+    }
     const uint32_t total = F.off[3];
     const uint32_t K2 = uint32_t(K) + 2;
     const SortBufs B = sort_bufs(counts, K);
@@ -1147,6 +1298,14 @@ hipError_t launch_multi_ext(hipStream_t s, int32_t kind, const void* tab, const 
         case kMkExtRowsReg16: return launch_me<MExtRowsReg<16>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
         case kMkExtLines1: return launch_me<MExtLines<1>>(s, "k_multi_ext_lines", tab, map, nblocks, active);
         case kMkExtLines2: return launch_me<MExtLines<2>>(s, "k_multi_ext_lines", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 0: return launch_me<MExtLinesSeg<1, kLineSegSizes[0]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 1: return launch_me<MExtLinesSeg<1, kLineSegSizes[1]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 2: return launch_me<MExtLinesSeg<1, kLineSegSizes[2]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 3: return launch_me<MExtLinesSeg<1, kLineSegSizes[3]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 4: return launch_me<MExtLinesSeg<2, kLineSegSizes[0]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 5: return launch_me<MExtLinesSeg<2, kLineSegSizes[1]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 6: return launch_me<MExtLinesSeg<2, kLineSegSizes[2]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesSeg + 7: return launch_me<MExtLinesSeg<2, kLineSegSizes[3]>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
         case kMkExtZero: return launch_me<MExtZero>(s, "k_multi_ext_zero", tab, map, nblocks, active);
         case kMkExtCount: return launch_me<MExtCount>(s, "k_multi_ext_count", tab, map, nblocks, active);
         case kMkExtScan: return launch_me<MExtScan>(s, "k_multi_ext_scan", tab, map, nblocks, active);

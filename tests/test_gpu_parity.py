@@ -157,11 +157,14 @@ def test_extrapolate_bitexact(ctx, orc, layers):
         ctx.free(p)
 
 
-@pytest.mark.parametrize("res", [(300, 20, 18), (600, 9, 7), (1100, 4, 3), (8, 530, 6), (7, 6, 520)])
+@pytest.mark.parametrize("res", [(300, 20, 18), (600, 9, 7), (1100, 4, 3), (8, 530, 6), (7, 6, 520),
+                                 (6, 300, 5), (6, 900, 5), (5, 4, 1030)])
 def test_extrapolate_bitexact_long_lines(ctx, orc, res):
-    """Grids whose rows select the other distance-transform kernels: rows of
-    301 and 601 faces (8 and 16 register chunks per lane), of 1101 (the
-    two-pass row kernel); y and z lines of 531 and 521 faces."""
+    """Grids whose rows and lines select the other distance-transform
+    kernels: rows of 301 and 601 faces (8 and 16 register chunks per lane), of
+    1101 (the two-pass row kernel); y and z lines of 301, 531, 521 and 901
+    faces (the segmented line kernel's classes up to 320, 576 and 1024 faces)
+    and z lines of 1031 (the two-sweep line kernel)."""
     rng = np.random.default_rng(sum(res))
     u, v, w = rand_vel(rng, res, 1.0)
     for a in (u, v, w):
