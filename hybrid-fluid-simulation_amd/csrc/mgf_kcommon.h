@@ -24,6 +24,10 @@ __device__ __forceinline__ uint32_t xcd_of(uint32_t b, uint32_t G) {
     return (b % kXcds) * (G / kXcds) + b / kXcds;
 }
 
+// Word 3 of a raw buffer resource (__builtin_amdgcn_make_buffer_rsrc) on
+// gfx9: 32-bit data format, no swizzle.
+constexpr int32_t kBufWord3 = 0x00020000;
+
 inline uint32_t grid_for(size_t n) {
     size_t g = (n + kBlock - 1) / kBlock;
     g = (g + kXcds - 1) / kXcds * kXcds;

@@ -228,8 +228,6 @@ constexpr int kLineSegWaves = 16;
 constexpr int kLineSegThreads = 64 * kLineSegWaves;
 constexpr int kLineSegSizes[4] = {12, 20, 36, 64};   // SEG classes: lines of at most 16 SEG faces
 constexpr int32_t kSegNone = 1 << 20;                 // past the line's end: above any d(j) +- j
-// Word 3 of a raw buffer resource on gfx9 (32-bit data format, no swizzle).
-constexpr int32_t kBufWord3 = 0x00020000;
 
 // Stripes of 64 lines in component c (the last one partial).
 template <int AXIS>
