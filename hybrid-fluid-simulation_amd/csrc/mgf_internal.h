@@ -426,7 +426,7 @@ enum MultiKind : int32_t {
     kMkRefine0, kMkCodes, kMkFlatWet, kMkCoarsen, kMkMarchWet, kMkMarchPlan,
     kMkAdvectFc, kMkAdvectCc, kMkAdvectCc4, kMkMarkers, kMkRhs, kMkProject, kMkCopyWord,
     kMkExtRows, kMkExtRowsReg4, kMkExtRowsReg8, kMkExtRowsReg16, kMkExtLines1, kMkExtLines2, kMkExtZero, kMkExtCount,
-    kMkExtScan, kMkExtScatter, kMkExtTail, kMkCopyOffs,
+    kMkExtScan, kMkExtScatter, kMkExtTail, kMkCopyOffs, kMkExtRowsReg5,
     kMkExtLinesSeg,                          // 8 kinds: (axis - 1) * 4 + segment size class
     kMkExtLinesSegLast = kMkExtLinesSeg + 7,
     kMkCount

@@ -317,50 +317,70 @@ struct MExtLinesSeg {
     __device__ void run(uint32_t vb) const { k_ext_lines_seg_run<AXIS, SEG>(F, dist, xcd_of(vb, G), vb); }
 };
 
-// 1a'. The same row transform with the row held in registers: NCH chunks of
-// 64 faces per lane (rows of at most 64 NCH faces), one read of the faces and
-// one write of the distances (the two-pass kernel above re-reads and
-// re-writes them).  The same operations, so the same distances.
+// 1a'. The row transform from a ballot of the known faces: a row's
+// distances start as 0 (known) or INF, and the two min-plus sweeps of
+// k_ext_rows leave at x  min(x - p, n - x, INF), p and n the nearest known
+// faces at or before and at or after x (INF where neither exists).  One wave a
+// row of at most 64 NCH faces, held in registers; per chunk of 64 faces the
+// wave takes the ballot of its known lanes, a lane finds p and n in it with a
+// leading- and a trailing-zero count, and the nearest known faces in the
+// chunks before and after come from scalar scans of the ballots.  The same
+// integers as the sweeps, from one read of the faces and one write of the
+// distances.  The faces and distances are raw buffers (the lane's face in a
+// 32-bit offset).  At 256^3 this takes 104 us against 125 us for wave-wide
+// min-plus scans of the same registers; two, four or eight rows a wave
+// measured slower.
 template <int NCH>
 __device__ __forceinline__ void k_ext_rows_reg_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk, uint32_t slot) {
     (void)blk;
-    (void)slot;
-    const uint32_t wave = slot * (kBlock / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (wave >= F.rows[3]) return;
-    const int c = comp_of(F.rows, wave);
+    constexpr int32_t kFar = 1 << 20;   // no known face on that side
+    // wave-uniform (readfirstlane), so that the row's component, buffer and
+    // bounds live in scalar registers
+    const uint32_t row = slot * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (row >= F.rows[3]) return;
+    const int c = comp_of(F.rows, row);
     const uint32_t ex = pick(F.ex, c);
-    const uint32_t local = (wave - pick4(F.rows, c)) * ex;
-    const float* f = pick(F.f, c);
-    uint16_t* d = dist + pick4(F.off, c) + local;
-    const uint32_t nch = (ex + 63) / 64;
-    int32_t v[NCH];
+    const uint32_t local = (row - pick4(F.rows, c)) * ex;
+    const uint32_t fb = pick4(F.off, c) + local;   // the row's first face (global index)
+    const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
+        pick(F.f, c), int16_t(0), int32_t(4u * (c == 0 ? F.off[1] - F.off[0] : (c == 1 ? F.off[2] - F.off[1] : F.off[3] - F.off[2]))),
+        kBufWord3);
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dist, int16_t(0), int32_t(2u * F.off[3]), kBufWord3);
+    float fv[NCH];
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
-        const uint32_t x = uint32_t(ch) * 64 + uint32_t(lane);
-        v[ch] = (x < ex && f[local + x] != MGF_UNKNOWN) ? 0 : kFaceInf;
+        const uint32_t x = uint32_t(ch) * 64 + lane;
+        fv[ch] = x < ex ? __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rf, 4u * (local + x), 0, 0)) : MGF_UNKNOWN;
     }
-    // forward: v = min(x + min_{k<=x} (d(k) - k), INF)
-    int32_t carry = kFaceInf;
+    uint64_t m[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) m[ch] = __ballot(fv[ch] != MGF_UNKNOWN);
+    // nearest known faces in the chunks before and after each chunk (scalar)
+    int32_t pc[NCH], nc[NCH];
+    int32_t run = -kFar;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
-        if (uint32_t(ch) < nch) {
-            const int32_t x = ch * 64 + lane;
-            const int32_t g = min(wave_min_prefix(v[ch] - x), carry);
-            carry = __builtin_amdgcn_readlane(g, 63);
-            v[ch] = min(g + x, kFaceInf);
-        }
+        pc[ch] = run;
+        if (m[ch]) run = ch * 64 + 63 - __clzll(m[ch]);
     }
-    // backward: d = min(min_{k>=x} (v(k) + k) - x, INF)
-    carry = 2 * kFaceInf;
+    run = kFar;
 #pragma unroll
     for (int ch = NCH - 1; ch >= 0; --ch) {
-        if (uint32_t(ch) < nch) {
-            const int32_t x = ch * 64 + lane;
-            const bool ok = uint32_t(x) < ex;
-            const int32_t h = min(wave_min_suffix(ok ? v[ch] + x : 2 * kFaceInf), carry);
-            carry = __builtin_amdgcn_readlane(h, 0);
-            if (ok) d[x] = uint16_t(min(h - x, kFaceInf));
+        nc[ch] = run;
+        if (m[ch]) run = ch * 64 + __ffsll((unsigned long long)m[ch]) - 1;
+    }
+    const uint64_t below = (2ull << lane) - 1ull;   // lanes 0 .. lane
+    const uint64_t above = ~0ull << lane;            // lanes lane .. 63
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int32_t x = ch * 64 + int32_t(lane);
+        if (uint32_t(x) < ex) {
+            const uint64_t lo = m[ch] & below, hi = m[ch] & above;
+            const int32_t p = lo ? ch * 64 + 63 - __clzll(lo) : pc[ch];
+            const int32_t n = hi ? ch * 64 + __ffsll((unsigned long long)hi) - 1 : nc[ch];
+            const int32_t d = min(min(x - p, n - x), kFaceInf);
+            __builtin_amdgcn_raw_buffer_store_b16(uint16_t(d), rd, 2u * (fb + uint32_t(x)), 0, 0);
         }
     }
 }
@@ -1017,6 +1037,12 @@ static Faces faces_of(const GridDev& g, float* u, float* v, float* w) {
 
 int32_t ext_max_distance(const GridDev& g) { return g.nx + g.ny + g.nz + 3; }
 
+// Work-groups of k_ext_rows_reg: a row a wave, 4 waves a group.
+static uint32_t rows_grid(const Faces& F, int nch) {
+    (void)nch;
+    return (F.rows[3] + 3) / 4;
+}
+
 // The smallest SEG class of k_ext_lines_seg that holds the longest line along
 // `axis` (1 = y, 2 = z) in kLineSegWaves segments, or -1 (k_ext_lines then;
 // also for 2^31 faces or more, whose byte offsets exceed 32 bits).
@@ -1041,13 +1067,14 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
     const Faces F = faces_of(g, const_cast<float*>(u), const_cast<float*>(v), const_cast<float*>(w));
     // rows: held in registers when they fit 16 chunks of 64 faces
     const uint32_t exm = F.ex[0];   // the x-faces have the longest rows
-    const dim3 rg((F.rows[3] + 3) / 4);
     const double nf = double(F.off[3]);
     probe_next_bytes(6.0 * nf);   // face value 4 read, distance 2 written
-    if (exm <= 256) (void)launch_dims(k_ext_rows_reg<4>, "k_ext_rows_reg", rg, dim3(kBlock), 0, s, F, dist);
-    else if (exm <= 512) (void)launch_dims(k_ext_rows_reg<8>, "k_ext_rows_reg", rg, dim3(kBlock), 0, s, F, dist);
-    else if (exm <= 1024) (void)launch_dims(k_ext_rows_reg<16>, "k_ext_rows_reg", rg, dim3(kBlock), 0, s, F, dist);
-    else (void)launch_dims(k_ext_rows, "k_ext_rows", rg, dim3(kBlock), 0, s, F, dist);
+    const dim3 b(kBlock);
+    if (exm <= 256) (void)launch_dims(k_ext_rows_reg<4>, "k_ext_rows_reg", dim3(rows_grid(F, 4)), b, 0, s, F, dist);
+    else if (exm <= 320) (void)launch_dims(k_ext_rows_reg<5>, "k_ext_rows_reg", dim3(rows_grid(F, 5)), b, 0, s, F, dist);
+    else if (exm <= 512) (void)launch_dims(k_ext_rows_reg<8>, "k_ext_rows_reg", dim3(rows_grid(F, 8)), b, 0, s, F, dist);
+    else if (exm <= 1024) (void)launch_dims(k_ext_rows_reg<16>, "k_ext_rows_reg", dim3(rows_grid(F, 16)), b, 0, s, F, dist);
+    else (void)launch_dims(k_ext_rows, "k_ext_rows", dim3((F.rows[3] + 3) / 4), b, 0, s, F, dist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     for (int axis = 1; axis <= 2; ++axis) {
@@ -1153,9 +1180,10 @@ void rec_ext_device(std::vector<MRec>& out, const GridDev& g, float* u, float* v
     const Faces F = faces_of(g, u, v, w);
     const uint32_t exm = F.ex[0];
     const uint32_t rg = (F.rows[3] + 3) / 4;
-    if (exm <= 256) push_rec_e(out, kMkExtRowsReg4, MExtRowsReg<4>{F, dist, rg}, rg);
-    else if (exm <= 512) push_rec_e(out, kMkExtRowsReg8, MExtRowsReg<8>{F, dist, rg}, rg);
-    else if (exm <= 1024) push_rec_e(out, kMkExtRowsReg16, MExtRowsReg<16>{F, dist, rg}, rg);
+    if (exm <= 256) push_rec_e(out, kMkExtRowsReg4, MExtRowsReg<4>{F, dist, rows_grid(F, 4)}, rows_grid(F, 4));
+    else if (exm <= 320) push_rec_e(out, kMkExtRowsReg5, MExtRowsReg<5>{F, dist, rows_grid(F, 5)}, rows_grid(F, 5));
+    else if (exm <= 512) push_rec_e(out, kMkExtRowsReg8, MExtRowsReg<8>{F, dist, rows_grid(F, 8)}, rows_grid(F, 8));
+    else if (exm <= 1024) push_rec_e(out, kMkExtRowsReg16, MExtRowsReg<16>{F, dist, rows_grid(F, 16)}, rows_grid(F, 16));
     else push_rec_e(out, kMkExtRows, MExtRows{F, dist, rg}, rg);
     for (int axis = 1; axis <= 2; ++axis) {
         const int cls = line_seg_class(F, axis);
@@ -1292,6 +1320,7 @@ hipError_t launch_multi_ext(hipStream_t s, int32_t kind, const void* tab, const 
     switch (kind) {
         case kMkExtRows: return launch_me<MExtRows>(s, "k_multi_ext_rows", tab, map, nblocks, active);
         case kMkExtRowsReg4: return launch_me<MExtRowsReg<4>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
+        case kMkExtRowsReg5: return launch_me<MExtRowsReg<5>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
         case kMkExtRowsReg8: return launch_me<MExtRowsReg<8>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
         case kMkExtRowsReg16: return launch_me<MExtRowsReg<16>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
         case kMkExtLines1: return launch_me<MExtLines<1>>(s, "k_multi_ext_lines", tab, map, nblocks, active);
