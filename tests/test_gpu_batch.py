@@ -124,6 +124,36 @@ def test_batch_rejects_bad_arguments(ctx):
     a.close()
 
 
+def elongated(nxyz):
+    """A scene of nx x ny x nz interior cells (spacing 1/64) with a fluid
+    block over the lower part of x and y and the full z extent."""
+    h = 1.0 / 64
+    ext = [v * float(np.float32(h)) for v in nxyz]
+    return scenes.SceneSpec(nxyz, 2e-3, (0.0, 0.0, 0.0), (0.4 * ext[0], 0.5 * ext[1], ext[2]), [],
+                            "elongated_%dx%dx%d" % tuple(nxyz), h=h)
+
+
+def test_batch_long_rows_and_lines_match_scenes_stepped_alone(ctx):
+    """Scenes whose x rows (303 faces) and y and z lines (533 and 903 faces)
+    select the distance transform's larger kernel classes: stepped together
+    in lockstep (multi-scene records of those classes), their fields equal the
+    same scenes stepped alone, bit for bit."""
+    sp = [elongated((300, 10, 8)), elongated((9, 530, 8)), elongated((8, 9, 900))]
+    kw = desc_kw()
+    solo = [scenes.build_gpu_scene(mgf, ctx, p, **kw) for p in sp]
+    batch = [scenes.build_gpu_scene(mgf, ctx, p, **kw) for p in sp]
+    nsteps = 2
+    ref = [s.step(nsteps) for s in solo]
+    status, stats, _ = mgf.batch_step(ctx, batch, nsteps, workers=2)
+    assert status == [mgf.OK] * len(sp), status
+    for i, p in enumerate(sp):
+        assert stats[i].steps == nsteps and stats[i].converged == 1, p.name
+        assert stats[i].cg_iterations == ref[i]["cg_iterations"], (p.name, stats[i].cg_iterations)
+        assert_same(snapshot(batch[i]), snapshot(solo[i]), p.name)
+    for s in solo + batch:
+        s.close()
+
+
 def test_scene_bytes_for_matches_a_stepped_scene(ctx):
     for p in specs(3, seed0=500):
         d = mgf.make_desc(p.res, p.origin, p.spacing, p.dt, **desc_kw())
