@@ -24,6 +24,9 @@ struct GridDev {
     // reciprocal is a normal float (set by set_recip)
     float rhx = 0, rhy = 0, rhz = 0;
     int pow2 = 0;
+    // 1/d of the index decodes' divisors (cells: nx, nx ny; u faces: nx + 1,
+    // (nx + 1) ny; v faces: nx, nx (ny + 1); w faces as cells), set by set_recip
+    double r_nx = 0, r_nxy = 0, r_nx1 = 0, r_nx1y = 0, r_nxy1 = 0;
 };
 
 // Division by a spacing h = 2^k and multiplication by 2^-k round the same
@@ -39,6 +42,35 @@ inline void set_recip(GridDev& g) {
     };
     const bool a = p2(g.hx, g.rhx), b = p2(g.hy, g.rhy), c = p2(g.hz, g.rhz);
     g.pow2 = (a && b && c) ? 1 : 0;
+    const double nx = double(g.nx), ny = double(g.ny);
+    g.r_nx = 1.0 / nx;
+    g.r_nxy = 1.0 / (nx * ny);
+    g.r_nx1 = 1.0 / (nx + 1.0);
+    g.r_nx1y = 1.0 / ((nx + 1.0) * ny);
+    g.r_nxy1 = 1.0 / (nx * (ny + 1.0));
+}
+
+// a / d for a < 2^32, d >= 1, from rd = 1/d in FP64: a * rd is within one of
+// the quotient and is corrected once, so the quotient is exact (the integer
+// divide's instruction sequence avoided).
+__device__ __forceinline__ uint32_t udiv_rd(uint32_t a, uint32_t d, double rd) {
+    uint32_t q = uint32_t(double(a) * rd);
+    const uint32_t r = a - q * d;
+    if (int32_t(r) < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+}
+
+// (x, y, z) of index i in an ex x ey x .. array: rex = 1/ex, rexy = 1/(ex ey).
+__device__ __forceinline__ void decode3(uint32_t i, uint32_t ex, uint32_t ey, double rex, double rexy, int& x,
+                                        int& y, int& z) {
+    const uint32_t exy = ex * ey;
+    const uint32_t zz = udiv_rd(i, exy, rexy);
+    const uint32_t rem = i - zz * exy;
+    const uint32_t yy = udiv_rd(rem, ex, rex);
+    x = int(rem - yy * ex);
+    y = int(yy);
+    z = int(zz);
 }
 
 __device__ __forceinline__ float div_h(float x, float h, float rh, int pow2) {

@@ -55,7 +55,8 @@ __device__ __forceinline__ void k_advect_fc_run(GridDev g, Vel in, Vel V, RkCoef
     else if (i < nfx + nfy + nfz) { c = 2; i -= nfx + nfy; }
     else return;
     const uint32_t ex = uint32_t(g.nx + (c == 0)), ey = uint32_t(g.ny + (c == 1));
-    const int x = int(i % ex), y = int((i / ex) % ey), z = int(i / (ex * ey));
+    int x, y, z;
+    decode3(i, ex, ey, c == 0 ? g.r_nx1 : g.r_nx, c == 0 ? g.r_nx1y : (c == 1 ? g.r_nxy1 : g.r_nxy), x, y, z);
     float px = c == 0 ? fpos(x, g.hx, g.ox) : cpos(x, g.hx, g.ox);
     float py = c == 1 ? fpos(y, g.hy, g.oy) : cpos(y, g.hy, g.oy);
     float pz = c == 2 ? fpos(z, g.hz, g.oz) : cpos(z, g.hz, g.oz);
@@ -104,7 +105,8 @@ __device__ __forceinline__ void k_advect_cc_run(GridDev g, const float* __restri
     const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
-    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    int x, y, z;
+    decode3(i, nx, ny, g.r_nx, g.r_nxy, x, y, z);
     float px = cpos(x, g.hx, g.ox), py = cpos(y, g.hy, g.oy), pz = cpos(z, g.hz, g.oz);
     backtrace(V, g, order, k, px, py, pz);
     out[i] = trilerp(in, g.nx, g.ny, g.nz, g.ox, g.oy, g.oz, g, px, py, pz);
@@ -142,7 +144,8 @@ __device__ __forceinline__ void k_advect_cc4_run(GridDev g, Cc4 F, Vel V, RkCoef
     const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
-    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    int x, y, z;
+    decode3(i, nx, ny, g.r_nx, g.r_nxy, x, y, z);
     float px = cpos(x, g.hx, g.ox), py = cpos(y, g.hy, g.oy), pz = cpos(z, g.hz, g.oz);
     backtrace(V, g, order, k, px, py, pz);
 #pragma unroll
@@ -523,7 +526,8 @@ __global__ void __launch_bounds__(kBlock) k_resample_solid(GridDev g, const floa
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
-    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    int x, y, z;
+    decode3(i, nx, ny, g.r_nx, g.r_nxy, x, y, z);
     const float cx = g.ox + 0.5f * g.hx + g.hx * float(x);
     const float cy = g.oy + 0.5f * g.hy + g.hy * float(y);
     const float cz = g.oz + 0.5f * g.hz + g.hz * float(z);
@@ -568,7 +572,8 @@ __global__ void __launch_bounds__(kBlock) k_solids_vel(GridDev g, const float* _
     else return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const uint32_t ex = uint32_t(nx + (c == 0)), ey = uint32_t(ny + (c == 1));
-    const int x = int(i % ex), y = int((i / ex) % ey), z = int(i / (ex * ey));
+    int x, y, z;
+    decode3(i, ex, ey, c == 0 ? g.r_nx1 : g.r_nx, c == 0 ? g.r_nx1y : (c == 1 ? g.r_nxy1 : g.r_nxy), x, y, z);
     int cell[2][3] = {{x, y, z}, {x, y, z}};  // [0] upper (same index), [1] lower
     cell[1][c] -= 1;
     const int up_lim = (c == 0 ? nx : (c == 1 ? ny : nz));
@@ -596,7 +601,8 @@ __global__ void __launch_bounds__(kBlock) k_fluid_box(GridDev g, float ax, float
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t nx = uint32_t(g.nx), ny = uint32_t(g.ny);
-    const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (nx * ny));
+    int x, y, z;
+    decode3(i, nx, ny, g.r_nx, g.r_nxy, x, y, z);
     const float cx = g.ox + 0.5f * g.hx + g.hx * float(x);
     const float cy = g.oy + 0.5f * g.hy + g.hy * float(y);
     const float cz = g.oz + 0.5f * g.hz + g.hz * float(z);
