@@ -218,8 +218,8 @@ __device__ __forceinline__ void k_rhs_run(GridDev g, Vel U, Vel S, const uint8_t
     const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    int x, y, z;
+    decode3(uint32_t(i), uint32_t(nx), uint32_t(ny), g.r_nx, g.r_nxy, x, y, z);
     if (m[i] != kFluid) { b[i] = 0.0f; return; }
     const uint32_t fl = lin3(x, y, z, nx + 1, ny, nz), fr = lin3(x + 1, y, z, nx + 1, ny, nz);
     const uint32_t fd = lin3(x, y, z, nx, ny + 1, nz), fu = lin3(x, y + 1, z, nx, ny + 1, nz);
@@ -263,8 +263,8 @@ __global__ void __launch_bounds__(kBlock) k_divergence(GridDev g, Vel U, float* 
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    int x, y, z;
+    decode3(uint32_t(i), uint32_t(nx), uint32_t(ny), g.r_nx, g.r_nxy, x, y, z);
     const float ul = U.u[lin3(x, y, z, nx + 1, ny, nz)], ur = U.u[lin3(x + 1, y, z, nx + 1, ny, nz)];
     const float vd = U.v[lin3(x, y, z, nx, ny + 1, nz)], vu = U.v[lin3(x, y + 1, z, nx, ny + 1, nz)];
     const float wb = U.w[lin3(x, y, z, nx, ny, nz + 1)], wf = U.w[lin3(x, y, z + 1, nx, ny, nz + 1)];
@@ -278,8 +278,8 @@ __global__ void __launch_bounds__(kBlock) k_matrix_a(GridDev g, const uint8_t* _
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    int x, y, z;
+    decode3(uint32_t(i), uint32_t(nx), uint32_t(ny), g.r_nx, g.r_nxy, x, y, z);
     MarkerView mv{m, nx, ny, nz};
     const Row r = matrix_row(mv, x, y, z, nx, ny, nz, sx, sy, sz);
     A[4 * size_t(i)] = r.d;
@@ -346,8 +346,8 @@ __global__ void __launch_bounds__(kBlock) k_matvec(GridDev g, const uint8_t* __r
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int x = int(i % uint32_t(nx)), yy = int((i / uint32_t(nx)) % uint32_t(ny)),
-              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    int x, yy, z;
+    decode3(uint32_t(i), uint32_t(nx), uint32_t(ny), g.r_nx, g.r_nxy, x, yy, z);
     const Stencil st = stencil_at(m, x, yy, z, nx, ny, nz, sx, sy, sz);
     double acc = double(st.coef[0] * xv[st.idx[0]]);
 #pragma unroll
@@ -416,8 +416,8 @@ __global__ void __launch_bounds__(kBlock) k_residual32(GridDev g, const uint8_t*
     double s[2] = {0.0, 0.0};
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-                  z = int(i / (uint32_t(nx) * uint32_t(ny)));
+        int x, y, z;
+        decode3(uint32_t(i), uint32_t(nx), uint32_t(ny), g.r_nx, g.r_nxy, x, y, z);
         const Stencil st = stencil_at(m, x, y, z, nx, ny, nz, sx, sy, sz);
         double ax = double(st.coef[0]) * double(xv[st.idx[0]]);
 #pragma unroll
@@ -450,8 +450,8 @@ __device__ __forceinline__ void k_project_run(GridDev g, const uint8_t* __restri
     const uint32_t i = blk * kBlock + threadIdx.x;
     if (i >= n) return;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int x = int(i % uint32_t(nx)), y = int((i / uint32_t(nx)) % uint32_t(ny)),
-              z = int(i / (uint32_t(nx) * uint32_t(ny)));
+    int x, y, z;
+    decode3(uint32_t(i), uint32_t(nx), uint32_t(ny), g.r_nx, g.r_nxy, x, y, z);
     const uint32_t l = lin3(x - 1, y, z, nx, ny, nz), d = lin3(x, y - 1, z, nx, ny, nz),
                    bk = lin3(x, y, z - 1, nx, ny, nz);
     const uint8_t mc = m[i], ml = m[l], md = m[d], mb = m[bk];
