@@ -692,13 +692,26 @@ struct MExtScatter {
 // the reference's order (-x, +x, -y, +y, -z, +z) with one divide; the
 // conditional sums as selects (an unused neighbour leaves the sum untouched,
 // a -0.0 included).
-__device__ __forceinline__ float layer_mean(const float* f, uint32_t i, uint32_t m, uint32_t ex, uint32_t sz) {
+// The flagged neighbours' values (0 for the others, never summed): only the
+// flagged ones are loaded.
+__device__ __forceinline__ void layer_loads(const float* f, uint32_t i, uint32_t m, uint32_t ex, uint32_t sz,
+                                            float (&fv)[6]) {
     const uint32_t j[6] = {(m & 1u) ? i - 1 : i, (m & 2u) ? i + 1 : i, (m & 4u) ? i - ex : i,
                            (m & 8u) ? i + ex : i, (m & 16u) ? i - sz : i, (m & 32u) ? i + sz : i};
-    float fv[6];
-    // only the flagged neighbours are loaded (the others are never summed)
 #pragma unroll
     for (int q = 0; q < 6; ++q) fv[q] = ((m >> q) & 1u) ? f[j[q]] : 0.0f;
+}
+
+// Every loaded value in a register here, before any is used: without this
+// the compiler moves the first sum (0 + the first value) into that load's
+// branch, whose wait then holds back the other loads by a memory round trip.
+template <int N>
+__device__ __forceinline__ void loads_done(float (&v)[N]) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) asm volatile("" : "+v"(v[q]));
+}
+
+__device__ __forceinline__ float layer_sum(uint32_t m, const float (&fv)[6]) {
     float acc = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -708,6 +721,13 @@ __device__ __forceinline__ float layer_mean(const float* f, uint32_t i, uint32_t
     }
     if (cnt != 0.0f) acc /= cnt;
     return acc;
+}
+
+__device__ __forceinline__ float layer_mean(const float* f, uint32_t i, uint32_t m, uint32_t ex, uint32_t sz) {
+    float fv[6];
+    layer_loads(f, i, m, ex, sz, fv);
+    loads_done(fv);
+    return layer_sum(m, fv);
 }
 
 __global__ void __launch_bounds__(kBlock) k_ext_layer(Faces F, const uint32_t* __restrict__ list,
@@ -753,9 +773,15 @@ __global__ void __launch_bounds__(kBlock) k_ext_layer2(Faces F, const uint32_t* 
     uint32_t mn[6];
 #pragma unroll
     for (int q = 0; q < 6; ++q) mn[q] = ((m >> q) & 1u) ? uint32_t(fmask[o + j[q]]) : 0u;
+    // the layer-L neighbours' means: all their loads, then the sums
+    float nv[6][6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) layer_loads(f, j[q], mn[q], ex, sz, nv[q]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) loads_done(nv[q]);
     float fv[6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) fv[q] = layer_mean(f, j[q], mn[q], ex, sz);
+    for (int q = 0; q < 6; ++q) fv[q] = layer_sum(mn[q], nv[q]);
     float acc = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
@@ -1250,15 +1276,23 @@ __global__ void __launch_bounds__(kBlock) k_multi_layer(const MLayer* __restrict
         gi[k] = a.list[tc];
         m[k] = t < end ? uint32_t(a.lmask[tc]) : 0u;
     }
+    // every entry's loads, then the sums and stores
+    float fv[kLayerCpt][6];
+#pragma unroll
+    for (int k = 0; k < kLayerCpt; ++k) {
+        const int c = comp_of(a.F.off, gi[k]);
+        const uint32_t i = gi[k] - pick4(a.F.off, c);
+        const uint32_t ex = pick(a.F.ex, c), sz = ex * pick(a.F.ey, c);
+        layer_loads(pick(a.F.f, c), i, m[k], ex, sz, fv[k]);   // m = 0 past the end: nothing loaded
+    }
+#pragma unroll
+    for (int k = 0; k < kLayerCpt; ++k) loads_done(fv[k]);
 #pragma unroll
     for (int k = 0; k < kLayerCpt; ++k) {
         const uint32_t t = base + uint32_t(k * kBlock);
         if (t >= end) continue;
         const int c = comp_of(a.F.off, gi[k]);
-        const uint32_t i = gi[k] - pick4(a.F.off, c);
-        const uint32_t ex = pick(a.F.ex, c), sz = ex * pick(a.F.ey, c);
-        float* f = pick(a.F.f, c);
-        f[i] = layer_mean(f, i, m[k], ex, sz);
+        pick(a.F.f, c)[gi[k] - pick4(a.F.off, c)] = layer_sum(m[k], fv[k]);
     }
 }
 
