@@ -1190,8 +1190,10 @@ struct LockPlan {
 };
 
 // Merges per-scene keyed records into launches and lays out their tables and
-// maps (appended to `blob` at 256-byte boundaries).
-LSeq merge_seq(const std::vector<std::vector<KRec>>& per, std::vector<unsigned char>& blob) {
+// maps from byte offset `at` on, at 256-byte boundaries, advancing `at` past
+// them.  With blob null only the offsets are computed (the size to
+// allocate); otherwise blob holds the final size, zero where the tables go.
+LSeq merge_seq(const std::vector<std::vector<KRec>>& per, unsigned char* blob, size_t& at) {
     struct Key {
         int a, b, c, d, kind;
         bool operator<(const Key& o) const {
@@ -1206,7 +1208,7 @@ LSeq merge_seq(const std::vector<std::vector<KRec>>& per, std::vector<unsigned c
     for (size_t i = 0; i < per.size(); ++i)
         for (const KRec& k : per[i]) all.push_back({Key{k.a, k.b, k.c, k.d, k.r.kind}, {int(i), &k.r}});
     std::stable_sort(all.begin(), all.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-    auto align = [&]() { blob.resize((blob.size() + 255) / 256 * 256); };
+    auto align = [&]() { at = (at + 255) / 256 * 256; };
     LSeq seq;
     for (size_t i = 0; i < all.size();) {
         size_t j = i;
@@ -1215,9 +1217,9 @@ LSeq merge_seq(const std::vector<std::vector<KRec>>& per, std::vector<unsigned c
         align();
         // one entry per scene of the batch, at the scene's index (the map
         // names the scene); scenes without this launch leave theirs zero
-        sl.tab = blob.size();
+        sl.tab = at;
         const size_t bytes = all[i].second.second->bytes;
-        blob.resize(blob.size() + bytes * per.size(), 0);
+        at += bytes * per.size();
         std::vector<uint8_t> seen(per.size(), 0);
         for (size_t q = i; q < j; ++q) {
             const MRec* r = all[q].second.second;
@@ -1227,20 +1229,23 @@ LSeq merge_seq(const std::vector<std::vector<KRec>>& per, std::vector<unsigned c
                 seq.clear();
                 return seq;
             }
-            std::memcpy(blob.data() + sl.tab + scene * bytes, r->args, bytes);
+            if (blob) std::memcpy(blob + sl.tab + scene * bytes, r->args, bytes);
             sl.shmem = std::max(sl.shmem, r->shmem);
+            sl.nblocks += r->blocks;
         }
         align();
-        sl.map = blob.size();
-        for (size_t q = i; q < j; ++q) {
-            const MRec* r = all[q].second.second;
-            const uint32_t scene = uint32_t(all[q].second.first);
-            for (uint32_t b = 0; b < r->blocks; ++b) {
-                const uint32_t e = (scene << kMapShift) | b;
-                const unsigned char* pe = reinterpret_cast<const unsigned char*>(&e);
-                blob.insert(blob.end(), pe, pe + 4);
+        sl.map = at;
+        at += 4 * size_t(sl.nblocks);
+        if (blob) {
+            unsigned char* mp = blob + sl.map;
+            for (size_t q = i; q < j; ++q) {
+                const uint32_t scene = uint32_t(all[q].second.first) << kMapShift;
+                const uint32_t nb = all[q].second.second->blocks;
+                for (uint32_t b = 0; b < nb; ++b, mp += 4) {
+                    const uint32_t e = scene | b;
+                    std::memcpy(mp, &e, 4);
+                }
             }
-            sl.nblocks += r->blocks;
         }
         seq.push_back(sl);
         i = j;
@@ -1295,6 +1300,8 @@ mgf_status build_phase(const std::vector<mgf_scene>& sc, const std::vector<uint8
     Q.layer_tab = (Q.offs_words * sizeof(uint32_t) + 255) / 256 * 256;
     const size_t head = Q.layer_tab + (n * layer_arg_bytes() + 255) / 256 * 256;
     std::vector<std::vector<KRec>> pre(n), post(n);
+    // the layout first (records with null addresses), then allocate and fill
+    size_t total = 0;
     for (int pass = 0; pass < 2; ++pass) {
         uint32_t* packed = reinterpret_cast<uint32_t*>(pass == 0 ? nullptr : Q.dev);
         for (size_t i = 0; i < n; ++i) {
@@ -1302,25 +1309,29 @@ mgf_status build_phase(const std::vector<mgf_scene>& sc, const std::vector<uint8
             post[i].clear();
             TRY(phase_records(sc[i], cur[i], pre[i], post[i], packed + Q.offs_at[i]));
         }
-        std::vector<unsigned char> blob(head, 0);
-        for (size_t i = 0; i < n; ++i) {
+        std::vector<unsigned char> blob(pass == 0 ? 0 : total, 0);
+        unsigned char* bp = pass == 0 ? nullptr : blob.data();
+        for (size_t i = 0; bp && i < n; ++i) {
             mgf_scene s = sc[i];
             const int c1 = 1 - cur[i];
-            layer_args(blob.data() + Q.layer_tab + i * layer_arg_bytes(), s->g, s->u[c1], s->v[c1], s->w[c1], s->list,
+            layer_args(bp + Q.layer_tab + i * layer_arg_bytes(), s->g, s->u[c1], s->v[c1], s->w[c1], s->list,
                        s->lmask, s->counts, ext_max_distance(s->g));
         }
-        Q.pre = merge_seq(pre, blob);
-        Q.post = merge_seq(post, blob);
+        size_t at = head;
+        Q.pre = merge_seq(pre, bp, at);
+        Q.post = merge_seq(post, bp, at);
         if (Q.pre.empty() || Q.post.empty()) return fail(MGF_ERR_INVALID, "lockstep phases: a scene has two records under one key");
         if (pass == 0) {
+            total = at;
             Q.ctx = sc[0]->ctx;
-            Q.dev = static_cast<unsigned char*>(pool_get(Q.ctx, blob.size(), false));
+            Q.dev = static_cast<unsigned char*>(pool_get(Q.ctx, total, false));
             if (!Q.dev) return fail(MGF_ERR_NOMEM, "lockstep phases: hipMalloc failed");
             Q.offs_host = static_cast<uint32_t*>(pool_get(Q.ctx, std::max<size_t>(Q.offs_words, 1) * 4, true));
             if (!Q.offs_host) return fail(MGF_ERR_NOMEM, "lockstep phases: hipHostMalloc failed");
             continue;
         }
-        MGF_HIP(hipMemcpyAsync(Q.dev, blob.data(), blob.size(), hipMemcpyHostToDevice, st));
+        if (at != total) return fail(MGF_ERR_INVALID, "lockstep phases: the layout changed between its passes");
+        MGF_HIP(hipMemcpyAsync(Q.dev, bp, total, hipMemcpyHostToDevice, st));
         MGF_HIP(hipStreamSynchronize(st));
     }
     return MGF_OK;
@@ -1339,6 +1350,7 @@ mgf_status build_plan(LockPlan& P) {
                  act_off = gout_off + (n * 2 * sizeof(double) + 255) / 256 * 256,
                  act_stride = (n + 255) / 256 * 256, head = act_off + act_stride;
     // size the tables first with null addresses, then allocate and fill
+    size_t total = 0;
     for (int pass = 0; pass < 2; ++pass) {
         unsigned char* base = pass == 0 ? nullptr : P.dev;
         CgScalars* gsc = reinterpret_cast<CgScalars*>(base + gsc_off);
@@ -1377,18 +1389,22 @@ mgf_status build_plan(LockPlan& P) {
             keyed(gat[i], r, 0, 0, 0);
         }
         std::vector<unsigned char>& blob = P.blob;
-        blob.assign(head, 0);
-        P.start_b = merge_seq(start_b, blob);
-        P.start_r = merge_seq(start_r, blob);
-        P.cg = merge_seq(cg, blob);
-        P.vcycle = merge_seq(vc, blob);
-        P.refine1 = merge_seq(ref1, blob);
-        P.refine2 = merge_seq(ref2, blob);
-        P.gather = merge_seq(gat, blob);
+        blob.assign(pass == 0 ? 0 : total, 0);
+        unsigned char* bp = pass == 0 ? nullptr : blob.data();
+        size_t at = head;
+        P.start_b = merge_seq(start_b, bp, at);
+        P.start_r = merge_seq(start_r, bp, at);
+        P.cg = merge_seq(cg, bp, at);
+        P.vcycle = merge_seq(vc, bp, at);
+        P.refine1 = merge_seq(ref1, bp, at);
+        P.refine2 = merge_seq(ref2, bp, at);
+        P.gather = merge_seq(gat, bp, at);
         for (const LSeq* q : {&P.start_b, &P.start_r, &P.cg, &P.vcycle, &P.refine1, &P.refine2, &P.gather})
             if (q->empty()) return fail(MGF_ERR_INVALID, "lockstep plan: a scene has two records under one key");
+        if (pass == 1 && at != total) return fail(MGF_ERR_INVALID, "lockstep plan: the layout changed between its passes");
+        total = at;
         if (pass == 0) {
-            P.dev = static_cast<unsigned char*>(pool_get(P.ctx, blob.size(), false));
+            P.dev = static_cast<unsigned char*>(pool_get(P.ctx, total, false));
             if (!P.dev) return fail(MGF_ERR_NOMEM, "lockstep plan: hipMalloc failed");
             const size_t hb = act_off + kActiveRing * act_stride;
             P.host = static_cast<unsigned char*>(pool_get(P.ctx, hb, true));
@@ -1956,6 +1972,44 @@ mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int
     mgf_status result = build_plan(P);
     if (result == MGF_OK) result = capture_iteration(P);
     if (result != MGF_OK) return fin(result);
+    if (nsteps >= 2) {
+        // Every scene flips its buffers each step, so the steps alternate
+        // between two phase plans.  Both are built now, the second on a
+        // helper thread with a stream of its own: built at the second step,
+        // it would hold the device idle for the host's whole build.  The
+        // scenes' hierarchies and schedules exist (lock_prepare) and nothing
+        // changes them while the two builds read them.
+        std::vector<uint8_t> ca(static_cast<size_t>(n)), cb(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            ca[size_t(i)] = uint8_t(sc[size_t(i)]->cur);
+            cb[size_t(i)] = uint8_t(1 - sc[size_t(i)]->cur);
+        }
+        phases.emplace_back(ca, std::make_unique<PhasePlan>());
+        PhasePlan* qa = phases.back().second.get();
+        phases.emplace_back(cb, std::make_unique<PhasePlan>());
+        PhasePlan* qb = phases.back().second.get();
+        hipStream_t sb = pool_stream(ctx);
+        if (!sb) return fin(fail(MGF_ERR_DEVICE, "hipStreamCreate failed"));
+        mgf_status rb = MGF_OK;
+        std::string eb;
+        std::thread helper([&]() {
+            if (hipSetDevice(ctx->device) != hipSuccess) {
+                (void)hipGetLastError();
+                rb = fail(MGF_ERR_DEVICE, "lockstep phases: hipSetDevice failed on the helper thread");
+            } else {
+                rb = build_phase(sc, cb, sb, *qb);
+            }
+            if (rb != MGF_OK) eb = mgf_last_error();
+        });
+        result = build_phase(sc, ca, P.stream, *qa);
+        helper.join();
+        pool_stream_put(ctx, sb);
+        if (result == MGF_OK && rb != MGF_OK) {
+            set_error(eb);
+            result = rb;
+        }
+        if (result != MGF_OK) return fin(result);
+    }
     std::vector<uint8_t> cur(static_cast<size_t>(n), 0);
     for (int32_t step = 0; step < nsteps && result == MGF_OK; ++step) {
         const double t0 = now_ms();
