@@ -1992,7 +1992,7 @@ mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int
         if (!sb) return fin(fail(MGF_ERR_DEVICE, "hipStreamCreate failed"));
         mgf_status rb = MGF_OK;
         std::string eb;
-        std::thread helper([&]() {
+        auto build_b = [&]() {
             if (hipSetDevice(ctx->device) != hipSuccess) {
                 (void)hipGetLastError();
                 rb = fail(MGF_ERR_DEVICE, "lockstep phases: hipSetDevice failed on the helper thread");
@@ -2000,9 +2000,17 @@ mgf_status batch_lockstep(mgf_context ctx, const std::vector<mgf_scene>& sc, int
                 rb = build_phase(sc, cb, sb, *qb);
             }
             if (rb != MGF_OK) eb = mgf_last_error();
-        });
+        };
+        // a helper that cannot be started leaves the build to this thread:
+        // nothing crosses the C boundary as an exception
+        std::thread helper;
+        try {
+            helper = std::thread(build_b);
+        } catch (...) {
+        }
         result = build_phase(sc, ca, P.stream, *qa);
-        helper.join();
+        if (helper.joinable()) helper.join();
+        else if (result == MGF_OK) build_b();
         pool_stream_put(ctx, sb);
         if (result == MGF_OK && rb != MGF_OK) {
             set_error(eb);
