@@ -429,6 +429,7 @@ enum MultiKind : int32_t {
     kMkExtScan, kMkExtScatter, kMkExtTail, kMkCopyOffs, kMkExtRowsReg5,
     kMkExtLinesSeg,                          // 8 kinds: (axis - 1) * 4 + segment size class
     kMkExtLinesSegLast = kMkExtLinesSeg + 7,
+    kMkExtRowsShort1, kMkExtRowsShort2,
     kMkCount
 };
 constexpr size_t kMRecBytes = 512;

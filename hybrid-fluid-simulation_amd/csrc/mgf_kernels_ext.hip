@@ -385,6 +385,97 @@ __device__ __forceinline__ void k_ext_rows_reg_run(Faces F, uint16_t* __restrict
     }
 }
 
+// Short rows (at most NCH * 64 faces, NCH 1 or 2): the launch is bound by its
+// wave count, one wave a row, so a wave takes kRowsPerWave consecutive rows,
+// all their loads issued before the first row's distances; each row's
+// arithmetic is the one above.
+constexpr int kRowsPerWave = 4;
+
+template <int NCH>
+__device__ __forceinline__ void k_ext_rows_short_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk,
+                                                     uint32_t slot) {
+    (void)blk;
+    constexpr int32_t kFar = 1 << 20;
+    const uint32_t row0 = (slot * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) * kRowsPerWave;
+    const uint32_t lane = threadIdx.x & 63;
+    if (row0 >= F.rows[3]) return;
+    const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dist, int16_t(0), int32_t(2u * F.off[3]), kBufWord3);
+    float fv[kRowsPerWave][NCH];
+    uint32_t exr[kRowsPerWave], fbr[kRowsPerWave];
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const uint32_t row = row0 + uint32_t(r);
+        const bool ok = row < F.rows[3];
+        const int c = comp_of(F.rows, ok ? row : row0);
+        const uint32_t ex = ok ? pick(F.ex, c) : 0u;   // a row past the end loads and stores nothing
+        const uint32_t local = ok ? (row - pick4(F.rows, c)) * ex : 0u;
+        exr[r] = ex;
+        fbr[r] = pick4(F.off, c) + local;
+        const __amdgpu_buffer_rsrc_t rf = __builtin_amdgcn_make_buffer_rsrc(
+            pick(F.f, c), int16_t(0),
+            int32_t(4u * (c == 0 ? F.off[1] - F.off[0] : (c == 1 ? F.off[2] - F.off[1] : F.off[3] - F.off[2]))), kBufWord3);
+        // loaded unconditionally (past the component's end the buffer reads
+        // 0), then the faces past the row replaced
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const uint32_t x = uint32_t(ch) * 64 + lane;
+            const float v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rf, 4u * (local + x), 0, 0));
+            fv[r][ch] = x < ex ? v : MGF_UNKNOWN;
+        }
+    }
+    // every row's values in registers before the first row's ballots
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r)
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) asm volatile("" : "+v"(fv[r][ch]));
+    const uint64_t below = (2ull << lane) - 1ull;   // lanes 0 .. lane
+    const uint64_t above = ~0ull << lane;            // lanes lane .. 63
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        uint64_t m[NCH];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) m[ch] = __ballot(fv[r][ch] != MGF_UNKNOWN);
+        int32_t pc[NCH], nc[NCH];
+        int32_t run = -kFar;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            pc[ch] = run;
+            if (m[ch]) run = ch * 64 + 63 - __clzll(m[ch]);
+        }
+        run = kFar;
+#pragma unroll
+        for (int ch = NCH - 1; ch >= 0; --ch) {
+            nc[ch] = run;
+            if (m[ch]) run = ch * 64 + __ffsll((unsigned long long)m[ch]) - 1;
+        }
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int32_t x = ch * 64 + int32_t(lane);
+            if (uint32_t(x) < exr[r]) {
+                const uint64_t lo = m[ch] & below, hi = m[ch] & above;
+                const int32_t p = lo ? ch * 64 + 63 - __clzll(lo) : pc[ch];
+                const int32_t n = hi ? ch * 64 + __ffsll((unsigned long long)hi) - 1 : nc[ch];
+                const int32_t d = min(min(x - p, n - x), kFaceInf);
+                __builtin_amdgcn_raw_buffer_store_b16(uint16_t(d), rd, 2u * (fbr[r] + uint32_t(x)), 0, 0);
+            }
+        }
+    }
+}
+
+template <int NCH>
+__global__ void __launch_bounds__(kBlock) k_ext_rows_short(Faces F, uint16_t* __restrict__ dist) {
+    k_ext_rows_short_run<NCH>(F, dist, xcd_block(), blockIdx.x);
+}
+
+template <int NCH>
+struct MExtRowsShort {
+    static constexpr int kThreads = kBlock;
+    Faces F;
+    uint16_t* dist;
+    uint32_t G;
+    __device__ void run(uint32_t vb) const { k_ext_rows_short_run<NCH>(F, dist, xcd_of(vb, G), vb); }
+};
+
 template <int NCH>
 __global__ void __launch_bounds__(kBlock) k_ext_rows_reg(Faces F, uint16_t* __restrict__ dist) {
     k_ext_rows_reg_run<NCH>(F, dist, xcd_block(), blockIdx.x);
@@ -1069,6 +1160,11 @@ static uint32_t rows_grid(const Faces& F, int nch) {
     return (F.rows[3] + 3) / 4;
 }
 
+// Work-groups of k_ext_rows_short: kRowsPerWave rows a wave, 4 waves a group.
+static uint32_t rows_short_grid(const Faces& F) {
+    return (F.rows[3] + 4 * kRowsPerWave - 1) / (4 * kRowsPerWave);
+}
+
 // The smallest SEG class of k_ext_lines_seg that holds the longest line along
 // `axis` (1 = y, 2 = z) in kLineSegWaves segments, or -1 (k_ext_lines then;
 // also for 2^31 faces or more, whose byte offsets exceed 32 bits).
@@ -1096,7 +1192,9 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
     const double nf = double(F.off[3]);
     probe_next_bytes(6.0 * nf);   // face value 4 read, distance 2 written
     const dim3 b(kBlock);
-    if (exm <= 256) (void)launch_dims(k_ext_rows_reg<4>, "k_ext_rows_reg", dim3(rows_grid(F, 4)), b, 0, s, F, dist);
+    if (exm <= 64) (void)launch_dims(k_ext_rows_short<1>, "k_ext_rows_short", dim3(rows_short_grid(F)), b, 0, s, F, dist);
+    else if (exm <= 128) (void)launch_dims(k_ext_rows_short<2>, "k_ext_rows_short", dim3(rows_short_grid(F)), b, 0, s, F, dist);
+    else if (exm <= 256) (void)launch_dims(k_ext_rows_reg<4>, "k_ext_rows_reg", dim3(rows_grid(F, 4)), b, 0, s, F, dist);
     else if (exm <= 320) (void)launch_dims(k_ext_rows_reg<5>, "k_ext_rows_reg", dim3(rows_grid(F, 5)), b, 0, s, F, dist);
     else if (exm <= 512) (void)launch_dims(k_ext_rows_reg<8>, "k_ext_rows_reg", dim3(rows_grid(F, 8)), b, 0, s, F, dist);
     else if (exm <= 1024) (void)launch_dims(k_ext_rows_reg<16>, "k_ext_rows_reg", dim3(rows_grid(F, 16)), b, 0, s, F, dist);
@@ -1206,7 +1304,9 @@ void rec_ext_device(std::vector<MRec>& out, const GridDev& g, float* u, float* v
     const Faces F = faces_of(g, u, v, w);
     const uint32_t exm = F.ex[0];
     const uint32_t rg = (F.rows[3] + 3) / 4;
-    if (exm <= 256) push_rec_e(out, kMkExtRowsReg4, MExtRowsReg<4>{F, dist, rows_grid(F, 4)}, rows_grid(F, 4));
+    if (exm <= 64) push_rec_e(out, kMkExtRowsShort1, MExtRowsShort<1>{F, dist, rows_short_grid(F)}, rows_short_grid(F));
+    else if (exm <= 128) push_rec_e(out, kMkExtRowsShort2, MExtRowsShort<2>{F, dist, rows_short_grid(F)}, rows_short_grid(F));
+    else if (exm <= 256) push_rec_e(out, kMkExtRowsReg4, MExtRowsReg<4>{F, dist, rows_grid(F, 4)}, rows_grid(F, 4));
     else if (exm <= 320) push_rec_e(out, kMkExtRowsReg5, MExtRowsReg<5>{F, dist, rows_grid(F, 5)}, rows_grid(F, 5));
     else if (exm <= 512) push_rec_e(out, kMkExtRowsReg8, MExtRowsReg<8>{F, dist, rows_grid(F, 8)}, rows_grid(F, 8));
     else if (exm <= 1024) push_rec_e(out, kMkExtRowsReg16, MExtRowsReg<16>{F, dist, rows_grid(F, 16)}, rows_grid(F, 16));
@@ -1353,6 +1453,8 @@ hipError_t launch_multi_ext(hipStream_t s, int32_t kind, const void* tab, const 
                             uint32_t, const uint8_t* active) {
     switch (kind) {
         case kMkExtRows: return launch_me<MExtRows>(s, "k_multi_ext_rows", tab, map, nblocks, active);
+        case kMkExtRowsShort1: return launch_me<MExtRowsShort<1>>(s, "k_multi_ext_rows_short", tab, map, nblocks, active);
+        case kMkExtRowsShort2: return launch_me<MExtRowsShort<2>>(s, "k_multi_ext_rows_short", tab, map, nblocks, active);
         case kMkExtRowsReg4: return launch_me<MExtRowsReg<4>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
         case kMkExtRowsReg5: return launch_me<MExtRowsReg<5>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
         case kMkExtRowsReg8: return launch_me<MExtRowsReg<8>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
