@@ -430,6 +430,8 @@ enum MultiKind : int32_t {
     kMkExtLinesSeg,                          // 8 kinds: (axis - 1) * 4 + segment size class
     kMkExtLinesSegLast = kMkExtLinesSeg + 7,
     kMkExtRowsShort1, kMkExtRowsShort2,
+    kMkExtLinesShort,                        // 4 kinds: (axis - 1) * 2 + segment size class, 4 waves a group
+    kMkExtLinesShortLast = kMkExtLinesShort + 3,
     kMkCount
 };
 constexpr size_t kMRecBytes = 512;

@@ -237,10 +237,10 @@ __device__ __forceinline__ uint32_t seg_stripes(const Faces& F, int c) {
     return (n + 63) >> 6;
 }
 
-template <int AXIS, int SEG>
+template <int AXIS, int SEG, int WAVES = kLineSegWaves>
 __device__ __forceinline__ void k_ext_lines_seg_run(Faces F, uint16_t* __restrict__ dist, uint32_t blk, uint32_t slot) {
     (void)slot;
-    __shared__ int32_t sp[kLineSegWaves][64], ss[kLineSegWaves][64];
+    __shared__ int32_t sp[WAVES][64], ss[WAVES][64];
     const uint32_t(&pre)[4] = AXIS == 1 ? F.ylines : F.zlines;
     const uint32_t n0 = seg_stripes<AXIS>(F, 0), n1 = seg_stripes<AXIS>(F, 1), n2 = seg_stripes<AXIS>(F, 2);
     if (blk >= n0 + n1 + n2) return;
@@ -258,7 +258,7 @@ __device__ __forceinline__ void k_ext_lines_seg_run(Faces F, uint16_t* __restric
     if (AXIS == 1) { start = w * ex * ey + x; stride = ex; len = ey; }
     else { start = w * ex + x; stride = ex * ey; len = pick(F.ez, c); }
     const uint32_t lb = 2u * (pick4(F.off, c) + start);   // this lane's line start, in bytes (< 2^32)
-    const uint32_t seg = (len + kLineSegWaves - 1) / kLineSegWaves;
+    const uint32_t seg = (len + WAVES - 1) / WAVES;
     const uint32_t a = wv * seg;
     const uint32_t n = a < len ? min(seg, len - a) : 0u;   // faces of this wave's segment
     // Buffer accesses: the lane's line start in the VGPR offset, the row's
@@ -282,7 +282,7 @@ __device__ __forceinline__ void k_ext_lines_seg_run(Faces F, uint16_t* __restric
     __syncthreads();
     int32_t f = kSegNone, g = kSegNone;
 #pragma unroll
-    for (int m = 0; m < kLineSegWaves; ++m) {
+    for (int m = 0; m < WAVES; ++m) {
         if (uint32_t(m) < wv) f = min(f, sp[m][lane]);
         if (uint32_t(m) > wv) g = min(g, ss[m][lane]);
     }
@@ -303,19 +303,25 @@ __device__ __forceinline__ void k_ext_lines_seg_run(Faces F, uint16_t* __restric
     }
 }
 
-template <int AXIS, int SEG>
-__global__ void __launch_bounds__(kLineSegThreads) k_ext_lines_seg(Faces F, uint16_t* __restrict__ dist) {
-    k_ext_lines_seg_run<AXIS, SEG>(F, dist, xcd_block(), blockIdx.x);
+template <int AXIS, int SEG, int WAVES = kLineSegWaves>
+__global__ void __launch_bounds__(64 * WAVES) k_ext_lines_seg(Faces F, uint16_t* __restrict__ dist) {
+    k_ext_lines_seg_run<AXIS, SEG, WAVES>(F, dist, xcd_block(), blockIdx.x);
 }
 
-template <int AXIS, int SEG>
+template <int AXIS, int SEG, int WAVES = kLineSegWaves>
 struct MExtLinesSeg {
-    static constexpr int kThreads = kLineSegThreads;
+    static constexpr int kThreads = 64 * WAVES;
     Faces F;
     uint16_t* dist;
     uint32_t G;
-    __device__ void run(uint32_t vb) const { k_ext_lines_seg_run<AXIS, SEG>(F, dist, xcd_of(vb, G), vb); }
+    __device__ void run(uint32_t vb) const { k_ext_lines_seg_run<AXIS, SEG, WAVES>(F, dist, xcd_of(vb, G), vb); }
 };
+
+// Short lines (at most kLineShortWaves * SEG faces) in work-groups of
+// kLineShortWaves waves: with 16 waves, a line of 33 faces gives each wave
+// 3 faces behind the same loads, exchange and barrier.
+constexpr int kLineShortWaves = 4;
+constexpr int kLineShortSizes[2] = {12, 36};
 
 // 1a'. The row transform from a ballot of the known faces: a row's
 // distances start as 0 (known) or INF, and the two min-plus sweeps of
@@ -1177,6 +1183,17 @@ static int line_seg_class(const Faces& F, int axis) {
     return -1;
 }
 
+// The class of k_ext_lines_seg with kLineShortWaves waves for the longest
+// line along `axis`, or -1.
+static int line_short_class(const Faces& F, int axis) {
+    if (F.off[3] >= (1u << 31)) return -1;
+    uint32_t len = 0;
+    for (int c = 0; c < 3; ++c) len = std::max(len, axis == 1 ? F.ey[c] : F.ez[c]);
+    for (int k = 0; k < 2; ++k)
+        if (len <= uint32_t(kLineShortWaves * kLineShortSizes[k])) return k;
+    return -1;
+}
+
 static uint32_t line_seg_grid(const Faces& F, int axis) {
     const uint32_t* pre = axis == 1 ? F.ylines : F.zlines;
     uint32_t n = 0;
@@ -1203,6 +1220,19 @@ hipError_t launch_ext_distance(hipStream_t s, const GridDev& g, const float* u, 
     if (e != hipSuccess) return e;
     for (int axis = 1; axis <= 2; ++axis) {
         probe_next_bytes(4.0 * nf);   // distance 2 read and written
+        const int sc = line_short_class(F, axis);
+        if (sc >= 0) {
+            const dim3 gr(line_seg_grid(F, axis)), bl(64 * kLineShortWaves);
+            switch ((axis - 1) * 2 + sc) {
+                case 0: (void)launch_dims(k_ext_lines_seg<1, kLineShortSizes[0], kLineShortWaves>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 1: (void)launch_dims(k_ext_lines_seg<1, kLineShortSizes[1], kLineShortWaves>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                case 2: (void)launch_dims(k_ext_lines_seg<2, kLineShortSizes[0], kLineShortWaves>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+                default: (void)launch_dims(k_ext_lines_seg<2, kLineShortSizes[1], kLineShortWaves>, "k_ext_lines_seg", gr, bl, 0, s, F, dist); break;
+            }
+            const hipError_t e1 = hipGetLastError();
+            if (e1 != hipSuccess) return e1;
+            continue;
+        }
         const int cls = line_seg_class(F, axis);
         if (cls < 0) {
             const uint32_t n = axis == 1 ? F.ylines[3] : F.zlines[3];
@@ -1312,6 +1342,18 @@ void rec_ext_device(std::vector<MRec>& out, const GridDev& g, float* u, float* v
     else if (exm <= 1024) push_rec_e(out, kMkExtRowsReg16, MExtRowsReg<16>{F, dist, rows_grid(F, 16)}, rows_grid(F, 16));
     else push_rec_e(out, kMkExtRows, MExtRows{F, dist, rg}, rg);
     for (int axis = 1; axis <= 2; ++axis) {
+        const int sc = line_short_class(F, axis);
+        if (sc >= 0) {
+            const uint32_t gl = line_seg_grid(F, axis);
+            const int32_t kind = kMkExtLinesShort + (axis - 1) * 2 + sc;
+            switch ((axis - 1) * 2 + sc) {
+                case 0: push_rec_e(out, kind, MExtLinesSeg<1, kLineShortSizes[0], kLineShortWaves>{F, dist, gl}, gl); break;
+                case 1: push_rec_e(out, kind, MExtLinesSeg<1, kLineShortSizes[1], kLineShortWaves>{F, dist, gl}, gl); break;
+                case 2: push_rec_e(out, kind, MExtLinesSeg<2, kLineShortSizes[0], kLineShortWaves>{F, dist, gl}, gl); break;
+                default: push_rec_e(out, kind, MExtLinesSeg<2, kLineShortSizes[1], kLineShortWaves>{F, dist, gl}, gl); break;
+            }
+            continue;
+        }
         const int cls = line_seg_class(F, axis);
         if (cls < 0) {
             const uint32_t gl = ((axis == 1 ? F.ylines[3] : F.zlines[3]) + kBlock - 1) / kBlock;
@@ -1453,6 +1495,10 @@ hipError_t launch_multi_ext(hipStream_t s, int32_t kind, const void* tab, const 
                             uint32_t, const uint8_t* active) {
     switch (kind) {
         case kMkExtRows: return launch_me<MExtRows>(s, "k_multi_ext_rows", tab, map, nblocks, active);
+        case kMkExtLinesShort + 0: return launch_me<MExtLinesSeg<1, kLineShortSizes[0], kLineShortWaves>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesShort + 1: return launch_me<MExtLinesSeg<1, kLineShortSizes[1], kLineShortWaves>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesShort + 2: return launch_me<MExtLinesSeg<2, kLineShortSizes[0], kLineShortWaves>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
+        case kMkExtLinesShort + 3: return launch_me<MExtLinesSeg<2, kLineShortSizes[1], kLineShortWaves>>(s, "k_multi_ext_lines_seg", tab, map, nblocks, active);
         case kMkExtRowsShort1: return launch_me<MExtRowsShort<1>>(s, "k_multi_ext_rows_short", tab, map, nblocks, active);
         case kMkExtRowsShort2: return launch_me<MExtRowsShort<2>>(s, "k_multi_ext_rows_short", tab, map, nblocks, active);
         case kMkExtRowsReg4: return launch_me<MExtRowsReg<4>>(s, "k_multi_ext_rows_reg", tab, map, nblocks, active);
