@@ -383,8 +383,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: this build runs the MAC-grid step on a single GPU "
-              "(multi-GPU sharding is described in DESIGN.md, not built).", file=sys.stderr)
+        print("bench.py measures one MI355X: the library runs the MAC-grid step on a single GPU "
+              "(sharding a scene over several GPUs is not built).", file=sys.stderr)
         sys.exit(2)
 
     import mgf
